@@ -1,0 +1,184 @@
+/*
+ * magla.h -- C ABI of the MI355X (gfx950) UCHOL / CGGP library, libmagla.so.
+ *
+ * Drop-in boundary for MAGLA's CAESolvers (C#/.NET 8).  A managed wrapper
+ * binds these with [LibraryImport] (blittable pointers, opaque handles); the
+ * binding is shown in INTEGRATION.md.  Each entry point names the reference
+ * member it replaces (paths relative to the MAGLA repository root).
+ *
+ * Conventions
+ *   - The caller owns every input array; the library copies what it keeps.
+ *   - Matrices are MAGLA's SymmetricCSRMatrix layout: upper triangle with the
+ *     diagonal, rows ascending, columns ascending within a row, int32
+ *     indices, float64 values (SymmetricCSRMatrix.cs:373-390).
+ *   - Every function returns an mg_status; the message of the last failure
+ *     on the calling thread is returned by mg_last_error().
+ *   - "_dev" entry points take device pointers (HBM) and a hipStream_t passed
+ *     as void*; all others take host pointers and are synchronous.
+ */
+#ifndef MAGLA_H
+#define MAGLA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; each maps to the exception the reference throws. */
+typedef enum {
+    MG_OK = 0,
+    MG_E_NULL = 1,      /* ArgumentNullException            (SymmetricUtduSolver.cs:188-189,203-204) */
+    MG_E_ARG = 2,       /* ArgumentException                (UtduNumericFactorization.cs:96-98, UtduSymbolicFactorization.cs:306-319) */
+    MG_E_RANGE = 3,     /* ArgumentOutOfRangeException / IndexOutOfRangeException (IterativeLinearSolver.cs:68-75) */
+    MG_E_NOT_PD = 4,    /* InvalidOperationException: non-positive pivot (UtduNumericFactorization.cs:559-566), p'Ap<=0 (ConjugateGradientGaussPreSolver.cs:139-141), zero diagonal (JacobiPreconditioner.cs:224-225) */
+    MG_E_TOO_DENSE = 5, /* InvalidOperationException: front/block too large (UtduNumericFactorization.cs:298-304, FactorStorageLayout.cs:744-747) */
+    MG_E_DEVICE = 6,    /* InvalidOperationException wrapping a HIP error (UtduNumericFactorization.cs:1017-1022) */
+    MG_E_INTERNAL = 7,  /* InvalidOperationException: internal symbolic/AMD error (ApproximateMinimumDegreeOrdering.cs:220-223) */
+    MG_E_NOMEM = 8      /* OutOfMemoryException */
+} mg_status;
+
+typedef enum { MG_ORDER_NATURAL = 0, MG_ORDER_AMD = 1 } mg_ordering; /* FillReducingOrdering (UtduSymbolicFactorization.cs:8-24) */
+typedef enum { MG_FP64 = 0, MG_FP32 = 1 } mg_precision;              /* new: factor precision */
+
+/* UtduSolverOptions (UtduSymbolicFactorization.cs:31-98) + device fields. */
+typedef struct {
+    int32_t ordering;                     /* default MG_ORDER_AMD */
+    int32_t max_degree_of_parallelism;    /* host threads for the symbolic phase; 0 = all cores */
+    int32_t supernode_amalgamation_limit; /* default 64 */
+    int32_t panel_width;                  /* default 32, clamped to 4..128 (reference semantics); the
+                                             device factorization blocks by 64 independently */
+    int32_t allow_diagonal_regularization;/* default 1 */
+    int32_t precision;                    /* MG_FP64 (default) or MG_FP32 */
+    double pivot_tolerance;               /* default 1e-13 */
+    double diagonal_regularization;       /* default 1e-10 */
+    int64_t parallel_update_threshold;    /* default 1<<20 (kept for ABI parity; unused on the GPU) */
+} mg_options;
+
+typedef struct mg_symbolic mg_symbolic;   /* UtduSymbolicFactorization */
+typedef struct mg_numeric mg_numeric;     /* UtduNumericFactorization */
+typedef struct mg_matrix mg_matrix;       /* device-resident SymmetricCSRMatrix (for SpMV / CG / refinement) */
+
+/* UtduSymbolicFactorization public properties (UtduSymbolicFactorization.cs:126-232). */
+typedef struct {
+    int32_t size;
+    int32_t supernode_count;
+    int32_t max_front_size;
+    int32_t max_supernode_width;
+    int64_t factor_entry_count;
+    int64_t strict_factor_nonzero_count;
+    int64_t factor_operation_count;
+    int64_t estimated_factor_bytes;
+    int64_t max_front_bytes;
+    int64_t peak_contribution_bytes;
+    int64_t structure_bytes;
+    int32_t level_count;                  /* new: supernodal-tree levels scheduled on the device */
+    int32_t matrix_nonzero_count;
+} mg_symbolic_info;
+
+/* UtduNumericFactorization statistics (UtduNumericFactorization.cs:57-84). */
+typedef struct {
+    int32_t regularized_pivot_count;
+    int32_t negative_pivot_count;
+    double smallest_pivot_magnitude;
+    double largest_pivot_magnitude;
+    int32_t is_positive_definite;
+    int32_t precision;
+} mg_numeric_info;
+
+/* Report of a (refined) solve. */
+typedef struct {
+    int32_t refinement_steps;            /* correction solves after the first one */
+    int32_t converged;                   /* max relative residual <= tolerance */
+    double relative_residual;            /* max over right-hand sides of ||b-Ax||/||b|| (FP64) */
+} mg_solve_report;
+
+/* IterativeSolverResult (CGGP/IterativeSolverResult.cs:8-39), without the solution array. */
+typedef struct {
+    int32_t iterations;
+    int32_t converged;
+    double residual_norm;
+    double relative_residual;
+} mg_pcg_result;
+
+/* ---- library ---- */
+const char *mg_last_error(void);
+int mg_device_count(int *count);
+void mg_options_default(mg_options *opts);   /* new UtduSolverOptions() */
+
+/* ---- host-side matrix assembly (SymmetricCSRMatrix ctor / builder, LineCross) ---- */
+/* SymmetricCSRMatrix(int, IEnumerable<MatrixEntry>) (SymmetricCSRMatrix.cs:36-90): (row,col)
+ * normalised to (min,max), duplicates summed in input order, |sum|<=1e-15 dropped, columns
+ * ascending.  Two-call protocol: pass NULL outputs to get *nnz_out, then call again with
+ * rowptr[n+1], colidx[nnz], values[nnz]. */
+int mg_csr_from_coo(int32_t n, int64_t count, const int32_t *rows, const int32_t *cols,
+                    const double *vals, int32_t *nnz_out, int32_t *rowptr, int32_t *colidx,
+                    double *values);
+/* SymmetricCSRMatrix.LineCross (SymmetricCSRMatrix.cs:125-167) for a set of constrained
+ * equations with prescribed values, applied in the given order; values and rhs updated in place. */
+int mg_csr_line_cross(int32_t n, const int32_t *rowptr, const int32_t *colidx, double *values,
+                      double *rhs, int32_t count, const int32_t *indices, const double *prescribed);
+
+/* ---- symbolic phase (host) ---- */
+/* UtduSymbolicFactorization.Analyze (UtduSymbolicFactorization.cs:240-295) /
+ * SymmetricUtduSolver.Analyze (SymmetricUtduSolver.cs:186-192). */
+int mg_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const mg_options *opts,
+               mg_symbolic **out);
+/* Permutation / InversePermutation / cost estimates (UtduSymbolicFactorization.cs:126-232);
+ * perm and iperm may be NULL. */
+int mg_symbolic_info_get(const mg_symbolic *sym, mg_symbolic_info *info, int32_t *perm, int32_t *iperm);
+/* Internal structure for parity tests: EliminationTreeParent, ColumnCounts (n each),
+ * fill-reducing order before postorder (n), FirstColumn (nsuper+1), supernode Parent (nsuper),
+ * PatternPointers (nsuper+1), PatternRows (pattern_pointers[nsuper]).  Any may be NULL. */
+int mg_symbolic_structure(const mg_symbolic *sym, int32_t *etree, int32_t *colcount,
+                          int32_t *fill_order, int32_t *first_column, int32_t *super_parent,
+                          int32_t *pattern_pointers, int32_t *pattern_rows);
+/* UtduSymbolicFactorization.EstimatePeakBytes (UtduSymbolicFactorization.cs:203-219). */
+int64_t mg_symbolic_estimate_peak_bytes(const mg_symbolic *sym, int32_t worker_count);
+void mg_symbolic_release(mg_symbolic *sym);
+
+/* ---- numeric phase (device) ---- */
+/* SymmetricUtduSolver.Factorize(matrix, symbolic) (SymmetricUtduSolver.cs:200-212):
+ * EnsureCompatible (n and nnz) then the multifrontal factorization on the GPU. */
+int mg_factorize(mg_symbolic *sym, int32_t n, int32_t nnz, const double *values,
+                 const mg_options *opts, mg_numeric **out);
+/* Same with values already in HBM; refactorizes into *num when *num is non-NULL and was
+ * created from the same symbolic handle and precision (no allocation on that path). */
+int mg_factorize_dev(mg_symbolic *sym, int32_t nnz, const double *d_values, const mg_options *opts,
+                     void *stream, mg_numeric **num);
+/* RegularizedPivotCount ... Diagonal (UtduNumericFactorization.cs:57-84); diag may be NULL
+ * (length n, elimination order). */
+int mg_numeric_info_get(const mg_numeric *num, mg_numeric_info *info, double *diag);
+void mg_numeric_release(mg_numeric *num);
+
+/* UtduNumericFactorization.Solve (UtduNumericFactorization.cs:90-112) for nrhs columns
+ * (b and x column-major with leading dimensions ldb, ldx >= n).  With an FP32 factor each
+ * column is refined in FP64 against the factorized matrix until ||b-Ax||/||b|| <= tol or
+ * max_refine correction steps; with an FP64 factor tol/max_refine apply the same way
+ * (max_refine = 0: one plain solve).  report may be NULL. */
+int mg_solve(mg_numeric *num, int32_t nrhs, const double *b, int32_t ldb, double *x, int32_t ldx,
+             double tol, int32_t max_refine, mg_solve_report *report);
+int mg_solve_dev(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x,
+                 int32_t ldx, double tol, int32_t max_refine, void *stream, mg_solve_report *report);
+
+/* ---- device matrix, SpMV, CGGP ---- */
+int mg_matrix_create(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *values,
+                     mg_matrix **out);
+void mg_matrix_release(mg_matrix *m);
+/* SymmetricCSRMatrix.Multiply(x, result) (SymmetricCSRMatrix.cs:293-324), nrhs columns. */
+int mg_spmv_dev(mg_matrix *m, int32_t nrhs, const double *d_x, int32_t ldx, double *d_y, int32_t ldy,
+                void *stream);
+int mg_spmv(mg_matrix *m, const double *x, double *y);
+/* ConjugateGradientGaussPreSolver.SolveWithInitialGuess (CGGP/ConjugateGradientGaussPreSolver.cs:78-166):
+ * x0 may be NULL (x0 = 0); max_iterations = 0 means n; use_jacobi selects JacobiPreconditioner
+ * (CGGP/JacobiPreconditioner.cs:209-253).  x receives the last iterate. */
+int mg_pcg_solve(mg_matrix *m, const double *b, const double *x0, double relative_tolerance,
+                 int32_t max_iterations, int32_t use_jacobi, double *x, mg_pcg_result *result);
+int mg_pcg_solve_dev(mg_matrix *m, const double *d_b, const double *d_x0, double relative_tolerance,
+                     int32_t max_iterations, int32_t use_jacobi, double *d_x, void *stream,
+                     mg_pcg_result *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,477 @@
+"""MI355X UCHOL / CGGP solver library: Python binding of libmagla.so.
+
+The classes mirror MAGLA's C# surface (CAESolvers) so that tests and the
+benchmark read like the reference's own code:
+
+  SymmetricCSRMatrix          SymmetricCSRMatrix.cs (upper triangle + diagonal)
+  UtduSolverOptions           UCHOL/UtduSymbolicFactorization.cs:31-98
+  SymmetricUtduSolver         UCHOL/SymmetricUtduSolver.cs:161-225
+  UtduSymbolicFactorization   UCHOL/UtduSymbolicFactorization.cs:109-320
+  UtduNumericFactorization    UCHOL/UtduNumericFactorization.cs:24-235
+  ConjugateGradientGaussPreSolver  CGGP/ConjugateGradientGaussPreSolver.cs:40-196
+
+All numeric work runs in libmagla.so on the GPU; there is no CPU fallback:
+loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmagla.so")
+
+MG_OK, MG_E_NULL, MG_E_ARG, MG_E_RANGE, MG_E_NOT_PD, MG_E_TOO_DENSE, MG_E_DEVICE, MG_E_INTERNAL, MG_E_NOMEM = range(9)
+NATURAL, AMD = 0, 1
+FP64, FP32 = 0, 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _Options(C.Structure):
+    _fields_ = [("ordering", C.c_int32), ("max_degree_of_parallelism", C.c_int32),
+                ("supernode_amalgamation_limit", C.c_int32), ("panel_width", C.c_int32),
+                ("allow_diagonal_regularization", C.c_int32), ("precision", C.c_int32),
+                ("pivot_tolerance", C.c_double), ("diagonal_regularization", C.c_double),
+                ("parallel_update_threshold", C.c_int64)]
+
+
+class _SymInfo(C.Structure):
+    _fields_ = [("size", C.c_int32), ("supernode_count", C.c_int32), ("max_front_size", C.c_int32),
+                ("max_supernode_width", C.c_int32), ("factor_entry_count", C.c_int64),
+                ("strict_factor_nonzero_count", C.c_int64), ("factor_operation_count", C.c_int64),
+                ("estimated_factor_bytes", C.c_int64), ("max_front_bytes", C.c_int64),
+                ("peak_contribution_bytes", C.c_int64), ("structure_bytes", C.c_int64),
+                ("level_count", C.c_int32), ("matrix_nonzero_count", C.c_int32)]
+
+
+class _NumInfo(C.Structure):
+    _fields_ = [("regularized_pivot_count", C.c_int32), ("negative_pivot_count", C.c_int32),
+                ("smallest_pivot_magnitude", C.c_double), ("largest_pivot_magnitude", C.c_double),
+                ("is_positive_definite", C.c_int32), ("precision", C.c_int32)]
+
+
+class _SolveReport(C.Structure):
+    _fields_ = [("refinement_steps", C.c_int32), ("converged", C.c_int32), ("relative_residual", C.c_double)]
+
+
+class _PcgResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual_norm", C.c_double),
+                ("relative_residual", C.c_double)]
+
+
+class MaglaError(RuntimeError):
+    """Status code + message; the exception class mirrors the C# one."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class ArgumentNullError(MaglaError, ValueError):
+    pass
+
+
+class ArgumentError(MaglaError, ValueError):
+    pass
+
+
+class ArgumentOutOfRangeError(MaglaError, ValueError):
+    pass
+
+
+class InvalidOperationError(MaglaError):
+    pass
+
+
+_EXC = {MG_E_NULL: ArgumentNullError, MG_E_ARG: ArgumentError, MG_E_RANGE: ArgumentOutOfRangeError}
+
+_lib = None
+
+
+def lib():
+    """Load libmagla.so (built by magla_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libmagla.so not built: run `python -m magla_amd.build` ({LIB_PATH})")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.mg_last_error.restype = C.c_char_p
+    L.mg_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.mg_options_default.argtypes = [C.POINTER(_Options)]
+    L.mg_options_default.restype = None
+    L.mg_csr_from_coo.argtypes = [C.c_int32, C.c_int64, _i32p, _i32p, _f64p, C.POINTER(C.c_int32), vp, vp, vp]
+    L.mg_csr_line_cross.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, C.c_int32, _i32p, _f64p]
+    L.mg_analyze.argtypes = [C.c_int32, _i32p, vp, C.POINTER(_Options), C.POINTER(vp)]
+    L.mg_symbolic_info_get.argtypes = [vp, C.POINTER(_SymInfo), vp, vp]
+    L.mg_symbolic_structure.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mg_symbolic_estimate_peak_bytes.argtypes = [vp, C.c_int32]
+    L.mg_symbolic_estimate_peak_bytes.restype = C.c_int64
+    L.mg_symbolic_release.argtypes = [vp]
+    L.mg_symbolic_release.restype = None
+    L.mg_factorize.argtypes = [vp, C.c_int32, C.c_int32, vp, C.POINTER(_Options), C.POINTER(vp)]
+    L.mg_factorize_dev.argtypes = [vp, C.c_int32, vp, C.POINTER(_Options), vp, C.POINTER(vp)]
+    L.mg_numeric_info_get.argtypes = [vp, C.POINTER(_NumInfo), vp]
+    L.mg_numeric_release.argtypes = [vp]
+    L.mg_numeric_release.restype = None
+    L.mg_solve.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_double, C.c_int32,
+                           C.POINTER(_SolveReport)]
+    L.mg_solve_dev.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_double, C.c_int32, vp,
+                               C.POINTER(_SolveReport)]
+    L.mg_matrix_create.argtypes = [C.c_int32, _i32p, vp, vp, C.POINTER(vp)]
+    L.mg_matrix_release.argtypes = [vp]
+    L.mg_matrix_release.restype = None
+    L.mg_spmv.argtypes = [vp, _f64p, _f64p]
+    L.mg_spmv_dev.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, vp]
+    L.mg_pcg_solve.argtypes = [vp, _f64p, vp, C.c_double, C.c_int32, C.c_int32, _f64p, C.POINTER(_PcgResult)]
+    L.mg_pcg_solve_dev.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, C.POINTER(_PcgResult)]
+    L.mg_test_numeric_front.argtypes = [vp, C.c_int32, vp]
+    L.mg_test_mfma_layout.argtypes = [C.c_int32, vp]
+    _lib = L
+    return L
+
+
+def test_mfma_layout(precision):
+    """Probe MFMA product placed by the kernel's layout rule (include/magla_testing.h): (16, 16) array
+    that must equal i + 16 j."""
+    out = np.zeros(256)
+    _check(lib().mg_test_mfma_layout(precision, _ptr(out)))
+    return out.reshape(16, 16)
+
+
+def test_numeric_front(numeric, supernode, front_size):
+    """Front of one supernode after factorization as an (f, f) array (lower part valid)."""
+    out = np.zeros(front_size * front_size)
+    _check(lib().mg_test_numeric_front(numeric._h, supernode, _ptr(out)))
+    return out.reshape(front_size, front_size, order="F")
+
+
+def _check(rc):
+    if rc != MG_OK:
+        msg = lib().mg_last_error().decode(errors="replace")
+        raise _EXC.get(rc, InvalidOperationError)(rc, msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count():
+    c = C.c_int(0)
+    _check(lib().mg_device_count(C.byref(c)))
+    return c.value
+
+
+# ---------------------------------------------------------------------------
+# SymmetricCSRMatrix
+# ---------------------------------------------------------------------------
+class SymmetricCSRMatrix:
+    """Upper-triangle CSR with the diagonal (SymmetricCSRMatrix.cs:19-90)."""
+
+    def __init__(self, size, rowptr, colidx, values):
+        self.size = int(size)
+        self.row_pointers = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.column_indices = np.ascontiguousarray(colidx, dtype=np.int32)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+
+    @classmethod
+    def from_coo(cls, size, rows, cols, vals):
+        """SymmetricCSRMatrix(int, IEnumerable<MatrixEntry>): normalise, sum duplicates in input order,
+        drop |v| <= 1e-15, sort columns (SymmetricCSRMatrix.cs:36-90)."""
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        nnz = C.c_int32(0)
+        _check(lib().mg_csr_from_coo(size, len(rows), rows, cols, vals, C.byref(nnz), None, None, None))
+        rp = np.zeros(size + 1, dtype=np.int32)
+        ci = np.zeros(nnz.value, dtype=np.int32)
+        va = np.zeros(nnz.value, dtype=np.float64)
+        _check(lib().mg_csr_from_coo(size, len(rows), rows, cols, vals, C.byref(nnz), _ptr(rp), _ptr(ci), _ptr(va)))
+        return cls(size, rp, ci, va)
+
+    @property
+    def nonzero_count(self):
+        return int(self.row_pointers[-1])
+
+    def line_cross(self, rhs, prescribed, indices):
+        """LineCross (SymmetricCSRMatrix.cs:125-167) for each index in order; in place."""
+        idx = np.ascontiguousarray(np.atleast_1d(indices), dtype=np.int32)
+        pv = np.ascontiguousarray(np.broadcast_to(np.asarray(prescribed, dtype=np.float64), idx.shape))
+        _check(lib().mg_csr_line_cross(self.size, self.row_pointers, self.column_indices, self.values, rhs,
+                                       len(idx), idx, pv))
+
+    def diagonal(self):
+        d = np.zeros(self.size)
+        rp, ci = self.row_pointers, self.column_indices
+        rows = np.repeat(np.arange(self.size), np.diff(rp))
+        m = rows == ci
+        d[rows[m]] = self.values[m]
+        return d
+
+
+class DeviceMatrix:
+    """Device-resident matrix for SpMV and PCG (full CSR in HBM)."""
+
+    def __init__(self, a: SymmetricCSRMatrix):
+        self.size = a.size
+        h = C.c_void_p()
+        _check(lib().mg_matrix_create(a.size, a.row_pointers, _ptr(a.column_indices), _ptr(a.values), C.byref(h)))
+        self._h = h
+
+    def multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.size)
+        _check(lib().mg_spmv(self._h, x, y))
+        return y
+
+    def multiply_dev(self, x_ptr, y_ptr, nrhs=1, ldx=None, ldy=None, stream=None):
+        _check(lib().mg_spmv_dev(self._h, nrhs, x_ptr, ldx or self.size, y_ptr, ldy or self.size, stream))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_matrix_release(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+# UCHOL
+# ---------------------------------------------------------------------------
+@dataclass
+class UtduSolverOptions:
+    """UtduSolverOptions (UtduSymbolicFactorization.cs:31-98) plus the factor precision."""
+    ordering: int = AMD
+    max_degree_of_parallelism: int = 0
+    supernode_amalgamation_limit: int = 64
+    panel_width: int = 32
+    allow_diagonal_regularization: bool = True
+    pivot_tolerance: float = 1e-13
+    diagonal_regularization: float = 1e-10
+    parallel_update_threshold: int = 1 << 20
+    precision: int = FP64
+
+    def _c(self):
+        o = _Options()
+        lib().mg_options_default(C.byref(o))
+        o.ordering = self.ordering
+        o.max_degree_of_parallelism = self.max_degree_of_parallelism
+        o.supernode_amalgamation_limit = self.supernode_amalgamation_limit
+        o.panel_width = self.panel_width
+        o.allow_diagonal_regularization = 1 if self.allow_diagonal_regularization else 0
+        o.pivot_tolerance = self.pivot_tolerance
+        o.diagonal_regularization = self.diagonal_regularization
+        o.parallel_update_threshold = self.parallel_update_threshold
+        o.precision = self.precision
+        return o
+
+
+class UtduSymbolicFactorization:
+    def __init__(self, handle, options):
+        self._h = handle
+        self.options = options
+        info = _SymInfo()
+        n = self._info_raw(info)
+        self.size = n
+        self.permutation = np.zeros(n, dtype=np.int32)
+        self.inverse_permutation = np.zeros(n, dtype=np.int32)
+        _check(lib().mg_symbolic_info_get(self._h, C.byref(info), _ptr(self.permutation),
+                                          _ptr(self.inverse_permutation)))
+        self._info = info
+
+    def _info_raw(self, info):
+        _check(lib().mg_symbolic_info_get(self._h, C.byref(info), None, None))
+        return info.size
+
+    def info(self):
+        info = _SymInfo()
+        self._info_raw(info)
+        return {k: getattr(info, k) for k, _ in _SymInfo._fields_}
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        keys = {"supernode_count", "max_front_size", "factor_entry_count", "strict_factor_nonzero_count",
+                "factor_operation_count", "estimated_factor_bytes", "max_front_bytes", "peak_contribution_bytes",
+                "structure_bytes", "max_supernode_width", "level_count"}
+        if name in keys:
+            return self.info()[name]
+        raise AttributeError(name)
+
+    def estimate_peak_bytes(self, worker_count):
+        return lib().mg_symbolic_estimate_peak_bytes(self._h, worker_count)
+
+    def structure(self):
+        """etree, column counts, fill order, supernode boundaries / parents / row patterns."""
+        inf = self.info()
+        n, ns = self.size, inf["supernode_count"]
+        etree = np.zeros(n, np.int32)
+        cc = np.zeros(n, np.int32)
+        fo = np.zeros(n, np.int32)
+        first = np.zeros(ns + 1, np.int32)
+        sp = np.zeros(ns, np.int32)
+        pp = np.zeros(ns + 1, np.int32)
+        _check(lib().mg_symbolic_structure(self._h, _ptr(etree), _ptr(cc), _ptr(fo), _ptr(first), _ptr(sp),
+                                           _ptr(pp), None))
+        pr = np.zeros(int(pp[-1]), np.int32)
+        _check(lib().mg_symbolic_structure(self._h, None, None, None, None, None, None, _ptr(pr)))
+        return dict(etree=etree, colcount=cc, fill_order=fo, first=first, sparent=sp, pat_ptr=pp, pat_rows=pr)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_symbolic_release(self._h)
+            self._h = None
+
+
+class UtduNumericFactorization:
+    def __init__(self, handle, symbolic):
+        self._h = handle
+        self.symbolic = symbolic
+        self.size = symbolic.size
+
+    def _stats(self, with_diag=False):
+        info = _NumInfo()
+        diag = np.zeros(self.size) if with_diag else None
+        _check(lib().mg_numeric_info_get(self._h, C.byref(info), _ptr(diag)))
+        return info, diag
+
+    @property
+    def regularized_pivot_count(self):
+        return self._stats()[0].regularized_pivot_count
+
+    @property
+    def negative_pivot_count(self):
+        return self._stats()[0].negative_pivot_count
+
+    @property
+    def smallest_pivot_magnitude(self):
+        return self._stats()[0].smallest_pivot_magnitude
+
+    @property
+    def largest_pivot_magnitude(self):
+        return self._stats()[0].largest_pivot_magnitude
+
+    @property
+    def is_positive_definite(self):
+        return bool(self._stats()[0].is_positive_definite)
+
+    @property
+    def diagonal(self):
+        return self._stats(True)[1]
+
+    def solve(self, b, tol=0.0, max_refine=0, report=None):
+        """Solve(double[]) (UtduNumericFactorization.cs:90-112); b may be (n,) or (n, k)."""
+        b = np.asarray(b, dtype=np.float64)
+        if b.ndim == 1:
+            if b.shape[0] != self.size:
+                raise ArgumentError(MG_E_ARG, f"The right-hand side length {b.shape[0]} does not match the matrix "
+                                              f"size {self.size}.")
+            bb = np.ascontiguousarray(b)
+            k = 1
+        else:
+            if b.shape[0] != self.size:
+                raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the matrix size.")
+            bb = np.asfortranarray(b)
+            k = b.shape[1]
+        x = np.zeros_like(bb, order="F") if b.ndim == 2 else np.zeros(self.size)
+        rep = _SolveReport()
+        _check(lib().mg_solve(self._h, k, _ptr(bb), max(self.size, 1), _ptr(x), max(self.size, 1), tol, max_refine,
+                              C.byref(rep)))
+        if report is not None:
+            report.update(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
+                          relative_residual=rep.relative_residual)
+        return x
+
+    def solve_dev(self, b_ptr, x_ptr, nrhs, tol=0.0, max_refine=0, stream=None, ldb=None, ldx=None):
+        rep = _SolveReport()
+        _check(lib().mg_solve_dev(self._h, nrhs, b_ptr, ldb or self.size, x_ptr, ldx or self.size, tol, max_refine,
+                                  stream, C.byref(rep)))
+        return dict(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
+                    relative_residual=rep.relative_residual)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_numeric_release(self._h)
+            self._h = None
+
+
+class SymmetricUtduSolver:
+    """SymmetricUtduSolver (UCHOL/SymmetricUtduSolver.cs:161-225) on the GPU."""
+
+    def __init__(self, options: UtduSolverOptions | None = None):
+        self.options = options or UtduSolverOptions()
+
+    def analyze(self, matrix: SymmetricCSRMatrix) -> UtduSymbolicFactorization:
+        if matrix is None:
+            raise ArgumentNullError(MG_E_NULL, "matrix")
+        h = C.c_void_p()
+        _check(lib().mg_analyze(matrix.size, matrix.row_pointers, _ptr(matrix.column_indices),
+                                C.byref(self.options._c()), C.byref(h)))
+        return UtduSymbolicFactorization(h, self.options)
+
+    def factorize(self, matrix: SymmetricCSRMatrix, symbolic: UtduSymbolicFactorization | None = None):
+        if matrix is None:
+            raise ArgumentNullError(MG_E_NULL, "matrix")
+        if symbolic is None:
+            symbolic = self.analyze(matrix)
+        h = C.c_void_p()
+        _check(lib().mg_factorize(symbolic._h, matrix.size, matrix.nonzero_count, _ptr(matrix.values),
+                                  C.byref(self.options._c()), C.byref(h)))
+        return UtduNumericFactorization(h, symbolic)
+
+    def factorize_dev(self, symbolic, values_ptr, nnz, numeric=None, stream=None):
+        """Values already in HBM; refactorizes in place into `numeric` when given."""
+        h = C.c_void_p(numeric._h if numeric is not None else None)
+        _check(lib().mg_factorize_dev(symbolic._h, nnz, values_ptr, C.byref(self.options._c()), stream, C.byref(h)))
+        return numeric if numeric is not None else UtduNumericFactorization(h, symbolic)
+
+    def solve(self, matrix: SymmetricCSRMatrix, rhs):
+        """ILinearSolver.Solve via SolveCore (SymmetricUtduSolver.cs:221-224)."""
+        return self.factorize(matrix).solve(rhs)
+
+
+# ---------------------------------------------------------------------------
+# CGGP
+# ---------------------------------------------------------------------------
+@dataclass
+class IterativeSolverResult:
+    """CGGP/IterativeSolverResult.cs:8-39."""
+    solution: np.ndarray
+    iterations: int
+    converged: bool
+    residual_norm: float
+    relative_residual: float
+
+
+class ConjugateGradientGaussPreSolver:
+    """Jacobi-preconditioned CG (CGGP/ConjugateGradientGaussPreSolver.cs:40-196) on the GPU."""
+
+    def __init__(self, relative_tolerance=1e-8, max_iterations=0, use_preconditioner=True):
+        self.relative_tolerance = relative_tolerance
+        self.max_iterations = max_iterations
+        self.use_preconditioner = use_preconditioner
+        self.last_result = None
+
+    def solve(self, matrix: SymmetricCSRMatrix | DeviceMatrix, rhs, initial_guess=None):
+        self.last_result = None
+        dm = matrix if isinstance(matrix, DeviceMatrix) else DeviceMatrix(matrix)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if b.shape[0] != dm.size:
+            raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the matrix row count.")
+        x0 = None
+        if initial_guess is not None:
+            x0 = np.ascontiguousarray(initial_guess, dtype=np.float64)
+            if x0.shape[0] != dm.size:
+                raise ArgumentError(MG_E_ARG, f"The initial guess length {x0.shape[0]} does not match the matrix "
+                                              f"size {dm.size}.")
+        x = np.zeros(dm.size)
+        res = _PcgResult()
+        _check(lib().mg_pcg_solve(dm._h, b, _ptr(x0), self.relative_tolerance, self.max_iterations,
+                                  1 if self.use_preconditioner else 0, x, C.byref(res)))
+        self.last_result = IterativeSolverResult(x, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return x

@@ -1,0 +1,339 @@
+// capi.cpp -- extern "C" entry points of libmagla.so (include/magla.h).
+// Every call converts mg::Error / std::bad_alloc into an mg_status and keeps
+// the message for mg_last_error() on the calling thread.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "device.h"
+#include "../../include/magla_testing.h"
+
+namespace mg {
+void numeric_front(const Numeric *N, int s, double *out);
+void selftest_mfma(int precision, double *host_out);
+void csr_from_coo(int n, int64_t count, const int *rows, const int *cols, const double *vals, int *nnz_out,
+                  int *rowptr, int *colidx, double *values);
+void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values, double *rhs, int count,
+                    const int *indices, const double *prescribed);
+}  // namespace mg
+
+struct mg_symbolic {
+    mg::Symbolic s;
+};
+struct mg_numeric {
+    mg::Numeric *num = nullptr;
+    mg_symbolic *owner = nullptr;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        g_last_error.clear();
+        return MG_OK;
+    } catch (const mg::Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "Out of host memory.";
+        return MG_E_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return MG_E_INTERNAL;
+    }
+}
+
+void require(bool ok, int code, const char *msg) {
+    if (!ok) throw mg::Error(code, msg);
+}
+
+// Structural checks the C# types guarantee by construction.
+void check_csr(int n, const int32_t *rowptr, const int32_t *colidx) {
+    require(n >= 0, MG_E_ARG, "Matrix size cannot be negative.");
+    require(rowptr != nullptr, MG_E_NULL, "rowptr is null.");
+    require(rowptr[0] == 0, MG_E_ARG, "rowptr[0] must be 0.");
+    for (int r = 0; r < n; ++r) {
+        require(rowptr[r + 1] >= rowptr[r], MG_E_ARG, "rowptr must be non-decreasing.");
+        for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            require(colidx[k] >= r && colidx[k] < n, MG_E_ARG, "Column indices must describe the upper triangle.");
+            require(k == rowptr[r] || colidx[k] > colidx[k - 1], MG_E_ARG, "Column indices must ascend within a row.");
+        }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char *mg_last_error(void) { return g_last_error.c_str(); }
+
+int mg_device_count(int *count) {
+    return guarded([&] {
+        require(count != nullptr, MG_E_NULL, "count is null.");
+        MG_HIP(hipGetDeviceCount(count));
+    });
+}
+
+void mg_options_default(mg_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->ordering = MG_ORDER_AMD;
+    o->max_degree_of_parallelism = 0;
+    o->supernode_amalgamation_limit = 64;
+    o->panel_width = 32;
+    o->allow_diagonal_regularization = 1;
+    o->precision = MG_FP64;
+    o->pivot_tolerance = 1e-13;
+    o->diagonal_regularization = 1e-10;
+    o->parallel_update_threshold = 1LL << 20;
+}
+
+int mg_csr_from_coo(int32_t n, int64_t count, const int32_t *rows, const int32_t *cols, const double *vals,
+                    int32_t *nnz_out, int32_t *rowptr, int32_t *colidx, double *values) {
+    return guarded([&] {
+        require(nnz_out != nullptr, MG_E_NULL, "nnz_out is null.");
+        require(count == 0 || (rows && cols && vals), MG_E_NULL, "elements are null.");
+        require(n >= 0 && count >= 0, MG_E_ARG, "Negative size.");
+        mg::csr_from_coo(n, count, rows, cols, vals, nnz_out, rowptr, colidx, values);
+    });
+}
+
+int mg_csr_line_cross(int32_t n, const int32_t *rowptr, const int32_t *colidx, double *values, double *rhs,
+                      int32_t count, const int32_t *indices, const double *prescribed) {
+    return guarded([&] {
+        require(rhs != nullptr, MG_E_NULL, "rightHandSide is null.");
+        require(count == 0 || indices, MG_E_NULL, "indices are null.");
+        check_csr(n, rowptr, colidx);
+        mg::csr_line_cross(n, rowptr, colidx, values, rhs, count, indices, prescribed);
+    });
+}
+
+int mg_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const mg_options *opts, mg_symbolic **out) {
+    return guarded([&] {
+        require(out != nullptr && opts != nullptr, MG_E_NULL, "matrix or options is null.");
+        require(n == 0 || colidx != nullptr || rowptr[n] == 0, MG_E_NULL, "colidx is null.");
+        check_csr(n, rowptr, colidx);
+        auto h = std::make_unique<mg_symbolic>();
+        mg::analyze(n, rowptr, colidx, *opts, h->s);
+        *out = h.release();
+    });
+}
+
+int mg_symbolic_info_get(const mg_symbolic *sym, mg_symbolic_info *info, int32_t *perm, int32_t *iperm) {
+    return guarded([&] {
+        require(sym != nullptr && info != nullptr, MG_E_NULL, "symbolic or info is null.");
+        const mg::Symbolic &S = sym->s;
+        std::memset(info, 0, sizeof(*info));
+        info->size = S.n;
+        info->supernode_count = S.nsuper;
+        info->max_front_size = S.max_front;
+        info->max_supernode_width = S.max_width;
+        info->factor_entry_count = S.factor_entries;
+        info->strict_factor_nonzero_count = S.strict_nonzeros;
+        info->factor_operation_count = S.factor_ops;
+        info->estimated_factor_bytes = S.factor_entries * 8;
+        info->max_front_bytes = (int64_t)S.max_front * (S.max_front + 1) / 2 * 8;
+        info->peak_contribution_bytes = S.peak_contribution * 8;
+        info->structure_bytes = ((int64_t)S.prows.size() + (int64_t)S.lvidx.size() + (int64_t)S.pat_rows.size()) * 4;
+        info->level_count = S.plan ? mg::plan_level_count(*S.plan) : -1;
+        info->matrix_nonzero_count = S.matrix_nnz;
+        if (perm && S.n) std::memcpy(perm, S.perm.data(), sizeof(int) * S.n);
+        if (iperm && S.n) std::memcpy(iperm, S.iperm.data(), sizeof(int) * S.n);
+    });
+}
+
+int mg_symbolic_structure(const mg_symbolic *sym, int32_t *etree, int32_t *colcount, int32_t *fill_order,
+                          int32_t *first_column, int32_t *super_parent, int32_t *pattern_pointers,
+                          int32_t *pattern_rows) {
+    return guarded([&] {
+        require(sym != nullptr, MG_E_NULL, "symbolic is null.");
+        const mg::Symbolic &S = sym->s;
+        auto put = [](int32_t *dst, const std::vector<int> &v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(int) * v.size());
+        };
+        put(etree, S.etree);
+        put(colcount, S.colcount);
+        put(fill_order, S.fill_order);
+        put(first_column, S.first);
+        put(super_parent, S.sparent);
+        put(pattern_pointers, S.pat_ptr);
+        put(pattern_rows, S.pat_rows);
+    });
+}
+
+int64_t mg_symbolic_estimate_peak_bytes(const mg_symbolic *sym, int32_t worker_count) {
+    if (!sym) return -1;
+    const mg::Symbolic &S = sym->s;
+    const int64_t workers = worker_count > 1 ? worker_count : 1;
+    const int64_t max_front_bytes = (int64_t)S.max_front * (S.max_front + 1) / 2 * 8;
+    const int64_t retained = std::min<int64_t>(max_front_bytes, (int64_t)(1 << 25) * 8);
+    const int64_t structure = ((int64_t)S.prows.size() + (int64_t)S.lvidx.size() + (int64_t)S.pat_rows.size()) * 4;
+    return S.factor_entries * 8 + structure + S.peak_contribution * 8 + workers * retained + max_front_bytes;
+}
+
+void mg_symbolic_release(mg_symbolic *sym) { delete sym; }
+
+int mg_factorize_dev(mg_symbolic *sym, int32_t nnz, const double *d_values, const mg_options *opts, void *stream,
+                     mg_numeric **num) {
+    return guarded([&] {
+        require(sym != nullptr && opts != nullptr && num != nullptr, MG_E_NULL, "matrix or symbolic is null.");
+        mg::Symbolic &S = sym->s;
+        // EnsureCompatible (UtduSymbolicFactorization.cs:306-319)
+        require(nnz == S.matrix_nnz, MG_E_ARG,
+                "The matrix sparse structure has changed since symbolic analysis. Run Analyze again.");
+        require(S.matrix_nnz == 0 || d_values != nullptr, MG_E_NULL, "values are null.");
+        mg_numeric *h = *num;
+        if (h) require(h->owner == sym, MG_E_ARG, "The numeric handle belongs to another symbolic factorization.");
+        mg::Numeric *raw = h ? h->num : nullptr;
+        mg::factorize_device(S, d_values, opts->precision, *opts, (hipStream_t)stream, raw);
+        if (!h) {
+            h = new mg_numeric();
+            h->num = raw;
+            h->owner = sym;
+            *num = h;
+        }
+    });
+}
+
+int mg_factorize(mg_symbolic *sym, int32_t n, int32_t nnz, const double *values, const mg_options *opts,
+                 mg_numeric **out) {
+    return guarded([&] {
+        require(sym != nullptr && opts != nullptr && out != nullptr, MG_E_NULL, "matrix or symbolic is null.");
+        mg::Symbolic &S = sym->s;
+        require(n == S.n, MG_E_ARG, "Matrix size does not match the symbolic factorization.");
+        require(nnz == S.matrix_nnz, MG_E_ARG,
+                "The matrix sparse structure has changed since symbolic analysis. Run Analyze again.");
+        require(nnz == 0 || values != nullptr, MG_E_NULL, "values are null.");
+        mg::DevArray<double> dv(std::max(nnz, 1));
+        if (nnz) MG_HIP(hipMemcpy(dv.p, values, sizeof(double) * nnz, hipMemcpyHostToDevice));
+        mg::Numeric *raw = nullptr;
+        mg::factorize_device(S, dv.p, opts->precision, *opts, 0, raw);
+        MG_HIP(hipDeviceSynchronize());
+        auto *h = new mg_numeric();
+        h->num = raw;
+        h->owner = sym;
+        *out = h;
+    });
+}
+
+int mg_numeric_info_get(const mg_numeric *num, mg_numeric_info *info, double *diag) {
+    return guarded([&] {
+        require(num != nullptr && info != nullptr, MG_E_NULL, "numeric or info is null.");
+        mg::numeric_info(num->num, *info, diag);
+    });
+}
+
+void mg_numeric_release(mg_numeric *num) {
+    if (!num) return;
+    mg::numeric_release(num->num);
+    delete num;
+}
+
+int mg_solve_dev(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x, int32_t ldx, double tol,
+                 int32_t max_refine, void *stream, mg_solve_report *report) {
+    return guarded([&] {
+        require(num != nullptr, MG_E_NULL, "numeric is null.");
+        require(nrhs == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        mg::solve_device(num->num, nrhs, d_b, ldb, d_x, ldx, tol, max_refine, (hipStream_t)stream, report);
+    });
+}
+
+int mg_solve(mg_numeric *num, int32_t nrhs, const double *b, int32_t ldb, double *x, int32_t ldx, double tol,
+             int32_t max_refine, mg_solve_report *report) {
+    return guarded([&] {
+        require(num != nullptr, MG_E_NULL, "numeric is null.");
+        require(nrhs == 0 || (b && x), MG_E_NULL, "rightHandSide is null.");
+        const int n = num->owner->s.n;
+        require(nrhs >= 0, MG_E_ARG, "nrhs cannot be negative.");
+        require(ldb >= n && ldx >= n, MG_E_ARG, "The right-hand side length does not match the matrix size.");
+        if (n == 0 || nrhs == 0) {
+            if (report) *report = mg_solve_report{0, 1, 0.0};
+            return;
+        }
+        mg::DevArray<double> db((size_t)n * nrhs), dx((size_t)n * nrhs);
+        for (int c = 0; c < nrhs; ++c)
+            MG_HIP(hipMemcpy(db.p + (size_t)c * n, b + (size_t)c * ldb, sizeof(double) * n, hipMemcpyHostToDevice));
+        mg::solve_device(num->num, nrhs, db.p, n, dx.p, n, tol, max_refine, 0, report);
+        MG_HIP(hipDeviceSynchronize());
+        for (int c = 0; c < nrhs; ++c)
+            MG_HIP(hipMemcpy(x + (size_t)c * ldx, dx.p + (size_t)c * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_matrix_create(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *values, mg_matrix **out) {
+    return guarded([&] {
+        require(out != nullptr, MG_E_NULL, "out is null.");
+        check_csr(n, rowptr, colidx);
+        require(rowptr[n] == 0 || values != nullptr, MG_E_NULL, "values are null.");
+        auto h = std::make_unique<mg_matrix>();
+        h->m.build(n, rowptr, colidx);
+        h->m.set_values_host(values);
+        *out = h.release();
+    });
+}
+
+void mg_matrix_release(mg_matrix *m) { delete m; }
+
+int mg_spmv_dev(mg_matrix *m, int32_t nrhs, const double *d_x, int32_t ldx, double *d_y, int32_t ldy, void *stream) {
+    return guarded([&] {
+        require(m != nullptr, MG_E_NULL, "matrix is null.");
+        mg::spmv_device(m->m, nrhs, d_x, ldx, d_y, ldy, (hipStream_t)stream);
+    });
+}
+
+int mg_spmv(mg_matrix *m, const double *x, double *y) {
+    return guarded([&] {
+        require(m != nullptr && x != nullptr && y != nullptr, MG_E_NULL, "argument is null.");
+        const int n = m->m.n;
+        mg::DevArray<double> dx(std::max(n, 1)), dy(std::max(n, 1));
+        MG_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+        mg::spmv_device(m->m, 1, dx.p, n, dy.p, n, 0);
+        MG_HIP(hipDeviceSynchronize());
+        MG_HIP(hipMemcpy(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_pcg_solve_dev(mg_matrix *m, const double *d_b, const double *d_x0, double tol, int32_t max_iterations,
+                     int32_t use_jacobi, double *d_x, void *stream, mg_pcg_result *result) {
+    return guarded([&] {
+        require(m != nullptr && result != nullptr, MG_E_NULL, "argument is null.");
+        require(m->m.n == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        mg::pcg_device(m->m, d_b, d_x0, tol, max_iterations, use_jacobi, d_x, (hipStream_t)stream, *result);
+    });
+}
+
+int mg_pcg_solve(mg_matrix *m, const double *b, const double *x0, double tol, int32_t max_iterations,
+                 int32_t use_jacobi, double *x, mg_pcg_result *result) {
+    return guarded([&] {
+        require(m != nullptr && result != nullptr, MG_E_NULL, "argument is null.");
+        const int n = m->m.n;
+        require(n == 0 || (b && x), MG_E_NULL, "rightHandSide is null.");
+        mg::DevArray<double> db(std::max(n, 1)), dx(std::max(n, 1)), dx0;
+        if (n) MG_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (x0 && n) {
+            dx0.alloc(n);
+            MG_HIP(hipMemcpy(dx0.p, x0, sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+        mg::pcg_device(m->m, db.p, x0 ? dx0.p : nullptr, tol, max_iterations, use_jacobi, dx.p, 0, *result);
+        MG_HIP(hipDeviceSynchronize());
+        if (n) MG_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out) {
+    return guarded([&] {
+        require(num != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
+        mg::numeric_front(num->num, supernode, out);
+    });
+}
+
+int mg_test_mfma_layout(int32_t precision, double *out) {
+    return guarded([&] {
+        require(out != nullptr, MG_E_NULL, "out is null.");
+        mg::selftest_mfma(precision, out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// internal.h -- host-side types shared by the symbolic phase, the device plan
+// and the C ABI of libmagla.so.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/magla.h"
+
+namespace mg {
+
+// Error carrying an mg_status; the C ABI converts it to a return code plus
+// the thread-local message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+struct DevicePlan;  // numeric.hip
+
+// Result of the symbolic phase, restating UtduSymbolicFactorization and the
+// structures it owns (PermutedSymmetricPattern, SupernodalStructure).
+struct Symbolic {
+    int n = 0;
+    int matrix_nnz = 0;
+    mg_options opts{};
+    std::vector<int> a_rowptr, a_colidx;  // structure of the analysed matrix (upper CSR)
+
+    std::vector<int> fill_order;  // AMD (or identity) before the postorder
+    std::vector<int> perm, iperm;
+    std::vector<int> etree, colcount;
+
+    // PermutedSymmetricPattern of the final permutation
+    std::vector<int> pptr, prows, lower_start, lvptr, lvidx, dvidx;
+
+    // SupernodalStructure
+    int nsuper = 0;
+    std::vector<int> first, super_of, sparent, child_ptr, children, pat_ptr, pat_rows;
+    int64_t factor_entries = 0, strict_nonzeros = 0, factor_ops = 0, peak_contribution = 0;
+    int max_front = 0, max_width = 0;
+
+    std::shared_ptr<DevicePlan> plan;  // device index maps and level schedule (lazy)
+
+    int front(int s) const { return pat_ptr[s + 1] - pat_ptr[s]; }
+    int width(int s) const { return first[s + 1] - first[s]; }
+};
+
+// Symbolic phase entry (symbolic.cpp).
+void analyze(int n, const int *rowptr, const int *colidx, const mg_options &opts, Symbolic &out);
+// ApproximateMinimumDegreeOrdering.Compute over the pattern of an upper CSR.
+void amd_order(int n, const int *rowptr, const int *colidx, std::vector<int> &perm);
+
+}  // namespace mg

@@ -1,0 +1,376 @@
+// spmv.hip -- symmetric SpMV and the Jacobi-PCG loop (CGGP) on MI355X.
+//
+// The upper-triangle SymmetricCSRMatrix is expanded once into a full CSR
+// (both triangles, columns ascending) so that y = A x is a row-parallel
+// gather with no atomics: a group of `lanes` lanes owns one row, partial sums
+// are combined by a fixed xor-shuffle tree (deterministic).  Algorithmic
+// traffic per product: 12 B per stored entry + 4 B per row pointer + 16 B per
+// row and right-hand side (x read once, y written once).
+//
+// PCG keeps every vector and scalar in HBM; each iteration is seven short
+// kernels whose reductions are two-stage over a fixed grid (deterministic),
+// and a device flag turns the remaining launches into no-ops once
+// ||r|| <= tol*||b|| (same test, same place as
+// ConjugateGradientGaussPreSolver.cs:151-153).
+#include <algorithm>
+#include <cmath>
+
+#include "device.h"
+
+namespace mg {
+
+namespace {
+constexpr int kRedBlocks = 512;  // fixed grid for deterministic reductions
+constexpr int kBS = 256;
+
+inline unsigned nblk(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
+}  // namespace
+
+__global__ void k_full_gather(const int *__restrict__ src, int64_t nnz, const double *__restrict__ upper,
+                              double *__restrict__ full) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) full[e] = upper[src[e]];
+}
+
+// Jacobi inverse diagonal (JacobiPreconditioner.cs:221-228); status[0] = 1 on
+// a zero / non-finite diagonal.
+__global__ void k_inv_diag(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v, int n,
+                           double *__restrict__ invd, int *__restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) d = v[k];
+    if (!isfinite(d) || fabs(d) <= 1e-300) {
+        atomicOr(status, 1);
+        invd[i] = 0.0;
+    } else {
+        invd[i] = 1.0 / d;
+    }
+}
+
+template <int LANES>
+__global__ __launch_bounds__(256) void k_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
+                                              const double *__restrict__ v, int n, const double *__restrict__ x,
+                                              double *__restrict__ y) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = gid / LANES, sub = gid % LANES;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) acc += v[k] * x[ci[k]];
+    for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+    if (sub == 0) y[row] = acc;
+}
+
+void Matrix::build(int n_, const int *rowptr, const int *colidx) {
+    n = n_;
+    nnz_upper = n ? rowptr[n] : 0;
+    h_rowptr.assign(rowptr, rowptr + n + 1);
+    h_colidx.assign(colidx, colidx + nnz_upper);
+    std::vector<int> frp(n + 1, 0);
+    for (int r = 0; r < n; ++r)
+        for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            ++frp[r + 1];
+            if (colidx[k] != r) ++frp[colidx[k] + 1];
+        }
+    for (int i = 0; i < n; ++i) frp[i + 1] += frp[i];
+    nnz_full = frp[n];
+    std::vector<int> fci(nnz_full), fsrc(nnz_full), at(frp.begin(), frp.end() - 1);
+    for (int r = 0; r < n; ++r)
+        for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const int c = colidx[k];
+            fci[at[r]] = c;
+            fsrc[at[r]++] = k;
+            if (c != r) {
+                fci[at[c]] = r;
+                fsrc[at[c]++] = k;
+            }
+        }
+    // row [lower entries from earlier rows (ascending), own upper entries] is sorted;
+    // rows are filled in ascending r, so the lower part arrives ascending.
+    this->rowptr.upload(frp);
+    this->colidx.upload(fci);
+    src.upload(fsrc);
+    vals.alloc(std::max<int64_t>(nnz_full, 1));
+    invdiag.alloc(std::max(n, 1));
+    const double avg = n ? (double)nnz_full / n : 1.0;
+    lanes = avg > 48 ? 64 : avg > 24 ? 32 : avg > 12 ? 16 : avg > 6 ? 8 : 4;
+}
+
+void Matrix::set_values_dev(const double *d_values, hipStream_t st) {
+    if (nnz_full > 0)
+        hipLaunchKernelGGL(k_full_gather, dim3(nblk(nnz_full, kBS)), dim3(kBS), 0, st, src.p, nnz_full, d_values,
+                           vals.p);
+    MG_HIP(hipGetLastError());
+}
+
+void Matrix::set_values_host(const double *values) {
+    DevArray<double> tmp(std::max(nnz_upper, 1));
+    if (nnz_upper) MG_HIP(hipMemcpy(tmp.p, values, sizeof(double) * nnz_upper, hipMemcpyHostToDevice));
+    set_values_dev(tmp.p, 0);
+    MG_HIP(hipDeviceSynchronize());
+    DevArray<int> status(1);
+    MG_HIP(hipMemset(status.p, 0, sizeof(int)));
+    if (n > 0)
+        hipLaunchKernelGGL(k_inv_diag, dim3(nblk(n, kBS)), dim3(kBS), 0, 0, rowptr.p, colidx.p, vals.p, n, invdiag.p,
+                           status.p);
+    int h = 0;
+    MG_HIP(hipMemcpy(&h, status.p, sizeof(int), hipMemcpyDeviceToHost));
+    diag_status = h ? MG_E_NOT_PD : 0;
+}
+
+static void launch_spmv(const Matrix &A, const double *x, double *y, hipStream_t st) {
+    const int64_t threads = (int64_t)A.n * A.lanes;
+    const dim3 g(nblk(threads, kBS)), b(kBS);
+    switch (A.lanes) {
+        case 4: hipLaunchKernelGGL(k_spmv<4>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y); break;
+        case 8: hipLaunchKernelGGL(k_spmv<8>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y); break;
+        case 16: hipLaunchKernelGGL(k_spmv<16>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y); break;
+        case 32: hipLaunchKernelGGL(k_spmv<32>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y); break;
+        default: hipLaunchKernelGGL(k_spmv<64>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y); break;
+    }
+}
+
+void spmv_device(const Matrix &A, int nrhs, const double *d_x, int ldx, double *d_y, int ldy, hipStream_t st) {
+    if (A.n == 0) return;
+    for (int c = 0; c < nrhs; ++c) launch_spmv(A, d_x + (int64_t)c * ldx, d_y + (int64_t)c * ldy, st);
+    MG_HIP(hipGetLastError());
+}
+
+// ---- PCG ---------------------------------------------------------------------
+
+struct PcgScalars {
+    double rz, pap, alpha, beta, rnorm, thr, bnorm;
+    int done, iters, fail, pad;
+};
+
+__device__ inline double block_sum(double v, double *sm) {
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kBS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
+        __syncthreads();
+    }
+    return sm[0];
+}
+
+// partial[b] = sum_i a_i * c_i over a grid-stride range (fixed grid)
+__global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ a, const double *__restrict__ c, int n,
+                                                  double *__restrict__ part, const PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S && S->done) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) acc += a[i] * c[i];
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__device__ inline double finish_sum(const double *part, double *sm) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += kBS) acc += part[b];
+    return block_sum(acc, sm);
+}
+
+// r = b - A x0 already computed in r; s->rnorm, threshold and early exit
+__global__ void k_pcg_init(const double *__restrict__ part_rr, const double *__restrict__ part_bb, double tol,
+                           PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    const double rr = finish_sum(part_rr, sm);
+    __syncthreads();
+    const double bb = finish_sum(part_bb, sm);
+    if (threadIdx.x == 0) {
+        const double bn = sqrt(bb);
+        const double scale = bn > 0.0 ? bn : 1.0;
+        S->bnorm = scale;
+        S->thr = tol * scale;
+        S->rnorm = sqrt(rr);
+        S->done = S->rnorm <= S->thr ? 1 : 0;
+        S->iters = 0;
+        S->fail = 0;
+    }
+}
+
+__global__ void k_pcg_rz0(const double *__restrict__ part, PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double rz = finish_sum(part, sm);
+    if (threadIdx.x == 0) S->rz = rz;
+}
+
+__global__ void k_residual0(const double *__restrict__ b, const double *__restrict__ ax, int n, double *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - ax[i];
+}
+
+__global__ void k_precond(const double *__restrict__ invd, const double *__restrict__ r, int n, double *__restrict__ z,
+                          int use, const PcgScalars *__restrict__ S) {
+    if (S->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = use ? invd[i] * r[i] : r[i];
+}
+
+template <int LANES>
+__global__ __launch_bounds__(256) void k_spmv_guard(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                    const double *__restrict__ v, int n, const double *__restrict__ x,
+                                                    double *__restrict__ y, const PcgScalars *__restrict__ S) {
+    if (S->done) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = gid / LANES, sub = gid % LANES;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) acc += v[k] * x[ci[k]];
+    for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+    if (sub == 0) y[row] = acc;
+}
+
+// alpha = rz / p'Ap  (ConjugateGradientGaussPreSolver.cs:137-143)
+__global__ void k_pcg_alpha(const double *__restrict__ part, PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double pap = finish_sum(part, sm);
+    if (threadIdx.x == 0) {
+        S->pap = pap;
+        if (pap <= 0.0) {
+            S->fail = 1;
+            S->done = 1;
+        }
+        S->alpha = S->rz / pap;
+    }
+}
+
+// x += alpha p ; r -= alpha Ap ; partial ||r||^2  (:145-151)
+__global__ __launch_bounds__(256) void k_pcg_update(double *__restrict__ x, double *__restrict__ r,
+                                                    const double *__restrict__ p, const double *__restrict__ ap, int n,
+                                                    double *__restrict__ part, const PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double alpha = S->alpha;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * ap[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// convergence test (:151-153)
+__global__ void k_pcg_norm(const double *__restrict__ part, int it, PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double rr = finish_sum(part, sm);
+    if (threadIdx.x == 0) {
+        S->rnorm = sqrt(rr);
+        S->iters = it;
+        if (S->rnorm <= S->thr) S->done = 1;
+    }
+}
+
+// z = M r and partial r'z  (:155-156)
+__global__ __launch_bounds__(256) void k_pcg_precond_dot(const double *__restrict__ invd, const double *__restrict__ r,
+                                                         double *__restrict__ z, int n, int use,
+                                                         double *__restrict__ part, const PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const double zi = use ? invd[i] * r[i] : r[i];
+        z[i] = zi;
+        acc += r[i] * zi;
+    }
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// beta = rz_new / rz (:157), rz = rz_new (:162)
+__global__ void k_pcg_beta(const double *__restrict__ part, PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double rz = finish_sum(part, sm);
+    if (threadIdx.x == 0) {
+        S->beta = rz / S->rz;
+        S->rz = rz;
+    }
+}
+
+// p = z + beta p (:159-160)
+__global__ void k_pcg_pupdate(const double *__restrict__ z, double *__restrict__ p, int n,
+                              const PcgScalars *__restrict__ S) {
+    if (S->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + S->beta * p[i];
+}
+
+static void launch_spmv_guard(const Matrix &A, const double *x, double *y, const PcgScalars *S, hipStream_t st) {
+    const int64_t threads = (int64_t)A.n * A.lanes;
+    const dim3 g(nblk(threads, kBS)), b(kBS);
+    switch (A.lanes) {
+        case 4: hipLaunchKernelGGL(k_spmv_guard<4>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y, S); break;
+        case 8: hipLaunchKernelGGL(k_spmv_guard<8>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y, S); break;
+        case 16: hipLaunchKernelGGL(k_spmv_guard<16>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y, S); break;
+        case 32: hipLaunchKernelGGL(k_spmv_guard<32>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y, S); break;
+        default: hipLaunchKernelGGL(k_spmv_guard<64>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, x, y, S); break;
+    }
+}
+
+// ConjugateGradientGaussPreSolver.SolveWithInitialGuess (CGGP/ConjugateGradientGaussPreSolver.cs:78-166)
+void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double tol, int max_it, int use_jacobi,
+                double *d_x, hipStream_t st, mg_pcg_result &res) {
+    if (!std::isfinite(tol) || tol < 0.0) throw Error(MG_E_RANGE, "RelativeTolerance is out of range.");
+    if (max_it < 0) throw Error(MG_E_RANGE, "MaxIterations: the maximum iteration count cannot be negative.");
+    const int n = A.n;
+    res = mg_pcg_result{0, 1, 0.0, 0.0};
+    if (n == 0) return;
+    if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+    DevArray<double> r(n), z(n), p(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
+    DevArray<PcgScalars> S(1);
+    const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks);
+    // r0 = b - A x0 (:115-117), ||r0||, ||b||
+    launch_spmv(A, d_x, ap.p, st);
+    hipLaunchKernelGGL(k_residual0, gv, bv, 0, st, d_b, ap.p, n, r.p);
+    hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, r.p, r.p, n, part.p, (const PcgScalars *)nullptr);
+    hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, d_b, d_b, n, part2.p, (const PcgScalars *)nullptr);
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), bv, 0, st, part.p, part2.p, tol, S.p);
+    PcgScalars h{};
+    MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    if (!h.done) {
+        if (use_jacobi && A.diag_status)
+            throw Error(MG_E_NOT_PD, "The diagonal element is zero or invalid while building the Jacobi preconditioner.");
+        hipLaunchKernelGGL(k_pcg_precond_dot, gr, bv, 0, st, A.invdiag.p, r.p, z.p, n, use_jacobi, part.p, S.p);
+        hipLaunchKernelGGL(k_pcg_rz0, dim3(1), bv, 0, st, part.p, S.p);
+        MG_HIP(hipMemcpyAsync(p.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        const int maxit = max_it > 0 ? max_it : n;
+        const int check_every = 16;
+        for (int it = 1; it <= maxit; ++it) {
+            launch_spmv_guard(A, p.p, ap.p, S.p, st);
+            hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, p.p, ap.p, n, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_update, gr, bv, 0, st, d_x, r.p, p.p, ap.p, n, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_norm, dim3(1), bv, 0, st, part.p, it, S.p);
+            hipLaunchKernelGGL(k_pcg_precond_dot, gr, bv, 0, st, A.invdiag.p, r.p, z.p, n, use_jacobi, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_beta, dim3(1), bv, 0, st, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_pupdate, gv, bv, 0, st, z.p, p.p, n, S.p);
+            if (it % check_every == 0 || it == maxit) {
+                MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+                MG_HIP(hipStreamSynchronize(st));
+                if (h.done) break;
+            }
+        }
+        MG_HIP(hipGetLastError());
+        MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipStreamSynchronize(st));
+        if (h.fail)
+            throw Error(MG_E_NOT_PD, "p^T A p <= 0. The matrix is not positive definite, so the CG method cannot be applied.");
+        if (!h.done) h.iters = maxit;
+    }
+    res.iterations = h.iters;
+    res.residual_norm = h.rnorm;
+    res.relative_residual = h.rnorm / h.bnorm;
+    res.converged = res.relative_residual <= tol;
+}
+
+}  // namespace mg

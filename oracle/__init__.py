@@ -1,0 +1,270 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product and the CPU
+baseline timed by bench.py.  The product package (magla_amd) never imports
+this module.  Each wrapped C function cites the MAGLA reference lines it
+restates (see oracle.c).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("rowptr", C.POINTER(C.c_int)),
+                ("colidx", C.POINTER(C.c_int)), ("values", C.POINTER(C.c_double)),
+                ("diag", C.POINTER(C.c_int))]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("ordering", C.c_int), ("amalgamation_limit", C.c_int), ("panel_width", C.c_int),
+                ("allow_regularization", C.c_int), ("pivot_tolerance", C.c_double),
+                ("regularization", C.c_double)]
+
+
+class _Sym(C.Structure):
+    _fields_ = [("n", C.c_int), ("matrix_nnz", C.c_int),
+                ("amd", C.POINTER(C.c_int)), ("perm", C.POINTER(C.c_int)), ("iperm", C.POINTER(C.c_int)),
+                ("etree", C.POINTER(C.c_int)), ("colcount", C.POINTER(C.c_int)),
+                ("pptr", C.POINTER(C.c_int)), ("prows", C.POINTER(C.c_int)),
+                ("lower_start", C.POINTER(C.c_int)), ("lvptr", C.POINTER(C.c_int)),
+                ("lvidx", C.POINTER(C.c_int)), ("dvidx", C.POINTER(C.c_int)),
+                ("nsuper", C.c_int), ("first", C.POINTER(C.c_int)), ("super_of", C.POINTER(C.c_int)),
+                ("sparent", C.POINTER(C.c_int)), ("child_ptr", C.POINTER(C.c_int)),
+                ("children", C.POINTER(C.c_int)), ("pat_ptr", C.POINTER(C.c_int)),
+                ("pat_rows", C.POINTER(C.c_int)), ("block_off", C.POINTER(C.c_int64)),
+                ("factor_entries", C.c_int64), ("strict_nonzeros", C.c_int64),
+                ("factor_ops", C.c_int64), ("max_front", C.c_int), ("max_width", C.c_int),
+                ("peak_contribution", C.c_int64)]
+
+
+class _Num(C.Structure):
+    _fields_ = [("sym", C.c_void_p), ("factor", C.POINTER(C.c_double)), ("diag", C.POINTER(C.c_double)),
+                ("regularized", C.c_int), ("negative", C.c_int), ("smallest", C.c_double),
+                ("largest", C.c_double)]
+
+
+class _PcgRes(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual_norm", C.c_double),
+                ("relative_residual", C.c_double)]
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: oracle status {code}")
+        self.code = code
+
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_csr_from_coo.argtypes = [C.c_int, C.c_int64, _i32p, _i32p, _f64p, C.POINTER(C.POINTER(_Csr))]
+        L.or_csr_create.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.POINTER(C.POINTER(_Csr))]
+        L.or_csr_free.argtypes = [C.POINTER(_Csr)]
+        L.or_spmv.argtypes = [C.POINTER(_Csr), _f64p, _f64p]
+        L.or_linecross.argtypes = [C.POINTER(_Csr), _f64p, C.c_double, C.c_int]
+        L.or_amd.argtypes = [C.c_int, _i32p, _i32p, _i32p]
+        L.or_options_default.argtypes = [C.POINTER(_Opts)]
+        L.or_analyze.argtypes = [C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.POINTER(_Sym))]
+        L.or_symbolic_free.argtypes = [C.POINTER(_Sym)]
+        L.or_factorize.argtypes = [C.POINTER(_Sym), C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.POINTER(_Num))]
+        L.or_numeric_free.argtypes = [C.POINTER(_Num)]
+        L.or_solve.argtypes = [C.POINTER(_Num), _f64p, _f64p]
+        L.or_pcg.argtypes = [C.POINTER(_Csr), _f64p, C.c_void_p, C.c_double, C.c_int, C.c_int, _f64p,
+                             C.POINTER(_PcgRes)]
+        _lib = L
+    return _lib
+
+
+def _arr(ptr, count, dtype):
+    if count == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise OracleError(rc, what)
+
+
+class Csr:
+    """Owned oracle CSR (upper triangle + diagonal)."""
+
+    def __init__(self, ptr):
+        self._p = ptr
+
+    @classmethod
+    def from_coo(cls, n, rows, cols, vals):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        p = C.POINTER(_Csr)()
+        _check(lib().or_csr_from_coo(n, len(rows), rows, cols, vals, C.byref(p)), "csr_from_coo")
+        return cls(p)
+
+    @classmethod
+    def from_csr(cls, n, rowptr, colidx, values):
+        p = C.POINTER(_Csr)()
+        _check(lib().or_csr_create(n, np.ascontiguousarray(rowptr, np.int32),
+                                   np.ascontiguousarray(colidx, np.int32),
+                                   np.ascontiguousarray(values, np.float64), C.byref(p)), "csr_create")
+        return cls(p)
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_csr_free(self._p)
+            self._p = None
+
+    @property
+    def n(self):
+        return self._p.contents.n
+
+    @property
+    def nnz(self):
+        return self._p.contents.nnz
+
+    def arrays(self):
+        c = self._p.contents
+        return (_arr(c.rowptr, c.n + 1, np.int32), _arr(c.colidx, c.nnz, np.int32),
+                _arr(c.values, c.nnz, np.float64))
+
+    def diag_index(self):
+        c = self._p.contents
+        return _arr(c.diag, c.n, np.int32)
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n, dtype=np.float64)
+        lib().or_spmv(self._p, x, y)
+        return y
+
+    def linecross(self, rhs, prescribed, index):
+        """In place on both the matrix values and rhs (a numpy float64 array)."""
+        _check(lib().or_linecross(self._p, rhs, float(prescribed), int(index)), "linecross")
+
+    def dense(self):
+        rp, ci, v = self.arrays()
+        a = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for k in range(rp[r], rp[r + 1]):
+                a[r, ci[k]] = v[k]
+                a[ci[k], r] = v[k]
+        return a
+
+
+def options(ordering=1, amalgamation_limit=64, panel_width=32, allow_regularization=True,
+            pivot_tolerance=1e-13, regularization=1e-10):
+    o = _Opts()
+    lib().or_options_default(C.byref(o))
+    o.ordering = ordering
+    o.amalgamation_limit = amalgamation_limit
+    o.panel_width = panel_width
+    o.allow_regularization = 1 if allow_regularization else 0
+    o.pivot_tolerance = pivot_tolerance
+    o.regularization = regularization
+    return o
+
+
+def amd(n, rowptr, colidx):
+    perm = np.zeros(n, dtype=np.int32)
+    _check(lib().or_amd(n, np.ascontiguousarray(rowptr, np.int32), np.ascontiguousarray(colidx, np.int32), perm),
+           "amd")
+    return perm
+
+
+class Symbolic:
+    def __init__(self, csr: Csr, opts=None):
+        self.opts = opts or options()
+        p = C.POINTER(_Sym)()
+        _check(lib().or_analyze(csr._p, C.byref(self.opts), C.byref(p)), "analyze")
+        self._p = p
+        s = p.contents
+        n, ns = s.n, s.nsuper
+        self.n, self.nsuper = n, ns
+        self.amd = _arr(s.amd, n, np.int32)
+        self.perm = _arr(s.perm, n, np.int32)
+        self.iperm = _arr(s.iperm, n, np.int32)
+        self.etree = _arr(s.etree, n, np.int32)
+        self.colcount = _arr(s.colcount, n, np.int32)
+        self.first = _arr(s.first, ns + 1, np.int32)
+        self.sparent = _arr(s.sparent, ns, np.int32)
+        self.child_ptr = _arr(s.child_ptr, ns + 1, np.int32)
+        self.children = _arr(s.children, int(self.child_ptr[-1]), np.int32)
+        self.pat_ptr = _arr(s.pat_ptr, ns + 1, np.int32)
+        self.pat_rows = _arr(s.pat_rows, int(self.pat_ptr[-1]), np.int32)
+        self.block_off = _arr(s.block_off, ns + 1, np.int64)
+        self.factor_entries = s.factor_entries
+        self.strict_nonzeros = s.strict_nonzeros
+        self.factor_ops = s.factor_ops
+        self.max_front = s.max_front
+        self.max_width = s.max_width
+        self.peak_contribution = s.peak_contribution
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_symbolic_free(self._p)
+            self._p = None
+
+
+class Numeric:
+    def __init__(self, sym: Symbolic, csr: Csr):
+        self.sym = sym
+        p = C.POINTER(_Num)()
+        _check(lib().or_factorize(sym._p, csr._p, C.byref(sym.opts), C.byref(p)), "factorize")
+        self._p = p
+        f = p.contents
+        self.diag = _arr(f.diag, sym.n, np.float64)
+        self.factor = _arr(f.factor, int(sym.factor_entries), np.float64)
+        self.regularized = f.regularized
+        self.negative = f.negative
+        self.smallest = f.smallest
+        self.largest = f.largest
+
+    @property
+    def is_positive_definite(self):
+        return self.regularized == 0 and self.negative == 0
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        _check(lib().or_solve(self._p, b, x), "solve")
+        return x
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_numeric_free(self._p)
+            self._p = None
+
+
+def pcg(csr: Csr, b, x0=None, tol=1e-8, max_iterations=0, use_jacobi=True):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(csr.n, dtype=np.float64)
+    res = _PcgRes()
+    x0p = None
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        x0p = x0.ctypes.data_as(C.c_void_p)
+    _check(lib().or_pcg(csr._p, b, x0p, tol, max_iterations, 1 if use_jacobi else 0, x, C.byref(res)), "pcg")
+    return x, dict(iterations=res.iterations, converged=bool(res.converged),
+                   residual_norm=res.residual_norm, relative_residual=res.relative_residual)

@@ -1,0 +1,1328 @@
+/*
+ * oracle.c -- CPU restatement of the MAGLA UCHOL / AMD / CGGP path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * product library and the CPU timing baseline; the product never links,
+ * loads or calls it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may use it.
+ *
+ * Reference files (MAGLA repository, CAESolvers/...) are cited per function.
+ * Operation order follows the C# exactly so that, compiled with
+ * -ffp-contract=off -fwrapv, the FP64 factor equals the reference's bit for
+ * bit (the reference's per-element sums never depend on Vector<double>.Count;
+ * the one exception, DenseKernels.Dot in the backward solve, is restated for
+ * Vector<double>.Count == 4, see or_dot4).
+ *
+ * Parity pins (tests/test_oracle_*.py): CgComparisonData and workbook values,
+ * closed forms, dense LDL^T / solve in numpy, brute-force symbolic
+ * elimination for etree / column counts / supernode patterns, hand traces of
+ * the reference AMD on small graphs.  "Parity unpinned" (no .NET here):
+ * bit-exactness of the AMD permutation against the C# binary itself, and the
+ * last bit of the backward-solve dot product.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EMPTY (-1)
+
+static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz ? sz : 1); }
+static void *xmalloc(size_t n) { return malloc(n ? n : 1); }
+
+/* ======================================================================
+ * SymmetricCSRMatrix
+ * ====================================================================== */
+
+typedef struct {
+    int row, col;
+    int64_t order;
+    double value;
+} coo_item;
+
+static int coo_cmp(const void *pa, const void *pb)
+{
+    const coo_item *a = (const coo_item *)pa, *b = (const coo_item *)pb;
+    if (a->row != b->row) return a->row < b->row ? -1 : 1;
+    if (a->col != b->col) return a->col < b->col ? -1 : 1;
+    return a->order < b->order ? -1 : (a->order > b->order ? 1 : 0);
+}
+
+/* SymmetricCSRMatrix.cs:36-90 -- normalise (min,max) (:40-52), sum duplicate
+ * positions in input order (:54-56, Enumerable.Sum adds sequentially from
+ * 0.0), drop |sum| <= 1e-15 (:57, NaN is dropped too), columns ascending
+ * (:59), diagonal index table (:66-87).  Out-of-range indices are rejected as
+ * SymmetricCSRMatrixBuilder.AddToElement does (SymmetricCSRMatrixBuilder.cs:24-25). */
+int or_csr_from_coo(int n, int64_t count, const int *rows, const int *cols,
+                    const double *vals, or_csr **out)
+{
+    if (!out || (count > 0 && (!rows || !cols || !vals))) return OR_E_NULL;
+    if (n < 0 || count < 0) return OR_E_ARG;
+    for (int64_t k = 0; k < count; k++)
+        if (rows[k] < 0 || rows[k] >= n || cols[k] < 0 || cols[k] >= n) return OR_E_RANGE;
+
+    coo_item *items = (coo_item *)xmalloc((size_t)count * sizeof(coo_item));
+    for (int64_t k = 0; k < count; k++) {
+        int r = rows[k], c = cols[k];
+        if (r > c) { int t = r; r = c; c = t; }
+        items[k].row = r; items[k].col = c; items[k].order = k; items[k].value = vals[k];
+    }
+    qsort(items, (size_t)count, sizeof(coo_item), coo_cmp);
+
+    /* reduce groups */
+    int64_t kept = 0;
+    for (int64_t k = 0; k < count;) {
+        int64_t e = k;
+        double sum = 0.0;
+        while (e < count && items[e].row == items[k].row && items[e].col == items[k].col) {
+            sum += items[e].value;
+            e++;
+        }
+        if (fabs(sum) > 1e-15) {
+            items[kept] = items[k];
+            items[kept].value = sum;
+            kept++;
+        }
+        k = e;
+    }
+
+    or_csr *a = (or_csr *)xcalloc(1, sizeof(or_csr));
+    a->n = n;
+    a->nnz = (int)kept;
+    a->rowptr = (int *)xcalloc((size_t)n + 1, sizeof(int));
+    a->colidx = (int *)xmalloc((size_t)kept * sizeof(int));
+    a->values = (double *)xmalloc((size_t)kept * sizeof(double));
+    a->diag = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; i++) a->diag[i] = EMPTY;
+    for (int64_t k = 0; k < kept; k++) {
+        a->rowptr[items[k].row + 1]++;
+        a->colidx[k] = items[k].col;
+        a->values[k] = items[k].value;
+        if (items[k].row == items[k].col) a->diag[items[k].row] = (int)k;
+    }
+    for (int i = 0; i < n; i++) a->rowptr[i + 1] += a->rowptr[i];
+    free(items);
+    *out = a;
+    return OR_OK;
+}
+
+int or_csr_create(int n, const int *rowptr, const int *colidx, const double *values, or_csr **out)
+{
+    if (!out || !rowptr || (n > 0 && (!colidx || !values))) return OR_E_NULL;
+    int nnz = rowptr[n];
+    or_csr *a = (or_csr *)xcalloc(1, sizeof(or_csr));
+    a->n = n;
+    a->nnz = nnz;
+    a->rowptr = (int *)xmalloc(((size_t)n + 1) * sizeof(int));
+    a->colidx = (int *)xmalloc((size_t)nnz * sizeof(int));
+    a->values = (double *)xmalloc((size_t)nnz * sizeof(double));
+    a->diag = (int *)xmalloc((size_t)n * sizeof(int));
+    memcpy(a->rowptr, rowptr, ((size_t)n + 1) * sizeof(int));
+    if (nnz) {
+        memcpy(a->colidx, colidx, (size_t)nnz * sizeof(int));
+        memcpy(a->values, values, (size_t)nnz * sizeof(double));
+    }
+    for (int i = 0; i < n; i++) {
+        a->diag[i] = EMPTY;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++)
+            if (colidx[k] == i) a->diag[i] = k;
+    }
+    *out = a;
+    return OR_OK;
+}
+
+void or_csr_free(or_csr *a)
+{
+    if (!a) return;
+    free(a->rowptr); free(a->colidx); free(a->values); free(a->diag);
+    free(a);
+}
+
+/* SymmetricCSRMatrix.cs:293-324 -- row accumulator plus mirrored scatter. */
+void or_spmv(const or_csr *a, const double *x, double *y)
+{
+    int n = a->n;
+    for (int i = 0; i < n; i++) y[i] = 0.0;
+    for (int row = 0; row < n; row++) {
+        double xrow = x[row];
+        double acc = 0.0;
+        int end = a->rowptr[row + 1];
+        for (int k = a->rowptr[row]; k < end; k++) {
+            int col = a->colidx[k];
+            double v = a->values[k];
+            acc += v * x[col];
+            if (col != row) y[col] += v * xrow;
+        }
+        y[row] += acc;
+    }
+}
+
+/* SymmetricCSRMatrix.cs:125-167 */
+int or_linecross(or_csr *a, double *rhs, double prescribed, int index)
+{
+    if (!a || !rhs) return OR_E_NULL;
+    if (index < 0 || index >= a->n) return OR_E_RANGE;
+    int d = a->diag[index];
+    if (d < 0) return OR_E_NOT_PD; /* InvalidOperationException: diagonal absent */
+    for (int row = 0; row < a->n; row++) {
+        for (int pos = a->rowptr[row]; pos < a->rowptr[row + 1]; pos++) {
+            int col = a->colidx[pos];
+            if (row == col) continue;
+            int other;
+            if (row == index) other = col;
+            else if (col == index) other = row;
+            else continue;
+            rhs[other] -= a->values[pos] * prescribed;
+            a->values[pos] = 0.0;
+        }
+    }
+    a->values[d] = 1.0;
+    rhs[index] = prescribed;
+    return OR_OK;
+}
+
+double or_norm2(int n, const double *v)
+{
+    /* ResidualCalculator.cs:99-107 */
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += v[i] * v[i];
+    return sqrt(s);
+}
+
+/* ======================================================================
+ * AMD -- AMD/ApproximateMinimumDegreeOrdering.cs and its helper classes.
+ * Signed arithmetic on wflg / w[] wraps as C# unchecked int does (-fwrapv).
+ * ====================================================================== */
+
+typedef struct {
+    int n;
+    /* FactorGraph (AMD/FactorGraph.cs) */
+    int *iw;
+    int iwlen;
+    int pfree;
+    int *pe, *len, *elen;
+    int *corder, *ckeys;
+    /* WorkSpace arrays */
+    int *nv, *degree, *w;
+    int wflg, wbig;
+    /* DegreeBuckets (AMD/DegreeBuckets.cs) */
+    int *dhead, *dnext, *dlast;
+    int mindeg;
+    /* HashBuckets (AMD/HashBuckets.cs) */
+    int *hhead, *hnext, *hprev, *hused;
+    int hcount;
+    /* MemberLists (AMD/MemberLists.cs) */
+    int *mtail, *mnext;
+    /* output */
+    int *perm;
+    int pcur;
+    int *dense;
+    int ndense;
+} amd_state;
+
+static inline int flip(int i) { return -i - 2; }
+
+/* DegreeBuckets.cs:56-69 */
+static void deg_insert(amd_state *s, int i, int d)
+{
+    int following = s->dhead[d];
+    if (following != EMPTY) s->dlast[following] = i;
+    s->dnext[i] = following;
+    s->dlast[i] = EMPTY;
+    s->dhead[d] = i;
+    if (d < s->mindeg) s->mindeg = d;
+}
+
+/* DegreeBuckets.cs:77-89 */
+static void deg_remove(amd_state *s, int i)
+{
+    int d = s->degree[i];
+    int previous = s->dlast[i], following = s->dnext[i];
+    if (following != EMPTY) s->dlast[following] = previous;
+    if (previous != EMPTY) s->dnext[previous] = following;
+    else s->dhead[d] = following;
+}
+
+/* DegreeBuckets.cs:96-114 */
+static int deg_pop_min(amd_state *s, int maxdeg)
+{
+    for (int d = s->mindeg; d <= maxdeg; d++) {
+        int cand = s->dhead[d];
+        if (cand == EMPTY) continue;
+        s->mindeg = d;
+        int following = s->dnext[cand];
+        if (following != EMPTY) s->dlast[following] = EMPTY;
+        s->dhead[d] = following;
+        return cand;
+    }
+    return EMPTY;
+}
+
+/* HashBuckets.cs:54-67 */
+static void hash_add(amd_state *s, int64_t hash, int i)
+{
+    int bucket = (int)(hash % s->n);
+    int old = s->hhead[bucket];
+    if (old == EMPTY) s->hused[s->hcount++] = bucket;
+    else s->hprev[old] = i;
+    s->hnext[i] = old;
+    s->hprev[i] = EMPTY;
+    s->hhead[bucket] = i;
+}
+
+/* HashBuckets.cs:86-94 */
+static void hash_unlink(amd_state *s, int i)
+{
+    int p = s->hprev[i], after = s->hnext[i];
+    s->hnext[p] = after;
+    if (after != EMPTY) s->hprev[after] = p;
+}
+
+/* MemberLists.cs:42-46 (head[i] == i for every i, never rewritten) */
+static void member_append(amd_state *s, int target, int source)
+{
+    s->mnext[s->mtail[target]] = source;
+    s->mtail[target] = s->mtail[source];
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:666-670 */
+static void emit(amd_state *s, int rep)
+{
+    for (int v = rep; v != EMPTY; v = s->mnext[v]) s->perm[s->pcur++] = v;
+}
+
+typedef struct { int key, val; } kv_pair;
+static int kv_cmp(const void *pa, const void *pb)
+{
+    int a = ((const kv_pair *)pa)->key, b = ((const kv_pair *)pb)->key;
+    return a < b ? -1 : (a > b);
+}
+
+/* FactorGraph.cs:218-266 -- live lists (Pe>=0, Len>0) moved left in offset
+ * order, then the half-built element; grow when still short of room. */
+static int fg_compact(amd_state *s, int *first_entry)
+{
+    int n = s->n, count = 0;
+    kv_pair *pairs = (kv_pair *)xmalloc((size_t)n * sizeof(kv_pair));
+    for (int j = 0; j < n; j++)
+        if (s->pe[j] >= 0 && s->len[j] > 0) { pairs[count].key = s->pe[j]; pairs[count].val = j; count++; }
+    qsort(pairs, (size_t)count, sizeof(kv_pair), kv_cmp);
+    int dest = 0;
+    for (int t = 0; t < count; t++) {
+        int j = pairs[t].val, src = s->pe[j], length = s->len[j];
+        if (dest != src) memmove(s->iw + dest, s->iw + src, (size_t)length * sizeof(int));
+        s->pe[j] = dest;
+        dest += length;
+    }
+    free(pairs);
+    int partial = s->pfree - *first_entry;
+    if (partial > 0 && dest != *first_entry)
+        memmove(s->iw + dest, s->iw + *first_entry, (size_t)partial * sizeof(int));
+    *first_entry = dest;
+    s->pfree = dest + partial;
+    int required = s->pfree + n + 16;
+    if (required > s->iwlen) {
+        int grown = s->iwlen + s->iwlen / 2;
+        s->iwlen = required > grown ? required : grown;
+        int *niw = (int *)realloc(s->iw, (size_t)s->iwlen * sizeof(int));
+        if (!niw) return OR_E_NOMEM;
+        s->iw = niw;
+    }
+    return OR_OK;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:647-659 */
+static void ensure_mark_capacity(amd_state *s, int required)
+{
+    if (s->wflg + required < s->wbig) return;
+    for (int x = 0; x < s->n; x++)
+        if (s->w[x] != 0) s->w[x] = 1;
+    s->wflg = 2;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:231-269 */
+static int init_degree_lists(amd_state *s)
+{
+    int n = s->n;
+    int dense = (int)(10.0 * sqrt((double)n));
+    if (dense < 16) dense = 16;
+    if (dense > n) dense = n;
+    int eliminated = 0;
+    for (int i = 0; i < n; i++) {
+        int d = s->degree[i];
+        if (d == 0) {
+            s->pe[i] = EMPTY; s->w[i] = 0; s->nv[i] = 0; s->elen[i] = EMPTY;
+            eliminated++;
+            emit(s, i);
+        } else if (d > dense) {
+            s->pe[i] = EMPTY; s->nv[i] = 0; s->elen[i] = EMPTY;
+            eliminated++;
+            s->dense[s->ndense++] = i;
+        } else {
+            deg_insert(s, i, d);
+        }
+    }
+    return eliminated;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:285-304 */
+static void build_from_variables(amd_state *s, int pivot, int *edeg, int *fe, int *le)
+{
+    int degsum = 0, first = s->pe[pivot], last = first - 1;
+    int end = first + s->len[pivot];
+    for (int p = first; p < end; p++) {
+        int i = s->iw[p], nvi = s->nv[i];
+        if (nvi <= 0) continue;
+        degsum += nvi;
+        s->nv[i] = -nvi;
+        s->iw[++last] = i;
+        deg_remove(s, i);
+    }
+    *edeg = degsum; *fe = first; *le = last;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:313-393 */
+static int build_from_elements(amd_state *s, int pivot, int pivot_elen, int *edeg, int *fe, int *le)
+{
+    int degsum = 0;
+    int p = s->pe[pivot];
+    int first = s->pfree;
+    int pvc = s->len[pivot] - pivot_elen;
+    for (int outer = 1; outer <= pivot_elen + 1; outer++) {
+        int e, cursor, count;
+        int own = outer > pivot_elen;
+        if (own) { e = pivot; cursor = p; count = pvc; }
+        else { e = s->iw[p++]; cursor = s->pe[e]; count = s->len[e]; }
+        for (int inner = 1; inner <= count; inner++) {
+            int i = s->iw[cursor++], nvi = s->nv[i];
+            if (nvi <= 0) continue;
+            if (s->pfree >= s->iwlen) {
+                if (!own) {
+                    s->pe[pivot] = p;
+                    s->len[pivot] = (pivot_elen - outer) + pvc;
+                    if (s->len[pivot] == 0) s->pe[pivot] = EMPTY;
+                }
+                s->pe[e] = cursor;
+                s->len[e] = count - inner;
+                if (s->len[e] == 0) s->pe[e] = EMPTY;
+                int rc = fg_compact(s, &first);
+                if (rc) return rc;
+                cursor = s->pe[e];
+                p = own ? cursor : s->pe[pivot];
+            }
+            degsum += nvi;
+            s->nv[i] = -nvi;
+            s->iw[s->pfree++] = i;
+            deg_remove(s, i);
+        }
+        if (e != pivot) { s->pe[e] = flip(pivot); s->w[e] = 0; }
+    }
+    *edeg = degsum; *fe = first; *le = s->pfree - 1;
+    return OR_OK;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:402-428 */
+static void element_set_differences(amd_state *s, int fe, int le)
+{
+    for (int pos = fe; pos <= le; pos++) {
+        int i = s->iw[pos], ec = s->elen[i];
+        if (ec <= 0) continue;
+        int nvi = -s->nv[i], wnvi = s->wflg - nvi;
+        int end = s->pe[i] + ec;
+        for (int p = s->pe[i]; p < end; p++) {
+            int e = s->iw[p], we = s->w[e];
+            if (we >= s->wflg) we -= nvi;
+            else if (we != 0) we = s->degree[e] + wnvi;
+            s->w[e] = we;
+        }
+    }
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:436-522 */
+static void approximate_degrees(amd_state *s, int pivot, int fe, int le, int *edeg, int *pnv, int *elim)
+{
+    s->hcount = 0;
+    for (int pos = fe; pos <= le; pos++) {
+        int i = s->iw[pos];
+        int ls = s->pe[i], eend = ls + s->elen[i], wr = ls;
+        int64_t hash = 0;
+        int d = 0;
+        for (int p = ls; p < eend; p++) {
+            int e = s->iw[p];
+            if (s->w[e] == 0) continue;
+            int ext = s->w[e] - s->wflg;
+            if (ext > 0) { d += ext; s->iw[wr++] = e; hash += e; }
+            else if (ext == 0) { s->pe[e] = flip(pivot); s->w[e] = 0; }
+        }
+        s->elen[i] = wr - ls + 1;
+        int vs = wr, lend = ls + s->len[i];
+        for (int p = eend; p < lend; p++) {
+            int j = s->iw[p], nvj = s->nv[j];
+            if (nvj <= 0) continue;
+            d += nvj;
+            s->iw[wr++] = j;
+            hash += j;
+        }
+        if (s->elen[i] == 1 && vs == wr) {
+            s->pe[i] = flip(pivot);
+            int nvi = -s->nv[i];
+            *edeg -= nvi; *pnv += nvi; *elim += nvi;
+            s->nv[i] = 0;
+            s->elen[i] = EMPTY;
+            member_append(s, pivot, i);
+        } else {
+            if (d < s->degree[i]) s->degree[i] = d;
+            s->iw[wr] = s->iw[vs];
+            s->iw[vs] = s->iw[ls];
+            s->iw[ls] = pivot;
+            s->len[i] = wr - ls + 1;
+            hash_add(s, hash, i);
+        }
+    }
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:531-592 */
+static void detect_indistinguishable(amd_state *s)
+{
+    for (int b = 0; b < s->hcount; b++) {
+        int bucket = s->hused[b];
+        for (int i = s->hhead[bucket]; i != EMPTY; i = s->hnext[i]) {
+            if (s->nv[i] >= 0) continue;
+            int length = s->len[i], ec = s->elen[i];
+            int start = s->pe[i] + 1, end = s->pe[i] + length;
+            for (int p = start; p < end; p++) s->w[s->iw[p]] = s->wflg;
+            int j = s->hnext[i];
+            while (j != EMPTY) {
+                int same = s->nv[j] < 0 && s->len[j] == length && s->elen[j] == ec;
+                if (same) {
+                    int js = s->pe[j] + 1, je = s->pe[j] + length;
+                    for (int p = js; p < je; p++)
+                        if (s->w[s->iw[p]] != s->wflg) { same = 0; break; }
+                }
+                int next = s->hnext[j];
+                if (same) {
+                    s->pe[j] = flip(i);
+                    s->nv[i] += s->nv[j];
+                    s->nv[j] = 0;
+                    s->elen[j] = EMPTY;
+                    member_append(s, i, j);
+                    hash_unlink(s, j);
+                }
+                j = next;
+            }
+            s->wflg++;
+        }
+        s->hhead[bucket] = EMPTY;
+    }
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:599-640 */
+static void restore_degree_lists(amd_state *s, int pivot, int pivot_elen, int edeg, int pnv, int elim, int fe, int le)
+{
+    int wr = fe, remaining = s->n - elim;
+    for (int pos = fe; pos <= le; pos++) {
+        int i = s->iw[pos], nvi = -s->nv[i];
+        if (nvi <= 0) continue;
+        s->nv[i] = nvi;
+        int d = s->degree[i] + edeg - nvi;
+        if (remaining - nvi < d) d = remaining - nvi;
+        if (d < 0) d = 0;
+        s->degree[i] = d;
+        deg_insert(s, i, d);
+        s->iw[wr++] = i;
+    }
+    s->nv[pivot] = pnv;
+    s->len[pivot] = wr - fe;
+    if (s->len[pivot] == 0) { s->pe[pivot] = EMPTY; s->w[pivot] = 0; }
+    if (pivot_elen != 0) s->pfree = wr;
+}
+
+/* ApproximateMinimumDegreeOrdering.cs:71-84 + WorkSpace ctor (:129-155),
+ * Run (:157-224); SymmetricPatternGraph.FromMatrix (AMD/SymmetricPatternGraph.cs:71-121);
+ * FactorGraph ctor (AMD/FactorGraph.cs:171-195). */
+int or_amd(int n, const int *rowptr, const int *colidx, int *perm_out)
+{
+    if (n == 0) return OR_OK;
+    /* pattern graph: degree pass then placement pass, both in CSR order */
+    int *gptr = (int *)xcalloc((size_t)n + 1, sizeof(int));
+    for (int row = 0; row < n; row++)
+        for (int k = rowptr[row]; k < rowptr[row + 1]; k++) {
+            int col = colidx[k];
+            if (col == row) continue;
+            gptr[row + 1]++;
+            gptr[col + 1]++;
+        }
+    for (int i = 0; i < n; i++) gptr[i + 1] += gptr[i];
+    int edges = gptr[n];
+    int *cursor = (int *)xmalloc((size_t)n * sizeof(int));
+    memcpy(cursor, gptr, (size_t)n * sizeof(int));
+
+    amd_state S;
+    memset(&S, 0, sizeof(S));
+    amd_state *s = &S;
+    s->n = n;
+    s->iwlen = edges + edges / 5 + n + 16;
+    s->iw = (int *)xmalloc((size_t)s->iwlen * sizeof(int));
+    for (int row = 0; row < n; row++)
+        for (int k = rowptr[row]; k < rowptr[row + 1]; k++) {
+            int col = colidx[k];
+            if (col == row) continue;
+            s->iw[cursor[row]++] = col;
+            s->iw[cursor[col]++] = row;
+        }
+    free(cursor);
+    s->pfree = edges;
+    s->pe = (int *)xmalloc((size_t)n * sizeof(int));
+    s->len = (int *)xmalloc((size_t)n * sizeof(int));
+    s->elen = (int *)xcalloc((size_t)n, sizeof(int));
+    s->nv = (int *)xmalloc((size_t)n * sizeof(int));
+    s->degree = (int *)xmalloc((size_t)n * sizeof(int));
+    s->w = (int *)xmalloc((size_t)n * sizeof(int));
+    s->dhead = (int *)xmalloc(((size_t)n + 1) * sizeof(int));
+    s->dnext = (int *)xmalloc((size_t)n * sizeof(int));
+    s->dlast = (int *)xmalloc((size_t)n * sizeof(int));
+    s->hhead = (int *)xmalloc((size_t)n * sizeof(int));
+    s->hnext = (int *)xmalloc((size_t)n * sizeof(int));
+    s->hprev = (int *)xmalloc((size_t)n * sizeof(int));
+    s->hused = (int *)xmalloc((size_t)n * sizeof(int));
+    s->mtail = (int *)xmalloc((size_t)n * sizeof(int));
+    s->mnext = (int *)xmalloc((size_t)n * sizeof(int));
+    s->dense = (int *)xmalloc((size_t)n * sizeof(int));
+    s->perm = perm_out;
+    for (int i = 0; i <= n; i++) s->dhead[i] = EMPTY;
+    for (int i = 0; i < n; i++) {
+        s->pe[i] = gptr[i];
+        s->len[i] = gptr[i + 1] - gptr[i];
+        s->nv[i] = 1;
+        s->w[i] = 1;
+        s->degree[i] = s->len[i];
+        s->dnext[i] = s->dlast[i] = EMPTY;
+        s->hhead[i] = s->hnext[i] = s->hprev[i] = EMPTY;
+        s->mtail[i] = i;
+        s->mnext[i] = EMPTY;
+    }
+    free(gptr);
+    s->wbig = 2147483647 - n - 1;
+    s->mindeg = 0;
+
+    int rc = OR_OK;
+    int eliminated = init_degree_lists(s);
+    s->wflg = 2;
+    s->mindeg = 0;
+    int max_element = 0;
+    while (eliminated < n) {
+        ensure_mark_capacity(s, 2 * n + 2);
+        int pivot = deg_pop_min(s, n);
+        if (pivot == EMPTY) { rc = OR_E_INTERNAL; break; }
+        int pivot_elen = s->elen[pivot];
+        int pnv = s->nv[pivot];
+        eliminated += pnv;
+        s->nv[pivot] = -pnv;
+        int edeg, fe, le;
+        if (pivot_elen == 0) build_from_variables(s, pivot, &edeg, &fe, &le);
+        else {
+            rc = build_from_elements(s, pivot, pivot_elen, &edeg, &fe, &le);
+            if (rc) break;
+        }
+        s->pe[pivot] = fe;
+        s->len[pivot] = le - fe + 1;
+        element_set_differences(s, fe, le);
+        approximate_degrees(s, pivot, fe, le, &edeg, &pnv, &eliminated);
+        s->degree[pivot] = edeg;
+        s->elen[pivot] = flip(pnv + edeg);
+        if (edeg > max_element) max_element = edeg;
+        detect_indistinguishable(s);
+        restore_degree_lists(s, pivot, pivot_elen, edeg, pnv, eliminated, fe, le);
+        emit(s, pivot);
+        s->wflg += max_element + 1;
+    }
+    if (rc == OR_OK) {
+        for (int t = 0; t < s->ndense; t++) emit(s, s->dense[t]);
+        if (s->pcur != n) rc = OR_E_INTERNAL;
+    }
+    free(s->iw); free(s->pe); free(s->len); free(s->elen); free(s->nv); free(s->degree); free(s->w);
+    free(s->dhead); free(s->dnext); free(s->dlast); free(s->hhead); free(s->hnext); free(s->hprev);
+    free(s->hused); free(s->mtail); free(s->mnext); free(s->dense);
+    return rc;
+}
+
+/* ======================================================================
+ * Symbolic phase -- UCHOL/UtduSymbolicFactorization.cs, PermutedSymmetricPattern.cs,
+ * EliminationTree.cs, SupernodalStructure.cs
+ * ====================================================================== */
+
+typedef struct {
+    int n;
+    int *ptr, *rows, *lower_start, *lvptr, *lvidx, *dvidx;
+} perm_pattern;
+
+static int int_cmp(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* PermutedSymmetricPattern.cs:81-185 */
+static void pattern_create(const or_csr *a, const int *iperm, perm_pattern *pp)
+{
+    int n = a->n;
+    pp->n = n;
+    pp->ptr = (int *)xcalloc((size_t)n + 1, sizeof(int));
+    pp->lvptr = (int *)xcalloc((size_t)n + 1, sizeof(int));
+    pp->dvidx = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; i++) pp->dvidx[i] = EMPTY;
+    for (int row = 0; row < n; row++) {
+        int prow = iperm[row];
+        for (int k = a->rowptr[row]; k < a->rowptr[row + 1]; k++) {
+            int col = a->colidx[k];
+            if (col == row) { pp->dvidx[prow] = k; continue; }
+            int pcol = iperm[col];
+            pp->ptr[prow + 1]++;
+            pp->ptr[pcol + 1]++;
+            pp->lvptr[(prow < pcol ? prow : pcol) + 1]++;
+        }
+    }
+    for (int j = 0; j < n; j++) { pp->ptr[j + 1] += pp->ptr[j]; pp->lvptr[j + 1] += pp->lvptr[j]; }
+    pp->rows = (int *)xmalloc((size_t)pp->ptr[n] * sizeof(int));
+    pp->lvidx = (int *)xmalloc((size_t)pp->lvptr[n] * sizeof(int));
+    int *lrows = (int *)xmalloc((size_t)pp->lvptr[n] * sizeof(int));
+    int *cur = (int *)xmalloc((size_t)n * sizeof(int));
+    int *lcur = (int *)xmalloc((size_t)n * sizeof(int));
+    memcpy(cur, pp->ptr, (size_t)n * sizeof(int));
+    memcpy(lcur, pp->lvptr, (size_t)n * sizeof(int));
+    for (int row = 0; row < n; row++) {
+        int prow = iperm[row];
+        for (int k = a->rowptr[row]; k < a->rowptr[row + 1]; k++) {
+            int col = a->colidx[k];
+            if (col == row) continue;
+            int pcol = iperm[col];
+            pp->rows[cur[prow]++] = pcol;
+            pp->rows[cur[pcol]++] = prow;
+            int low = prow < pcol ? prow : pcol, high = prow + pcol - low;
+            int pos = lcur[low]++;
+            lrows[pos] = high;
+            pp->lvidx[pos] = k;
+        }
+    }
+    /* column sorts (:485-496); keys are unique, so any sort gives one answer */
+    kv_pair *tmp = NULL;
+    size_t tmpcap = 0;
+    for (int j = 0; j < n; j++) {
+        int st = pp->ptr[j], ln = pp->ptr[j + 1] - st;
+        if (ln > 1) qsort(pp->rows + st, (size_t)ln, sizeof(int), int_cmp);
+        int ls = pp->lvptr[j], ll = pp->lvptr[j + 1] - ls;
+        if (ll > 1) {
+            if ((size_t)ll > tmpcap) { tmpcap = (size_t)ll; tmp = (kv_pair *)realloc(tmp, tmpcap * sizeof(kv_pair)); }
+            for (int t = 0; t < ll; t++) { tmp[t].key = lrows[ls + t]; tmp[t].val = pp->lvidx[ls + t]; }
+            qsort(tmp, (size_t)ll, sizeof(kv_pair), kv_cmp);
+            for (int t = 0; t < ll; t++) { lrows[ls + t] = tmp[t].key; pp->lvidx[ls + t] = tmp[t].val; }
+        }
+    }
+    free(tmp);
+    pp->lower_start = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int j = 0; j < n; j++) pp->lower_start[j] = pp->ptr[j + 1] - (pp->lvptr[j + 1] - pp->lvptr[j]);
+    free(lrows); free(cur); free(lcur);
+}
+
+static void pattern_free(perm_pattern *pp)
+{
+    free(pp->ptr); free(pp->rows); free(pp->lower_start); free(pp->lvptr); free(pp->lvidx); free(pp->dvidx);
+}
+
+/* EliminationTree.cs:39-71 */
+static void etree_build(int n, const int *ptr, const int *rows, int *parent)
+{
+    int *anc = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int k = 0; k < n; k++) {
+        parent[k] = EMPTY;
+        anc[k] = EMPTY;
+        for (int p = ptr[k]; p < ptr[k + 1]; p++) {
+            int i = rows[p];
+            if (i >= k) break;
+            while (i != EMPTY && i < k) {
+                int next = anc[i];
+                anc[i] = k;
+                if (next == EMPTY) parent[i] = k;
+                i = next;
+            }
+        }
+    }
+    free(anc);
+}
+
+/* EliminationTree.cs:80-139 */
+static void etree_postorder(int n, const int *parent, int *post)
+{
+    int *head = (int *)xmalloc((size_t)n * sizeof(int));
+    int *next = (int *)xmalloc((size_t)n * sizeof(int));
+    int *stack = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int j = 0; j < n; j++) head[j] = EMPTY;
+    for (int j = n - 1; j >= 0; j--) {
+        if (parent[j] == EMPTY) continue;
+        next[j] = head[parent[j]];
+        head[parent[j]] = j;
+    }
+    int written = 0;
+    for (int j = 0; j < n; j++) {
+        if (parent[j] != EMPTY) continue;
+        int top = 0;
+        stack[0] = j;
+        while (top >= 0) {
+            int node = stack[top], child = head[node];
+            if (child == EMPTY) { top--; post[written++] = node; }
+            else { head[node] = next[child]; stack[++top] = child; }
+        }
+    }
+    free(head); free(next); free(stack);
+}
+
+/* EliminationTree.cs:152-249 (Gilbert-Ng-Peyton) */
+static void etree_colcounts(int n, const int *ptr, const int *rows, const int *parent, const int *post, int *delta)
+{
+    int *anc = (int *)xmalloc((size_t)n * sizeof(int));
+    int *maxfirst = (int *)xmalloc((size_t)n * sizeof(int));
+    int *prevleaf = (int *)xmalloc((size_t)n * sizeof(int));
+    int *first = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; i++) { maxfirst[i] = EMPTY; prevleaf[i] = EMPTY; first[i] = EMPTY; }
+    for (int k = 0; k < n; k++) {
+        int j = post[k];
+        delta[j] = first[j] == EMPTY ? 1 : 0;
+        for (; j != EMPTY && first[j] == EMPTY; j = parent[j]) first[j] = k;
+    }
+    for (int i = 0; i < n; i++) anc[i] = i;
+    for (int k = 0; k < n; k++) {
+        int j = post[k];
+        if (parent[j] != EMPTY) delta[parent[j]]--;
+        for (int p = ptr[j]; p < ptr[j + 1]; p++) {
+            int i = rows[p];
+            if (i <= j || first[j] <= maxfirst[i]) continue;
+            maxfirst[i] = first[j];
+            int jprev = prevleaf[i];
+            prevleaf[i] = j;
+            if (jprev == EMPTY) { delta[j]++; continue; }
+            int root = jprev;
+            while (root != anc[root]) root = anc[root];
+            for (int q = jprev; q != root;) { int nx = anc[q]; anc[q] = root; q = nx; }
+            delta[j]++;
+            delta[root]--;
+        }
+        if (parent[j] != EMPTY) anc[j] = parent[j];
+    }
+    for (int j = 0; j < n; j++)
+        if (parent[j] != EMPTY) delta[parent[j]] += delta[j];
+    free(anc); free(maxfirst); free(prevleaf); free(first);
+}
+
+/* SupernodalStructure.cs:271-286 */
+static int merge_worthwhile(int columns, int64_t added, int64_t stored)
+{
+    if (added <= 0) return 1;
+    double fraction = (double)added / (double)stored;
+    if (columns <= 4) return fraction <= 0.8;
+    if (columns <= 16) return fraction <= 0.1;
+    if (columns <= 48) return fraction <= 0.05;
+    return 0;
+}
+
+void or_options_default(or_options *o)
+{
+    o->ordering = 1;
+    o->amalgamation_limit = 64;
+    o->panel_width = 32;
+    o->allow_regularization = 1;
+    o->pivot_tolerance = 1e-13;
+    o->regularization = 1e-10;
+}
+
+void or_symbolic_free(or_symbolic *s)
+{
+    if (!s) return;
+    free(s->amd); free(s->perm); free(s->iperm); free(s->etree); free(s->colcount);
+    free(s->pptr); free(s->prows); free(s->lower_start); free(s->lvptr); free(s->lvidx); free(s->dvidx);
+    free(s->first); free(s->super_of); free(s->sparent); free(s->child_ptr); free(s->children);
+    free(s->pat_ptr); free(s->pat_rows); free(s->block_off);
+    free(s);
+}
+
+/* UtduSymbolicFactorization.Analyze (UtduSymbolicFactorization.cs:240-295),
+ * SupernodalStructure.Build (:102-166), FindFundamentalSupernodes (:175-193),
+ * Amalgamate (:203-262), BuildPatterns (:296-355), ComputeCosts (:370-418),
+ * FactorStorageLayout.Build size check (FactorStorageLayout.cs:67-76). */
+int or_analyze(const or_csr *a, const or_options *o, or_symbolic **out)
+{
+    if (!a || !o || !out) return OR_E_NULL;
+    int n = a->n;
+    or_symbolic *s = (or_symbolic *)xcalloc(1, sizeof(or_symbolic));
+    s->n = n;
+    s->matrix_nnz = a->nnz;
+    s->amd = (int *)xmalloc((size_t)n * sizeof(int));
+    if (o->ordering == 1) {
+        int rc = or_amd(n, a->rowptr, a->colidx, s->amd);
+        if (rc) { or_symbolic_free(s); return rc; }
+    } else {
+        for (int i = 0; i < n; i++) s->amd[i] = i;
+    }
+    int *inv = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int k = 0; k < n; k++) inv[s->amd[k]] = k;
+    perm_pattern pp;
+    pattern_create(a, inv, &pp);
+    int *par1 = (int *)xmalloc((size_t)n * sizeof(int));
+    int *post = (int *)xmalloc((size_t)n * sizeof(int));
+    etree_build(n, pp.ptr, pp.rows, par1);
+    etree_postorder(n, par1, post);
+    pattern_free(&pp);
+    s->perm = (int *)xmalloc((size_t)n * sizeof(int));
+    s->iperm = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int k = 0; k < n; k++) s->perm[k] = s->amd[post[k]];
+    for (int k = 0; k < n; k++) s->iperm[s->perm[k]] = k;
+    free(inv); free(par1);
+
+    pattern_create(a, s->iperm, &pp);
+    s->pptr = pp.ptr; s->prows = pp.rows; s->lower_start = pp.lower_start;
+    s->lvptr = pp.lvptr; s->lvidx = pp.lvidx; s->dvidx = pp.dvidx;
+    s->etree = (int *)xmalloc((size_t)n * sizeof(int));
+    s->colcount = (int *)xmalloc((size_t)n * sizeof(int));
+    etree_build(n, s->pptr, s->prows, s->etree);
+    etree_postorder(n, s->etree, post);
+    etree_colcounts(n, s->pptr, s->prows, s->etree, post, s->colcount);
+    free(post);
+
+    int amalg = o->amalgamation_limit > 1 ? o->amalgamation_limit : 1;
+    const int *parent = s->etree, *cc = s->colcount;
+    int *childcount = (int *)xcalloc((size_t)n, sizeof(int));
+    for (int j = 0; j < n; j++) if (parent[j] >= 0) childcount[parent[j]]++;
+    int *fstart = (int *)xmalloc(((size_t)n + 1) * sizeof(int));
+    int nf = 0;
+    for (int j = 0; j < n; j++)
+        if (j == 0 || parent[j - 1] != j || childcount[j] > 1 || cc[j - 1] != cc[j] + 1) fstart[nf++] = j;
+    fstart[nf] = n;
+    free(childcount);
+
+    int64_t *ccp = (int64_t *)xcalloc((size_t)n + 1, sizeof(int64_t));
+    for (int j = 0; j < n; j++) ccp[j + 1] = ccp[j] + cc[j];
+    int *bounds = (int *)xmalloc(((size_t)n + 1) * sizeof(int));
+    int *fronts = (int *)xmalloc(((size_t)n + 1) * sizeof(int));
+    int ns = 0;
+    for (int idx = 0; idx < nf;) {
+        int first = fstart[idx], last = fstart[idx + 1] - 1;
+        int columns = last - first + 1;
+        int fsize = cc[first];
+        int64_t nz = ccp[last + 1] - ccp[first];
+        while (idx + 1 < nf) {
+            if (parent[last] != last + 1) break;
+            int nlast = fstart[idx + 2] - 1, nfirst = fstart[idx + 1];
+            int mcols = columns + (nlast - nfirst + 1);
+            if (mcols > amalg) break;
+            int64_t mfront = (int64_t)columns + cc[nfirst];
+            int64_t stored = mfront * mcols - (int64_t)mcols * (mcols - 1) / 2;
+            int64_t mnz = nz + (ccp[nlast + 1] - ccp[nfirst]);
+            if (!merge_worthwhile(mcols, stored - mnz, stored)) break;
+            last = nlast; columns = mcols; fsize = (int)mfront; nz = mnz;
+            idx++;
+        }
+        bounds[ns] = first;
+        fronts[ns] = fsize;
+        ns++;
+        idx++;
+    }
+    bounds[ns] = n;
+    free(fstart); free(ccp);
+
+    s->nsuper = ns;
+    s->first = (int *)xmalloc(((size_t)ns + 1) * sizeof(int));
+    memcpy(s->first, bounds, ((size_t)ns + 1) * sizeof(int));
+    free(bounds);
+    s->super_of = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int q = 0; q < ns; q++)
+        for (int j = s->first[q]; j < s->first[q + 1]; j++) s->super_of[j] = q;
+    s->sparent = (int *)xmalloc((size_t)ns * sizeof(int));
+    int *scc = (int *)xcalloc((size_t)ns, sizeof(int));
+    for (int q = 0; q < ns; q++) {
+        int pc = parent[s->first[q + 1] - 1];
+        s->sparent[q] = pc < 0 ? -1 : s->super_of[pc];
+        if (s->sparent[q] >= 0) scc[s->sparent[q]]++;
+    }
+    s->child_ptr = (int *)xcalloc((size_t)ns + 1, sizeof(int));
+    for (int q = 0; q < ns; q++) s->child_ptr[q + 1] = s->child_ptr[q] + scc[q];
+    s->children = (int *)xmalloc((size_t)s->child_ptr[ns] * sizeof(int));
+    int *ccur = (int *)xmalloc((size_t)ns * sizeof(int));
+    memcpy(ccur, s->child_ptr, (size_t)ns * sizeof(int));
+    for (int q = 0; q < ns; q++)
+        if (s->sparent[q] >= 0) s->children[ccur[s->sparent[q]]++] = q;
+    free(ccur); free(scc);
+    s->pat_ptr = (int *)xcalloc((size_t)ns + 1, sizeof(int));
+    for (int q = 0; q < ns; q++) s->pat_ptr[q + 1] = s->pat_ptr[q] + fronts[q];
+    free(fronts);
+    s->pat_rows = (int *)xmalloc((size_t)s->pat_ptr[ns] * sizeof(int));
+
+    /* BuildPatterns */
+    int rc = OR_OK;
+    int *flag = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int j = 0; j < n; j++) flag[j] = -1;
+    for (int q = 0; q < ns && rc == OR_OK; q++) {
+        int first = s->first[q], last = s->first[q + 1] - 1;
+        int wr = s->pat_ptr[q];
+        for (int j = first; j <= last; j++) { flag[j] = q; s->pat_rows[wr++] = j; }
+        for (int j = first; j <= last; j++)
+            for (int p = s->lower_start[j]; p < s->pptr[j + 1]; p++) {
+                int row = s->prows[p];
+                if (flag[row] == q) continue;
+                flag[row] = q;
+                if (wr >= s->pat_ptr[q + 1]) { rc = OR_E_INTERNAL; break; }
+                s->pat_rows[wr++] = row;
+            }
+        for (int c = s->child_ptr[q]; c < s->child_ptr[q + 1] && rc == OR_OK; c++) {
+            int ch = s->children[c];
+            int cw = s->first[ch + 1] - s->first[ch];
+            for (int t = s->pat_ptr[ch] + cw; t < s->pat_ptr[ch + 1]; t++) {
+                int row = s->pat_rows[t];
+                if (flag[row] == q) continue;
+                flag[row] = q;
+                if (wr >= s->pat_ptr[q + 1]) { rc = OR_E_INTERNAL; break; }
+                s->pat_rows[wr++] = row;
+            }
+        }
+        if (rc == OR_OK && wr != s->pat_ptr[q + 1]) rc = OR_E_INTERNAL;
+        if (rc == OR_OK)
+            qsort(s->pat_rows + s->pat_ptr[q], (size_t)(s->pat_ptr[q + 1] - s->pat_ptr[q]), sizeof(int), int_cmp);
+    }
+    free(flag);
+    if (rc) { or_symbolic_free(s); return rc; }
+
+    /* ComputeCosts */
+    int64_t entries = 0, ops = 0, live = 0, peak = 0;
+    int maxf = 0, maxw = 0;
+    int64_t *contrib = (int64_t *)xcalloc((size_t)ns, sizeof(int64_t));
+    s->block_off = (int64_t *)xcalloc((size_t)ns + 1, sizeof(int64_t));
+    for (int q = 0; q < ns; q++) {
+        int64_t f = s->pat_ptr[q + 1] - s->pat_ptr[q];
+        int64_t w = s->first[q + 1] - s->first[q];
+        int64_t blk = f * w - w * (w - 1) / 2;
+        if (blk > 2147483647LL) rc = OR_E_TOO_DENSE;
+        entries += blk;
+        s->block_off[q + 1] = s->block_off[q] + blk;
+        for (int64_t k = 0; k < w; k++) ops += (f - k) * (f - k);
+        if (f > maxf) maxf = (int)f;
+        if (w > maxw) maxw = (int)w;
+        if (live + f * (f + 1) / 2 > peak) peak = live + f * (f + 1) / 2;
+        for (int c = s->child_ptr[q]; c < s->child_ptr[q + 1]; c++) live -= contrib[s->children[c]];
+        int64_t t = f - w;
+        contrib[q] = t * (t + 1) / 2;
+        live += contrib[q];
+        if (live > peak) peak = live;
+    }
+    free(contrib);
+    s->factor_entries = entries;
+    s->factor_ops = ops;
+    s->max_front = maxf;
+    s->max_width = maxw;
+    s->peak_contribution = peak;
+    int64_t strict = 0;
+    for (int j = 0; j < n; j++) strict += s->colcount[j];
+    s->strict_nonzeros = strict;
+    if (rc) { or_symbolic_free(s); return rc; }
+    *out = s;
+    return OR_OK;
+}
+
+/* ======================================================================
+ * Numeric phase -- UCHOL/UtduNumericFactorization.cs
+ * ====================================================================== */
+
+void or_numeric_free(or_numeric *f)
+{
+    if (!f) return;
+    free(f->factor); free(f->diag); free(f);
+}
+
+/* SupernodalMultifrontalFactorizer.Factorize (:281-403) run serially in
+ * ascending supernode order (the ready-stack order only changes WHEN a front is
+ * processed; each front depends only on its children, so the numbers are the
+ * same), FactorizeSupernode (:420-531), FactorizeFront (:541-601),
+ * ApplyPanelUpdate / UpdateTrailingColumns / UpdateColumnTile / PanelDot
+ * (:624-852): every trailing element gets F[i,j] -= sum_k (L[i,k]*d_k)*L[j,k],
+ * the sum taken over the panel in ascending k from 0.0 and subtracted once. */
+int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o, or_numeric **out)
+{
+    if (!s || !a || !o || !out) return OR_E_NULL;
+    if (a->n != s->n || a->nnz != s->matrix_nnz) return OR_E_ARG; /* EnsureCompatible :306-319 */
+    int n = s->n, ns = s->nsuper;
+    or_numeric *F = (or_numeric *)xcalloc(1, sizeof(or_numeric));
+    F->sym = s;
+    F->diag = (double *)xcalloc((size_t)n, sizeof(double));
+    F->factor = (double *)xmalloc((size_t)s->factor_entries * sizeof(double));
+    if (ns == 0) { *out = F; return OR_OK; }
+    int64_t maxlen = (int64_t)s->max_front * (s->max_front + 1) / 2;
+    if (maxlen > 2147483647LL) { or_numeric_free(F); return OR_E_TOO_DENSE; }
+
+    /* ComputeDiagonalScale :411-418 (Math.Max propagates NaN) */
+    double scale = 0.0;
+    for (int i = 0; i < n; i++) {
+        double d = a->diag[i] >= 0 ? a->values[a->diag[i]] : 0.0;
+        double m = fabs(d);
+        if (isnan(m) || isnan(scale)) scale = NAN;
+        else if (m > scale) scale = m;
+    }
+    if (!(scale > 0.0)) scale = 1.0;
+    int pw = o->panel_width < 4 ? 4 : (o->panel_width > 128 ? 128 : o->panel_width);
+    double thr = o->pivot_tolerance * scale;
+    double regv = o->regularization * scale;
+
+    double **updates = (double **)xcalloc((size_t)ns, sizeof(double *));
+    int *offs = (int *)xmalloc(((size_t)s->max_front + 1) * sizeof(int));
+    int *rowmap = (int *)xmalloc((size_t)n * sizeof(int));
+    int *rel = (int *)xmalloc((size_t)s->max_front * sizeof(int) + sizeof(int));
+    double *front = (double *)xmalloc((size_t)maxlen * sizeof(double));
+    int regularized = 0, negative = 0;
+    double smallest = DBL_MAX, largest = 0.0;
+    int rc = OR_OK;
+
+    for (int q = 0; q < ns && rc == OR_OK; q++) {
+        int first = s->first[q];
+        int w = s->first[q + 1] - first;
+        int f = s->pat_ptr[q + 1] - s->pat_ptr[q];
+        const int *prow = s->pat_rows + s->pat_ptr[q];
+        int flen = (int)((int64_t)f * (f + 1) / 2);
+        offs[0] = 0;
+        for (int j = 0; j < f; j++) offs[j + 1] = offs[j] + (f - j);
+        memset(front, 0, (size_t)flen * sizeof(double));
+        for (int t = 0; t < f; t++) rowmap[prow[t]] = t;
+        /* assembly of original entries :450-468 */
+        for (int loc = 0; loc < w; loc++) {
+            int col = first + loc;
+            int di = s->dvidx[col];
+            if (di >= 0) front[offs[loc]] += a->values[di];
+            int lb = s->lvptr[col], st = s->lower_start[col], en = s->pptr[col + 1];
+            int cb = offs[loc] - loc;
+            for (int p = st; p < en; p++) {
+                int row = rowmap[s->prows[p]];
+                front[cb + row] += a->values[s->lvidx[lb + (p - st)]];
+            }
+        }
+        /* extend-add :471-509 */
+        for (int c = s->child_ptr[q]; c < s->child_ptr[q + 1]; c++) {
+            int ch = s->children[c];
+            double *upd = updates[ch];
+            if (!upd) continue;
+            int cw = s->first[ch + 1] - s->first[ch];
+            int ct = (s->pat_ptr[ch + 1] - s->pat_ptr[ch]) - cw;
+            const int *crow = s->pat_rows + s->pat_ptr[ch] + cw;
+            for (int x = 0; x < ct; x++) rel[x] = rowmap[crow[x]];
+            int src = 0;
+            for (int x = 0; x < ct; x++) {
+                int tc = rel[x];
+                int tb = offs[tc] - tc;
+                for (int y = x; y < ct; y++) front[tb + rel[y]] += upd[src + (y - x)];
+                src += ct - x;
+            }
+            free(upd);
+            updates[ch] = NULL;
+        }
+        /* FactorizeFront :541-601 */
+        for (int ps = 0; ps < w && rc == OR_OK; ps += pw) {
+            int pe = ps + pw < w ? ps + pw : w;
+            for (int k = ps; k < pe; k++) {
+                int co = offs[k], cl = f - k;
+                double pivot = front[co];
+                if (!(pivot > thr)) {
+                    if (!o->allow_regularization) { rc = OR_E_NOT_PD; break; }
+                    if (pivot < 0.0) negative++;
+                    regularized++;
+                    pivot = regv;
+                }
+                F->diag[first + k] = pivot;
+                double mag = fabs(pivot);
+                if (mag < smallest) smallest = mag;
+                if (mag > largest) largest = mag;
+                double inv = 1.0 / pivot;
+                for (int t = 1; t < cl; t++) front[co + t] *= inv;
+                for (int j = k + 1; j < pe; j++) {
+                    double factor = pivot * front[co + (j - k)];
+                    if (factor == 0.0) continue;
+                    double *tgt = front + offs[j];
+                    const double *src = front + co + (j - k);
+                    for (int t = 0; t < f - j; t++) tgt[t] -= src[t] * factor;
+                }
+                front[co] = 1.0;
+            }
+            if (rc != OR_OK || pe >= f) continue;
+            int np = pe - ps;
+            /* panel x D, row-major per trailing row: buf[(i-pe)*np + kk] */
+            int trail = f - pe;
+            double *buf = (double *)xmalloc((size_t)trail * np * sizeof(double));
+            for (int kk = 0; kk < np; kk++) {
+                int col = ps + kk;
+                double d = F->diag[first + col];
+                const double *L = front + offs[col] - col;
+                for (int i = pe; i < f; i++) buf[(size_t)(i - pe) * np + kk] = L[i] * d;
+            }
+            for (int j = pe; j < f; j++) {
+                double *Fj = front + offs[j] - j;
+                for (int i = j; i < f; i++) {
+                    const double *bi = buf + (size_t)(i - pe) * np;
+                    double sum = 0.0;
+                    for (int kk = 0; kk < np; kk++) {
+                        int col = ps + kk;
+                        sum += bi[kk] * front[offs[col] - col + j];
+                    }
+                    Fj[i] -= sum;
+                }
+            }
+            free(buf);
+        }
+        if (rc != OR_OK) break;
+        int blen = offs[w];
+        memcpy(F->factor + s->block_off[q], front, (size_t)blen * sizeof(double));
+        if (f - w > 0) {
+            int ulen = flen - blen;
+            updates[q] = (double *)xmalloc((size_t)ulen * sizeof(double));
+            memcpy(updates[q], front + blen, (size_t)ulen * sizeof(double));
+        }
+    }
+    for (int q = 0; q < ns; q++) free(updates[q]);
+    free(updates); free(offs); free(rowmap); free(rel); free(front);
+    if (rc) { or_numeric_free(F); return rc; }
+    if (smallest == DBL_MAX) smallest = 0.0;
+    F->regularized = regularized;
+    F->negative = negative;
+    F->smallest = smallest;
+    F->largest = largest;
+    *out = F;
+    return OR_OK;
+}
+
+/* DenseKernels.Dot (DenseKernels.cs:91-113) for Vector<double>.Count == 4; the
+ * horizontal sum of the accumulator is taken as (a0+a1)+(a2+a3). */
+static double or_dot4(const double *l, const double *r, int len)
+{
+    int t = 0;
+    double sum = 0.0;
+    if (len >= 4) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (; t <= len - 4; t += 4) {
+            a0 += l[t] * r[t];
+            a1 += l[t + 1] * r[t + 1];
+            a2 += l[t + 2] * r[t + 2];
+            a3 += l[t + 3] * r[t + 3];
+        }
+        sum = (a0 + a1) + (a2 + a3);
+    }
+    for (; t < len; t++) sum += l[t] * r[t];
+    return sum;
+}
+
+/* UtduNumericFactorization.Solve (:90-112), SolvePermuted (:118-129),
+ * ForwardSubstitution (:139-173), BackwardSubstitution (:179-207). */
+int or_solve(const or_numeric *F, const double *b, double *x)
+{
+    if (!F || !b || !x) return OR_E_NULL;
+    const or_symbolic *s = F->sym;
+    int n = s->n;
+    double *wk = (double *)xmalloc((size_t)n * sizeof(double));
+    double *sc = (double *)xmalloc(((size_t)s->max_front + 1) * sizeof(double));
+    for (int i = 0; i < n; i++) wk[i] = b[s->perm[i]];
+    for (int q = 0; q < s->nsuper; q++) {
+        int f = s->pat_ptr[q + 1] - s->pat_ptr[q], w = s->first[q + 1] - s->first[q];
+        const int *pr = s->pat_rows + s->pat_ptr[q];
+        const double *blk = F->factor + s->block_off[q];
+        for (int t = 0; t < f; t++) sc[t] = wk[pr[t]];
+        int off = 0;
+        for (int k = 0; k < w; k++) {
+            int cl = f - k;
+            double v = sc[k];
+            if (v != 0.0)
+                for (int t = 0; t < cl - 1; t++) sc[k + 1 + t] -= blk[off + 1 + t] * v;
+            off += cl;
+        }
+        for (int t = 0; t < f; t++) wk[pr[t]] = sc[t];
+    }
+    for (int i = 0; i < n; i++) wk[i] /= F->diag[i];
+    for (int q = s->nsuper - 1; q >= 0; q--) {
+        int f = s->pat_ptr[q + 1] - s->pat_ptr[q], w = s->first[q + 1] - s->first[q];
+        const int *pr = s->pat_rows + s->pat_ptr[q];
+        const double *blk = F->factor + s->block_off[q];
+        for (int t = 0; t < f; t++) sc[t] = wk[pr[t]];
+        for (int k = w - 1; k >= 0; k--) {
+            int off = k * f - k * (k - 1) / 2;
+            int cl = f - k;
+            sc[k] -= or_dot4(blk + off + 1, sc + k + 1, cl - 1);
+        }
+        for (int t = 0; t < w; t++) wk[pr[t]] = sc[t];
+    }
+    for (int i = 0; i < n; i++) x[s->perm[i]] = wk[i];
+    free(wk); free(sc);
+    return OR_OK;
+}
+
+/* ======================================================================
+ * CGGP -- CGGP/ConjugateGradientGaussPreSolver.cs:78-175,
+ * CGGP/JacobiPreconditioner.cs:209-253, IterativeLinearSolver.cs:68-75
+ * ====================================================================== */
+int or_pcg(const or_csr *a, const double *b, const double *x0, double tol,
+           int max_iterations, int use_jacobi, double *x, or_pcg_result *res)
+{
+    if (!a || !b || !x || !res) return OR_E_NULL;
+    if (!isfinite(tol) || tol < 0.0) return OR_E_RANGE;
+    if (max_iterations < 0) return OR_E_RANGE;
+    int n = a->n;
+    for (int i = 0; i < n; i++) x[i] = x0 ? x0[i] : 0.0;
+    double bn = or_norm2(n, b);
+    double scale = bn > 0.0 ? bn : 1.0;
+    res->iterations = 0;
+    res->converged = 1;
+    res->residual_norm = 0.0;
+    res->relative_residual = 0.0;
+    if (n == 0) { res->converged = 0.0 / scale <= tol; return OR_OK; }
+    double thr = tol * scale;
+    double *r = (double *)xmalloc((size_t)n * sizeof(double));
+    double *z = (double *)xmalloc((size_t)n * sizeof(double));
+    double *p = (double *)xmalloc((size_t)n * sizeof(double));
+    double *ap = (double *)xmalloc((size_t)n * sizeof(double));
+    double *invd = NULL;
+    int rc = OR_OK, iters = 0;
+    or_spmv(a, x, ap);
+    for (int i = 0; i < n; i++) r[i] = b[i] - ap[i];
+    double rn = or_norm2(n, r);
+    if (rn <= thr) goto done;
+    if (use_jacobi) {
+        invd = (double *)xmalloc((size_t)n * sizeof(double));
+        for (int i = 0; i < n; i++) {
+            double d = a->diag[i] >= 0 ? a->values[a->diag[i]] : 0.0;
+            if (!isfinite(d) || fabs(d) <= 1e-300) { rc = OR_E_NOT_PD; goto done; }
+            invd[i] = 1.0 / d;
+        }
+    }
+    for (int i = 0; i < n; i++) z[i] = invd ? invd[i] * r[i] : r[i];
+    memcpy(p, z, (size_t)n * sizeof(double));
+    double rz = 0.0;
+    for (int i = 0; i < n; i++) rz += r[i] * z[i];
+    int maxit = max_iterations > 0 ? max_iterations : n;
+    iters = maxit;
+    for (int it = 1; it <= maxit; it++) {
+        or_spmv(a, p, ap);
+        double pap = 0.0;
+        for (int i = 0; i < n; i++) pap += p[i] * ap[i];
+        if (pap <= 0.0) { rc = OR_E_NOT_PD; goto done; }
+        double alpha = rz / pap;
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * ap[i]; }
+        rn = or_norm2(n, r);
+        if (rn <= thr) { iters = it; goto done; }
+        for (int i = 0; i < n; i++) z[i] = invd ? invd[i] * r[i] : r[i];
+        double rz2 = 0.0;
+        for (int i = 0; i < n; i++) rz2 += r[i] * z[i];
+        double beta = rz2 / rz;
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+        rz = rz2;
+    }
+done:
+    if (rc == OR_OK) {
+        res->iterations = iters;
+        res->residual_norm = rn;
+        res->relative_residual = rn / scale;
+        res->converged = res->relative_residual <= tol;
+    }
+    free(r); free(z); free(p); free(ap); free(invd);
+    return rc;
+}

@@ -1,0 +1,136 @@
+/*
+ * oracle.h -- CPU restatement of MAGLA's UCHOL / CGGP path, used ONLY as a
+ * test checker and as the CPU timing baseline.  Nothing in the product
+ * library (magla_amd/) links, loads or calls this code.
+ *
+ * Every function follows a reference file (paths relative to the MAGLA
+ * repository root, CAESolvers/...); the exact lines are cited at each
+ * definition in oracle.c.  All arithmetic is IEEE binary64 in the
+ * reference's operation order; build with -ffp-contract=off (RyuJIT never
+ * fuses a multiply into an add).
+ */
+#ifndef MAGLA_ORACLE_H
+#define MAGLA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: the same values as the product ABI (include/magla.h) */
+#define OR_OK 0
+#define OR_E_NULL 1
+#define OR_E_ARG 2
+#define OR_E_RANGE 3
+#define OR_E_NOT_PD 4
+#define OR_E_TOO_DENSE 5
+#define OR_E_INTERNAL 7
+#define OR_E_NOMEM 8
+
+/* ---- symmetric CSR (upper triangle incl. diagonal) ---- */
+typedef struct {
+    int n;
+    int nnz;
+    int *rowptr;   /* n+1 */
+    int *colidx;   /* nnz, ascending inside a row */
+    double *values;/* nnz */
+    int *diag;     /* n, position of A[i,i] or -1 */
+} or_csr;
+
+/* COO -> CSR: SymmetricCSRMatrix(int, IEnumerable<MatrixEntry>) */
+int or_csr_from_coo(int n, int64_t count, const int *rows, const int *cols,
+                    const double *vals, or_csr **out);
+/* wrap existing arrays (copied) */
+int or_csr_create(int n, const int *rowptr, const int *colidx, const double *values,
+                  or_csr **out);
+void or_csr_free(or_csr *a);
+/* y = A x, SymmetricCSRMatrix.Multiply(double[], double[]) */
+void or_spmv(const or_csr *a, const double *x, double *y);
+/* SymmetricCSRMatrix.LineCross */
+int or_linecross(or_csr *a, double *rhs, double prescribed, int index);
+
+/* ---- symbolic phase ---- */
+typedef struct {
+    int ordering;              /* 0 natural, 1 AMD */
+    int amalgamation_limit;    /* 64 */
+    int panel_width;           /* 32 (clamped 4..128) */
+    int allow_regularization;  /* 1 */
+    double pivot_tolerance;    /* 1e-13 */
+    double regularization;     /* 1e-10 */
+} or_options;
+
+void or_options_default(or_options *o);
+
+typedef struct {
+    int n;
+    int matrix_nnz;
+    int *amd;          /* fill-reducing order before postorder (n) */
+    int *perm;         /* perm[k] = original index eliminated k-th */
+    int *iperm;
+    int *etree;        /* final elimination tree (permuted numbering) */
+    int *colcount;     /* final column counts */
+    /* permuted pattern (PermutedSymmetricPattern) */
+    int *pptr;         /* n+1 */
+    int *prows;        /* full structure, no diagonal, ascending */
+    int *lower_start;  /* n */
+    int *lvptr;        /* n+1 */
+    int *lvidx;        /* lower value indices */
+    int *dvidx;        /* diagonal value indices (-1 if absent) */
+    /* supernodes */
+    int nsuper;
+    int *first;        /* nsuper+1 */
+    int *super_of;     /* n */
+    int *sparent;      /* nsuper */
+    int *child_ptr;    /* nsuper+1 */
+    int *children;
+    int *pat_ptr;      /* nsuper+1 */
+    int *pat_rows;
+    int64_t *block_off;/* nsuper+1 : packed factor block offsets */
+    int64_t factor_entries;
+    int64_t strict_nonzeros;
+    int64_t factor_ops;
+    int max_front;
+    int max_width;
+    int64_t peak_contribution;
+} or_symbolic;
+
+int or_amd(int n, const int *rowptr, const int *colidx, int *perm_out);
+int or_analyze(const or_csr *a, const or_options *o, or_symbolic **out);
+void or_symbolic_free(or_symbolic *s);
+
+/* ---- numeric phase ---- */
+typedef struct {
+    const or_symbolic *sym;
+    double *factor;    /* packed blocks, block_off[s] .. */
+    double *diag;      /* n, elimination order */
+    int regularized;
+    int negative;
+    double smallest;
+    double largest;
+} or_numeric;
+
+int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o,
+                 or_numeric **out);
+void or_numeric_free(or_numeric *f);
+/* UtduNumericFactorization.Solve: x = A^-1 b (b unchanged) */
+int or_solve(const or_numeric *f, const double *b, double *x);
+
+/* ---- CGGP ---- */
+typedef struct {
+    int iterations;
+    int converged;
+    double residual_norm;
+    double relative_residual;
+} or_pcg_result;
+
+/* use_jacobi: 1 = JacobiPreconditioner, 0 = identity.  x0 may be NULL. */
+int or_pcg(const or_csr *a, const double *b, const double *x0, double tol,
+           int max_iterations, int use_jacobi, double *x, or_pcg_result *res);
+
+double or_norm2(int n, const double *v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
