@@ -11,9 +11,9 @@
 extern "C" {
 #endif
 
-/* Front of one supernode after factorization: f x f column-major (lower part
- * valid), converted to float64.  Columns [0, w) are the factor block (unit
- * diagonal, L below), the trailing (f-w)^2 block is the contribution. */
+/* Factor block of one supernode after factorization, as MAGLA stores it:
+ * f x w column-major over the supernode's pattern rows (unit diagonal, L
+ * below), converted to float64 (f, w = front size and width of the supernode). */
 int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out);
 
 /* Runs one 16x16x4 MFMA (MG_FP64 or MG_FP32) whose product is D[i][j] = i + 16 j
@@ -21,6 +21,20 @@ int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out)
  * accumulator register where the kernel's layout rule places it: out[i*16 + j]
  * must equal i + 16 j (256 values). */
 int mg_test_mfma_layout(int32_t precision, double *out);
+
+/* Device schedule summary computed on the host (no GPU needed): out[0..5] =
+ * fronts (fused supernode chains), levels, arena elements (sum f^2), largest
+ * front, widest front, kernel launches per factorization. */
+int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out);
+
+/* Launch timing.  mg_profile_enable(1) clears the tallies and brackets every
+ * factorization launch with HIP events on its stream; mg_profile_read(kind, ...)
+ * returns 1 and the kernel's name, launch count, summed event time (ms) and
+ * summed algorithmic flops / HBM bytes (kinds 0..6: assemble_pull,
+ * assemble_values, panel_diag, panel_rows, schur, forward, backward), 0 past
+ * the last kind, or a negative status. */
+void mg_profile_enable(int32_t on);
+int mg_profile_read(int32_t kind, const char **name, int64_t *launches, double *ms, double *flops, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,11 @@ def lib():
     L.mg_pcg_solve_dev.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, C.POINTER(_PcgResult)]
     L.mg_test_numeric_front.argtypes = [vp, C.c_int32, vp]
     L.mg_test_mfma_layout.argtypes = [C.c_int32, vp]
+    L.mg_test_plan_stats.argtypes = [vp, vp]
+    L.mg_profile_enable.argtypes = [C.c_int32]
+    L.mg_profile_enable.restype = None
+    L.mg_profile_read.argtypes = [C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -145,11 +150,39 @@ def test_mfma_layout(precision):
     return out.reshape(16, 16)
 
 
-def test_numeric_front(numeric, supernode, front_size):
-    """Front of one supernode after factorization as an (f, f) array (lower part valid)."""
-    out = np.zeros(front_size * front_size)
+def plan_stats(symbolic):
+    """Device schedule summary (host only): fronts, levels, arena elements, max front, max width,
+    launches per factorization."""
+    out = np.zeros(6, dtype=np.int64)
+    _check(lib().mg_test_plan_stats(symbolic._h, _ptr(out)))
+    return dict(zip(["fronts", "levels", "arena_elems", "max_front", "max_width", "launches"], out.tolist()))
+
+
+def test_factor_block(numeric, supernode, front_size, width):
+    """Factor block of one supernode, (f, w): unit diagonal, L below (include/magla_testing.h)."""
+    out = np.zeros(front_size * width)
     _check(lib().mg_test_numeric_front(numeric._h, supernode, _ptr(out)))
-    return out.reshape(front_size, front_size, order="F")
+    return out.reshape(front_size, width, order="F")
+
+
+def profile_enable(on=True):
+    """Bracket every factorization launch with HIP events (see include/magla_testing.h)."""
+    lib().mg_profile_enable(1 if on else 0)
+
+
+def profile_read():
+    """{kernel: dict(launches, ms, flops, bytes)} accumulated since profile_enable()."""
+    out = {}
+    k = 0
+    while True:
+        name, n, ms, fl, by = C.c_char_p(), C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        rc = lib().mg_profile_read(k, C.byref(name), C.byref(n), C.byref(ms), C.byref(fl), C.byref(by))
+        if rc < 0:
+            _check(-rc)
+        if rc == 0:
+            return out
+        out[name.value.decode()] = dict(launches=n.value, ms=ms.value, flops=fl.value, bytes=by.value)
+        k += 1
 
 
 def _check(rc):
@@ -425,7 +458,7 @@ class SymmetricUtduSolver:
 
     def factorize_dev(self, symbolic, values_ptr, nnz, numeric=None, stream=None):
         """Values already in HBM; refactorizes in place into `numeric` when given."""
-        h = C.c_void_p(numeric._h if numeric is not None else None)
+        h = C.c_void_p(numeric._h.value) if numeric is not None else C.c_void_p()
         _check(lib().mg_factorize_dev(symbolic._h, nnz, values_ptr, C.byref(self.options._c()), stream, C.byref(h)))
         return numeric if numeric is not None else UtduNumericFactorization(h, symbolic)
 

@@ -11,6 +11,9 @@
 namespace mg {
 void numeric_front(const Numeric *N, int s, double *out);
 void selftest_mfma(int precision, double *host_out);
+void profile_enable(int on);
+void plan_stats(const Symbolic &S, int64_t *out);
+int profile_read(int kind, const char **name, int64_t *launches, double *ms, double *flops, double *bytes);
 void csr_from_coo(int n, int64_t count, const int *rows, const int *cols, const double *vals, int *nnz_out,
                   int *rowptr, int *colidx, double *values);
 void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values, double *rhs, int count,
@@ -334,6 +337,24 @@ int mg_test_mfma_layout(int32_t precision, double *out) {
         require(out != nullptr, MG_E_NULL, "out is null.");
         mg::selftest_mfma(precision, out);
     });
+}
+
+int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out) {
+    return guarded([&] {
+        require(sym != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
+        mg::plan_stats(sym->s, out);
+    });
+}
+
+void mg_profile_enable(int32_t on) { mg::profile_enable(on); }
+
+int mg_profile_read(int32_t kind, const char **name, int64_t *launches, double *ms, double *flops, double *bytes) {
+    int found = 0;
+    const int rc = guarded([&] {
+        require(name && launches && ms && flops && bytes, MG_E_NULL, "argument is null.");
+        found = mg::profile_read(kind, name, launches, ms, flops, bytes);
+    });
+    return rc != MG_OK ? -rc : found;
 }
 
 }  // extern "C"

@@ -1,7 +1,8 @@
 // mfma.h -- 64x64 Schur-update tile on CDNA4 matrix cores.
 //
 // C[i0+r, j0+c] -= sum_k As[k][r] * Bs[k][c] for the lower triangle
-// (i0+r >= j0+c) of a column-major front with leading dimension f.
+// (i0+r >= j0+c) of a column-major front with leading dimension f; K is
+// accumulated in registers over LDS chunks (schur_acc_tile), then stored once.
 //
 // Four waves; wave v owns the 32x32 quadrant (rows 32*(v&1), cols 32*(v>>1))
 // as 2x2 blocks of 16x16 MFMA tiles, k advancing by 4.  The product is formed
@@ -40,16 +41,19 @@ struct MfmaAcc<float> {
 };
 
 template <typename T>
-__device__ inline void schur_tile_update(const T (*As)[65], const T (*Bs)[65], int nk, T *__restrict__ M, int f,
-                                         int i0, int j0) {
-    typedef typename MfmaAcc<T>::type acc_t;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
-    const int l16 = lane & 15, lk = lane >> 4;
-    acc_t acc[2][2];
+__device__ inline void schur_acc_zero(typename MfmaAcc<T>::type (&acc)[2][2]) {
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
+}
+
+// acc += As^T-panel x Bs-panel over nk (multiple of 4) rows of the LDS chunks.
+template <typename T>
+__device__ inline void schur_acc_tile(const T (*As)[65], const T (*Bs)[65], int nk,
+                                      typename MfmaAcc<T>::type (&acc)[2][2]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
+    const int l16 = lane & 15, lk = lane >> 4;
     for (int k0 = 0; k0 < nk; k0 += 4) {
         const int k = k0 + lk;
         T colv[2], rowv[2];
@@ -58,14 +62,23 @@ __device__ inline void schur_tile_update(const T (*As)[65], const T (*Bs)[65], i
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::step(colv[b], rowv[a], acc[a][b]);
     }
-    // acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)]
+}
+
+// M[j*f + i] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
+// acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].
+template <typename T>
+__device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2][2], T *__restrict__ M, int f,
+                                        int i0, int j0, int jend) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
+    const int l16 = lane & 15, lk = lane >> 4;
     for (int a = 0; a < 2; ++a) {
         const int i = i0 + rq + 16 * a + l16;
         if (i >= f) continue;
         for (int b = 0; b < 2; ++b)
             for (int q = 0; q < 4; ++q) {
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (j <= i) M[(int64_t)j * f + i] -= acc[a][b][q];
+                if (j <= i && j < jend) M[(int64_t)j * f + i] -= acc[a][b][q];
             }
     }
 }

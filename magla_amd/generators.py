@@ -90,3 +90,83 @@ def config_c2(m=48):
     """3D 7-point variable-coefficient diffusion, 48^3 (n = 110,592), 4 RHS."""
     n, r, c, v = diffusion_3d_lognormal(m, SEED_BASE + 2)
     return n, r, c, v, uniform_rhs(n, 4, SEED_BASE + 2 + 1000)
+
+
+def hex8_stiffness(h, nu):
+    """24x24 stiffness of a cube trilinear hexahedron of edge h for E = 1 (2x2x2 Gauss).
+    Local node a = (ax, ay, az) in {0,1}^3 ordered a = ax + 2 ay + 4 az; dof 3a + d."""
+    lam = nu / ((1 + nu) * (1 - 2 * nu))
+    mu = 1.0 / (2 * (1 + nu))
+    Cm = np.zeros((6, 6))
+    Cm[:3, :3] = lam
+    Cm[np.arange(3), np.arange(3)] += 2 * mu
+    Cm[np.arange(3, 6), np.arange(3, 6)] = mu
+    corners = np.array([(a & 1, (a >> 1) & 1, (a >> 2) & 1) for a in range(8)], dtype=float) * 2 - 1
+    g = 1.0 / np.sqrt(3.0)
+    K = np.zeros((24, 24))
+    for gx in (-g, g):
+        for gy in (-g, g):
+            for gz in (-g, g):
+                # dN/dxi, scaled by 2/h (reference cube [-1,1]^3 -> [0,h]^3)
+                dN = np.zeros((8, 3))
+                for a in range(8):
+                    sx, sy, sz = corners[a]
+                    dN[a, 0] = sx * (1 + sy * gy) * (1 + sz * gz) / 8
+                    dN[a, 1] = sy * (1 + sx * gx) * (1 + sz * gz) / 8
+                    dN[a, 2] = sz * (1 + sx * gx) * (1 + sy * gy) / 8
+                dN *= 2.0 / h
+                B = np.zeros((6, 24))
+                for a in range(8):
+                    B[0, 3 * a] = dN[a, 0]
+                    B[1, 3 * a + 1] = dN[a, 1]
+                    B[2, 3 * a + 2] = dN[a, 2]
+                    B[3, 3 * a] = dN[a, 1]
+                    B[3, 3 * a + 1] = dN[a, 0]
+                    B[4, 3 * a + 1] = dN[a, 2]
+                    B[4, 3 * a + 2] = dN[a, 1]
+                    B[5, 3 * a] = dN[a, 2]
+                    B[5, 3 * a + 2] = dN[a, 0]
+                K += B.T @ Cm @ B * (h / 2) ** 3
+    return 0.5 * (K + K.T)
+
+
+def elasticity_3d(m, seed, nu=0.3, sigma=0.5):
+    """Config C3 matrix: m^3 trilinear hex elements on the unit cube, (m+1)^3 nodes x 3 dof
+    (node (i,j,k) -> (k(m+1)+j)(m+1)+i, dof 3*node+d), Young's modulus lognormal(0, sigma) per
+    element, Poisson ratio nu.  Returns COO of the upper triangle (each element contributes its
+    upper entries once; duplicates are summed by the CSR builder) and the element moduli."""
+    g = rng(seed)
+    E = g.lognormal(0.0, sigma, m ** 3)
+    K0 = hex8_stiffness(1.0 / m, nu)
+    p = m + 1
+    ei, ej, ek = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    ei, ej, ek = ek.ravel(), ej.ravel(), ei.ravel()  # element e = (k m + j) m + i
+    base = (ek * p + ej) * p + ei
+    local = np.array([(a & 1) + ((a >> 1) & 1) * p + ((a >> 2) & 1) * p * p for a in range(8)])
+    nodes = base[:, None] + local[None, :]  # (ne, 8)
+    dofs = (3 * nodes[:, :, None] + np.arange(3)[None, None, :]).reshape(-1, 24)  # (ne, 24)
+    iu, ju = np.triu_indices(24)
+    r = dofs[:, iu]
+    c = dofs[:, ju]
+    v = E[:, None] * K0[iu, ju][None, :]
+    n = 3 * p ** 3
+    return n, r.ravel().astype(np.int32), c.ravel().astype(np.int32), v.ravel()
+
+
+def clamped_dofs_x0(m):
+    """DOF of the nodes on the face x = 0 (i = 0)."""
+    p = m + 1
+    kk, jj = np.meshgrid(np.arange(p), np.arange(p), indexing="ij")
+    nodes = ((kk * p + jj) * p).ravel()
+    return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel().astype(np.int32)
+
+
+def config_c3(m=40, k=8):
+    """3D elasticity, m^3 hex elements, x=0 face clamped through LineCross, k nodal-load RHS N(0,1)
+    on the free DOF.  Returns (SymmetricCSRMatrix-ready arrays via the caller's builder)."""
+    n, r, c, v = elasticity_3d(m, SEED_BASE + 3)
+    clamped = clamped_dofs_x0(m)
+    g = rng(SEED_BASE + 3 + 1000)
+    b = g.standard_normal((k, n))
+    b[:, clamped] = 0.0
+    return n, r, c, v, clamped, np.asfortranarray(b.T)

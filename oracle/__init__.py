@@ -29,7 +29,7 @@ class _Csr(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("ordering", C.c_int), ("amalgamation_limit", C.c_int), ("panel_width", C.c_int),
                 ("allow_regularization", C.c_int), ("pivot_tolerance", C.c_double),
-                ("regularization", C.c_double)]
+                ("regularization", C.c_double), ("time_budget_s", C.c_double)]
 
 
 class _Sym(C.Structure):
@@ -51,7 +51,7 @@ class _Sym(C.Structure):
 class _Num(C.Structure):
     _fields_ = [("sym", C.c_void_p), ("factor", C.POINTER(C.c_double)), ("diag", C.POINTER(C.c_double)),
                 ("regularized", C.c_int), ("negative", C.c_int), ("smallest", C.c_double),
-                ("largest", C.c_double)]
+                ("largest", C.c_double), ("supernodes_done", C.c_int), ("ops_done", C.c_int64)]
 
 
 class _PcgRes(C.Structure):
@@ -174,7 +174,7 @@ class Csr:
 
 
 def options(ordering=1, amalgamation_limit=64, panel_width=32, allow_regularization=True,
-            pivot_tolerance=1e-13, regularization=1e-10):
+            pivot_tolerance=1e-13, regularization=1e-10, time_budget_s=0.0):
     o = _Opts()
     lib().or_options_default(C.byref(o))
     o.ordering = ordering
@@ -183,6 +183,7 @@ def options(ordering=1, amalgamation_limit=64, panel_width=32, allow_regularizat
     o.allow_regularization = 1 if allow_regularization else 0
     o.pivot_tolerance = pivot_tolerance
     o.regularization = regularization
+    o.time_budget_s = time_budget_s
     return o
 
 
@@ -240,6 +241,8 @@ class Numeric:
         self.negative = f.negative
         self.smallest = f.smallest
         self.largest = f.largest
+        self.supernodes_done = f.supernodes_done
+        self.ops_done = f.ops_done
 
     @property
     def is_positive_definite(self):

@@ -23,6 +23,7 @@
 #include "oracle.h"
 
 #include <float.h>
+#include <time.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -834,6 +835,14 @@ void or_options_default(or_options *o)
     o->allow_regularization = 1;
     o->pivot_tolerance = 1e-13;
     o->regularization = 1e-10;
+    o->time_budget_s = 0.0;
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
 void or_symbolic_free(or_symbolic *s)
@@ -1078,8 +1087,12 @@ int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o, or_
     int regularized = 0, negative = 0;
     double smallest = DBL_MAX, largest = 0.0;
     int rc = OR_OK;
+    const double t0 = now_s();
+    int done = 0;
+    int64_t ops_done = 0;
 
     for (int q = 0; q < ns && rc == OR_OK; q++) {
+        if (o->time_budget_s > 0.0 && now_s() - t0 >= o->time_budget_s) break;
         int first = s->first[q];
         int w = s->first[q + 1] - first;
         int f = s->pat_ptr[q + 1] - s->pat_ptr[q];
@@ -1180,6 +1193,8 @@ int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o, or_
             updates[q] = (double *)xmalloc((size_t)ulen * sizeof(double));
             memcpy(updates[q], front + blen, (size_t)ulen * sizeof(double));
         }
+        for (int64_t k = 0; k < w; k++) ops_done += ((int64_t)f - k) * ((int64_t)f - k);
+        done++;
     }
     for (int q = 0; q < ns; q++) free(updates[q]);
     free(updates); free(offs); free(rowmap); free(rel); free(front);
@@ -1189,6 +1204,8 @@ int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o, or_
     F->negative = negative;
     F->smallest = smallest;
     F->largest = largest;
+    F->supernodes_done = done;
+    F->ops_done = ops_done;
     *out = F;
     return OR_OK;
 }

@@ -58,6 +58,8 @@ typedef struct {
     int allow_regularization;  /* 1 */
     double pivot_tolerance;    /* 1e-13 */
     double regularization;     /* 1e-10 */
+    double time_budget_s;      /* 0 = factor everything; else stop after the supernode that
+                                  crosses this wall time (bounded CPU-baseline sample) */
 } or_options;
 
 void or_options_default(or_options *o);
@@ -108,6 +110,8 @@ typedef struct {
     int negative;
     double smallest;
     double largest;
+    int supernodes_done;       /* < nsuper only when time_budget_s cut the run short */
+    int64_t ops_done;          /* FactorOperationCount of the supernodes factored */
 } or_numeric;
 
 int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o,

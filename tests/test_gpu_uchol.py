@@ -214,9 +214,16 @@ def test_mfma_accumulator_layout(precision):
 
 
 @pytest.mark.parametrize("ordering", [M.NATURAL, M.AMD])
-def test_factor_blocks_match_oracle(ordering):
-    """Every factor column of every supernode against the oracle's packed blocks."""
-    n, r, c, v = random_matrix(55, 0.15, 7)
+@pytest.mark.parametrize("case", ["random", "grid", "dense"])
+def test_factor_blocks_match_oracle(ordering, case):
+    """Every factor column of every supernode against the oracle's packed blocks (covers fused
+    chains, multi-panel and blocked Schur paths: the dense case is one 300-column front)."""
+    if case == "random":
+        n, r, c, v = random_matrix(55, 0.15, 7)
+    elif case == "grid":
+        n, r, c, v = grid_matrix(9, 9, 9, 2)
+    else:
+        n, r, c, v = random_matrix(300, 1.0, 8)
     pm, oc = build(n, r, c, v)
     solver = M.SymmetricUtduSolver(M.UtduSolverOptions(ordering=ordering))
     sym = solver.analyze(pm)
@@ -227,7 +234,7 @@ def test_factor_blocks_match_oracle(ordering):
     for s in range(os_.nsuper):
         fs = int(st["pat_ptr"][s + 1] - st["pat_ptr"][s])
         w = int(st["first"][s + 1] - st["first"][s])
-        F = M.test_numeric_front(f, s, fs)
+        F = M.test_factor_block(f, s, fs, w)
         blk = of.factor[os_.block_off[s]:os_.block_off[s + 1]]
         off = 0
         for k in range(w):

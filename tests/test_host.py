@@ -1,0 +1,170 @@
+"""Host-side product code (no GPU): the C ABI library loads and exports every
+declared symbol; the symbolic phase equals the oracle's independent
+restatement bit for bit; COO assembly and LineCross follow the reference."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import magla_amd as M
+import oracle as O
+from conftest import ROOT, grid_matrix, random_matrix, tridiagonal
+from magla_amd import generators as G
+
+
+def declared_symbols():
+    names = set()
+    for h in ("magla.h", "magla_testing.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(mg_[a-z0-9_]+)\s*\(", src))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = M.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for name in sorted(names):
+        assert hasattr(lib, name), name
+
+
+def test_options_defaults_mirror_reference():
+    o = M.UtduSolverOptions()._c()
+    assert (o.ordering, o.supernode_amalgamation_limit, o.panel_width, o.allow_diagonal_regularization) == (1, 64, 32, 1)
+    assert (o.pivot_tolerance, o.diagonal_regularization, o.parallel_update_threshold) == (1e-13, 1e-10, 1 << 20)
+
+
+CASES = [grid_matrix(7, 6, 5, 2), random_matrix(250, 0.02, 99), random_matrix(45, 0.9, 3), tridiagonal(400, 4, -1),
+         G.laplacian_2d(64), G.laplacian_2d(33, nine_point=True), G.diffusion_3d_lognormal(14, 1),
+         G.elasticity_3d(4, 2)]
+
+
+@pytest.mark.parametrize("idx", range(len(CASES)))
+@pytest.mark.parametrize("ordering", [M.NATURAL, M.AMD])
+def test_symbolic_equals_oracle(idx, ordering):
+    n, r, c, v = CASES[idx]
+    pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+    oc = O.Csr.from_coo(n, r, c, v)
+    rp, ci, va = oc.arrays()
+    np.testing.assert_array_equal(pm.row_pointers, rp)
+    np.testing.assert_array_equal(pm.column_indices, ci)
+    np.testing.assert_array_equal(pm.values, va)
+    sym = M.SymmetricUtduSolver(M.UtduSolverOptions(ordering=ordering)).analyze(pm)
+    st = sym.structure()
+    os_ = O.Symbolic(oc, O.options(ordering=ordering))
+    np.testing.assert_array_equal(st["fill_order"], os_.amd)
+    np.testing.assert_array_equal(sym.permutation, os_.perm)
+    np.testing.assert_array_equal(sym.inverse_permutation, os_.iperm)
+    np.testing.assert_array_equal(st["etree"], os_.etree)
+    np.testing.assert_array_equal(st["colcount"], os_.colcount)
+    np.testing.assert_array_equal(st["first"], os_.first)
+    np.testing.assert_array_equal(st["sparent"], os_.sparent)
+    np.testing.assert_array_equal(st["pat_ptr"], os_.pat_ptr)
+    np.testing.assert_array_equal(st["pat_rows"], os_.pat_rows)
+    inf = sym.info()
+    assert inf["factor_entry_count"] == os_.factor_entries
+    assert inf["factor_operation_count"] == os_.factor_ops
+    assert inf["strict_factor_nonzero_count"] == os_.strict_nonzeros
+    assert inf["max_front_size"] == os_.max_front
+    assert inf["peak_contribution_bytes"] == 8 * os_.peak_contribution
+
+
+def test_symbolic_on_random_graphs_many_seeds():
+    rng = np.random.default_rng(0)
+    for trial in range(150):
+        n = int(rng.integers(1, 120))
+        dens = float(rng.uniform(0.0, 0.3))
+        args = random_matrix(n, dens, int(rng.integers(1 << 30)))
+        pm = M.SymmetricCSRMatrix.from_coo(*args)
+        oc = O.Csr.from_coo(*args)
+        sym = M.SymmetricUtduSolver().analyze(pm)
+        os_ = O.Symbolic(oc)
+        assert np.array_equal(sym.permutation, os_.perm), trial
+        st = sym.structure()
+        assert np.array_equal(st["pat_rows"], os_.pat_rows), trial
+
+
+def test_cost_estimates_consistent():
+    # SymmetricUtduSolverTests.cs:662-686
+    pm = M.SymmetricCSRMatrix.from_coo(*grid_matrix(10, 10, 10, 2))
+    sym = M.SymmetricUtduSolver().analyze(pm)
+    inf = sym.info()
+    assert inf["size"] == pm.size and inf["supernode_count"] > 0
+    assert inf["strict_factor_nonzero_count"] >= pm.nonzero_count
+    assert inf["factor_entry_count"] >= inf["strict_factor_nonzero_count"]
+    assert inf["estimated_factor_bytes"] == 8 * inf["factor_entry_count"]
+    assert sym.estimate_peak_bytes(4) > inf["estimated_factor_bytes"]
+    assert inf["factor_operation_count"] > inf["strict_factor_nonzero_count"]
+    assert 0 < inf["max_front_size"] <= pm.size
+    assert sorted(sym.permutation) == list(range(pm.size))
+    assert np.array_equal(sym.inverse_permutation[sym.permutation], np.arange(pm.size))
+
+
+@pytest.mark.parametrize("scenario", range(4))
+def test_amd_valid_permutation(scenario):
+    # SymmetricUtduSolverTests.cs:469-505
+    if scenario == 0:
+        args = grid_matrix(7, 6, 5, 2)
+    elif scenario == 1:
+        args = random_matrix(250, 0.02, 99)
+    elif scenario == 2:
+        args = (60, np.arange(60), np.arange(60), np.arange(1, 61, dtype=float))
+    else:
+        r = np.concatenate([np.arange(99), np.arange(100, 199), np.arange(200)])
+        c = np.concatenate([np.arange(1, 100), np.arange(101, 200), np.arange(200)])
+        v = np.concatenate([np.full(198, -1.0), np.full(200, 4.0)])
+        args = (200, r, c, v)
+    pm = M.SymmetricCSRMatrix.from_coo(*args)
+    sym = M.SymmetricUtduSolver().analyze(pm)
+    assert sorted(sym.permutation) == list(range(pm.size))
+
+
+def test_coo_semantics_equal_oracle_randomized():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        n = int(rng.integers(1, 40))
+        cnt = int(rng.integers(0, 300))
+        r = rng.integers(0, n, cnt)
+        c = rng.integers(0, n, cnt)
+        v = rng.choice([1.0, -1.0, 0.5, 1e-16, 2.0], cnt)
+        pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+        rp, ci, va = O.Csr.from_coo(n, r, c, v).arrays()
+        assert np.array_equal(pm.row_pointers, rp) and np.array_equal(pm.column_indices, ci)
+        assert np.array_equal(pm.values, va)
+
+
+def test_coo_out_of_range_raises():
+    with pytest.raises(M.ArgumentOutOfRangeError):
+        M.SymmetricCSRMatrix.from_coo(2, [0, 2], [0, 0], [1.0, 1.0])
+
+
+def test_line_cross_equals_oracle_sequence():
+    n, r, c, v = grid_matrix(4, 4, 3, 2)
+    pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+    oc = O.Csr.from_coo(n, r, c, v)
+    rhs = np.linspace(-1, 1, n)
+    rhs2 = rhs.copy()
+    idx = np.array([5, 0, 17, 30, 2], dtype=np.int32)
+    pv = np.array([1.5, -2.0, 0.25, 3.0, 0.0])
+    pm.line_cross(rhs, pv, idx)
+    for i, p in zip(idx, pv):
+        oc.linecross(rhs2, p, int(i))
+    np.testing.assert_array_equal(rhs, rhs2)
+    np.testing.assert_array_equal(pm.values, oc.arrays()[2])
+
+
+def test_elasticity_generator_shape():
+    n, r, c, v = G.elasticity_3d(3, 1)
+    assert n == 3 * 4 ** 3
+    pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+    oc = O.Csr.from_coo(n, r, c, v)
+    Ad = oc.dense()
+    # rigid translations are in the null space of the unconstrained stiffness
+    for d in range(3):
+        t = np.zeros(n)
+        t[d::3] = 1.0
+        assert np.abs(Ad @ t).max() < 1e-12 * np.abs(Ad).max()
+    assert np.all(np.linalg.eigvalsh(Ad) > -1e-10)
+    assert pm.nonzero_count == oc.nnz
