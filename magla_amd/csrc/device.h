@@ -55,6 +55,7 @@ struct Matrix {
     std::vector<int> h_rowptr, h_colidx;  // upper structure (host copy)
     DevArray<int> rowptr, colidx;         // full structure
     DevArray<int> src;                    // full entry -> upper value index
+    DevArray<int> dpos;                   // full-CSR position of A[i,i] or -1
     DevArray<double> vals;                // full values
     DevArray<double> invdiag;             // Jacobi: 1 / A[i,i]
     int diag_status = 0;                  // 0 ok, else MG_E_NOT_PD (zero/invalid diagonal)

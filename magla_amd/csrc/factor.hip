@@ -31,7 +31,8 @@ namespace mg {
 constexpr int kTile = 64;   // tile edge of assembly / Schur tiles
 constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag / panel_rows pass
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
-constexpr int kMaxFrontWidth = 4096;  // widest fused front (LDS-resident solve vectors)
+constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
+constexpr int kFootChunk = 1024;      // footer rows per partial product in the backward solve
 
 // A device front is a chain of consecutive supernodes s, s+1, ..., each the
 // elimination-tree parent of the previous one (the last column of s has the
@@ -54,15 +55,24 @@ struct DevicePlan {
     std::vector<int> lvl_npanels;
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_upd_ptr, lvl_diag_ptr;
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
+    std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
+    std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
+
+    std::vector<std::vector<int>> lvl_sfwd_ptr, lvl_sbwd_ptr;  // per 64-column panel
     int max_panel_tasks = 0;        // largest panel-task launch (solve partial-sum buffer)
     int64_t vsum = 0;               // sum f over fronts (solve front vectors)
     // algorithmic work per launch
     std::vector<double> lvl_pull_elems, lvl_pull_index;
     std::vector<std::vector<double>> lvl_upd_flops, lvl_upd_elems;
+    std::vector<std::vector<double>> lvl_panel_elems;  // sum over fronts of (f - pe) * nb (L below the panel)
 
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, diag_nodes, diag_tb;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
-    DevArray<int2> init_tasks;
+    DevArray<int2> init_tasks, sfwd_tasks, sbwd_tasks;
+    DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
+    DevArray<int4> foot_tasks;     // (front, column block, row chunk, index of the chunk in the block)
+
+    int max_foot_tasks = 0;
     DevArray<int> asm_vidx;
     DevArray<int4> pull_tasks, upd_tasks;
     DevArray<int2> panel_tasks;
@@ -213,6 +223,11 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     for (int g = 0; g < nf; ++g) vbase[g + 1] = vbase[g] + P->h_f[g];
     P->vsum = vbase[nf];
     P->lvl_init_ptr.push_back(0);
+    P->lvl_foot_ptr.push_back(0);
+    P->lvl_footr_ptr.push_back(0);
+    std::vector<int2> sfwd, sbwd;
+    std::vector<int4> foot, footr;
+
     std::vector<int64_t> asm_pos;
     std::vector<int> asm_vidx;
     P->lvl_node_ptr.push_back(0);
@@ -250,6 +265,34 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
                 }
             }
         }
+        // solve tasks
+        const int foot_base = (int)foot.size();
+        for (int g : bylevel[L]) {
+            const int f = P->h_f[g], w = P->h_w[g];
+            const int nchunk = (f - w + kFootChunk - 1) / kFootChunk;
+            for (int cb = 0; cb < (w + kPanel - 1) / kPanel; ++cb) {
+                footr.push_back(make_int4(g, cb, (int)foot.size() - foot_base, nchunk));
+                for (int ch = 0; ch < nchunk; ++ch) foot.push_back(make_int4(g, cb, ch, 0));
+            }
+        }
+        P->max_foot_tasks = std::max(P->max_foot_tasks, (int)foot.size() - foot_base);
+        std::vector<int> sf{(int)sfwd.size()}, sb{(int)sbwd.size()};
+        for (int q = 0; q < npan; ++q) {
+            for (int g : bylevel[L]) {
+                const int f = P->h_f[g], w = P->h_w[g];
+                if (w <= q * kPanel) continue;
+                const int pe = std::min(q * kPanel + kPanel, w);
+                sfwd.push_back(make_int2(g, -1));
+                for (int rb = 0; rb < (f - pe + kTile - 1) / kTile; ++rb) sfwd.push_back(make_int2(g, rb));
+                for (int cb = 0; cb <= q; ++cb) sbwd.push_back(make_int2(g, cb));
+            }
+            sf.push_back((int)sfwd.size());
+            sb.push_back((int)sbwd.size());
+        }
+        P->lvl_sfwd_ptr.push_back(sf);
+        P->lvl_sbwd_ptr.push_back(sb);
+        P->lvl_foot_ptr.push_back((int)foot.size());
+        P->lvl_footr_ptr.push_back((int)footr.size());
         P->lvl_node_ptr.push_back((int)level_nodes.size());
         P->lvl_init_ptr.push_back((int)init.size());
         P->lvl_pull_ptr.push_back((int)pull.size());
@@ -258,7 +301,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_pull_elems.push_back(pe_elems);
         P->lvl_pull_index.push_back(pe_index);
         std::vector<int> pp{(int)panel.size()}, up{(int)upd.size()}, dp{(int)diag_nodes.size()};
-        std::vector<double> ufl(npan, 0.0), uel(npan, 0.0);
+        std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
         for (int q = 0; q < npan; ++q) {
             const int pbase = (int)panel.size();
             for (int g : bylevel[L]) {
@@ -267,6 +310,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
                 diag_nodes.push_back(g);
                 diag_tb.push_back((int)panel.size() - pbase);
                 const int pe = std::min(q * kPanel + kPanel, w);
+                pel[q] += (double)(f - pe) * (pe - q * kPanel);
                 const int nrb = (f - pe + kTile - 1) / kTile;
                 for (int rb = 0; rb < nrb; ++rb) panel.push_back(make_int2(g, rb));
             }
@@ -292,6 +336,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_diag_ptr.push_back(dp);
         P->lvl_upd_flops.push_back(ufl);
         P->lvl_upd_elems.push_back(uel);
+        P->lvl_panel_elems.push_back(pel);
     }
 
     if (!upload) return P;
@@ -306,6 +351,11 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->diag_nodes.upload(diag_nodes);
     P->diag_tb.upload(diag_tb);
     P->init_tasks.upload(init);
+    P->foot_tasks.upload(foot);
+    P->footr_tasks.upload(footr);
+
+    P->sfwd_tasks.upload(sfwd);
+    P->sbwd_tasks.upload(sbwd);
     P->vbase.upload(vbase);
     P->inv_map.upload(inv_map);
     P->perm.upload(S.perm);
@@ -394,73 +444,124 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // pivot > tol*scale (NaN fails), otherwise regularize to reg*scale and count
 // (negative counted separately; bit 2 marks a refused pivot when
 // regularization is off); the comparison is made in float64 for both
-// precisions.  Column scaling multiplies by the reciprocal as the reference
-// does (:577).
+// precisions.  One barrier per pivot: every thread evaluates the pivot, the
+// trailing update uses the unscaled column (a_rc -= a_rk (a_ck / d)), the
+// column is scaled by the reciprocal (:577) during the next step, and the
+// inverse Li = L11^{-1} is built alongside (Li <- (I - l_k e_k^T) Li).  Li's
+// strict lower part is stored transposed in the unused upper triangle of the
+// diagonal block (front row ps+m, column ps+k holds Li[k][m]) for the
+// row-block kernel and the solves.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
-                                                    const int *__restrict__ F, const int *__restrict__ W,
-                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                    int ps, const double *__restrict__ scale, double tol, double regf,
-                                                    int allow_reg, double *__restrict__ D,
-                                                    unsigned char *__restrict__ flags) {
-    __shared__ T L11[kPanel][kPanel + 1];
-    __shared__ T dv[kPanel], dinv[kPanel];
+__device__ inline T lane_value(T v, int lane);
+template <>
+__device__ inline float lane_value<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ inline double lane_value<double>(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// State of one 64x64 diagonal-block factorization held by a single wave.
+struct PanelDiagCtx {
+    int r, nb, allow_reg, fc;
+    double thr, regv;
+    double *D;
+    unsigned char *flags;
+};
+
+// Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600):
+// pivot rule, scaled column K, rank-1 update of the rows below, and the same row
+// operation on the inverse.  K is a template argument so every broadcast is a
+// readlane with a constant lane and every row entry a fixed register.
+template <typename T, int K>
+__device__ __forceinline__ void panel_diag_step(T (&a)[kPanel], T (&li)[kPanel], const PanelDiagCtx &x) {
+    const T akk = lane_value<T>(a[K], K);
+    double piv = (double)akk;
+    unsigned char fl = 0;
+    if (K < x.nb && !(piv > x.thr)) {
+        if (!x.allow_reg) fl |= 4;
+        if (piv < 0.0) fl |= 2;
+        fl |= 1;
+        piv = x.regv;
+    }
+    const T pv = (T)piv, pinv = T(1) / pv;
+    if (x.r == 0 && K < x.nb) {
+        x.D[x.fc + K] = (double)pv;
+        x.flags[x.fc + K] = fl;
+    }
+    const bool below = x.r > K;
+    const T l = a[K] * pinv;  // meaningful for lanes below K
+    const T lb = below ? l : T(0);
+    const T t = lb * pv;
+    a[K] = below ? l : a[K];
+    // Entries above a lane's diagonal take updates too: they are never read.
+#pragma unroll
+    for (int c = K + 1; c < kPanel; ++c) a[c] -= t * lane_value<T>(l, c);
+#pragma unroll
+    for (int c = 0; c <= K; ++c) li[c] -= lb * lane_value<T>(li[c], K);
+    // Materialize every row entry at the end of the step; otherwise the compiler
+    // defers updates to their first use and keeps broadcast values alive across
+    // steps in scalar registers.
+#pragma unroll
+    for (int c = 0; c < kPanel; ++c) asm volatile("" : "+v"(a[c]), "+v"(li[c]));
+}
+
+template <typename T, int... K>
+__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...>, T (&a)[kPanel], T (&li)[kPanel],
+                                                 const PanelDiagCtx &x) {
+    (panel_diag_step<T, K>(a, li, x), ...);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
+                                                   const int *__restrict__ F, const int *__restrict__ W,
+                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                                   int ps, const double *__restrict__ scale, double tol, double regf,
+                                                   int allow_reg, double *__restrict__ D,
+                                                   unsigned char *__restrict__ flags) {
+    // One wave; lane r keeps row r of the block (a) and of its inverse (li) in registers.
+    // Rows/columns past nb are padded with the identity (pivot 1, never reported).
     const int s = nodes[blockIdx.x];
     const int f = F[s], w = W[s];
     if (w <= ps) return;
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     T *M = arena + off[s];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r >= c) L11[r][c] = M[(int64_t)(ps + c) * f + ps + r];
+    const int r = threadIdx.x;
+    const double sc = scale[0];
+    PanelDiagCtx x{r, nb, allow_reg, firstcol[s] + ps, tol * sc, regf * sc, D, flags};
+    T a[kPanel], li[kPanel];
+#pragma unroll
+    for (int c = 0; c < kPanel; ++c) {
+        a[c] = (r < nb && c < nb && r >= c) ? M[(int64_t)(ps + c) * f + ps + r] : (r == c ? T(1) : T(0));
+        li[c] = r == c ? T(1) : T(0);
     }
-    __syncthreads();
-    for (int k = 0; k < nb; ++k) {
-        if (tid == 0) {
-            const double thr = tol * scale[0], regv = regf * scale[0];
-            double piv = (double)L11[k][k];
-            unsigned char fl = 0;
-            if (!(piv > thr)) {
-                if (!allow_reg) fl |= 4;
-                if (piv < 0.0) fl |= 2;
-                fl |= 1;
-                piv = regv;
-            }
-            const T pv = (T)piv;
-            dv[k] = pv;
-            dinv[k] = T(1) / pv;
-            D[firstcol[s] + ps + k] = (double)pv;
-            flags[firstcol[s] + ps + k] = fl;
+    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x);
+    if (r < nb) {
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) {
+            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[c];   // L below the diagonal
+            else if (c == r) M[(int64_t)(ps + c) * f + ps + r] = T(1);
         }
-        __syncthreads();
-        for (int r = k + 1 + tid; r < nb; r += 256) L11[r][k] *= dinv[k];
-        __syncthreads();
-        const int m = nb - k - 1;
-        for (int idx = tid; idx < m * m; idx += 256) {
-            const int r = k + 1 + idx % m, c = k + 1 + idx / m;
-            if (r >= c) L11[r][c] -= L11[r][k] * (dv[k] * L11[c][k]);
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r > c) M[(int64_t)(ps + c) * f + ps + r] = L11[r][c];
-        else if (r == c) M[(int64_t)(ps + c) * f + ps + r] = T(1);
+#pragma unroll
+        for (int m = 0; m < kPanel; ++m)
+            if (m < r) M[(int64_t)(ps + r) * f + ps + m] = li[m];  // Li[r][m] at row ps+m, column ps+r
     }
 }
 
-// Panel LDL^T, step 2: each workgroup solves one 64-row block below the
-// factored diagonal block, X := X L11^{-T} D^{-1} (right-looking over the
-// panel columns, as the reference's in-panel loop :580-588).
+// Panel LDL^T, step 2: each workgroup turns one 64-row block below the diagonal
+// block into L rows: X := X B with B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k
+// (Li from the diagonal block's upper triangle, Li[k][k] = 1).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tasks, T *__restrict__ arena,
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                     int ps, const double *__restrict__ D) {
-    __shared__ T L11[kPanel][kPanel + 1];
+    __shared__ T Bm[kPanel][kPanel + 1];  // Bm[m][k], m <= k
     __shared__ T X[kTile][kPanel + 1];
-    __shared__ T dv[kPanel], dinv[kPanel];
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
     const int f = F[s], w = W[s];
@@ -468,37 +569,41 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     const int r0 = pe + rb * kTile;
     const int nr = min(kTile, f - r0);
     if (nr <= 0) return;
-    const T *M0 = arena + off[s];
     T *M = arena + off[s];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r > c) L11[r][c] = M0[(int64_t)(ps + c) * f + ps + r];
-    }
-    for (int k = tid; k < nb; k += 256) {
-        const T pv = (T)D[firstcol[s] + ps + k];
-        dv[k] = pv;
-        dinv[k] = T(1) / pv;
-    }
-    for (int idx = tid; idx < nr * nb; idx += 256) {
-        const int r = idx % nr, c = idx / nr;
-        X[r][c] = M0[(int64_t)(ps + c) * f + r0 + r];
+    const int fc = firstcol[s] + ps;
+    {
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int k = wv; k < nb; k += 4) {
+            const T dk = (T)D[fc + k];
+            if (lane < k) Bm[lane][k] = M[(int64_t)(ps + k) * f + ps + lane] / dk;
+            else if (lane == k) Bm[lane][k] = T(1) / dk;
+            if (lane < nr) X[lane][k] = M[(int64_t)(ps + k) * f + r0 + lane];
+        }
     }
     __syncthreads();
-    for (int k = 0; k < nb; ++k) {
-        for (int r = tid; r < nr; r += 256) X[r][k] *= dinv[k];
-        __syncthreads();
-        const int m = nb - k - 1;
-        for (int idx = tid; idx < nr * m; idx += 256) {
-            const int r = idx % nr, c = k + 1 + idx / nr;
-            X[r][c] -= X[r][k] * (dv[k] * L11[c][k]);
+    // thread -> (row r, columns k = c0, c0+4, ...)
+    const int r = tid & 63, c0 = tid >> 6;
+    T out[kPanel / 4];
+#pragma unroll
+    for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
+    if (r < nr) {
+        for (int m = 0; m < nb; ++m) {
+            const T xm = X[r][m];
+#pragma unroll
+            for (int q = 0; q < kPanel / 4; ++q) {
+                const int k = c0 + 4 * q;
+                if (k < nb && m <= k) out[q] += xm * Bm[m][k];
+            }
         }
-        __syncthreads();
     }
-    for (int idx = tid; idx < nr * nb; idx += 256) {
-        const int r = idx % nr, c = idx / nr;
-        M[(int64_t)(ps + c) * f + r0 + r] = X[r][c];
-    }
+    __syncthreads();
+    if (r < nr)
+#pragma unroll
+        for (int q = 0; q < kPanel / 4; ++q) {
+            const int k = c0 + 4 * q;
+            if (k < nb) M[(int64_t)(ps + k) * f + r0 + r] = out[q];
+        }
 }
 
 // Schur update after panel q (cf. ApplyPanelUpdate, UtduNumericFactorization.cs:624-852):
@@ -608,18 +713,18 @@ __global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ 
     else *dst = (double)Xe[e];
 }
 
-// Supernodal solves, level by level over the fronts, in 64-column panels
+// Supernodal solves, level by level over the fronts
 // (cf. ForwardSubstitution / BackwardSubstitution, UtduNumericFactorization.cs:139-207).
 // Every front owns a vector V (f rows x nrhs) in a workspace: the forward pass
 // gathers it (own rows from the permuted right-hand side, plus the children's
-// footer vectors through the inverse row maps), then per panel solves the unit
-// lower triangle (k_fwd_diag) and subtracts the panel's product from the rows
-// below (k_fwd_update); the footer left in V is the update the parent pulls.
-// The backward pass walks the panels from the last one down: partial products
-// of the rows below with the final x (k_bwd_partial, one 64-row block each) are
-// summed in a fixed order by k_bwd_diag, which divides by D and solves the
-// transposed unit triangle.  Y holds y after the forward pass and x after the
-// backward pass (elimination order, column-major n x nrhs).
+// footer vectors through the inverse row maps), then walks the own columns in
+// 64-column panels, one launch each (k_fwd_panel: y from the stored 64x64
+// inverse, then the product with the rows below); the footer left in V is the
+// update the parent pulls.  The backward pass starts each level with the
+// footer product (two-stage: k_bwd_foot_part / k_bwd_foot_red, including the
+// division by D) and walks the panels from the last one down, right-looking
+// (k_bwd_panel).  Y holds y after the
+// forward pass and x after the backward pass (elimination order, n x nrhs).
 constexpr int kRhsChunk = 16;
 
 template <typename T>
@@ -646,157 +751,168 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     }
 }
 
+// Stages dst[r][c] = M[(c0 + c) * f + r0 + r] for r < nr, c < nc (zero elsewhere).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_diag(const int *__restrict__ nodes, const T *__restrict__ arena,
-                                                  const int *__restrict__ F, const int *__restrict__ W,
-                                                  const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                  const int64_t *__restrict__ vbase, T *__restrict__ V,
-                                                  T *__restrict__ Y, int ps, int n, int nrhs) {
-    __shared__ T L[kPanel][kPanel + 1];
-    __shared__ T y[kPanel][kRhsChunk + 1];
-    const int g = nodes[blockIdx.x];
-    const int f = F[g], w = W[g];
-    const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const T *M = arena + off[g];
-    T *Vg = V + vbase[g] * nrhs;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r > c) L[r][c] = M[(int64_t)(ps + c) * f + ps + r];
-    }
-    for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nrhs - r0);
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            y[k][rr] = Vg[(int64_t)(r0 + rr) * f + ps + k];
-        }
-        __syncthreads();
-        for (int k = 0; k < nb - 1; ++k) {
-            const int m = nb - k - 1;
-            for (int idx = tid; idx < m * nr; idx += 256) {
-                const int q = k + 1 + idx % m, rr = idx / m;
-                const T yk = y[k][rr];
-                if (yk != T(0)) y[q][rr] -= L[q][k] * yk;
-            }
-            __syncthreads();
-        }
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            Vg[(int64_t)(r0 + rr) * f + ps + k] = y[k][rr];
-            Y[(int64_t)(r0 + rr) * n + firstcol[g] + ps + k] = y[k][rr];
-        }
-        __syncthreads();
+__device__ inline void stage64(T (*dst)[kPanel + 1], const T *__restrict__ M, int f, int r0, int c0, int nr, int nc) {
+    for (int idx = threadIdx.x; idx < kPanel * kPanel; idx += 256) {
+        const int r = idx % kPanel, c = idx / kPanel;
+        dst[r][c] = (r < nr && c < nc) ? M[(int64_t)(c0 + c) * f + r0 + r] : T(0);
     }
 }
 
+// Forward panel (columns [ps, pe) of a front): every workgroup forms y = Li v[ps:pe)
+// from the stored 64x64 inverse; task rb = -1 writes y to Y, task rb >= 0 subtracts
+// L[rows, ps:pe) y from its 64-row block of V below the panel.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_update(const int2 *__restrict__ tasks, const T *__restrict__ arena,
-                                                    const int *__restrict__ F, const int *__restrict__ W,
-                                                    const int64_t *__restrict__ off, const int64_t *__restrict__ vbase,
-                                                    T *__restrict__ V, int ps, int nrhs) {
-    __shared__ T y[kPanel][kRhsChunk + 1];
+__global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
+                                                   const int *__restrict__ F, const int *__restrict__ W,
+                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                                   const int64_t *__restrict__ vbase, T *__restrict__ V,
+                                                   T *__restrict__ Y, int ps, int n, int nrhs) {
+    __shared__ T inv[kPanel][kPanel + 1];
+    __shared__ T blk[kPanel][kPanel + 1];
+    __shared__ T vv[kPanel][kRhsChunk + 1];
+    __shared__ T yv[kPanel][kRhsChunk + 1];
     const int2 tk = tasks[blockIdx.x];
-    const int g = tk.x, f = F[g], w = W[g];
+    const int g = tk.x, rb = tk.y, f = F[g], w = W[g];
     const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const int r0 = pe + tk.y * kTile, nrow = min(kTile, f - r0);
     const T *M = arena + off[g];
     T *Vg = V + vbase[g] * nrhs;
-    const int tid = threadIdx.x, row = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    const int r0 = pe + (rb < 0 ? 0 : rb) * kTile, nrow = rb < 0 ? 0 : min(kTile, f - r0);
+    stage64<T>(inv, M, f, ps, ps, nb, nb);          // inv[mm][kk] = Li[kk][mm], mm < kk
+    if (rb >= 0) stage64<T>(blk, M, f, r0, ps, nrow, nb);  // blk[i][k] = L[r0+i][ps+k]
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            y[k][rr] = Vg[(int64_t)(c0 + rr) * f + ps + k];
-        }
+        for (int rr = grp; rr < nr; rr += 4)
+            if (lane < nb) vv[lane][rr] = Vg[(int64_t)(c0 + rr) * f + ps + lane];
         __syncthreads();
-        if (row < nrow)
-            for (int rr = tid >> 6; rr < nr; rr += 4) {
-                T acc = Vg[(int64_t)(c0 + rr) * f + r0 + row];
-                for (int k = 0; k < nb; ++k) {
-                    const T yk = y[k][rr];
-                    if (yk != T(0)) acc -= M[(int64_t)(ps + k) * f + r0 + row] * yk;
-                }
-                Vg[(int64_t)(c0 + rr) * f + r0 + row] = acc;
+        for (int rr = grp; rr < nr; rr += 4)
+            if (lane < nb) {
+                T acc = vv[lane][rr];
+                for (int mm = 0; mm < lane; ++mm) acc += inv[mm][lane] * vv[mm][rr];
+                yv[lane][rr] = acc;
             }
+        __syncthreads();
+        if (rb < 0) {
+            for (int rr = grp; rr < nr; rr += 4)
+                if (lane < nb) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + lane] = yv[lane][rr];
+        } else if (lane < nrow) {
+            for (int rr = grp; rr < nr; rr += 4) {
+                T acc = T(0);
+                for (int k = 0; k < nb; ++k) acc += blk[lane][k] * yv[k][rr];
+                Vg[(int64_t)(c0 + rr) * f + r0 + lane] -= acc;
+            }
+        }
         __syncthreads();
     }
 }
 
+// Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
+// L[i, 64cb+col] x[row i] (x of the footer rows are final: ancestors are solved first).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_partial(const int2 *__restrict__ tasks, const T *__restrict__ arena,
-                                                     const int *__restrict__ F, const int *__restrict__ W,
-                                                     const int64_t *__restrict__ off, const int *__restrict__ pat_ptr,
-                                                     const int *__restrict__ pat_rows, const T *__restrict__ Y,
-                                                     T *__restrict__ part, int ps, int n, int nrhs) {
-    __shared__ T Lb[kTile][kPanel + 1];
+__global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ tasks, const T *__restrict__ arena,
+                                                       const int *__restrict__ F, const int *__restrict__ W,
+                                                       const int64_t *__restrict__ off, const int *__restrict__ pat_ptr,
+                                                       const int *__restrict__ pat_rows, const T *__restrict__ Y,
+                                                       T *__restrict__ part, int n, int nrhs) {
+    __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T xs[kTile][kRhsChunk + 1];
-    const int2 tk = tasks[blockIdx.x];
+    const int4 tk = tasks[blockIdx.x];
     const int g = tk.x, f = F[g], w = W[g];
-    const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const int r0 = pe + tk.y * kTile, nrow = min(kTile, f - r0);
+    const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
+    const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
     const T *M = arena + off[g];
-    const int *rows = pat_rows + pat_ptr[g] + r0;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kTile * nb; idx += 256) {
-        const int i = idx % kTile, k = idx / kTile;
-        Lb[i][k] = i < nrow ? M[(int64_t)(ps + k) * f + r0 + i] : T(0);
-    }
+    const int *rows = pat_rows + pat_ptr[g];
+    const int tid = threadIdx.x, col = tid & 63, grp = tid >> 6;
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        for (int idx = tid; idx < kTile * nr; idx += 256) {
-            const int i = idx % kTile, rr = idx / kTile;
-            xs[i][rr] = i < nrow ? Y[(int64_t)(c0 + rr) * n + rows[i]] : T(0);
+        T acc[kRhsChunk / 4];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk / 4; ++j) acc[j] = T(0);
+        for (int i0 = ibeg; i0 < iend; i0 += kTile) {
+            const int ni = min(kTile, iend - i0);
+            __syncthreads();
+            stage64<T>(blk, M, f, i0, cb0, ni, ncol);  // blk[i][c] = L[i0+i][cb0+c]
+            for (int idx = tid; idx < kTile * nr; idx += 256) {
+                const int i = idx % kTile, rr = idx / kTile;
+                xs[i][rr] = i < ni ? Y[(int64_t)(c0 + rr) * n + rows[i0 + i]] : T(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j) {
+                const int rr = grp + 4 * j;
+                if (rr < nr)
+                    for (int i = 0; i < ni; ++i) acc[j] += blk[i][col] * xs[i][rr];
+            }
         }
-        __syncthreads();
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            T acc = T(0);
-            for (int i = 0; i < nrow; ++i) acc += Lb[i][k] * xs[i][rr];
-            part[((int64_t)blockIdx.x * kPanel + k) * nrhs + c0 + rr] = acc;
+#pragma unroll
+        for (int j = 0; j < kRhsChunk / 4; ++j) {
+            const int rr = grp + 4 * j;
+            if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = acc[j];
         }
-        __syncthreads();
     }
 }
 
+// Backward footer, stage 2: Z[own c] = y_c / d_c - sum of the partials (fixed order).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_diag(const int *__restrict__ nodes, const int *__restrict__ task_begin,
-                                                  const T *__restrict__ arena, const int *__restrict__ F,
-                                                  const int *__restrict__ W, const int64_t *__restrict__ off,
-                                                  const int *__restrict__ firstcol, const double *__restrict__ D,
-                                                  T *__restrict__ Y, const T *__restrict__ part, int ps, int n,
-                                                  int nrhs) {
-    __shared__ T L[kPanel][kPanel + 1];
-    __shared__ T z[kPanel][kRhsChunk + 1];
-    const int g = nodes[blockIdx.x];
-    const int f = F[g], w = W[g];
-    const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const int nrb = (f - pe + kTile - 1) / kTile, tb = task_begin[blockIdx.x];
-    const int fc = firstcol[g] + ps;
-    const T *M = arena + off[g];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r > c) L[r][c] = M[(int64_t)(ps + c) * f + ps + r];
+__global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ tasks, const int *__restrict__ W,
+                                                      const int *__restrict__ firstcol, const double *__restrict__ D,
+                                                      const T *__restrict__ Y, const T *__restrict__ part,
+                                                      T *__restrict__ Z, int n, int nrhs) {
+    const int4 tk = tasks[blockIdx.x];
+    const int g = tk.x, cb0 = tk.y * kPanel, tb = tk.z, nch = tk.w;
+    const int ncol = min(kPanel, W[g] - cb0), fc = firstcol[g] + cb0;
+    for (int idx = threadIdx.x; idx < ncol * nrhs; idx += 256) {
+        const int col = idx % ncol, rr = idx / ncol;
+        T s = T(0);
+        for (int ch = 0; ch < nch; ++ch) s += part[((int64_t)(tb + ch) * kPanel + col) * nrhs + rr];
+        const int64_t e = (int64_t)rr * n + fc + col;
+        Z[e] = Y[e] / (T)D[fc + col] - s;
     }
+}
+
+// Backward panel (columns [ps, pe)), right-looking: every workgroup forms
+// x = Li^T z[ps:pe) from the stored inverse; the task of block q writes x to Y, the
+// tasks of blocks cb < q subtract L[ps:pe, block]^T x from their block of Z.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
+                                                   const int *__restrict__ F, const int *__restrict__ W,
+                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                                   T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs) {
+    __shared__ T blk[kPanel][kPanel + 1];
+    __shared__ T inv[kPanel][kPanel + 1];
+    __shared__ T zz[kPanel][kRhsChunk + 1];
+    __shared__ T xv[kPanel][kRhsChunk + 1];
+    const int2 tk = tasks[blockIdx.x];
+    const int g = tk.x, cb = tk.y, f = F[g], w = W[g];
+    const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
+    const int fc = firstcol[g];
+    const T *M = arena + off[g];
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    stage64<T>(inv, M, f, ps, ps, nb, nb);  // inv[mm][kk] = Li[kk][mm], mm < kk
+    if (cb < q) stage64<T>(blk, M, f, ps, cb * kPanel, nb, kPanel);  // blk[k][c] = L[ps+k][64cb+c]
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            T s = T(0);
-            for (int t = 0; t < nrb; ++t) s += part[((int64_t)(tb + t) * kPanel + k) * nrhs + c0 + rr];
-            z[k][rr] = Y[(int64_t)(c0 + rr) * n + fc + k] / (T)D[fc + k] - s;
-        }
+        for (int rr = grp; rr < nr; rr += 4)
+            if (lane < nb) zz[lane][rr] = Z[(int64_t)(c0 + rr) * n + fc + ps + lane];
         __syncthreads();
-        for (int k = nb - 1; k > 0; --k) {
-            for (int idx = tid; idx < k * nr; idx += 256) {
-                const int m = idx % k, rr = idx / k;
-                z[m][rr] -= L[k][m] * z[k][rr];
+        for (int rr = grp; rr < nr; rr += 4)
+            if (lane < nb) {
+                T acc = zz[lane][rr];
+                for (int kk = lane + 1; kk < nb; ++kk) acc += inv[lane][kk] * zz[kk][rr];
+                xv[lane][rr] = acc;
             }
-            __syncthreads();
-        }
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int k = idx % nb, rr = idx / nb;
-            Y[(int64_t)(c0 + rr) * n + fc + k] = z[k][rr];
+        __syncthreads();
+        if (cb == q) {
+            for (int rr = grp; rr < nr; rr += 4)
+                if (lane < nb) Y[(int64_t)(c0 + rr) * n + fc + ps + lane] = xv[lane][rr];
+        } else {
+            for (int rr = grp; rr < nr; rr += 4) {
+                T acc = T(0);
+                for (int k = 0; k < nb; ++k) acc += blk[k][lane] * xv[k][rr];
+                Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] -= acc;
+            }
         }
         __syncthreads();
     }
@@ -804,25 +920,30 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const int *__restrict__ nodes,
 
 // ---- refinement helpers ------------------------------------------------------
 
-// R = B - A X over full CSR (one row per group of lanes), and per-column
-// partial sums of R^2 and B^2 for the relative residual.
-__global__ void k_residual(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v, int n,
-                           int nrhs, const double *__restrict__ B, int ldb, const double *__restrict__ X, int ldx,
-                           double *__restrict__ R, double *__restrict__ part) {
-    // one thread per row; part[(col * gridDim.x + block) * 2 + {0,1}]
+// R = B - A X over the full CSR (LANES lanes per row, fixed xor-tree per row),
+// and per-column block partial sums of R^2 and B^2 for the relative residual:
+// part[(col * gridDim.x + block) * 2 + {0,1}].
+template <int LANES>
+__global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const double *__restrict__ v, int n, int nrhs,
+                                                  const double *__restrict__ B, int ldb, const double *__restrict__ X,
+                                                  int ldx, double *__restrict__ R, double *__restrict__ part) {
     __shared__ double sr[256], sb[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.x * (256 / LANES) + threadIdx.x / LANES, sub = threadIdx.x % LANES;
     for (int c = 0; c < nrhs; ++c) {
         double rr = 0.0, bb = 0.0;
-        if (i < n) {
+        if (row < n) {
             const double *xc = X + (int64_t)c * ldx;
             double acc = 0.0;
-            for (int k = rp[i]; k < rp[i + 1]; ++k) acc += v[k] * xc[ci[k]];
-            const double b = B[(int64_t)c * ldb + i];
-            const double res = b - acc;
-            R[(int64_t)c * n + i] = res;
-            rr = res * res;
-            bb = b * b;
+            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) acc += v[k] * xc[ci[k]];
+            for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+            if (sub == 0) {
+                const double b = B[(int64_t)c * ldb + row];
+                const double res = b - acc;
+                R[(int64_t)c * n + row] = res;
+                rr = res * res;
+                bb = b * b;
+            }
         }
         sr[threadIdx.x] = rr;
         sb[threadIdx.x] = bb;
@@ -885,25 +1006,33 @@ __global__ void k_gather_values(const int *__restrict__ src, int64_t nnz, const 
 // ---------------------------------------------------------------------------
 
 // Max |A_ii| (ComputeDiagonalScale, UtduNumericFactorization.cs:411-418; Math.Max
-// propagates NaN; 1 when the maximum is not positive).
-__global__ __launch_bounds__(1024) void k_diag_scale(const int *__restrict__ rowptr, const int *__restrict__ colidx,
-                                                     const double *__restrict__ vals, int n, double *__restrict__ out) {
-    __shared__ double sm[1024];
+// propagates NaN; 1 when the maximum is not positive), two-stage over a fixed grid.
+__device__ inline double nanmax(double a, double b) { return (isnan(a) || isnan(b)) ? NAN : fmax(a, b); }
+
+__global__ __launch_bounds__(256) void k_diag_scale_part(const int *__restrict__ dpos, const double *__restrict__ vals,
+                                                         int n, double *__restrict__ part) {
+    __shared__ double sm[256];
     double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        double d = 0.0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
-            if (colidx[k] == i) d = vals[k];
-        const double a = fabs(d);
-        m = (isnan(a) || isnan(m)) ? NAN : fmax(m, a);
-    }
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        m = nanmax(m, dpos[i] >= 0 ? fabs(vals[dpos[i]]) : 0.0);
     sm[threadIdx.x] = m;
     __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            const double a = sm[threadIdx.x], b = sm[threadIdx.x + h];
-            sm[threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmax(a, b);
-        }
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = nanmax(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_diag_scale_final(const double *__restrict__ part, int nparts,
+                                                          double *__restrict__ out) {
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 256) m = nanmax(m, part[b]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = nanmax(sm[threadIdx.x], sm[threadIdx.x + h]);
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sm[0] > 0.0 ? sm[0] : 1.0;
@@ -918,15 +1047,17 @@ struct Numeric {
     DevArray<double> diag;
     DevArray<unsigned char> flags;
     DevArray<int> counts;
-    DevArray<double> mags, scale;
+    DevArray<double> mags, scale, wsScale;
     DevArray<double> values;  // upper CSR values (float64), kept for refinement
     Matrix A;                 // full CSR for residuals
     mg_numeric_info info{};
     // solve workspaces
-    DevArray<unsigned char> wsY, wsU, wsP;
+    DevArray<unsigned char> wsY, wsU, wsP, wsQ;
     DevArray<double> wsR, wsX, wsPart, wsRel;
     int ws_nrhs = 0;
 };
+
+constexpr int kScaleParts = 256;
 
 static inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
@@ -1042,7 +1173,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             const int d0 = P.lvl_diag_ptr[L][q], d1 = P.lvl_diag_ptr[L][q + 1];
             if (d1 > d0) {
                 ProfScope ps_(st, KP_PDIAG, 0.0, 0.0);
-                hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
+                hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(64), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.scale.p, opts.pivot_tolerance,
                                    opts.diagonal_regularization, opts.allow_diagonal_regularization, N.diag.p,
                                    N.flags.p);
@@ -1090,6 +1221,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->counts.alloc(4);
             N->mags.alloc(4);
             N->scale.alloc(1);
+            N->wsScale.alloc(kScaleParts);
             N->values.alloc(std::max(S.matrix_nnz, 1));
             N->A.build(S.n, S.a_rowptr.data(), S.a_colidx.data());
         }
@@ -1099,8 +1231,9 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             if (S.matrix_nnz > 0)
                 MG_HIP(hipMemcpyAsync(N->values.p, d_values, sizeof(double) * S.matrix_nnz, hipMemcpyDeviceToDevice, st));
             N->A.set_values_dev(N->values.p, st);
-            hipLaunchKernelGGL(k_diag_scale, dim3(1), dim3(1024), 0, st, N->A.rowptr.p, N->A.colidx.p, N->A.vals.p, S.n,
-                               N->scale.p);
+            hipLaunchKernelGGL(k_diag_scale_part, dim3(kScaleParts), dim3(256), 0, st, N->A.dpos.p, N->A.vals.p, S.n,
+                               N->wsScale.p);
+            hipLaunchKernelGGL(k_diag_scale_final, dim3(1), dim3(256), 0, st, N->wsScale.p, kScaleParts, N->scale.p);
             if (precision == MG_FP32) run_factor<float>(*N, opts, st);
             else run_factor<double>(*N, opts, st);
             hipLaunchKernelGGL(k_pivot_stats, dim3(1), dim3(1024), 0, st, N->diag.p, N->flags.p, S.n, N->counts.p,
@@ -1142,35 +1275,35 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     const int n = N.n;
     T *Y = reinterpret_cast<T *>(N.wsY.p);
     T *V = reinterpret_cast<T *>(N.wsU.p);
-    T *part = reinterpret_cast<T *>(N.wsP.p);
     const T *arena = reinterpret_cast<const T *>(N.arena.p);
     const int64_t tot = (int64_t)n * nrhs;
     hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y);
+    T *Z = reinterpret_cast<T *>(N.wsP.p);
+    T *part = reinterpret_cast<T *>(N.wsQ.p);
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
+        ProfScope pi_(st, KP_FWD, 0.0, 0.0);
         hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p, P.first.p,
                            P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n, nrhs);
-        for (int q = 0; q < P.lvl_npanels[L]; ++q) {
-            const int ps = q * kPanel;
-            const int d0 = P.lvl_diag_ptr[L][q], d1 = P.lvl_diag_ptr[L][q + 1];
-            const int b0 = P.lvl_panel_ptr[L][q], b1 = P.lvl_panel_ptr[L][q + 1];
-            hipLaunchKernelGGL(k_fwd_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, arena, P.f.p, P.w.p,
-                               P.off.p, P.first.p, P.vbase.p, V, Y, ps, n, nrhs);
-            if (b1 > b0)
-                hipLaunchKernelGGL(k_fwd_update<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.vbase.p, V, ps, nrhs);
+        for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
+            const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
+            hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, P.f.p, P.w.p,
+                               P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
-        for (int q = P.lvl_npanels[L] - 1; q >= 0; --q) {
-            const int ps = q * kPanel;
-            const int d0 = P.lvl_diag_ptr[L][q], d1 = P.lvl_diag_ptr[L][q + 1];
-            const int b0 = P.lvl_panel_ptr[L][q], b1 = P.lvl_panel_ptr[L][q + 1];
-            if (b1 > b0)
-                hipLaunchKernelGGL(k_bwd_partial<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.pat_ptr.p, P.pat_rows.p, Y, part, ps, n, nrhs);
-            hipLaunchKernelGGL(k_bwd_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, P.diag_tb.p + d0,
-                               arena, P.f.p, P.w.p, P.off.p, P.first.p, N.diag.p, Y, part, ps, n, nrhs);
+        ProfScope pb_(st, KP_BWD, 0.0, 0.0);
+        const int f0 = P.lvl_foot_ptr[L], f1 = P.lvl_foot_ptr[L + 1];
+        if (f1 > f0)
+            hipLaunchKernelGGL(k_bwd_foot_part<T>, dim3(f1 - f0), dim3(256), 0, st, P.foot_tasks.p + f0, arena, P.f.p,
+                               P.w.p, P.off.p, P.pat_ptr.p, P.pat_rows.p, Y, part, n, nrhs);
+        const int r0 = P.lvl_footr_ptr[L], r1 = P.lvl_footr_ptr[L + 1];
+        hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p, P.first.p,
+                           N.diag.p, Y, part, Z, n, nrhs);
+        for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
+            const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
+            hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p, P.w.p,
+                               P.off.p, P.first.p, Y, Z, q, n, nrhs);
         }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.perm.p, n, nrhs, d_x, ldx,
@@ -1184,9 +1317,10 @@ static void ensure_ws(Numeric &N, int nrhs) {
     const int n = std::max(N.n, 1);
     N.wsY.alloc((size_t)n * nrhs * esz);
     N.wsU.alloc((size_t)std::max<int64_t>(N.plan->vsum, 1) * nrhs * esz);
-    N.wsP.alloc((size_t)std::max(N.plan->max_panel_tasks, 1) * kPanel * nrhs * esz);
+    N.wsP.alloc((size_t)n * nrhs * esz);
+    N.wsQ.alloc((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * nrhs * esz);
     N.wsR.alloc((size_t)n * nrhs);
-    const int nb = (int)blocks_for(n, 256);
+    const int nb = (int)blocks_for((int64_t)n * N.A.lanes, 256);
     N.wsPart.alloc((size_t)nb * nrhs * 2);
     N.wsRel.alloc(nrhs);
     N.ws_nrhs = nrhs;
@@ -1196,9 +1330,20 @@ static void ensure_ws(Numeric &N, int nrhs) {
 static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const double *d_x, int ldx, hipStream_t st,
                      std::vector<double> &relres) {
     const int n = N.n;
-    const int nb = (int)blocks_for(n, 256);
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, N.A.rowptr.p, N.A.colidx.p, N.A.vals.p, n, nrhs, d_b, ldb,
-                       d_x, ldx, N.wsR.p, N.wsPart.p);
+    const int lanes = N.A.lanes;
+    const int nb = (int)blocks_for((int64_t)n * lanes, 256);
+    const Matrix &A = N.A;
+#define MG_RES(LN)                                                                                               \
+    hipLaunchKernelGGL(k_residual<LN>, dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, nrhs, d_b, \
+                       ldb, d_x, ldx, N.wsR.p, N.wsPart.p)
+    switch (lanes) {
+        case 4: MG_RES(4); break;
+        case 8: MG_RES(8); break;
+        case 16: MG_RES(16); break;
+        case 32: MG_RES(32); break;
+        default: MG_RES(64); break;
+    }
+#undef MG_RES
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, N.wsPart.p, nb, nrhs, N.wsRel.p);
     relres.resize(nrhs);
     MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
@@ -1223,13 +1368,18 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
     residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
-    double worst = *std::max_element(rel.begin(), rel.end());
+    auto worst_of = [](const std::vector<double> &v) {
+        double m = 0.0;
+        for (double x : v) m = (std::isnan(x) || std::isnan(m)) ? NAN : std::max(m, x);
+        return m;
+    };
+    double worst = worst_of(rel);
     int steps = 0;
     while (steps < max_refine && !(worst <= tol)) {
         once(N.wsR.p, N.n, 1);
         ++steps;
         residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
-        worst = *std::max_element(rel.begin(), rel.end());
+        worst = worst_of(rel);
     }
     if (rep) {
         rep->refinement_steps = steps;

@@ -88,9 +88,14 @@ void Matrix::build(int n_, const int *rowptr, const int *colidx) {
         }
     // row [lower entries from earlier rows (ascending), own upper entries] is sorted;
     // rows are filled in ascending r, so the lower part arrives ascending.
+    std::vector<int> dp(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int k = frp[i]; k < frp[i + 1]; ++k)
+            if (fci[k] == i) dp[i] = k;
     this->rowptr.upload(frp);
     this->colidx.upload(fci);
     src.upload(fsrc);
+    dpos.upload(dp);
     vals.alloc(std::max<int64_t>(nnz_full, 1));
     invdiag.alloc(std::max(n, 1));
     const double avg = n ? (double)nnz_full / n : 1.0;
