@@ -33,6 +33,8 @@ constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag /
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
 constexpr int kFootChunk = 1024;      // footer rows per partial product in the backward solve
+constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
+static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread each (256 threads)");
 
 // A device front is a chain of consecutive supernodes s, s+1, ..., each the
 // elimination-tree parent of the previous one (the last column of s has the
@@ -451,6 +453,69 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // strict lower part is stored transposed in the unused upper triangle of the
 // diagonal block (front row ps+m, column ps+k holds Li[k][m]) for the
 // row-block kernel and the solves.
+// 64x64 tiles staged through registers: thread (lane r, group g = tid / 64) holds
+// column c = g + 4j, j < 16, of rows r0 + r.  Loads are unconditional (addresses
+// clamped into the tile) and raw; the masks are applied when the values are stored
+// to LDS, so every load of a tile (and of all tiles a kernel needs) is in flight
+// before the first one is consumed.
+template <typename T>
+__device__ inline void tile_load(T (&v)[kPanel / 4], const T *__restrict__ M, int f, int r0, int c0, int nr, int nc) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    if (nr <= 0 || nc <= 0) {  // tile_store writes zeros
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) v[j] = T(0);
+        return;
+    }
+    const T *base = M + (int64_t)c0 * f + r0 + min(r, nr - 1);
+#pragma unroll
+    for (int j = 0; j < kPanel / 4; ++j) v[j] = base[(int64_t)min(g + 4 * j, nc - 1) * f];
+}
+// dst[r][c] = tile value for r < nr, c < nc, zero elsewhere
+template <typename T>
+__device__ inline void tile_store(T (*dst)[kPanel + 1], const T (&v)[kPanel / 4], int nr, int nc) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kPanel / 4; ++j) {
+        const int c = g + 4 * j;
+        dst[r][c] = (r < nr && c < nc) ? v[j] : T(0);
+    }
+}
+// The stored inverse of a diagonal block (nb x nb tile) as a full matrix U with
+// U[m][k] = Li[k][m]: strict upper part from the block (Li transposed), unit
+// diagonal below nb, zero elsewhere.
+template <typename T>
+__device__ inline void tile_store_inv(T (*dst)[kPanel + 1], const T (&v)[kPanel / 4], int nb) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kPanel / 4; ++j) {
+        const int c = g + 4 * j;
+        dst[r][c] = (r < c && c < nb) ? v[j] : (r == c && r < nb ? T(1) : T(0));
+    }
+}
+// Vector block: dst[i][rr] = src[rr * ld + i] for i < ni, rr < nr (zero-padded to
+// 64 x kRhsChunk); four values per thread.
+template <typename T>
+__device__ inline void vec_load(T (&v)[4], const T *__restrict__ src, int64_t ld, int ni, int nr) {
+    if (ni <= 0 || nr <= 0) {  // vec_store writes zeros
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = T(0);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int idx = threadIdx.x + 256 * j, i = idx & 63, rr = idx >> 6;
+        v[j] = src[(int64_t)min(rr, nr - 1) * ld + min(i, ni - 1)];
+    }
+}
+template <typename T>
+__device__ inline void vec_store(T (*dst)[kRhsChunk + 1], const T (&v)[4], int ni, int nr) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int idx = threadIdx.x + 256 * j, i = idx & 63, rr = idx >> 6;
+        dst[i][rr] = (i < ni && rr < nr) ? v[j] : T(0);
+    }
+}
+
 template <typename T>
 __device__ inline T lane_value(T v, int lane);
 template <>
@@ -465,90 +530,129 @@ __device__ inline double lane_value<double>(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// State of one 64x64 diagonal-block factorization held by a single wave.
+// State of one 64x64 diagonal-block factorization: four waves, lane r of wave v
+// holds row r of the block for columns 16v .. 16v+15 (a) and the same part of
+// row r of the block's inverse (li).
 struct PanelDiagCtx {
-    int r, nb, allow_reg, fc;
+    int r, v, nb, allow_reg;
     double thr, regv;
-    double *D;
-    unsigned char *flags;
+    double *dsh;          // pivots (LDS)
+    unsigned char *fsh;   // pivot flags (LDS)
 };
+constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel_diag
 
-// Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600):
-// pivot rule, scaled column K, rank-1 update of the rows below, and the same row
-// operation on the inverse.  K is a template argument so every broadcast is a
-// readlane with a constant lane and every row entry a fixed register.
+// Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600).
+// The wave owning column K applies the pivot rule and publishes the scaled column
+// (l_r = a[r][K] / d_K for r > K, zero for r <= K) and the pivot through LDS; lane K
+// of every wave publishes its part of row K of the inverse (final after step K-1).
+// Both are double-buffered by the parity of K, so one barrier per step suffices.
+// Then every wave updates its 16 columns, a[r][c] -= l_r d_K l_c (l_c = 0 for
+// c <= K, so eliminated columns are left as they are), and applies the same row
+// operation to the inverse, li[r][c] -= l_r li[K][c] (li[K][c] = 0 for c > K).
 template <typename T, int K>
-__device__ __forceinline__ void panel_diag_step(T (&a)[kPanel], T (&li)[kPanel], const PanelDiagCtx &x) {
-    const T akk = lane_value<T>(a[K], K);
-    double piv = (double)akk;
-    unsigned char fl = 0;
-    if (K < x.nb && !(piv > x.thr)) {
-        if (!x.allow_reg) fl |= 4;
-        if (piv < 0.0) fl |= 2;
-        fl |= 1;
-        piv = x.regv;
+__device__ __forceinline__ void panel_diag_step(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
+                                                T (*colk)[kPanel], T (*rowk)[kPanel], T *pvs) {
+    constexpr int kw = K / kDiagCols, kj = K % kDiagCols, buf = K & 1;
+    const int cb = x.v * kDiagCols;
+    if (x.r == K) {
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) rowk[buf][cb + j] = li[j];
     }
-    const T pv = (T)piv, pinv = T(1) / pv;
-    if (x.r == 0 && K < x.nb) {
-        x.D[x.fc + K] = (double)pv;
-        x.flags[x.fc + K] = fl;
+    if (x.v == kw) {
+        const T akk = lane_value<T>(a[kj], K);
+        double piv = (double)akk;
+        unsigned char fl = 0;
+        if (K < x.nb && !(piv > x.thr)) {
+            if (!x.allow_reg) fl |= 4;
+            if (piv < 0.0) fl |= 2;
+            fl |= 1;
+            piv = x.regv;
+        }
+        const T pv = (T)piv, pinv = T(1) / pv;
+        const bool below = x.r > K;
+        const T l = a[kj] * pinv;
+        a[kj] = below ? l : a[kj];
+        colk[buf][x.r] = below ? l : T(0);
+        if (x.r == 0) {
+            pvs[buf] = pv;
+            x.dsh[K] = (double)pv;  // written to D after the last step
+            x.fsh[K] = fl;
+        }
     }
-    const bool below = x.r > K;
-    const T l = a[K] * pinv;  // meaningful for lanes below K
-    const T lb = below ? l : T(0);
-    const T t = lb * pv;
-    a[K] = below ? l : a[K];
+    __syncthreads();
+    const T lb = colk[buf][x.r];
+    const T t = lb * pvs[buf];
+    T lc[kDiagCols], rk[kDiagCols];
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) {
+        lc[j] = colk[buf][cb + j];
+        rk[j] = rowk[buf][cb + j];
+    }
     // Entries above a lane's diagonal take updates too: they are never read.
 #pragma unroll
-    for (int c = K + 1; c < kPanel; ++c) a[c] -= t * lane_value<T>(l, c);
+    for (int j = 0; j < kDiagCols; ++j) a[j] -= t * lc[j];
 #pragma unroll
-    for (int c = 0; c <= K; ++c) li[c] -= lb * lane_value<T>(li[c], K);
-    // Materialize every row entry at the end of the step; otherwise the compiler
-    // defers updates to their first use and keeps broadcast values alive across
-    // steps in scalar registers.
+    for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
+    // Materialize every row entry at the end of the step (keeps the compiler from
+    // deferring updates across steps).
 #pragma unroll
-    for (int c = 0; c < kPanel; ++c) asm volatile("" : "+v"(a[c]), "+v"(li[c]));
+    for (int j = 0; j < kDiagCols; ++j) asm volatile("" : "+v"(a[j]), "+v"(li[j]));
 }
 
 template <typename T, int... K>
-__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...>, T (&a)[kPanel], T (&li)[kPanel],
-                                                 const PanelDiagCtx &x) {
-    (panel_diag_step<T, K>(a, li, x), ...);
+__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...>, T (&a)[kDiagCols],
+                                                 T (&li)[kDiagCols], const PanelDiagCtx &x, T (*colk)[kPanel],
+                                                 T (*rowk)[kPanel], T *pvs) {
+    (panel_diag_step<T, K>(a, li, x, colk, rowk, pvs), ...);
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
-                                                   const int *__restrict__ F, const int *__restrict__ W,
-                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                   int ps, const double *__restrict__ scale, double tol, double regf,
-                                                   int allow_reg, double *__restrict__ D,
-                                                   unsigned char *__restrict__ flags) {
-    // One wave; lane r keeps row r of the block (a) and of its inverse (li) in registers.
+__global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
+                                                    const int *__restrict__ F, const int *__restrict__ W,
+                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                                    int ps, const double *__restrict__ scale, double tol, double regf,
+                                                    int allow_reg, double *__restrict__ D,
+                                                    unsigned char *__restrict__ flags) {
     // Rows/columns past nb are padded with the identity (pivot 1, never reported).
+    __shared__ __align__(16) T colk[2][kPanel];
+    __shared__ __align__(16) T rowk[2][kPanel];
+    __shared__ T pvs[2];
+    __shared__ double dsh[kPanel];
+    __shared__ unsigned char fsh[kPanel];
     const int s = nodes[blockIdx.x];
     const int f = F[s], w = W[s];
     if (w <= ps) return;
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     T *M = arena + off[s];
-    const int r = threadIdx.x;
+    const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double sc = scale[0];
-    PanelDiagCtx x{r, nb, allow_reg, firstcol[s] + ps, tol * sc, regf * sc, D, flags};
-    T a[kPanel], li[kPanel];
+    PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
+    T a[kDiagCols], li[kDiagCols];
 #pragma unroll
-    for (int c = 0; c < kPanel; ++c) {
-        a[c] = (r < nb && c < nb && r >= c) ? M[(int64_t)(ps + c) * f + ps + r] : (r == c ? T(1) : T(0));
-        li[c] = r == c ? T(1) : T(0);
+    for (int j = 0; j < kDiagCols; ++j) {
+        const int c = v * kDiagCols + j;
+        a[j] = (r < nb && c < nb && r >= c) ? M[(int64_t)(ps + c) * f + ps + r] : (r == c ? T(1) : T(0));
+        li[j] = r == c ? T(1) : T(0);
     }
-    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x);
+    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x, colk, rowk, pvs);
+    __syncthreads();
+    if (threadIdx.x < nb) {
+        const int fc = firstcol[s] + ps;
+        D[fc + threadIdx.x] = dsh[threadIdx.x];
+        flags[fc + threadIdx.x] = fsh[threadIdx.x];
+    }
     if (r < nb) {
 #pragma unroll
-        for (int c = 0; c < kPanel; ++c) {
-            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[c];   // L below the diagonal
+        for (int j = 0; j < kDiagCols; ++j) {
+            const int c = v * kDiagCols + j;
+            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[j];   // L below the diagonal
             else if (c == r) M[(int64_t)(ps + c) * f + ps + r] = T(1);
         }
 #pragma unroll
-        for (int m = 0; m < kPanel; ++m)
-            if (m < r) M[(int64_t)(ps + r) * f + ps + m] = li[m];  // Li[r][m] at row ps+m, column ps+r
+        for (int j = 0; j < kDiagCols; ++j) {
+            const int c = v * kDiagCols + j;
+            if (c < r) M[(int64_t)(ps + r) * f + ps + c] = li[j];  // Li[r][c] at row ps+c, column ps+r
+        }
     }
 }
 
@@ -560,7 +664,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                     int ps, const double *__restrict__ D) {
-    __shared__ T Bm[kPanel][kPanel + 1];  // Bm[m][k], m <= k
+    __shared__ T Bm[kPanel][kPanel + 1];  // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0
     __shared__ T X[kTile][kPanel + 1];
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
@@ -570,34 +674,37 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     const int nr = min(kTile, f - r0);
     if (nr <= 0) return;
     T *M = arena + off[s];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
     const int fc = firstcol[s] + ps;
     {
-        const int lane = tid & 63, wv = tid >> 6;
-        for (int k = wv; k < nb; k += 4) {
-            const T dk = (T)D[fc + k];
-            if (lane < k) Bm[lane][k] = M[(int64_t)(ps + k) * f + ps + lane] / dk;
-            else if (lane == k) Bm[lane][k] = T(1) / dk;
-            if (lane < nr) X[lane][k] = M[(int64_t)(ps + k) * f + r0 + lane];
+        T ti[kPanel / 4], tx[kPanel / 4], td[kPanel / 4];
+        tile_load<T>(ti, M, f, ps, ps, nb, nb);
+        tile_load<T>(tx, M, f, r0, ps, nr, nb);
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const int k = g + 4 * j;
+            td[j] = (T)D[fc + min(k, nb - 1)];
         }
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const int k = g + 4 * j;
+            const T rdk = T(1) / td[j];
+            Bm[lane][k] = (lane < k && k < nb) ? ti[j] * rdk : (lane == k && k < nb ? rdk : T(0));
+        }
+        tile_store<T>(X, tx, nr, nb);
     }
     __syncthreads();
     // thread -> (row r, columns k = c0, c0+4, ...)
-    const int r = tid & 63, c0 = tid >> 6;
+    const int r = lane, c0 = g;
     T out[kPanel / 4];
 #pragma unroll
     for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
-    if (r < nr) {
-        for (int m = 0; m < nb; ++m) {
-            const T xm = X[r][m];
+#pragma unroll 4
+    for (int m = 0; m < kPanel; ++m) {
+        const T xm = X[r][m];
 #pragma unroll
-            for (int q = 0; q < kPanel / 4; ++q) {
-                const int k = c0 + 4 * q;
-                if (k < nb && m <= k) out[q] += xm * Bm[m][k];
-            }
-        }
+        for (int q = 0; q < kPanel / 4; ++q) out[q] += xm * Bm[m][c0 + 4 * q];
     }
-    __syncthreads();
     if (r < nr)
 #pragma unroll
         for (int q = 0; q < kPanel / 4; ++q) {
@@ -617,6 +724,7 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
                                                int q, const double *__restrict__ D) {
     __shared__ T As[kPanel][kTile + 1];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ T Bs[kPanel][kTile + 1];  // Bs[k][r] = L[j0+r, kc+k]
+    __shared__ T Ds[kBlock];             // d_k of the step's K columns
     const int4 tk = tasks[blockIdx.x];
     const int s = tk.x;
     const int f = F[s], w = W[s];
@@ -626,26 +734,41 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
     const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
     const int i0 = base + tk.y * kTile, j0 = base + tk.z * kTile;
     T *M = arena + off[s];
-    const int tid = threadIdx.x;
     const double *Dp = D + firstcol[s];
+    // loader mapping: lane r = row of the tile, wave kq = k offset (k = kq + 4j)
+    const int r = threadIdx.x & 63, kq = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool ia = i0 + r < f, ib = j0 + r < f;
+    const int rowa = min(i0 + r, f - 1), rowb = min(j0 + r, f - 1);
+    Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
+    T ra[kPanel / 4], rb[kPanel / 4];
+    // Chunk kc of K into registers; all loads are unconditional (addresses clamped
+    // into the front) and issued before any is consumed, so the next chunk is in
+    // flight while the current one runs on the MFMA units.
+    auto fetch = [&](int kc) {
+        const int nk = min(kPanel, ke - kc);
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const int k = kq + 4 * j;
+            const T *col = M + (int64_t)(kc + min(k, nk - 1)) * f;
+            ra[j] = col[rowa];
+            rb[j] = col[rowb];
+        }
+    };
     typename MfmaAcc<T>::type acc[2][2];
     schur_acc_zero<T>(acc);
+    fetch(kb);
     for (int kc = kb; kc < ke; kc += kPanel) {
         const int nk = min(kPanel, ke - kc);
-        for (int idx = tid; idx < kPanel * kTile; idx += 256) {
-            const int r = idx % kTile, k = idx / kTile;
-            T av = T(0), bv = T(0);
-            if (k < nk) {
-                const T *col = M + (int64_t)(kc + k) * f;
-                if (i0 + r < f) av = col[i0 + r] * (T)Dp[kc + k];
-                if (j0 + r < f) bv = col[j0 + r];
-            }
-            As[k][r] = av;
-            Bs[k][r] = bv;
+        __syncthreads();  // the previous chunk's MFMA reads are done
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const int k = kq + 4 * j;
+            As[k][r] = (k < nk && ia) ? ra[j] * Ds[min(kc - kb + k, kBlock - 1)] : T(0);
+            Bs[k][r] = (k < nk && ib) ? rb[j] : T(0);
         }
         __syncthreads();
-        schur_acc_tile<T>(As, Bs, (nk + 3) & ~3, acc);
-        __syncthreads();
+        if (kc + kPanel < ke) fetch(kc + kPanel);
+        schur_acc_tile<T>(As, Bs, acc);
     }
     schur_store_tile<T>(acc, M, f, i0, j0, cend);
 }
@@ -725,7 +848,6 @@ __global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ 
 // division by D) and walks the panels from the last one down, right-looking
 // (k_bwd_panel).  Y holds y after the
 // forward pass and x after the backward pass (elimination order, n x nrhs).
-constexpr int kRhsChunk = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks, const int *__restrict__ F,
@@ -751,15 +873,6 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     }
 }
 
-// Stages dst[r][c] = M[(c0 + c) * f + r0 + r] for r < nr, c < nc (zero elsewhere).
-template <typename T>
-__device__ inline void stage64(T (*dst)[kPanel + 1], const T *__restrict__ M, int f, int r0, int c0, int nr, int nc) {
-    for (int idx = threadIdx.x; idx < kPanel * kPanel; idx += 256) {
-        const int r = idx % kPanel, c = idx / kPanel;
-        dst[r][c] = (r < nr && c < nc) ? M[(int64_t)(c0 + c) * f + r0 + r] : T(0);
-    }
-}
-
 // Forward panel (columns [ps, pe) of a front): every workgroup forms y = Li v[ps:pe)
 // from the stored 64x64 inverse; task rb = -1 writes y to Y, task rb >= 0 subtracts
 // L[rows, ps:pe) y from its 64-row block of V below the panel.
@@ -780,32 +893,67 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
     T *Vg = V + vbase[g] * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
     const int r0 = pe + (rb < 0 ? 0 : rb) * kTile, nrow = rb < 0 ? 0 : min(kTile, f - r0);
-    stage64<T>(inv, M, f, ps, ps, nb, nb);          // inv[mm][kk] = Li[kk][mm], mm < kk
-    if (rb >= 0) stage64<T>(blk, M, f, r0, ps, nrow, nb);  // blk[i][k] = L[r0+i][ps+k]
+    {
+        T ti[kPanel / 4], tb[kPanel / 4];
+        tile_load<T>(ti, M, f, ps, ps, nb, nb);
+        tile_load<T>(tb, M, f, r0, ps, nrow, nb);  // zero when rb < 0
+        tile_store_inv<T>(inv, ti, nb);            // inv[m][k] = Li[k][m]
+        tile_store<T>(blk, tb, nrow, nb);          // blk[i][k] = L[r0+i][ps+k]
+    }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        for (int rr = grp; rr < nr; rr += 4)
-            if (lane < nb) vv[lane][rr] = Vg[(int64_t)(c0 + rr) * f + ps + lane];
+        {
+            T tv[4];
+            vec_load<T>(tv, Vg + (int64_t)c0 * f + ps, f, nb, nr);
+            __syncthreads();
+            vec_store<T>(vv, tv, nb, nr);
+        }
         __syncthreads();
-        for (int rr = grp; rr < nr; rr += 4)
-            if (lane < nb) {
-                T acc = vv[lane][rr];
-                for (int mm = 0; mm < lane; ++mm) acc += inv[mm][lane] * vv[mm][rr];
-                yv[lane][rr] = acc;
-            }
+        for (int rr = grp; rr < nr; rr += 4) {
+            T acc = T(0);
+#pragma unroll 16
+            for (int m = 0; m < kPanel; ++m) acc += inv[m][lane] * vv[m][rr];
+            yv[lane][rr] = acc;
+        }
         __syncthreads();
         if (rb < 0) {
             for (int rr = grp; rr < nr; rr += 4)
                 if (lane < nb) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + lane] = yv[lane][rr];
         } else if (lane < nrow) {
-            for (int rr = grp; rr < nr; rr += 4) {
-                T acc = T(0);
-                for (int k = 0; k < nb; ++k) acc += blk[lane][k] * yv[k][rr];
-                Vg[(int64_t)(c0 + rr) * f + r0 + lane] -= acc;
+            T old[kRhsChunk / 4];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j) {
+                const int rr = grp + 4 * j;
+                old[j] = rr < nr ? Vg[(int64_t)(c0 + rr) * f + r0 + lane] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j) {
+                const int rr = grp + 4 * j;
+                if (rr < nr) {
+                    T acc = T(0);
+#pragma unroll 16
+                    for (int k = 0; k < kPanel; ++k) acc += blk[lane][k] * yv[k][rr];
+                    Vg[(int64_t)(c0 + rr) * f + r0 + lane] = old[j] - acc;
+                }
             }
         }
-        __syncthreads();
     }
+}
+
+// Backward footer, gather: the x values of every front's footer rows (final:
+// ancestors are solved first) into the front's vector, V[g][rr][w + a] = x[rows[w + a]].
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_gather(const int2 *__restrict__ tasks, const int *__restrict__ F,
+                                                    const int *__restrict__ W, const int *__restrict__ pat_ptr,
+                                                    const int *__restrict__ pat_rows, const int64_t *__restrict__ vbase,
+                                                    const T *__restrict__ Y, T *__restrict__ V, int n, int nrhs) {
+    const int2 tk = tasks[blockIdx.x];
+    const int g = tk.x, f = F[g], w = W[g];
+    const int i = tk.y * 256 + threadIdx.x;
+    if (i < w || i >= f) return;
+    const int row = pat_rows[pat_ptr[g] + i];
+    T *Vg = V + vbase[g] * nrhs;
+    for (int r = 0; r < nrhs; ++r) Vg[(int64_t)r * f + i] = Y[(int64_t)r * n + row];
 }
 
 // Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
@@ -813,9 +961,9 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ tasks, const T *__restrict__ arena,
                                                        const int *__restrict__ F, const int *__restrict__ W,
-                                                       const int64_t *__restrict__ off, const int *__restrict__ pat_ptr,
-                                                       const int *__restrict__ pat_rows, const T *__restrict__ Y,
-                                                       T *__restrict__ part, int n, int nrhs) {
+                                                       const int64_t *__restrict__ off,
+                                                       const int64_t *__restrict__ vbase, const T *__restrict__ V,
+                                                       T *__restrict__ part, int nrhs) {
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T xs[kTile][kRhsChunk + 1];
     const int4 tk = tasks[blockIdx.x];
@@ -823,27 +971,33 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
     const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
     const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
     const T *M = arena + off[g];
-    const int *rows = pat_rows + pat_ptr[g];
+    const T *Vg = V + vbase[g] * nrhs;
     const int tid = threadIdx.x, col = tid & 63, grp = tid >> 6;
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
         T acc[kRhsChunk / 4];
 #pragma unroll
         for (int j = 0; j < kRhsChunk / 4; ++j) acc[j] = T(0);
+        T tb[kPanel / 4], tx[4];
+        tile_load<T>(tb, M, f, ibeg, cb0, iend - ibeg, ncol);
+        vec_load<T>(tx, Vg + (int64_t)c0 * f + ibeg, f, iend - ibeg, nr);
         for (int i0 = ibeg; i0 < iend; i0 += kTile) {
-            const int ni = min(kTile, iend - i0);
             __syncthreads();
-            stage64<T>(blk, M, f, i0, cb0, ni, ncol);  // blk[i][c] = L[i0+i][cb0+c]
-            for (int idx = tid; idx < kTile * nr; idx += 256) {
-                const int i = idx % kTile, rr = idx / kTile;
-                xs[i][rr] = i < ni ? Y[(int64_t)(c0 + rr) * n + rows[i0 + i]] : T(0);
+            tile_store<T>(blk, tb, iend - i0, ncol);  // blk[i][c] = L[i0+i][cb0+c]
+            vec_store<T>(xs, tx, iend - i0, nr);      // xs[i][rr] = x[row i0+i][c0+rr]
+            __syncthreads();
+            const int nx = i0 + kTile;
+            if (nx < iend) {  // next block in flight during this one's products
+                tile_load<T>(tb, M, f, nx, cb0, iend - nx, ncol);
+                vec_load<T>(tx, Vg + (int64_t)c0 * f + nx, f, iend - nx, nr);
             }
-            __syncthreads();
 #pragma unroll
             for (int j = 0; j < kRhsChunk / 4; ++j) {
                 const int rr = grp + 4 * j;
-                if (rr < nr)
-                    for (int i = 0; i < ni; ++i) acc[j] += blk[i][col] * xs[i][rr];
+                if (rr < nr) {
+#pragma unroll 16
+                    for (int i = 0; i < kTile; ++i) acc[j] += blk[i][col] * xs[i][rr];
+                }
             }
         }
 #pragma unroll
@@ -851,6 +1005,7 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
             const int rr = grp + 4 * j;
             if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = acc[j];
         }
+        __syncthreads();
     }
 }
 
@@ -890,102 +1045,149 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
     const int fc = firstcol[g];
     const T *M = arena + off[g];
     const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
-    stage64<T>(inv, M, f, ps, ps, nb, nb);  // inv[mm][kk] = Li[kk][mm], mm < kk
-    if (cb < q) stage64<T>(blk, M, f, ps, cb * kPanel, nb, kPanel);  // blk[k][c] = L[ps+k][64cb+c]
+    {
+        T ti[kPanel / 4], tb[kPanel / 4];
+        tile_load<T>(ti, M, f, ps, ps, nb, nb);
+        tile_load<T>(tb, M, f, ps, cb * kPanel, cb < q ? nb : 0, kPanel);
+        tile_store_inv<T>(inv, ti, nb);                     // inv[m][k] = Li[k][m]
+        tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);    // blk[k][c] = L[ps+k][64cb+c]
+    }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        for (int rr = grp; rr < nr; rr += 4)
-            if (lane < nb) zz[lane][rr] = Z[(int64_t)(c0 + rr) * n + fc + ps + lane];
+        {
+            T tz[4];
+            vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
+            __syncthreads();
+            vec_store<T>(zz, tz, nb, nr);
+        }
         __syncthreads();
-        for (int rr = grp; rr < nr; rr += 4)
-            if (lane < nb) {
-                T acc = zz[lane][rr];
-                for (int kk = lane + 1; kk < nb; ++kk) acc += inv[lane][kk] * zz[kk][rr];
-                xv[lane][rr] = acc;
-            }
+        for (int rr = grp; rr < nr; rr += 4) {
+            T acc = T(0);
+#pragma unroll 16
+            for (int k = 0; k < kPanel; ++k) acc += inv[lane][k] * zz[k][rr];
+            xv[lane][rr] = acc;
+        }
         __syncthreads();
         if (cb == q) {
             for (int rr = grp; rr < nr; rr += 4)
                 if (lane < nb) Y[(int64_t)(c0 + rr) * n + fc + ps + lane] = xv[lane][rr];
         } else {
-            for (int rr = grp; rr < nr; rr += 4) {
-                T acc = T(0);
-                for (int k = 0; k < nb; ++k) acc += blk[k][lane] * xv[k][rr];
-                Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] -= acc;
+            T old[kRhsChunk / 4];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j) {
+                const int rr = grp + 4 * j;
+                old[j] = rr < nr ? Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j) {
+                const int rr = grp + 4 * j;
+                if (rr < nr) {
+                    T acc = T(0);
+#pragma unroll 16
+                    for (int k = 0; k < kPanel; ++k) acc += blk[k][lane] * xv[k][rr];
+                    Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] = old[j] - acc;
+                }
             }
         }
-        __syncthreads();
     }
 }
 
 // ---- refinement helpers ------------------------------------------------------
 
-// R = B - A X over the full CSR (LANES lanes per row, fixed xor-tree per row),
-// and per-column block partial sums of R^2 and B^2 for the relative residual:
-// part[(col * gridDim.x + block) * 2 + {0,1}].
+// R = B - A X over the full CSR (LANES lanes per row, fixed xor-tree per row, all
+// right-hand sides of a chunk of kResCols per pass over A), and per-block partial
+// sums of R^2 and B^2 for the relative residual: part[(col * gridDim.x + block) * 2 + {0,1}].
+// One row per LANES-lane group (grid = n * LANES / 256 blocks), so the partial sums and
+// their order depend only on n and LANES.
+constexpr int kResCols = 8;
+static inline int residual_blocks(int n, int lanes) { return (int)(((int64_t)n * lanes + 255) / 256); }
 template <int LANES>
 __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const double *__restrict__ v, int n, int nrhs,
                                                   const double *__restrict__ B, int ldb, const double *__restrict__ X,
                                                   int ldx, double *__restrict__ R, double *__restrict__ part) {
-    __shared__ double sr[256], sb[256];
-    const int row = blockIdx.x * (256 / LANES) + threadIdx.x / LANES, sub = threadIdx.x % LANES;
-    for (int c = 0; c < nrhs; ++c) {
-        double rr = 0.0, bb = 0.0;
-        if (row < n) {
-            const double *xc = X + (int64_t)c * ldx;
-            double acc = 0.0;
-            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) acc += v[k] * xc[ci[k]];
-            for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+    __shared__ double sr[kResCols][256 / 64], sb[kResCols][256 / 64];
+    constexpr int RPB = 256 / LANES;  // rows per block pass
+    const int sub = threadIdx.x % LANES;
+    for (int c0 = 0; c0 < nrhs; c0 += kResCols) {
+        const int nc = min(kResCols, nrhs - c0);
+        double rr[kResCols], bb[kResCols];
+#pragma unroll
+        for (int c = 0; c < kResCols; ++c) rr[c] = bb[c] = 0.0;
+        for (int row = blockIdx.x * RPB + threadIdx.x / LANES; row < n; row += gridDim.x * RPB) {
+            double acc[kResCols];
+#pragma unroll
+            for (int c = 0; c < kResCols; ++c) acc[c] = 0.0;
+#pragma unroll 2
+            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
+                const double a = v[k];
+                const int64_t col = ci[k];
+                // columns past nc re-read the last one (unconditional loads issue together)
+#pragma unroll
+                for (int c = 0; c < kResCols; ++c) acc[c] += a * X[(int64_t)(c0 + min(c, nc - 1)) * ldx + col];
+            }
+#pragma unroll
+            for (int c = 0; c < kResCols; ++c)
+                for (int o = LANES / 2; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, LANES);
             if (sub == 0) {
-                const double b = B[(int64_t)c * ldb + row];
-                const double res = b - acc;
-                R[(int64_t)c * n + row] = res;
-                rr = res * res;
-                bb = b * b;
+#pragma unroll
+                for (int c = 0; c < kResCols; ++c)
+                    if (c < nc) {
+                        const double b = B[(int64_t)(c0 + c) * ldb + row];
+                        const double res = b - acc[c];
+                        R[(int64_t)(c0 + c) * n + row] = res;
+                        rr[c] += res * res;
+                        bb[c] += b * b;
+                    }
             }
         }
-        sr[threadIdx.x] = rr;
-        sb[threadIdx.x] = bb;
+        // block reduction: wave xor-tree, then the 4 waves in fixed order
+#pragma unroll
+        for (int c = 0; c < kResCols; ++c)
+            for (int o = 32; o > 0; o >>= 1) {
+                rr[c] += __shfl_xor(rr[c], o, 64);
+                bb[c] += __shfl_xor(bb[c], o, 64);
+            }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < kResCols; ++c) {
+                sr[c][wv] = rr[c];
+                sb[c][wv] = bb[c];
+            }
         __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) {
-                sr[threadIdx.x] += sr[threadIdx.x + h];
-                sb[threadIdx.x] += sb[threadIdx.x + h];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            part[((int64_t)c * gridDim.x + blockIdx.x) * 2 + 0] = sr[0];
-            part[((int64_t)c * gridDim.x + blockIdx.x) * 2 + 1] = sb[0];
+        if (threadIdx.x < nc) {
+            const int c = threadIdx.x;
+            part[((int64_t)(c0 + c) * gridDim.x + blockIdx.x) * 2 + 0] = (sr[c][0] + sr[c][1]) + (sr[c][2] + sr[c][3]);
+            part[((int64_t)(c0 + c) * gridDim.x + blockIdx.x) * 2 + 1] = (sb[c][0] + sb[c][1]) + (sb[c][2] + sb[c][3]);
         }
         __syncthreads();
     }
 }
 
-__global__ void k_residual_final(const double *__restrict__ part, int nblocks, int nrhs, double *__restrict__ relres) {
+// relres[c] = sqrt(sum R^2) / sqrt(sum B^2) per column (one workgroup per column).
+__global__ __launch_bounds__(256) void k_residual_final(const double *__restrict__ part, int nblocks,
+                                                        double *__restrict__ relres) {
     __shared__ double sr[256], sb[256];
-    for (int c = 0; c < nrhs; ++c) {
-        double rr = 0.0, bb = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) {
-            rr += part[((int64_t)c * nblocks + b) * 2 + 0];
-            bb += part[((int64_t)c * nblocks + b) * 2 + 1];
-        }
-        sr[threadIdx.x] = rr;
-        sb[threadIdx.x] = bb;
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if ((int)threadIdx.x < h) {
-                sr[threadIdx.x] += sr[threadIdx.x + h];
-                sb[threadIdx.x] += sb[threadIdx.x + h];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const double bn = sqrt(sb[0]);
-            relres[c] = sqrt(sr[0]) / (bn > 0.0 ? bn : 1.0);
+    const int c = blockIdx.x;
+    double rr = 0.0, bb = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        rr += part[((int64_t)c * nblocks + b) * 2 + 0];
+        bb += part[((int64_t)c * nblocks + b) * 2 + 1];
+    }
+    sr[threadIdx.x] = rr;
+    sb[threadIdx.x] = bb;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sr[threadIdx.x] += sr[threadIdx.x + h];
+            sb[threadIdx.x] += sb[threadIdx.x + h];
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double bn = sqrt(sb[0]);
+        relres[c] = sqrt(sr[0]) / (bn > 0.0 ? bn : 1.0);
     }
 }
 
@@ -1173,7 +1375,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             const int d0 = P.lvl_diag_ptr[L][q], d1 = P.lvl_diag_ptr[L][q + 1];
             if (d1 > d0) {
                 ProfScope ps_(st, KP_PDIAG, 0.0, 0.0);
-                hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(64), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
+                hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.scale.p, opts.pivot_tolerance,
                                    opts.diagonal_regularization, opts.allow_diagonal_regularization, N.diag.p,
                                    N.flags.p);
@@ -1294,9 +1496,13 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     for (int L = P.nlevels - 1; L >= 0; --L) {
         ProfScope pb_(st, KP_BWD, 0.0, 0.0);
         const int f0 = P.lvl_foot_ptr[L], f1 = P.lvl_foot_ptr[L + 1];
-        if (f1 > f0)
+        if (f1 > f0) {
+            const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
+            hipLaunchKernelGGL(k_bwd_gather<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
+                               P.pat_ptr.p, P.pat_rows.p, P.vbase.p, Y, V, n, nrhs);
             hipLaunchKernelGGL(k_bwd_foot_part<T>, dim3(f1 - f0), dim3(256), 0, st, P.foot_tasks.p + f0, arena, P.f.p,
-                               P.w.p, P.off.p, P.pat_ptr.p, P.pat_rows.p, Y, part, n, nrhs);
+                               P.w.p, P.off.p, P.vbase.p, V, part, nrhs);
+        }
         const int r0 = P.lvl_footr_ptr[L], r1 = P.lvl_footr_ptr[L + 1];
         hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p, P.first.p,
                            N.diag.p, Y, part, Z, n, nrhs);
@@ -1320,8 +1526,7 @@ static void ensure_ws(Numeric &N, int nrhs) {
     N.wsP.alloc((size_t)n * nrhs * esz);
     N.wsQ.alloc((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * nrhs * esz);
     N.wsR.alloc((size_t)n * nrhs);
-    const int nb = (int)blocks_for((int64_t)n * N.A.lanes, 256);
-    N.wsPart.alloc((size_t)nb * nrhs * 2);
+    N.wsPart.alloc((size_t)residual_blocks(n, N.A.lanes) * nrhs * 2);
     N.wsRel.alloc(nrhs);
     N.ws_nrhs = nrhs;
 }
@@ -1331,7 +1536,7 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
                      std::vector<double> &relres) {
     const int n = N.n;
     const int lanes = N.A.lanes;
-    const int nb = (int)blocks_for((int64_t)n * lanes, 256);
+    const int nb = residual_blocks(n, lanes);
     const Matrix &A = N.A;
 #define MG_RES(LN)                                                                                               \
     hipLaunchKernelGGL(k_residual<LN>, dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, nrhs, d_b, \
@@ -1344,7 +1549,7 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
         default: MG_RES(64); break;
     }
 #undef MG_RES
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, N.wsPart.p, nb, nrhs, N.wsRel.p);
+    hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, N.wsPart.p, nb, N.wsRel.p);
     relres.resize(nrhs);
     MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));

@@ -47,40 +47,81 @@ __device__ inline void schur_acc_zero(typename MfmaAcc<T>::type (&acc)[2][2]) {
             for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
 }
 
-// acc += As^T-panel x Bs-panel over nk (multiple of 4) rows of the LDS chunks.
+// acc += As^T-panel x Bs-panel over the 64 rows of the LDS chunks (rows past the
+// chunk's length hold zeros).  The k-steps go in batches whose operands are read
+// from LDS while the previous batch runs on the matrix cores (software pipeline;
+// the scheduling barriers keep the compiler from sinking the reads to their uses).
 template <typename T>
-__device__ inline void schur_acc_tile(const T (*As)[65], const T (*Bs)[65], int nk,
-                                      typename MfmaAcc<T>::type (&acc)[2][2]) {
+__device__ inline void schur_read_ops(const T (*As)[65], const T (*Bs)[65], int k0, T (&colv)[2], T (&rowv)[2]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
-    const int l16 = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < nk; k0 += 4) {
-        const int k = k0 + lk;
-        T colv[2], rowv[2];
-        for (int b = 0; b < 2; ++b) colv[b] = Bs[k][cq + 16 * b + l16];
-        for (int a = 0; a < 2; ++a) rowv[a] = As[k][rq + 16 * a + l16];
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::step(colv[b], rowv[a], acc[a][b]);
+    const int k = k0 + (lane >> 4), l16 = lane & 15;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) colv[b] = Bs[k][cq + 16 * b + l16];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) rowv[a] = As[k][rq + 16 * a + l16];
+}
+
+template <typename T>
+__device__ inline void schur_acc_tile(const T (*As)[65], const T (*Bs)[65],
+                                      typename MfmaAcc<T>::type (&acc)[2][2]) {
+    constexpr int kSteps = 4;                 // k-steps (of 4) per batch
+    constexpr int kBatches = 64 / (4 * kSteps);
+    T colv[2][kSteps][2], rowv[2][kSteps][2];  // double-buffered operand batches
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) schur_read_ops<T>(As, Bs, 4 * s, colv[0][s], rowv[0][s]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int bt = 0; bt < kBatches; ++bt) {
+        const int cur = bt & 1;
+        if (bt + 1 < kBatches) {
+#pragma unroll
+            for (int s = 0; s < kSteps; ++s)
+                schur_read_ops<T>(As, Bs, 4 * (kSteps * (bt + 1) + s), colv[cur ^ 1][s], rowv[cur ^ 1][s]);
+        }
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = MfmaAcc<T>::step(colv[cur][s][b], rowv[cur][s][a], acc[a][b]);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // M[j*f + i] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
-// acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].
+// acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].  All 16 reads
+// are issued before the first write so their latencies overlap.
 template <typename T>
 __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2][2], T *__restrict__ M, int f,
                                         int i0, int j0, int jend) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = lane >> 4;
-    for (int a = 0; a < 2; ++a) {
-        const int i = i0 + rq + 16 * a + l16;
-        if (i >= f) continue;
+    T old[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
+                const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (j <= i && j < jend) M[(int64_t)j * f + i] -= acc[a][b][q];
+                const int ic = min(i, f - 1), jc = min(j, ic);  // clamped: unconditional load
+                const T x = M[(int64_t)jc * f + ic];
+                old[a][b][q] = (i < f && j <= i && j < jend) ? x : T(0);
             }
-    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + rq + 16 * a + l16;
+                const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
+                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[a][b][q] - acc[a][b][q];
+            }
 }
 
 }  // namespace mg

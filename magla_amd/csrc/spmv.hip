@@ -99,7 +99,9 @@ void Matrix::build(int n_, const int *rowptr, const int *colidx) {
     vals.alloc(std::max<int64_t>(nnz_full, 1));
     invdiag.alloc(std::max(n, 1));
     const double avg = n ? (double)nnz_full / n : 1.0;
-    lanes = avg > 48 ? 64 : avg > 24 ? 32 : avg > 12 ? 16 : avg > 6 ? 8 : 4;
+    // About ten entries per lane: a wave then works on several rows at once, which
+    // keeps enough independent loads in flight (one row per wave is latency-bound).
+    lanes = avg > 512 ? 64 : avg > 256 ? 32 : avg > 128 ? 16 : avg > 24 ? 8 : 4;
 }
 
 void Matrix::set_values_dev(const double *d_values, hipStream_t st) {
