@@ -2,9 +2,9 @@
  * magla.h -- C ABI of the MI355X (gfx950) UCHOL / CGGP library, libmagla.so.
  *
  * Drop-in boundary for MAGLA's CAESolvers (C#/.NET 8).  A managed wrapper
- * binds these with [LibraryImport] (blittable pointers, opaque handles); the
- * binding is shown in INTEGRATION.md.  Each entry point names the reference
- * member it replaces (paths relative to the MAGLA repository root).
+ * binds these with [LibraryImport] (blittable pointers, opaque handles).
+ * Each entry point names the reference member it replaces (paths relative to
+ * the MAGLA repository root).
  *
  * Conventions
  *   - The caller owns every input array; the library copies what it keeps.
