@@ -39,7 +39,8 @@ typedef enum {
 } mg_status;
 
 typedef enum { MG_ORDER_NATURAL = 0, MG_ORDER_AMD = 1 } mg_ordering; /* FillReducingOrdering (UtduSymbolicFactorization.cs:8-24) */
-typedef enum { MG_FP64 = 0, MG_FP32 = 1 } mg_precision;              /* new: factor precision */
+typedef enum { MG_FP64 = 0, MG_FP32 = 1, MG_BF16 = 2 } mg_precision; /* new: factor precision (FP64/FP32);
+                                                                         BF16 only for the learned preconditioner */
 
 /* UtduSolverOptions (UtduSymbolicFactorization.cs:31-98) + device fields. */
 typedef struct {
@@ -53,6 +54,10 @@ typedef struct {
     double pivot_tolerance;               /* default 1e-13 */
     double diagonal_regularization;       /* default 1e-10 */
     int64_t parallel_update_threshold;    /* default 1<<20 (kept for ABI parity; unused on the GPU) */
+    int32_t device_graph;                 /* new: capture the factorization launches in a hipGraph on the
+                                             first mg_factorize_dev of a numeric handle and replay it on
+                                             refactorization (default 0) */
+    int32_t reserved;
 } mg_options;
 
 typedef struct mg_symbolic mg_symbolic;   /* UtduSymbolicFactorization */
@@ -177,6 +182,64 @@ int mg_pcg_solve(mg_matrix *m, const double *b, const double *x0, double relativ
 int mg_pcg_solve_dev(mg_matrix *m, const double *d_b, const double *d_x0, double relative_tolerance,
                      int32_t max_iterations, int32_t use_jacobi, double *d_x, void *stream,
                      mg_pcg_result *result);
+
+/* ---- batched independent systems ----
+ * Many SymmetricCSRMatrix systems factored and solved together (the symbolic phase of each is
+ * the reference's, UtduSymbolicFactorization.Analyze (UtduSymbolicFactorization.cs:240-295),
+ * run over host threads).  The systems are merged into one block-diagonal plan: all fronts of
+ * one tree level, from every system, go through the same launches; pivot scale and pivot rule
+ * are per system (UtduNumericFactorization.cs:411-418,559-572).  Values and right-hand sides
+ * are concatenated in system order (offsets from mg_batch_layout); a system with fewer
+ * right-hand sides than nrhs passes zero columns. */
+typedef struct mg_batch mg_batch;
+int mg_batch_analyze(int32_t count, const int32_t *sizes, const int32_t *const *rowptrs,
+                     const int32_t *const *colidxs, const mg_options *opts, mg_batch **out);
+/* row_begin and nnz_begin (count+1 each, may be NULL); info of the merged factorization (may be NULL) */
+int mg_batch_layout(const mg_batch *batch, int32_t *row_begin, int32_t *nnz_begin, mg_symbolic_info *info);
+int mg_batch_factorize_dev(mg_batch *batch, const double *d_values, const mg_options *opts, void *stream);
+/* every system and column refined until its own ||b-Ax||/||b|| <= tol (report: worst over all) */
+int mg_batch_solve_dev(mg_batch *batch, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x, int32_t ldx,
+                       double tol, int32_t max_refine, void *stream, mg_solve_report *report);
+/* host arrays per system: values[t] (nnz of system t), b[t] and x[t] column-major n_t x nrhs[t] */
+int mg_batch_factorize(mg_batch *batch, const double *const *values, const mg_options *opts);
+int mg_batch_solve(mg_batch *batch, const int32_t *nrhs, const double *const *b, double *const *x, double tol,
+                   int32_t max_refine, mg_solve_report *report);
+int mg_batch_numeric_info(const mg_batch *batch, mg_numeric_info *info);
+void mg_batch_release(mg_batch *batch);
+
+/* ---- learned preconditioner ----
+ * No counterpart in the reference: a message-passing network on the matrix graph (three
+ * layers of width 32) whose output corrects JacobiPreconditioner (CGGP/JacobiPreconditioner.cs:12-56)
+ * inside a flexible PCG.  It is trained on the GPU from (residual, exact error) pairs that the
+ * direct solver produces; inference uses BF16 weights with FP32 accumulation on MFMA.  The
+ * solve falls back to plain Jacobi-PCG when the learned correction stalls. */
+typedef struct mg_mpnn mg_mpnn;
+int32_t mg_mpnn_param_count(void);
+int mg_mpnn_create(uint64_t seed, mg_mpnn **out);
+void mg_mpnn_release(mg_mpnn *net);
+int mg_mpnn_get_params(const mg_mpnn *net, float *params);   /* FP32 master copy, param_count values */
+int mg_mpnn_set_params(mg_mpnn *net, const float *params);   /* also refreshes the BF16 copy */
+/* z = M(r) with the FP32 network (MG_FP32) or the BF16 MFMA network (MG_BF16) */
+int mg_mpnn_apply_dev(mg_mpnn *net, mg_matrix *m, const double *d_r, double *d_z, int32_t precision,
+                      void *stream);
+/* mean A-norm loss over ns pairs (columns of d_R = residuals and d_E = exact errors, leading
+ * dimension ld) and its gradient (host array of param_count floats); no update */
+int mg_mpnn_loss_grad_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E,
+                          int64_t ld, void *stream, double *loss, float *grad);
+/* Adam training, `epochs` passes over the pairs in seeded shuffled mini-batches; epoch_losses
+ * (may be NULL) receives the mean loss of each epoch */
+int mg_mpnn_train_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
+                      int32_t epochs, int32_t batch, float learning_rate, uint64_t seed, void *stream,
+                      double *epoch_losses);
+/* SolveWithInitialGuess (CGGP/ConjugateGradientGaussPreSolver.cs:78-166) with the learned
+ * preconditioner: same stopping rule and result; *fell_back = 1 when it switched to Jacobi
+ * (r'z <= 0, or no new minimum of ||r|| for stall_window iterations; 0 selects 50). */
+int mg_pcg_solve_learned_dev(mg_matrix *m, mg_mpnn *net, const double *d_b, const double *d_x0,
+                             double relative_tolerance, int32_t max_iterations, int32_t stall_window, double *d_x,
+                             void *stream, mg_pcg_result *result, int32_t *fell_back);
+int mg_pcg_solve_learned(mg_matrix *m, mg_mpnn *net, const double *b, const double *x0, double relative_tolerance,
+                         int32_t max_iterations, int32_t stall_window, double *x, mg_pcg_result *result,
+                         int32_t *fell_back);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,9 @@ int mg_test_mfma_layout(int32_t precision, double *out);
  * fronts (fused supernode chains), levels, arena elements (sum f^2), largest
  * front, widest front, kernel launches per factorization. */
 int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out);
+/* One v_mfma_f32_16x16x16_bf16 with the operand layout of the learned preconditioner's
+ * layer kernel: d = a b for 16x16 row-major a, b (rounded to bf16), d in FP32. */
+int mg_test_mfma_bf16(const float *a, const float *b, float *d);
 
 /* Launch timing.  mg_profile_enable(1) clears the tallies and brackets every
  * factorization launch with HIP events on its stream; mg_profile_read(kind, ...)

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libmagla.so")
 
 MG_OK, MG_E_NULL, MG_E_ARG, MG_E_RANGE, MG_E_NOT_PD, MG_E_TOO_DENSE, MG_E_DEVICE, MG_E_INTERNAL, MG_E_NOMEM = range(9)
 NATURAL, AMD = 0, 1
-FP64, FP32 = 0, 1
+FP64, FP32, BF16 = 0, 1, 2
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -37,7 +37,7 @@ class _Options(C.Structure):
                 ("supernode_amalgamation_limit", C.c_int32), ("panel_width", C.c_int32),
                 ("allow_diagonal_regularization", C.c_int32), ("precision", C.c_int32),
                 ("pivot_tolerance", C.c_double), ("diagonal_regularization", C.c_double),
-                ("parallel_update_threshold", C.c_int64)]
+                ("parallel_update_threshold", C.c_int64), ("device_graph", C.c_int32), ("reserved", C.c_int32)]
 
 
 class _SymInfo(C.Structure):
@@ -131,6 +131,31 @@ def lib():
     L.mg_spmv_dev.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, vp]
     L.mg_pcg_solve.argtypes = [vp, _f64p, vp, C.c_double, C.c_int32, C.c_int32, _f64p, C.POINTER(_PcgResult)]
     L.mg_pcg_solve_dev.argtypes = [vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp, C.POINTER(_PcgResult)]
+    L.mg_batch_analyze.argtypes = [C.c_int32, vp, vp, vp, C.POINTER(_Options), C.POINTER(vp)]
+    L.mg_batch_layout.argtypes = [vp, vp, vp, C.POINTER(_SymInfo)]
+    L.mg_batch_factorize_dev.argtypes = [vp, vp, C.POINTER(_Options), vp]
+    L.mg_batch_solve_dev.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_double, C.c_int32, vp,
+                                     C.POINTER(_SolveReport)]
+    L.mg_batch_factorize.argtypes = [vp, vp, C.POINTER(_Options)]
+    L.mg_batch_solve.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, C.POINTER(_SolveReport)]
+    L.mg_batch_numeric_info.argtypes = [vp, C.POINTER(_NumInfo)]
+    L.mg_batch_release.argtypes = [vp]
+    L.mg_batch_release.restype = None
+    L.mg_mpnn_param_count.restype = C.c_int32
+    L.mg_mpnn_create.argtypes = [C.c_uint64, C.POINTER(vp)]
+    L.mg_mpnn_release.argtypes = [vp]
+    L.mg_mpnn_release.restype = None
+    L.mg_mpnn_get_params.argtypes = [vp, vp]
+    L.mg_mpnn_set_params.argtypes = [vp, vp]
+    L.mg_mpnn_apply_dev.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
+    L.mg_mpnn_loss_grad_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int64, vp, C.POINTER(C.c_double), vp]
+    L.mg_mpnn_train_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                    C.c_uint64, vp, vp]
+    L.mg_pcg_solve_learned_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp,
+                                           C.POINTER(_PcgResult), C.POINTER(C.c_int32)]
+    L.mg_pcg_solve_learned.argtypes = [vp, vp, _f64p, vp, C.c_double, C.c_int32, C.c_int32, _f64p,
+                                       C.POINTER(_PcgResult), C.POINTER(C.c_int32)]
+    L.mg_test_mfma_bf16.argtypes = [vp, vp, vp]
     L.mg_test_numeric_front.argtypes = [vp, C.c_int32, vp]
     L.mg_test_mfma_layout.argtypes = [C.c_int32, vp]
     L.mg_test_plan_stats.argtypes = [vp, vp]
@@ -287,6 +312,7 @@ class UtduSolverOptions:
     diagonal_regularization: float = 1e-10
     parallel_update_threshold: int = 1 << 20
     precision: int = FP64
+    device_graph: bool = False  # replay the factorization launches from a captured hipGraph
 
     def _c(self):
         o = _Options()
@@ -300,6 +326,7 @@ class UtduSolverOptions:
         o.diagonal_regularization = self.diagonal_regularization
         o.parallel_update_threshold = self.parallel_update_threshold
         o.precision = self.precision
+        o.device_graph = 1 if self.device_graph else 0
         return o
 
 
@@ -508,3 +535,195 @@ class ConjugateGradientGaussPreSolver:
         self.last_result = IterativeSolverResult(x, res.iterations, bool(res.converged), res.residual_norm,
                                                  res.relative_residual)
         return x
+
+    def solve_dev(self, matrix: "DeviceMatrix", b_ptr, x_ptr, x0_ptr=None, stream=None):
+        """Same on HBM vectors; returns the result record (solution left in x_ptr)."""
+        res = _PcgResult()
+        _check(lib().mg_pcg_solve_dev(matrix._h, C.c_void_p(b_ptr), C.c_void_p(x0_ptr), self.relative_tolerance,
+                                      self.max_iterations, 1 if self.use_preconditioner else 0, C.c_void_p(x_ptr),
+                                      C.c_void_p(stream), C.byref(res)))
+        self.last_result = IterativeSolverResult(None, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return self.last_result
+
+
+def test_mfma_bf16(a, b):
+    """One bf16 MFMA with the learned preconditioner's operand layout: returns a @ b (FP32)."""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(16, 16)
+    b = np.ascontiguousarray(b, dtype=np.float32).reshape(16, 16)
+    d = np.zeros((16, 16), dtype=np.float32)
+    _check(lib().mg_test_mfma_bf16(_ptr(a), _ptr(b), _ptr(d)))
+    return d
+
+
+class LearnedPreconditioner:
+    """Message-passing correction on top of Jacobi (no counterpart in MAGLA): three layers of
+    width 32 on the matrix graph, z = (r + s net(r)) / diag(A).  Trained on the GPU (FP32, Adam)
+    from (residual, exact error) pairs; applied in BF16 on MFMA."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        _check(lib().mg_mpnn_create(C.c_uint64(seed), C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def param_count():
+        return int(lib().mg_mpnn_param_count())
+
+    @property
+    def params(self):
+        p = np.zeros(self.param_count(), dtype=np.float32)
+        _check(lib().mg_mpnn_get_params(self._h, _ptr(p)))
+        return p
+
+    @params.setter
+    def params(self, values):
+        p = np.ascontiguousarray(values, dtype=np.float32)
+        assert p.size == self.param_count()
+        _check(lib().mg_mpnn_set_params(self._h, _ptr(p)))
+
+    def apply_dev(self, matrix: "DeviceMatrix", r_ptr, z_ptr, precision=BF16, stream=None):
+        _check(lib().mg_mpnn_apply_dev(self._h, matrix._h, C.c_void_p(r_ptr), C.c_void_p(z_ptr), precision,
+                                       C.c_void_p(stream)))
+
+    def loss_grad_dev(self, matrix: "DeviceMatrix", R_ptr, E_ptr, count, ld, stream=None):
+        loss = C.c_double()
+        grad = np.zeros(self.param_count(), dtype=np.float32)
+        _check(lib().mg_mpnn_loss_grad_dev(self._h, matrix._h, count, C.c_void_p(R_ptr), C.c_void_p(E_ptr), ld,
+                                           C.c_void_p(stream), C.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def train_dev(self, matrix: "DeviceMatrix", R_ptr, E_ptr, count, ld, epochs=10, batch=16, learning_rate=1e-3,
+                  seed=0, stream=None):
+        losses = np.zeros(max(epochs, 1))
+        _check(lib().mg_mpnn_train_dev(self._h, matrix._h, count, C.c_void_p(R_ptr), C.c_void_p(E_ptr), ld, epochs,
+                                       batch, C.c_float(learning_rate), C.c_uint64(seed), C.c_void_p(stream),
+                                       _ptr(losses)))
+        return losses[:epochs]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_mpnn_release(self._h)
+            self._h = None
+
+
+class LearnedConjugateGradientSolver(ConjugateGradientGaussPreSolver):
+    """ConjugateGradientGaussPreSolver with the learned preconditioner in a flexible PCG; falls
+    back to plain Jacobi-PCG when the learned correction stalls (`fell_back`)."""
+
+    def __init__(self, network: LearnedPreconditioner, relative_tolerance=1e-8, max_iterations=0,
+                 stall_window=50):
+        super().__init__(relative_tolerance, max_iterations, True)
+        self.network = network
+        self.stall_window = stall_window
+        self.fell_back = False
+
+    def solve(self, matrix, rhs, initial_guess=None):
+        self.last_result = None
+        dm = matrix if isinstance(matrix, DeviceMatrix) else DeviceMatrix(matrix)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if b.shape[0] != dm.size:
+            raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the matrix row count.")
+        x0 = None if initial_guess is None else np.ascontiguousarray(initial_guess, dtype=np.float64)
+        x = np.zeros(dm.size)
+        res = _PcgResult()
+        fb = C.c_int32()
+        _check(lib().mg_pcg_solve_learned(dm._h, self.network._h, b, _ptr(x0), self.relative_tolerance,
+                                          self.max_iterations, self.stall_window, x, C.byref(res), C.byref(fb)))
+        self.fell_back = bool(fb.value)
+        self.last_result = IterativeSolverResult(x, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return x
+
+    def solve_dev(self, matrix: "DeviceMatrix", b_ptr, x_ptr, x0_ptr=None, stream=None):
+        res = _PcgResult()
+        fb = C.c_int32()
+        _check(lib().mg_pcg_solve_learned_dev(matrix._h, self.network._h, C.c_void_p(b_ptr), C.c_void_p(x0_ptr),
+                                              self.relative_tolerance, self.max_iterations, self.stall_window,
+                                              C.c_void_p(x_ptr), C.c_void_p(stream), C.byref(res), C.byref(fb)))
+        self.fell_back = bool(fb.value)
+        self.last_result = IterativeSolverResult(None, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return self.last_result
+
+
+class UtduBatch:
+    """Many independent systems factored and solved together (mg_batch_*): each system's
+    symbolic phase is the reference's; the numeric phase runs every tree level of all systems
+    in the same launches, with per-system pivot scale and per-system residual norms."""
+
+    def __init__(self, matrices, options: UtduSolverOptions | None = None):
+        self.options = options or UtduSolverOptions()
+        self.matrices = list(matrices)
+        self._keep = [(np.ascontiguousarray(m.row_pointers, dtype=np.int32),
+                       np.ascontiguousarray(m.column_indices, dtype=np.int32)) for m in self.matrices]
+        count = len(self.matrices)
+        sizes = np.array([m.size for m in self.matrices], dtype=np.int32)
+        rps = (C.c_void_p * max(count, 1))(*[rp.ctypes.data for rp, _ in self._keep])
+        cis = (C.c_void_p * max(count, 1))(*[ci.ctypes.data for _, ci in self._keep])
+        o = self.options._c()
+        h = C.c_void_p()
+        _check(lib().mg_batch_analyze(count, _ptr(sizes), rps, cis, C.byref(o), C.byref(h)))
+        self._h = h
+        self.count = count
+        self.row_begin = np.zeros(count + 1, dtype=np.int32)
+        self.nnz_begin = np.zeros(count + 1, dtype=np.int32)
+        info = _SymInfo()
+        _check(lib().mg_batch_layout(h, _ptr(self.row_begin), _ptr(self.nnz_begin), C.byref(info)))
+        self._info = {name: getattr(info, name) for name, _ in _SymInfo._fields_}
+
+    def info(self):
+        info = _SymInfo()
+        _check(lib().mg_batch_layout(self._h, None, None, C.byref(info)))
+        return {name: getattr(info, name) for name, _ in _SymInfo._fields_}
+
+    def values(self):
+        """Concatenated upper-CSR values of all systems (system order)."""
+        return np.concatenate([m.values for m in self.matrices]) if self.count else np.zeros(0)
+
+    def factorize(self, precision=None):
+        o = self.options._c()
+        if precision is not None:
+            o.precision = precision
+        vals = [np.ascontiguousarray(m.values, dtype=np.float64) for m in self.matrices]
+        arr = (C.c_void_p * max(self.count, 1))(*[v.ctypes.data for v in vals])
+        _check(lib().mg_batch_factorize(self._h, arr, C.byref(o)))
+
+    def factorize_dev(self, values_ptr, precision=None, stream=None, device_graph=None):
+        o = self.options._c()
+        if precision is not None:
+            o.precision = precision
+        if device_graph is not None:
+            o.device_graph = 1 if device_graph else 0
+        _check(lib().mg_batch_factorize_dev(self._h, C.c_void_p(values_ptr), C.byref(o), C.c_void_p(stream)))
+
+    def solve(self, rhs, tol=1e-10, max_refine=30):
+        """rhs: list of (n_t,) or (n_t, k_t) arrays; returns the solutions in the same shapes."""
+        bs = [np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(m.size, -1)) for b, m in zip(rhs, self.matrices)]
+        xs = [np.zeros_like(b, order="F") for b in bs]
+        k = np.array([b.shape[1] for b in bs], dtype=np.int32)
+        barr = (C.c_void_p * max(self.count, 1))(*[b.ctypes.data for b in bs])
+        xarr = (C.c_void_p * max(self.count, 1))(*[x.ctypes.data for x in xs])
+        rep = _SolveReport()
+        _check(lib().mg_batch_solve(self._h, _ptr(k), barr, xarr, tol, max_refine, C.byref(rep)))
+        self.last_report = dict(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
+                                relative_residual=rep.relative_residual)
+        return [x.reshape(np.asarray(b).shape) for x, b in zip(xs, rhs)]
+
+    def solve_dev(self, b_ptr, x_ptr, nrhs, tol=1e-10, max_refine=30, stream=None, ldb=None, ldx=None):
+        n = int(self.row_begin[-1])
+        rep = _SolveReport()
+        _check(lib().mg_batch_solve_dev(self._h, nrhs, C.c_void_p(b_ptr), ldb or n, C.c_void_p(x_ptr), ldx or n, tol,
+                                        max_refine, C.c_void_p(stream), C.byref(rep)))
+        return dict(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
+                    relative_residual=rep.relative_residual)
+
+    def numeric_info(self):
+        info = _NumInfo()
+        _check(lib().mg_batch_numeric_info(self._h, C.byref(info)))
+        return {name: getattr(info, name) for name, _ in _NumInfo._fields_}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_batch_release(self._h)
+            self._h = None

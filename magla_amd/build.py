@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmagla.so")
 
-SOURCES = ["symbolic.cpp", "csr.cpp", "capi.cpp", "factor.hip", "spmv.hip"]
+SOURCES = ["symbolic.cpp", "csr.cpp", "capi.cpp", "factor.hip", "spmv.hip", "mpnn.hip"]
 ARCH = os.environ.get("MAGLA_OFFLOAD_ARCH", "gfx950")
 
 
@@ -56,7 +56,7 @@ def build(verbose=False):
             print(err, file=sys.stderr)
     objs = [o for o, _ in results]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

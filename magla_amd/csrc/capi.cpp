@@ -18,6 +18,7 @@ void csr_from_coo(int n, int64_t count, const int *rows, const int *cols, const 
                   int *rowptr, int *colidx, double *values);
 void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values, double *rhs, int count,
                     const int *indices, const double *prescribed);
+void selftest_mfma_bf16(const float *a, const float *b, float *d);
 }  // namespace mg
 
 struct mg_symbolic {
@@ -26,6 +27,12 @@ struct mg_symbolic {
 struct mg_numeric {
     mg::Numeric *num = nullptr;
     mg_symbolic *owner = nullptr;
+};
+struct mg_batch {
+    mg::Symbolic s;  // merged block-diagonal symbolic factorization
+    int count = 0;
+    std::vector<int> row_begin, nnz_begin;
+    mg::Numeric *num = nullptr;
 };
 
 namespace {
@@ -51,6 +58,23 @@ int guarded(F &&fn) {
 
 void require(bool ok, int code, const char *msg) {
     if (!ok) throw mg::Error(code, msg);
+}
+
+void fill_symbolic_info(const mg::Symbolic &S, mg_symbolic_info &info) {
+    std::memset(&info, 0, sizeof(info));
+    info.size = S.n;
+    info.supernode_count = S.nsuper;
+    info.max_front_size = S.max_front;
+    info.max_supernode_width = S.max_width;
+    info.factor_entry_count = S.factor_entries;
+    info.strict_factor_nonzero_count = S.strict_nonzeros;
+    info.factor_operation_count = S.factor_ops;
+    info.estimated_factor_bytes = S.factor_entries * 8;
+    info.max_front_bytes = (int64_t)S.max_front * (S.max_front + 1) / 2 * 8;
+    info.peak_contribution_bytes = S.peak_contribution * 8;
+    info.structure_bytes = ((int64_t)S.prows.size() + (int64_t)S.lvidx.size() + (int64_t)S.pat_rows.size()) * 4;
+    info.level_count = S.plan ? mg::plan_level_count(*S.plan) : -1;
+    info.matrix_nonzero_count = S.matrix_nnz;
 }
 
 // Structural checks the C# types guarantee by construction.
@@ -128,20 +152,7 @@ int mg_symbolic_info_get(const mg_symbolic *sym, mg_symbolic_info *info, int32_t
     return guarded([&] {
         require(sym != nullptr && info != nullptr, MG_E_NULL, "symbolic or info is null.");
         const mg::Symbolic &S = sym->s;
-        std::memset(info, 0, sizeof(*info));
-        info->size = S.n;
-        info->supernode_count = S.nsuper;
-        info->max_front_size = S.max_front;
-        info->max_supernode_width = S.max_width;
-        info->factor_entry_count = S.factor_entries;
-        info->strict_factor_nonzero_count = S.strict_nonzeros;
-        info->factor_operation_count = S.factor_ops;
-        info->estimated_factor_bytes = S.factor_entries * 8;
-        info->max_front_bytes = (int64_t)S.max_front * (S.max_front + 1) / 2 * 8;
-        info->peak_contribution_bytes = S.peak_contribution * 8;
-        info->structure_bytes = ((int64_t)S.prows.size() + (int64_t)S.lvidx.size() + (int64_t)S.pat_rows.size()) * 4;
-        info->level_count = S.plan ? mg::plan_level_count(*S.plan) : -1;
-        info->matrix_nonzero_count = S.matrix_nnz;
+        fill_symbolic_info(S, *info);
         if (perm && S.n) std::memcpy(perm, S.perm.data(), sizeof(int) * S.n);
         if (iperm && S.n) std::memcpy(iperm, S.iperm.data(), sizeof(int) * S.n);
     });
@@ -322,6 +333,244 @@ int mg_pcg_solve(mg_matrix *m, const double *b, const double *x0, double tol, in
         mg::pcg_device(m->m, db.p, x0 ? dx0.p : nullptr, tol, max_iterations, use_jacobi, dx.p, 0, *result);
         MG_HIP(hipDeviceSynchronize());
         if (n) MG_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_batch_analyze(int32_t count, const int32_t *sizes, const int32_t *const *rowptrs,
+                     const int32_t *const *colidxs, const mg_options *opts, mg_batch **out) {
+    return guarded([&] {
+        require(out != nullptr, MG_E_NULL, "out is null.");
+        require(count >= 0, MG_E_ARG, "The system count cannot be negative.");
+        require(count == 0 || (sizes && rowptrs && colidxs), MG_E_NULL, "matrix arrays are null.");
+        mg_options o;
+        mg_options_default(&o);
+        if (opts) o = *opts;
+        for (int t = 0; t < count; ++t) check_csr(sizes[t], rowptrs[t], colidxs[t]);
+        auto *b = new mg_batch();
+        try {
+            std::vector<mg::Symbolic> each;
+            mg::analyze_batch(count, sizes, rowptrs, colidxs, o, each);
+            mg::merge_symbolic(each, b->s);
+            b->count = count;
+            b->row_begin = b->s.sys_begin;
+            b->nnz_begin.assign(count + 1, 0);
+            for (int t = 0; t < count; ++t) b->nnz_begin[t + 1] = b->nnz_begin[t] + rowptrs[t][sizes[t]];
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int mg_batch_layout(const mg_batch *b, int32_t *row_begin, int32_t *nnz_begin, mg_symbolic_info *info) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        if (row_begin) std::copy(b->row_begin.begin(), b->row_begin.end(), row_begin);
+        if (nnz_begin) std::copy(b->nnz_begin.begin(), b->nnz_begin.end(), nnz_begin);
+        if (info) fill_symbolic_info(b->s, *info);
+    });
+}
+
+int mg_batch_factorize_dev(mg_batch *b, const double *d_values, const mg_options *opts, void *stream) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        require(b->s.matrix_nnz == 0 || d_values, MG_E_NULL, "values are null.");
+        mg_options o = b->s.opts;
+        if (opts) o = *opts;
+        mg::factorize_device(b->s, d_values, o.precision, o, (hipStream_t)stream, b->num);
+    });
+}
+
+int mg_batch_solve_dev(mg_batch *b, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x, int32_t ldx,
+                       double tol, int32_t max_refine, void *stream, mg_solve_report *report) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        require(b->num != nullptr, MG_E_ARG, "The batch has not been factorized.");
+        require(b->s.n == 0 || nrhs == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        mg::solve_device(b->num, nrhs, d_b, ldb, d_x, ldx, tol, max_refine, (hipStream_t)stream, report);
+    });
+}
+
+int mg_batch_factorize(mg_batch *b, const double *const *values, const mg_options *opts) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        require(b->count == 0 || values, MG_E_NULL, "values are null.");
+        const int64_t nnz = b->s.matrix_nnz;
+        std::vector<double> all((size_t)std::max<int64_t>(nnz, 1));
+        for (int t = 0; t < b->count; ++t)
+            std::copy(values[t], values[t] + (b->nnz_begin[t + 1] - b->nnz_begin[t]), all.begin() + b->nnz_begin[t]);
+        mg::DevArray<double> dv(all.size());
+        MG_HIP(hipMemcpy(dv.p, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice));
+        mg_options o = b->s.opts;
+        if (opts) o = *opts;
+        mg::factorize_device(b->s, dv.p, o.precision, o, 0, b->num);
+        MG_HIP(hipDeviceSynchronize());
+    });
+}
+
+int mg_batch_solve(mg_batch *b, const int32_t *nrhs, const double *const *rhs, double *const *x, double tol,
+                   int32_t max_refine, mg_solve_report *report) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        require(b->num != nullptr, MG_E_ARG, "The batch has not been factorized.");
+        require(b->count == 0 || (nrhs && rhs && x), MG_E_NULL, "argument is null.");
+        int k = 0;
+        for (int t = 0; t < b->count; ++t) {
+            require(nrhs[t] >= 0, MG_E_ARG, "The number of right-hand sides cannot be negative.");
+            k = std::max(k, nrhs[t]);
+        }
+        const int n = b->s.n;
+        if (k == 0 || n == 0) {
+            if (report) *report = mg_solve_report{0, 1, 0.0};
+            return;
+        }
+        std::vector<double> B((size_t)n * k, 0.0), X((size_t)n * k, 0.0);
+        for (int t = 0; t < b->count; ++t) {
+            const int r0 = b->row_begin[t], nt = b->row_begin[t + 1] - r0;
+            for (int c = 0; c < nrhs[t]; ++c)
+                std::copy(rhs[t] + (size_t)c * nt, rhs[t] + (size_t)(c + 1) * nt, B.begin() + (size_t)c * n + r0);
+        }
+        mg::DevArray<double> db(B.size()), dx(X.size());
+        MG_HIP(hipMemcpy(db.p, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice));
+        mg::solve_device(b->num, k, db.p, n, dx.p, n, tol, max_refine, 0, report);
+        MG_HIP(hipDeviceSynchronize());
+        MG_HIP(hipMemcpy(X.data(), dx.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost));
+        for (int t = 0; t < b->count; ++t) {
+            const int r0 = b->row_begin[t], nt = b->row_begin[t + 1] - r0;
+            for (int c = 0; c < nrhs[t]; ++c)
+                std::copy(X.begin() + (size_t)c * n + r0, X.begin() + (size_t)c * n + r0 + nt, x[t] + (size_t)c * nt);
+        }
+    });
+}
+
+int mg_batch_numeric_info(const mg_batch *b, mg_numeric_info *info) {
+    return guarded([&] {
+        require(b != nullptr && info != nullptr, MG_E_NULL, "argument is null.");
+        require(b->num != nullptr, MG_E_ARG, "The batch has not been factorized.");
+        mg::numeric_info(b->num, *info, nullptr);
+    });
+}
+
+void mg_batch_release(mg_batch *b) {
+    if (!b) return;
+    if (b->num) mg::numeric_release(b->num);
+    delete b;
+}
+
+int32_t mg_mpnn_param_count(void) { return mg::kMpP; }
+
+int mg_mpnn_create(uint64_t seed, mg_mpnn **out) {
+    return guarded([&] {
+        require(out != nullptr, MG_E_NULL, "out is null.");
+        auto *h = new mg_mpnn();
+        try {
+            mg::mpnn_init(h->net, seed, 0);
+            MG_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void mg_mpnn_release(mg_mpnn *net) { delete net; }
+
+int mg_mpnn_get_params(const mg_mpnn *net, float *params) {
+    return guarded([&] {
+        require(net != nullptr && params != nullptr, MG_E_NULL, "argument is null.");
+        MG_HIP(hipMemcpy(params, net->net.w.p, sizeof(float) * mg::kMpP, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_mpnn_set_params(mg_mpnn *net, const float *params) {
+    return guarded([&] {
+        require(net != nullptr && params != nullptr, MG_E_NULL, "argument is null.");
+        MG_HIP(hipMemcpy(net->net.w.p, params, sizeof(float) * mg::kMpP, hipMemcpyHostToDevice));
+        mg::mpnn_pack_bf16(net->net, 0);
+        MG_HIP(hipDeviceSynchronize());
+    });
+}
+
+int mg_mpnn_apply_dev(mg_mpnn *net, mg_matrix *m, const double *d_r, double *d_z, int32_t precision,
+                      void *stream) {
+    return guarded([&] {
+        require(net != nullptr && m != nullptr, MG_E_NULL, "argument is null.");
+        require(m->m.n == 0 || (d_r && d_z), MG_E_NULL, "vector is null.");
+        require(precision == MG_FP32 || precision == MG_BF16, MG_E_ARG, "precision must be MG_FP32 or MG_BF16.");
+        const hipStream_t st = (hipStream_t)stream;
+        if (precision == MG_FP32) mg::mpnn_apply_f32(net->net, m->m, d_r, d_z, st);
+        else mg::mpnn_apply_bf16(net->net, m->m, d_r, d_z, nullptr, nullptr, nullptr, nullptr, nullptr, st);
+    });
+}
+
+int mg_mpnn_loss_grad_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
+                          void *stream, double *loss, float *grad) {
+    return guarded([&] {
+        require(net != nullptr && m != nullptr && loss != nullptr, MG_E_NULL, "argument is null.");
+        require(ns >= 0 && ld >= m->m.n, MG_E_ARG, "sample count or leading dimension is invalid.");
+        require(ns == 0 || (d_R && d_E), MG_E_NULL, "samples are null.");
+        const hipStream_t st = (hipStream_t)stream;
+        *loss = ns ? mg::mpnn_loss_grad(net->net, m->m, ns, d_R, d_E, ld, st) : 0.0;
+        if (grad) {
+            MG_HIP(hipMemcpyAsync(grad, net->net.grad.p, sizeof(float) * mg::kMpP, hipMemcpyDeviceToHost, st));
+            MG_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
+int mg_mpnn_train_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
+                      int32_t epochs, int32_t batch, float learning_rate, uint64_t seed, void *stream,
+                      double *epoch_losses) {
+    return guarded([&] {
+        require(net != nullptr && m != nullptr, MG_E_NULL, "argument is null.");
+        require(ns >= 0 && ld >= m->m.n && epochs >= 0, MG_E_ARG, "invalid training arguments.");
+        require(ns == 0 || (d_R && d_E), MG_E_NULL, "samples are null.");
+        mg::mpnn_train(net->net, m->m, ns, d_R, d_E, ld, epochs, batch, learning_rate, seed, epoch_losses,
+                       (hipStream_t)stream);
+    });
+}
+
+int mg_pcg_solve_learned_dev(mg_matrix *m, mg_mpnn *net, const double *d_b, const double *d_x0, double tol,
+                             int32_t max_iterations, int32_t stall_window, double *d_x, void *stream,
+                             mg_pcg_result *result, int32_t *fell_back) {
+    return guarded([&] {
+        require(m != nullptr && net != nullptr && result != nullptr, MG_E_NULL, "argument is null.");
+        require(m->m.n == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        int fb = 0;
+        mg::fcg_learned_device(m->m, net->net, d_b, d_x0, tol, max_iterations, stall_window, d_x,
+                               (hipStream_t)stream, *result, fb);
+        if (fell_back) *fell_back = fb;
+    });
+}
+
+int mg_pcg_solve_learned(mg_matrix *m, mg_mpnn *net, const double *b, const double *x0, double tol,
+                         int32_t max_iterations, int32_t stall_window, double *x, mg_pcg_result *result,
+                         int32_t *fell_back) {
+    return guarded([&] {
+        require(m != nullptr && net != nullptr && result != nullptr, MG_E_NULL, "argument is null.");
+        const int n = m->m.n;
+        require(n == 0 || (b && x), MG_E_NULL, "rightHandSide is null.");
+        mg::DevArray<double> db(std::max(n, 1)), dx(std::max(n, 1)), dx0;
+        if (n) MG_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (x0 && n) {
+            dx0.alloc(n);
+            MG_HIP(hipMemcpy(dx0.p, x0, sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+        int fb = 0;
+        mg::fcg_learned_device(m->m, net->net, db.p, x0 ? dx0.p : nullptr, tol, max_iterations, stall_window, dx.p,
+                               0, *result, fb);
+        MG_HIP(hipDeviceSynchronize());
+        if (n) MG_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (fell_back) *fell_back = fb;
+    });
+}
+
+int mg_test_mfma_bf16(const float *a, const float *b, float *d) {
+    return guarded([&] {
+        require(a && b && d, MG_E_NULL, "argument is null.");
+        mg::selftest_mfma_bf16(a, b, d);
     });
 }
 

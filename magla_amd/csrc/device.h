@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <memory>
+
 #include "internal.h"
 
 namespace mg {
@@ -46,6 +49,8 @@ struct DevArray {
     }
 };
 
+struct MpGraph;
+
 // Device-resident SymmetricCSRMatrix expanded to both triangles for
 // row-parallel products.
 struct Matrix {
@@ -60,6 +65,7 @@ struct Matrix {
     DevArray<double> invdiag;             // Jacobi: 1 / A[i,i]
     int diag_status = 0;                  // 0 ok, else MG_E_NOT_PD (zero/invalid diagonal)
     int lanes = 8;                        // lanes per row in the SpMV kernel
+    std::shared_ptr<MpGraph> mp;          // learned-preconditioner graph data (built on first use)
     void build(int n, const int *rowptr, const int *colidx);
     void set_values_host(const double *values);
     void set_values_dev(const double *d_values, hipStream_t st);
@@ -84,8 +90,65 @@ void spmv_device(const Matrix &A, int nrhs, const double *d_x, int ldx, double *
 void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double tol, int max_it, int use_jacobi,
                 double *d_x, hipStream_t st, mg_pcg_result &res);
 
+constexpr int kRedBlocks = 512;  // fixed grid of the deterministic vector reductions
+
+// Device scalars of a (flexible) PCG run; kernels of an iteration turn into no-ops once done.
+struct PcgScalars {
+    double rz, pap, alpha, beta, rnorm, thr, bnorm, best;
+    int done, iters, fail, fallback, restart, best_it, pad0, pad1;
+};
+
+// ---- learned preconditioner (message-passing network on the matrix graph) ----
+constexpr int kMpW = 32;                                    // hidden width
+constexpr int kMpP = 2 * kMpW + 3 * 2 * kMpW * kMpW + kMpW;  // parameter count
+constexpr int kMpOffW = 2 * kMpW;                           // W1_0 (then W2_0, W1_1, ...)
+constexpr int kMpOffWd = 2 * kMpW + 6 * kMpW * kMpW;        // decoder
+constexpr int kMpBf16Layer = 2 * 4 * 64 * 4;                // packed B operands per layer (bf16)
+constexpr int kMpBlocks = 512;                              // fixed grid of the gradient kernels
+
+// Per-matrix graph data: normalized off-diagonal weights a_ij / sqrt(a_ii a_jj) on the full
+// CSR (0 on the diagonal), node feature log(a_ii / mean a_ii), and a_ii.
+struct MpGraph {
+    int n = 0;
+    DevArray<float> ew, cf;
+    DevArray<double> diag;
+};
+
+struct Mpnn {
+    DevArray<float> w;              // FP32 master parameters
+    DevArray<uint16_t> wb;          // BF16 copy: packed MFMA B operands (3 layers), then w_e, w_c, w_d
+    DevArray<float> grad, am, av;   // gradient and Adam moments
+    int adam_t = 0;
+    int ws_n = 0;                   // workspaces below are sized for this n
+    DevArray<float> H, Mb, Pre, dH, dHp, dM, part;
+    DevArray<uint16_t> hb0, hb1;
+    DevArray<double> z, az, dpart, scal;
+};
+
+void mp_graph_prepare(Matrix &A, hipStream_t st);
+void mpnn_init(Mpnn &net, uint64_t seed, hipStream_t st);
+void mpnn_pack_bf16(Mpnn &net, hipStream_t st);
+// z = (r + s net(r)) / diag with s = ||r|| / sqrt(n) from *d_rnorm (a device scalar);
+// when d_rnorm is null the norm is computed here.  FP32 network or BF16 MFMA network.
+void mpnn_apply_f32(Mpnn &net, Matrix &A, const double *d_r, double *d_z, hipStream_t st);
+// BF16 inference; S (may be null) gates the kernels: they do nothing once S->done or S->fallback.
+// rz_part / rap_part (may be null) receive fixed-grid partial sums of r.z and z.ap (kRedBlocks).
+void mpnn_apply_bf16(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const double *d_rnorm,
+                     const PcgScalars *S, const double *d_ap, double *rz_part, double *zap_part, hipStream_t st);
+// mean loss over ns pairs (columns of R, E with leading dimension ld); gradient into net.grad
+double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, hipStream_t st);
+void mpnn_adam_step(Mpnn &net, float lr, hipStream_t st);
+void mpnn_train(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, int epochs, int batch,
+                float lr, uint64_t seed, double *losses, hipStream_t st);
+void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d_x0, double tol, int max_it,
+                        int stall_window, double *d_x, hipStream_t st, mg_pcg_result &res, int &fell_back);
+
 }  // namespace mg
 
 struct mg_matrix {
     mg::Matrix m;
+};
+
+struct mg_mpnn {
+    mg::Mpnn net;
 };

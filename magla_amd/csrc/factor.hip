@@ -68,6 +68,9 @@ struct DevicePlan {
     std::vector<std::vector<double>> lvl_upd_flops, lvl_upd_elems;
     std::vector<std::vector<double>> lvl_panel_elems;  // sum over fronts of (f - pe) * nb (L below the panel)
 
+    int nsys = 1;                   // independent systems merged into this plan (batched solves)
+    std::vector<int> h_sys_begin;   // first row of each system, nsys+1
+    DevArray<int> front_sys, sys_begin;
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, diag_nodes, diag_tb;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks, sfwd_tasks, sbwd_tasks;
@@ -341,7 +344,17 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_panel_elems.push_back(pel);
     }
 
+    // system of every front (pivot scale and residual norms are per system)
+    P->h_sys_begin = S.sys_begin.empty() ? std::vector<int>{0, n} : S.sys_begin;
+    P->nsys = (int)P->h_sys_begin.size() - 1;
+    std::vector<int> fsys(nf, 0);
+    for (int g = 0; g < nf; ++g)
+        fsys[g] = (int)(std::upper_bound(P->h_sys_begin.begin(), P->h_sys_begin.end(), P->h_first[g]) -
+                        P->h_sys_begin.begin()) - 1;
+
     if (!upload) return P;
+    P->front_sys.upload(fsys);
+    P->sys_begin.upload(P->h_sys_begin);
     P->f.upload(P->h_f);
     P->w.upload(P->h_w);
     P->first.upload(P->h_first);
@@ -610,7 +623,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                    int ps, const double *__restrict__ scale, double tol, double regf,
+                                                    int ps, const double *__restrict__ scale,
+                                                    const int *__restrict__ front_sys, double tol, double regf,
                                                     int allow_reg, double *__restrict__ D,
                                                     unsigned char *__restrict__ flags) {
     // Rows/columns past nb are padded with the identity (pivot 1, never reported).
@@ -625,7 +639,7 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ node
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     T *M = arena + off[s];
     const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const double sc = scale[0];
+    const double sc = scale[front_sys[s]];
     PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
     T a[kDiagCols], li[kDiagCols];
 #pragma unroll
@@ -1191,6 +1205,36 @@ __global__ __launch_bounds__(256) void k_residual_final(const double *__restrict
     }
 }
 
+// relres[t * nrhs + c] = ||R[rows of system t, c]|| / ||B[rows of system t, c]|| (one workgroup
+// per system and column; ||B|| = 0 counts as 1)
+__global__ __launch_bounds__(256) void k_residual_seg(const double *__restrict__ R, int n,
+                                                      const double *__restrict__ B, int ldb,
+                                                      const int *__restrict__ sys_begin, int nrhs,
+                                                      double *__restrict__ relres) {
+    __shared__ double sr[256], sb[256];
+    const int t = blockIdx.x, c = blockIdx.y;
+    double rr = 0.0, bb = 0.0;
+    for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256) {
+        const double r = R[(int64_t)c * n + i], b = B[(int64_t)c * ldb + i];
+        rr += r * r;
+        bb += b * b;
+    }
+    sr[threadIdx.x] = rr;
+    sb[threadIdx.x] = bb;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sr[threadIdx.x] += sr[threadIdx.x + h];
+            sb[threadIdx.x] += sb[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double bn = sqrt(sb[0]);
+        relres[(int64_t)t * nrhs + c] = sqrt(sr[0]) / (bn > 0.0 ? bn : 1.0);
+    }
+}
+
 __global__ void k_zero_cols(double *__restrict__ X, int ldx, int n, int nrhs) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * nrhs) return;
@@ -1240,6 +1284,23 @@ __global__ __launch_bounds__(256) void k_diag_scale_final(const double *__restri
     if (threadIdx.x == 0) out[0] = sm[0] > 0.0 ? sm[0] : 1.0;
 }
 
+// max |A_ii| of every system of a batch (one workgroup per system), 1 when zero
+__global__ __launch_bounds__(256) void k_diag_scale_seg(const int *__restrict__ dpos, const double *__restrict__ vals,
+                                                        const int *__restrict__ sys_begin, double *__restrict__ out) {
+    __shared__ double sm[256];
+    const int t = blockIdx.x;
+    double m = 0.0;
+    for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256)
+        m = nanmax(m, dpos[i] >= 0 ? fabs(vals[dpos[i]]) : 0.0);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = nanmax(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = sm[0] > 0.0 ? sm[0] : 1.0;
+}
+
 struct Numeric {
     std::shared_ptr<DevicePlan> plan;
     const Symbolic *sym = nullptr;
@@ -1257,7 +1318,16 @@ struct Numeric {
     DevArray<unsigned char> wsY, wsU, wsP, wsQ;
     DevArray<double> wsR, wsX, wsPart, wsRel;
     int ws_nrhs = 0;
+    hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
+    int graph_precision = -1;
+    ~Numeric() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+    }
 };
+
+// hipGraph capture of the factorization launches: requested by mg_options.device_graph.
+static inline bool opts_use_graph(const mg_options &o) { return o.device_graph != 0; }
+static bool profile_on();
 
 constexpr int kScaleParts = 256;
 
@@ -1320,6 +1390,8 @@ struct ProfScope {
     }
 };
 
+static bool profile_on() { return g_prof.on; }
+
 void profile_enable(int on) {
     g_prof.on = on != 0;
     for (auto &t : g_prof.k) t.launches = 0, t.ms = 0, t.flops = 0, t.bytes = 0;
@@ -1376,7 +1448,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             if (d1 > d0) {
                 ProfScope ps_(st, KP_PDIAG, 0.0, 0.0);
                 hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, ps, N.scale.p, opts.pivot_tolerance,
+                                   P.w.p, P.off.p, P.first.p, ps, N.scale.p, P.front_sys.p, opts.pivot_tolerance,
                                    opts.diagonal_regularization, opts.allow_diagonal_regularization, N.diag.p,
                                    N.flags.p);
             }
@@ -1422,7 +1494,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->flags.alloc(std::max(S.n, 1));
             N->counts.alloc(4);
             N->mags.alloc(4);
-            N->scale.alloc(1);
+            N->scale.alloc(std::max(S.plan->nsys, 1));
             N->wsScale.alloc(kScaleParts);
             N->values.alloc(std::max(S.matrix_nnz, 1));
             N->A.build(S.n, S.a_rowptr.data(), S.a_colidx.data());
@@ -1433,11 +1505,47 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             if (S.matrix_nnz > 0)
                 MG_HIP(hipMemcpyAsync(N->values.p, d_values, sizeof(double) * S.matrix_nnz, hipMemcpyDeviceToDevice, st));
             N->A.set_values_dev(N->values.p, st);
-            hipLaunchKernelGGL(k_diag_scale_part, dim3(kScaleParts), dim3(256), 0, st, N->A.dpos.p, N->A.vals.p, S.n,
-                               N->wsScale.p);
-            hipLaunchKernelGGL(k_diag_scale_final, dim3(1), dim3(256), 0, st, N->wsScale.p, kScaleParts, N->scale.p);
-            if (precision == MG_FP32) run_factor<float>(*N, opts, st);
-            else run_factor<double>(*N, opts, st);
+            const bool graph = opts_use_graph(opts) && !profile_on();
+            if (graph && N->gexec && N->graph_precision == precision) {
+                MG_HIP(hipGraphLaunch(N->gexec, st));
+            } else {
+                if (graph) {
+                    if (N->gexec) {
+                        (void)hipGraphExecDestroy(N->gexec);
+                        N->gexec = nullptr;
+                    }
+                    MG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                }
+                try {
+                    if (S.plan->nsys > 1) {
+                        hipLaunchKernelGGL(k_diag_scale_seg, dim3(S.plan->nsys), dim3(256), 0, st, N->A.dpos.p,
+                                           N->A.vals.p, S.plan->sys_begin.p, N->scale.p);
+                    } else {
+                        hipLaunchKernelGGL(k_diag_scale_part, dim3(kScaleParts), dim3(256), 0, st, N->A.dpos.p,
+                                           N->A.vals.p, S.n, N->wsScale.p);
+                        hipLaunchKernelGGL(k_diag_scale_final, dim3(1), dim3(256), 0, st, N->wsScale.p, kScaleParts,
+                                           N->scale.p);
+                    }
+                    if (precision == MG_FP32) run_factor<float>(*N, opts, st);
+                    else run_factor<double>(*N, opts, st);
+                } catch (...) {
+                    if (graph) {
+                        hipGraph_t g = nullptr;
+                        (void)hipStreamEndCapture(st, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                    }
+                    throw;
+                }
+                if (graph) {
+                    hipGraph_t g = nullptr;
+                    MG_HIP(hipStreamEndCapture(st, &g));
+                    const hipError_t ie = hipGraphInstantiate(&N->gexec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    MG_HIP(ie);
+                    N->graph_precision = precision;
+                    MG_HIP(hipGraphLaunch(N->gexec, st));
+                }
+            }
             hipLaunchKernelGGL(k_pivot_stats, dim3(1), dim3(1024), 0, st, N->diag.p, N->flags.p, S.n, N->counts.p,
                                N->mags.p);
             MG_HIP(hipGetLastError());
@@ -1527,7 +1635,7 @@ static void ensure_ws(Numeric &N, int nrhs) {
     N.wsQ.alloc((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * nrhs * esz);
     N.wsR.alloc((size_t)n * nrhs);
     N.wsPart.alloc((size_t)residual_blocks(n, N.A.lanes) * nrhs * 2);
-    N.wsRel.alloc(nrhs);
+    N.wsRel.alloc((size_t)nrhs * std::max(N.plan->nsys, 1));
     N.ws_nrhs = nrhs;
 }
 
@@ -1549,9 +1657,14 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
         default: MG_RES(64); break;
     }
 #undef MG_RES
-    hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, N.wsPart.p, nb, N.wsRel.p);
-    relres.resize(nrhs);
-    MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    const int nsys = N.plan->nsys;
+    if (nsys > 1)
+        hipLaunchKernelGGL(k_residual_seg, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, n, d_b, ldb,
+                           N.plan->sys_begin.p, nrhs, N.wsRel.p);
+    else
+        hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, N.wsPart.p, nb, N.wsRel.p);
+    relres.resize((size_t)nrhs * nsys);
+    MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * relres.size(), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
 }
 

@@ -42,6 +42,10 @@ struct Symbolic {
     int64_t factor_entries = 0, strict_nonzeros = 0, factor_ops = 0, peak_contribution = 0;
     int max_front = 0, max_width = 0;
 
+    // Batched systems merged into one block-diagonal system: first row (original and
+    // elimination order alike) of each system, count+1 entries; empty for a single system.
+    std::vector<int> sys_begin;
+
     std::shared_ptr<DevicePlan> plan;  // device index maps and level schedule (lazy)
 
     int front(int s) const { return pat_ptr[s + 1] - pat_ptr[s]; }
@@ -50,6 +54,12 @@ struct Symbolic {
 
 // Symbolic phase entry (symbolic.cpp).
 void analyze(int n, const int *rowptr, const int *colidx, const mg_options &opts, Symbolic &out);
+// Independent systems: analyse each (in parallel over host threads, each result the same as
+// analyze() of that system alone), then relabel them into one block-diagonal Symbolic whose
+// supernodal forest is the union of the systems' forests.
+void analyze_batch(int count, const int *sizes, const int *const *rowptrs, const int *const *colidxs,
+                   const mg_options &opts, std::vector<Symbolic> &each);
+void merge_symbolic(const std::vector<Symbolic> &each, Symbolic &out);
 // ApproximateMinimumDegreeOrdering.Compute over the pattern of an upper CSR.
 void amd_order(int n, const int *rowptr, const int *colidx, std::vector<int> &perm);
 

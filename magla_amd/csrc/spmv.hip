@@ -20,7 +20,6 @@
 namespace mg {
 
 namespace {
-constexpr int kRedBlocks = 512;  // fixed grid for deterministic reductions
 constexpr int kBS = 256;
 
 inline unsigned nblk(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
@@ -146,10 +145,6 @@ void spmv_device(const Matrix &A, int nrhs, const double *d_x, int ldx, double *
 
 // ---- PCG ---------------------------------------------------------------------
 
-struct PcgScalars {
-    double rz, pap, alpha, beta, rnorm, thr, bnorm;
-    int done, iters, fail, pad;
-};
 
 __device__ inline double block_sum(double v, double *sm) {
     sm[threadIdx.x] = v;
@@ -373,6 +368,169 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
         if (h.fail)
             throw Error(MG_E_NOT_PD, "p^T A p <= 0. The matrix is not positive definite, so the CG method cannot be applied.");
         if (!h.done) h.iters = maxit;
+    }
+    res.iterations = h.iters;
+    res.residual_norm = h.rnorm;
+    res.relative_residual = h.rnorm / h.bnorm;
+    res.converged = res.relative_residual <= tol;
+}
+
+}  // namespace mg
+
+// ---- flexible PCG with the learned preconditioner --------------------------------
+//
+// ConjugateGradientGaussPreSolver's iteration (CGGP/ConjugateGradientGaussPreSolver.cs:78-166)
+// with z = M(r) from the message-passing network on top of Jacobi (mpnn.hip).  M is not a
+// fixed linear operator, so the new direction is A-orthogonalised against the previous one
+// (flexible CG): p = z + beta p with beta = -(z'Ap) / (p'Ap).  The stopping rule, the
+// initial residual and the p'Ap <= 0 failure are the reference's.  Fallback to plain
+// Jacobi-PCG (restart with p = z = D^{-1} r) when r'z <= 0 or when ||r|| has not reached a
+// new minimum for `stall` iterations.
+
+namespace mg {
+
+// convergence test (:151-153) and stall tracking
+__global__ void k_fcg_norm(const double *__restrict__ part, int it, int stall, PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double rr = finish_sum(part, sm);
+    if (threadIdx.x == 0) {
+        S->rnorm = sqrt(rr);
+        S->iters = it;
+        if (S->rnorm <= S->thr) {
+            S->done = 1;
+        } else if (!S->fallback) {
+            if (S->rnorm < S->best) {
+                S->best = S->rnorm;
+                S->best_it = it;
+            } else if (it - S->best_it >= stall) {
+                S->fallback = 1;
+                S->restart = 1;
+            }
+        }
+    }
+}
+
+// Jacobi z = D^{-1} r with partials of r'z and z'Ap; active only after the fallback
+__global__ __launch_bounds__(256) void k_fcg_jacobi(const double *__restrict__ invd, const double *__restrict__ r,
+                                                    const double *__restrict__ ap, double *__restrict__ z, int n,
+                                                    double *__restrict__ rz_part, double *__restrict__ zap_part,
+                                                    const PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done || !S->fallback) return;
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const double zi = invd[i] * r[i];
+        z[i] = zi;
+        a += r[i] * zi;
+        if (ap) b += zi * ap[i];
+    }
+    const double ta = block_sum(a, sm);
+    __syncthreads();
+    const double tb = block_sum(b, sm);
+    if (threadIdx.x == 0) {
+        rz_part[blockIdx.x] = ta;
+        zap_part[blockIdx.x] = tb;
+    }
+}
+
+// first direction: rz = r'z; a non-positive r'z switches to Jacobi before the first step
+__global__ void k_fcg_start(const double *__restrict__ rz_part, PcgScalars *__restrict__ S, int second) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    if (second && !S->fallback) return;
+    const double rz = finish_sum(rz_part, sm);
+    if (threadIdx.x == 0) {
+        S->rz = rz;
+        S->best = S->rnorm;
+        S->best_it = 0;
+        if (!second && !(rz > 0.0)) S->fallback = 1;
+    }
+}
+
+// beta = -(z'Ap) / (p'Ap) (0 on a restart); rz = r'z for the next step length
+__global__ void k_fcg_beta(const double *__restrict__ rz_part, const double *__restrict__ zap_part,
+                           PcgScalars *__restrict__ S) {
+    __shared__ double sm[kBS];
+    if (S->done) return;
+    const double rz = finish_sum(rz_part, sm);
+    __syncthreads();
+    const double zap = finish_sum(zap_part, sm);
+    if (threadIdx.x == 0) {
+        S->rz = rz;
+        if (S->restart || !(rz > 0.0)) {
+            S->beta = 0.0;
+            S->restart = 0;
+            if (!(rz > 0.0) && !S->fallback) {
+                S->fallback = 1;
+                S->restart = 1;
+            }
+        } else {
+            S->beta = -zap / S->pap;
+        }
+    }
+}
+
+void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d_x0, double tol, int max_it,
+                        int stall_window, double *d_x, hipStream_t st, mg_pcg_result &res, int &fell_back) {
+    if (!std::isfinite(tol) || tol < 0.0) throw Error(MG_E_RANGE, "RelativeTolerance is out of range.");
+    if (max_it < 0) throw Error(MG_E_RANGE, "MaxIterations: the maximum iteration count cannot be negative.");
+    const int n = A.n;
+    res = mg_pcg_result{0, 1, 0.0, 0.0};
+    fell_back = 0;
+    if (n == 0) return;
+    if (A.diag_status)
+        throw Error(MG_E_NOT_PD, "The diagonal element is zero or invalid while building the Jacobi preconditioner.");
+    mp_graph_prepare(A, st);
+    if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+    DevArray<double> r(n), z(n), p(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
+    DevArray<PcgScalars> S(1);
+    MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
+    const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks);
+    launch_spmv(A, d_x, ap.p, st);
+    hipLaunchKernelGGL(k_residual0, gv, bv, 0, st, d_b, ap.p, n, r.p);
+    hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, r.p, r.p, n, part.p, (const PcgScalars *)nullptr);
+    hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, d_b, d_b, n, part2.p, (const PcgScalars *)nullptr);
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), bv, 0, st, part.p, part2.p, tol, S.p);
+    PcgScalars h{};
+    MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    if (!h.done) {
+        const double *rnorm = &S.p->rnorm;
+        // z0 = M(r0), then Jacobi instead if r0'z0 <= 0
+        mpnn_apply_bf16(net, A, r.p, z.p, rnorm, S.p, nullptr, part.p, nullptr, st);
+        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, S.p, 0);
+        hipLaunchKernelGGL(k_fcg_jacobi, gr, bv, 0, st, A.invdiag.p, r.p, (const double *)nullptr, z.p, n, part.p,
+                           part2.p, S.p);
+        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, S.p, 1);
+        MG_HIP(hipMemcpyAsync(p.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        const int maxit = max_it > 0 ? max_it : n;
+        const int check_every = 16;
+        const int stall = stall_window > 0 ? stall_window : 50;
+        for (int it = 1; it <= maxit; ++it) {
+            launch_spmv_guard(A, p.p, ap.p, S.p, st);
+            hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, p.p, ap.p, n, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
+            hipLaunchKernelGGL(k_pcg_update, gr, bv, 0, st, d_x, r.p, p.p, ap.p, n, part.p, S.p);
+            hipLaunchKernelGGL(k_fcg_norm, dim3(1), bv, 0, st, part.p, it, stall, S.p);
+            mpnn_apply_bf16(net, A, r.p, z.p, rnorm, S.p, ap.p, part.p, part2.p, st);
+            hipLaunchKernelGGL(k_fcg_jacobi, gr, bv, 0, st, A.invdiag.p, r.p, ap.p, z.p, n, part.p, part2.p, S.p);
+            hipLaunchKernelGGL(k_fcg_beta, dim3(1), bv, 0, st, part.p, part2.p, S.p);
+            hipLaunchKernelGGL(k_pcg_pupdate, gv, bv, 0, st, z.p, p.p, n, S.p);
+            if (it % check_every == 0 || it == maxit) {
+                MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+                MG_HIP(hipStreamSynchronize(st));
+                if (h.done) break;
+            }
+        }
+        MG_HIP(hipGetLastError());
+        MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipStreamSynchronize(st));
+        if (h.fail)
+            throw Error(MG_E_NOT_PD, "p^T A p <= 0. The matrix is not positive definite, so the CG method cannot be applied.");
+        if (!h.done) h.iters = maxit;
+        fell_back = h.fallback;
     }
     res.iterations = h.iters;
     res.residual_norm = h.rnorm;

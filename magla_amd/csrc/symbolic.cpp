@@ -6,9 +6,14 @@
 // states it; the data layout and code organisation are this library's own.
 // Built with -fwrapv: the reference's int arithmetic wraps on overflow.
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <exception>
 #include <numeric>
+#include <thread>
 
 #include "internal.h"
 
@@ -685,6 +690,98 @@ void analyze(int n, const int *rowptr, const int *colidx, const mg_options &opts
     int64_t strict = 0;
     for (int j = 0; j < n; ++j) strict += S.colcount[j];
     S.strict_nonzeros = strict;
+}
+
+}  // namespace mg
+
+// ---- batched systems ---------------------------------------------------------------
+
+namespace mg {
+
+void analyze_batch(int count, const int *sizes, const int *const *rowptrs, const int *const *colidxs,
+                   const mg_options &opts, std::vector<Symbolic> &each) {
+    each.clear();
+    each.resize(count);
+    int threads = opts.max_degree_of_parallelism > 0 ? opts.max_degree_of_parallelism
+                                                      : (int)std::thread::hardware_concurrency();
+    threads = std::max(1, std::min(threads, count));
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> errors(threads);
+    auto work = [&](int tid) {
+        try {
+            for (int t = next++; t < count; t = next++) analyze(sizes[t], rowptrs[t], colidxs[t], opts, each[t]);
+        } catch (...) {
+            errors[tid] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) pool.emplace_back(work, k);
+    work(0);
+    for (auto &th : pool) th.join();
+    for (auto &e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
+void merge_symbolic(const std::vector<Symbolic> &each, Symbolic &S) {
+    S = Symbolic();
+    const int count = (int)each.size();
+    int64_t n = 0, nnz = 0, ns = 0;
+    for (const auto &t : each) {
+        n += t.n;
+        nnz += t.matrix_nnz;
+        ns += t.nsuper;
+    }
+    if (n > INT32_MAX || nnz > INT32_MAX) throw Error(MG_E_TOO_DENSE, "The batch exceeds 2^31 rows or entries.");
+    S.n = (int)n;
+    S.matrix_nnz = (int)nnz;
+    S.nsuper = (int)ns;
+    if (count) S.opts = each[0].opts;
+    S.sys_begin.assign(count + 1, 0);
+    S.a_rowptr.assign(1, 0);
+    S.pptr.assign(1, 0);
+    S.lvptr.assign(1, 0);
+    S.first.assign(1, 0);
+    S.child_ptr.assign(1, 0);
+    S.pat_ptr.assign(1, 0);
+    const auto shift = [](std::vector<int> &dst, const std::vector<int> &src, size_t from, int64_t by, bool keep_neg) {
+        for (size_t k = from; k < src.size(); ++k) dst.push_back(keep_neg && src[k] < 0 ? src[k] : (int)(src[k] + by));
+    };
+    int64_t r0 = 0, v0 = 0, s0 = 0;
+    for (int t = 0; t < count; ++t) {
+        const Symbolic &T = each[t];
+        S.sys_begin[t] = (int)r0;
+        const int64_t p0 = S.pptr.back(), l0 = S.lvptr.back(), c0 = S.child_ptr.back(), q0 = S.pat_ptr.back();
+        shift(S.a_rowptr, T.a_rowptr, 1, v0, false);
+        shift(S.a_colidx, T.a_colidx, 0, r0, false);
+        shift(S.fill_order, T.fill_order, 0, r0, false);
+        shift(S.perm, T.perm, 0, r0, false);
+        shift(S.iperm, T.iperm, 0, r0, false);
+        shift(S.etree, T.etree, 0, r0, true);
+        S.colcount.insert(S.colcount.end(), T.colcount.begin(), T.colcount.end());
+        shift(S.pptr, T.pptr, 1, p0, false);
+        shift(S.prows, T.prows, 0, r0, false);
+        shift(S.lower_start, T.lower_start, 0, p0, false);
+        shift(S.lvptr, T.lvptr, 1, l0, false);
+        shift(S.lvidx, T.lvidx, 0, v0, false);
+        shift(S.dvidx, T.dvidx, 0, v0, true);
+        shift(S.first, T.first, 1, r0, false);
+        shift(S.super_of, T.super_of, 0, s0, false);
+        shift(S.sparent, T.sparent, 0, s0, true);
+        shift(S.child_ptr, T.child_ptr, 1, c0, false);
+        shift(S.children, T.children, 0, s0, false);
+        shift(S.pat_ptr, T.pat_ptr, 1, q0, false);
+        shift(S.pat_rows, T.pat_rows, 0, r0, false);
+        S.factor_entries += T.factor_entries;
+        S.strict_nonzeros += T.strict_nonzeros;
+        S.factor_ops += T.factor_ops;
+        S.peak_contribution = std::max(S.peak_contribution, T.peak_contribution);
+        S.max_front = std::max(S.max_front, T.max_front);
+        S.max_width = std::max(S.max_width, T.max_width);
+        r0 += T.n;
+        v0 += T.matrix_nnz;
+        s0 += T.nsuper;
+    }
+    S.sys_begin[count] = (int)r0;
 }
 
 }  // namespace mg

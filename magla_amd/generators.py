@@ -170,3 +170,63 @@ def config_c3(m=40, k=8):
     b = g.standard_normal((k, n))
     b[:, clamped] = 0.0
     return n, r, c, v, clamped, np.asfortranarray(b.T)
+
+
+def gaussian_random_field(m, seed, length=0.1, variance=2.0, embed=2):
+    """Stationary Gaussian random field on the m^3 interior nodes of the unit cube (spacing
+    h = 1/(m+1), node (i,j,k) at ((i+1)h, (j+1)h, (k+1)h)) with covariance
+    variance * exp(-|x-y|^2 / (2 length^2)), by circulant embedding on an (embed*m)^3 periodic
+    grid of the same spacing: g = IFFT(sqrt(lambda) * FFT(w)) with w white N(0,1) and lambda the
+    FFT of the periodic covariance row (negative eigenvalues clipped to 0); the m^3 window is kept.
+    Returned as (m, m, m) indexed [z, y, x]."""
+    h = 1.0 / (m + 1)
+    M = embed * m
+    d = np.minimum(np.arange(M), M - np.arange(M)) * h
+    r2 = d[:, None, None] ** 2 + d[None, :, None] ** 2 + d[None, None, :] ** 2
+    row = variance * np.exp(-r2 / (2.0 * length * length))
+    lam = np.maximum(np.fft.fftn(row).real, 0.0)
+    w = rng(seed).standard_normal((M, M, M))
+    g = np.fft.ifftn(np.sqrt(lam) * np.fft.fftn(w)).real
+    return g[:m, :m, :m].copy()
+
+
+def diffusion_3d_grf(m, seed, length=0.1, variance=2.0):
+    """Config C5 matrix: 3D 7-point diffusion on m^3 interior nodes with node coefficients
+    kappa = exp(GRF); interior face conductivity = harmonic mean of its two node kappas; a
+    Dirichlet boundary face takes the kappa of its node."""
+    kap = np.exp(gaussian_random_field(m, seed, length, variance))
+    hm = lambda a, b: 2.0 * a * b / (a + b)
+    kx = hm(kap[:, :, :-1], kap[:, :, 1:])
+    ky = hm(kap[:, :-1, :], kap[:, 1:, :])
+    kz = hm(kap[:-1, :, :], kap[1:, :, :])
+    kb = np.zeros((m, m, m))
+    for axis in range(3):
+        for side in (0, -1):
+            sl = [slice(None)] * 3
+            sl[axis] = side
+            kb[tuple(sl)] += kap[tuple(sl)]
+    return diffusion_3d_faces(m, kx, ky, kz, kb)
+
+
+def config_c5(m=64):
+    """Learned-preconditioner config: 3D 7-point diffusion, m^3 (64^3: n = 262,144), kappa = exp(GRF,
+    length 0.1, variance 2); test right-hand side uniform[-1,1]."""
+    n, r, c, v = diffusion_3d_grf(m, SEED_BASE + 5)
+    return n, r, c, v, uniform_rhs(n, 1, SEED_BASE + 5 + 1000)
+
+
+def config_c4(count=1024, seed=SEED_BASE + 4, side_lo=16, side_hi=128, max_rhs=32):
+    """Batched config: `count` independent SPD systems, each a 2D 5-point or 9-point Laplacian (p = 1/2)
+    on an s x s grid, s ~ U{side_lo..side_hi}, plus a diagonal shift ~ U[0, 0.1]; 1..max_rhs
+    right-hand sides each, uniform[-1,1].  Returns a list of (n, rows, cols, vals, b (n, k))."""
+    g = rng(seed)
+    out = []
+    for t in range(count):
+        s = int(g.integers(side_lo, side_hi + 1))
+        nine = bool(g.random() < 0.5)
+        shift = float(0.1 * g.random())
+        k = int(g.integers(1, max_rhs + 1))
+        n, r, c, v = laplacian_2d(s, shift, nine)
+        b = 2.0 * g.random((k, n)) - 1.0
+        out.append((n, r, c, v, np.asfortranarray(b.T)))
+    return out

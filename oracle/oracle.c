@@ -1343,3 +1343,196 @@ done:
     free(r); free(z); free(p); free(ap); free(invd);
     return rc;
 }
+
+/* ---- learned preconditioner network (no reference counterpart; see oracle.h) ---- */
+
+typedef struct {
+    int n;
+    int *ptr, *nbr;   /* full neighbour lists without the diagonal */
+    double *ew;       /* a_ij / sqrt(a_ii a_jj) */
+    double *diag, *cf;/* a_ii, log(a_ii / mean) */
+} mp_graph;
+
+static void mp_graph_free(mp_graph *g) {
+    free(g->ptr); free(g->nbr); free(g->ew); free(g->diag); free(g->cf);
+}
+
+static int mp_graph_build(const or_csr *a, mp_graph *g) {
+    const int n = a->n;
+    memset(g, 0, sizeof(*g));
+    g->n = n;
+    g->ptr = (int *)calloc((size_t)n + 1, sizeof(int));
+    g->diag = (double *)calloc((size_t)n + 1, sizeof(double));
+    g->cf = (double *)calloc((size_t)n + 1, sizeof(double));
+    if (!g->ptr || !g->diag || !g->cf) return OR_E_NOMEM;
+    for (int i = 0; i < n; ++i)
+        for (int k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
+            const int j = a->colidx[k];
+            if (j == i) g->diag[i] = a->values[k];
+            else { g->ptr[i + 1]++; g->ptr[j + 1]++; }
+        }
+    for (int i = 0; i < n; ++i) g->ptr[i + 1] += g->ptr[i];
+    const int m = g->ptr[n];
+    g->nbr = (int *)malloc(sizeof(int) * (size_t)(m + 1));
+    g->ew = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    if (!g->nbr || !g->ew || !fill) { free(fill); return OR_E_NOMEM; }
+    memcpy(fill, g->ptr, sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
+            const int j = a->colidx[k];
+            if (j == i) continue;
+            const double w = a->values[k] / sqrt(g->diag[i] * g->diag[j]);
+            g->nbr[fill[i]] = j; g->ew[fill[i]++] = w;
+            g->nbr[fill[j]] = i; g->ew[fill[j]++] = w;
+        }
+    free(fill);
+    double mean = 0.0;
+    for (int i = 0; i < n; ++i) mean += g->diag[i];
+    mean = n ? mean / n : 1.0;
+    for (int i = 0; i < n; ++i) g->cf[i] = log(g->diag[i] / mean);
+    return OR_OK;
+}
+
+/* forward pass; keeps h_l (4 x n x W), m_l and pre_l (3 x n x W) when the buffers are given */
+static void mp_forward(const mp_graph *g, const double *P, const double *r, double *z, double *s_out, double *H,
+                       double *Mb, double *Pre) {
+    const int n = g->n, W = OR_MP_W;
+    double ss = 0.0;
+    for (int i = 0; i < n; ++i) ss += r[i] * r[i];
+    const double s = sqrt(ss / (n ? n : 1));
+    *s_out = s;
+    const double *we = P, *wc = P + W, *wd = P + 2 * W + 6 * W * W;
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int f = 0; f < W; ++f) H[(size_t)i * W + f] = r[i] * inv * (we[f] + g->cf[i] * wc[f]);
+    for (int l = 0; l < 3; ++l) {
+        const double *W1 = P + 2 * W + 2 * l * W * W, *W2 = W1 + W * W;
+        const double *h = H + (size_t)l * n * W;
+        double *ho = H + (size_t)(l + 1) * n * W, *m = Mb + (size_t)l * n * W, *pre = Pre + (size_t)l * n * W;
+        for (int i = 0; i < n; ++i) {
+            for (int f = 0; f < W; ++f) {
+                double acc = 0.0;
+                for (int k = g->ptr[i]; k < g->ptr[i + 1]; ++k) acc += g->ew[k] * h[(size_t)g->nbr[k] * W + f];
+                m[(size_t)i * W + f] = acc;
+            }
+            for (int f = 0; f < W; ++f) {
+                double acc = 0.0;
+                for (int q = 0; q < W; ++q) acc += W1[f * W + q] * h[(size_t)i * W + q] + W2[f * W + q] * m[(size_t)i * W + q];
+                pre[(size_t)i * W + f] = acc;
+                ho[(size_t)i * W + f] = (l < 2 && !(acc > 0.0)) ? 0.0 : acc;
+            }
+        }
+    }
+    const double *h3 = H + (size_t)3 * n * W;
+    for (int i = 0; i < n; ++i) {
+        double y = 0.0;
+        for (int f = 0; f < W; ++f) y += wd[f] * h3[(size_t)i * W + f];
+        z[i] = (r[i] + s * y) / g->diag[i];
+    }
+}
+
+int or_mpnn_apply(const or_csr *a, const double *params, const double *r, double *z) {
+    if (!a || !params || !r || !z) return OR_E_NULL;
+    mp_graph g;
+    if (mp_graph_build(a, &g)) { mp_graph_free(&g); return OR_E_NOMEM; }
+    const size_t nw = (size_t)a->n * OR_MP_W;
+    double *H = (double *)malloc(sizeof(double) * (4 * nw + 1));
+    double *Mb = (double *)malloc(sizeof(double) * (3 * nw + 1));
+    double *Pre = (double *)malloc(sizeof(double) * (3 * nw + 1));
+    int rc = OR_E_NOMEM;
+    if (H && Mb && Pre) {
+        double s;
+        mp_forward(&g, params, r, z, &s, H, Mb, Pre);
+        rc = OR_OK;
+    }
+    free(H); free(Mb); free(Pre);
+    mp_graph_free(&g);
+    return rc;
+}
+
+int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const double *R, const double *E,
+                      double *loss, double *grad) {
+    if (!a || !params || !R || !E || !loss || !grad) return OR_E_NULL;
+    const int n = a->n, W = OR_MP_W;
+    mp_graph g;
+    if (mp_graph_build(a, &g)) { mp_graph_free(&g); return OR_E_NOMEM; }
+    const size_t nw = (size_t)n * W;
+    double *H = (double *)malloc(sizeof(double) * (4 * nw + 1));
+    double *Mb = (double *)malloc(sizeof(double) * (3 * nw + 1));
+    double *Pre = (double *)malloc(sizeof(double) * (3 * nw + 1));
+    double *dH = (double *)malloc(sizeof(double) * (nw + 1));
+    double *dHp = (double *)malloc(sizeof(double) * (nw + 1));
+    double *dM = (double *)malloc(sizeof(double) * (nw + 1));
+    double *z = (double *)malloc(sizeof(double) * ((size_t)n + 1));
+    double *az = (double *)malloc(sizeof(double) * ((size_t)n + 1));
+    if (!H || !Mb || !Pre || !dH || !dHp || !dM || !z || !az) {
+        free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(z); free(az);
+        mp_graph_free(&g);
+        return OR_E_NOMEM;
+    }
+    memset(grad, 0, sizeof(double) * OR_MP_P);
+    double total = 0.0;
+    double *gwe = grad, *gwc = grad + W, *gwd = grad + 2 * W + 6 * W * W;
+    const double *wd = params + 2 * W + 6 * W * W, *we = params, *wc = params + W;
+    for (int t = 0; t < nsamples; ++t) {
+        const double *r = R + (size_t)t * n, *e = E + (size_t)t * n;
+        double s;
+        mp_forward(&g, params, r, z, &s, H, Mb, Pre);
+        or_spmv(a, z, az);
+        double zaz = 0.0, zr = 0.0, re = 0.0;
+        for (int i = 0; i < n; ++i) { zaz += z[i] * az[i]; zr += z[i] * r[i]; re += r[i] * e[i]; }
+        total += (zaz - 2.0 * zr) / re + 1.0;
+        /* dL/dz = 2 (Az - r) / (r'e) / nsamples; dy_i = dz_i s / a_ii */
+        const double *h3 = H + (size_t)3 * n * W;
+        for (int i = 0; i < n; ++i) {
+            const double dz = 2.0 * (az[i] - r[i]) / re / nsamples;
+            const double dy = dz * s / g.diag[i];
+            for (int f = 0; f < W; ++f) {
+                gwd[f] += dy * h3[(size_t)i * W + f];
+                dH[(size_t)i * W + f] = dy * wd[f];
+            }
+        }
+        for (int l = 2; l >= 0; --l) {
+            const double *W1 = params + 2 * W + 2 * l * W * W, *W2 = W1 + W * W;
+            double *gW1 = grad + 2 * W + 2 * l * W * W, *gW2 = gW1 + W * W;
+            const double *h = H + (size_t)l * n * W, *m = Mb + (size_t)l * n * W, *pre = Pre + (size_t)l * n * W;
+            for (int i = 0; i < n; ++i) {
+                double dp[OR_MP_W];
+                for (int f = 0; f < W; ++f) {
+                    const double p = pre[(size_t)i * W + f];
+                    dp[f] = (l < 2 && !(p > 0.0)) ? 0.0 : dH[(size_t)i * W + f];
+                }
+                for (int f = 0; f < W; ++f)
+                    for (int q = 0; q < W; ++q) {
+                        gW1[f * W + q] += dp[f] * h[(size_t)i * W + q];
+                        gW2[f * W + q] += dp[f] * m[(size_t)i * W + q];
+                    }
+                for (int q = 0; q < W; ++q) {
+                    double a1 = 0.0, a2 = 0.0;
+                    for (int f = 0; f < W; ++f) { a1 += W1[f * W + q] * dp[f]; a2 += W2[f * W + q] * dp[f]; }
+                    dHp[(size_t)i * W + q] = a1;
+                    dM[(size_t)i * W + q] = a2;
+                }
+            }
+            /* transpose of the aggregation (ew symmetric): dh_j += sum_i ew_ij dm_i */
+            for (int i = 0; i < n; ++i)
+                for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
+                    for (int q = 0; q < W; ++q) dHp[(size_t)i * W + q] += g.ew[k] * dM[(size_t)g.nbr[k] * W + q];
+            memcpy(dH, dHp, sizeof(double) * nw);
+        }
+        const double inv = s > 0.0 ? 1.0 / s : 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double u = r[i] * inv;
+            for (int f = 0; f < W; ++f) {
+                gwe[f] += u * dH[(size_t)i * W + f];
+                gwc[f] += u * g.cf[i] * dH[(size_t)i * W + f];
+            }
+        }
+        (void)we; (void)wc;
+    }
+    *loss = nsamples ? total / nsamples : 0.0;
+    free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(z); free(az);
+    mp_graph_free(&g);
+    return OR_OK;
+}

@@ -134,6 +134,26 @@ int or_pcg(const or_csr *a, const double *b, const double *x0, double tol,
 
 double or_norm2(int n, const double *v);
 
+/* ---- learned preconditioner network (checker for the HIP training/inference kernels) ----
+ * The reference has no learned component: these functions restate the network that this
+ * project defines (three message-passing layers of width 32 on the matrix graph, a
+ * correction on top of Jacobi), in binary64, so that the GPU forward pass and gradients
+ * can be checked.  Parameter layout, OR_MP_P values:
+ *   w_e[32] | w_c[32] | for l = 0..2: W1_l[32][32], W2_l[32][32] (row f, column g) | w_d[32]
+ * Node inputs: u_i = r_i / s with s = sqrt(sum r^2 / n), c_i = log(a_ii / mean a);
+ * edge weights ew_ij = a_ij / sqrt(a_ii a_jj) (j != i).
+ *   h0_i = u_i (w_e + c_i w_c)
+ *   m_i = sum_j ew_ij h_j ;  h'_i = act(W1_l h_i + W2_l m_i), act = ReLU for l < 2, identity for l = 2
+ *   y_i = w_d . h3_i ;      z_i = (r_i + s y_i) / a_ii
+ * Loss of one (r, e = A^{-1} r) pair: the squared A-norm error of z relative to e,
+ *   L = (z - e)' A (z - e) / (e' A e) = (z'Az - 2 z'r) / (r'e) + 1. */
+#define OR_MP_W 32
+#define OR_MP_P (2 * OR_MP_W + 3 * 2 * OR_MP_W * OR_MP_W + OR_MP_W)
+int or_mpnn_apply(const or_csr *a, const double *params, const double *r, double *z);
+/* mean loss over nsamples pairs (columns of R and E, length n each) and its gradient */
+int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const double *R, const double *E,
+                      double *loss, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

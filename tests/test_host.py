@@ -168,3 +168,18 @@ def test_elasticity_generator_shape():
         assert np.abs(Ad @ t).max() < 1e-12 * np.abs(Ad).max()
     assert np.all(np.linalg.eigvalsh(Ad) > -1e-10)
     assert pm.nonzero_count == oc.nnz
+
+
+def test_batch_layout_and_costs_are_sums_of_systems():
+    sysl = G.config_c4(16, seed=3, side_lo=5, side_hi=20, max_rhs=3)
+    mats = [M.SymmetricCSRMatrix.from_coo(n, r, c, v) for n, r, c, v, b in sysl]
+    batch = M.UtduBatch(mats)
+    info = batch.info()
+    each = [M.SymmetricUtduSolver().analyze(m).info() for m in mats]
+    assert info["size"] == sum(e["size"] for e in each)
+    assert info["supernode_count"] == sum(e["supernode_count"] for e in each)
+    assert info["factor_entry_count"] == sum(e["factor_entry_count"] for e in each)
+    assert info["factor_operation_count"] == sum(e["factor_operation_count"] for e in each)
+    assert info["max_front_size"] == max(e["max_front_size"] for e in each)
+    np.testing.assert_array_equal(batch.row_begin, np.concatenate([[0], np.cumsum([m.size for m in mats])]))
+    np.testing.assert_array_equal(batch.nnz_begin, np.concatenate([[0], np.cumsum([m.nonzero_count for m in mats])]))
