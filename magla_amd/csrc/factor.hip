@@ -32,6 +32,9 @@ constexpr int kTile = 64;   // tile edge of assembly / Schur tiles
 constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag / panel_rows pass
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
+constexpr int kBig128 = 1 << 30;      // fronts this large take 128x128 Schur tiles (off: slower here)
+constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
+typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
 constexpr int kFootChunk = 1024;      // footer rows per partial product in the backward solve
 constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
 static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread each (256 threads)");
@@ -56,6 +59,7 @@ struct DevicePlan {
     std::vector<int64_t> lvl_asm_ptr;
     std::vector<int> lvl_npanels;
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_upd_ptr, lvl_diag_ptr;
+    std::vector<std::vector<int>> lvl_upd_mid;  // per panel: first 128-tile task (64-tile tasks before it)
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
@@ -305,7 +309,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_npanels.push_back(npan);
         P->lvl_pull_elems.push_back(pe_elems);
         P->lvl_pull_index.push_back(pe_index);
-        std::vector<int> pp{(int)panel.size()}, up{(int)upd.size()}, dp{(int)diag_nodes.size()};
+        std::vector<int> pp{(int)panel.size()}, up{(int)upd.size()}, dp{(int)diag_nodes.size()}, umid;
         std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
         for (int q = 0; q < npan; ++q) {
             const int pbase = (int)panel.size();
@@ -322,22 +326,29 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             P->max_panel_tasks = std::max(P->max_panel_tasks, (int)panel.size() - pbase);
             dp.push_back((int)diag_nodes.size());
             pp.push_back((int)panel.size());
-            for (int g : bylevel[L]) {
-                const int f = P->h_f[g], w = P->h_w[g];
-                if (w <= q * kPanel) continue;
-                const SchurStep st = schur_step(f, w, q);
-                if (!st.any) continue;
-                for (int tj = 0; tj < st.col_tiles; ++tj)
-                    for (int ti = tj; ti < st.row_tiles; ++ti) upd.push_back(make_int4(g, ti, tj, 0));
-                double elems = 0;
-                for (int j = st.base; j < st.cend; ++j) elems += f - j;
-                ufl[q] += 2.0 * (st.ke - st.kb) * elems;
-                uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+            // 64x64 tiles for fronts below kBig128, then 128x128 tiles for the large ones
+            for (int big = 0; big < 2; ++big) {
+                for (int g : bylevel[L]) {
+                    const int f = P->h_f[g], w = P->h_w[g];
+                    if (w <= q * kPanel || (f >= kBig128) != (big == 1)) continue;
+                    const SchurStep st = schur_step(f, w, q);
+                    if (!st.any) continue;
+                    const int tile = big ? 128 : kTile;
+                    const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
+                    for (int tj = 0; tj < ct; ++tj)
+                        for (int ti = tj; ti < rt; ++ti) upd.push_back(make_int4(g, ti, tj, big));
+                    double elems = 0;
+                    for (int j = st.base; j < st.cend; ++j) elems += f - j;
+                    ufl[q] += 2.0 * (st.ke - st.kb) * elems;
+                    uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+                }
+                if (big == 0) umid.push_back((int)upd.size());
             }
             up.push_back((int)upd.size());
         }
         P->lvl_panel_ptr.push_back(pp);
         P->lvl_upd_ptr.push_back(up);
+        P->lvl_upd_mid.push_back(umid);
         P->lvl_diag_ptr.push_back(dp);
         P->lvl_upd_flops.push_back(ufl);
         P->lvl_upd_elems.push_back(uel);
@@ -749,23 +760,26 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
     const int i0 = base + tk.y * kTile, j0 = base + tk.z * kTile;
     T *M = arena + off[s];
     const double *Dp = D + firstcol[s];
-    // loader mapping: lane r = row of the tile, wave kq = k offset (k = kq + 4j)
-    const int r = threadIdx.x & 63, kq = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const bool ia = i0 + r < f, ib = j0 + r < f;
-    const int rowa = min(i0 + r, f - 1), rowb = min(j0 + r, f - 1);
+    // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
+    // pass (k = cq + CPI j).  The vector start is clamped into the front (reads past a
+    // column's end land in the next column or the arena's tail padding and are masked).
+    constexpr int VEC = 16 / sizeof(T), TPC = kTile / VEC, CPI = 256 / TPC, NJ = kPanel / CPI;
+    const int qv = threadIdx.x % TPC, cq = threadIdx.x / TPC;
+    const int ra0 = i0 + VEC * qv, rb0 = j0 + VEC * qv;
+    const int rowa = min(ra0, f - 1), rowb = min(rb0, f - 1);
     Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
-    T ra[kPanel / 4], rb[kPanel / 4];
-    // Chunk kc of K into registers; all loads are unconditional (addresses clamped
-    // into the front) and issued before any is consumed, so the next chunk is in
-    // flight while the current one runs on the MFMA units.
+    T ra[NJ][VEC], rb[NJ][VEC];
+    // Chunk kc of K into registers; all loads are unconditional and issued before any is
+    // consumed, so the next chunk is in flight while the current one runs on the MFMA units.
     auto fetch = [&](int kc) {
         const int nk = min(kPanel, ke - kc);
 #pragma unroll
-        for (int j = 0; j < kPanel / 4; ++j) {
-            const int k = kq + 4 * j;
-            const T *col = M + (int64_t)(kc + min(k, nk - 1)) * f;
-            ra[j] = col[rowa];
-            rb[j] = col[rowb];
+        for (int j = 0; j < NJ; ++j) {
+            const T *col = M + (int64_t)(kc + min(cq + CPI * j, nk - 1)) * f;
+            const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + rowa);
+            const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + rowb);
+            __builtin_memcpy(ra[j], &va, 16);
+            __builtin_memcpy(rb[j], &vb, 16);
         }
     };
     typename MfmaAcc<T>::type acc[2][2];
@@ -775,16 +789,74 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
         const int nk = min(kPanel, ke - kc);
         __syncthreads();  // the previous chunk's MFMA reads are done
 #pragma unroll
-        for (int j = 0; j < kPanel / 4; ++j) {
-            const int k = kq + 4 * j;
-            As[k][r] = (k < nk && ia) ? ra[j] * Ds[min(kc - kb + k, kBlock - 1)] : T(0);
-            Bs[k][r] = (k < nk && ib) ? rb[j] : T(0);
+        for (int j = 0; j < NJ; ++j) {
+            const int k = cq + CPI * j;
+            const T dk = Ds[min(kc - kb + k, kBlock - 1)];
+#pragma unroll
+            for (int i = 0; i < VEC; ++i) {
+                As[k][VEC * qv + i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
+                Bs[k][VEC * qv + i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+            }
         }
         __syncthreads();
         if (kc + kPanel < ke) fetch(kc + kPanel);
         schur_acc_tile<T>(As, Bs, acc);
     }
     schur_store_tile<T>(acc, M, f, i0, j0, cend);
+}
+
+// The same update with 128x128 output tiles (large fronts): K streamed through LDS in chunks
+// of 32, each thread stages rows r = tid % 128 of k = tid / 128 + 2j, j < 16.
+template <typename T>
+__global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks, T *__restrict__ arena,
+                                                  const int *__restrict__ F, const int *__restrict__ W,
+                                                  const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                                  int q, const double *__restrict__ D) {
+    constexpr int KC = 32;
+    __shared__ T As[KC][kT128Pad];  // As[k][r] = L[i0+r, kc+k] * d_k
+    __shared__ T Bs[KC][kT128Pad];  // Bs[k][r] = L[j0+r, kc+k]
+    __shared__ T Ds[kBlock];
+    const int4 tk = tasks[blockIdx.x];
+    const int s = tk.x;
+    const int f = F[s], w = W[s];
+    const int ps = q * kPanel;
+    const int pe = min(ps + kPanel, w), bs = (ps / kBlock) * kBlock, be = min(bs + kBlock, w);
+    const bool inner = pe < be;
+    const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
+    const int i0 = base + tk.y * 128, j0 = base + tk.z * 128;
+    T *M = arena + off[s];
+    const double *Dp = D + firstcol[s];
+    const int r = threadIdx.x & 127, kh = threadIdx.x >> 7;
+    const bool ia = i0 + r < f, ib = j0 + r < f;
+    const int rowa = min(i0 + r, f - 1), rowb = min(j0 + r, f - 1);
+    Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
+    T ra[KC / 2], rb[KC / 2];
+    auto fetch = [&](int kc) {
+        const int nk = min(KC, ke - kc);
+#pragma unroll
+        for (int j = 0; j < KC / 2; ++j) {
+            const T *col = M + (int64_t)(kc + min(kh + 2 * j, nk - 1)) * f;
+            ra[j] = col[rowa];
+            rb[j] = col[rowb];
+        }
+    };
+    typename MfmaAcc<T>::type acc[4][4];
+    schur128_acc_zero<T>(acc);
+    fetch(kb);
+    for (int kc = kb; kc < ke; kc += KC) {
+        const int nk = min(KC, ke - kc);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KC / 2; ++j) {
+            const int k = kh + 2 * j;
+            As[k][r] = (k < nk && ia) ? ra[j] * Ds[min(kc - kb + k, kBlock - 1)] : T(0);
+            Bs[k][r] = (k < nk && ib) ? rb[j] : T(0);
+        }
+        __syncthreads();
+        if (kc + KC < ke) fetch(kc + KC);
+        schur128_acc_tile<T>(As, Bs, acc);
+    }
+    schur128_store_tile<T>(acc, M, f, i0, j0, cend);
 }
 
 // Deterministic statistics over D and the pivot flags (one workgroup).
@@ -1319,9 +1391,11 @@ struct Numeric {
     DevArray<double> wsR, wsX, wsPart, wsRel;
     int ws_nrhs = 0;
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
+    hipStream_t cap_stream = nullptr;
     int graph_precision = -1;
     ~Numeric() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 };
 
@@ -1457,11 +1531,15 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                 hipLaunchKernelGGL(k_panel_rows<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.diag.p);
             }
-            const int u0 = P.lvl_upd_ptr[L][q], u1 = P.lvl_upd_ptr[L][q + 1];
+            const int u0 = P.lvl_upd_ptr[L][q], um = P.lvl_upd_mid[L][q], u1 = P.lvl_upd_ptr[L][q + 1];
             if (u1 > u0) {
                 ProfScope ps_(st, KP_SCHUR, P.lvl_upd_flops[L][q], e * P.lvl_upd_elems[L][q]);
-                hipLaunchKernelGGL(k_schur<T>, dim3(u1 - u0), dim3(256), 0, st, P.upd_tasks.p + u0, arena, P.f.p, P.w.p,
-                                   P.off.p, P.first.p, q, N.diag.p);
+                if (um > u0)
+                    hipLaunchKernelGGL(k_schur<T>, dim3(um - u0), dim3(256), 0, st, P.upd_tasks.p + u0, arena, P.f.p,
+                                       P.w.p, P.off.p, P.first.p, q, N.diag.p);
+                if (u1 > um)
+                    hipLaunchKernelGGL(k_schur128<T>, dim3(u1 - um), dim3(256), 0, st, P.upd_tasks.p + um, arena,
+                                       P.f.p, P.w.p, P.off.p, P.first.p, q, N.diag.p);
             }
         }
     }
@@ -1489,7 +1567,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->sym = &S;
             N->precision = precision;
             N->n = S.n;
-            N->arena.alloc((size_t)S.plan->arena * esz);
+            N->arena.alloc((size_t)(S.plan->arena + kArenaPad) * esz);
             N->diag.alloc(std::max(S.n, 1));
             N->flags.alloc(std::max(S.n, 1));
             N->counts.alloc(4);
@@ -1509,36 +1587,41 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             if (graph && N->gexec && N->graph_precision == precision) {
                 MG_HIP(hipGraphLaunch(N->gexec, st));
             } else {
+                // capture on a private stream (the null stream cannot be captured); the graph
+                // is then launched on the caller's stream
+                hipStream_t cs = st;
                 if (graph) {
                     if (N->gexec) {
                         (void)hipGraphExecDestroy(N->gexec);
                         N->gexec = nullptr;
                     }
-                    MG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    if (!N->cap_stream) MG_HIP(hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking));
+                    cs = N->cap_stream;
+                    MG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
                 }
                 try {
                     if (S.plan->nsys > 1) {
-                        hipLaunchKernelGGL(k_diag_scale_seg, dim3(S.plan->nsys), dim3(256), 0, st, N->A.dpos.p,
+                        hipLaunchKernelGGL(k_diag_scale_seg, dim3(S.plan->nsys), dim3(256), 0, cs, N->A.dpos.p,
                                            N->A.vals.p, S.plan->sys_begin.p, N->scale.p);
                     } else {
-                        hipLaunchKernelGGL(k_diag_scale_part, dim3(kScaleParts), dim3(256), 0, st, N->A.dpos.p,
+                        hipLaunchKernelGGL(k_diag_scale_part, dim3(kScaleParts), dim3(256), 0, cs, N->A.dpos.p,
                                            N->A.vals.p, S.n, N->wsScale.p);
-                        hipLaunchKernelGGL(k_diag_scale_final, dim3(1), dim3(256), 0, st, N->wsScale.p, kScaleParts,
+                        hipLaunchKernelGGL(k_diag_scale_final, dim3(1), dim3(256), 0, cs, N->wsScale.p, kScaleParts,
                                            N->scale.p);
                     }
-                    if (precision == MG_FP32) run_factor<float>(*N, opts, st);
-                    else run_factor<double>(*N, opts, st);
+                    if (precision == MG_FP32) run_factor<float>(*N, opts, cs);
+                    else run_factor<double>(*N, opts, cs);
                 } catch (...) {
                     if (graph) {
                         hipGraph_t g = nullptr;
-                        (void)hipStreamEndCapture(st, &g);
+                        (void)hipStreamEndCapture(cs, &g);
                         if (g) (void)hipGraphDestroy(g);
                     }
                     throw;
                 }
                 if (graph) {
                     hipGraph_t g = nullptr;
-                    MG_HIP(hipStreamEndCapture(st, &g));
+                    MG_HIP(hipStreamEndCapture(cs, &g));
                     const hipError_t ie = hipGraphInstantiate(&N->gexec, g, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(g);
                     MG_HIP(ie);

@@ -124,4 +124,82 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
             }
 }
 
+// ---- 128x128 tiles (large fronts) --------------------------------------------------
+//
+// Four waves as 2x2; wave v owns the 64x64 quadrant (rows 64*(v&1), cols 64*(v>>1)) as
+// 4x4 blocks of 16x16 MFMA tiles, so every operand read from LDS feeds four MFMAs.
+// LDS chunks hold 32 k-rows of 128 (+16 pad) elements.
+constexpr int kT128Pad = 128 + 16;
+
+template <typename T>
+__device__ inline void schur128_acc_zero(typename MfmaAcc<T>::type (&acc)[4][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[a][b][q] = T(0);
+}
+
+template <typename T>
+__device__ inline void schur128_read_ops(const T (*As)[kT128Pad], const T (*Bs)[kT128Pad], int k0, T (&colv)[4],
+                                         T (&rowv)[4]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
+    const int k = k0 + (lane >> 4), l16 = lane & 15;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) colv[b] = Bs[k][cq + 16 * b + l16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rowv[a] = As[k][rq + 16 * a + l16];
+}
+
+// acc += over the 32 k-rows of the LDS chunk (rows past the chunk's length hold zeros),
+// operands of the next k-step read while the current one runs (software pipeline).
+template <typename T>
+__device__ inline void schur128_acc_tile(const T (*As)[kT128Pad], const T (*Bs)[kT128Pad],
+                                         typename MfmaAcc<T>::type (&acc)[4][4]) {
+    T colv[2][4], rowv[2][4];
+    schur128_read_ops<T>(As, Bs, 0, colv[0], rowv[0]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < 8) schur128_read_ops<T>(As, Bs, 4 * (s + 1), colv[cur ^ 1], rowv[cur ^ 1]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = MfmaAcc<T>::step(colv[cur][b], rowv[cur][a], acc[a][b]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// M[j*f + i] -= C[i][j] on the tile's lower entries with i < f, j < jend (reads first, then writes)
+template <typename T>
+__device__ inline void schur128_store_tile(const typename MfmaAcc<T>::type (&acc)[4][4], T *__restrict__ M, int f,
+                                           int i0, int j0, int jend) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
+    const int l16 = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        T old[4][4];
+        const int i = i0 + rq + 16 * a + l16;
+        const int ic = min(i, f - 1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
+                old[b][q] = M[(int64_t)min(j, ic) * f + ic];
+            }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
+                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[b][q] - acc[a][b][q];
+            }
+    }
+}
+
 }  // namespace mg
