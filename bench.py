@@ -152,7 +152,7 @@ def bench_learned(args):
     log(f"[bench] {args.pairs} pairs in {pairs_s:.1f}s")
     net = M.LearnedPreconditioner(seed=G.SEED_BASE + 5 + 3000)
     t0 = time.time()
-    losses = learned.train(net, dm, R, E, epochs=args.epochs, batch=16, learning_rate=args.lr,
+    losses = learned.train(net, dm, R, E, epochs=args.epochs, batch=16, learning_rate=args.lr, kappa=args.kappa,
                            seed=G.SEED_BASE + 5 + 4000)
     train_s = time.time() - t0
     log(f"[bench] trained {args.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
@@ -185,7 +185,8 @@ def bench_learned(args):
         "ms_per_step": round(msl, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "data": "synthetic",
         "config": {"workload": f"3D 7-point diffusion {args.grid}^3 (n={n}), kappa=exp(GRF l=0.1 var=2), "
-                               f"PCG to 1e-8; 3-layer MPNN width 32 trained on {args.pairs} (r, A^-1 r) pairs",
+                               f"PCG to 1e-8; 3-layer MPNN width 32 -> two-hop factor G, z = G'G r, trained on {args.pairs} "
+                               f"(r, A^-1 r) pairs (kappa {args.kappa})",
                    "pairs": args.pairs, "epochs": args.epochs},
         "jacobi_iterations": resj.iterations, "jacobi_ms": round(msj, 3),
         "learned_iterations": resl.iterations, "learned_ms": round(msl, 3),
@@ -231,7 +232,8 @@ def main():
     ap.add_argument("--grid", type=int, default=64, help="c5: grid edge")
     ap.add_argument("--pairs", type=int, default=2000, help="c5: training pairs")
     ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
-    ap.add_argument("--lr", type=float, default=2e-3, help="c5: Adam learning rate")
+    ap.add_argument("--lr", type=float, default=3e-3, help="c5: Adam learning rate")
+    ap.add_argument("--kappa", type=float, default=1.0, help="c5: weight of the Kaporin condition-number term")
     ap.add_argument("--snapshot-its", type=int, default=256, help="c5: largest Jacobi-PCG snapshot iteration")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -209,27 +209,30 @@ void mg_batch_release(mg_batch *batch);
 
 /* ---- learned preconditioner ----
  * No counterpart in the reference: a message-passing network on the matrix graph (three
- * layers of width 32) whose output corrects JacobiPreconditioner (CGGP/JacobiPreconditioner.cs:12-56)
- * inside a flexible PCG.  It is trained on the GPU from (residual, exact error) pairs that the
- * direct solver produces; inference uses BF16 weights with FP32 accumulation on MFMA.  The
- * solve falls back to plain Jacobi-PCG when the learned correction stalls. */
+ * layers of width 32) predicts a sparse lower-triangular factor G (columns within two hops),
+ * and z = G'G r corrects JacobiPreconditioner (CGGP/JacobiPreconditioner.cs:12-56; zero
+ * readouts give Jacobi) inside a flexible PCG.  It is trained on the GPU from (residual, exact
+ * error) pairs that the direct solver produces, with a Kaporin condition-number term of
+ * weight kappa; inference runs the network once per matrix with BF16 weights and FP32
+ * accumulation on MFMA.  The solve falls back to plain Jacobi-PCG when it stalls. */
 typedef struct mg_mpnn mg_mpnn;
 int32_t mg_mpnn_param_count(void);
 int mg_mpnn_create(uint64_t seed, mg_mpnn **out);
 void mg_mpnn_release(mg_mpnn *net);
 int mg_mpnn_get_params(const mg_mpnn *net, float *params);   /* FP32 master copy, param_count values */
 int mg_mpnn_set_params(mg_mpnn *net, const float *params);   /* also refreshes the BF16 copy */
-/* z = M(r) with the FP32 network (MG_FP32) or the BF16 MFMA network (MG_BF16) */
+/* z = G'G r with G from the FP32 network (MG_FP32) or the BF16 MFMA network (MG_BF16) */
 int mg_mpnn_apply_dev(mg_mpnn *net, mg_matrix *m, const double *d_r, double *d_z, int32_t precision,
                       void *stream);
 /* mean A-norm loss over ns pairs (columns of d_R = residuals and d_E = exact errors, leading
- * dimension ld) and its gradient (host array of param_count floats); no update */
+ * dimension ld) plus kappa times the log Kaporin number of G A G', and its gradient (host array
+ * of param_count floats); no update */
 int mg_mpnn_loss_grad_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E,
-                          int64_t ld, void *stream, double *loss, float *grad);
+                          int64_t ld, double kappa, void *stream, double *loss, float *grad);
 /* Adam training, `epochs` passes over the pairs in seeded shuffled mini-batches; epoch_losses
  * (may be NULL) receives the mean loss of each epoch */
 int mg_mpnn_train_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
-                      int32_t epochs, int32_t batch, float learning_rate, uint64_t seed, void *stream,
+                      int32_t epochs, int32_t batch, float learning_rate, double kappa, uint64_t seed, void *stream,
                       double *epoch_losses);
 /* SolveWithInitialGuess (CGGP/ConjugateGradientGaussPreSolver.cs:78-166) with the learned
  * preconditioner: same stopping rule and result; *fell_back = 1 when it switched to Jacobi

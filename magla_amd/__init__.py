@@ -148,9 +148,10 @@ def lib():
     L.mg_mpnn_get_params.argtypes = [vp, vp]
     L.mg_mpnn_set_params.argtypes = [vp, vp]
     L.mg_mpnn_apply_dev.argtypes = [vp, vp, vp, vp, C.c_int32, vp]
-    L.mg_mpnn_loss_grad_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int64, vp, C.POINTER(C.c_double), vp]
+    L.mg_mpnn_loss_grad_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int64, C.c_double, vp,
+                                        C.POINTER(C.c_double), vp]
     L.mg_mpnn_train_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float,
-                                    C.c_uint64, vp, vp]
+                                    C.c_double, C.c_uint64, vp, vp]
     L.mg_pcg_solve_learned_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp,
                                            C.POINTER(_PcgResult), C.POINTER(C.c_int32)]
     L.mg_pcg_solve_learned.argtypes = [vp, vp, _f64p, vp, C.c_double, C.c_int32, C.c_int32, _f64p,
@@ -557,9 +558,10 @@ def test_mfma_bf16(a, b):
 
 
 class LearnedPreconditioner:
-    """Message-passing correction on top of Jacobi (no counterpart in MAGLA): three layers of
-    width 32 on the matrix graph, z = (r + s net(r)) / diag(A).  Trained on the GPU (FP32, Adam)
-    from (residual, exact error) pairs; applied in BF16 on MFMA."""
+    """Learned correction on top of Jacobi (no counterpart in MAGLA): a three-layer, width-32
+    message-passing network on the matrix graph predicts a sparse factor G (two-hop lower
+    pattern) and z = G'G r.  Trained on the GPU (FP32, Adam) from (residual, exact error) pairs
+    with a Kaporin condition-number term; the network runs once per matrix in BF16 on MFMA."""
 
     def __init__(self, seed=0):
         h = C.c_void_p()
@@ -586,19 +588,19 @@ class LearnedPreconditioner:
         _check(lib().mg_mpnn_apply_dev(self._h, matrix._h, C.c_void_p(r_ptr), C.c_void_p(z_ptr), precision,
                                        C.c_void_p(stream)))
 
-    def loss_grad_dev(self, matrix: "DeviceMatrix", R_ptr, E_ptr, count, ld, stream=None):
+    def loss_grad_dev(self, matrix: "DeviceMatrix", R_ptr, E_ptr, count, ld, kappa=0.0, stream=None):
         loss = C.c_double()
         grad = np.zeros(self.param_count(), dtype=np.float32)
         _check(lib().mg_mpnn_loss_grad_dev(self._h, matrix._h, count, C.c_void_p(R_ptr), C.c_void_p(E_ptr), ld,
-                                           C.c_void_p(stream), C.byref(loss), _ptr(grad)))
+                                           float(kappa), C.c_void_p(stream), C.byref(loss), _ptr(grad)))
         return loss.value, grad
 
     def train_dev(self, matrix: "DeviceMatrix", R_ptr, E_ptr, count, ld, epochs=10, batch=16, learning_rate=1e-3,
-                  seed=0, stream=None):
+                  kappa=1.0, seed=0, stream=None):
         losses = np.zeros(max(epochs, 1))
         _check(lib().mg_mpnn_train_dev(self._h, matrix._h, count, C.c_void_p(R_ptr), C.c_void_p(E_ptr), ld, epochs,
-                                       batch, C.c_float(learning_rate), C.c_uint64(seed), C.c_void_p(stream),
-                                       _ptr(losses)))
+                                       batch, C.c_float(learning_rate), float(kappa), C.c_uint64(seed),
+                                       C.c_void_p(stream), _ptr(losses)))
         return losses[:epochs]
 
     def __del__(self):

@@ -500,19 +500,18 @@ int mg_mpnn_apply_dev(mg_mpnn *net, mg_matrix *m, const double *d_r, double *d_z
         require(m->m.n == 0 || (d_r && d_z), MG_E_NULL, "vector is null.");
         require(precision == MG_FP32 || precision == MG_BF16, MG_E_ARG, "precision must be MG_FP32 or MG_BF16.");
         const hipStream_t st = (hipStream_t)stream;
-        if (precision == MG_FP32) mg::mpnn_apply_f32(net->net, m->m, d_r, d_z, st);
-        else mg::mpnn_apply_bf16(net->net, m->m, d_r, d_z, nullptr, nullptr, nullptr, nullptr, nullptr, st);
+        mg::mpnn_apply(net->net, m->m, d_r, d_z, precision, st);
     });
 }
 
 int mg_mpnn_loss_grad_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
-                          void *stream, double *loss, float *grad) {
+                          double kappa, void *stream, double *loss, float *grad) {
     return guarded([&] {
         require(net != nullptr && m != nullptr && loss != nullptr, MG_E_NULL, "argument is null.");
         require(ns >= 0 && ld >= m->m.n, MG_E_ARG, "sample count or leading dimension is invalid.");
         require(ns == 0 || (d_R && d_E), MG_E_NULL, "samples are null.");
         const hipStream_t st = (hipStream_t)stream;
-        *loss = ns ? mg::mpnn_loss_grad(net->net, m->m, ns, d_R, d_E, ld, st) : 0.0;
+        *loss = mg::mpnn_loss_grad(net->net, m->m, ns, d_R, d_E, ld, kappa, st);
         if (grad) {
             MG_HIP(hipMemcpyAsync(grad, net->net.grad.p, sizeof(float) * mg::kMpP, hipMemcpyDeviceToHost, st));
             MG_HIP(hipStreamSynchronize(st));
@@ -521,13 +520,13 @@ int mg_mpnn_loss_grad_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *
 }
 
 int mg_mpnn_train_dev(mg_mpnn *net, mg_matrix *m, int32_t ns, const double *d_R, const double *d_E, int64_t ld,
-                      int32_t epochs, int32_t batch, float learning_rate, uint64_t seed, void *stream,
+                      int32_t epochs, int32_t batch, float learning_rate, double kappa, uint64_t seed, void *stream,
                       double *epoch_losses) {
     return guarded([&] {
         require(net != nullptr && m != nullptr, MG_E_NULL, "argument is null.");
         require(ns >= 0 && ld >= m->m.n && epochs >= 0, MG_E_ARG, "invalid training arguments.");
         require(ns == 0 || (d_R && d_E), MG_E_NULL, "samples are null.");
-        mg::mpnn_train(net->net, m->m, ns, d_R, d_E, ld, epochs, batch, learning_rate, seed, epoch_losses,
+        mg::mpnn_train(net->net, m->m, ns, d_R, d_E, ld, epochs, batch, learning_rate, kappa, seed, epoch_losses,
                        (hipStream_t)stream);
     });
 }

@@ -99,47 +99,58 @@ struct PcgScalars {
 };
 
 // ---- learned preconditioner (message-passing network on the matrix graph) ----
-constexpr int kMpW = 32;                                    // hidden width
-constexpr int kMpP = 2 * kMpW + 3 * 2 * kMpW * kMpW + kMpW;  // parameter count
-constexpr int kMpOffW = 2 * kMpW;                           // W1_0 (then W2_0, W1_1, ...)
-constexpr int kMpOffWd = 2 * kMpW + 6 * kMpW * kMpW;        // decoder
-constexpr int kMpBf16Layer = 2 * 4 * 64 * 4;                // packed B operands per layer (bf16)
-constexpr int kMpBlocks = 512;                              // fixed grid of the gradient kernels
+constexpr int kMpW = 32;                                      // hidden width
+constexpr int kMpOffW = 3 * kMpW;                             // W1_0 (then W2_0, W1_1, ...)
+constexpr int kMpOffT = 3 * kMpW + 6 * kMpW * kMpW;           // w_t, then (u1, v1, w1), (u2, v2, w2)
+constexpr int kMpP = kMpOffT + 7 * kMpW;                      // parameter count
+constexpr int kMpBf16Layer = 2 * 4 * 64 * 4;                  // packed B operands per layer (bf16)
+constexpr int kMpBf16Vec = 3 * kMpBf16Layer;                  // then W_in (96) and the readouts (224)
+constexpr int kMpBlocks = 512;                                // fixed grid of the gradient kernels
 
-// Per-matrix graph data: normalized off-diagonal weights a_ij / sqrt(a_ii a_jj) on the full
-// CSR (0 on the diagonal), node feature log(a_ii / mean a_ii), and a_ii.
+// Per-matrix graph data: e1 = a_ij / sqrt(a_ii a_jj) on the full CSR (0 on the diagonal), node
+// features [1, log(a_ii / mean a_ii), sum_j |e1_ij|], a_ii, and the pattern of the learned
+// factor G (row i: columns j <= i within two hops, ascending, diagonal last) with the weights
+// e1_ij and e2_ij = sum_k e1_ik e1_kj of every entry, plus the transposed pattern.
 struct MpGraph {
     int n = 0;
-    DevArray<float> ew, cf;
+    int64_t ng = 0;
+    DevArray<float> ew, x, ge1, ge2;
     DevArray<double> diag;
+    DevArray<int> gptr, gcol, gtptr, gtrow, gtidx;
 };
 
 struct Mpnn {
     DevArray<float> w;              // FP32 master parameters
-    DevArray<uint16_t> wb;          // BF16 copy: packed MFMA B operands (3 layers), then w_e, w_c, w_d
+    DevArray<uint16_t> wb;          // BF16 copy: packed MFMA B operands (3 layers), W_in, readouts
     DevArray<float> grad, am, av;   // gradient and Adam moments
     int adam_t = 0;
-    int ws_n = 0;                   // workspaces below are sized for this n
-    DevArray<float> H, Mb, Pre, dH, dHp, dM, part;
+    int ws_n = 0;                   // workspaces below are sized for this n (and ng)
+    int64_t ws_ng = 0;
+    DevArray<float> H, Mb, Pre, dH, dHp, dM, part, gvf, dG, gaE;
     DevArray<uint16_t> hb0, hb1;
-    DevArray<double> z, az, dpart, scal;
+    DevArray<double> gv, gvt;       // factor values for the solves (from the BF16 network), also transposed order
+    const MpGraph *gv_graph = nullptr;  // graph the factor values belong to
+    DevArray<double> y, z, az, u, dpart, scal;
 };
 
 void mp_graph_prepare(Matrix &A, hipStream_t st);
 void mpnn_init(Mpnn &net, uint64_t seed, hipStream_t st);
 void mpnn_pack_bf16(Mpnn &net, hipStream_t st);
-// z = (r + s net(r)) / diag with s = ||r|| / sqrt(n) from *d_rnorm (a device scalar);
-// when d_rnorm is null the norm is computed here.  FP32 network or BF16 MFMA network.
-void mpnn_apply_f32(Mpnn &net, Matrix &A, const double *d_r, double *d_z, hipStream_t st);
-// BF16 inference; S (may be null) gates the kernels: they do nothing once S->done or S->fallback.
-// rz_part / rap_part (may be null) receive fixed-grid partial sums of r.z and z.ap (kRedBlocks).
-void mpnn_apply_bf16(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const double *d_rnorm,
-                     const PcgScalars *S, const double *d_ap, double *rz_part, double *zap_part, hipStream_t st);
-// mean loss over ns pairs (columns of R, E with leading dimension ld); gradient into net.grad
-double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, hipStream_t st);
+// z = G'G r with the FP32 network (training precision) or the BF16 MFMA network
+void mpnn_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, int precision, hipStream_t st);
+// factor values from the BF16 network (once per matrix and network), then per call
+// z = G'G r with per-block partial sums of r.z and z.ap (n / 256 blocks each); the kernels do
+// nothing once S->done or S->fallback
+void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st);
+void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const PcgScalars *S, const double *d_ap,
+                       double *rz_part, double *zap_part, hipStream_t st);
+// mean loss over ns pairs (columns of R, E with leading dimension ld) plus kappa times the log
+// Kaporin number of G A G'; gradient into net.grad
+double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, double kappa,
+                      hipStream_t st);
 void mpnn_adam_step(Mpnn &net, float lr, hipStream_t st);
 void mpnn_train(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, int epochs, int batch,
-                float lr, uint64_t seed, double *losses, hipStream_t st);
+                float lr, double kappa, uint64_t seed, double *losses, hipStream_t st);
 void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d_x0, double tol, int max_it,
                         int stall_window, double *d_x, hipStream_t st, mg_pcg_result &res, int &fell_back);
 

@@ -33,6 +33,7 @@ constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag /
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
 constexpr int kBig128 = 1 << 30;      // fronts this large take 128x128 Schur tiles (off: slower here)
+constexpr int kSmallFront = 64;       // fronts this small are solved by one wave each (all panels, all RHS)
 constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
 constexpr int kFootChunk = 1024;      // footer rows per partial product in the backward solve
@@ -61,6 +62,8 @@ struct DevicePlan {
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_upd_ptr, lvl_diag_ptr;
     std::vector<std::vector<int>> lvl_upd_mid;  // per panel: first 128-tile task (64-tile tasks before it)
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
+    std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each)
+    DevArray<int> small_nodes;
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
 
@@ -232,6 +235,8 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     for (int g = 0; g < nf; ++g) vbase[g + 1] = vbase[g] + P->h_f[g];
     P->vsum = vbase[nf];
     P->lvl_init_ptr.push_back(0);
+    P->lvl_small_ptr.push_back(0);
+    std::vector<int> small;
     P->lvl_foot_ptr.push_back(0);
     P->lvl_footr_ptr.push_back(0);
     std::vector<int2> sfwd, sbwd;
@@ -249,7 +254,9 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             level_nodes.push_back(g);
             const int f = P->h_f[g], w = P->h_w[g];
             npan = std::max(npan, (w + kPanel - 1) / kPanel);
-            for (int c = 0; c < (f + 255) / 256; ++c) init.push_back(make_int2(g, c));
+            if (f <= kSmallFront) small.push_back(g);
+            else
+                for (int c = 0; c < (f + 255) / 256; ++c) init.push_back(make_int2(g, c));
             const int T = (f + kTile - 1) / kTile;
             for (int tj = 0; tj < T; ++tj)
                 for (int ti = tj; ti < T; ++ti) pull.push_back(make_int4(g, ti, tj, 0));
@@ -278,6 +285,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         const int foot_base = (int)foot.size();
         for (int g : bylevel[L]) {
             const int f = P->h_f[g], w = P->h_w[g];
+            if (f <= kSmallFront) continue;
             const int nchunk = (f - w + kFootChunk - 1) / kFootChunk;
             for (int cb = 0; cb < (w + kPanel - 1) / kPanel; ++cb) {
                 footr.push_back(make_int4(g, cb, (int)foot.size() - foot_base, nchunk));
@@ -289,7 +297,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         for (int q = 0; q < npan; ++q) {
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
-                if (w <= q * kPanel) continue;
+                if (w <= q * kPanel || f <= kSmallFront) continue;
                 const int pe = std::min(q * kPanel + kPanel, w);
                 sfwd.push_back(make_int2(g, -1));
                 for (int rb = 0; rb < (f - pe + kTile - 1) / kTile; ++rb) sfwd.push_back(make_int2(g, rb));
@@ -304,6 +312,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_footr_ptr.push_back((int)footr.size());
         P->lvl_node_ptr.push_back((int)level_nodes.size());
         P->lvl_init_ptr.push_back((int)init.size());
+        P->lvl_small_ptr.push_back((int)small.size());
         P->lvl_pull_ptr.push_back((int)pull.size());
         P->lvl_asm_ptr.push_back((int64_t)asm_pos.size());
         P->lvl_npanels.push_back(npan);
@@ -377,6 +386,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->diag_nodes.upload(diag_nodes);
     P->diag_tb.upload(diag_tb);
     P->init_tasks.upload(init);
+    P->small_nodes.upload(small);
     P->foot_tasks.upload(foot);
     P->footr_tasks.upload(footr);
 
@@ -577,6 +587,8 @@ template <typename T, int K>
 __device__ __forceinline__ void panel_diag_step(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
                                                 T (*colk)[kPanel], T (*rowk)[kPanel], T *pvs) {
     constexpr int kw = K / kDiagCols, kj = K % kDiagCols, buf = K & 1;
+    // Steps past the block's width would only pivot the identity padding: skipped (uniform).
+    if (K >= x.nb) return;
     const int cb = x.v * kDiagCols;
     if (x.r == K) {
 #pragma unroll
@@ -725,7 +737,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
 #pragma unroll
     for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
 #pragma unroll 4
-    for (int m = 0; m < kPanel; ++m) {
+    for (int m = 0; m < nb; ++m) {
         const T xm = X[r][m];
 #pragma unroll
         for (int q = 0; q < kPanel / 4; ++q) out[q] += xm * Bm[m][c0 + 4 * q];
@@ -959,6 +971,96 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     }
 }
 
+// Small fronts (f <= 64, hence one panel): one wave per front does the whole forward
+// step -- the gather of the front vector (own right-hand-side rows plus the children's
+// footer contributions through the inverse row maps), y = Li v on the own rows and
+// v -= L y on the footer rows -- for all right-hand sides.  Lane l handles row l % FR of
+// right-hand side l / FR (FR = 16, 32 or 64 >= f); broadcasts are wave shuffles.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes, int count,
+                                                   const int *__restrict__ F, const int *__restrict__ W,
+                                                   const int *__restrict__ firstcol, const int *__restrict__ child_ptr,
+                                                   const int *__restrict__ children,
+                                                   const int64_t *__restrict__ inv_off, const int *__restrict__ inv_map,
+                                                   const int64_t *__restrict__ vbase, const T *__restrict__ arena,
+                                                   const int64_t *__restrict__ off, T *__restrict__ V,
+                                                   T *__restrict__ Y, int n, int nrhs) {
+    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (idx >= count) return;  // whole wave
+    const int lane = threadIdx.x & 63;
+    const int g = nodes[idx], f = F[g], w = W[g], fc = firstcol[g];
+    const int FR = f <= 16 ? 16 : (f <= 32 ? 32 : 64), RG = 64 / FR;
+    const int i = lane % FR, rg = lane / FR, base = rg * FR;
+    const T *M = arena + off[g];
+    T *Vg = V + vbase[g] * nrhs;
+    const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
+    for (int r0 = 0; r0 < nrhs; r0 += RG) {
+        const int r = r0 + rg;
+        const bool act = r < nrhs && i < f;
+        T v = T(0);
+        if (act) {
+            v = i < w ? Y[(int64_t)r * n + fc + i] : T(0);
+            for (int c = c0; c < c1; ++c) {
+                const int x = children[c];
+                const int a = inv_map[inv_off[x] + i];
+                if (a >= 0) v += V[vbase[x] * nrhs + (int64_t)r * F[x] + W[x] + a];
+            }
+        }
+        // y_i = v_i + sum_{m<i} Li[i][m] v_m  (Li[i][m] stored at row m, column i)
+        T y = v;
+        for (int m = 0; m < w; ++m) {
+            const T vm = __shfl(v, base + m);
+            if (m < i && i < w) y += M[(int64_t)i * f + m] * vm;
+        }
+        // footer: v_i -= sum_k L[i][k] y_k
+        for (int k = 0; k < w; ++k) {
+            const T yk = __shfl(y, base + k);
+            if (i >= w && i < f) v -= M[(int64_t)k * f + i] * yk;
+        }
+        if (act) {
+            if (i < w) Y[(int64_t)r * n + fc + i] = y;
+            else Vg[(int64_t)r * f + i] = v;
+        }
+    }
+}
+
+// Small fronts, backward: z = y / d - L_foot' x_foot, then x = Li' z on the own rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes, int count,
+                                                   const int *__restrict__ F, const int *__restrict__ W,
+                                                   const int *__restrict__ firstcol, const int *__restrict__ pat_ptr,
+                                                   const int *__restrict__ pat_rows, const T *__restrict__ arena,
+                                                   const int64_t *__restrict__ off, const double *__restrict__ D,
+                                                   T *__restrict__ Y, int n, int nrhs) {
+    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (idx >= count) return;  // whole wave
+    const int lane = threadIdx.x & 63;
+    const int g = nodes[idx], f = F[g], w = W[g], fc = firstcol[g];
+    const int FR = f <= 16 ? 16 : (f <= 32 ? 32 : 64), RG = 64 / FR;
+    const int i = lane % FR, rg = lane / FR, base = rg * FR;
+    const T *M = arena + off[g];
+    const int *rows = pat_rows + pat_ptr[g];
+    for (int r0 = 0; r0 < nrhs; r0 += RG) {
+        const int r = r0 + rg;
+        const bool act = r < nrhs;
+        const T xf = (act && i >= w && i < f) ? Y[(int64_t)r * n + rows[i]] : T(0);
+        T s = T(0);
+        for (int t = w; t < f; ++t) {
+            const T xt = __shfl(xf, base + t);
+            if (i < w) s += M[(int64_t)i * f + t] * xt;
+        }
+        T z = T(0);
+        if (act && i < w) z = Y[(int64_t)r * n + fc + i] / (T)D[fc + i] - s;
+        // x_i = z_i + sum_{m>i} Li[m][i] z_m  (Li[m][i] stored at row i, column m)
+        T x = z;
+        for (int m = 0; m < w; ++m) {
+            const T zm = __shfl(z, base + m);
+            if (m > i && i < w) x += M[(int64_t)m * f + i] * zm;
+        }
+        if (act && i < w) Y[(int64_t)r * n + fc + i] = x;
+    }
+}
+
 // Forward panel (columns [ps, pe) of a front): every workgroup forms y = Li v[ps:pe)
 // from the stored 64x64 inverse; task rb = -1 writes y to Y, task rb >= 0 subtracts
 // L[rows, ps:pe) y from its 64-row block of V below the panel.
@@ -997,8 +1099,8 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
         __syncthreads();
         for (int rr = grp; rr < nr; rr += 4) {
             T acc = T(0);
-#pragma unroll 16
-            for (int m = 0; m < kPanel; ++m) acc += inv[m][lane] * vv[m][rr];
+#pragma unroll 8
+            for (int m = 0; m < nb; ++m) acc += inv[m][lane] * vv[m][rr];
             yv[lane][rr] = acc;
         }
         __syncthreads();
@@ -1017,8 +1119,8 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
                 const int rr = grp + 4 * j;
                 if (rr < nr) {
                     T acc = T(0);
-#pragma unroll 16
-                    for (int k = 0; k < kPanel; ++k) acc += blk[lane][k] * yv[k][rr];
+#pragma unroll 8
+                    for (int k = 0; k < nb; ++k) acc += blk[lane][k] * yv[k][rr];
                     Vg[(int64_t)(c0 + rr) * f + r0 + lane] = old[j] - acc;
                 }
             }
@@ -1149,8 +1251,8 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
         __syncthreads();
         for (int rr = grp; rr < nr; rr += 4) {
             T acc = T(0);
-#pragma unroll 16
-            for (int k = 0; k < kPanel; ++k) acc += inv[lane][k] * zz[k][rr];
+#pragma unroll 8
+            for (int k = 0; k < nb; ++k) acc += inv[lane][k] * zz[k][rr];
             xv[lane][rr] = acc;
         }
         __syncthreads();
@@ -1169,8 +1271,8 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
                 const int rr = grp + 4 * j;
                 if (rr < nr) {
                     T acc = T(0);
-#pragma unroll 16
-                    for (int k = 0; k < kPanel; ++k) acc += blk[k][lane] * xv[k][rr];
+#pragma unroll 8
+                    for (int k = 0; k < nb; ++k) acc += blk[k][lane] * xv[k][rr];
                     Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] = old[j] - acc;
                 }
             }
@@ -1675,13 +1777,21 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     T *part = reinterpret_cast<T *>(N.wsQ.p);
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
+        const int m0 = P.lvl_small_ptr[L], m1 = P.lvl_small_ptr[L + 1];
         ProfScope pi_(st, KP_FWD, 0.0, 0.0);
-        hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p, P.first.p,
-                           P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n, nrhs);
+        if (m1 > m0)
+            hipLaunchKernelGGL(k_fwd_small<T>, dim3((m1 - m0 + 3) / 4), dim3(256), 0, st, P.small_nodes.p + m0, m1 - m0,
+                               P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
+                               P.vbase.p, arena, P.off.p, V, Y, n, nrhs);
+        if (i1 > i0)
+            hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
+                               P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n,
+                               nrhs);
         for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
             const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
-            hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, P.f.p, P.w.p,
-                               P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs);
+            if (s1 > s0)
+                hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, P.f.p,
+                                   P.w.p, P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -1695,13 +1805,20 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                                P.w.p, P.off.p, P.vbase.p, V, part, nrhs);
         }
         const int r0 = P.lvl_footr_ptr[L], r1 = P.lvl_footr_ptr[L + 1];
-        hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p, P.first.p,
-                           N.diag.p, Y, part, Z, n, nrhs);
+        if (r1 > r0)
+            hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p,
+                               P.first.p, N.diag.p, Y, part, Z, n, nrhs);
         for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
             const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
-            hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p, P.w.p,
-                               P.off.p, P.first.p, Y, Z, q, n, nrhs);
+            if (s1 > s0)
+                hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p,
+                                   P.w.p, P.off.p, P.first.p, Y, Z, q, n, nrhs);
         }
+        const int m0 = P.lvl_small_ptr[L], m1 = P.lvl_small_ptr[L + 1];
+        if (m1 > m0)
+            hipLaunchKernelGGL(k_bwd_small<T>, dim3((m1 - m0 + 3) / 4), dim3(256), 0, st, P.small_nodes.p + m0, m1 - m0,
+                               P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p, arena, P.off.p, N.diag.p, Y, n,
+                               nrhs);
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.perm.p, n, nrhs, d_x, ldx,
                        accumulate);

@@ -1,22 +1,26 @@
 // mpnn.hip -- learned preconditioner: a small message-passing network on the matrix
-// graph, used as a correction on top of CGGP's Jacobi preconditioner
-// (CGGP/JacobiPreconditioner.cs:12-56) inside a flexible PCG.  MAGLA has no
-// counterpart; the network is defined here:
+// graph predicts a sparse lower-triangular factor G (pattern: columns j <= i within two
+// hops of i), and z = G'G r corrects CGGP's Jacobi preconditioner
+// (CGGP/JacobiPreconditioner.cs:12-56) inside a flexible PCG.  MAGLA has no counterpart;
+// the network is defined here:
 //
-//   inputs   u_i = r_i / s, s = ||r|| / sqrt(n);  c_i = log(a_ii / mean a_ii)
-//   edges    ew_ij = a_ij / sqrt(a_ii a_jj)  (j != i, full symmetric pattern)
-//   h0_i   = u_i (w_e + c_i w_c)                                  (width 32)
-//   layer l: m_i = sum_j ew_ij h_j ;  h'_i = act(W1_l h_i + W2_l m_i)   (l = 0, 1, 2;
-//            act = ReLU for l < 2, identity for l = 2)
-//   output  y_i = w_d . h3_i ;  z_i = (r_i + s y_i) / a_ii
+//   edges    e1_ij = a_ij / sqrt(a_ii a_jj) (j != i);  e2_ij = sum_k e1_ik e1_kj
+//   nodes    x_i = [1, c_i, s_i], c_i = log(a_ii / mean a_ii), s_i = sum_j |e1_ij|
+//   h0_i   = W_in x_i                                                  (width 32)
+//   layer l: m_i = sum_j e1_ij h_j ;  h'_i = act(W1_l h_i + W2_l m_i)  (act = ReLU for
+//            l < 2, identity for l = 2)
+//   factor   G_ii = a_ii^{-1/2} exp(w_t . h3_i)
+//            G_ij = a_jj^{-1/2} (e1_ij p1_ij + e2_ij p2_ij),
+//            p_ij = u . (h3_i * h3_j) + v . h3_i + w . h3_j  with (u1, v1, w1) or (u2, v2, w2)
 //
-// No biases and ReLU keep z(alpha r) = alpha z(r) for alpha > 0.  Training
-// minimises the A-norm error of z against the exact error e = A^{-1} r of
-// each (r, e) pair, L = (z - e)'A(z - e) / (e'Ae) = (z'Az - 2 z'r) / (r'e) + 1,
-// so dL/dz = 2 (Az - r) / (r'e).  Training runs in FP32 (SIMT kernels, Adam);
-// inference in BF16 weights and activations with FP32 accumulation on MFMA
-// (v_mfma_f32_16x16x16_bf16: per 16-node tile, [h | m] (16 x 64) x [W1 | W2]^T
-// (64 x 32)).  Every reduction has a fixed grid and order.
+// Zero readouts give G = D^{-1/2}, i.e. Jacobi.  z = G'G r is linear, symmetric and positive
+// definite for any parameters.  Training (FP32 SIMT kernels, Adam) minimises, over (r, e = A^{-1} r)
+// pairs, the squared A-norm error of z relative to e, L = (z'Az - 2 z'r) / (r'e) + 1, plus
+// kappa times the log Kaporin condition number of G A G', log(tr(G A G') / n) - (2/n) sum log G_ii,
+// which keeps small eigenvalues of the preconditioned operator away from zero.  Inference runs
+// the network once per matrix in BF16 (weights and activations) with FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x16_bf16: per 16-node tile, [h | m] (16 x 64) x [W1 | W2]^T (64 x 32)); the
+// solves then apply G and G' in FP64.  Every reduction has a fixed grid and order.
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -44,26 +48,10 @@ __device__ inline uint16_t f2bf(float x) {  // round to nearest even
 __device__ inline float bf2f(uint32_t b) { return __uint_as_float(b << 16); }
 
 __device__ inline bool gated(const PcgScalars *S) { return S && (S->done || S->fallback); }
+__device__ inline float to_f32(float v) { return v; }
+__device__ inline float to_f32(uint16_t v) { return bf2f(v); }
 
 // ---- graph data ------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_mp_diag(const int *__restrict__ dpos, const double *__restrict__ vals, int n,
-                                                 double *__restrict__ diag, double *__restrict__ part) {
-    __shared__ double sm[256];
-    double acc = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double d = dpos[i] >= 0 ? vals[dpos[i]] : 0.0;
-        diag[i] = d;
-        acc += d;
-    }
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
-}
 
 __global__ __launch_bounds__(256) void k_sum_final(const double *__restrict__ part, int nparts, double scale,
                                                    double *__restrict__ out) {
@@ -77,47 +65,6 @@ __global__ __launch_bounds__(256) void k_sum_final(const double *__restrict__ pa
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sm[0] * scale;
-}
-
-__global__ void k_mp_graph(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ vals,
-                           const double *__restrict__ diag, const double *__restrict__ mean, int n,
-                           float *__restrict__ ew, float *__restrict__ cf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double di = diag[i];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-        const int j = ci[k];
-        ew[k] = j == i ? 0.0f : (float)(vals[k] / sqrt(di * diag[j]));
-    }
-    cf[i] = (float)log(di / mean[0]);
-}
-
-// ---- FP32 forward ----------------------------------------------------------------
-
-// sqrt(sum r^2), two stages over the fixed grid
-__global__ __launch_bounds__(256) void k_mp_sq_part(const double *__restrict__ r, int n, double *__restrict__ part) {
-    __shared__ double sm[256];
-    double acc = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += r[i] * r[i];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
-}
-__global__ void k_sqrt_inplace(double *__restrict__ v) { v[0] = sqrt(v[0]); }
-
-__global__ __launch_bounds__(256) void k_mp_input(const double *__restrict__ r, const double *__restrict__ rnorm,
-                                                  double rsqn, const float *__restrict__ cf,
-                                                  const float *__restrict__ w, int n, float *__restrict__ H0) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int i = (int)(e / W), f = (int)(e % W);
-    if (i >= n) return;
-    const double s = rnorm[0] * rsqn;
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    H0[e] = (float)(r[i] * inv) * (w[f] + cf[i] * w[W + f]);
 }
 
 // One message-passing layer: m_i = sum_j ew_ij h_j, pre = W1 h_i + W2 m_i, h' = act(pre).
@@ -153,17 +100,6 @@ __global__ __launch_bounds__(256) void k_mp_layer(const float *__restrict__ wl, 
         }
         __syncthreads();
     }
-}
-
-__global__ __launch_bounds__(256) void k_mp_output(const double *__restrict__ r, const double *__restrict__ rnorm,
-                                                   double rsqn, const double *__restrict__ diag,
-                                                   const float *__restrict__ wd, const float *__restrict__ H3, int n,
-                                                   double *__restrict__ z) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int i = (int)(e / W), f = (int)(e % W);
-    float y = i < n ? wd[f] * H3[e] : 0.0f;
-    for (int o = W / 2; o > 0; o >>= 1) y += __shfl_xor(y, o, W);
-    if (f == 0 && i < n) z[i] = (r[i] + rnorm[0] * rsqn * (double)y) / diag[i];
 }
 
 // ---- loss and FP32 backward ----------------------------------------------------
@@ -221,28 +157,6 @@ __device__ inline void group_reduce_add(float v, float (*sm)[W], float *__restri
         dst[f] += t;
     }
     __syncthreads();
-}
-
-// dy_i = dL/dz_i * s / a_ii; dH3_i = dy_i w_d; gradient of w_d.
-__global__ __launch_bounds__(256) void k_mp_out_bwd(const double *__restrict__ r, const double *__restrict__ az,
-                                                    const double *__restrict__ rnorm, double rsqn,
-                                                    const double *__restrict__ diag, const double *__restrict__ scal,
-                                                    int ns, const float *__restrict__ wd, const float *__restrict__ H3,
-                                                    int n, float *__restrict__ dH, float *__restrict__ part) {
-    __shared__ float sm[kNB][W];
-    const int g = threadIdx.x / W, f = threadIdx.x % W;
-    const double s = rnorm[0] * rsqn, re = scal[1];
-    float acc = 0.0f;
-    for (int base = blockIdx.x * kNB; base < n; base += gridDim.x * kNB) {
-        const int i = base + g;
-        if (i < n) {
-            const float dy = (float)(2.0 * (az[i] - r[i]) / re / ns * s / diag[i]);
-            const int64_t e = (int64_t)i * W + f;
-            dH[e] = dy * wd[f];
-            acc += dy * H3[e];
-        }
-    }
-    group_reduce_add(acc, sm, part + (int64_t)blockIdx.x * kMpP + kMpOffWd);
 }
 
 // Backward of one layer: dpre = dh (x ReLU mask); dh_self = W1' dpre, dm = W2' dpre;
@@ -318,26 +232,6 @@ __global__ __launch_bounds__(256) void k_mp_agg_bwd(const int *__restrict__ rp, 
     dHp[e] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_mp_in_bwd(const double *__restrict__ r, const double *__restrict__ rnorm,
-                                                   double rsqn, const float *__restrict__ cf,
-                                                   const float *__restrict__ dH0, int n, float *__restrict__ part) {
-    __shared__ float sm[kNB][W];
-    const int g = threadIdx.x / W, f = threadIdx.x % W;
-    const double s = rnorm[0] * rsqn;
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    float ae = 0.0f, ac = 0.0f;
-    for (int base = blockIdx.x * kNB; base < n; base += gridDim.x * kNB) {
-        const int i = base + g;
-        if (i < n) {
-            const float u = (float)(r[i] * inv), d = dH0[(int64_t)i * W + f];
-            ae += u * d;
-            ac += u * cf[i] * d;
-        }
-    }
-    group_reduce_add(ae, sm, part + (int64_t)blockIdx.x * kMpP);
-    group_reduce_add(ac, sm, part + (int64_t)blockIdx.x * kMpP + W);
-}
-
 __global__ void k_mp_grad_reduce(const float *__restrict__ part, int nparts, float *__restrict__ grad) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMpP) return;
@@ -367,38 +261,7 @@ __global__ void k_mp_adam(float *__restrict__ w, const float *__restrict__ g, fl
     w[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
 }
 
-// BF16 copy: for layer l, output block nb (features 16nb..16nb+15), k chunk kk, lane, i:
-// B[k][col] = [W1_l | W2_l][col][k] with k = 16kk + 4(lane/16) + i, col = 16nb + lane%16
-// (the B-operand layout of v_mfma_f32_16x16x16_bf16); then w_e, w_c, w_d.
-__global__ void k_mp_pack(const float *__restrict__ w, uint16_t *__restrict__ wb) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < 3 * kMpBf16Layer) {
-        const int i = idx & 3, lane = (idx >> 2) & 63, kk = (idx >> 8) & 3, nb = (idx >> 10) & 1, l = idx >> 11;
-        const int k = 16 * kk + 4 * (lane >> 4) + i, col = 16 * nb + (lane & 15);
-        const float *wl = w + kMpOffW + l * 2 * W * W;
-        const float v = k < W ? wl[col * W + k] : wl[W * W + col * W + (k - W)];
-        wb[idx] = f2bf(v);
-    } else if (idx < 3 * kMpBf16Layer + 3 * W) {
-        const int t = idx - 3 * kMpBf16Layer;  // 0..95: w_e, w_c, w_d
-        const float v = t < 2 * W ? w[t] : w[kMpOffWd + (t - 2 * W)];
-        wb[idx] = f2bf(v);
-    }
-}
-
 // ---- BF16 inference --------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_mpb_input(const double *__restrict__ r, const double *__restrict__ rnorm,
-                                                   double rsqn, const float *__restrict__ cf,
-                                                   const uint16_t *__restrict__ wv, int n, uint16_t *__restrict__ H0,
-                                                   const PcgScalars *__restrict__ S) {
-    if (gated(S)) return;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int i = (int)(e / W), f = (int)(e % W);
-    if (i >= n) return;
-    const double s = rnorm[0] * rsqn;
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    H0[e] = f2bf((float)(r[i] * inv) * (bf2f(wv[f]) + cf[i] * bf2f(wv[W + f])));
-}
 
 // One layer on MFMA.  Each wave takes 16-node tiles: lane l gathers features 8(l%4)..+7
 // of node l/4 (own h and the weighted neighbour sum m, FP32), writes the bf16 row
@@ -471,28 +334,95 @@ __global__ __launch_bounds__(256) void k_mpb_layer(const uint16_t *__restrict__ 
     }
 }
 
-// z_i = (r_i + s y_i) / a_ii, y_i = w_d . h3_i; optional fixed-grid partial sums of r.z and z.ap.
-__global__ __launch_bounds__(256) void k_mpb_output(const double *__restrict__ r, const double *__restrict__ rnorm,
-                                                    double rsqn, const double *__restrict__ diag,
-                                                    const uint16_t *__restrict__ wv, const uint16_t *__restrict__ H3,
-                                                    int n, double *__restrict__ z, const double *__restrict__ ap,
-                                                    double *__restrict__ rz_part, double *__restrict__ zap_part,
-                                                    const PcgScalars *__restrict__ S) {
-    if (gated(S)) return;
-    __shared__ double sm[2][256];
+// ---- learned-factor kernels -------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_mp_input(const float *__restrict__ x, const float *__restrict__ w, int n,
+                                                  float *__restrict__ H0) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int i = (int)(e / W), f = (int)(e % W);
+    if (i >= n) return;
+    H0[e] = w[3 * f] * x[3 * i] + w[3 * f + 1] * x[3 * i + 1] + w[3 * f + 2] * x[3 * i + 2];
+}
+
+// readout values of one factor entry from the node states (32 lanes = features, reduced over
+// the lanes of the node's group)
+template <typename H, typename R>
+__device__ inline float factor_entry(int i, int j, const H *hi, const H *hj, const R *rd, int f, float e1, float e2,
+                                     const double *__restrict__ diag, double &out) {
+    (void)i;
+    const float a = to_f32(hi[f]), b = to_f32(hj[f]);
+    float p1 = to_f32(rd[W + f]) * a * b + to_f32(rd[2 * W + f]) * a + to_f32(rd[3 * W + f]) * b;
+    float p2 = to_f32(rd[4 * W + f]) * a * b + to_f32(rd[5 * W + f]) * a + to_f32(rd[6 * W + f]) * b;
+    for (int o = W / 2; o > 0; o >>= 1) {
+        p1 += __shfl_xor(p1, o, W);
+        p2 += __shfl_xor(p2, o, W);
+    }
+    out = ((double)e1 * p1 + (double)e2 * p2) / sqrt(diag[j]);
+    return 0.0f;
+}
+
+// G values of every row from h3 (FP32 or BF16 states and readout weights); 32 lanes per node.
+template <typename H, typename R, typename G>
+__global__ __launch_bounds__(256) void k_mp_readout(const H *__restrict__ h3, const R *__restrict__ rd,
+                                                    const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                    const float *__restrict__ ge1, const float *__restrict__ ge2,
+                                                    const double *__restrict__ diag, int n, G *__restrict__ gv) {
     const int g = threadIdx.x / W, f = threadIdx.x % W;
-    const double s = rnorm[0] * rsqn;
-    const float wdf = bf2f(wv[2 * W + f]);
-    double arz = 0.0, azap = 0.0;
     for (int base = blockIdx.x * kNB; base < n; base += gridDim.x * kNB) {
         const int i = base + g;
-        float y = i < n ? wdf * bf2f(H3[(int64_t)i * W + f]) : 0.0f;
-        for (int o = W / 2; o > 0; o >>= 1) y += __shfl_xor(y, o, W);
-        if (f == 0 && i < n) {
-            const double zi = (r[i] + s * (double)y) / diag[i];
-            z[i] = zi;
-            arz += r[i] * zi;
-            if (ap) azap += zi * ap[i];
+        if (i >= n) continue;  // whole 32-lane group
+        const H *hi = h3 + (int64_t)i * W;
+        for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+            const int j = gcol[e];
+            double val;
+            if (j == i) {
+                float t = to_f32(rd[f]) * to_f32(hi[f]);
+                for (int o = W / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, W);
+                val = exp((double)t) / sqrt(diag[i]);
+            } else {
+                factor_entry(i, j, hi, h3 + (int64_t)j * W, rd, f, ge1[e], ge2[e], diag, val);
+            }
+            if (f == 0) gv[e] = (G)val;
+        }
+    }
+}
+
+// y_i = sum_e G_e v_j over row i (one thread per row)
+template <typename G>
+__global__ __launch_bounds__(256) void k_g_apply(const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                 const G *__restrict__ gv, const double *__restrict__ v, int n,
+                                                 double *__restrict__ y, const PcgScalars *__restrict__ S) {
+    if (gated(S)) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int e = gptr[i]; e < gptr[i + 1]; ++e) acc += (double)gv[e] * v[gcol[e]];
+    y[i] = acc;
+}
+
+// z_j = sum over column j of G of G_ij y_i (transposed pattern; gvt = G values in that order,
+// or gv through gtidx when gvt is null), one thread per row, with optional per-block partial
+// sums of r.z and z.ap
+template <typename G>
+__global__ __launch_bounds__(256) void k_gt_apply(const int *__restrict__ gtptr, const int *__restrict__ gtrow,
+                                                  const int *__restrict__ gtidx, const G *__restrict__ gv,
+                                                  const G *__restrict__ gvt, const double *__restrict__ y, int n,
+                                                  double *__restrict__ z, const double *__restrict__ r,
+                                                  const double *__restrict__ ap, double *__restrict__ rz_part,
+                                                  double *__restrict__ zap_part, const PcgScalars *__restrict__ S) {
+    if (gated(S)) return;
+    __shared__ double sm[2][256];
+    double arz = 0.0, azap = 0.0;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        double acc = 0.0;
+        if (gvt)
+            for (int t = gtptr[j]; t < gtptr[j + 1]; ++t) acc += (double)gvt[t] * y[gtrow[t]];
+        else
+            for (int t = gtptr[j]; t < gtptr[j + 1]; ++t) acc += (double)gv[gtidx[t]] * y[gtrow[t]];
+        z[j] = acc;
+        if (rz_part) {
+            arz += r[j] * acc;
+            if (ap) azap += acc * ap[j];
         }
     }
     if (!rz_part) return;
@@ -508,13 +438,238 @@ __global__ __launch_bounds__(256) void k_mpb_output(const double *__restrict__ r
     }
     if (threadIdx.x == 0) {
         rz_part[blockIdx.x] = sm[0][0];
-        if (zap_part) zap_part[blockIdx.x] = sm[1][0];
+        zap_part[blockIdx.x] = sm[1][0];
     }
 }
 
-void ensure_ws(Mpnn &net, int n) {
-    if (net.ws_n >= n) return;
-    const size_t nw = (size_t)std::max(n, 1) * W;
+// gvt[t] = gv[gtidx[t]]: factor values in the order of the transposed pattern
+__global__ void k_gather_t(const int *__restrict__ gtidx, const double *__restrict__ gv, int64_t ng,
+                           double *__restrict__ gvt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ng) gvt[t] = gv[gtidx[t]];
+}
+
+// u = G g with g_j = 2 (Az_j - r_j) / (r'e) / ns computed on the fly (scal[1] = r'e)
+__global__ __launch_bounds__(256) void k_u_apply(const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                 const float *__restrict__ gv, const double *__restrict__ az,
+                                                 const double *__restrict__ r, const double *__restrict__ scal, int ns,
+                                                 int n, double *__restrict__ u) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double c = 2.0 / scal[1] / ns;
+    double acc = 0.0;
+    for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+        const int j = gcol[e];
+        acc += (double)gv[e] * (c * (az[j] - r[j]));
+    }
+    u[i] = acc;
+}
+
+// dL/dG_ij += y_i g_j + u_i r_j on the pattern of G
+__global__ __launch_bounds__(256) void k_dg_acc(const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                const double *__restrict__ y, const double *__restrict__ u,
+                                                const double *__restrict__ az, const double *__restrict__ r,
+                                                const double *__restrict__ scal, int ns, int n, float *__restrict__ dG) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double c = 2.0 / scal[1] / ns;
+    const double yi = y[i], ui = u[i];
+    for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+        const int j = gcol[e];
+        dG[e] += (float)(yi * (c * (az[j] - r[j])) + ui * r[j]);
+    }
+}
+
+// A_kj from the full CSR (rows ascending)
+__device__ inline double a_entry(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+                                 int k, int j) {
+    for (int q = rp[k]; q < rp[k + 1]; ++q)
+        if (ci[q] == j) return v[q];
+    return 0.0;
+}
+
+// Kaporin term, stage 1: (G A)_ij on the pattern of G, per-row trace terms and log G_ii,
+// fixed-grid partial sums (part[b], part[kRedBlocks + b])
+__global__ __launch_bounds__(256) void k_kap_rows(const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                  const float *__restrict__ gv, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci, const double *__restrict__ av, int n,
+                                                  float *__restrict__ gaE, double *__restrict__ part) {
+    __shared__ double sm[2][256];
+    double tr = 0.0, ld = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+            const int j = gcol[e];
+            double ga = 0.0;
+            for (int e2 = gptr[i]; e2 < gptr[i + 1]; ++e2) ga += (double)gv[e2] * a_entry(rp, ci, av, gcol[e2], j);
+            gaE[e] = (float)ga;
+            tr += (double)gv[e] * ga;
+            if (j == i) ld += log((double)gv[e]);
+        }
+    }
+    sm[0][threadIdx.x] = tr;
+    sm[1][threadIdx.x] = ld;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + h];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sm[0][0];
+        part[kRedBlocks + blockIdx.x] = sm[1][0];
+    }
+}
+
+// stage 2: scal[4] = tr(G A G'), scal[2] += kappa (log(T / n) - (2/n) sum log G_ii)
+__global__ __launch_bounds__(256) void k_kap_final(const double *__restrict__ part, int n, double kappa,
+                                                   double *__restrict__ scal) {
+    __shared__ double sm[2][256];
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < kRedBlocks; q += 256) {
+        a += part[q];
+        b += part[kRedBlocks + q];
+    }
+    sm[0][threadIdx.x] = a;
+    sm[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + h];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[4] = sm[0][0];
+        scal[2] += kappa * (log(sm[0][0] / n) - 2.0 / n * sm[1][0]);
+    }
+}
+
+// stage 3: dL_K/dG_ij = kappa ((2/T) (G A)_ij - (2/n) [j == i] / G_ii)
+__global__ __launch_bounds__(256) void k_kap_grad(const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                  const float *__restrict__ gv, const float *__restrict__ gaE,
+                                                  const double *__restrict__ scal, double kappa, int n,
+                                                  float *__restrict__ dG) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double T = scal[4];
+    for (int e = gptr[i]; e < gptr[i + 1]; ++e)
+        dG[e] += (float)(kappa * (2.0 / T * (double)gaE[e] - (gcol[e] == i ? 2.0 / n / (double)gv[e] : 0.0)));
+}
+
+// Backward of the readouts: per node (32 lanes), the entries of its row (node as i) and of its
+// column (node as j, transposed pattern); weight gradients per block into part.
+__global__ __launch_bounds__(256) void k_mp_readout_bwd(const float *__restrict__ h3, const float *__restrict__ w,
+                                                        const int *__restrict__ gptr, const int *__restrict__ gcol,
+                                                        const int *__restrict__ gtptr, const int *__restrict__ gtrow,
+                                                        const int *__restrict__ gtidx, const float *__restrict__ ge1,
+                                                        const float *__restrict__ ge2, const float *__restrict__ gv,
+                                                        const float *__restrict__ dG, const double *__restrict__ diag,
+                                                        int n, float *__restrict__ dH, float *__restrict__ part) {
+    __shared__ float sm[kNB][W];
+    const int g = threadIdx.x / W, f = threadIdx.x % W;
+    const float *rd = w + kMpOffT;  // w_t, u1, v1, w1, u2, v2, w2
+    float acc[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] = 0.0f;
+    for (int base = blockIdx.x * kNB; base < n; base += gridDim.x * kNB) {
+        const int i = base + g;
+        if (i >= n) continue;
+        const float hi = h3[(int64_t)i * W + f];
+        float dh = 0.0f;
+        for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+            const int j = gcol[e];
+            if (j == i) {
+                const float dt = dG[e] * gv[e];
+                acc[0] += dt * hi;
+                dh += dt * rd[f];
+                continue;
+            }
+            const float hj = h3[(int64_t)j * W + f];
+            const float sj = (float)(1.0 / sqrt(diag[j]));
+            const float d1 = dG[e] * ge1[e] * sj, d2 = dG[e] * ge2[e] * sj;
+            acc[1] += d1 * hi * hj;
+            acc[2] += d1 * hi;
+            acc[3] += d1 * hj;
+            acc[4] += d2 * hi * hj;
+            acc[5] += d2 * hi;
+            acc[6] += d2 * hj;
+            dh += d1 * (rd[W + f] * hj + rd[2 * W + f]) + d2 * (rd[4 * W + f] * hj + rd[5 * W + f]);
+        }
+        const float si = (float)(1.0 / sqrt(diag[i]));
+        for (int t = gtptr[i]; t < gtptr[i + 1]; ++t) {
+            const int k = gtrow[t];  // entry (k, i) with k > i
+            if (k == i) continue;
+            const int e = gtidx[t];
+            const float hk = h3[(int64_t)k * W + f];
+            const float d1 = dG[e] * ge1[e] * si, d2 = dG[e] * ge2[e] * si;
+            dh += d1 * (rd[W + f] * hk + rd[3 * W + f]) + d2 * (rd[4 * W + f] * hk + rd[6 * W + f]);
+        }
+        dH[(int64_t)i * W + f] = dh;
+    }
+    float *dst = part + (int64_t)blockIdx.x * kMpP + kMpOffT;
+    for (int q = 0; q < 7; ++q) group_reduce_add(acc[q], sm, dst + q * W);
+}
+
+__global__ __launch_bounds__(256) void k_mp_in_bwd(const float *__restrict__ x, const float *__restrict__ dH0, int n,
+                                                   float *__restrict__ part) {
+    __shared__ float sm[kNB][W];
+    const int g = threadIdx.x / W, f = threadIdx.x % W;
+    float a[3] = {0.0f, 0.0f, 0.0f};
+    for (int base = blockIdx.x * kNB; base < n; base += gridDim.x * kNB) {
+        const int i = base + g;
+        if (i >= n) continue;
+        const float d = dH0[(int64_t)i * W + f];
+        for (int c = 0; c < 3; ++c) a[c] += d * x[3 * i + c];
+    }
+    // W_in[f][c] at 3f + c: reduce each c, then scatter
+    for (int c = 0; c < 3; ++c) {
+        sm[g][f] = a[c];
+        __syncthreads();
+        if (g == 0) {
+            float t = 0.0f;
+            for (int q = 0; q < kNB; ++q) t += sm[q][f];
+            part[(int64_t)blockIdx.x * kMpP + 3 * f + c] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// BF16 copy: for layer l, output block nb (features 16nb..16nb+15), k chunk kk, lane, i:
+// B[k][col] = [W1_l | W2_l][col][k] with k = 16kk + 4(lane/16) + i, col = 16nb + lane%16
+// (the B-operand layout of v_mfma_f32_16x16x16_bf16); then W_in and the readouts.
+__global__ void k_mp_pack(const float *__restrict__ w, uint16_t *__restrict__ wb) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < kMpBf16Vec) {
+        const int i = idx & 3, lane = (idx >> 2) & 63, kk = (idx >> 8) & 3, nb = (idx >> 10) & 1, l = idx >> 11;
+        const int k = 16 * kk + 4 * (lane >> 4) + i, col = 16 * nb + (lane & 15);
+        const float *wl = w + kMpOffW + l * 2 * W * W;
+        const float v = k < W ? wl[col * W + k] : wl[W * W + col * W + (k - W)];
+        wb[idx] = f2bf(v);
+    } else if (idx < kMpBf16Vec + 3 * W + 7 * W) {
+        const int t = idx - kMpBf16Vec;  // W_in (96), then the readouts (224)
+        wb[idx] = f2bf(t < 3 * W ? w[t] : w[kMpOffT + (t - 3 * W)]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mpb_input(const float *__restrict__ x, const uint16_t *__restrict__ wv, int n,
+                                                   uint16_t *__restrict__ H0) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int i = (int)(e / W), f = (int)(e % W);
+    if (i >= n) return;
+    H0[e] = f2bf(bf2f(wv[3 * f]) * x[3 * i] + bf2f(wv[3 * f + 1]) * x[3 * i + 1] + bf2f(wv[3 * f + 2]) * x[3 * i + 2]);
+}
+
+void ensure_ws(Mpnn &net, int n, int64_t ng) {
+    if (!net.part.n) {
+        net.part.alloc((size_t)kMpBlocks * kMpP);
+        net.dpart.alloc(2 * kRedBlocks + 3 * kRedBlocks);
+        net.scal.alloc(8);
+    }
+    if (net.ws_n >= n && net.ws_ng >= ng) return;
+    const size_t nw = (size_t)std::max(n, 1) * W, nn = std::max(n, 1), gg = (size_t)std::max<int64_t>(ng, 1);
     net.H.alloc(4 * nw);
     net.Mb.alloc(3 * nw);
     net.Pre.alloc(3 * nw);
@@ -523,53 +678,136 @@ void ensure_ws(Mpnn &net, int n) {
     net.dM.alloc(nw);
     net.hb0.alloc(nw);
     net.hb1.alloc(nw);
-    net.z.alloc(std::max(n, 1));
-    net.az.alloc(std::max(n, 1));
+    net.gvf.alloc(gg);
+    net.dG.alloc(gg);
+    net.gaE.alloc(gg);
+    net.gv.alloc(gg);
+    net.gvt.alloc(gg);
+    net.gv_graph = nullptr;
+    net.y.alloc(nn);
+    net.z.alloc(nn);
+    net.az.alloc(nn);
+    net.u.alloc(nn);
     net.ws_n = n;
-}
-
-void ensure_fixed(Mpnn &net) {
-    if (net.part.n) return;
-    net.part.alloc((size_t)kMpBlocks * kMpP);
-    net.dpart.alloc(3 * kRedBlocks);
-    net.scal.alloc(8);
+    net.ws_ng = ng;
 }
 
 }  // namespace
 
 // ---- host side -------------------------------------------------------------------
 
+// Graph data of the learned factor, built on the host from the full CSR in the same order
+// as the oracle's restatement (rows ascending; the factor pattern of row i is i, its lower
+// neighbours and their neighbours below i, sorted).
 void mp_graph_prepare(Matrix &A, hipStream_t st) {
     if (A.mp && A.mp->n == A.n) return;
     if (A.diag_status)
         throw Error(MG_E_NOT_PD, "The diagonal element is zero or invalid while building the learned preconditioner.");
-    auto g = std::make_shared<MpGraph>();
     const int n = A.n;
-    g->n = n;
-    g->ew.alloc(std::max<int64_t>(A.nnz_full, 1));
-    g->cf.alloc(std::max(n, 1));
-    g->diag.alloc(std::max(n, 1));
-    DevArray<double> part(kRedBlocks), mean(1);
-    hipLaunchKernelGGL(k_mp_diag, dim3(kRedBlocks), dim3(256), 0, st, A.dpos.p, A.vals.p, n, g->diag.p, part.p);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, part.p, kRedBlocks, n ? 1.0 / n : 1.0, mean.p);
-    if (n > 0)
-        hipLaunchKernelGGL(k_mp_graph, dim3(nblk(n, 256)), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p,
-                           g->diag.p, mean.p, n, g->ew.p, g->cf.p);
-    MG_HIP(hipGetLastError());
+    const int64_t nnz = A.nnz_full;
+    std::vector<int> rp(n + 1), ci(std::max<int64_t>(nnz, 1));
+    std::vector<double> av(std::max<int64_t>(nnz, 1));
     MG_HIP(hipStreamSynchronize(st));
+    MG_HIP(hipMemcpy(rp.data(), A.rowptr.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    if (nnz) {
+        MG_HIP(hipMemcpy(ci.data(), A.colidx.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+        MG_HIP(hipMemcpy(av.data(), A.vals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> d(n, 0.0);
+    double mean = 0.0;
+    for (int i = 0; i < n; ++i) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) d[i] = av[k];
+        mean += d[i];
+    }
+    mean = n ? mean / n : 1.0;
+    std::vector<float> ew(std::max<int64_t>(nnz, 1), 0.0f), x(3 * (size_t)std::max(n, 1));
+    std::vector<double> e1(std::max<int64_t>(nnz, 1), 0.0);
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (j == i) continue;
+            e1[k] = av[k] / std::sqrt(d[i] * d[j]);
+            ew[k] = (float)e1[k];
+            s += std::fabs(e1[k]);
+        }
+        x[3 * i] = 1.0f;
+        x[3 * i + 1] = (float)std::log(d[i] / mean);
+        x[3 * i + 2] = (float)s;
+    }
+    std::vector<int> gptr(n + 1, 0), gcol;
+    std::vector<float> ge1, ge2;
+    std::vector<double> w1(std::max(n, 1), 0.0), w2(std::max(n, 1), 0.0);
+    std::vector<char> seen(std::max(n, 1), 0);
+    std::vector<int> cols;
+    for (int i = 0; i < n; ++i) {
+        cols.clear();
+        seen[i] = 1;
+        cols.push_back(i);
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (j == i) continue;
+            if (j < i) {
+                w1[j] += e1[k];
+                if (!seen[j]) { seen[j] = 1; cols.push_back(j); }
+            }
+            for (int q = rp[j]; q < rp[j + 1]; ++q) {
+                const int l = ci[q];
+                if (l == j || l >= i) continue;
+                w2[l] += e1[k] * e1[q];
+                if (!seen[l]) { seen[l] = 1; cols.push_back(l); }
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            gcol.push_back(j);
+            ge1.push_back((float)w1[j]);
+            ge2.push_back((float)w2[j]);
+            w1[j] = w2[j] = 0.0;
+            seen[j] = 0;
+        }
+        gptr[i + 1] = (int)gcol.size();
+    }
+    const int64_t ng = (int64_t)gcol.size();
+    std::vector<int> gtptr(n + 1, 0), gtrow(std::max<int64_t>(ng, 1)), gtidx(std::max<int64_t>(ng, 1));
+    for (int64_t e = 0; e < ng; ++e) ++gtptr[gcol[e] + 1];
+    for (int j = 0; j < n; ++j) gtptr[j + 1] += gtptr[j];
+    {
+        std::vector<int> at(gtptr.begin(), gtptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+                const int t = at[gcol[e]]++;
+                gtrow[t] = i;
+                gtidx[t] = e;
+            }
+    }
+    auto g = std::make_shared<MpGraph>();
+    g->n = n;
+    g->ng = ng;
+    g->ew.upload(ew);
+    g->x.upload(x);
+    g->diag.upload(d);
+    g->gptr.upload(gptr);
+    g->gcol.upload(gcol);
+    g->ge1.upload(ge1);
+    g->ge2.upload(ge2);
+    g->gtptr.upload(gtptr);
+    g->gtrow.upload(gtrow);
+    g->gtidx.upload(gtidx);
     A.mp = g;
 }
 
-// Deterministic initialisation: W1, W2 ~ N(0, 1/(2*32)), w_e, w_c ~ N(0, 1), w_d ~ N(0, 0.01^2)
-// (a small decoder starts the correction near plain Jacobi).
+// Deterministic initialisation: W_in ~ N(0, 1), W1, W2 ~ N(0, 1/(2*32)), readouts ~ N(0, 0.01^2)
+// (small readouts start the factor near Jacobi).
 void mpnn_init(Mpnn &net, uint64_t seed, hipStream_t st) {
     std::mt19937_64 gen(seed);
     std::normal_distribution<double> nd(0.0, 1.0);
     std::vector<float> w(kMpP);
     for (int p = 0; p < kMpP; ++p) {
         double sd = 1.0;
-        if (p >= kMpOffW && p < kMpOffWd) sd = 1.0 / std::sqrt(2.0 * W);
-        else if (p >= kMpOffWd) sd = 0.01;
+        if (p >= kMpOffW && p < kMpOffT) sd = 1.0 / std::sqrt(2.0 * W);
+        else if (p >= kMpOffT) sd = 0.01;
         w[p] = (float)(nd(gen) * sd);
     }
     net.w.upload(w);
@@ -578,50 +816,76 @@ void mpnn_init(Mpnn &net, uint64_t seed, hipStream_t st) {
     net.am.upload(zero);
     net.av.upload(zero);
     net.adam_t = 0;
-    net.wb.alloc(3 * kMpBf16Layer + 3 * W);
+    net.wb.alloc(kMpBf16Vec + 10 * W);
     mpnn_pack_bf16(net, st);
 }
 
 void mpnn_pack_bf16(Mpnn &net, hipStream_t st) {
-    const int total = 3 * kMpBf16Layer + 3 * W;
+    const int total = kMpBf16Vec + 10 * W;
     hipLaunchKernelGGL(k_mp_pack, dim3(nblk(total, 256)), dim3(256), 0, st, net.w.p, net.wb.p);
     MG_HIP(hipGetLastError());
+    net.gv_graph = nullptr;  // factor values are stale
 }
 
 namespace {
 
-// forward in FP32 keeping the activations for the backward pass; z into d_z, ||r|| in scal[0]
-void forward_f32(Mpnn &net, Matrix &A, const double *d_r, double *d_z, hipStream_t st) {
+// FP32 forward of the network and the factor values (gvf), keeping the activations
+void forward_f32(Mpnn &net, Matrix &A, hipStream_t st) {
     const int n = A.n;
     const MpGraph &g = *A.mp;
-    const double rsqn = 1.0 / std::sqrt((double)std::max(n, 1));
     const int64_t nw = (int64_t)n * W;
-    hipLaunchKernelGGL(k_mp_sq_part, dim3(kRedBlocks), dim3(256), 0, st, d_r, n, net.dpart.p);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, net.dpart.p, kRedBlocks, 1.0, net.scal.p);
-    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(1), 0, st, net.scal.p);
-    hipLaunchKernelGGL(k_mp_input, dim3(nblk(nw, 256)), dim3(256), 0, st, d_r, net.scal.p, rsqn, g.cf.p, net.w.p, n,
-                       net.H.p);
+    hipLaunchKernelGGL(k_mp_input, dim3(nblk(nw, 256)), dim3(256), 0, st, g.x.p, net.w.p, n, net.H.p);
     for (int l = 0; l < 3; ++l)
         hipLaunchKernelGGL(k_mp_layer, dim3(kMpBlocks), dim3(256), 0, st, net.w.p + kMpOffW + l * 2 * W * W,
                            A.rowptr.p, A.colidx.p, g.ew.p, n, net.H.p + l * nw, net.Mb.p + l * nw, net.Pre.p + l * nw,
                            net.H.p + (l + 1) * nw, l < 2 ? 1 : 0);
-    hipLaunchKernelGGL(k_mp_output, dim3(nblk(nw, 256)), dim3(256), 0, st, d_r, net.scal.p, rsqn, g.diag.p,
-                       net.w.p + kMpOffWd, net.H.p + 3 * nw, n, d_z);
+    hipLaunchKernelGGL((k_mp_readout<float, float, float>), dim3(kMpBlocks), dim3(256), 0, st, net.H.p + 3 * nw,
+                       net.w.p + kMpOffT, g.gptr.p, g.gcol.p, g.ge1.p, g.ge2.p, g.diag.p, n, net.gvf.p);
 }
 
-// loss and gradient of one pair, accumulated into net.part and scal[2]
+void begin_batch(Mpnn &net, Matrix &A, hipStream_t st) {
+    const int64_t cnt = (int64_t)kMpBlocks * kMpP;
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblk(cnt, 256)), dim3(256), 0, st, net.part.p, cnt, 0.0f);
+    const int64_t ng = A.mp->ng;
+    if (ng) hipLaunchKernelGGL(k_fill_f32, dim3(nblk(ng, 256)), dim3(256), 0, st, net.dG.p, ng, 0.0f);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(3), 0, st, net.scal.p, 3, 0.0);
+}
+
+// pair term of one sample: loss into scal[2], dL/dG into dG
 void sample_grad(Mpnn &net, Matrix &A, const double *d_r, const double *d_e, int ns, hipStream_t st) {
     const int n = A.n;
     const MpGraph &g = *A.mp;
-    const double rsqn = 1.0 / std::sqrt((double)std::max(n, 1));
-    const int64_t nw = (int64_t)n * W;
-    forward_f32(net, A, d_r, net.z.p, st);
+    const dim3 gn(nblk(n, 256));
+    hipLaunchKernelGGL((k_g_apply<float>), gn, dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, d_r, n, net.y.p,
+                       (const PcgScalars *)nullptr);
+    hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p, g.gtidx.p,
+                       net.gvf.p, (const float *)nullptr, net.y.p, n, net.z.p, (const double *)nullptr,
+                       (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
     spmv_device(A, 1, net.z.p, n, net.az.p, n, st);
     hipLaunchKernelGGL(k_mp_loss_part, dim3(kRedBlocks), dim3(256), 0, st, net.z.p, net.az.p, d_r, d_e, n,
                        net.dpart.p);
     hipLaunchKernelGGL(k_mp_loss_final, dim3(1), dim3(256), 0, st, net.dpart.p, ns, net.scal.p);
-    hipLaunchKernelGGL(k_mp_out_bwd, dim3(kMpBlocks), dim3(256), 0, st, d_r, net.az.p, net.scal.p, rsqn, g.diag.p,
-                       net.scal.p, ns, net.w.p + kMpOffWd, net.H.p + 3 * nw, n, net.dH.p, net.part.p);
+    hipLaunchKernelGGL(k_u_apply, gn, dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, net.az.p, d_r, net.scal.p, ns,
+                       n, net.u.p);
+    hipLaunchKernelGGL(k_dg_acc, gn, dim3(256), 0, st, g.gptr.p, g.gcol.p, net.y.p, net.u.p, net.az.p, d_r,
+                       net.scal.p, ns, n, net.dG.p);
+}
+
+// Kaporin term, readout and network backward, gradient reduction (after the samples of a batch)
+void finish_batch(Mpnn &net, Matrix &A, double kappa, hipStream_t st) {
+    const int n = A.n;
+    const MpGraph &g = *A.mp;
+    const int64_t nw = (int64_t)n * W;
+    if (kappa != 0.0 && n > 0) {
+        hipLaunchKernelGGL(k_kap_rows, dim3(kRedBlocks), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, A.rowptr.p,
+                           A.colidx.p, A.vals.p, n, net.gaE.p, net.dpart.p);
+        hipLaunchKernelGGL(k_kap_final, dim3(1), dim3(256), 0, st, net.dpart.p, n, kappa, net.scal.p);
+        hipLaunchKernelGGL(k_kap_grad, dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, net.gaE.p,
+                           net.scal.p, kappa, n, net.dG.p);
+    }
+    hipLaunchKernelGGL(k_mp_readout_bwd, dim3(kMpBlocks), dim3(256), 0, st, net.H.p + 3 * nw, net.w.p, g.gptr.p,
+                       g.gcol.p, g.gtptr.p, g.gtrow.p, g.gtidx.p, g.ge1.p, g.ge2.p, net.gvf.p, net.dG.p, g.diag.p, n,
+                       net.dH.p, net.part.p);
     float *dh = net.dH.p, *dhp = net.dHp.p;
     for (int l = 2; l >= 0; --l) {
         hipLaunchKernelGGL(k_mp_layer_bwd, dim3(kMpBlocks), dim3(256), 0, st, net.w.p + kMpOffW + l * 2 * W * W, n,
@@ -631,71 +895,83 @@ void sample_grad(Mpnn &net, Matrix &A, const double *d_r, const double *d_e, int
                            net.dM.p, dhp);
         std::swap(dh, dhp);
     }
-    hipLaunchKernelGGL(k_mp_in_bwd, dim3(kMpBlocks), dim3(256), 0, st, d_r, net.scal.p, rsqn, g.cf.p, dh, n,
-                       net.part.p);
-}
-
-void begin_batch(Mpnn &net, hipStream_t st) {
-    const int64_t cnt = (int64_t)kMpBlocks * kMpP;
-    hipLaunchKernelGGL(k_fill_f32, dim3(nblk(cnt, 256)), dim3(256), 0, st, net.part.p, cnt, 0.0f);
-    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(3), 0, st, net.scal.p, 3, 0.0);
-}
-
-void end_batch(Mpnn &net, hipStream_t st) {
+    hipLaunchKernelGGL(k_mp_in_bwd, dim3(kMpBlocks), dim3(256), 0, st, g.x.p, dh, n, net.part.p);
     hipLaunchKernelGGL(k_mp_grad_reduce, dim3(nblk(kMpP, 256)), dim3(256), 0, st, net.part.p, kMpBlocks, net.grad.p);
 }
 
+__global__ void k_mp_epoch_acc(double *__restrict__ scal, double weight) { scal[3] += scal[2] * weight; }
+
 }  // namespace
 
-void mpnn_apply_f32(Mpnn &net, Matrix &A, const double *d_r, double *d_z, hipStream_t st) {
+void mpnn_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, int precision, hipStream_t st) {
     mp_graph_prepare(A, st);
-    ensure_ws(net, A.n);
-    ensure_fixed(net);
-    if (A.n == 0) return;
-    forward_f32(net, A, d_r, d_z, st);
-    MG_HIP(hipGetLastError());
-}
-
-void mpnn_apply_bf16(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const double *d_rnorm,
-                     const PcgScalars *S, const double *d_ap, double *rz_part, double *zap_part, hipStream_t st) {
-    mp_graph_prepare(A, st);
-    ensure_ws(net, A.n);
-    ensure_fixed(net);
+    ensure_ws(net, A.n, A.mp->ng);
     const int n = A.n;
     if (n == 0) return;
     const MpGraph &g = *A.mp;
-    const double rsqn = 1.0 / std::sqrt((double)n);
-    const int64_t nw = (int64_t)n * W;
-    if (!d_rnorm) {
-        hipLaunchKernelGGL(k_mp_sq_part, dim3(kRedBlocks), dim3(256), 0, st, d_r, n, net.dpart.p);
-        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, net.dpart.p, kRedBlocks, 1.0, net.scal.p);
-        hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(1), 0, st, net.scal.p);
-        d_rnorm = net.scal.p;
+    if (precision == MG_BF16) {
+        mpnn_factor_bf16(net, A, st);
+        hipLaunchKernelGGL((k_g_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gv.p,
+                           d_r, n, net.y.p, (const PcgScalars *)nullptr);
+        hipLaunchKernelGGL((k_gt_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
+                           g.gtidx.p, net.gv.p, net.gvt.p, net.y.p, n, d_z, (const double *)nullptr,
+                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
+    } else {
+        forward_f32(net, A, st);
+        hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p,
+                           d_r, n, net.y.p, (const PcgScalars *)nullptr);
+        hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
+                           g.gtidx.p, net.gvf.p, (const float *)nullptr, net.y.p, n, d_z, (const double *)nullptr,
+                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
     }
-    const uint16_t *wv = net.wb.p + 3 * kMpBf16Layer;
-    hipLaunchKernelGGL(k_mpb_input, dim3(nblk(nw, 256)), dim3(256), 0, st, d_r, d_rnorm, rsqn, g.cf.p, wv, n,
-                       net.hb0.p, S);
+    MG_HIP(hipGetLastError());
+}
+
+void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st) {
+    mp_graph_prepare(A, st);
+    ensure_ws(net, A.n, A.mp->ng);
+    const MpGraph &g = *A.mp;
+    if (net.gv_graph == &g) return;
+    const int n = A.n;
+    if (n == 0) return;
+    const int64_t nw = (int64_t)n * W;
+    const uint16_t *wv = net.wb.p + kMpBf16Vec;
+    hipLaunchKernelGGL(k_mpb_input, dim3(nblk(nw, 256)), dim3(256), 0, st, g.x.p, wv, n, net.hb0.p);
     uint16_t *hin = net.hb0.p, *hout = net.hb1.p;
     const int tiles = (n + 15) / 16;
     const int grid = std::min(nblk(tiles, 4), 2048);
     for (int l = 0; l < 3; ++l) {
         hipLaunchKernelGGL(k_mpb_layer, dim3(grid), dim3(256), 0, st, net.wb.p + l * kMpBf16Layer, A.rowptr.p,
-                           A.colidx.p, g.ew.p, n, hin, hout, l < 2 ? 1 : 0, S);
+                           A.colidx.p, g.ew.p, n, hin, hout, l < 2 ? 1 : 0, (const PcgScalars *)nullptr);
         std::swap(hin, hout);
     }
-    hipLaunchKernelGGL(k_mpb_output, dim3(kRedBlocks), dim3(256), 0, st, d_r, d_rnorm, rsqn, g.diag.p, wv, hin, n,
-                       d_z, d_ap, rz_part, zap_part, S);
+    hipLaunchKernelGGL((k_mp_readout<uint16_t, uint16_t, double>), dim3(kMpBlocks), dim3(256), 0, st, hin,
+                       wv + 3 * W, g.gptr.p, g.gcol.p, g.ge1.p, g.ge2.p, g.diag.p, n, net.gv.p);
+    if (g.ng)
+        hipLaunchKernelGGL(k_gather_t, dim3(nblk(g.ng, 256)), dim3(256), 0, st, g.gtidx.p, net.gv.p, g.ng, net.gvt.p);
     MG_HIP(hipGetLastError());
+    net.gv_graph = &g;
 }
 
-double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld,
+void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const PcgScalars *S, const double *d_ap,
+                       double *rz_part, double *zap_part, hipStream_t st) {
+    const MpGraph &g = *A.mp;
+    const int n = A.n;
+    hipLaunchKernelGGL((k_g_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gv.p, d_r,
+                       n, net.y.p, S);
+    hipLaunchKernelGGL((k_gt_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p, g.gtidx.p,
+                       net.gv.p, net.gvt.p, net.y.p, n, d_z, d_r, d_ap, rz_part, zap_part, S);
+}
+
+double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, double kappa,
                       hipStream_t st) {
     mp_graph_prepare(A, st);
-    ensure_ws(net, A.n);
-    ensure_fixed(net);
-    begin_batch(net, st);
+    ensure_ws(net, A.n, A.mp->ng);
+    if (A.n == 0) return 0.0;
+    begin_batch(net, A, st);
+    forward_f32(net, A, st);
     for (int t = 0; t < ns; ++t) sample_grad(net, A, d_R + t * ld, d_E + t * ld, ns, st);
-    end_batch(net, st);
+    finish_batch(net, A, kappa, st);
     MG_HIP(hipGetLastError());
     double scal[8];
     MG_HIP(hipMemcpyAsync(scal, net.scal.p, sizeof(scal), hipMemcpyDeviceToHost, st));
@@ -703,19 +979,15 @@ double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const dou
     return scal[2];
 }
 
-namespace {
-__global__ void k_mp_epoch_acc(double *__restrict__ scal, double weight) { scal[3] += scal[2] * weight; }
-}  // namespace
-
 // Mini-batch Adam over the ns pairs (columns of R and E), `epochs` passes in a seeded
 // shuffled order; losses[e] = mean training loss of epoch e (before each batch's update).
 void mpnn_train(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, int epochs, int batch,
-                float lr, uint64_t seed, double *losses, hipStream_t st) {
+                float lr, double kappa, uint64_t seed, double *losses, hipStream_t st) {
     if (ns <= 0 || epochs <= 0) return;
     if (batch <= 0) batch = 16;
     mp_graph_prepare(A, st);
-    ensure_ws(net, A.n);
-    ensure_fixed(net);
+    ensure_ws(net, A.n, A.mp->ng);
+    if (A.n == 0) return;
     std::mt19937_64 gen(seed);
     std::vector<int> order(ns);
     for (int t = 0; t < ns; ++t) order[t] = t;
@@ -724,12 +996,13 @@ void mpnn_train(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(8), 0, st, net.scal.p, 8, 0.0);
         for (int b0 = 0; b0 < ns; b0 += batch) {
             const int nb = std::min(batch, ns - b0);
-            begin_batch(net, st);
+            begin_batch(net, A, st);
+            forward_f32(net, A, st);
             for (int t = 0; t < nb; ++t) {
                 const int s = order[b0 + t];
                 sample_grad(net, A, d_R + s * ld, d_E + s * ld, nb, st);
             }
-            end_batch(net, st);
+            finish_batch(net, A, kappa, st);
             hipLaunchKernelGGL(k_mp_epoch_acc, dim3(1), dim3(1), 0, st, net.scal.p, (double)nb / ns);
             mpnn_adam_step(net, lr, st);
         }

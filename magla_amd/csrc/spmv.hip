@@ -167,6 +167,12 @@ __global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ a, 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+__device__ inline double finish_sum_n(const double *part, int nparts, double *sm) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBS) acc += part[b];
+    return block_sum(acc, sm);
+}
+
 __device__ inline double finish_sum(const double *part, double *sm) {
     double acc = 0.0;
     for (int b = threadIdx.x; b < kRedBlocks; b += kBS) acc += part[b];
@@ -380,12 +386,13 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
 // ---- flexible PCG with the learned preconditioner --------------------------------
 //
 // ConjugateGradientGaussPreSolver's iteration (CGGP/ConjugateGradientGaussPreSolver.cs:78-166)
-// with z = M(r) from the message-passing network on top of Jacobi (mpnn.hip).  M is not a
-// fixed linear operator, so the new direction is A-orthogonalised against the previous one
-// (flexible CG): p = z + beta p with beta = -(z'Ap) / (p'Ap).  The stopping rule, the
-// initial residual and the p'Ap <= 0 failure are the reference's.  Fallback to plain
-// Jacobi-PCG (restart with p = z = D^{-1} r) when r'z <= 0 or when ||r|| has not reached a
-// new minimum for `stall` iterations.
+// with z = G'G r, G the network-predicted factor (mpnn.hip).  The new direction is
+// A-orthogonalised against the previous one, p = z + beta p with beta = -(z'Ap) / (p'Ap)
+// (flexible CG; the same as PCG for a fixed preconditioner, and robust when the
+// preconditioner changes at the fallback).  The stopping rule, the initial residual and the
+// p'Ap <= 0 failure are the reference's.  Fallback to plain Jacobi-PCG (restart with
+// p = z = D^{-1} r) when r'z <= 0 or when ||r|| has not reached a new minimum for `stall`
+// iterations.
 
 namespace mg {
 
@@ -435,11 +442,11 @@ __global__ __launch_bounds__(256) void k_fcg_jacobi(const double *__restrict__ i
 }
 
 // first direction: rz = r'z; a non-positive r'z switches to Jacobi before the first step
-__global__ void k_fcg_start(const double *__restrict__ rz_part, PcgScalars *__restrict__ S, int second) {
+__global__ void k_fcg_start(const double *__restrict__ rz_part, int nparts, PcgScalars *__restrict__ S, int second) {
     __shared__ double sm[kBS];
     if (S->done) return;
     if (second && !S->fallback) return;
-    const double rz = finish_sum(rz_part, sm);
+    const double rz = finish_sum_n(rz_part, nparts, sm);
     if (threadIdx.x == 0) {
         S->rz = rz;
         S->best = S->rnorm;
@@ -449,13 +456,13 @@ __global__ void k_fcg_start(const double *__restrict__ rz_part, PcgScalars *__re
 }
 
 // beta = -(z'Ap) / (p'Ap) (0 on a restart); rz = r'z for the next step length
-__global__ void k_fcg_beta(const double *__restrict__ rz_part, const double *__restrict__ zap_part,
+__global__ void k_fcg_beta(const double *__restrict__ rz_part, const double *__restrict__ zap_part, int nparts,
                            PcgScalars *__restrict__ S) {
     __shared__ double sm[kBS];
     if (S->done) return;
-    const double rz = finish_sum(rz_part, sm);
+    const double rz = finish_sum_n(rz_part, nparts, sm);
     __syncthreads();
-    const double zap = finish_sum(zap_part, sm);
+    const double zap = finish_sum_n(zap_part, nparts, sm);
     if (threadIdx.x == 0) {
         S->rz = rz;
         if (S->restart || !(rz > 0.0)) {
@@ -484,10 +491,12 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
     mp_graph_prepare(A, st);
     if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    DevArray<double> r(n), z(n), p(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
+    // the preconditioner kernels write one partial sum per block of a full grid (n / 256 blocks)
+    const int npart = nblk(n, kBS);
+    DevArray<double> r(n), z(n), p(n), ap(n), part(std::max(npart, kRedBlocks)), part2(std::max(npart, kRedBlocks));
     DevArray<PcgScalars> S(1);
     MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
-    const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks);
+    const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks), gp(npart);
     launch_spmv(A, d_x, ap.p, st);
     hipLaunchKernelGGL(k_residual0, gv, bv, 0, st, d_b, ap.p, n, r.p);
     hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, r.p, r.p, n, part.p, (const PcgScalars *)nullptr);
@@ -497,13 +506,13 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
     MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
     if (!h.done) {
-        const double *rnorm = &S.p->rnorm;
-        // z0 = M(r0), then Jacobi instead if r0'z0 <= 0
-        mpnn_apply_bf16(net, A, r.p, z.p, rnorm, S.p, nullptr, part.p, nullptr, st);
-        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, S.p, 0);
-        hipLaunchKernelGGL(k_fcg_jacobi, gr, bv, 0, st, A.invdiag.p, r.p, (const double *)nullptr, z.p, n, part.p,
+        // factor values from the BF16 network (once per matrix and network), z0 = G'G r0
+        mpnn_factor_bf16(net, A, st);
+        mpnn_apply_factor(net, A, r.p, z.p, S.p, nullptr, part.p, part2.p, st);
+        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, npart, S.p, 0);
+        hipLaunchKernelGGL(k_fcg_jacobi, gp, bv, 0, st, A.invdiag.p, r.p, (const double *)nullptr, z.p, n, part.p,
                            part2.p, S.p);
-        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, S.p, 1);
+        hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, npart, S.p, 1);
         MG_HIP(hipMemcpyAsync(p.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         const int maxit = max_it > 0 ? max_it : n;
         const int check_every = 16;
@@ -514,9 +523,9 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
             hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
             hipLaunchKernelGGL(k_pcg_update, gr, bv, 0, st, d_x, r.p, p.p, ap.p, n, part.p, S.p);
             hipLaunchKernelGGL(k_fcg_norm, dim3(1), bv, 0, st, part.p, it, stall, S.p);
-            mpnn_apply_bf16(net, A, r.p, z.p, rnorm, S.p, ap.p, part.p, part2.p, st);
-            hipLaunchKernelGGL(k_fcg_jacobi, gr, bv, 0, st, A.invdiag.p, r.p, ap.p, z.p, n, part.p, part2.p, S.p);
-            hipLaunchKernelGGL(k_fcg_beta, dim3(1), bv, 0, st, part.p, part2.p, S.p);
+            mpnn_apply_factor(net, A, r.p, z.p, S.p, ap.p, part.p, part2.p, st);
+            hipLaunchKernelGGL(k_fcg_jacobi, gp, bv, 0, st, A.invdiag.p, r.p, ap.p, z.p, n, part.p, part2.p, S.p);
+            hipLaunchKernelGGL(k_fcg_beta, dim3(1), bv, 0, st, part.p, part2.p, npart, S.p);
             hipLaunchKernelGGL(k_pcg_pupdate, gv, bv, 0, st, z.p, p.p, n, S.p);
             if (it % check_every == 0 || it == maxit) {
                 MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));

@@ -60,9 +60,9 @@ def make_pairs(a: SymmetricCSRMatrix, dm: DeviceMatrix, count, seed, max_iterati
     return R, E
 
 
-def train(net: LearnedPreconditioner, dm: DeviceMatrix, R, E, epochs=20, batch=16, learning_rate=2e-3, seed=0,
-          stream=None):
+def train(net: LearnedPreconditioner, dm: DeviceMatrix, R, E, epochs=20, batch=16, learning_rate=3e-3, kappa=1.0,
+          seed=0, stream=None):
     """Adam on the GPU over the pairs (rows of R, E); returns the mean loss of each epoch."""
     count, n = R.shape
     return net.train_dev(dm, R.data_ptr(), E.data_ptr(), count, n, epochs=epochs, batch=batch,
-                         learning_rate=learning_rate, seed=seed, stream=stream)
+                         learning_rate=learning_rate, kappa=kappa, seed=seed, stream=stream)

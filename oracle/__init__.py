@@ -94,7 +94,8 @@ def lib():
         L.or_pcg.argtypes = [C.POINTER(_Csr), _f64p, C.c_void_p, C.c_double, C.c_int, C.c_int, _f64p,
                              C.POINTER(_PcgRes)]
         L.or_mpnn_apply.argtypes = [C.POINTER(_Csr), _f64p, _f64p, _f64p]
-        L.or_mpnn_loss_grad.argtypes = [C.POINTER(_Csr), _f64p, C.c_int, _f64p, _f64p, C.POINTER(C.c_double), _f64p]
+        L.or_mpnn_loss_grad.argtypes = [C.POINTER(_Csr), _f64p, C.c_int, _f64p, _f64p, C.c_double,
+                                        C.POINTER(C.c_double), _f64p]
         _lib = L
     return _lib
 
@@ -277,11 +278,11 @@ def pcg(csr: Csr, b, x0=None, tol=1e-8, max_iterations=0, use_jacobi=True):
 
 # ---- learned preconditioner network (checker; no reference counterpart) ----
 MP_W = 32
-MP_P = 2 * MP_W + 3 * 2 * MP_W * MP_W + MP_W
+MP_P = 3 * MP_W + 3 * 2 * MP_W * MP_W + 7 * MP_W
 
 
 def mpnn_apply(csr: Csr, params, r):
-    """z = (r + s * net(r)) / diag(A) in binary64 (see oracle.h for the network)."""
+    """z = G'G r with the network's factor G, in binary64 (see oracle.h for the network)."""
     params = np.ascontiguousarray(params, dtype=np.float64)
     assert params.size == MP_P
     r = np.ascontiguousarray(r, dtype=np.float64)
@@ -290,13 +291,15 @@ def mpnn_apply(csr: Csr, params, r):
     return z
 
 
-def mpnn_loss_grad(csr: Csr, params, R, E):
-    """Mean A-norm loss over the rows of R (residuals) and E (exact errors), and its gradient."""
+def mpnn_loss_grad(csr: Csr, params, R, E, kappa=0.0):
+    """Mean A-norm loss over the rows of R (residuals) and E (exact errors), plus kappa times
+    the log Kaporin condition number of G A G', and its gradient."""
     params = np.ascontiguousarray(params, dtype=np.float64)
     R = np.ascontiguousarray(np.atleast_2d(R), dtype=np.float64)
     E = np.ascontiguousarray(np.atleast_2d(E), dtype=np.float64)
     assert R.shape == E.shape and R.shape[1] == csr.n
     loss = C.c_double()
     grad = np.zeros(MP_P, dtype=np.float64)
-    _check(lib().or_mpnn_loss_grad(csr._p, params, R.shape[0], R, E, C.byref(loss), grad), "mpnn_loss_grad")
+    _check(lib().or_mpnn_loss_grad(csr._p, params, R.shape[0], R, E, float(kappa), C.byref(loss), grad),
+           "mpnn_loss_grad")
     return loss.value, grad

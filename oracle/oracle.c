@@ -1348,14 +1348,19 @@ done:
 
 typedef struct {
     int n;
-    int *ptr, *nbr;   /* full neighbour lists without the diagonal */
-    double *ew;       /* a_ij / sqrt(a_ii a_jj) */
-    double *diag, *cf;/* a_ii, log(a_ii / mean) */
+    int *ptr, *nbr;          /* full neighbour lists without the diagonal, ascending */
+    double *ew;              /* e1_ij */
+    double *diag, *x;        /* a_ii; node features [1, c_i, s_i] */
+    int *gptr, *gcol;        /* pattern of G: row i = {j <= i within two hops}, ascending (diagonal last) */
+    double *ge1, *ge2;       /* e1_ij, e2_ij on the pattern of G */
 } mp_graph;
 
 static void mp_graph_free(mp_graph *g) {
-    free(g->ptr); free(g->nbr); free(g->ew); free(g->diag); free(g->cf);
+    free(g->ptr); free(g->nbr); free(g->ew); free(g->diag); free(g->x);
+    free(g->gptr); free(g->gcol); free(g->ge1); free(g->ge2);
 }
+
+static int cmp_int(const void *a, const void *b) { return (*(const int *)a > *(const int *)b) - (*(const int *)a < *(const int *)b); }
 
 static int mp_graph_build(const or_csr *a, mp_graph *g) {
     const int n = a->n;
@@ -1363,8 +1368,9 @@ static int mp_graph_build(const or_csr *a, mp_graph *g) {
     g->n = n;
     g->ptr = (int *)calloc((size_t)n + 1, sizeof(int));
     g->diag = (double *)calloc((size_t)n + 1, sizeof(double));
-    g->cf = (double *)calloc((size_t)n + 1, sizeof(double));
-    if (!g->ptr || !g->diag || !g->cf) return OR_E_NOMEM;
+    g->x = (double *)calloc(3 * (size_t)n + 1, sizeof(double));
+    g->gptr = (int *)calloc((size_t)n + 1, sizeof(int));
+    if (!g->ptr || !g->diag || !g->x || !g->gptr) return OR_E_NOMEM;
     for (int i = 0; i < n; ++i)
         for (int k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
             const int j = a->colidx[k];
@@ -1378,36 +1384,95 @@ static int mp_graph_build(const or_csr *a, mp_graph *g) {
     int *fill = (int *)malloc(sizeof(int) * (size_t)(n + 1));
     if (!g->nbr || !g->ew || !fill) { free(fill); return OR_E_NOMEM; }
     memcpy(fill, g->ptr, sizeof(int) * (size_t)n);
+    /* ascending rows: lower neighbours (upper entries of earlier rows) first, then upper entries */
     for (int i = 0; i < n; ++i)
         for (int k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
             const int j = a->colidx[k];
             if (j == i) continue;
-            const double w = a->values[k] / sqrt(g->diag[i] * g->diag[j]);
-            g->nbr[fill[i]] = j; g->ew[fill[i]++] = w;
-            g->nbr[fill[j]] = i; g->ew[fill[j]++] = w;
+            g->nbr[fill[j]] = i; g->ew[fill[j]++] = a->values[k] / sqrt(g->diag[i] * g->diag[j]);
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
+            const int j = a->colidx[k];
+            if (j == i) continue;
+            g->nbr[fill[i]] = j; g->ew[fill[i]++] = a->values[k] / sqrt(g->diag[i] * g->diag[j]);
         }
     free(fill);
     double mean = 0.0;
     for (int i = 0; i < n; ++i) mean += g->diag[i];
     mean = n ? mean / n : 1.0;
-    for (int i = 0; i < n; ++i) g->cf[i] = log(g->diag[i] / mean);
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = g->ptr[i]; k < g->ptr[i + 1]; ++k) s += fabs(g->ew[k]);
+        g->x[3 * i] = 1.0;
+        g->x[3 * i + 1] = log(g->diag[i] / mean);
+        g->x[3 * i + 2] = s;
+    }
+    /* pattern of G and its weights: dense scratch rows w1/w2 indexed by column */
+    double *w1 = (double *)calloc((size_t)n + 1, sizeof(double)), *w2 = (double *)calloc((size_t)n + 1, sizeof(double));
+    char *seen = (char *)calloc((size_t)n + 1, 1);
+    int *cols = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    int cap = 16 * n + 16, used = 0;
+    g->gcol = (int *)malloc(sizeof(int) * (size_t)cap);
+    g->ge1 = (double *)malloc(sizeof(double) * (size_t)cap);
+    g->ge2 = (double *)malloc(sizeof(double) * (size_t)cap);
+    if (!w1 || !w2 || !seen || !cols || !g->gcol || !g->ge1 || !g->ge2) {
+        free(w1); free(w2); free(seen); free(cols);
+        return OR_E_NOMEM;
+    }
+    for (int i = 0; i < n; ++i) {
+        int nc = 0;
+        seen[i] = 1; cols[nc++] = i;
+        for (int k = g->ptr[i]; k < g->ptr[i + 1]; ++k) {
+            const int j = g->nbr[k];
+            if (j < i) { w1[j] += g->ew[k]; if (!seen[j]) { seen[j] = 1; cols[nc++] = j; } }
+            for (int q = g->ptr[j]; q < g->ptr[j + 1]; ++q) {
+                const int l = g->nbr[q];
+                if (l >= i) continue;
+                w2[l] += g->ew[k] * g->ew[q];
+                if (!seen[l]) { seen[l] = 1; cols[nc++] = l; }
+            }
+        }
+        qsort(cols, (size_t)nc, sizeof(int), cmp_int);
+        if (used + nc > cap) {
+            cap = 2 * (used + nc);
+            g->gcol = (int *)realloc(g->gcol, sizeof(int) * (size_t)cap);
+            g->ge1 = (double *)realloc(g->ge1, sizeof(double) * (size_t)cap);
+            g->ge2 = (double *)realloc(g->ge2, sizeof(double) * (size_t)cap);
+            if (!g->gcol || !g->ge1 || !g->ge2) { free(w1); free(w2); free(seen); free(cols); return OR_E_NOMEM; }
+        }
+        for (int t = 0; t < nc; ++t) {
+            const int j = cols[t];
+            g->gcol[used] = j;
+            g->ge1[used] = w1[j];
+            g->ge2[used] = w2[j];
+            ++used;
+            w1[j] = w2[j] = 0.0;
+            seen[j] = 0;
+        }
+        g->gptr[i + 1] = used;
+    }
+    free(w1); free(w2); free(seen); free(cols);
     return OR_OK;
 }
 
-/* forward pass; keeps h_l (4 x n x W), m_l and pre_l (3 x n x W) when the buffers are given */
-static void mp_forward(const mp_graph *g, const double *P, const double *r, double *z, double *s_out, double *H,
-                       double *Mb, double *Pre) {
+/* G values on its pattern */
+typedef struct {
+    double *gv;
+} mp_factor;
+
+#define MP_OFF_T (3 * OR_MP_W + 6 * OR_MP_W * OR_MP_W)
+
+/* forward pass of the network: keeps h_l (4 x n x W), m_l and pre_l (3 x n x W); fills G */
+static void mp_forward(const mp_graph *g, const double *P, double *H, double *Mb, double *Pre, mp_factor *G) {
     const int n = g->n, W = OR_MP_W;
-    double ss = 0.0;
-    for (int i = 0; i < n; ++i) ss += r[i] * r[i];
-    const double s = sqrt(ss / (n ? n : 1));
-    *s_out = s;
-    const double *we = P, *wc = P + W, *wd = P + 2 * W + 6 * W * W;
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    const double *Win = P, *wt = P + MP_OFF_T;
     for (int i = 0; i < n; ++i)
-        for (int f = 0; f < W; ++f) H[(size_t)i * W + f] = r[i] * inv * (we[f] + g->cf[i] * wc[f]);
+        for (int f = 0; f < W; ++f)
+            H[(size_t)i * W + f] = Win[3 * f] * g->x[3 * i] + Win[3 * f + 1] * g->x[3 * i + 1] +
+                                   Win[3 * f + 2] * g->x[3 * i + 2];
     for (int l = 0; l < 3; ++l) {
-        const double *W1 = P + 2 * W + 2 * l * W * W, *W2 = W1 + W * W;
+        const double *W1 = P + 3 * W + 2 * l * W * W, *W2 = W1 + W * W;
         const double *h = H + (size_t)l * n * W;
         double *ho = H + (size_t)(l + 1) * n * W, *m = Mb + (size_t)l * n * W, *pre = Pre + (size_t)l * n * W;
         for (int i = 0; i < n; ++i) {
@@ -1426,113 +1491,195 @@ static void mp_forward(const mp_graph *g, const double *P, const double *r, doub
     }
     const double *h3 = H + (size_t)3 * n * W;
     for (int i = 0; i < n; ++i) {
-        double y = 0.0;
-        for (int f = 0; f < W; ++f) y += wd[f] * h3[(size_t)i * W + f];
-        z[i] = (r[i] + s * y) / g->diag[i];
+        const double *hi = h3 + (size_t)i * W;
+        for (int e = g->gptr[i]; e < g->gptr[i + 1]; ++e) {
+            const int j = g->gcol[e];
+            if (j == i) {
+                double t = 0.0;
+                for (int f = 0; f < W; ++f) t += wt[f] * hi[f];
+                G->gv[e] = exp(t) / sqrt(g->diag[i]);
+                continue;
+            }
+            const double *hj = h3 + (size_t)j * W;
+            double p[2] = {0.0, 0.0};
+            for (int s = 0; s < 2; ++s) {
+                const double *u = P + MP_OFF_T + (1 + 3 * s) * W, *v = u + W, *w = v + W;
+                for (int f = 0; f < W; ++f) p[s] += u[f] * hi[f] * hj[f] + v[f] * hi[f] + w[f] * hj[f];
+            }
+            G->gv[e] = (g->ge1[e] * p[0] + g->ge2[e] * p[1]) / sqrt(g->diag[j]);
+        }
     }
 }
+
+/* y = G r ; z = G' y */
+static void mp_apply_g(const mp_graph *g, const mp_factor *G, const double *r, double *y, double *z) {
+    const int n = g->n;
+    for (int i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int e = g->gptr[i]; e < g->gptr[i + 1]; ++e) acc += G->gv[e] * r[g->gcol[e]];
+        y[i] = acc;
+    }
+    for (int i = 0; i < n; ++i) z[i] = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int e = g->gptr[i]; e < g->gptr[i + 1]; ++e) z[g->gcol[e]] += G->gv[e] * y[i];
+}
+
+/* scale != 0: ga += row i of G A (A_kk = a_kk, A_kj = e1_kj sqrt(a_kk a_jj)); scale == 0: clear
+   the entries that row touches */
+static void mp_row_ga(const mp_graph *g, const mp_factor *G, int i, double *ga, double scale) {
+    for (int e = g->gptr[i]; e < g->gptr[i + 1]; ++e) {
+        const int k = g->gcol[e];
+        const double gik = G->gv[e];
+        if (scale == 0.0) {
+            ga[k] = 0.0;
+            for (int q = g->ptr[k]; q < g->ptr[k + 1]; ++q) ga[g->nbr[q]] = 0.0;
+        } else {
+            ga[k] += gik * g->diag[k];
+            for (int q = g->ptr[k]; q < g->ptr[k + 1]; ++q)
+                ga[g->nbr[q]] += gik * g->ew[q] * sqrt(g->diag[k] * g->diag[g->nbr[q]]);
+        }
+    }
+}
+
+static void *mp_alloc(size_t count) { return malloc(sizeof(double) * (count + 1)); }
 
 int or_mpnn_apply(const or_csr *a, const double *params, const double *r, double *z) {
     if (!a || !params || !r || !z) return OR_E_NULL;
     mp_graph g;
     if (mp_graph_build(a, &g)) { mp_graph_free(&g); return OR_E_NOMEM; }
     const size_t nw = (size_t)a->n * OR_MP_W;
-    double *H = (double *)malloc(sizeof(double) * (4 * nw + 1));
-    double *Mb = (double *)malloc(sizeof(double) * (3 * nw + 1));
-    double *Pre = (double *)malloc(sizeof(double) * (3 * nw + 1));
+    double *H = (double *)mp_alloc(4 * nw), *Mb = (double *)mp_alloc(3 * nw), *Pre = (double *)mp_alloc(3 * nw);
+    double *y = (double *)mp_alloc((size_t)a->n);
+    mp_factor G = {(double *)mp_alloc((size_t)g.gptr[a->n])};
     int rc = OR_E_NOMEM;
-    if (H && Mb && Pre) {
-        double s;
-        mp_forward(&g, params, r, z, &s, H, Mb, Pre);
+    if (H && Mb && Pre && y && G.gv) {
+        mp_forward(&g, params, H, Mb, Pre, &G);
+        mp_apply_g(&g, &G, r, y, z);
         rc = OR_OK;
     }
-    free(H); free(Mb); free(Pre);
+    free(H); free(Mb); free(Pre); free(y); free(G.gv);
     mp_graph_free(&g);
     return rc;
 }
 
 int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const double *R, const double *E,
-                      double *loss, double *grad) {
-    if (!a || !params || !R || !E || !loss || !grad) return OR_E_NULL;
+                      double kappa, double *loss, double *grad) {
+    if (!a || !params || (nsamples > 0 && (!R || !E)) || !loss || !grad) return OR_E_NULL;
     const int n = a->n, W = OR_MP_W;
     mp_graph g;
     if (mp_graph_build(a, &g)) { mp_graph_free(&g); return OR_E_NOMEM; }
     const size_t nw = (size_t)n * W;
-    double *H = (double *)malloc(sizeof(double) * (4 * nw + 1));
-    double *Mb = (double *)malloc(sizeof(double) * (3 * nw + 1));
-    double *Pre = (double *)malloc(sizeof(double) * (3 * nw + 1));
-    double *dH = (double *)malloc(sizeof(double) * (nw + 1));
-    double *dHp = (double *)malloc(sizeof(double) * (nw + 1));
-    double *dM = (double *)malloc(sizeof(double) * (nw + 1));
-    double *z = (double *)malloc(sizeof(double) * ((size_t)n + 1));
-    double *az = (double *)malloc(sizeof(double) * ((size_t)n + 1));
-    if (!H || !Mb || !Pre || !dH || !dHp || !dM || !z || !az) {
-        free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(z); free(az);
+    const int ng = g.gptr[n];
+    double *H = (double *)mp_alloc(4 * nw), *Mb = (double *)mp_alloc(3 * nw), *Pre = (double *)mp_alloc(3 * nw);
+    double *dH = (double *)calloc(nw + 1, sizeof(double)), *dHp = (double *)mp_alloc(nw), *dM = (double *)mp_alloc(nw);
+    double *v = (double *)mp_alloc(6 * (size_t)n), *dG = (double *)calloc((size_t)ng + 1, sizeof(double));
+    mp_factor G = {(double *)mp_alloc((size_t)ng)};
+    if (!H || !Mb || !Pre || !dH || !dHp || !dM || !v || !dG || !G.gv) {
+        free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(v); free(dG); free(G.gv);
         mp_graph_free(&g);
         return OR_E_NOMEM;
     }
     memset(grad, 0, sizeof(double) * OR_MP_P);
-    double total = 0.0;
-    double *gwe = grad, *gwc = grad + W, *gwd = grad + 2 * W + 6 * W * W;
-    const double *wd = params + 2 * W + 6 * W * W, *we = params, *wc = params + W;
+    mp_forward(&g, params, H, Mb, Pre, &G);
+    double *y = v, *z = v + n, *az = v + 2 * (size_t)n, *gg = v + 3 * (size_t)n, *u = v + 4 * (size_t)n,
+           *tmp = v + 5 * (size_t)n;
+    double total = 0.0, kaporin_value = 0.0;
     for (int t = 0; t < nsamples; ++t) {
         const double *r = R + (size_t)t * n, *e = E + (size_t)t * n;
-        double s;
-        mp_forward(&g, params, r, z, &s, H, Mb, Pre);
+        mp_apply_g(&g, &G, r, y, z);
         or_spmv(a, z, az);
         double zaz = 0.0, zr = 0.0, re = 0.0;
         for (int i = 0; i < n; ++i) { zaz += z[i] * az[i]; zr += z[i] * r[i]; re += r[i] * e[i]; }
         total += (zaz - 2.0 * zr) / re + 1.0;
-        /* dL/dz = 2 (Az - r) / (r'e) / nsamples; dy_i = dz_i s / a_ii */
-        const double *h3 = H + (size_t)3 * n * W;
-        for (int i = 0; i < n; ++i) {
-            const double dz = 2.0 * (az[i] - r[i]) / re / nsamples;
-            const double dy = dz * s / g.diag[i];
-            for (int f = 0; f < W; ++f) {
-                gwd[f] += dy * h3[(size_t)i * W + f];
-                dH[(size_t)i * W + f] = dy * wd[f];
-            }
-        }
-        for (int l = 2; l >= 0; --l) {
-            const double *W1 = params + 2 * W + 2 * l * W * W, *W2 = W1 + W * W;
-            double *gW1 = grad + 2 * W + 2 * l * W * W, *gW2 = gW1 + W * W;
-            const double *h = H + (size_t)l * n * W, *m = Mb + (size_t)l * n * W, *pre = Pre + (size_t)l * n * W;
-            for (int i = 0; i < n; ++i) {
-                double dp[OR_MP_W];
-                for (int f = 0; f < W; ++f) {
-                    const double p = pre[(size_t)i * W + f];
-                    dp[f] = (l < 2 && !(p > 0.0)) ? 0.0 : dH[(size_t)i * W + f];
-                }
-                for (int f = 0; f < W; ++f)
-                    for (int q = 0; q < W; ++q) {
-                        gW1[f * W + q] += dp[f] * h[(size_t)i * W + q];
-                        gW2[f * W + q] += dp[f] * m[(size_t)i * W + q];
-                    }
-                for (int q = 0; q < W; ++q) {
-                    double a1 = 0.0, a2 = 0.0;
-                    for (int f = 0; f < W; ++f) { a1 += W1[f * W + q] * dp[f]; a2 += W2[f * W + q] * dp[f]; }
-                    dHp[(size_t)i * W + q] = a1;
-                    dM[(size_t)i * W + q] = a2;
-                }
-            }
-            /* transpose of the aggregation (ew symmetric): dh_j += sum_i ew_ij dm_i */
-            for (int i = 0; i < n; ++i)
-                for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
-                    for (int q = 0; q < W; ++q) dHp[(size_t)i * W + q] += g.ew[k] * dM[(size_t)g.nbr[k] * W + q];
-            memcpy(dH, dHp, sizeof(double) * nw);
-        }
-        const double inv = s > 0.0 ? 1.0 / s : 0.0;
-        for (int i = 0; i < n; ++i) {
-            const double u = r[i] * inv;
-            for (int f = 0; f < W; ++f) {
-                gwe[f] += u * dH[(size_t)i * W + f];
-                gwc[f] += u * g.cf[i] * dH[(size_t)i * W + f];
-            }
-        }
-        (void)we; (void)wc;
+        for (int i = 0; i < n; ++i) gg[i] = 2.0 * (az[i] - r[i]) / re / nsamples;
+        mp_apply_g(&g, &G, gg, u, tmp); /* u = G g */
+        /* dL/dG_ij = y_i g_j + u_i r_j on the pattern of G */
+        for (int i = 0; i < n; ++i)
+            for (int q = g.gptr[i]; q < g.gptr[i + 1]; ++q) dG[q] += y[i] * gg[g.gcol[q]] + u[i] * r[g.gcol[q]];
     }
-    *loss = nsamples ? total / nsamples : 0.0;
-    free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(z); free(az);
+    /* Kaporin term: T = tr(G A G') = sum_i g_i' A g_i (g_i = row i of G);
+       dL_K/dG_ij = (2/T) (G A)_ij - (2/n) [j == i] / G_ii */
+    if (kappa != 0.0 && n > 0) {
+        double *ga = tmp; /* dense row of G A; zero between rows */
+        for (int i = 0; i < n; ++i) ga[i] = 0.0;
+        double T = 0.0, logdiag = 0.0;
+        for (int sweep = 0; sweep < 2; ++sweep)
+            for (int i = 0; i < n; ++i) {
+                mp_row_ga(&g, &G, i, ga, 1.0);
+                for (int q = g.gptr[i]; q < g.gptr[i + 1]; ++q) {
+                    const int j = g.gcol[q];
+                    if (sweep == 0) {
+                        T += G.gv[q] * ga[j];
+                        if (j == i) logdiag += log(G.gv[q]);
+                    } else {
+                        dG[q] += kappa * (2.0 / T * ga[j] - (j == i ? 2.0 / n / G.gv[q] : 0.0));
+                    }
+                }
+                mp_row_ga(&g, &G, i, ga, 0.0);
+            }
+        kaporin_value = log(T / n) - 2.0 / n * logdiag;
+    }
+    /* readouts */
+    const double *h3 = H + (size_t)3 * n * W;
+    const double *wt = params + MP_OFF_T;
+    double *gwt = grad + MP_OFF_T;
+    for (int i = 0; i < n; ++i) {
+        const double *hi = h3 + (size_t)i * W;
+        for (int q = g.gptr[i]; q < g.gptr[i + 1]; ++q) {
+            const int j = g.gcol[q];
+            if (j == i) {
+                const double dt = dG[q] * G.gv[q];
+                for (int f = 0; f < W; ++f) { gwt[f] += dt * hi[f]; dH[(size_t)i * W + f] += dt * wt[f]; }
+                continue;
+            }
+            const double *hj = h3 + (size_t)j * W;
+            for (int s = 0; s < 2; ++s) {
+                const double dp = dG[q] * (s == 0 ? g.ge1[q] : g.ge2[q]) / sqrt(g.diag[j]);
+                const double *uu = params + MP_OFF_T + (1 + 3 * s) * W, *vv = uu + W, *ww = vv + W;
+                double *gu = grad + MP_OFF_T + (1 + 3 * s) * W, *gv = gu + W, *gw = gv + W;
+                for (int f = 0; f < W; ++f) {
+                    gu[f] += dp * hi[f] * hj[f];
+                    gv[f] += dp * hi[f];
+                    gw[f] += dp * hj[f];
+                    dH[(size_t)i * W + f] += dp * (uu[f] * hj[f] + vv[f]);
+                    dH[(size_t)j * W + f] += dp * (uu[f] * hi[f] + ww[f]);
+                }
+            }
+        }
+    }
+    for (int l = 2; l >= 0; --l) {
+        const double *W1 = params + 3 * W + 2 * l * W * W, *W2 = W1 + W * W;
+        double *gW1 = grad + 3 * W + 2 * l * W * W, *gW2 = gW1 + W * W;
+        const double *h = H + (size_t)l * n * W, *mm = Mb + (size_t)l * n * W, *pre = Pre + (size_t)l * n * W;
+        for (int i = 0; i < n; ++i) {
+            double dp[OR_MP_W];
+            for (int f = 0; f < W; ++f) {
+                const double p = pre[(size_t)i * W + f];
+                dp[f] = (l < 2 && !(p > 0.0)) ? 0.0 : dH[(size_t)i * W + f];
+            }
+            for (int f = 0; f < W; ++f)
+                for (int q = 0; q < W; ++q) {
+                    gW1[f * W + q] += dp[f] * h[(size_t)i * W + q];
+                    gW2[f * W + q] += dp[f] * mm[(size_t)i * W + q];
+                }
+            for (int q = 0; q < W; ++q) {
+                double a1 = 0.0, a2 = 0.0;
+                for (int f = 0; f < W; ++f) { a1 += W1[f * W + q] * dp[f]; a2 += W2[f * W + q] * dp[f]; }
+                dHp[(size_t)i * W + q] = a1;
+                dM[(size_t)i * W + q] = a2;
+            }
+        }
+        /* transpose of the aggregation (e1 symmetric): dh_i += sum_j e1_ij dm_j */
+        for (int i = 0; i < n; ++i)
+            for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
+                for (int q = 0; q < W; ++q) dHp[(size_t)i * W + q] += g.ew[k] * dM[(size_t)g.nbr[k] * W + q];
+        memcpy(dH, dHp, sizeof(double) * nw);
+    }
+    for (int i = 0; i < n; ++i)
+        for (int f = 0; f < W; ++f)
+            for (int c = 0; c < 3; ++c) grad[3 * f + c] += dH[(size_t)i * W + f] * g.x[3 * i + c];
+    *loss = (nsamples ? total / nsamples : 0.0) + kappa * kaporin_value;
+    free(H); free(Mb); free(Pre); free(dH); free(dHp); free(dM); free(v); free(dG); free(G.gv);
     mp_graph_free(&g);
     return OR_OK;
 }

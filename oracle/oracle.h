@@ -135,24 +135,31 @@ int or_pcg(const or_csr *a, const double *b, const double *x0, double tol,
 double or_norm2(int n, const double *v);
 
 /* ---- learned preconditioner network (checker for the HIP training/inference kernels) ----
- * The reference has no learned component: these functions restate the network that this
- * project defines (three message-passing layers of width 32 on the matrix graph, a
- * correction on top of Jacobi), in binary64, so that the GPU forward pass and gradients
- * can be checked.  Parameter layout, OR_MP_P values:
- *   w_e[32] | w_c[32] | for l = 0..2: W1_l[32][32], W2_l[32][32] (row f, column g) | w_d[32]
- * Node inputs: u_i = r_i / s with s = sqrt(sum r^2 / n), c_i = log(a_ii / mean a);
- * edge weights ew_ij = a_ij / sqrt(a_ii a_jj) (j != i).
- *   h0_i = u_i (w_e + c_i w_c)
- *   m_i = sum_j ew_ij h_j ;  h'_i = act(W1_l h_i + W2_l m_i), act = ReLU for l < 2, identity for l = 2
- *   y_i = w_d . h3_i ;      z_i = (r_i + s y_i) / a_ii
- * Loss of one (r, e = A^{-1} r) pair: the squared A-norm error of z relative to e,
- *   L = (z - e)' A (z - e) / (e' A e) = (z'Az - 2 z'r) / (r'e) + 1. */
+ * The reference has no learned component: these functions restate the preconditioner that this
+ * project defines, in binary64, so that the GPU forward pass and gradients can be checked.
+ * A message-passing network (three layers of width 32) on the matrix graph predicts a sparse
+ * lower-triangular factor G on the pattern of tril(A + A^2); the preconditioner is z = G'G r,
+ * a correction on top of Jacobi (G = D^{-1/2} when the readouts are zero).
+ *   edge weights   e1_ij = a_ij / sqrt(a_ii a_jj) (j != i);  e2_ij = sum_k e1_ik e1_kj (j != i)
+ *   node features  x_i = [1, c_i, s_i], c_i = log(a_ii / mean a), s_i = sum_j |e1_ij|
+ *   h0_i = W_in x_i ;  layer l: m_i = sum_j e1_ij h_j, h'_i = act(W1_l h_i + W2_l m_i)
+ *                      (act = ReLU for l < 2, identity for l = 2)
+ *   t_i  = w_t . h3_i                                  G_ii = a_ii^{-1/2} exp(t_i)
+ *   p_ij = u . (h3_i * h3_j) + v . h3_i + w . h3_j     with (u, v, w) = (u1, v1, w1) or (u2, v2, w2)
+ *   G_ij = a_jj^{-1/2} (e1_ij p1_ij + e2_ij p2_ij)     (j < i, j within two hops of i)
+ * Parameter layout, OR_MP_P values: W_in[32][3] | for l = 0..2: W1_l[32][32], W2_l[32][32]
+ * (row f, column g) | w_t | u1 | v1 | w1 | u2 | v2 | w2 (32 each).
+ * Loss: the mean over (r, e = A^{-1} r) pairs of the squared A-norm error of z relative to e,
+ *   L_pair = (z - e)' A (z - e) / (e' A e) = (z'Az - 2 z'r) / (r'e) + 1,
+ * plus kappa times the log of Kaporin's condition number of G A G' (up to the constant log det A),
+ *   L_K = log(tr(G A G') / n) - (2/n) sum_i log G_ii,
+ * which keeps the small eigenvalues of G A G' away from zero (the pair term alone does not). */
 #define OR_MP_W 32
-#define OR_MP_P (2 * OR_MP_W + 3 * 2 * OR_MP_W * OR_MP_W + OR_MP_W)
+#define OR_MP_P (3 * OR_MP_W + 3 * 2 * OR_MP_W * OR_MP_W + 7 * OR_MP_W)
 int or_mpnn_apply(const or_csr *a, const double *params, const double *r, double *z);
 /* mean loss over nsamples pairs (columns of R and E, length n each) and its gradient */
 int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const double *R, const double *E,
-                      double *loss, double *grad);
+                      double kappa, double *loss, double *grad);
 
 #ifdef __cplusplus
 }

@@ -1,8 +1,8 @@
 """GPU checks of the learned preconditioner: the bf16 MFMA operand layout, the FP32 and
-BF16 forward passes and the FP32 gradient against the oracle's binary64 restatement
-(itself pinned by finite differences in test_oracle_mpnn.py), training on pairs from the
-direct solver, the flexible PCG against Jacobi-PCG with the direct solve as the judge of the
-error, and the Jacobi fallback."""
+BF16 network factors and the FP32 gradient (pair term and Kaporin term) against the oracle's
+binary64 restatement (itself pinned by finite differences in test_oracle_mpnn.py), training on
+pairs from the direct solver, the flexible PCG against Jacobi-PCG with the direct solve as the
+judge of the error, and the Jacobi fallback."""
 import numpy as np
 import pytest
 
@@ -21,8 +21,10 @@ def system(m, seed=11, length=0.3, variance=1.0):
     return a, O.Csr.from_coo(n, r, c, v), M.DeviceMatrix(a)
 
 
-def random_params(seed, scale=0.4):
-    return (np.random.default_rng(seed).standard_normal(O.MP_P) * scale).astype(np.float32)
+def random_params(seed, scale=0.2, readout=0.05):
+    p = np.random.default_rng(seed).standard_normal(O.MP_P) * scale
+    p[3 * O.MP_W + 6 * O.MP_W * O.MP_W:] *= readout / scale
+    return p.astype(np.float32)
 
 
 def dev(x):
@@ -59,7 +61,8 @@ def test_forward_matches_oracle(precision, tol):
     assert err <= tol, err
 
 
-def test_loss_and_gradient_match_oracle():
+@pytest.mark.parametrize("kappa", [0.0, 0.5])
+def test_loss_and_gradient_match_oracle(kappa):
     a, oc, dm = system(5)
     A = oc.dense()
     net = M.LearnedPreconditioner(seed=4)
@@ -68,16 +71,17 @@ def test_loss_and_gradient_match_oracle():
     g = np.random.default_rng(6)
     R = g.uniform(-1, 1, (3, a.size))
     E = np.linalg.solve(A, R.T).T
-    loss_ref, grad_ref = O.mpnn_loss_grad(oc, p.astype(np.float64), R, E)
+    loss_ref, grad_ref = O.mpnn_loss_grad(oc, p.astype(np.float64), R, E, kappa=kappa)
     Rd, Ed = dev(R), dev(E)
-    loss, grad = net.loss_grad_dev(dm, Rd.data_ptr(), Ed.data_ptr(), 3, a.size)
+    loss, grad = net.loss_grad_dev(dm, Rd.data_ptr(), Ed.data_ptr(), 3, a.size, kappa=kappa)
     assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref), (loss, loss_ref)
     err = np.linalg.norm(grad - grad_ref) / np.linalg.norm(grad_ref)
     assert err <= 1e-4, err
     # every parameter block is exercised
     W = O.MP_W
-    for lo, hi in [(0, W), (W, 2 * W), (2 * W, 2 * W + W * W), (2 * W + 5 * W * W, 2 * W + 6 * W * W),
-                   (O.MP_P - W, O.MP_P)]:
+    off_t = 3 * W + 6 * W * W
+    for lo, hi in [(0, 3 * W), (3 * W, 3 * W + W * W), (3 * W + 5 * W * W, off_t), (off_t, off_t + W),
+                   (off_t + W, off_t + 4 * W), (off_t + 4 * W, O.MP_P)]:
         assert np.abs(grad_ref[lo:hi]).max() > 0
         np.testing.assert_allclose(grad[lo:hi], grad_ref[lo:hi], rtol=2e-3, atol=2e-4 * np.abs(grad_ref).max())
 
@@ -89,7 +93,7 @@ def trained():
     a, oc, dm = system(16, seed=21, length=0.1, variance=2.0)
     R, E = learned.make_pairs(a, dm, 192, seed=22, max_iterations=128)
     net = M.LearnedPreconditioner(seed=23)
-    losses = learned.train(net, dm, R, E, epochs=30, batch=16, learning_rate=3e-3, seed=24)
+    losses = learned.train(net, dm, R, E, epochs=30, batch=16, learning_rate=3e-3, kappa=1.0, seed=24)
     return a, dm, net, losses, R, E
 
 
@@ -105,7 +109,10 @@ def test_pairs_are_exact_errors(trained):
 def test_training_reduces_loss(trained):
     a, dm, net, losses, R, E = trained
     assert np.all(np.isfinite(losses))
-    assert losses[-1] < 0.7 * losses[0], losses
+    # the loss includes the Kaporin term, whose floor (the best factor on this pattern) is well
+    # above zero: a few percent is a real gain over the Jacobi-like start
+    assert losses[-1] < 0.95 * losses[0], losses
+    assert losses[-1] <= min(losses) + 0.02 * losses[0], losses
 
 
 def test_learned_pcg_needs_fewer_iterations(trained):
@@ -129,8 +136,7 @@ def test_learned_pcg_needs_fewer_iterations(trained):
 def test_fallback_to_jacobi_on_a_bad_network():
     a, oc, dm = system(10, seed=31, length=0.1, variance=2.0)
     net = M.LearnedPreconditioner(seed=32)
-    p = random_params(33, scale=3.0)
-    p[-O.MP_W:] *= -50.0  # a decoder that overwhelms Jacobi
+    p = random_params(33, scale=1.0, readout=3.0)  # a wildly scaled factor: G'G A is badly conditioned
     net.params = p
     b = G.uniform_rhs(a.size, 1, 34)
     lrn = M.LearnedConjugateGradientSolver(net, relative_tolerance=1e-8, stall_window=20)
