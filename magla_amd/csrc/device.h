@@ -128,7 +128,7 @@ struct Mpnn {
     int64_t ws_ng = 0;
     DevArray<float> H, Mb, Pre, dH, dHp, dM, part, gvf, dG, gaE;
     DevArray<uint16_t> hb0, hb1;
-    DevArray<double> gv, gvt;       // factor values for the solves (from the BF16 network), also transposed order
+    DevArray<float> gv, gvt;        // factor values for the solves (from the BF16 network), also transposed order
     const MpGraph *gv_graph = nullptr;  // graph the factor values belong to
     DevArray<double> y, z, az, u, dpart, scal;
 };
@@ -139,11 +139,12 @@ void mpnn_pack_bf16(Mpnn &net, hipStream_t st);
 // z = G'G r with the FP32 network (training precision) or the BF16 MFMA network
 void mpnn_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, int precision, hipStream_t st);
 // factor values from the BF16 network (once per matrix and network), then per call
-// z = G'G r with per-block partial sums of r.z and z.ap (n / 256 blocks each); the kernels do
+// z = G'G r with per-block partial sums of r.z and z.ap (mpnn_partial_blocks(n) each); the kernels do
 // nothing once S->done or S->fallback
 void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st);
 void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const PcgScalars *S, const double *d_ap,
                        double *rz_part, double *zap_part, hipStream_t st);
+int mpnn_partial_blocks(int n);  // partial sums written by mpnn_apply_factor
 // mean loss over ns pairs (columns of R, E with leading dimension ld) plus kappa times the log
 // Kaporin number of G A G'; gradient into net.grad
 double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, double kappa,

@@ -387,42 +387,59 @@ __global__ __launch_bounds__(256) void k_mp_readout(const H *__restrict__ h3, co
     }
 }
 
-// y_i = sum_e G_e v_j over row i (one thread per row)
+// Row products of the factor: kRowLanes lanes per row (row entries strided over the lanes,
+// fixed xor-tree per row), so a wave reads consecutive entries of a few rows at once.
+constexpr int kRowLanes = 8;
+
+// y_i = sum_e G_e v_j over row i
 template <typename G>
 __global__ __launch_bounds__(256) void k_g_apply(const int *__restrict__ gptr, const int *__restrict__ gcol,
                                                  const G *__restrict__ gv, const double *__restrict__ v, int n,
                                                  double *__restrict__ y, const PcgScalars *__restrict__ S) {
     if (gated(S)) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int i = gid / kRowLanes, sub = gid % kRowLanes;
     double acc = 0.0;
-    for (int e = gptr[i]; e < gptr[i + 1]; ++e) acc += (double)gv[e] * v[gcol[e]];
-    y[i] = acc;
+    if (i < n)
+        for (int e = gptr[i] + sub; e < gptr[i + 1]; e += kRowLanes) acc += (double)gv[e] * v[gcol[e]];
+    for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+    if (i < n && sub == 0) y[i] = acc;
 }
 
 // z_j = sum over column j of G of G_ij y_i (transposed pattern; gvt = G values in that order,
-// or gv through gtidx when gvt is null), one thread per row, with optional per-block partial
-// sums of r.z and z.ap
+// or gv through gtidx when gvt is null), with optional per-block partial sums of r.z and z.ap
+// Rows are strided over the grid (any grid size), so the partial sums have one value per
+// block of a fixed grid.  After a fallback (S->fallback) it computes the Jacobi z = D^{-1} r.
 template <typename G>
 __global__ __launch_bounds__(256) void k_gt_apply(const int *__restrict__ gtptr, const int *__restrict__ gtrow,
                                                   const int *__restrict__ gtidx, const G *__restrict__ gv,
                                                   const G *__restrict__ gvt, const double *__restrict__ y, int n,
                                                   double *__restrict__ z, const double *__restrict__ r,
                                                   const double *__restrict__ ap, double *__restrict__ rz_part,
-                                                  double *__restrict__ zap_part, const PcgScalars *__restrict__ S) {
-    if (gated(S)) return;
+                                                  double *__restrict__ zap_part, const double *__restrict__ invd,
+                                                  const PcgScalars *__restrict__ S) {
+    if (S && S->done) return;
+    const bool jacobi = S && S->fallback;
     __shared__ double sm[2][256];
+    const int sub = threadIdx.x % kRowLanes;
     double arz = 0.0, azap = 0.0;
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+    for (int jb = (blockIdx.x * 256 + threadIdx.x) / kRowLanes; jb < n + 0; jb += gridDim.x * (256 / kRowLanes)) {
+        const int j = jb;
         double acc = 0.0;
-        if (gvt)
-            for (int t = gtptr[j]; t < gtptr[j + 1]; ++t) acc += (double)gvt[t] * y[gtrow[t]];
-        else
-            for (int t = gtptr[j]; t < gtptr[j + 1]; ++t) acc += (double)gv[gtidx[t]] * y[gtrow[t]];
-        z[j] = acc;
-        if (rz_part) {
-            arz += r[j] * acc;
-            if (ap) azap += acc * ap[j];
+        if (jacobi) {
+            acc = sub == 0 ? invd[j] * r[j] : 0.0;
+        } else if (gvt) {
+            for (int t = gtptr[j] + sub; t < gtptr[j + 1]; t += kRowLanes) acc += (double)gvt[t] * y[gtrow[t]];
+        } else {
+            for (int t = gtptr[j] + sub; t < gtptr[j + 1]; t += kRowLanes) acc += (double)gv[gtidx[t]] * y[gtrow[t]];
+        }
+        for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
+        if (sub == 0) {
+            z[j] = acc;
+            if (rz_part) {
+                arz += r[j] * acc;
+                if (ap) azap += acc * ap[j];
+            }
         }
     }
     if (!rz_part) return;
@@ -443,8 +460,8 @@ __global__ __launch_bounds__(256) void k_gt_apply(const int *__restrict__ gtptr,
 }
 
 // gvt[t] = gv[gtidx[t]]: factor values in the order of the transposed pattern
-__global__ void k_gather_t(const int *__restrict__ gtidx, const double *__restrict__ gv, int64_t ng,
-                           double *__restrict__ gvt) {
+__global__ void k_gather_t(const int *__restrict__ gtidx, const float *__restrict__ gv, int64_t ng,
+                           float *__restrict__ gvt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ng) gvt[t] = gv[gtidx[t]];
 }
@@ -855,12 +872,13 @@ void begin_batch(Mpnn &net, Matrix &A, hipStream_t st) {
 void sample_grad(Mpnn &net, Matrix &A, const double *d_r, const double *d_e, int ns, hipStream_t st) {
     const int n = A.n;
     const MpGraph &g = *A.mp;
-    const dim3 gn(nblk(n, 256));
-    hipLaunchKernelGGL((k_g_apply<float>), gn, dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, d_r, n, net.y.p,
+    const dim3 gn(nblk(n, 256)), gr8(nblk((int64_t)n * kRowLanes, 256));
+    hipLaunchKernelGGL((k_g_apply<float>), gr8, dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p, d_r, n, net.y.p,
                        (const PcgScalars *)nullptr);
-    hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p, g.gtidx.p,
+    hipLaunchKernelGGL((k_gt_apply<float>), gr8, dim3(256), 0, st, g.gtptr.p, g.gtrow.p, g.gtidx.p,
                        net.gvf.p, (const float *)nullptr, net.y.p, n, net.z.p, (const double *)nullptr,
-                       (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
+                       (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                       (const PcgScalars *)nullptr);
     spmv_device(A, 1, net.z.p, n, net.az.p, n, st);
     hipLaunchKernelGGL(k_mp_loss_part, dim3(kRedBlocks), dim3(256), 0, st, net.z.p, net.az.p, d_r, d_e, n,
                        net.dpart.p);
@@ -911,18 +929,20 @@ void mpnn_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, int precis
     const MpGraph &g = *A.mp;
     if (precision == MG_BF16) {
         mpnn_factor_bf16(net, A, st);
-        hipLaunchKernelGGL((k_g_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gv.p,
-                           d_r, n, net.y.p, (const PcgScalars *)nullptr);
-        hipLaunchKernelGGL((k_gt_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
+        hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gptr.p,
+                           g.gcol.p, net.gv.p, d_r, n, net.y.p, (const PcgScalars *)nullptr);
+        hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
                            g.gtidx.p, net.gv.p, net.gvt.p, net.y.p, n, d_z, (const double *)nullptr,
-                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
+                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                           (const PcgScalars *)nullptr);
     } else {
         forward_f32(net, A, st);
-        hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gvf.p,
-                           d_r, n, net.y.p, (const PcgScalars *)nullptr);
-        hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
+        hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gptr.p,
+                           g.gcol.p, net.gvf.p, d_r, n, net.y.p, (const PcgScalars *)nullptr);
+        hipLaunchKernelGGL((k_gt_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
                            g.gtidx.p, net.gvf.p, (const float *)nullptr, net.y.p, n, d_z, (const double *)nullptr,
-                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const PcgScalars *)nullptr);
+                           (const double *)nullptr, (double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                           (const PcgScalars *)nullptr);
     }
     MG_HIP(hipGetLastError());
 }
@@ -945,7 +965,7 @@ void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st) {
                            A.colidx.p, g.ew.p, n, hin, hout, l < 2 ? 1 : 0, (const PcgScalars *)nullptr);
         std::swap(hin, hout);
     }
-    hipLaunchKernelGGL((k_mp_readout<uint16_t, uint16_t, double>), dim3(kMpBlocks), dim3(256), 0, st, hin,
+    hipLaunchKernelGGL((k_mp_readout<uint16_t, uint16_t, float>), dim3(kMpBlocks), dim3(256), 0, st, hin,
                        wv + 3 * W, g.gptr.p, g.gcol.p, g.ge1.p, g.ge2.p, g.diag.p, n, net.gv.p);
     if (g.ng)
         hipLaunchKernelGGL(k_gather_t, dim3(nblk(g.ng, 256)), dim3(256), 0, st, g.gtidx.p, net.gv.p, g.ng, net.gvt.p);
@@ -957,10 +977,12 @@ void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, con
                        double *rz_part, double *zap_part, hipStream_t st) {
     const MpGraph &g = *A.mp;
     const int n = A.n;
-    hipLaunchKernelGGL((k_g_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gptr.p, g.gcol.p, net.gv.p, d_r,
-                       n, net.y.p, S);
-    hipLaunchKernelGGL((k_gt_apply<double>), dim3(nblk(n, 256)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p, g.gtidx.p,
-                       net.gv.p, net.gvt.p, net.y.p, n, d_z, d_r, d_ap, rz_part, zap_part, S);
+    // the transposed product runs on a fixed grid (one partial sum per block; after a fallback it
+    // computes the Jacobi z); the row product is skipped after a fallback
+    hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gptr.p,
+                       g.gcol.p, net.gv.p, d_r, n, net.y.p, S);
+    hipLaunchKernelGGL((k_gt_apply<float>), dim3(mpnn_partial_blocks(n)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
+                       g.gtidx.p, net.gv.p, net.gvt.p, net.y.p, n, d_z, d_r, d_ap, rz_part, zap_part, A.invdiag.p, S);
 }
 
 double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, double kappa,
@@ -1049,5 +1071,7 @@ void selftest_mfma_bf16(const float *a, const float *b, float *d) {
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpy(d, dd.p, sizeof(float) * 256, hipMemcpyDeviceToHost));
 }
+
+int mpnn_partial_blocks(int n) { return std::max(1, std::min(nblk((int64_t)n * kRowLanes, 256), 2048)); }
 
 }  // namespace mg

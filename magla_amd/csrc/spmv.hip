@@ -491,8 +491,8 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
     mp_graph_prepare(A, st);
     if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    // the preconditioner kernels write one partial sum per block of a full grid (n / 256 blocks)
-    const int npart = nblk(n, kBS);
+    // the preconditioner kernels write one partial sum per block of their full grid
+    const int npart = mpnn_partial_blocks(n);
     DevArray<double> r(n), z(n), p(n), ap(n), part(std::max(npart, kRedBlocks)), part2(std::max(npart, kRedBlocks));
     DevArray<PcgScalars> S(1);
     MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
@@ -510,6 +510,7 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
         mpnn_factor_bf16(net, A, st);
         mpnn_apply_factor(net, A, r.p, z.p, S.p, nullptr, part.p, part2.p, st);
         hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, npart, S.p, 0);
+        // on a fallback at the start, the same call now computes the Jacobi z0
         hipLaunchKernelGGL(k_fcg_jacobi, gp, bv, 0, st, A.invdiag.p, r.p, (const double *)nullptr, z.p, n, part.p,
                            part2.p, S.p);
         hipLaunchKernelGGL(k_fcg_start, dim3(1), bv, 0, st, part.p, npart, S.p, 1);
@@ -523,8 +524,7 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
             hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
             hipLaunchKernelGGL(k_pcg_update, gr, bv, 0, st, d_x, r.p, p.p, ap.p, n, part.p, S.p);
             hipLaunchKernelGGL(k_fcg_norm, dim3(1), bv, 0, st, part.p, it, stall, S.p);
-            mpnn_apply_factor(net, A, r.p, z.p, S.p, ap.p, part.p, part2.p, st);
-            hipLaunchKernelGGL(k_fcg_jacobi, gp, bv, 0, st, A.invdiag.p, r.p, ap.p, z.p, n, part.p, part2.p, S.p);
+            mpnn_apply_factor(net, A, r.p, z.p, S.p, ap.p, part.p, part2.p, st);  // Jacobi after a fallback
             hipLaunchKernelGGL(k_fcg_beta, dim3(1), bv, 0, st, part.p, part2.p, npart, S.p);
             hipLaunchKernelGGL(k_pcg_pupdate, gv, bv, 0, st, z.p, p.p, n, S.p);
             if (it % check_every == 0 || it == maxit) {
