@@ -93,6 +93,20 @@ _EXC = {MG_E_NULL: ArgumentNullError, MG_E_ARG: ArgumentError, MG_E_RANGE: Argum
 _lib = None
 
 
+def _init_torch_runtime_first():
+    """PyTorch-ROCm ships its own HIP runtime (a libamdhip64.so without the SONAME of ROCm's
+    libamdhip64.so.7, so the loader keeps both).  Two runtimes can share one process only if
+    torch's initializes the device first; the other order leaves torch with no devices.  When
+    torch is importable its runtime is therefore started before libmagla.so is loaded."""
+    if os.environ.get("MAGLA_SKIP_TORCH_INIT"):
+        return
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     """Load libmagla.so (built by magla_amd.build); raises if it is missing."""
     global _lib
@@ -100,6 +114,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libmagla.so not built: run `python -m magla_amd.build` ({LIB_PATH})")
+    _init_torch_runtime_first()
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.mg_last_error.restype = C.c_char_p

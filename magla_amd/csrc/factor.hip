@@ -62,7 +62,8 @@ struct DevicePlan {
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_upd_ptr, lvl_diag_ptr;
     std::vector<std::vector<int>> lvl_upd_mid;  // per panel: first 128-tile task (64-tile tasks before it)
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
-    std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each)
+    std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each),
+                                    // per level four classes: f <= 8, <= 16, <= 32, <= 64
     DevArray<int> small_nodes;
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
@@ -250,11 +251,12 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     for (int L = 0; L < nl; ++L) {
         int npan = 0;
         double pe_elems = 0, pe_index = 0;
+        std::vector<int> small_cls[4];
         for (int g : bylevel[L]) {
             level_nodes.push_back(g);
             const int f = P->h_f[g], w = P->h_w[g];
             npan = std::max(npan, (w + kPanel - 1) / kPanel);
-            if (f <= kSmallFront) small.push_back(g);
+            if (f <= kSmallFront) small_cls[f <= 8 ? 0 : (f <= 16 ? 1 : (f <= 32 ? 2 : 3))].push_back(g);
             else
                 for (int c = 0; c < (f + 255) / 256; ++c) init.push_back(make_int2(g, c));
             const int T = (f + kTile - 1) / kTile;
@@ -312,7 +314,10 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_footr_ptr.push_back((int)footr.size());
         P->lvl_node_ptr.push_back((int)level_nodes.size());
         P->lvl_init_ptr.push_back((int)init.size());
-        P->lvl_small_ptr.push_back((int)small.size());
+        for (int c = 0; c < 4; ++c) {
+            small.insert(small.end(), small_cls[c].begin(), small_cls[c].end());
+            P->lvl_small_ptr.push_back((int)small.size());
+        }
         P->lvl_pull_ptr.push_back((int)pull.size());
         P->lvl_asm_ptr.push_back((int64_t)asm_pos.size());
         P->lvl_npanels.push_back(npan);
@@ -971,93 +976,210 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     }
 }
 
-// Small fronts (f <= 64, hence one panel): one wave per front does the whole forward
-// step -- the gather of the front vector (own right-hand-side rows plus the children's
-// footer contributions through the inverse row maps), y = Li v on the own rows and
-// v -= L y on the footer rows -- for all right-hand sides.  Lane l handles row l % FR of
-// right-hand side l / FR (FR = 16, 32 or 64 >= f); broadcasts are wave shuffles.
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes, int count,
-                                                   const int *__restrict__ F, const int *__restrict__ W,
-                                                   const int *__restrict__ firstcol, const int *__restrict__ child_ptr,
-                                                   const int *__restrict__ children,
-                                                   const int64_t *__restrict__ inv_off, const int *__restrict__ inv_map,
-                                                   const int64_t *__restrict__ vbase, const T *__restrict__ arena,
-                                                   const int64_t *__restrict__ off, T *__restrict__ V,
-                                                   T *__restrict__ Y, int n, int nrhs) {
-    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (idx >= count) return;  // whole wave
-    const int lane = threadIdx.x & 63;
-    const int g = nodes[idx], f = F[g], w = W[g], fc = firstcol[g];
-    const int FR = f <= 16 ? 16 : (f <= 32 ? 32 : 64), RG = 64 / FR;
-    const int i = lane % FR, rg = lane / FR, base = rg * FR;
-    const T *M = arena + off[g];
-    T *Vg = V + vbase[g] * nrhs;
-    const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
-    for (int r0 = 0; r0 < nrhs; r0 += RG) {
-        const int r = r0 + rg;
-        const bool act = r < nrhs && i < f;
-        T v = T(0);
-        if (act) {
-            v = i < w ? Y[(int64_t)r * n + fc + i] : T(0);
-            for (int c = c0; c < c1; ++c) {
-                const int x = children[c];
-                const int a = inv_map[inv_off[x] + i];
-                if (a >= 0) v += V[vbase[x] * nrhs + (int64_t)r * F[x] + W[x] + a];
+// Small fronts (f <= 64, hence one panel).  A wave holds 64 / FR fronts of one size class
+// (FR = 8, 16, 32 or 64 >= f): lane l works on row i = l % FR of front l / FR, for every
+// right-hand side.  Each lane keeps the FR front entries its row needs in registers, loaded
+// once per front.  Right-hand sides go in batches of KP: all loads of a batch are issued
+// together, and the vector values the rows multiply are broadcast through LDS (one slot per
+// lane and right-hand side of the batch, broadcast reads, two wave-level syncs per batch).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum_{j < FR} R[j] * b[j], b 16-byte aligned in LDS
+template <typename T, int FR>
+__device__ __forceinline__ T lds_dot(const T (&R)[FR], const T *b) {
+    constexpr int VW = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(VW)));
+    T acc0 = T(0), acc1 = T(0);
+#pragma unroll
+    for (int j = 0; j < FR; j += VW) {
+        const vec q = *reinterpret_cast<const vec *>(b + j);
+#pragma unroll
+        for (int u = 0; u < VW; ++u) {
+            if ((j / VW + u) & 1) acc1 += R[j + u] * q[u];
+            else acc0 += R[j + u] * q[u];
+        }
+    }
+    return acc0 + acc1;
+}
+
+template <typename T, int FR>
+struct SmallCfg {
+    static constexpr int GW = 64 / FR;                             // fronts per wave
+    static constexpr int KP = (sizeof(T) == 8 && FR == 64) ? 4 : 8;  // right-hand sides per batch
+};
+
+struct SmallFwdArgs {
+    const int *F, *W, *firstcol, *child_ptr, *children;
+    const int64_t *inv_off;
+    const int *inv_map;
+    const int64_t *vbase, *off;
+    int n, nrhs;
+};
+
+// Forward: v = own right-hand-side rows plus the children's footer contributions (through
+// the inverse row maps); y = Li v on the own rows (Li[i][m], m < i, stored at row m,
+// column i; unit diagonal); v -= L y on the footer rows.
+template <typename T, int FR>
+__global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes, int count, SmallFwdArgs a,
+                                                   const T *__restrict__ arena, T *__restrict__ V,
+                                                   T *__restrict__ Y) {
+    constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
+    __shared__ __align__(16) T sb[4][2][KP][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave_first = (blockIdx.x * 4 + wv) * GW;
+    if (wave_first >= count) return;  // whole wave
+    const int i = lane % FR, base = lane - i;
+    const int idx = wave_first + lane / FR;
+    const bool live = idx < count;
+    const int g = nodes[live ? idx : count - 1];
+    const int f = live ? a.F[g] : 0, w = live ? a.W[g] : 0, fc = a.firstcol[g];
+    const bool own = i < w, foot = i >= w && i < f;
+    const T *M = arena + a.off[g];
+    // own lanes walk column i of the front (stride 1), footer lanes row i (stride f);
+    // offsets stay below 64 * 64, so 32-bit per-lane arithmetic
+    const int lim = own ? i : (foot ? w : 0);  // entries m < lim are used
+    const int start = own ? i * f : (foot ? i : 0), stride = own ? 1 : f, last = max(lim - 1, 0);
+    T R[FR];
+#pragma unroll
+    for (int m = 0; m < FR; ++m) {
+        const T val = M[start + min(m, last) * stride];  // clamped: always inside the front
+        R[m] = m < lim ? val : T(0);
+    }
+    // the first kKeep children's footer offsets stay in registers across right-hand sides
+    constexpr int kKeep = 4;
+    const int nrhs = a.nrhs, n = a.n;
+    const int c0 = live ? a.child_ptr[g] : 0, c1 = live ? a.child_ptr[g + 1] : 0;
+    int64_t cb[kKeep];
+    int cf[kKeep];
+#pragma unroll
+    for (int q = 0; q < kKeep; ++q) {
+        cb[q] = -1;
+        cf[q] = 0;
+        if (c0 + q < c1 && i < f) {
+            const int x = a.children[c0 + q];
+            const int am = a.inv_map[a.inv_off[x] + i];
+            cf[q] = a.F[x];
+            if (am >= 0) cb[q] = a.vbase[x] * nrhs + a.W[x] + am;
+        }
+    }
+    T *Vg = V + a.vbase[g] * nrhs;
+    const int yi = own ? fc + i : fc;  // clamped: other lanes read a valid row, then drop it
+    T(*b0)[64] = sb[wv][0];
+    T(*b1)[64] = sb[wv][1];
+    for (int r0 = 0; r0 < nrhs; r0 += KP) {
+        T v[KP], y[KP];
+#pragma unroll
+        for (int p = 0; p < KP; ++p) {
+            const int r = r0 + p;
+            const bool act = r < nrhs && i < f;
+            const int64_t rr = act ? r : 0;
+            const T yv = Y[rr * n + yi];
+            T acc = (act && own) ? yv : T(0);
+#pragma unroll
+            for (int q = 0; q < kKeep; ++q) {
+                const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : 0];
+                acc += (act && cb[q] >= 0) ? cv : T(0);
+            }
+            v[p] = acc;
+        }
+        if (c1 - c0 > kKeep && i < f) {
+#pragma unroll
+            for (int p = 0; p < KP; ++p) {
+                const int r = r0 + p;
+                if (r >= nrhs) break;
+                for (int c = c0 + kKeep; c < c1; ++c) {
+                    const int x = a.children[c];
+                    const int am = a.inv_map[a.inv_off[x] + i];
+                    if (am >= 0) v[p] += V[a.vbase[x] * nrhs + (int64_t)r * a.F[x] + a.W[x] + am];
+                }
             }
         }
-        // y_i = v_i + sum_{m<i} Li[i][m] v_m  (Li[i][m] stored at row m, column i)
-        T y = v;
-        for (int m = 0; m < w; ++m) {
-            const T vm = __shfl(v, base + m);
-            if (m < i && i < w) y += M[(int64_t)i * f + m] * vm;
-        }
-        // footer: v_i -= sum_k L[i][k] y_k
-        for (int k = 0; k < w; ++k) {
-            const T yk = __shfl(y, base + k);
-            if (i >= w && i < f) v -= M[(int64_t)k * f + i] * yk;
-        }
-        if (act) {
-            if (i < w) Y[(int64_t)r * n + fc + i] = y;
-            else Vg[(int64_t)r * f + i] = v;
+#pragma unroll
+        for (int p = 0; p < KP; ++p) b0[p][lane] = own ? v[p] : T(0);
+        wave_sync();
+#pragma unroll
+        for (int p = 0; p < KP; ++p) y[p] = v[p] + lds_dot<T, FR>(R, &b0[p][base]);
+#pragma unroll
+        for (int p = 0; p < KP; ++p) b1[p][lane] = own ? y[p] : T(0);
+        wave_sync();
+#pragma unroll
+        for (int p = 0; p < KP; ++p) {
+            const int r = r0 + p;
+            const T vf = v[p] - lds_dot<T, FR>(R, &b1[p][base]);
+            if (r < nrhs && i < f) {
+                if (own) Y[(int64_t)r * n + fc + i] = y[p];
+                else Vg[(int64_t)r * f + i] = vf;
+            }
         }
     }
 }
 
-// Small fronts, backward: z = y / d - L_foot' x_foot, then x = Li' z on the own rows.
-template <typename T>
+// Backward: z = y / d - L_foot' x_foot, then x = Li' z on the own rows.  Own lane i keeps
+// R[j] = Li[j][i] (row i, column j of the front) for i < j < w and R[j] = L[j][i]
+// (column i, row j) for w <= j < f: the two index ranges are disjoint, so one register row
+// serves both products, once against x_foot (zero below w) and once against z (zero from w).
+template <typename T, int FR>
 __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes, int count,
                                                    const int *__restrict__ F, const int *__restrict__ W,
                                                    const int *__restrict__ firstcol, const int *__restrict__ pat_ptr,
                                                    const int *__restrict__ pat_rows, const T *__restrict__ arena,
                                                    const int64_t *__restrict__ off, const double *__restrict__ D,
                                                    T *__restrict__ Y, int n, int nrhs) {
-    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (idx >= count) return;  // whole wave
-    const int lane = threadIdx.x & 63;
-    const int g = nodes[idx], f = F[g], w = W[g], fc = firstcol[g];
-    const int FR = f <= 16 ? 16 : (f <= 32 ? 32 : 64), RG = 64 / FR;
-    const int i = lane % FR, rg = lane / FR, base = rg * FR;
+    constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
+    __shared__ __align__(16) T sb[4][2][KP][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave_first = (blockIdx.x * 4 + wv) * GW;
+    if (wave_first >= count) return;  // whole wave
+    const int i = lane % FR, base = lane - i;
+    const int idx = wave_first + lane / FR;
+    const bool live = idx < count;
+    const int g = nodes[live ? idx : count - 1];
+    const int f = live ? F[g] : 0, w = live ? W[g] : 0, fc = firstcol[g];
+    const bool own = i < w, foot = i >= w && i < f;
     const T *M = arena + off[g];
-    const int *rows = pat_rows + pat_ptr[g];
-    for (int r0 = 0; r0 < nrhs; r0 += RG) {
-        const int r = r0 + rg;
-        const bool act = r < nrhs;
-        const T xf = (act && i >= w && i < f) ? Y[(int64_t)r * n + rows[i]] : T(0);
-        T s = T(0);
-        for (int t = w; t < f; ++t) {
-            const T xt = __shfl(xf, base + t);
-            if (i < w) s += M[(int64_t)i * f + t] * xt;
+    // j < w: used for i < j (row i, stride f); j >= w: used for j < f (column i, stride 1)
+    const int ii = own ? i : 0, flast = max(f - 1, 0);
+    T R[FR];
+#pragma unroll
+    for (int j = 0; j < FR; ++j) {
+        const int ix = j < w ? ii + j * f : ii * f + min(j, flast);  // clamped: inside the front
+        const T val = M[ix];
+        R[j] = (own && (j < w ? j > i : j < f)) ? val : T(0);
+    }
+    const int row = foot ? pat_rows[pat_ptr[g] + i] : 0;
+    const int yi = own ? fc + i : fc;
+    const T dv = own ? (T)D[fc + i] : T(1);
+    T(*b0)[64] = sb[wv][0];
+    T(*b1)[64] = sb[wv][1];
+    for (int r0 = 0; r0 < nrhs; r0 += KP) {
+        T xf[KP], yo[KP], z[KP];
+#pragma unroll
+        for (int p = 0; p < KP; ++p) {
+            const int r = r0 + p;
+            const bool act = r < nrhs;
+            const int64_t rr = act ? r : 0;
+            const T xv = Y[rr * n + row], yv = Y[rr * n + yi];
+            xf[p] = (act && foot) ? xv : T(0);
+            yo[p] = (act && own) ? yv : T(0);
         }
-        T z = T(0);
-        if (act && i < w) z = Y[(int64_t)r * n + fc + i] / (T)D[fc + i] - s;
-        // x_i = z_i + sum_{m>i} Li[m][i] z_m  (Li[m][i] stored at row i, column m)
-        T x = z;
-        for (int m = 0; m < w; ++m) {
-            const T zm = __shfl(z, base + m);
-            if (m > i && i < w) x += M[(int64_t)m * f + i] * zm;
+#pragma unroll
+        for (int p = 0; p < KP; ++p) b0[p][lane] = xf[p];
+        wave_sync();
+#pragma unroll
+        for (int p = 0; p < KP; ++p) z[p] = own ? yo[p] / dv - lds_dot<T, FR>(R, &b0[p][base]) : T(0);
+#pragma unroll
+        for (int p = 0; p < KP; ++p) b1[p][lane] = z[p];
+        wave_sync();
+#pragma unroll
+        for (int p = 0; p < KP; ++p) {
+            const int r = r0 + p;
+            const T x = z[p] + lds_dot<T, FR>(R, &b1[p][base]);
+            if (r < nrhs && own) Y[(int64_t)r * n + fc + i] = x;
         }
-        if (act && i < w) Y[(int64_t)r * n + fc + i] = x;
     }
 }
 
@@ -1775,14 +1897,20 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y);
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
+    const SmallFwdArgs sfa{P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
+                           P.vbase.p, P.off.p, n, nrhs};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
-        const int m0 = P.lvl_small_ptr[L], m1 = P.lvl_small_ptr[L + 1];
         ProfScope pi_(st, KP_FWD, 0.0, 0.0);
-        if (m1 > m0)
-            hipLaunchKernelGGL(k_fwd_small<T>, dim3((m1 - m0 + 3) / 4), dim3(256), 0, st, P.small_nodes.p + m0, m1 - m0,
-                               P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
-                               P.vbase.p, arena, P.off.p, V, Y, n, nrhs);
+        for (int c = 0; c < 4; ++c) {
+            const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
+            if (m1 <= m0) continue;
+            const int per_block = 4 * (8 >> c);  // 4 waves of 64 / FR fronts
+            auto kern = c == 0 ? k_fwd_small<T, 8>
+                               : (c == 1 ? k_fwd_small<T, 16> : (c == 2 ? k_fwd_small<T, 32> : k_fwd_small<T, 64>));
+            hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
+                               P.small_nodes.p + m0, m1 - m0, sfa, arena, V, Y);
+        }
         if (i1 > i0)
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
                                P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n,
@@ -1814,11 +1942,16 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                 hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, Y, Z, q, n, nrhs);
         }
-        const int m0 = P.lvl_small_ptr[L], m1 = P.lvl_small_ptr[L + 1];
-        if (m1 > m0)
-            hipLaunchKernelGGL(k_bwd_small<T>, dim3((m1 - m0 + 3) / 4), dim3(256), 0, st, P.small_nodes.p + m0, m1 - m0,
-                               P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p, arena, P.off.p, N.diag.p, Y, n,
-                               nrhs);
+        for (int c = 0; c < 4; ++c) {
+            const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
+            if (m1 <= m0) continue;
+            const int per_block = 4 * (8 >> c);
+            auto kern = c == 0 ? k_bwd_small<T, 8>
+                               : (c == 1 ? k_bwd_small<T, 16> : (c == 2 ? k_bwd_small<T, 32> : k_bwd_small<T, 64>));
+            hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
+                               P.small_nodes.p + m0, m1 - m0, P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p,
+                               arena, P.off.p, N.diag.p, Y, n, nrhs);
+        }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.perm.p, n, nrhs, d_x, ldx,
                        accumulate);
