@@ -99,9 +99,14 @@ def bench_batch(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, reps
 
-    for batch in (b32, b64):
+    tiles = {}
+    for name, batch in (("fp32", b32), ("fp64", b64)):
         for _ in range(args.warmup):
             rep = step(batch)
+        if args.autotune:
+            t, tms = batch.autotune_dev(vals.data_ptr(), stream=sp)
+            tiles[name] = t.tolist()
+            log(f"[bench] autotuned Schur tiles {name}: {t.tolist()} ({tms:.2f} ms factor)")
     ms32, reps32 = timed(b32, args.steps)
     log(f"[bench] batch fp32+IR: {ms32:.2f} ms/step {reps32[-1]}")
     ms64, reps64 = timed(b64, args.steps)
@@ -122,7 +127,7 @@ def bench_batch(args):
                                f"U[0,0.1], 1..32 RHS each (padded to {k}); batched factor+solve to relres<=1e-10 "
                                f"per system and column",
                    "n_total": nt, "rhs_total": nrhs_total, "factor_operation_count": info["factor_operation_count"],
-                   "hipgraph": bool(args.graph)},
+                   "hipgraph": bool(args.graph), "schur_tiles": tiles},
         "fp64_ms_per_step": round(ms64, 3), "speedup_fp32ir_vs_fp64": round(ms64 / ms32, 3),
         "fp32_relres": reps32[-1]["relative_residual"], "fp32_refinement_steps": reps32[-1]["refinement_steps"],
         "symbolic_s": round(symbolic_s, 3), "roofline": roof,
@@ -228,6 +233,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the factorization from a captured hipGraph")
+    ap.add_argument("--autotune", type=int, default=1, help="pick Schur tile shapes per front-size bucket first")
     ap.add_argument("--batch-count", type=int, default=1024, help="c4: number of systems")
     ap.add_argument("--grid", type=int, default=64, help="c5: grid edge")
     ap.add_argument("--pairs", type=int, default=2000, help="c5: training pairs")
@@ -289,10 +295,17 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, (time.time() - w0) * 1e3 / steps, reps
 
+    tiles = {}
     for prec in (M.FP32, M.FP64):
         for _ in range(args.warmup):
             rep = step(prec)
         log(f"[bench] warmup prec={prec} done: {rep}")
+        if args.autotune:
+            if prec not in num:
+                step(prec)
+            t, tms = num[prec].autotune_dev(d_vals.data_ptr(), stream=sp)
+            tiles["fp32" if prec == M.FP32 else "fp64"] = t.tolist()
+            log(f"[bench] autotuned Schur tiles prec={prec}: {t.tolist()} ({tms:.2f} ms factor)")
     ms32, wall32, reps32 = timed(M.FP32, args.steps)
     log(f"[bench] fp32+IR: {ms32:.2f} ms/step (wall {wall32:.2f}) {reps32[-1]}")
     ms64, wall64, reps64 = timed(M.FP64, args.steps)
@@ -352,7 +365,8 @@ def main():
         "config": {"workload": desc, "n": int(n), "nnz_upper": int(nnz), "rhs": int(k),
                    "supernodes": info["supernode_count"], "max_front": info["max_front_size"],
                    "factor_operation_count": info["factor_operation_count"], "tol": args.tol,
-                   "precision": "FP32 factor (MFMA Schur) + FP64 residual/refinement"},
+                   "precision": "FP32 factor (MFMA Schur) + FP64 residual/refinement", "schur_tiles": tiles,
+                   "hipgraph": bool(args.graph)},
         "fp64_ms_per_step": round(ms64, 3),
         "speedup_fp32ir_vs_fp64": round(ms64 / ms32, 3),
         "fp32_relres": reps32[-1]["relative_residual"],

@@ -30,6 +30,10 @@ int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out);
  * layer kernel: d = a b for 16x16 row-major a, b (rounded to bf16), d in FP32. */
 int mg_test_mfma_bf16(const float *a, const float *b, float *d);
 
+/* Sets the Schur tile choice (64 or 128 per front-size bucket, MG_SCHUR_BUCKETS entries)
+ * for the handle's precision without timing anything. */
+int mg_test_set_schur_tiles(mg_numeric *num, const int32_t *tiles);
+
 /* Launch timing.  mg_profile_enable(1) clears the tallies and brackets every
  * factorization launch with HIP events on its stream; mg_profile_read(kind, ...)
  * returns 1 and the kernel's name, launch count, summed event time (ms) and

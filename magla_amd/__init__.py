@@ -154,6 +154,9 @@ def lib():
     L.mg_batch_factorize.argtypes = [vp, vp, C.POINTER(_Options)]
     L.mg_batch_solve.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int32, C.POINTER(_SolveReport)]
     L.mg_batch_numeric_info.argtypes = [vp, C.POINTER(_NumInfo)]
+    L.mg_autotune_dev.argtypes = [vp, vp, C.POINTER(_Options), vp, vp, C.POINTER(C.c_double)]
+    L.mg_batch_autotune_dev.argtypes = [vp, vp, C.POINTER(_Options), vp, vp, C.POINTER(C.c_double)]
+    L.mg_test_set_schur_tiles.argtypes = [vp, vp]
     L.mg_batch_release.argtypes = [vp]
     L.mg_batch_release.restype = None
     L.mg_mpnn_param_count.restype = C.c_int32
@@ -204,6 +207,17 @@ def test_factor_block(numeric, supernode, front_size, width):
     out = np.zeros(front_size * width)
     _check(lib().mg_test_numeric_front(numeric._h, supernode, _ptr(out)))
     return out.reshape(front_size, width, order="F")
+
+
+SCHUR_BUCKETS = 6  # front sizes < 128, < 256, < 512, < 1024, < 2048, >= 2048
+
+
+def test_set_schur_tiles(numeric, tiles):
+    """Force the Schur tile edge (64 or 128) per front-size bucket for the handle's precision."""
+    t = np.ascontiguousarray(tiles, dtype=np.int32)
+    if t.shape != (SCHUR_BUCKETS,):
+        raise ArgumentError(MG_E_ARG, f"tiles must have {SCHUR_BUCKETS} entries")
+    _check(lib().mg_test_set_schur_tiles(numeric._h, _ptr(t)))
 
 
 def profile_enable(on=True):
@@ -462,6 +476,16 @@ class UtduNumericFactorization:
                           relative_residual=rep.relative_residual)
         return x
 
+    def autotune_dev(self, values_ptr, options=None, stream=None):
+        """Schur tile autotuner (mg_autotune_dev): returns (tiles per front-size bucket, tuned
+        factorization ms); the handle is left factorized from values_ptr."""
+        tiles = np.zeros(SCHUR_BUCKETS, dtype=np.int32)
+        ms = C.c_double()
+        o = (options or self.symbolic.options)._c()
+        _check(lib().mg_autotune_dev(self._h, C.c_void_p(values_ptr), C.byref(o), C.c_void_p(stream), _ptr(tiles),
+                                     C.byref(ms)))
+        return tiles, ms.value
+
     def solve_dev(self, b_ptr, x_ptr, nrhs, tol=0.0, max_refine=0, stream=None, ldb=None, ldx=None):
         rep = _SolveReport()
         _check(lib().mg_solve_dev(self._h, nrhs, b_ptr, ldb or self.size, x_ptr, ldx or self.size, tol, max_refine,
@@ -713,6 +737,18 @@ class UtduBatch:
         if device_graph is not None:
             o.device_graph = 1 if device_graph else 0
         _check(lib().mg_batch_factorize_dev(self._h, C.c_void_p(values_ptr), C.byref(o), C.c_void_p(stream)))
+
+    def autotune_dev(self, values_ptr, precision=None, stream=None):
+        """Schur tile autotuner over the merged batch (mg_batch_autotune_dev): returns (tiles per
+        front-size bucket, tuned factorization ms); the batch is left factorized."""
+        o = self.options._c()
+        if precision is not None:
+            o.precision = precision
+        tiles = np.zeros(SCHUR_BUCKETS, dtype=np.int32)
+        ms = C.c_double()
+        _check(lib().mg_batch_autotune_dev(self._h, C.c_void_p(values_ptr), C.byref(o), C.c_void_p(stream),
+                                           _ptr(tiles), C.byref(ms)))
+        return tiles, ms.value
 
     def solve(self, rhs, tol=1e-10, max_refine=30):
         """rhs: list of (n_t,) or (n_t, k_t) arrays; returns the solutions in the same shapes."""

@@ -232,6 +232,16 @@ int mg_factorize(mg_symbolic *sym, int32_t n, int32_t nnz, const double *values,
     });
 }
 
+int mg_autotune_dev(mg_numeric *num, const double *d_values, const mg_options *opts, void *stream, int32_t *tiles,
+                    double *ms) {
+    return guarded([&] {
+        require(num != nullptr && opts != nullptr, MG_E_NULL, "numeric or options is null.");
+        mg::Symbolic &S = num->owner->s;
+        require(S.matrix_nnz == 0 || d_values != nullptr, MG_E_NULL, "values are null.");
+        mg::autotune_device(S, d_values, *opts, (hipStream_t)stream, num->num, tiles, ms);
+    });
+}
+
 int mg_numeric_info_get(const mg_numeric *num, mg_numeric_info *info, double *diag) {
     return guarded([&] {
         require(num != nullptr && info != nullptr, MG_E_NULL, "numeric or info is null.");
@@ -379,6 +389,17 @@ int mg_batch_factorize_dev(mg_batch *b, const double *d_values, const mg_options
         mg_options o = b->s.opts;
         if (opts) o = *opts;
         mg::factorize_device(b->s, d_values, o.precision, o, (hipStream_t)stream, b->num);
+    });
+}
+
+int mg_batch_autotune_dev(mg_batch *b, const double *d_values, const mg_options *opts, void *stream, int32_t *tiles,
+                          double *ms) {
+    return guarded([&] {
+        require(b != nullptr, MG_E_NULL, "batch is null.");
+        require(b->s.matrix_nnz == 0 || d_values, MG_E_NULL, "values are null.");
+        mg_options o = b->s.opts;
+        if (opts) o = *opts;
+        mg::autotune_device(b->s, d_values, o, (hipStream_t)stream, b->num, tiles, ms);
     });
 }
 
@@ -577,6 +598,13 @@ int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out)
     return guarded([&] {
         require(num != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
         mg::numeric_front(num->num, supernode, out);
+    });
+}
+
+int mg_test_set_schur_tiles(mg_numeric *num, const int32_t *tiles) {
+    return guarded([&] {
+        require(num != nullptr && tiles != nullptr, MG_E_NULL, "numeric or tiles is null.");
+        mg::set_schur_tiles(num->num, tiles);
     });
 }
 

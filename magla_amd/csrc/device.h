@@ -80,6 +80,11 @@ int plan_level_count(const DevicePlan &P);
 // Factorization: values (upper CSR order, float64) already on the device.
 void factorize_device(Symbolic &S, const double *d_values, int precision, const mg_options &opts,
                       hipStream_t st, Numeric *&num);
+// Schur tile autotuner (per front-size bucket, per precision, stored in the plan): leaves
+// num factorized from d_values; tiles receives 64/128 per bucket, ms the tuned time.
+void autotune_device(Symbolic &S, const double *d_values, const mg_options &opts, hipStream_t st, Numeric *&num,
+                     int *tiles, double *ms);
+void set_schur_tiles(Numeric *num, const int *tiles);
 void numeric_release(Numeric *num);
 void numeric_info(const Numeric *num, mg_numeric_info &info, double *diag_host);
 void solve_device(Numeric *num, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol,

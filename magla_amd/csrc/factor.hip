@@ -32,7 +32,14 @@ constexpr int kTile = 64;   // tile edge of assembly / Schur tiles
 constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag / panel_rows pass
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
-constexpr int kBig128 = 1 << 30;      // fronts this large take 128x128 Schur tiles (off: slower here)
+// Schur tile shape per front-size bucket (f < 128, < 256, < 512, < 1024, < 2048, >= 2048):
+// 64x64 (k_schur) or 128x128 (k_schur128) output tiles, chosen by autotune_device; 64 until tuned.
+constexpr int kSchurBuckets = MG_SCHUR_BUCKETS;
+static inline int schur_bucket(int f) {
+    int b = 0;
+    for (int lim = 128; b < kSchurBuckets - 1 && f >= lim; lim *= 2) ++b;
+    return b;
+}
 constexpr int kSmallFront = 64;       // fronts this small are solved by one wave each (all panels, all RHS)
 constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
@@ -59,8 +66,12 @@ struct DevicePlan {
     std::vector<int> lvl_node_ptr, lvl_pull_ptr;
     std::vector<int64_t> lvl_asm_ptr;
     std::vector<int> lvl_npanels;
-    std::vector<std::vector<int>> lvl_panel_ptr, lvl_upd_ptr, lvl_diag_ptr;
-    std::vector<std::vector<int>> lvl_upd_mid;  // per panel: first 128-tile task (64-tile tasks before it)
+    std::vector<std::vector<int>> lvl_panel_ptr, lvl_diag_ptr;
+    // Schur tasks per level: for panel q and bucket b, tasks [ub[q * (kB + 1) + b], ub[... + b + 1])
+    // of upd_tasks (64x64 tiles) and, with ub128, of upd128_tasks (128x128 tiles)
+    std::vector<std::vector<int>> lvl_ub64, lvl_ub128;
+    int schur128[2][kSchurBuckets] = {};  // [precision is FP32][bucket]: 1 = 128x128 tiles
+    int64_t bucket_tasks[kSchurBuckets] = {};
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
     std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each),
                                     // per level four classes: f <= 8, <= 16, <= 32, <= 64
@@ -87,7 +98,7 @@ struct DevicePlan {
 
     int max_foot_tasks = 0;
     DevArray<int> asm_vidx;
-    DevArray<int4> pull_tasks, upd_tasks;
+    DevArray<int4> pull_tasks, upd_tasks, upd128_tasks;
     DevArray<int2> panel_tasks;
 };
 
@@ -230,7 +241,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     }
 
     std::vector<int> level_nodes, diag_nodes, diag_tb;
-    std::vector<int4> pull, upd;
+    std::vector<int4> pull, upd, upd128;
     std::vector<int2> panel, init;
     std::vector<int64_t> vbase(nf + 1, 0);
     for (int g = 0; g < nf; ++g) vbase[g + 1] = vbase[g] + P->h_f[g];
@@ -323,7 +334,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_npanels.push_back(npan);
         P->lvl_pull_elems.push_back(pe_elems);
         P->lvl_pull_index.push_back(pe_index);
-        std::vector<int> pp{(int)panel.size()}, up{(int)upd.size()}, dp{(int)diag_nodes.size()}, umid;
+        std::vector<int> pp{(int)panel.size()}, dp{(int)diag_nodes.size()}, ub64, ub128;
         std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
         for (int q = 0; q < npan; ++q) {
             const int pbase = (int)panel.size();
@@ -340,29 +351,35 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             P->max_panel_tasks = std::max(P->max_panel_tasks, (int)panel.size() - pbase);
             dp.push_back((int)diag_nodes.size());
             pp.push_back((int)panel.size());
-            // 64x64 tiles for fronts below kBig128, then 128x128 tiles for the large ones
+            // Schur tasks of this panel, grouped by front-size bucket, in both tile shapes
             for (int big = 0; big < 2; ++big) {
-                for (int g : bylevel[L]) {
-                    const int f = P->h_f[g], w = P->h_w[g];
-                    if (w <= q * kPanel || (f >= kBig128) != (big == 1)) continue;
-                    const SchurStep st = schur_step(f, w, q);
-                    if (!st.any) continue;
-                    const int tile = big ? 128 : kTile;
-                    const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
-                    for (int tj = 0; tj < ct; ++tj)
-                        for (int ti = tj; ti < rt; ++ti) upd.push_back(make_int4(g, ti, tj, big));
-                    double elems = 0;
-                    for (int j = st.base; j < st.cend; ++j) elems += f - j;
-                    ufl[q] += 2.0 * (st.ke - st.kb) * elems;
-                    uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+                std::vector<int4> &dst = big ? upd128 : upd;
+                std::vector<int> &ub = big ? ub128 : ub64;
+                for (int b = 0; b < kSchurBuckets; ++b) {
+                    ub.push_back((int)dst.size());
+                    for (int g : bylevel[L]) {
+                        const int f = P->h_f[g], w = P->h_w[g];
+                        if (w <= q * kPanel || schur_bucket(f) != b) continue;
+                        const SchurStep st = schur_step(f, w, q);
+                        if (!st.any) continue;
+                        const int tile = big ? 128 : kTile;
+                        const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
+                        for (int tj = 0; tj < ct; ++tj)
+                            for (int ti = tj; ti < rt; ++ti) dst.push_back(make_int4(g, ti, tj, big));
+                        if (big) continue;
+                        P->bucket_tasks[b] += (int64_t)ct * (2 * rt - ct + 1) / 2;
+                        double elems = 0;
+                        for (int j = st.base; j < st.cend; ++j) elems += f - j;
+                        ufl[q] += 2.0 * (st.ke - st.kb) * elems;
+                        uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+                    }
                 }
-                if (big == 0) umid.push_back((int)upd.size());
+                ub.push_back((int)dst.size());
             }
-            up.push_back((int)upd.size());
         }
         P->lvl_panel_ptr.push_back(pp);
-        P->lvl_upd_ptr.push_back(up);
-        P->lvl_upd_mid.push_back(umid);
+        P->lvl_ub64.push_back(ub64);
+        P->lvl_ub128.push_back(ub128);
         P->lvl_diag_ptr.push_back(dp);
         P->lvl_upd_flops.push_back(ufl);
         P->lvl_upd_elems.push_back(uel);
@@ -407,6 +424,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->asm_vidx.upload(asm_vidx);
     P->pull_tasks.upload(pull);
     P->upd_tasks.upload(upd);
+    P->upd128_tasks.upload(upd128);
     P->panel_tasks.upload(panel);
     return P;
 }
@@ -876,14 +894,30 @@ __global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks
     schur128_store_tile<T>(acc, M, f, i0, j0, cend);
 }
 
-// Deterministic statistics over D and the pivot flags (one workgroup).
-__global__ __launch_bounds__(1024) void k_pivot_stats(const double *__restrict__ D, const unsigned char *__restrict__ flags,
-                                                      int n, int *__restrict__ counts, double *__restrict__ mags) {
-    __shared__ int sc[3][1024];
-    __shared__ double smin[1024], smax[1024];
+// Deterministic statistics over D and the pivot flags: kStatParts workgroups reduce fixed
+// strided slices (counts of regularized / negative / failed pivots, min and max |d|), then
+// one workgroup combines the partials in a fixed order.
+constexpr int kStatParts = 256;
+
+__device__ inline void stat_reduce(double (*v)[256], int t) {
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            for (int q = 0; q < 3; ++q) v[q][t] += v[q][t + h];
+            v[3][t] = fmin(v[3][t], v[3][t + h]);
+            v[4][t] = fmax(v[4][t], v[4][t + h]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pivot_stats_part(const double *__restrict__ D,
+                                                          const unsigned char *__restrict__ flags, int n,
+                                                          double *__restrict__ part) {
+    __shared__ double v[5][256];
+    const int t = threadIdx.x;
     int reg = 0, neg = 0, bad = 0;
     double lo = 1.7976931348623157e308, hi = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < n; i += (int64_t)kStatParts * 256) {
         const unsigned char fl = flags[i];
         reg += fl & 1;
         neg += (fl >> 1) & 1;
@@ -892,26 +926,30 @@ __global__ __launch_bounds__(1024) void k_pivot_stats(const double *__restrict__
         lo = fmin(lo, m);
         hi = fmax(hi, m);
     }
-    sc[0][threadIdx.x] = reg;
-    sc[1][threadIdx.x] = neg;
-    sc[2][threadIdx.x] = bad;
-    smin[threadIdx.x] = lo;
-    smax[threadIdx.x] = hi;
+    v[0][t] = reg;
+    v[1][t] = neg;
+    v[2][t] = bad;
+    v[3][t] = lo;
+    v[4][t] = hi;
     __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            for (int q = 0; q < 3; ++q) sc[q][threadIdx.x] += sc[q][threadIdx.x + h];
-            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        counts[0] = sc[0][0];
-        counts[1] = sc[1][0];
-        counts[2] = sc[2][0];
-        mags[0] = smin[0];
-        mags[1] = smax[0];
+    stat_reduce(v, t);
+    if (t < 5) part[(int64_t)t * kStatParts + blockIdx.x] = v[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_pivot_stats_final(const double *__restrict__ part, int *__restrict__ counts,
+                                                           double *__restrict__ mags) {
+    static_assert(kStatParts == 256, "one partial per thread");
+    __shared__ double v[5][256];
+    const int t = threadIdx.x;
+    for (int q = 0; q < 5; ++q) v[q][t] = part[q * kStatParts + t];
+    __syncthreads();
+    stat_reduce(v, t);
+    if (t == 0) {
+        counts[0] = (int)v[0][0];
+        counts[1] = (int)v[1][0];
+        counts[2] = (int)v[2][0];
+        mags[0] = v[3][0];
+        mags[1] = v[4][0];
     }
 }
 
@@ -1606,7 +1644,7 @@ struct Numeric {
     DevArray<double> diag;
     DevArray<unsigned char> flags;
     DevArray<int> counts;
-    DevArray<double> mags, scale, wsScale;
+    DevArray<double> mags, scale, wsScale, wsStat;
     DevArray<double> values;  // upper CSR values (float64), kept for refinement
     Matrix A;                 // full CSR for residuals
     mg_numeric_info info{};
@@ -1755,15 +1793,26 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                 hipLaunchKernelGGL(k_panel_rows<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.diag.p);
             }
-            const int u0 = P.lvl_upd_ptr[L][q], um = P.lvl_upd_mid[L][q], u1 = P.lvl_upd_ptr[L][q + 1];
-            if (u1 > u0) {
+            const int *b64 = &P.lvl_ub64[L][q * (kSchurBuckets + 1)];
+            const int *b128 = &P.lvl_ub128[L][q * (kSchurBuckets + 1)];
+            if (b64[kSchurBuckets] > b64[0]) {
                 ProfScope ps_(st, KP_SCHUR, P.lvl_upd_flops[L][q], e * P.lvl_upd_elems[L][q]);
-                if (um > u0)
-                    hipLaunchKernelGGL(k_schur<T>, dim3(um - u0), dim3(256), 0, st, P.upd_tasks.p + u0, arena, P.f.p,
-                                       P.w.p, P.off.p, P.first.p, q, N.diag.p);
-                if (u1 > um)
-                    hipLaunchKernelGGL(k_schur128<T>, dim3(u1 - um), dim3(256), 0, st, P.upd_tasks.p + um, arena,
-                                       P.f.p, P.w.p, P.off.p, P.first.p, q, N.diag.p);
+                // one launch per run of consecutive buckets with the same tile shape
+                const int *big = P.schur128[sizeof(T) == 4];
+                for (int b = 0; b < kSchurBuckets;) {
+                    int c = b + 1;
+                    while (c < kSchurBuckets && big[c] == big[b]) ++c;
+                    if (big[b]) {
+                        if (b128[c] > b128[b])
+                            hipLaunchKernelGGL(k_schur128<T>, dim3(b128[c] - b128[b]), dim3(256), 0, st,
+                                               P.upd128_tasks.p + b128[b], arena, P.f.p, P.w.p, P.off.p, P.first.p, q,
+                                               N.diag.p);
+                    } else if (b64[c] > b64[b]) {
+                        hipLaunchKernelGGL(k_schur<T>, dim3(b64[c] - b64[b]), dim3(256), 0, st, P.upd_tasks.p + b64[b],
+                                           arena, P.f.p, P.w.p, P.off.p, P.first.p, q, N.diag.p);
+                    }
+                    b = c;
+                }
             }
         }
     }
@@ -1798,6 +1847,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->mags.alloc(4);
             N->scale.alloc(std::max(S.plan->nsys, 1));
             N->wsScale.alloc(kScaleParts);
+            N->wsStat.alloc(5 * kStatParts);
             N->values.alloc(std::max(S.matrix_nnz, 1));
             N->A.build(S.n, S.a_rowptr.data(), S.a_colidx.data());
         }
@@ -1853,8 +1903,9 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                     MG_HIP(hipGraphLaunch(N->gexec, st));
                 }
             }
-            hipLaunchKernelGGL(k_pivot_stats, dim3(1), dim3(1024), 0, st, N->diag.p, N->flags.p, S.n, N->counts.p,
-                               N->mags.p);
+            hipLaunchKernelGGL(k_pivot_stats_part, dim3(kStatParts), dim3(256), 0, st, N->diag.p, N->flags.p, S.n,
+                               N->wsStat.p);
+            hipLaunchKernelGGL(k_pivot_stats_final, dim3(1), dim3(256), 0, st, N->wsStat.p, N->counts.p, N->mags.p);
             MG_HIP(hipGetLastError());
             int counts[3];
             double mags[2];
@@ -1876,6 +1927,72 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
         throw;
     }
     num = N;
+}
+
+static void drop_graph(Numeric *N) {
+    if (N->gexec) {
+        (void)hipGraphExecDestroy(N->gexec);
+        N->gexec = nullptr;
+    }
+}
+
+template <typename T>
+static double factor_ms(Numeric &N, const mg_options &o, hipStream_t st) {
+    hipEvent_t e0, e1;
+    MG_HIP(hipEventCreate(&e0));
+    MG_HIP(hipEventCreate(&e1));
+    MG_HIP(hipEventRecord(e0, st));
+    run_factor<T>(N, o, st);
+    MG_HIP(hipEventRecord(e1, st));
+    MG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+void autotune_device(Symbolic &S, const double *d_values, const mg_options &opts, hipStream_t st, Numeric *&num,
+                     int *tiles, double *ms) {
+    mg_options o = opts;
+    o.device_graph = 0;
+    if (num) o.precision = num->precision;  // an existing handle keeps its precision
+    // a plain factorization first: it sets the values, the pivot scale and the workspaces
+    factorize_device(S, d_values, o.precision, o, st, num);
+    Numeric &N = *num;
+    DevicePlan &P = *N.plan;
+    const bool fp32 = N.precision == MG_FP32;
+    int *big = P.schur128[fp32];
+    // run_factor rebuilds every front from the values, so repeated runs are independent
+    auto best_of_3 = [&] {
+        double t = 1e300;
+        for (int r = 0; r < 3; ++r) t = std::min(t, fp32 ? factor_ms<float>(N, o, st) : factor_ms<double>(N, o, st));
+        return t;
+    };
+    double cur = best_of_3();
+    for (int b = 0; b < kSchurBuckets; ++b) {
+        if (P.bucket_tasks[b] == 0) continue;
+        const int keep = big[b];
+        big[b] = 1 - keep;
+        const double alt = best_of_3();
+        const double t64 = keep ? alt : cur, t128 = keep ? cur : alt;
+        big[b] = t128 < 0.98 * t64 ? 1 : 0;  // 128x128 only for a clear gain
+        cur = big[b] ? t128 : t64;
+    }
+    drop_graph(&N);
+    o.device_graph = opts.device_graph;
+    factorize_device(S, d_values, N.precision, o, st, num);
+    if (tiles)
+        for (int b = 0; b < kSchurBuckets; ++b) tiles[b] = big[b] ? 128 : 64;
+    if (ms) *ms = cur;
+}
+
+void set_schur_tiles(Numeric *N, const int *tiles) {
+    for (int b = 0; b < kSchurBuckets; ++b) {
+        if (tiles[b] != 64 && tiles[b] != 128) throw Error(MG_E_ARG, "Schur tiles must be 64 or 128.");
+        N->plan->schur128[N->precision == MG_FP32][b] = tiles[b] == 128;
+    }
+    drop_graph(N);
 }
 
 void numeric_release(Numeric *num) { delete num; }
