@@ -27,6 +27,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"hbm_gbs": 8000.0, "fp32_mfma_tflops": 157.3, "fp64_tflops": 78.6}
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs and scaled by the same counters on a copy of known size (--calibrate-copy)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+CALIBRATION_BYTES = 256 << 20
+
+
+def measured_traffic(config, kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)[config][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def calibrate_copy():
+    """One plain device copy of CALIBRATION_BYTES (read and written once each): the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run."""
+    import torch
+    src = torch.ones(CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def log(msg):
@@ -117,7 +141,7 @@ def bench_batch(args):
     prof = M.profile_read()
     M.profile_enable(False)
     name, t = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    roof = roofline_of(name, t)
+    roof = roofline_of(name, t, "c4")
     out = {
         "metric": "factor+solve time to 1e-10 residual, FP32-MFMA+IR vs FP64, 1 MI355X",
         "value": round(ms32, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -204,9 +228,12 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
-def roofline_of(name, t):
+MFMA_PHASES = {"schur"}  # the Schur update runs on the matrix cores; every other phase streams HBM
+
+
+def roofline_of(name, t, config=None):
     avg_ms = t["ms"] / max(t["launches"], 1)
-    if t["flops"] > 0:
+    if name in MFMA_PHASES:
         achieved = t["flops"] / (t["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma_tflops"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK["fp32_mfma_tflops"], "traffic": None}
@@ -214,6 +241,8 @@ def roofline_of(name, t):
         achieved = t["bytes"] / (t["ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                 "frac": achieved / PEAK["hbm_gbs"], "traffic": None}
+    if config is not None:
+        roof["traffic"] = measured_traffic(config, name)
     roof.update(kernel=name, launches=t["launches"], avg_launch_ms=avg_ms, kernel_ms_per_step=t["ms"])
     return roof
 
@@ -234,6 +263,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--graph", type=int, default=0, help="replay the factorization from a captured hipGraph")
     ap.add_argument("--autotune", type=int, default=1, help="pick Schur tile shapes per front-size bucket first")
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="run one plain copy of 256 MiB first (PMC scale)")
     ap.add_argument("--batch-count", type=int, default=1024, help="c4: number of systems")
     ap.add_argument("--grid", type=int, default=64, help="c5: grid edge")
     ap.add_argument("--pairs", type=int, default=2000, help="c5: training pairs")
@@ -246,6 +276,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
     if args.config == "c4":
         return bench_batch(args)
     if args.config == "c5":
@@ -299,7 +331,7 @@ def main():
     for prec in (M.FP32, M.FP64):
         for _ in range(args.warmup):
             rep = step(prec)
-        log(f"[bench] warmup prec={prec} done: {rep}")
+            log(f"[bench] warmup prec={prec} done: {rep}")
         if args.autotune:
             if prec not in num:
                 step(prec)
@@ -326,7 +358,7 @@ def main():
     prof = M.profile_read()
     M.profile_enable(False)
     name, t = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    roof = roofline_of(name, t)
+    roof = roofline_of(name, t, args.config)
     kernels = kernel_table(prof)
     log(f"[bench] kernels: {json.dumps(kernels)}")
 

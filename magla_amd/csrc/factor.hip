@@ -84,6 +84,9 @@ struct DevicePlan {
     int64_t vsum = 0;               // sum f over fronts (solve front vectors)
     // algorithmic work per launch
     std::vector<double> lvl_pull_elems, lvl_pull_index;
+    // solve work per level: factor entries the sweep reads (L block of each front with the
+    // stored inverse triangle, f w - w (w - 1) / 2), sum of f and sum of w over the fronts
+    std::vector<double> lvl_solve_fac, lvl_solve_f, lvl_solve_w;
     std::vector<std::vector<double>> lvl_upd_flops, lvl_upd_elems;
     std::vector<std::vector<double>> lvl_panel_elems;  // sum over fronts of (f - pe) * nb (L below the panel)
 
@@ -333,6 +336,18 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_asm_ptr.push_back((int64_t)asm_pos.size());
         P->lvl_npanels.push_back(npan);
         P->lvl_pull_elems.push_back(pe_elems);
+        {
+            double fac = 0, sf = 0, sw = 0;
+            for (int g : bylevel[L]) {
+                const double f = P->h_f[g], w = P->h_w[g];
+                fac += f * w - w * (w - 1) / 2;
+                sf += f;
+                sw += w;
+            }
+            P->lvl_solve_fac.push_back(fac);
+            P->lvl_solve_f.push_back(sf);
+            P->lvl_solve_w.push_back(sw);
+        }
         P->lvl_pull_index.push_back(pe_index);
         std::vector<int> pp{(int)panel.size()}, dp{(int)diag_nodes.size()}, ub64, ub128;
         std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
@@ -773,6 +788,16 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
         }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Block b
+// takes task xcd_task(b, n) instead of b, so that every XCD works through one contiguous run
+// of the task list: neighbouring tasks (tiles of one front sharing panel rows) then meet in
+// the same L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_task(int b, int n) {
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, r = b / kXcd, q = n / kXcd, rem = n % kXcd;
+    return x * q + min(x, rem) + r;
+}
+
 // Schur update after panel q (cf. ApplyPanelUpdate, UtduNumericFactorization.cs:624-852):
 // F[i,j] -= sum_k (L[i,k] d_k) L[j,k] over the step's K columns, on the lower part
 // of the step's target region (schur_step).  One 64x64 output tile per workgroup;
@@ -785,7 +810,7 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
     __shared__ T As[kPanel][kTile + 1];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ T Bs[kPanel][kTile + 1];  // Bs[k][r] = L[j0+r, kc+k]
     __shared__ T Ds[kBlock];             // d_k of the step's K columns
-    const int4 tk = tasks[blockIdx.x];
+    const int4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
     const int s = tk.x;
     const int f = F[s], w = W[s];
     const int ps = q * kPanel;
@@ -2018,7 +2043,10 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                            P.vbase.p, P.off.p, n, nrhs};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
-        ProfScope pi_(st, KP_FWD, 0.0, 0.0);
+        const double fl = 2.0 * nrhs * (P.lvl_solve_fac[L] - P.lvl_solve_w[L]);
+        const double by = sizeof(T) * P.lvl_solve_fac[L] + 4.0 * P.lvl_solve_f[L] +
+                          sizeof(T) * nrhs * (1.5 * P.lvl_solve_f[L] + 0.5 * P.lvl_solve_w[L]);
+        ProfScope pi_(st, KP_FWD, fl, by);
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
             if (m1 <= m0) continue;
@@ -2040,7 +2068,10 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
-        ProfScope pb_(st, KP_BWD, 0.0, 0.0);
+        const double fl = 2.0 * nrhs * (P.lvl_solve_fac[L] - P.lvl_solve_w[L]) + nrhs * P.lvl_solve_w[L];
+        const double by = sizeof(T) * P.lvl_solve_fac[L] + 4.0 * P.lvl_solve_f[L] +
+                          sizeof(T) * nrhs * (1.5 * P.lvl_solve_f[L] + 0.5 * P.lvl_solve_w[L]);
+        ProfScope pb_(st, KP_BWD, fl, by);
         const int f0 = P.lvl_foot_ptr[L], f1 = P.lvl_foot_ptr[L + 1];
         if (f1 > f0) {
             const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
