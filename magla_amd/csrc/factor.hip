@@ -93,7 +93,8 @@ struct DevicePlan {
     int nsys = 1;                   // independent systems merged into this plan (batched solves)
     std::vector<int> h_sys_begin;   // first row of each system, nsys+1
     DevArray<int> front_sys, sys_begin;
-    DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, diag_nodes, diag_tb;
+    DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm, diag_nodes,
+        diag_tb;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks, sfwd_tasks, sbwd_tasks;
     DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
@@ -432,6 +433,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->vbase.upload(vbase);
     P->inv_map.upload(inv_map);
     P->perm.upload(S.perm);
+    P->iperm.upload(S.iperm);
     P->off.upload(P->h_off);
     P->inv_off.upload(inv_off);
     P->u_off.upload(uoff);
@@ -990,16 +992,18 @@ __global__ void k_permute_in(const double *__restrict__ B, int ldb, const int *_
     Y[e] = (T)B[(int64_t)r * ldb + perm[i]];
 }
 
-// X[perm[i]] (+)= Xe[i]
+// X[j] (+)= Xe[iperm[j]]: the FP64 solution is read and written in order, the gather falls
+// on the (narrower) elimination-order vector
 template <typename T>
-__global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ perm, int n, int nrhs,
+__global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ iperm, int n, int nrhs,
                               double *__restrict__ X, int ldx, int accumulate) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * nrhs) return;
-    const int r = (int)(e / n), i = (int)(e % n);
-    double *dst = X + (int64_t)r * ldx + perm[i];
-    if (accumulate) *dst += (double)Xe[e];
-    else *dst = (double)Xe[e];
+    const int r = (int)(e / n), j = (int)(e % n);
+    const double v = (double)Xe[(int64_t)r * n + iperm[j]];
+    double *dst = X + (int64_t)r * ldx + j;
+    if (accumulate) *dst += v;
+    else *dst = v;
 }
 
 // Supernodal solves, level by level over the fronts
@@ -1472,9 +1476,9 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
 // sums of R^2 and B^2 for the relative residual: part[(col * gridDim.x + block) * 2 + {0,1}].
 // One row per LANES-lane group (grid = n * LANES / 256 blocks), so the partial sums and
 // their order depend only on n and LANES.
-constexpr int kResCols = 8;
+constexpr int kResColsMax = 16;
 static inline int residual_blocks(int n, int lanes) { return (int)(((int64_t)n * lanes + 255) / 256); }
-template <int LANES>
+template <int LANES, int kResCols, bool kNorms>
 __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const double *__restrict__ v, int n, int nrhs,
                                                   const double *__restrict__ B, int ldb, const double *__restrict__ X,
@@ -1509,11 +1513,14 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
                         const double b = B[(int64_t)(c0 + c) * ldb + row];
                         const double res = b - acc[c];
                         R[(int64_t)(c0 + c) * n + row] = res;
-                        rr[c] += res * res;
-                        bb[c] += b * b;
+                        if (kNorms) {
+                            rr[c] += res * res;
+                            bb[c] += b * b;
+                        }
                     }
             }
         }
+        if (!kNorms) continue;  // per-system norms come from R afterwards
         // block reduction: wave xor-tree, then the 4 waves in fixed order
 #pragma unroll
         for (int c = 0; c < kResCols; ++c)
@@ -2101,8 +2108,8 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                                arena, P.off.p, N.diag.p, Y, n, nrhs);
         }
     }
-    hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.perm.p, n, nrhs, d_x, ldx,
-                       accumulate);
+    hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
+                       ldx, accumulate);
     MG_HIP(hipGetLastError());
 }
 
@@ -2127,9 +2134,16 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
     const int lanes = N.A.lanes;
     const int nb = residual_blocks(n, lanes);
     const Matrix &A = N.A;
-#define MG_RES(LN)                                                                                               \
-    hipLaunchKernelGGL(k_residual<LN>, dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, nrhs, d_b, \
-                       ldb, d_x, ldx, N.wsR.p, N.wsPart.p)
+    // one system: 8 right-hand sides per pass over A with the norm partials; a batch takes its
+    // per-system norms from R afterwards, and 16 columns per pass (half the passes over A)
+    const bool seg = N.plan->nsys > 1;
+#define MG_RES(LN)                                                                                                  \
+    if (seg)                                                                                                        \
+        hipLaunchKernelGGL((k_residual<LN, kResColsMax, false>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, \
+                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p);                             \
+    else                                                                                                            \
+        hipLaunchKernelGGL((k_residual<LN, 8, true>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, \
+                           n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p)
     switch (lanes) {
         case 4: MG_RES(4); break;
         case 8: MG_RES(8); break;

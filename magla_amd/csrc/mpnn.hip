@@ -20,7 +20,8 @@
 // which keeps small eigenvalues of the preconditioned operator away from zero.  Inference runs
 // the network once per matrix in BF16 (weights and activations) with FP32 accumulation on MFMA
 // (v_mfma_f32_16x16x16_bf16: per 16-node tile, [h | m] (16 x 64) x [W1 | W2]^T (64 x 32)); the
-// solves then apply G and G' in FP64.  Every reduction has a fixed grid and order.
+// factor values are then stored in FP32 and the solves apply G and G' with FP64 accumulation
+// and FP64 vectors.  Every reduction has a fixed grid and order.
 #include <algorithm>
 #include <cmath>
 #include <random>

@@ -173,6 +173,25 @@ def test_multiple_rhs_on_one_factorization():
         np.testing.assert_allclose(X[:, k], f.solve(B[:, k]), rtol=0, atol=1e-14 * np.abs(X[:, k]).max())
 
 
+@pytest.mark.parametrize("k", [9, 20, 37])
+def test_refinement_with_many_rhs(k):
+    # residual passes of 8 and 16 columns with a ragged last pass; every column refined on its own
+    n, r, c, v = G.diffusion_3d_lognormal(12, seed=4)
+    pm, oc = build(n, r, c, v)
+    B = G.uniform_rhs(n, k, 21)
+    B[:, 3] = 0.0  # a zero column stays exactly zero
+    f = M.SymmetricUtduSolver(M.UtduSolverOptions(precision=M.FP32)).factorize(pm)
+    rep = {}
+    X = f.solve(B, tol=1e-10, max_refine=30, report=rep)
+    assert rep["converged"] and rep["refinement_steps"] >= 1
+    assert not X[:, 3].any()
+    for j in range(k):
+        if j != 3:
+            assert relres(oc, B[:, j], X[:, j]) <= 1e-10
+    X1 = f.solve(B[:, k - 1], tol=1e-10, max_refine=30)
+    assert relres(oc, B[:, k - 1], X1) <= 1e-10
+
+
 def test_diffusion_3d_matches_oracle():
     n, r, c, v = G.diffusion_3d_lognormal(20, seed=5)
     pm, oc = build(n, r, c, v)
