@@ -1025,8 +1025,10 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
                                                   const int *__restrict__ child_ptr, const int *__restrict__ children,
                                                   const int64_t *__restrict__ inv_off, const int *__restrict__ inv_map,
                                                   const int64_t *__restrict__ vbase, T *__restrict__ V,
-                                                  const T *__restrict__ Y, int n, int nrhs) {
+                                                  const T *__restrict__ Y, int n, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     const int2 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, f = F[g], w = W[g];
     const int i = tk.y * 256 + threadIdx.x;
     if (i >= f) return;
@@ -1085,6 +1087,8 @@ struct SmallFwdArgs {
     const int *inv_map;
     const int64_t *vbase, *off;
     int n, nrhs;
+    const int *fsys;           // system of each front
+    const unsigned char *act;  // per system: still refined (NULL: all)
 };
 
 // Forward: v = own right-hand-side rows plus the children's footer contributions (through
@@ -1101,8 +1105,11 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     if (wave_first >= count) return;  // whole wave
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
-    const bool live = idx < count;
-    const int g = nodes[live ? idx : count - 1];
+    const int g = nodes[idx < count ? idx : count - 1];
+    // a front of a converged system has a zero right-hand side: it is left out like a
+    // group past the end of the list (its vectors stay zero)
+    const bool live = idx < count && (!a.act || a.act[a.fsys[g]]);
+    if (!__any(live)) return;  // whole wave
     const int f = live ? a.F[g] : 0, w = live ? a.W[g] : 0, fc = a.firstcol[g];
     const bool own = i < w, foot = i >= w && i < f;
     const T *M = arena + a.off[g];
@@ -1122,8 +1129,10 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     const int c0 = live ? a.child_ptr[g] : 0, c1 = live ? a.child_ptr[g + 1] : 0;
     int64_t cb[kKeep];
     int cf[kKeep];
+    bool qany[kKeep];  // some front of the wave has a child in slot q (wave-uniform)
 #pragma unroll
     for (int q = 0; q < kKeep; ++q) {
+        qany[q] = __any(c0 + q < c1);
         cb[q] = -1;
         cf[q] = 0;
         if (c0 + q < c1 && i < f) {
@@ -1135,6 +1144,9 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     }
     T *Vg = V + a.vbase[g] * nrhs;
     const int yi = own ? fc + i : fc;  // clamped: other lanes read a valid row, then drop it
+    // a missing child's load is clamped into this front's own vector block: an address of the
+    // lane's neighbourhood (a fixed address would put every wave of the launch on one line)
+    const int64_t vself = a.vbase[g] * nrhs + min(i, max(f - 1, 0));
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
     for (int r0 = 0; r0 < nrhs; r0 += KP) {
@@ -1148,7 +1160,8 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
             T acc = (act && own) ? yv : T(0);
 #pragma unroll
             for (int q = 0; q < kKeep; ++q) {
-                const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : 0];
+                if (!qany[q]) break;
+                const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : vself + rr * f];
                 acc += (act && cb[q] >= 0) ? cv : T(0);
             }
             v[p] = acc;
@@ -1195,7 +1208,8 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
                                                    const int *__restrict__ firstcol, const int *__restrict__ pat_ptr,
                                                    const int *__restrict__ pat_rows, const T *__restrict__ arena,
                                                    const int64_t *__restrict__ off, const double *__restrict__ D,
-                                                   T *__restrict__ Y, int n, int nrhs) {
+                                                   T *__restrict__ Y, int n, int nrhs, const int *__restrict__ fsys,
+                                                   const unsigned char *__restrict__ act) {
     constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
     __shared__ __align__(16) T sb[4][2][KP][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1203,8 +1217,9 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     if (wave_first >= count) return;  // whole wave
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
-    const bool live = idx < count;
-    const int g = nodes[live ? idx : count - 1];
+    const int g = nodes[idx < count ? idx : count - 1];
+    const bool live = idx < count && (!act || act[fsys[g]]);  // converged systems are left out
+    if (!__any(live)) return;                                   // whole wave
     const int f = live ? F[g] : 0, w = live ? W[g] : 0, fc = firstcol[g];
     const bool own = i < w, foot = i >= w && i < f;
     const T *M = arena + off[g];
@@ -1217,8 +1232,8 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
         const T val = M[ix];
         R[j] = (own && (j < w ? j > i : j < f)) ? val : T(0);
     }
-    const int row = foot ? pat_rows[pat_ptr[g] + i] : 0;
     const int yi = own ? fc + i : fc;
+    const int row = foot ? pat_rows[pat_ptr[g] + i] : yi;  // clamped to the lane's own row
     const T dv = own ? (T)D[fc + i] : T(1);
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
@@ -1258,12 +1273,14 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
                                                    const int *__restrict__ F, const int *__restrict__ W,
                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                    const int64_t *__restrict__ vbase, T *__restrict__ V,
-                                                   T *__restrict__ Y, int ps, int n, int nrhs) {
+                                                   T *__restrict__ Y, int ps, int n, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     __shared__ T inv[kPanel][kPanel + 1];
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T vv[kPanel][kRhsChunk + 1];
     __shared__ T yv[kPanel][kRhsChunk + 1];
     const int2 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, rb = tk.y, f = F[g], w = W[g];
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     const T *M = arena + off[g];
@@ -1323,8 +1340,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_gather(const int2 *__restrict__ tasks, const int *__restrict__ F,
                                                     const int *__restrict__ W, const int *__restrict__ pat_ptr,
                                                     const int *__restrict__ pat_rows, const int64_t *__restrict__ vbase,
-                                                    const T *__restrict__ Y, T *__restrict__ V, int n, int nrhs) {
+                                                    const T *__restrict__ Y, T *__restrict__ V, int n, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     const int2 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, f = F[g], w = W[g];
     const int i = tk.y * 256 + threadIdx.x;
     if (i < w || i >= f) return;
@@ -1340,10 +1359,12 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
                                                        const int *__restrict__ F, const int *__restrict__ W,
                                                        const int64_t *__restrict__ off,
                                                        const int64_t *__restrict__ vbase, const T *__restrict__ V,
-                                                       T *__restrict__ part, int nrhs) {
+                                                       T *__restrict__ part, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T xs[kTile][kRhsChunk + 1];
     const int4 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, f = F[g], w = W[g];
     const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
     const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
@@ -1391,8 +1412,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ tasks, const int *__restrict__ W,
                                                       const int *__restrict__ firstcol, const double *__restrict__ D,
                                                       const T *__restrict__ Y, const T *__restrict__ part,
-                                                      T *__restrict__ Z, int n, int nrhs) {
+                                                      T *__restrict__ Z, int n, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     const int4 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, cb0 = tk.y * kPanel, tb = tk.z, nch = tk.w;
     const int ncol = min(kPanel, W[g] - cb0), fc = firstcol[g] + cb0;
     for (int idx = threadIdx.x; idx < ncol * nrhs; idx += 256) {
@@ -1411,12 +1434,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
                                                    const int *__restrict__ F, const int *__restrict__ W,
                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                   T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs) {
+                                                   T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs,
+        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T inv[kPanel][kPanel + 1];
     __shared__ T zz[kPanel][kRhsChunk + 1];
     __shared__ T xv[kPanel][kRhsChunk + 1];
     const int2 tk = tasks[blockIdx.x];
+    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, cb = tk.y, f = F[g], w = W[g];
     const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
     const int fc = firstcol[g];
@@ -1601,6 +1626,15 @@ __global__ __launch_bounds__(256) void k_residual_seg(const double *__restrict__
     }
 }
 
+// R[rows of system t, all columns] = 0 for the systems whose every column has converged
+// (one workgroup per system and column)
+__global__ __launch_bounds__(256) void k_zero_done(double *__restrict__ R, int n, const int *__restrict__ sys_begin,
+                                                   const unsigned char *__restrict__ act) {
+    const int t = blockIdx.x, c = blockIdx.y;
+    if (act[t]) return;
+    for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256) R[(int64_t)c * n + i] = 0.0;
+}
+
 __global__ void k_zero_cols(double *__restrict__ X, int ldx, int n, int nrhs) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * nrhs) return;
@@ -1683,6 +1717,8 @@ struct Numeric {
     // solve workspaces
     DevArray<unsigned char> wsY, wsU, wsP, wsQ;
     DevArray<double> wsR, wsX, wsPart, wsRel;
+    DevArray<unsigned char> wsAct;     // per system: still refined
+    std::vector<unsigned char> h_act;  // host copy (kept alive for the asynchronous upload)
     int ws_nrhs = 0;
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
     hipStream_t cap_stream = nullptr;
@@ -2036,7 +2072,7 @@ void numeric_info(const Numeric *N, mg_numeric_info &info, double *diag_host) {
 
 template <typename T>
 static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, double *d_x, int ldx, int accumulate,
-                           hipStream_t st) {
+                           hipStream_t st, const unsigned char *act = nullptr) {
     DevicePlan &P = *N.plan;
     const int n = N.n;
     T *Y = reinterpret_cast<T *>(N.wsY.p);
@@ -2047,7 +2083,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
     const SmallFwdArgs sfa{P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
-                           P.vbase.p, P.off.p, n, nrhs};
+                           P.vbase.p, P.off.p, n, nrhs, P.front_sys.p, act};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
         const double fl = 2.0 * nrhs * (P.lvl_solve_fac[L] - P.lvl_solve_w[L]);
@@ -2066,12 +2102,12 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         if (i1 > i0)
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
                                P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n,
-                               nrhs);
+                               nrhs, P.front_sys.p, act);
         for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
             const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
             if (s1 > s0)
                 hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs);
+                                   P.w.p, P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs, P.front_sys.p, act);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -2083,19 +2119,19 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         if (f1 > f0) {
             const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
             hipLaunchKernelGGL(k_bwd_gather<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
-                               P.pat_ptr.p, P.pat_rows.p, P.vbase.p, Y, V, n, nrhs);
+                               P.pat_ptr.p, P.pat_rows.p, P.vbase.p, Y, V, n, nrhs, P.front_sys.p, act);
             hipLaunchKernelGGL(k_bwd_foot_part<T>, dim3(f1 - f0), dim3(256), 0, st, P.foot_tasks.p + f0, arena, P.f.p,
-                               P.w.p, P.off.p, P.vbase.p, V, part, nrhs);
+                               P.w.p, P.off.p, P.vbase.p, V, part, nrhs, P.front_sys.p, act);
         }
         const int r0 = P.lvl_footr_ptr[L], r1 = P.lvl_footr_ptr[L + 1];
         if (r1 > r0)
             hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p,
-                               P.first.p, N.diag.p, Y, part, Z, n, nrhs);
+                               P.first.p, N.diag.p, Y, part, Z, n, nrhs, P.front_sys.p, act);
         for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
             const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
             if (s1 > s0)
                 hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, Y, Z, q, n, nrhs);
+                                   P.w.p, P.off.p, P.first.p, Y, Z, q, n, nrhs, P.front_sys.p, act);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
@@ -2105,7 +2141,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                                : (c == 1 ? k_bwd_small<T, 16> : (c == 2 ? k_bwd_small<T, 32> : k_bwd_small<T, 64>));
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
                                P.small_nodes.p + m0, m1 - m0, P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p,
-                               arena, P.off.p, N.diag.p, Y, n, nrhs);
+                               arena, P.off.p, N.diag.p, Y, n, nrhs, P.front_sys.p, act);
         }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
@@ -2124,6 +2160,7 @@ static void ensure_ws(Numeric &N, int nrhs) {
     N.wsR.alloc((size_t)n * nrhs);
     N.wsPart.alloc((size_t)residual_blocks(n, N.A.lanes) * nrhs * 2);
     N.wsRel.alloc((size_t)nrhs * std::max(N.plan->nsys, 1));
+    N.wsAct.alloc((size_t)std::max(N.plan->nsys, 1));
     N.ws_nrhs = nrhs;
 }
 
@@ -2173,11 +2210,11 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     if (N.n == 0 || nrhs == 0) return;
     if (ldb < N.n || ldx < N.n) throw Error(MG_E_ARG, "The right-hand side length does not match the matrix size.");
     ensure_ws(N, nrhs);
-    auto once = [&](const double *rhs, int ldr, int acc) {
-        if (N.precision == MG_FP32) run_solve_once<float>(N, nrhs, rhs, ldr, d_x, ldx, acc, st);
-        else run_solve_once<double>(N, nrhs, rhs, ldr, d_x, ldx, acc, st);
+    auto once = [&](const double *rhs, int ldr, int acc, const unsigned char *act) {
+        if (N.precision == MG_FP32) run_solve_once<float>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, act);
+        else run_solve_once<double>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, act);
     };
-    once(d_b, ldb, 0);
+    once(d_b, ldb, 0, nullptr);
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
     residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
@@ -2188,8 +2225,27 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     };
     double worst = worst_of(rel);
     int steps = 0;
+    const int nsys = N.plan->nsys;
     while (steps < max_refine && !(worst <= tol)) {
-        once(N.wsR.p, N.n, 1);
+        // in a batch, a correction solve leaves out the systems whose every column has
+        // converged: their residual rows are zeroed and their fronts skip all work
+        const unsigned char *act = nullptr;
+        if (nsys > 1) {
+            N.h_act.assign(nsys, 0);
+            int nact = 0;
+            for (int t = 0; t < nsys; ++t) {
+                for (int c = 0; c < nrhs; ++c)
+                    if (!(rel[(size_t)t * nrhs + c] <= tol)) N.h_act[t] = 1;
+                nact += N.h_act[t];
+            }
+            if (nact < nsys) {
+                MG_HIP(hipMemcpyAsync(N.wsAct.p, N.h_act.data(), nsys, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_zero_done, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, N.n,
+                                   N.plan->sys_begin.p, N.wsAct.p);
+                act = N.wsAct.p;
+            }
+        }
+        once(N.wsR.p, N.n, 1, act);
         ++steps;
         residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
         worst = worst_of(rel);

@@ -46,6 +46,32 @@ def test_batch_fp32_refinement_reaches_1e10_everywhere():
             assert np.linalg.norm(oc.spmv(x[:, k]) - b[:, k]) <= 1e-10 * np.linalg.norm(b[:, k])
 
 
+def test_converged_systems_leave_the_refinement():
+    # well-conditioned systems (diagonal shift 1) converge after the first correction, the
+    # shift-free 9-point Laplacians need more; the later corrections skip the converged ones,
+    # whose solutions must then stay as they were and still meet the tolerance
+    easy = [G.laplacian_2d(s, 1.0, False) for s in (12, 20, 9, 31)]
+    hard = [G.laplacian_2d(s, 0.0, True) for s in (64, 48)]
+    systems = [easy[0], hard[0], easy[1], easy[2], hard[1], easy[3]]
+    g = np.random.default_rng(3)
+    bs = [g.uniform(-1, 1, (n, 3)) for n, *_ in systems]
+    mats = [M.SymmetricCSRMatrix.from_coo(n, r, c, v) for n, r, c, v in systems]
+    batch = M.UtduBatch(mats, M.UtduSolverOptions(precision=M.FP32))
+    batch.factorize()
+    xs = batch.solve(bs, tol=1e-12, max_refine=30)
+    assert batch.last_report["converged"] and batch.last_report["refinement_steps"] >= 2
+    x1 = batch.solve(bs, tol=1e-12, max_refine=1)
+    for (n, r, c, v), b, x, xa in zip(systems, bs, xs, x1):
+        oc = O.Csr.from_coo(n, r, c, v)
+        for k in range(3):
+            assert np.linalg.norm(oc.spmv(x[:, k]) - b[:, k]) <= 1e-12 * np.linalg.norm(b[:, k])
+    for i in (0, 2, 3, 5):  # converged after one correction: later corrections left them alone
+        oc = O.Csr.from_coo(*systems[i])
+        if all(np.linalg.norm(oc.spmv(x1[i][:, k]) - bs[i][:, k]) <= 1e-12 * np.linalg.norm(bs[i][:, k])
+               for k in range(3)):
+            np.testing.assert_array_equal(xs[i], x1[i])
+
+
 def test_pivot_scale_is_per_system():
     # system 0 is singular ([[1,1],[1,1]] (+) 2 (+) 3: one regularized pivot); system 1 is an SPD
     # matrix of the same pattern times 1e14.  With one scale for the whole batch (4e14) every
