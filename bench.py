@@ -41,6 +41,28 @@ def measured_traffic(config, kernel):
         return None
 
 
+COLD_BYTES = 512 << 20  # written before every cold step: larger than the 256 MB Infinity Cache
+
+
+def cold_ms(step_fn, steps, stream):
+    """Mean time of `steps` steps, each after writing COLD_BYTES of scratch (caches hold no
+    operand of the step when it starts)."""
+    import torch
+    scratch = torch.empty(COLD_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    times = []
+    for i in range(steps):
+        scratch.fill_(float(i))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        step_fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    del scratch
+    return sum(times) / len(times)
+
+
 def calibrate_copy():
     """One plain device copy of CALIBRATION_BYTES (read and written once each): the reference
     point that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run."""
@@ -135,6 +157,9 @@ def bench_batch(args):
     log(f"[bench] batch fp32+IR: {ms32:.2f} ms/step {reps32[-1]}")
     ms64, reps64 = timed(b64, args.steps)
     log(f"[bench] batch fp64: {ms64:.2f} ms/step {reps64[-1]}")
+    cold32 = cold_ms(lambda: step(b32), args.steps, stream)
+    cold64 = cold_ms(lambda: step(b64), args.steps, stream)
+    log(f"[bench] batch cold caches: fp32+IR {cold32:.2f} ms/step, fp64 {cold64:.2f} ms/step")
     M.profile_enable(True)
     step(b32)
     torch.cuda.synchronize()
@@ -142,6 +167,28 @@ def bench_batch(args):
     M.profile_enable(False)
     name, t = max(prof.items(), key=lambda kv: kv[1]["ms"])
     roof = roofline_of(name, t, "c4")
+    cpu = None
+    if args.cpu_seconds > 0:
+        # oracle numeric factorization of the systems in order until the time budget is spent,
+        # scaled to the whole batch by FactorOperationCount (symbolic phase not timed, solves omitted)
+        import oracle as O
+        num_s, ops_done, k_done = 0.0, 0, 0
+        for n_, r_, c_, v_, _b in sysl:
+            oc = O.Csr.from_coo(n_, r_, c_, v_)
+            osym = O.Symbolic(oc)
+            c0 = time.time()
+            O.Numeric(osym, oc)
+            num_s += time.time() - c0
+            ops_done += osym.factor_ops
+            k_done += 1
+            if num_s >= args.cpu_seconds:
+                break
+        total_ops = info["factor_operation_count"]
+        cpu = {"value": round(num_s * total_ops / ops_done * 1e3, 3), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"oracle (serial FP64 C restatement of UtduNumericFactorization) factored the first "
+                         f"{k_done}/{len(sysl)} systems = {100.0 * ops_done / total_ops:.2f}% of FactorOperationCount "
+                         f"in {num_s:.1f}s; value = that time scaled by operation count to the whole batch "
+                         f"(solves omitted)"}
     out = {
         "metric": "factor+solve time to 1e-10 residual, FP32-MFMA+IR vs FP64, 1 MI355X",
         "value": round(ms32, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -153,9 +200,10 @@ def bench_batch(args):
                    "n_total": nt, "rhs_total": nrhs_total, "factor_operation_count": info["factor_operation_count"],
                    "hipgraph": bool(args.graph), "schur_tiles": tiles},
         "fp64_ms_per_step": round(ms64, 3), "speedup_fp32ir_vs_fp64": round(ms64 / ms32, 3),
+        "cold_ms_per_step": round(cold32, 3), "cold_fp64_ms_per_step": round(cold64, 3),
         "fp32_relres": reps32[-1]["relative_residual"], "fp32_refinement_steps": reps32[-1]["refinement_steps"],
         "symbolic_s": round(symbolic_s, 3), "roofline": roof,
-        "kernels": kernel_table(prof), "cpu_baseline": None,
+        "kernels": kernel_table(prof), "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -208,6 +256,38 @@ def bench_learned(args):
     msl, resl = min(x[0] for x in tl), tl[-1][1]
     log(f"[bench] jacobi-pcg: {resj.iterations} it, {msj:.1f} ms; learned: {resl.iterations} it, {msl:.1f} ms "
         f"(fell back: {lrn.fell_back})")
+    # the network runs once per matrix (BF16 inference of the factor G); setting the parameters
+    # again drops the cached factor, so this solve pays for the inference as well
+    net.params = net.params
+    ms_inf, _ = run(lrn)
+    log(f"[bench] learned solve including the per-matrix inference: {ms_inf:.1f} ms")
+
+    # roofline of the learned PCG iteration (HBM-bound): full-pattern SpMV, the products with G
+    # and G' (FP32 value + int32 column per entry, two FP64 vectors per row), and the 11
+    # n-vector streams of a PCG iteration
+    import scipy.sparse as sps
+    full = sps.coo_matrix((np.ones(len(r)), (r, c)), shape=(n, n)).tocsr()
+    full = ((full + full.T) != 0).astype(np.float64)
+    nnz_full = full.nnz
+    two_hop = sps.tril((full @ full) + full, format="csr")
+    two_hop.data[:] = 1.0
+    g_entries = int(two_hop.nnz)
+    bytes_it = 12.0 * nnz_full + 4.0 * (n + 1) + 16.0 * n + 2 * (8.0 * g_entries + 16.0 * n) + 88.0 * n
+    achieved = bytes_it * resl.iterations / (msl * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+            "frac": achieved / PEAK["hbm_gbs"], "traffic": None, "kernel": "learned PCG iteration",
+            "bytes_per_iteration": bytes_it, "g_entries": g_entries, "nnz_full": int(nnz_full)}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        import oracle as O
+        oc = O.Csr.from_coo(n, r, c, v)
+        c0 = time.time()
+        _, ores = O.pcg(oc, np.asarray(b, dtype=np.float64).reshape(-1), tol=1e-8)
+        cpu_s = time.time() - c0
+        cpu = {"value": round(cpu_s * 1e3, 3), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"oracle (serial C restatement of ConjugateGradientGaussPreSolver + JacobiPreconditioner), "
+                         f"the whole Jacobi-PCG solve of this system: {ores['iterations']} iterations to 1e-8"}
     out = {
         "metric": "PCG time to 1e-8 relative residual, learned vs Jacobi preconditioner, 1 MI355X",
         "value": round(msl, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -223,7 +303,8 @@ def bench_learned(args):
         "time_ratio": round(msj / msl, 3), "fell_back": lrn.fell_back,
         "learned_relres": resl.relative_residual, "jacobi_relres": resj.relative_residual,
         "train_losses": [round(float(x), 5) for x in losses], "pairs_s": round(pairs_s, 2),
-        "train_s": round(train_s, 2), "roofline": None, "cpu_baseline": None,
+        "train_s": round(train_s, 2), "learned_ms_incl_inference": round(ms_inf, 3),
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -342,6 +423,9 @@ def main():
     log(f"[bench] fp32+IR: {ms32:.2f} ms/step (wall {wall32:.2f}) {reps32[-1]}")
     ms64, wall64, reps64 = timed(M.FP64, args.steps)
     log(f"[bench] fp64: {ms64:.2f} ms/step (wall {wall64:.2f}) {reps64[-1]}")
+    cold32 = cold_ms(lambda: step(M.FP32), args.steps, stream)
+    cold64 = cold_ms(lambda: step(M.FP64), args.steps, stream)
+    log(f"[bench] cold caches: fp32+IR {cold32:.2f} ms/step, fp64 {cold64:.2f} ms/step")
 
     # forward error of the FP32+IR solution against the FP64 factor's solution
     step(M.FP64)
@@ -401,6 +485,8 @@ def main():
                    "hipgraph": bool(args.graph)},
         "fp64_ms_per_step": round(ms64, 3),
         "speedup_fp32ir_vs_fp64": round(ms64 / ms32, 3),
+        "cold_ms_per_step": round(cold32, 3),
+        "cold_fp64_ms_per_step": round(cold64, 3),
         "fp32_relres": reps32[-1]["relative_residual"],
         "fp32_refinement_steps": reps32[-1]["refinement_steps"],
         "fp64_relres": reps64[-1]["relative_residual"],
