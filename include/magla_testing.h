@@ -34,6 +34,10 @@ int mg_test_mfma_bf16(const float *a, const float *b, float *d);
  * for the handle's precision without timing anything. */
 int mg_test_set_schur_tiles(mg_numeric *num, const int32_t *tiles);
 
+/* ILU(0) factors in the matrix's pattern (unit-lower L strictly below the diagonal, U from the
+ * diagonal), nnz values in CSR order. */
+int mg_test_ilu0_factors(const mg_ilu0 *p, double *out);
+
 /* Launch timing.  mg_profile_enable(1) clears the tallies and brackets every
  * factorization launch with HIP events on its stream; mg_profile_read(kind, ...)
  * returns 1 and the kernel's name, launch count, summed event time (ms) and

@@ -9,6 +9,9 @@ benchmark read like the reference's own code:
   UtduSymbolicFactorization   UCHOL/UtduSymbolicFactorization.cs:109-320
   UtduNumericFactorization    UCHOL/UtduNumericFactorization.cs:24-235
   ConjugateGradientGaussPreSolver  CGGP/ConjugateGradientGaussPreSolver.cs:40-196
+  CSRMatrix                   CSRMatrix.cs (general square CSR)
+  Ilu0Preconditioner          BiCGSTAB/Ilu0Preconditioner.cs
+  BiCgStabSolver              BiCGSTAB/BiCgStabSolver.cs
 
 All numeric work runs in libmagla.so on the GPU; there is no CPU fallback:
 loading fails loudly when the library is missing.
@@ -157,6 +160,24 @@ def lib():
     L.mg_autotune_dev.argtypes = [vp, vp, C.POINTER(_Options), vp, vp, C.POINTER(C.c_double)]
     L.mg_batch_autotune_dev.argtypes = [vp, vp, C.POINTER(_Options), vp, vp, C.POINTER(C.c_double)]
     L.mg_test_set_schur_tiles.argtypes = [vp, vp]
+    L.mg_gcsr_from_coo.argtypes = [C.c_int32, C.c_int32, C.c_int64, _i32p, _i32p, _f64p, C.POINTER(C.c_int32),
+                                   vp, vp, vp]
+    L.mg_gcsr_line_cross.argtypes = [C.c_int32, C.c_int32, _i32p, vp, vp, _f64p, C.c_int32, _i32p, _f64p]
+    L.mg_gmatrix_create.argtypes = [C.c_int32, C.c_int32, _i32p, vp, vp, C.POINTER(vp)]
+    L.mg_gmatrix_release.argtypes = [vp]
+    L.mg_gmatrix_release.restype = None
+    L.mg_gspmv.argtypes = [vp, _f64p, _f64p]
+    L.mg_gspmv_dev.argtypes = [vp, vp, vp, vp]
+    L.mg_ilu0_create.argtypes = [vp, C.c_double, C.POINTER(vp)]
+    L.mg_ilu0_release.argtypes = [vp]
+    L.mg_ilu0_release.restype = None
+    L.mg_ilu0_apply.argtypes = [vp, _f64p, _f64p]
+    L.mg_ilu0_apply_dev.argtypes = [vp, vp, vp, vp]
+    L.mg_bicgstab_solve.argtypes = [vp, vp, C.c_int32, _f64p, vp, C.c_double, C.c_int32, C.c_double, _f64p,
+                                    C.POINTER(_PcgResult)]
+    L.mg_bicgstab_solve_dev.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_double, C.c_int32, C.c_double, vp, vp,
+                                        C.POINTER(_PcgResult)]
+    L.mg_test_ilu0_factors.argtypes = [vp, _f64p]
     L.mg_batch_release.argtypes = [vp]
     L.mg_batch_release.restype = None
     L.mg_mpnn_param_count.restype = C.c_int32
@@ -780,3 +801,167 @@ class UtduBatch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.mg_batch_release(self._h)
             self._h = None
+
+
+# ---------------------------------------------------------------------------
+# BiCGSTAB with ILU(0) on general square CSR matrices
+# ---------------------------------------------------------------------------
+class CSRMatrix:
+    """CSRMatrix (CAESolvers/CSRMatrix.cs): general rows x cols CSR, columns ascending per row.
+    The device copy (for SpMV, ILU(0) and BiCGSTAB) is made on first use and dropped when the
+    values change."""
+
+    def __init__(self, rows, cols, row_pointers, column_indices, values):
+        self.row_count, self.column_count = int(rows), int(cols)
+        self.row_pointers = np.ascontiguousarray(row_pointers, dtype=np.int32)
+        self.column_indices = np.ascontiguousarray(column_indices, dtype=np.int32)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        self._h = None
+
+    @classmethod
+    def from_coo(cls, rows, cols, row_idx, col_idx, vals):
+        """CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:33-75)."""
+        ri = np.ascontiguousarray(row_idx, dtype=np.int32)
+        cj = np.ascontiguousarray(col_idx, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        nnz = C.c_int32()
+        _check(lib().mg_gcsr_from_coo(rows, cols, len(ri), ri, cj, v, C.byref(nnz), None, None, None))
+        rp = np.zeros(rows + 1, dtype=np.int32)
+        ci = np.zeros(nnz.value, dtype=np.int32)
+        vv = np.zeros(nnz.value)
+        _check(lib().mg_gcsr_from_coo(rows, cols, len(ri), ri, cj, v, C.byref(nnz), _ptr(rp), _ptr(ci), _ptr(vv)))
+        return cls(rows, cols, rp, ci, vv)
+
+    @classmethod
+    def from_dense(cls, A):
+        A = np.asarray(A, dtype=np.float64)
+        ri, cj = np.nonzero(A)
+        return cls.from_coo(A.shape[0], A.shape[1], ri, cj, A[ri, cj])
+
+    @property
+    def nonzero_count(self):
+        return int(self.row_pointers[-1])
+
+    def _dev(self):
+        if self._h is None:
+            h = C.c_void_p()
+            _check(lib().mg_gmatrix_create(self.row_count, self.column_count, self.row_pointers,
+                                           _ptr(self.column_indices), _ptr(self.values), C.byref(h)))
+            self._h = h
+        return self._h
+
+    def multiply(self, x):
+        """CSRMatrix.Multiply (CSRMatrix.cs:244-262), on the GPU."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape[0] != self.column_count:
+            raise ArgumentError(MG_E_ARG, f"Vector length {x.shape[0]} does not match the matrix column count "
+                                          f"{self.column_count}.")
+        y = np.zeros(self.row_count)
+        _check(lib().mg_gspmv(self._dev(), x, y))
+        return y
+
+    def line_cross(self, rhs, prescribed_value, index):
+        """CSRMatrix.LineCross (CSRMatrix.cs:103-164): rhs is updated in place."""
+        if rhs is None:
+            raise ArgumentNullError(MG_E_NULL, "rightHandSide")
+        if len(rhs) != self.row_count:
+            raise ArgumentError(MG_E_ARG, f"The right-hand side length {len(rhs)} does not match the matrix row "
+                                          f"count {self.row_count}.")
+        idx = np.array([index], dtype=np.int32)
+        pv = np.array([prescribed_value], dtype=np.float64)
+        _check(lib().mg_gcsr_line_cross(self.row_count, self.column_count, self.row_pointers,
+                                        _ptr(self.column_indices), _ptr(self.values), rhs, 1, idx, pv))
+        self._release()
+
+    def _release(self):
+        if self._h is not None and _lib is not None:
+            _lib.mg_gmatrix_release(self._h)
+        self._h = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            self._release()
+
+
+class Ilu0Preconditioner:
+    """Ilu0Preconditioner (BiCGSTAB/Ilu0Preconditioner.cs): ILU(0) built on the GPU."""
+
+    def __init__(self, matrix: CSRMatrix, pivot_tolerance=1e-30):
+        if matrix is None:
+            raise ArgumentNullError(MG_E_NULL, "matrix")
+        self.matrix = matrix
+        self.size = matrix.row_count
+        self.pivot_tolerance = pivot_tolerance
+        h = C.c_void_p()
+        _check(lib().mg_ilu0_create(matrix._dev(), pivot_tolerance, C.byref(h)))
+        self._h = h
+
+    def apply(self, right_hand_side):
+        """Apply (:36-90): (L U)^-1 right_hand_side."""
+        b = np.ascontiguousarray(right_hand_side, dtype=np.float64)
+        if b.shape[0] != self.size:
+            raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the preconditioner size.")
+        x = np.zeros(self.size)
+        _check(lib().mg_ilu0_apply(self._h, b, x))
+        return x
+
+    def factors(self):
+        """Test hook: L (strictly lower, unit diagonal) and U in the matrix's pattern, CSR order."""
+        out = np.zeros(self.matrix.nonzero_count)
+        _check(lib().mg_test_ilu0_factors(self._h, out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mg_ilu0_release(self._h)
+            self._h = None
+
+
+class BiCgStabSolver:
+    """BiCgStabSolver (BiCGSTAB/BiCgStabSolver.cs) on the GPU."""
+
+    def __init__(self, relative_tolerance=1e-8, max_iterations=0, use_preconditioner=True,
+                 breakdown_tolerance=1e-30):
+        self.relative_tolerance = relative_tolerance
+        self.max_iterations = max_iterations
+        self.use_preconditioner = use_preconditioner
+        self.breakdown_tolerance = breakdown_tolerance
+        self.last_result = None
+
+    def solve(self, matrix: CSRMatrix, right_hand_side):
+        return self.solve_with_initial_guess(matrix, right_hand_side, None)
+
+    def solve_with_initial_guess(self, matrix: CSRMatrix, right_hand_side, initial_guess,
+                                 preconditioner: Ilu0Preconditioner | None = None):
+        self.last_result = None
+        if matrix is None:
+            raise ArgumentNullError(MG_E_NULL, "matrix")
+        b = np.ascontiguousarray(right_hand_side, dtype=np.float64)
+        if b.shape[0] != matrix.row_count:
+            raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the matrix size.")
+        x0 = None
+        if initial_guess is not None:
+            x0 = np.ascontiguousarray(initial_guess, dtype=np.float64)
+            if x0.shape[0] != matrix.row_count:
+                raise ArgumentError(MG_E_ARG, "The initial guess length does not match the matrix size.")
+        if preconditioner is not None and preconditioner.matrix is not matrix:
+            raise ArgumentError(MG_E_ARG, "The preconditioner was built for another matrix.")
+        x = np.zeros(matrix.row_count)
+        res = _PcgResult()
+        _check(lib().mg_bicgstab_solve(matrix._dev(), preconditioner._h if preconditioner is not None else None,
+                                       1 if self.use_preconditioner else 0, b, _ptr(x0), self.relative_tolerance,
+                                       self.max_iterations, self.breakdown_tolerance, x, C.byref(res)))
+        self.last_result = IterativeSolverResult(x, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return x
+
+    def solve_dev(self, matrix: CSRMatrix, b_ptr, x_ptr, x0_ptr=None, preconditioner=None, stream=None):
+        res = _PcgResult()
+        _check(lib().mg_bicgstab_solve_dev(matrix._dev(), preconditioner._h if preconditioner is not None else None,
+                                           1 if self.use_preconditioner else 0, C.c_void_p(b_ptr),
+                                           C.c_void_p(x0_ptr) if x0_ptr else None, self.relative_tolerance,
+                                           self.max_iterations, self.breakdown_tolerance, C.c_void_p(x_ptr),
+                                           C.c_void_p(stream) if stream else None, C.byref(res)))
+        self.last_result = IterativeSolverResult(None, res.iterations, bool(res.converged), res.residual_norm,
+                                                 res.relative_residual)
+        return self.last_result

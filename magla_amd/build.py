@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmagla.so")
 
-SOURCES = ["symbolic.cpp", "csr.cpp", "capi.cpp", "factor.hip", "spmv.hip", "mpnn.hip"]
+SOURCES = ["symbolic.cpp", "csr.cpp", "capi.cpp", "factor.hip", "spmv.hip", "mpnn.hip", "bicgstab.hip"]
 ARCH = os.environ.get("MAGLA_OFFLOAD_ARCH", "gfx950")
 
 

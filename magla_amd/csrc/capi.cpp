@@ -2,6 +2,8 @@
 // Every call converts mg::Error / std::bad_alloc into an mg_status and keeps
 // the message for mg_last_error() on the calling thread.
 #include <cstring>
+#include <exception>
+#include <memory>
 #include <new>
 #include <string>
 
@@ -19,7 +21,18 @@ void csr_from_coo(int n, int64_t count, const int *rows, const int *cols, const 
 void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values, double *rhs, int count,
                     const int *indices, const double *prescribed);
 void selftest_mfma_bf16(const float *a, const float *b, float *d);
+void gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
+                   int *nnz_out, int *rowptr, int *colidx, double *values);
+void gcsr_line_cross(int rows, int cols, const int *rowptr, const int *colidx, double *values, double *rhs,
+                     int count, const int *indices, const double *prescribed);
 }  // namespace mg
+
+struct mg_gmatrix {
+    mg::GMatrix g;
+};
+struct mg_ilu0 {
+    mg::Ilu0 p;
+};
 
 struct mg_symbolic {
     mg::Symbolic s;
@@ -134,6 +147,136 @@ int mg_csr_line_cross(int32_t n, const int32_t *rowptr, const int32_t *colidx, d
         require(count == 0 || indices, MG_E_NULL, "indices are null.");
         check_csr(n, rowptr, colidx);
         mg::csr_line_cross(n, rowptr, colidx, values, rhs, count, indices, prescribed);
+    });
+}
+
+int mg_gcsr_from_coo(int32_t rows, int32_t cols, int64_t count, const int32_t *row_idx, const int32_t *col_idx,
+                     const double *vals, int32_t *nnz_out, int32_t *rowptr, int32_t *colidx, double *values) {
+    return guarded([&] {
+        require(nnz_out != nullptr, MG_E_NULL, "nnz_out is null.");
+        require(count == 0 || (row_idx && col_idx && vals), MG_E_NULL, "elements are null.");
+        require(rows >= 0 && cols >= 0 && count >= 0, MG_E_ARG, "Negative size.");
+        mg::gcsr_from_coo(rows, cols, count, row_idx, col_idx, vals, nnz_out, rowptr, colidx, values);
+    });
+}
+
+int mg_gcsr_line_cross(int32_t rows, int32_t cols, const int32_t *rowptr, const int32_t *colidx, double *values,
+                       double *rhs, int32_t count, const int32_t *indices, const double *prescribed) {
+    return guarded([&] {
+        require(rhs != nullptr, MG_E_NULL, "rightHandSide is null.");
+        require(rowptr != nullptr && (rowptr[rows] == 0 || (colidx && values)), MG_E_NULL, "matrix is null.");
+        require(count == 0 || indices, MG_E_NULL, "indices are null.");
+        mg::gcsr_line_cross(rows, cols, rowptr, colidx, values, rhs, count, indices, prescribed);
+    });
+}
+
+int mg_gmatrix_create(int32_t rows, int32_t cols, const int32_t *rowptr, const int32_t *colidx,
+                      const double *values, mg_gmatrix **out) {
+    return guarded([&] {
+        require(out != nullptr && rowptr != nullptr, MG_E_NULL, "matrix is null.");
+        require(rows >= 0 && cols >= 0, MG_E_ARG, "Matrix dimensions cannot be negative.");
+        require(rowptr[rows] == 0 || (colidx && values), MG_E_NULL, "matrix is null.");
+        auto *h = new mg_gmatrix();
+        try {
+            mg::gmatrix_build(h->g, rows, cols, rowptr, colidx, values);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void mg_gmatrix_release(mg_gmatrix *m) { delete m; }
+
+int mg_gspmv_dev(mg_gmatrix *m, const double *d_x, double *d_y, void *stream) {
+    return guarded([&] {
+        require(m != nullptr && (m->g.rows == 0 || (d_x && d_y)), MG_E_NULL, "vector is null.");
+        mg::gspmv_device(m->g, d_x, d_y, (hipStream_t)stream);
+    });
+}
+
+int mg_gspmv(mg_gmatrix *m, const double *x, double *y) {
+    return guarded([&] {
+        require(m != nullptr && (m->g.rows == 0 || (x && y)), MG_E_NULL, "vector is null.");
+        const mg::GMatrix &G = m->g;
+        mg::DevArray<double> dx(std::max(G.cols, 1)), dy(std::max(G.rows, 1));
+        if (G.cols) MG_HIP(hipMemcpy(dx.p, x, sizeof(double) * G.cols, hipMemcpyHostToDevice));
+        mg::gspmv_device(G, dx.p, dy.p, 0);
+        if (G.rows) MG_HIP(hipMemcpy(y, dy.p, sizeof(double) * G.rows, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_ilu0_create(mg_gmatrix *m, double pivot_tolerance, mg_ilu0 **out) {
+    return guarded([&] {
+        require(m != nullptr && out != nullptr, MG_E_NULL, "matrix is null.");
+        auto *h = new mg_ilu0();
+        try {
+            mg::ilu0_build(h->p, m->g, pivot_tolerance, 0);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void mg_ilu0_release(mg_ilu0 *p) { delete p; }
+
+int mg_ilu0_apply_dev(mg_ilu0 *p, const double *d_b, double *d_x, void *stream) {
+    return guarded([&] {
+        require(p != nullptr && (p->p.n == 0 || (d_b && d_x)), MG_E_NULL, "rightHandSide is null.");
+        mg::ilu0_apply_device(p->p, d_b, d_x, (hipStream_t)stream);
+    });
+}
+
+int mg_ilu0_apply(mg_ilu0 *p, const double *b, double *x) {
+    return guarded([&] {
+        require(p != nullptr && (p->p.n == 0 || (b && x)), MG_E_NULL, "rightHandSide is null.");
+        const int n = p->p.n;
+        mg::DevArray<double> db(std::max(n, 1)), dx(std::max(n, 1));
+        if (n) MG_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        mg::ilu0_apply_device(p->p, db.p, dx.p, 0);
+        if (n) MG_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_bicgstab_solve_dev(mg_gmatrix *m, mg_ilu0 *pre, int32_t use_preconditioner, const double *d_b,
+                          const double *d_x0, double relative_tolerance, int32_t max_iterations,
+                          double breakdown_tolerance, double *d_x, void *stream, mg_pcg_result *result) {
+    return guarded([&] {
+        require(m != nullptr && result != nullptr, MG_E_NULL, "matrix is null.");
+        require(m->g.rows == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        std::unique_ptr<mg::Ilu0> owned;
+        mg::bicgstab_device(m->g, pre ? &pre->p : nullptr, use_preconditioner, d_b, d_x0, relative_tolerance,
+                            max_iterations, breakdown_tolerance, d_x, (hipStream_t)stream, *result, owned);
+        MG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+int mg_bicgstab_solve(mg_gmatrix *m, mg_ilu0 *pre, int32_t use_preconditioner, const double *b, const double *x0,
+                      double relative_tolerance, int32_t max_iterations, double breakdown_tolerance, double *x,
+                      mg_pcg_result *result) {
+    return guarded([&] {
+        require(m != nullptr && result != nullptr, MG_E_NULL, "matrix is null.");
+        const int n = m->g.rows;
+        require(n == 0 || (b && x), MG_E_NULL, "rightHandSide is null.");
+        mg::DevArray<double> db(std::max(n, 1)), dx0(std::max(n, 1)), dx(std::max(n, 1));
+        if (n) MG_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (n && x0) MG_HIP(hipMemcpy(dx0.p, x0, sizeof(double) * n, hipMemcpyHostToDevice));
+        std::unique_ptr<mg::Ilu0> owned;
+        mg_pcg_result r{};
+        std::exception_ptr err;
+        try {
+            mg::bicgstab_device(m->g, pre ? &pre->p : nullptr, use_preconditioner, db.p, x0 ? dx0.p : nullptr,
+                                relative_tolerance, max_iterations, breakdown_tolerance, dx.p, 0, r, owned);
+        } catch (...) {
+            err = std::current_exception();
+        }
+        // the last iterate is returned also when the iteration breaks down
+        if (n) MG_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (err) std::rethrow_exception(err);
+        *result = r;
     });
 }
 
@@ -598,6 +741,14 @@ int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out)
     return guarded([&] {
         require(num != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
         mg::numeric_front(num->num, supernode, out);
+    });
+}
+
+int mg_test_ilu0_factors(const mg_ilu0 *p, double *out) {
+    return guarded([&] {
+        require(p != nullptr && out != nullptr, MG_E_NULL, "preconditioner or out is null.");
+        const int nnz = p->p.A ? p->p.A->nnz : 0;
+        if (nnz) MG_HIP(hipMemcpy(out, p->p.f.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
     });
 }
 

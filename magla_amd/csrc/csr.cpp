@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "internal.h"
 
@@ -114,6 +115,85 @@ void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values,
             values[k] = 0.0;
         }
         values[diag[idx]] = 1.0;
+        rhs[idx] = pv;
+    }
+}
+
+// CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:33-75): duplicates of one
+// (row, col) summed in input order, |sum| <= 1e-15 dropped, columns ascending per row.
+// Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-76).
+void gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
+                   int *nnz_out, int *rowptr, int *colidx, double *values) {
+    for (int64_t k = 0; k < count; ++k)
+        if (ri[k] < 0 || ri[k] >= rows || cj[k] < 0 || cj[k] >= cols)
+            throw Error(MG_E_RANGE, "Indices are out of range.");
+    std::vector<int64_t> rstart(rows + 1, 0);
+    for (int64_t k = 0; k < count; ++k) ++rstart[ri[k] + 1];
+    for (int i = 0; i < rows; ++i) rstart[i + 1] += rstart[i];
+    std::vector<int64_t> byrow(count);
+    {
+        std::vector<int64_t> at(rstart.begin(), rstart.end() - 1);
+        for (int64_t k = 0; k < count; ++k) byrow[at[ri[k]]++] = k;
+    }
+    std::vector<int> rp(rows + 1, 0), ci;
+    std::vector<double> va;
+    std::vector<int64_t> colhead(cols, -1), next(count, -1), coltail(cols, -1);
+    std::vector<int> touched;
+    for (int r = 0; r < rows; ++r) {
+        touched.clear();
+        for (int64_t q = rstart[r]; q < rstart[r + 1]; ++q) {
+            const int64_t k = byrow[q];
+            const int c = cj[k];
+            if (colhead[c] < 0) { colhead[c] = k; touched.push_back(c); }
+            else next[coltail[c]] = k;
+            coltail[c] = k;
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int c : touched) {
+            double sum = 0.0;
+            for (int64_t k = colhead[c]; k >= 0; k = next[k]) sum += vals[k];
+            colhead[c] = -1;
+            if (std::fabs(sum) > 1e-15) {
+                ci.push_back(c);
+                va.push_back(sum);
+            }
+        }
+        rp[r + 1] = (int)ci.size();
+    }
+    if (ci.size() > 2147483647ULL) throw Error(MG_E_TOO_DENSE, "Too many nonzeros.");
+    *nnz_out = (int)ci.size();
+    if (rowptr) std::memcpy(rowptr, rp.data(), sizeof(int) * (rows + 1));
+    if (colidx && !ci.empty()) std::memcpy(colidx, ci.data(), sizeof(int) * ci.size());
+    if (values && !va.empty()) std::memcpy(values, va.data(), sizeof(double) * va.size());
+}
+
+// CSRMatrix.LineCross (CSRMatrix.cs:103-164) for each constrained equation in the given order:
+// the row's off-diagonal values are zeroed, the column's values move to the right-hand side
+// (rhs[row] -= a * prescribed) and are zeroed, the diagonal becomes 1, rhs[index] = prescribed.
+void gcsr_line_cross(int rows, int cols, const int *rowptr, const int *colidx, double *values, double *rhs,
+                     int count, const int *indices, const double *prescribed) {
+    if (rows != cols) throw Error(MG_E_NOT_PD, "A boundary condition can be applied only to a square matrix.");
+    for (int q = 0; q < count; ++q) {
+        const int idx = indices[q];
+        if (idx < 0 || idx >= rows) throw Error(MG_E_RANGE, "index is out of range.");
+        int dpos = -1;
+        for (int k = rowptr[idx]; k < rowptr[idx + 1]; ++k)
+            if (colidx[k] == idx) dpos = k;
+        if (dpos < 0)
+            throw Error(MG_E_NOT_PD, "Diagonal position (" + std::to_string(idx) + ", " + std::to_string(idx) +
+                                         ") is absent from the sparse structure.");
+        const double pv = prescribed ? prescribed[q] : 0.0;
+        for (int r = 0; r < rows; ++r)
+            for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+                const int c = colidx[k];
+                if (r == idx) {
+                    if (c != idx) values[k] = 0.0;
+                } else if (c == idx) {
+                    rhs[r] -= values[k] * pv;
+                    values[k] = 0.0;
+                }
+            }
+        values[dpos] = 1.0;
         rhs[idx] = pv;
     }
 }

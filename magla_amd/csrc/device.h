@@ -95,6 +95,29 @@ void spmv_device(const Matrix &A, int nrhs, const double *d_x, int ldx, double *
 void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double tol, int max_it, int use_jacobi,
                 double *d_x, hipStream_t st, mg_pcg_result &res);
 
+// General square/rectangular CSR matrix on the device (CSRMatrix.cs), ILU(0) of it and
+// BiCGSTAB (BiCgStabSolver.cs, Ilu0Preconditioner.cs): bicgstab.hip
+struct GMatrix {
+    int rows = 0, cols = 0, nnz = 0;
+    std::vector<int> h_rowptr, h_colidx;
+    DevArray<int> rowptr, colidx;
+    DevArray<double> vals;
+};
+struct Ilu0 {
+    const GMatrix *A = nullptr;
+    int n = 0, nlev_l = 0, nlev_u = 0;
+    double pivot_tol = 0.0;
+    DevArray<int> diag, lptr, lrows, uptr, urows, bad;  // diagonal positions, row levels of L and U
+    DevArray<double> f, w;                              // factors in A's pattern, forward workspace
+};
+void gmatrix_build(GMatrix &G, int rows, int cols, const int *rowptr, const int *colidx, const double *values);
+void gspmv_device(const GMatrix &G, const double *d_x, double *d_y, hipStream_t st);
+void ilu0_build(Ilu0 &P, const GMatrix &A, double pivot_tol, hipStream_t st);
+void ilu0_apply_device(const Ilu0 &P, const double *d_r, double *d_z, hipStream_t st, const int *done = nullptr);
+void bicgstab_device(const GMatrix &A, const Ilu0 *pre, int use_pre, const double *d_b, const double *d_x0,
+                     double tol, int max_it, double btol, double *d_x, hipStream_t st, mg_pcg_result &res,
+                     std::unique_ptr<Ilu0> &owned);
+
 constexpr int kRedBlocks = 512;  // fixed grid of the deterministic vector reductions
 
 // Device scalars of a (flexible) PCG run; kernels of an iteration turn into no-ops once done.

@@ -59,6 +59,16 @@ class _PcgRes(C.Structure):
                 ("relative_residual", C.c_double)]
 
 
+class _GCsr(C.Structure):
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int), ("rowptr", C.POINTER(C.c_int)),
+                ("colidx", C.POINTER(C.c_int)), ("values", C.POINTER(C.c_double)), ("diag", C.POINTER(C.c_int))]
+
+
+class _Ilu0(C.Structure):
+    _fields_ = [("a", C.POINTER(_GCsr)), ("pivot_tolerance", C.c_double), ("factors", C.POINTER(C.c_double)),
+                ("diag", C.POINTER(C.c_int))]
+
+
 class OracleError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"{what}: oracle status {code}")
@@ -96,6 +106,19 @@ def lib():
         L.or_mpnn_apply.argtypes = [C.POINTER(_Csr), _f64p, _f64p, _f64p]
         L.or_mpnn_loss_grad.argtypes = [C.POINTER(_Csr), _f64p, C.c_int, _f64p, _f64p, C.c_double,
                                         C.POINTER(C.c_double), _f64p]
+        L.or_gcsr_from_coo.argtypes = [C.c_int, C.c_int, C.c_int64, _i32p, _i32p, _f64p,
+                                       C.POINTER(C.POINTER(_GCsr))]
+        L.or_gcsr_create.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.POINTER(C.POINTER(_GCsr))]
+        L.or_gcsr_free.argtypes = [C.POINTER(_GCsr)]
+        L.or_gspmv.argtypes = [C.POINTER(_GCsr), _f64p, _f64p]
+        L.or_gcsr_linecross.argtypes = [C.POINTER(_GCsr), _f64p, C.c_double, C.c_int]
+        L.or_ilu0_create.argtypes = [C.POINTER(_GCsr), C.c_double, C.POINTER(C.POINTER(_Ilu0)),
+                                     C.POINTER(C.c_int)]
+        L.or_ilu0_apply.argtypes = [C.POINTER(_Ilu0), _f64p, _f64p]
+        L.or_ilu0_free.argtypes = [C.POINTER(_Ilu0)]
+        L.or_bicgstab.argtypes = [C.POINTER(_GCsr), C.POINTER(_Ilu0), C.c_int, _f64p, C.c_void_p, C.c_double,
+                                  C.c_int, C.c_double, _f64p, C.POINTER(_PcgRes), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -303,3 +326,118 @@ def mpnn_loss_grad(csr: Csr, params, R, E, kappa=0.0):
     _check(lib().or_mpnn_loss_grad(csr._p, params, R.shape[0], R, E, float(kappa), C.byref(loss), grad),
            "mpnn_loss_grad")
     return loss.value, grad
+
+
+# ---- general CSR, ILU(0), BiCGSTAB (CSRMatrix.cs, BiCGSTAB/*.cs) ----
+class GCsr:
+    """Owned oracle CSRMatrix (general, rows x cols)."""
+
+    def __init__(self, ptr):
+        self._p = ptr
+
+    @classmethod
+    def from_coo(cls, rows, cols, ri, cj, vals):
+        ri = np.ascontiguousarray(ri, dtype=np.int32)
+        cj = np.ascontiguousarray(cj, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        p = C.POINTER(_GCsr)()
+        _check(lib().or_gcsr_from_coo(rows, cols, len(ri), ri, cj, vals, C.byref(p)), "gcsr_from_coo")
+        return cls(p)
+
+    @classmethod
+    def from_dense(cls, A):
+        A = np.asarray(A, dtype=np.float64)
+        ri, cj = np.nonzero(A)
+        return cls.from_coo(A.shape[0], A.shape[1], ri, cj, A[ri, cj])
+
+    @classmethod
+    def from_csr(cls, rows, cols, rowptr, colidx, values):
+        p = C.POINTER(_GCsr)()
+        _check(lib().or_gcsr_create(rows, cols, np.ascontiguousarray(rowptr, np.int32),
+                                    np.ascontiguousarray(colidx, np.int32),
+                                    np.ascontiguousarray(values, np.float64), C.byref(p)), "gcsr_create")
+        return cls(p)
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_gcsr_free(self._p)
+            self._p = None
+
+    @property
+    def shape(self):
+        return self._p.contents.rows, self._p.contents.cols
+
+    def arrays(self):
+        a = self._p.contents
+        return (_arr(a.rowptr, a.rows + 1, np.int32), _arr(a.colidx, a.nnz, np.int32),
+                _arr(a.values, a.nnz, np.float64))
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self._p.contents.rows)
+        lib().or_gspmv(self._p, x, y)
+        return y
+
+    def linecross(self, rhs, prescribed, index):
+        _check(lib().or_gcsr_linecross(self._p, rhs, prescribed, index), "gcsr_linecross")
+
+    def dense(self):
+        rp, ci, v = self.arrays()
+        r, c = self.shape
+        A = np.zeros((r, c))
+        for i in range(r):
+            A[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+        return A
+
+
+class Ilu0:
+    """Oracle Ilu0Preconditioner; raises OracleError (with .row) on a missing diagonal / bad pivot."""
+
+    def __init__(self, a: GCsr, pivot_tolerance=1e-30):
+        self.a = a
+        p = C.POINTER(_Ilu0)()
+        bad = C.c_int(-1)
+        rc = lib().or_ilu0_create(a._p, pivot_tolerance, C.byref(p), C.byref(bad))
+        if rc != 0:
+            e = OracleError(rc, "ilu0")
+            e.row = bad.value
+            raise e
+        self._p = p
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.or_ilu0_free(self._p)
+            self._p = None
+
+    @property
+    def factors(self):
+        return _arr(self._p.contents.factors, self.a._p.contents.nnz, np.float64)
+
+    def apply(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        lib().or_ilu0_apply(self._p, b, x)
+        return x
+
+
+def bicgstab(a: GCsr, b, x0=None, tol=1e-8, max_iterations=0, use_preconditioner=True,
+             breakdown_tolerance=1e-30, preconditioner: Ilu0 | None = None):
+    """BiCgStabSolver.SolveWithInitialGuess: (x, result dict); raises OracleError with .fail_code and
+    .fail_it on a breakdown (1 rho, 2 omega, 3 (r0,Ap), 4 (t,t), 5 omega; 6/7 ILU(0) diagonal/pivot)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    res = _PcgRes()
+    code, it = C.c_int(0), C.c_int(0)
+    x0p = None
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        x0p = x0.ctypes.data_as(C.c_void_p)
+    pre = preconditioner._p if preconditioner is not None else None
+    rc = lib().or_bicgstab(a._p, pre, 1 if use_preconditioner else 0, b, x0p, tol, max_iterations,
+                           breakdown_tolerance, x, C.byref(res), C.byref(code), C.byref(it))
+    if rc != 0:
+        e = OracleError(rc, "bicgstab")
+        e.fail_code, e.fail_it = code.value, it.value
+        raise e
+    return x, dict(iterations=res.iterations, converged=bool(res.converged), residual_norm=res.residual_norm,
+                   relative_residual=res.relative_residual)

@@ -1683,3 +1683,321 @@ int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const
     mp_graph_free(&g);
     return OR_OK;
 }
+
+
+/* ======================================================================
+ * CSRMatrix, Ilu0Preconditioner, BiCgStabSolver
+ * ====================================================================== */
+
+/* CSRMatrix.cs:33-75 -- group by (row, col) and sum in input order (:38-40), drop
+ * |sum| <= 1e-15 (:41), columns ascending per row (:43), diagonal table (:65-66).
+ * Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-76). */
+int or_gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
+                     or_gcsr **out)
+{
+    if (!out || (count > 0 && (!ri || !cj || !vals))) return OR_E_NULL;
+    if (rows < 0 || cols < 0 || count < 0) return OR_E_ARG;
+    for (int64_t k = 0; k < count; k++)
+        if (ri[k] < 0 || ri[k] >= rows || cj[k] < 0 || cj[k] >= cols) return OR_E_RANGE;
+    coo_item *items = (coo_item *)xmalloc((size_t)count * sizeof(coo_item));
+    for (int64_t k = 0; k < count; k++) {
+        items[k].row = ri[k]; items[k].col = cj[k]; items[k].order = k; items[k].value = vals[k];
+    }
+    qsort(items, (size_t)count, sizeof(coo_item), coo_cmp);
+    int64_t kept = 0;
+    for (int64_t k = 0; k < count;) {
+        int64_t e = k;
+        double sum = 0.0;
+        while (e < count && items[e].row == items[k].row && items[e].col == items[k].col) {
+            sum += items[e].value;
+            e++;
+        }
+        if (fabs(sum) > 1e-15) {
+            items[kept] = items[k];
+            items[kept].value = sum;
+            kept++;
+        }
+        k = e;
+    }
+    or_gcsr *a = (or_gcsr *)xcalloc(1, sizeof(or_gcsr));
+    a->rows = rows; a->cols = cols; a->nnz = (int)kept;
+    a->rowptr = (int *)xcalloc((size_t)rows + 1, sizeof(int));
+    a->colidx = (int *)xmalloc((size_t)kept * sizeof(int));
+    a->values = (double *)xmalloc((size_t)kept * sizeof(double));
+    a->diag = (int *)xmalloc((size_t)rows * sizeof(int));
+    for (int i = 0; i < rows; i++) a->diag[i] = EMPTY;
+    for (int64_t k = 0; k < kept; k++) {
+        a->rowptr[items[k].row + 1]++;
+        a->colidx[k] = items[k].col;
+        a->values[k] = items[k].value;
+        if (items[k].row == items[k].col) a->diag[items[k].row] = (int)k;
+    }
+    for (int i = 0; i < rows; i++) a->rowptr[i + 1] += a->rowptr[i];
+    free(items);
+    *out = a;
+    return OR_OK;
+}
+
+int or_gcsr_create(int rows, int cols, const int *rowptr, const int *colidx, const double *values, or_gcsr **out)
+{
+    if (!out || !rowptr) return OR_E_NULL;
+    int nnz = rowptr[rows];
+    if (nnz > 0 && (!colidx || !values)) return OR_E_NULL;
+    or_gcsr *a = (or_gcsr *)xcalloc(1, sizeof(or_gcsr));
+    a->rows = rows; a->cols = cols; a->nnz = nnz;
+    a->rowptr = (int *)xmalloc(((size_t)rows + 1) * sizeof(int));
+    a->colidx = (int *)xmalloc((size_t)nnz * sizeof(int));
+    a->values = (double *)xmalloc((size_t)nnz * sizeof(double));
+    a->diag = (int *)xmalloc((size_t)rows * sizeof(int));
+    memcpy(a->rowptr, rowptr, ((size_t)rows + 1) * sizeof(int));
+    if (nnz) {
+        memcpy(a->colidx, colidx, (size_t)nnz * sizeof(int));
+        memcpy(a->values, values, (size_t)nnz * sizeof(double));
+    }
+    for (int i = 0; i < rows; i++) {
+        a->diag[i] = EMPTY;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++)
+            if (colidx[k] == i) a->diag[i] = k;
+    }
+    *out = a;
+    return OR_OK;
+}
+
+void or_gcsr_free(or_gcsr *a)
+{
+    if (!a) return;
+    free(a->rowptr); free(a->colidx); free(a->values); free(a->diag); free(a);
+}
+
+/* CSRMatrix.Multiply (CSRMatrix.cs:244-262), also BiCgStabSolver.Multiply (:173-187) */
+void or_gspmv(const or_gcsr *a, const double *x, double *y)
+{
+    for (int i = 0; i < a->rows; i++) {
+        double sum = 0.0;
+        for (int j = a->rowptr[i]; j < a->rowptr[i + 1]; j++) sum += a->values[j] * x[a->colidx[j]];
+        y[i] = sum;
+    }
+}
+
+/* CSRMatrix.LineCross (CSRMatrix.cs:103-164) */
+int or_gcsr_linecross(or_gcsr *a, double *rhs, double prescribed, int index)
+{
+    if (!a || !rhs) return OR_E_NULL;
+    if (a->rows != a->cols) return OR_E_NOT_PD;
+    if (index < 0 || index >= a->rows) return OR_E_RANGE;
+    int di = a->diag[index];
+    if (di < 0) return OR_E_NOT_PD;
+    for (int row = 0; row < a->rows; row++)
+        for (int p = a->rowptr[row]; p < a->rowptr[row + 1]; p++) {
+            int col = a->colidx[p];
+            if (row == index) {
+                if (col != index) a->values[p] = 0.0;
+            } else if (col == index) {
+                rhs[row] -= a->values[p] * prescribed;
+                a->values[p] = 0.0;
+            }
+        }
+    a->values[di] = 1.0;
+    rhs[index] = prescribed;
+    return OR_OK;
+}
+
+/* Ilu0Preconditioner.FindPosition (:146-162): binary search of (row, col) */
+static int ilu_find(const or_gcsr *a, int row, int col)
+{
+    int left = a->rowptr[row], right = a->rowptr[row + 1] - 1;
+    while (left <= right) {
+        int middle = left + (right - left) / 2;
+        int c = a->colidx[middle];
+        if (c == col) return middle;
+        if (c < col) left = middle + 1;
+        else right = middle - 1;
+    }
+    return -1;
+}
+
+static int ilu_bad_pivot(double pivot, double tol) { return !isfinite(pivot) || fabs(pivot) < tol; }
+
+/* Ilu0Preconditioner(CSRMatrix, double) (:14-31): FindDiagonalIndices (:92-116) takes the
+ * first position of column i in row i; Factorize (:118-144) eliminates row by row. */
+int or_ilu0_create(const or_gcsr *a, double pivot_tolerance, or_ilu0 **out, int *bad_row)
+{
+    if (!a || !out) return OR_E_NULL;
+    if (bad_row) *bad_row = -1;
+    if (a->rows != a->cols) return OR_E_ARG;
+    if (!isfinite(pivot_tolerance) || pivot_tolerance <= 0.0) return OR_E_RANGE;
+    int n = a->rows;
+    int *dg = (int *)xmalloc((size_t)n * sizeof(int));
+    for (int row = 0; row < n; row++) {
+        dg[row] = -1;
+        for (int p = a->rowptr[row]; p < a->rowptr[row + 1]; p++)
+            if (a->colidx[p] == row) { dg[row] = p; break; }
+        if (dg[row] < 0) {
+            if (bad_row) *bad_row = row;
+            free(dg);
+            return OR_E_NOT_PD;
+        }
+    }
+    double *f = (double *)xmalloc((size_t)a->nnz * sizeof(double));
+    if (a->nnz) memcpy(f, a->values, (size_t)a->nnz * sizeof(double));
+    for (int row = 0; row < n; row++) {
+        for (int p = a->rowptr[row]; p < dg[row]; p++) {
+            int column = a->colidx[p];
+            double pivot = f[dg[column]];
+            if (ilu_bad_pivot(pivot, pivot_tolerance)) {
+                if (bad_row) *bad_row = column;
+                free(dg); free(f);
+                return OR_E_NOT_PD;
+            }
+            double multiplier = f[p] / pivot;
+            f[p] = multiplier;
+            for (int u = dg[column] + 1; u < a->rowptr[column + 1]; u++) {
+                int t = ilu_find(a, row, a->colidx[u]);
+                if (t >= 0) f[t] -= multiplier * f[u];
+            }
+        }
+        if (ilu_bad_pivot(f[dg[row]], pivot_tolerance)) {
+            if (bad_row) *bad_row = row;
+            free(dg); free(f);
+            return OR_E_NOT_PD;
+        }
+    }
+    or_ilu0 *pc = (or_ilu0 *)xcalloc(1, sizeof(or_ilu0));
+    pc->a = a; pc->pivot_tolerance = pivot_tolerance; pc->factors = f; pc->diag = dg;
+    *out = pc;
+    return OR_OK;
+}
+
+/* Ilu0Preconditioner.Apply (:43-90) */
+void or_ilu0_apply(const or_ilu0 *pc, const double *b, double *x)
+{
+    const or_gcsr *a = pc->a;
+    int n = a->rows;
+    double *w = (double *)xmalloc((size_t)n * sizeof(double));
+    for (int row = 0; row < n; row++) {
+        double sum = b[row];
+        for (int p = a->rowptr[row]; p < pc->diag[row]; p++) sum -= pc->factors[p] * w[a->colidx[p]];
+        w[row] = sum;
+    }
+    for (int row = n - 1; row >= 0; row--) {
+        double sum = w[row];
+        for (int p = pc->diag[row] + 1; p < a->rowptr[row + 1]; p++) sum -= pc->factors[p] * x[a->colidx[p]];
+        x[row] = sum / pc->factors[pc->diag[row]];
+    }
+    free(w);
+}
+
+void or_ilu0_free(or_ilu0 *pc)
+{
+    if (!pc) return;
+    free(pc->factors); free(pc->diag); free(pc);
+}
+
+static double bg_dot(int n, const double *l, const double *r)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += l[i] * r[i];
+    return s;
+}
+
+static int bg_breakdown(double v, double tol) { return !isfinite(v) || fabs(v) < tol; }
+
+/* BiCgStabSolver.SolveWithInitialGuess (BiCgStabSolver.cs:40-143) */
+int or_bicgstab(const or_gcsr *a, const or_ilu0 *pre, int use_pre, const double *b, const double *x0, double tol,
+                int max_iterations, double btol, double *x, or_pcg_result *res, int *fail_code, int *fail_it)
+{
+    if (fail_code) *fail_code = 0;
+    if (fail_it) *fail_it = 0;
+    if (!a || !b || !x || !res) return OR_E_NULL;
+    if (!isfinite(tol) || tol < 0.0) return OR_E_RANGE;
+    if (max_iterations < 0) return OR_E_RANGE;
+    if (!isfinite(btol) || btol <= 0.0) return OR_E_RANGE;
+    if (a->rows != a->cols) return OR_E_ARG;
+    if (pre && pre->a != a) return OR_E_ARG;
+    int n = a->rows;
+    for (int i = 0; i < n; i++) x[i] = x0 ? x0[i] : 0.0;
+    double bn = or_norm2(n, b);
+    double scale = bn > 0.0 ? bn : 1.0;
+    res->iterations = 0;
+    res->residual_norm = 0.0;
+    res->relative_residual = 0.0;
+    res->converged = 1;
+    if (n == 0) return OR_OK;
+    double *r = (double *)xmalloc((size_t)n * sizeof(double));
+    double *rh = (double *)xmalloc((size_t)n * sizeof(double));
+    double *p = (double *)xmalloc((size_t)n * sizeof(double));
+    double *ph = (double *)xmalloc((size_t)n * sizeof(double));
+    double *v = (double *)xmalloc((size_t)n * sizeof(double));
+    double *s = (double *)xmalloc((size_t)n * sizeof(double));
+    double *sh = (double *)xmalloc((size_t)n * sizeof(double));
+    double *t = (double *)xmalloc((size_t)n * sizeof(double));
+    or_ilu0 *owned = NULL;
+    int rc = OR_OK, iters = 0;
+    or_gspmv(a, x, v);
+    for (int i = 0; i < n; i++) r[i] = b[i] - v[i];
+    memcpy(rh, r, (size_t)n * sizeof(double));
+    double thr = tol * scale;
+    double rn = or_norm2(n, r);
+    if (rn <= thr) goto done;
+    const or_ilu0 *M = pre;
+    if (!M && use_pre) {
+        int bad = -1;
+        rc = or_ilu0_create(a, btol, &owned, &bad);
+        if (rc != OR_OK) {
+            if (fail_code) *fail_code = bad >= 0 && a->diag[bad] < 0 ? 6 : 7;
+            if (fail_it) *fail_it = bad;
+            goto done;
+        }
+        M = owned;
+    }
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    int maxit = max_iterations > 0 ? max_iterations : n;
+    iters = maxit;
+    for (int it = 1; it <= maxit; it++) {
+        double rho = bg_dot(n, rh, r);
+        if (bg_breakdown(rho, btol)) { rc = OR_E_NOT_PD; if (fail_code) *fail_code = 1; if (fail_it) *fail_it = it; goto done; }
+        if (it == 1) {
+            memcpy(p, r, (size_t)n * sizeof(double));
+        } else {
+            if (bg_breakdown(omega, btol)) { rc = OR_E_NOT_PD; if (fail_code) *fail_code = 2; if (fail_it) *fail_it = it; goto done; }
+            double beta = rho / rho_prev * (alpha / omega);
+            for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        if (M) or_ilu0_apply(M, p, ph); else memcpy(ph, p, (size_t)n * sizeof(double));
+        or_gspmv(a, ph, v);
+        double den = bg_dot(n, rh, v);
+        if (bg_breakdown(den, btol)) { rc = OR_E_NOT_PD; if (fail_code) *fail_code = 3; if (fail_it) *fail_it = it; goto done; }
+        alpha = rho / den;
+        for (int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+        double sn = or_norm2(n, s);
+        if (sn <= thr) {
+            for (int i = 0; i < n; i++) x[i] += alpha * ph[i];
+            rn = sn;
+            iters = it;
+            goto done;
+        }
+        if (M) or_ilu0_apply(M, s, sh); else memcpy(sh, s, (size_t)n * sizeof(double));
+        or_gspmv(a, sh, t);
+        double tt = bg_dot(n, t, t);
+        if (bg_breakdown(tt, btol)) { rc = OR_E_NOT_PD; if (fail_code) *fail_code = 4; if (fail_it) *fail_it = it; goto done; }
+        omega = bg_dot(n, t, s) / tt;
+        if (bg_breakdown(omega, btol)) { rc = OR_E_NOT_PD; if (fail_code) *fail_code = 5; if (fail_it) *fail_it = it; goto done; }
+        for (int i = 0; i < n; i++) {
+            x[i] += alpha * ph[i] + omega * sh[i];
+            r[i] = s[i] - omega * t[i];
+        }
+        rn = or_norm2(n, r);
+        if (rn <= thr) { iters = it; goto done; }
+        rho_prev = rho;
+    }
+done:
+    if (rc == OR_OK) {
+        res->iterations = iters;
+        res->residual_norm = rn;
+        res->relative_residual = rn / scale;
+        res->converged = res->relative_residual <= tol;
+    }
+    or_ilu0_free(owned);
+    free(r); free(rh); free(p); free(ph); free(v); free(s); free(sh); free(t);
+    return rc;
+}

@@ -161,6 +161,41 @@ int or_mpnn_apply(const or_csr *a, const double *params, const double *r, double
 int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const double *R, const double *E,
                       double kappa, double *loss, double *grad);
 
+/* ---- general CSR, ILU(0), BiCGSTAB (CSRMatrix.cs, BiCGSTAB/BiCgStabSolver.cs,
+ *      BiCGSTAB/Ilu0Preconditioner.cs) ---- */
+typedef struct {
+    int rows, cols, nnz;
+    int *rowptr, *colidx;
+    double *values;
+    int *diag;     /* rows, position of A[i,i] or -1 */
+} or_gcsr;
+/* CSRMatrix(int, int, IEnumerable<MatrixEntry>): duplicates summed in input order, |sum| <= 1e-15
+ * dropped, columns ascending */
+int or_gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
+                     or_gcsr **out);
+int or_gcsr_create(int rows, int cols, const int *rowptr, const int *colidx, const double *values, or_gcsr **out);
+void or_gcsr_free(or_gcsr *a);
+/* CSRMatrix.Multiply */
+void or_gspmv(const or_gcsr *a, const double *x, double *y);
+/* CSRMatrix.LineCross */
+int or_gcsr_linecross(or_gcsr *a, double *rhs, double prescribed, int index);
+typedef struct {
+    const or_gcsr *a;
+    double pivot_tolerance;
+    double *factors; /* nnz, L (unit, strictly lower) and U in A's pattern */
+    int *diag;
+} or_ilu0;
+/* Ilu0Preconditioner(CSRMatrix, pivotTolerance); *bad_row receives the failing row (or -1) */
+int or_ilu0_create(const or_gcsr *a, double pivot_tolerance, or_ilu0 **out, int *bad_row);
+void or_ilu0_apply(const or_ilu0 *p, const double *b, double *x);
+void or_ilu0_free(or_ilu0 *p);
+/* BiCgStabSolver.SolveWithInitialGuess; pre may be NULL (use_pre builds ILU(0) with pivot tolerance
+ * btol); a breakdown returns OR_E_NOT_PD with *fail_code 1 rho, 2 omega (before beta),
+ * 3 (r0, A M^-1 p), 4 (A M^-1 s, A M^-1 s), 5 omega, and *fail_it its iteration; an ILU(0)
+ * failure returns OR_E_NOT_PD with *fail_code 6 (missing diagonal) or 7 (pivot). */
+int or_bicgstab(const or_gcsr *a, const or_ilu0 *pre, int use_pre, const double *b, const double *x0, double tol,
+                int max_iterations, double btol, double *x, or_pcg_result *res, int *fail_code, int *fail_it);
+
 #ifdef __cplusplus
 }
 #endif

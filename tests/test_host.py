@@ -183,3 +183,44 @@ def test_batch_layout_and_costs_are_sums_of_systems():
     assert info["max_front_size"] == max(e["max_front_size"] for e in each)
     np.testing.assert_array_equal(batch.row_begin, np.concatenate([[0], np.cumsum([m.size for m in mats])]))
     np.testing.assert_array_equal(batch.nnz_begin, np.concatenate([[0], np.cumsum([m.nonzero_count for m in mats])]))
+
+
+# ---- general CSR assembly (host code of the library, no GPU) ----
+
+def test_general_coo_semantics_equal_oracle_randomized():
+    g = np.random.default_rng(17)
+    for trial in range(20):
+        rows, cols = int(g.integers(1, 30)), int(g.integers(1, 30))
+        k = int(g.integers(0, 200))
+        ri = g.integers(0, rows, k)
+        cj = g.integers(0, cols, k)
+        v = g.choice([1.0, -1.0, 0.5, 1e-16, 2.0], k)
+        a = M.CSRMatrix.from_coo(rows, cols, ri, cj, v)
+        o = O.GCsr.from_coo(rows, cols, ri, cj, v)
+        rp, ci, vv = o.arrays()
+        np.testing.assert_array_equal(a.row_pointers, rp)
+        np.testing.assert_array_equal(a.column_indices, ci)
+        np.testing.assert_array_equal(a.values, vv)
+
+
+def test_general_coo_out_of_range_raises():
+    with pytest.raises(M.ArgumentOutOfRangeError):
+        M.CSRMatrix.from_coo(2, 3, [0, 2], [0, 0], [1.0, 1.0])
+
+
+def test_general_line_cross_equals_oracle():
+    g = np.random.default_rng(5)
+    A = (g.random((12, 12)) < 0.4) * g.uniform(-1, 1, (12, 12))
+    np.fill_diagonal(A, 3.0)
+    a = M.CSRMatrix.from_dense(A)
+    o = O.GCsr.from_dense(A)
+    r1 = g.uniform(-1, 1, 12)
+    r2 = r1.copy()
+    for idx, pv in [(3, 1.5), (7, -2.0), (3, 0.25)]:
+        a.line_cross(r1, pv, idx)
+        o.linecross(r2, pv, idx)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(a.values, o.arrays()[2])
+    b = M.CSRMatrix.from_dense(np.array([[1.0, 2.0], [3.0, 0.0]]))
+    with pytest.raises(M.InvalidOperationError):
+        b.line_cross(np.zeros(2), 1.0, 1)  # structurally zero diagonal
