@@ -230,3 +230,24 @@ def config_c4(count=1024, seed=SEED_BASE + 4, side_lo=16, side_hi=128, max_rhs=3
         b = 2.0 * g.random((k, n)) - 1.0
         out.append((n, r, c, v, np.asfortranarray(b.T)))
     return out
+
+
+def convection_diffusion_2d(m, peclet=2.0, seed=SEED_BASE + 6):
+    """Nonsymmetric test system for BiCGSTAB: upwind 5-point convection-diffusion on an m x m
+    grid in natural order (i = y*m + x), Dirichlet neighbours dropped; per node the upwind
+    weights wx = peclet*(1 + U/2), wy = peclet*(1/2 + U) with U ~ U[0,1); diagonal 4 + wx + wy,
+    west -1 - wx, south -1 - wy, east and north -1 (an M-matrix, diagonally dominant).
+    Returns (n, rows, cols, vals) with duplicate-free COO entries."""
+    g = rng(seed)
+    n = m * m
+    wx = peclet * (1.0 + 0.5 * g.random(n))
+    wy = peclet * (0.5 + g.random(n))
+    idx = np.arange(n).reshape(m, m)
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [4.0 + wx + wy]
+    for dy, dx, w in ((0, -1, -1.0 - wx), (0, 1, -np.ones(n)), (-1, 0, -1.0 - wy), (1, 0, -np.ones(n))):
+        src = idx[max(0, -dy):m - max(0, dy), max(0, -dx):m - max(0, dx)].ravel()
+        dst = idx[max(0, dy):m - max(0, -dy), max(0, dx):m - max(0, -dx)].ravel()
+        rows.append(src)
+        cols.append(dst)
+        vals.append(w[src])
+    return n, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
