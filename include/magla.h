@@ -187,12 +187,12 @@ int mg_pcg_solve_dev(mg_matrix *m, const double *d_b, const double *d_x0, double
                      mg_pcg_result *result);
 
 /* ---- general square CSR, ILU(0) and BiCGSTAB (BiCgStabSolver.cs, Ilu0Preconditioner.cs, CSRMatrix.cs) ---- */
-/* CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:33-75): duplicates of one (row, col)
+/* CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:34-80): duplicates of one (row, col)
  * summed in input order, |sum| <= 1e-15 dropped, columns ascending; indices checked as in
- * CSRMatrixBuilder.AddToElement (CSRMatrixBuilder.cs:72-80).  Two-call protocol as mg_csr_from_coo. */
+ * CSRMatrixBuilder.AddToElement (CSRMatrixBuilder.cs:74-81).  Two-call protocol as mg_csr_from_coo. */
 int mg_gcsr_from_coo(int32_t rows, int32_t cols, int64_t count, const int32_t *row_idx, const int32_t *col_idx,
                      const double *vals, int32_t *nnz_out, int32_t *rowptr, int32_t *colidx, double *values);
-/* CSRMatrix.LineCross (CSRMatrix.cs:103-164) for each of `count` equations in order; values and rhs in place. */
+/* CSRMatrix.LineCross (CSRMatrix.cs:110-164) for each of `count` equations in order; values and rhs in place. */
 int mg_gcsr_line_cross(int32_t rows, int32_t cols, const int32_t *rowptr, const int32_t *colidx, double *values,
                        double *rhs, int32_t count, const int32_t *indices, const double *prescribed);
 typedef struct mg_gmatrix mg_gmatrix;
@@ -200,20 +200,20 @@ typedef struct mg_gmatrix mg_gmatrix;
 int mg_gmatrix_create(int32_t rows, int32_t cols, const int32_t *rowptr, const int32_t *colidx,
                       const double *values, mg_gmatrix **out);
 void mg_gmatrix_release(mg_gmatrix *m);
-/* CSRMatrix.Multiply (CSRMatrix.cs:244-262): y = A x, per-row sums in position order */
+/* CSRMatrix.Multiply (CSRMatrix.cs:230-248): y = A x, per-row sums in position order */
 int mg_gspmv(mg_gmatrix *m, const double *x, double *y);
 int mg_gspmv_dev(mg_gmatrix *m, const double *d_x, double *d_y, void *stream);
 typedef struct mg_ilu0 mg_ilu0;
-/* Ilu0Preconditioner(CSRMatrix, pivotTolerance) (BiCGSTAB/Ilu0Preconditioner.cs:14-31,118-168):
+/* Ilu0Preconditioner(CSRMatrix, pivotTolerance) (BiCGSTAB/Ilu0Preconditioner.cs:14-31,86-166):
  * ILU(0) in the matrix's own pattern, level-scheduled on the GPU with the reference's order of
  * operations.  MG_E_ARG for a non-square matrix, MG_E_RANGE for a bad tolerance, MG_E_NOT_PD for a
  * missing diagonal or a zero/invalid pivot (the message names the first such row). */
 int mg_ilu0_create(mg_gmatrix *m, double pivot_tolerance, mg_ilu0 **out);
 void mg_ilu0_release(mg_ilu0 *p);
-/* Ilu0Preconditioner.Apply (:43-90): x = (LU)^-1 b */
+/* Ilu0Preconditioner.Apply (:44-84): x = (LU)^-1 b */
 int mg_ilu0_apply(mg_ilu0 *p, const double *b, double *x);
 int mg_ilu0_apply_dev(mg_ilu0 *p, const double *d_b, double *d_x, void *stream);
-/* BiCgStabSolver.SolveWithInitialGuess (BiCGSTAB/BiCgStabSolver.cs:40-143): x0 may be NULL; pre may be
+/* BiCgStabSolver.SolveWithInitialGuess (BiCGSTAB/BiCgStabSolver.cs:37-148): x0 may be NULL; pre may be
  * NULL (then use_preconditioner builds ILU(0) with pivot tolerance = breakdown_tolerance once the initial
  * residual has not passed); max_iterations = 0 means n.  Breakdowns return MG_E_NOT_PD with the
  * reference's message; x receives the last iterate. */

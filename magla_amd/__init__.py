@@ -820,7 +820,7 @@ class CSRMatrix:
 
     @classmethod
     def from_coo(cls, rows, cols, row_idx, col_idx, vals):
-        """CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:33-75)."""
+        """CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:34-80)."""
         ri = np.ascontiguousarray(row_idx, dtype=np.int32)
         cj = np.ascontiguousarray(col_idx, dtype=np.int32)
         v = np.ascontiguousarray(vals, dtype=np.float64)
@@ -851,7 +851,7 @@ class CSRMatrix:
         return self._h
 
     def multiply(self, x):
-        """CSRMatrix.Multiply (CSRMatrix.cs:244-262), on the GPU."""
+        """CSRMatrix.Multiply (CSRMatrix.cs:230-248), on the GPU."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape[0] != self.column_count:
             raise ArgumentError(MG_E_ARG, f"Vector length {x.shape[0]} does not match the matrix column count "
@@ -861,7 +861,7 @@ class CSRMatrix:
         return y
 
     def line_cross(self, rhs, prescribed_value, index):
-        """CSRMatrix.LineCross (CSRMatrix.cs:103-164): rhs is updated in place."""
+        """CSRMatrix.LineCross (CSRMatrix.cs:110-164): rhs is updated in place."""
         if rhs is None:
             raise ArgumentNullError(MG_E_NULL, "rightHandSide")
         if len(rhs) != self.row_count:
@@ -897,7 +897,7 @@ class Ilu0Preconditioner:
         self._h = h
 
     def apply(self, right_hand_side):
-        """Apply (:36-90): (L U)^-1 right_hand_side."""
+        """Apply (:44-84): (L U)^-1 right_hand_side."""
         b = np.ascontiguousarray(right_hand_side, dtype=np.float64)
         if b.shape[0] != self.size:
             raise ArgumentError(MG_E_ARG, "The right-hand side length does not match the preconditioner size.")

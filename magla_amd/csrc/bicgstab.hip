@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_ilu0_factor(const int *__rest
     for (int L = 0; L < nlev; ++L) {
         for (int q = lptr[L] + (int)threadIdx.x; q < lptr[L + 1]; q += kLevelThreads) {
             const int row = lrows[q], end = rp[row + 1], dr = diag[row];
-            // Factorize (Ilu0Preconditioner.cs:122-144): for each lower position in order
+            // Factorize (Ilu0Preconditioner.cs:111-137): for each lower position in order
             for (int p = rp[row]; p < dr; ++p) {
                 const int col = ci[p];
                 const double mult = f[p] / f[diag[col]];
@@ -49,13 +49,13 @@ __global__ __launch_bounds__(kLevelThreads) void k_ilu0_factor(const int *__rest
                 }
             }
             const double d = f[dr];
-            if (!isfinite(d) || fabs(d) < tol) atomicMin(bad, row);  // EnsurePivot (:164-168)
+            if (!isfinite(d) || fabs(d) < tol) atomicMin(bad, row);  // EnsurePivot (:162-166)
         }
         __syncthreads();
     }
 }
 
-// w = L^{-1} r (Ilu0Preconditioner.cs:73-80)
+// w = L^{-1} r (Ilu0Preconditioner.cs:67-74)
 __global__ __launch_bounds__(kLevelThreads) void k_ilu0_lower(const int *__restrict__ rp, const int *__restrict__ ci,
                                                               const int *__restrict__ diag,
                                                               const double *__restrict__ f,
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_ilu0_lower(const int *__restr
     }
 }
 
-// x = U^{-1} w (Ilu0Preconditioner.cs:82-89)
+// x = U^{-1} w (Ilu0Preconditioner.cs:76-83)
 __global__ __launch_bounds__(kLevelThreads) void k_ilu0_upper(const int *__restrict__ rp, const int *__restrict__ ci,
                                                               const int *__restrict__ diag,
                                                               const double *__restrict__ f,
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_ilu0_upper(const int *__restr
     }
 }
 
-// y = A x, one thread per row in position order (BiCgStabSolver.cs:173-187, CSRMatrix.Multiply)
+// y = A x, one thread per row in position order (BiCgStabSolver.cs:167-180, CSRMatrix.cs:230-248)
 __global__ __launch_bounds__(256) void k_gspmv(const int *__restrict__ rp, const int *__restrict__ ci,
                                                const double *__restrict__ v, int rows, const double *__restrict__ x,
                                                double *__restrict__ y, const int *__restrict__ done) {
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_gspmv(const int *__restrict__ rp, const
     y[i] = sum;
 }
 
-// ---- BiCGSTAB (BiCgStabSolver.cs:40-136) ----
+// ---- BiCGSTAB (BiCgStabSolver.cs:37-148) ----
 
 struct BicgScalars {
     double rho, rho_prev, alpha, omega, beta, rnorm, snorm, thr, bnorm;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBgBS) void k_bg_dot(const double *__restrict__ a, 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// r = b - A x (:65-67)
+// r = b - A x (:69-73)
 __global__ void k_bg_residual0(const double *__restrict__ b, const double *__restrict__ ax, int n,
                                double *__restrict__ r, double *__restrict__ rhat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,7 +164,7 @@ __global__ void k_bg_residual0(const double *__restrict__ b, const double *__res
     }
 }
 
-// residualScale, threshold and the initial test (:56-75)
+// residualScale, threshold and the initial test (:54-79)
 __global__ void k_bg_init(const double *__restrict__ prr, const double *__restrict__ pbb, double tol,
                           BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
@@ -186,7 +186,7 @@ __global__ void k_bg_init(const double *__restrict__ prr, const double *__restri
     }
 }
 
-// rho, the breakdown tests and beta (:86-100)
+// rho, the breakdown tests and beta (:92-102)
 __global__ void k_bg_rho(const double *__restrict__ part, int it, double btol, BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
     if (S->done) return;
@@ -202,7 +202,7 @@ __global__ void k_bg_rho(const double *__restrict__ part, int it, double btol, B
     }
 }
 
-// p = r (first iteration) or r + beta (p - omega v) (:92-100)
+// p = r (first iteration) or r + beta (p - omega v) (:95-105)
 __global__ void k_bg_direction(const double *__restrict__ r, double *__restrict__ p, const double *__restrict__ v,
                                int n, int it, const BicgScalars *__restrict__ S) {
     MG_NO_CONTRACT
@@ -219,7 +219,7 @@ __global__ void k_bg_copy(const double *__restrict__ a, double *__restrict__ c, 
     if (i < n) c[i] = a[i];
 }
 
-// alpha = rho / (r0, A M^-1 p) (:105-108)
+// alpha = rho / (r0, A M^-1 p) (:110-112)
 __global__ void k_bg_alpha(const double *__restrict__ part, int it, double btol, BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
     if (S->done) return;
@@ -230,7 +230,7 @@ __global__ void k_bg_alpha(const double *__restrict__ part, int it, double btol,
     }
 }
 
-// s = r - alpha v and its squared norm (:110-113)
+// s = r - alpha v and its squared norm (:114-117)
 __global__ __launch_bounds__(kBgBS) void k_bg_s(const double *__restrict__ r, const double *__restrict__ v,
                                                 double *__restrict__ s, int n, double *__restrict__ part,
                                                 const BicgScalars *__restrict__ S) {
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBgBS) void k_bg_s(const double *__restrict__ r, co
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// ||s|| <= threshold: x += alpha M^-1 p and stop (:114-120)
+// ||s|| <= threshold: x += alpha M^-1 p and stop (:118-124)
 __global__ void k_bg_snorm(const double *__restrict__ part, BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
     if (S->done) return;
@@ -274,7 +274,7 @@ __global__ void k_bg_early_done(int it, BicgScalars *__restrict__ S) {
     S->rnorm = S->snorm;
 }
 
-// omega = (t, s) / (t, t) with its two breakdown tests (:125-129)
+// omega = (t, s) / (t, t) with its two breakdown tests (:129-132)
 __global__ void k_bg_omega(const double *__restrict__ ptt, const double *__restrict__ pts, int it, double btol,
                            BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
@@ -293,7 +293,7 @@ __global__ void k_bg_omega(const double *__restrict__ ptt, const double *__restr
     }
 }
 
-// x += alpha M^-1 p + omega M^-1 s; r = s - omega t; ||r||^2 (:131-135)
+// x += alpha M^-1 p + omega M^-1 s; r = s - omega t; ||r||^2 (:134-140)
 __global__ __launch_bounds__(kBgBS) void k_bg_update(double *__restrict__ x, double *__restrict__ r,
                                                      const double *__restrict__ ph, const double *__restrict__ sh,
                                                      const double *__restrict__ s, const double *__restrict__ t,
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kBgBS) void k_bg_update(double *__restrict__ x, dou
     if (threadIdx.x == 0) part[blockIdx.x] = tv;
 }
 
-// ||r|| <= threshold ends the iteration (:137-141)
+// ||r|| <= threshold ends the iteration (:140-144)
 __global__ void k_bg_rnorm(const double *__restrict__ part, int it, BicgScalars *__restrict__ S) {
     __shared__ double sm[kBgBS];
     if (S->done) return;
@@ -400,7 +400,7 @@ void ilu0_build(Ilu0 &P, const GMatrix &A, double pivot_tol, hipStream_t st) {
     P.n = n;
     P.pivot_tol = pivot_tol;
     std::vector<int> dg(n, -1);
-    for (int i = 0; i < n; ++i) {  // FindDiagonalIndices (:92-116)
+    for (int i = 0; i < n; ++i) {  // FindDiagonalIndices (:86-109)
         for (int p = A.h_rowptr[i]; p < A.h_rowptr[i + 1]; ++p)
             if (A.h_colidx[p] == i) {
                 dg[i] = p;
@@ -451,7 +451,7 @@ void ilu0_apply_device(const Ilu0 &P, const double *d_r, double *d_z, hipStream_
 void bicgstab_device(const GMatrix &A, const Ilu0 *pre, int use_pre, const double *d_b, const double *d_x0,
                      double tol, int max_it, double btol, double *d_x, hipStream_t st, mg_pcg_result &res,
                      std::unique_ptr<Ilu0> &owned) {
-    // validation order of SolveWithInitialGuess (BiCgStabSolver.cs:40-55)
+    // validation order of SolveWithInitialGuess (BiCgStabSolver.cs:39-51)
     if (!std::isfinite(tol) || tol < 0.0) throw Error(MG_E_RANGE, "RelativeTolerance is out of range.");
     if (max_it < 0) throw Error(MG_E_RANGE, "MaxIterations: the maximum iteration count cannot be negative.");
     if (!std::isfinite(btol) || btol <= 0.0) throw Error(MG_E_RANGE, "BreakdownTolerance is out of range.");
@@ -475,7 +475,7 @@ void bicgstab_device(const GMatrix &A, const Ilu0 *pre, int use_pre, const doubl
     MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
     if (!h.done) {
-        // the preconditioner is built only once the initial residual has not passed (:77-79)
+        // the preconditioner is built only once the initial residual has not passed (:78-83)
         const Ilu0 *M = pre;
         if (!M && use_pre) {
             owned.reset(new Ilu0());

@@ -119,9 +119,9 @@ void csr_line_cross(int n, const int *rowptr, const int *colidx, double *values,
     }
 }
 
-// CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:33-75): duplicates of one
+// CSRMatrix(int, int, IEnumerable<MatrixEntry>) (CSRMatrix.cs:34-80): duplicates of one
 // (row, col) summed in input order, |sum| <= 1e-15 dropped, columns ascending per row.
-// Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-76).
+// Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-81).
 void gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
                    int *nnz_out, int *rowptr, int *colidx, double *values) {
     for (int64_t k = 0; k < count; ++k)
@@ -167,7 +167,7 @@ void gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *
     if (values && !va.empty()) std::memcpy(values, va.data(), sizeof(double) * va.size());
 }
 
-// CSRMatrix.LineCross (CSRMatrix.cs:103-164) for each constrained equation in the given order:
+// CSRMatrix.LineCross (CSRMatrix.cs:110-164) for each constrained equation in the given order:
 // the row's off-diagonal values are zeroed, the column's values move to the right-hand side
 // (rhs[row] -= a * prescribed) and are zeroed, the diagonal becomes 1, rhs[index] = prescribed.
 void gcsr_line_cross(int rows, int cols, const int *rowptr, const int *colidx, double *values, double *rhs,

@@ -1689,9 +1689,9 @@ int or_mpnn_loss_grad(const or_csr *a, const double *params, int nsamples, const
  * CSRMatrix, Ilu0Preconditioner, BiCgStabSolver
  * ====================================================================== */
 
-/* CSRMatrix.cs:33-75 -- group by (row, col) and sum in input order (:38-40), drop
- * |sum| <= 1e-15 (:41), columns ascending per row (:43), diagonal table (:65-66).
- * Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-76). */
+/* CSRMatrix.cs:34-80 -- group by (row, col) and sum in input order (:39-41), drop
+ * |sum| <= 1e-15 (:42), columns ascending per row (:44), diagonal table (:65-66).
+ * Indices are checked as CSRMatrixBuilder.AddToElement does (CSRMatrixBuilder.cs:74-81). */
 int or_gcsr_from_coo(int rows, int cols, int64_t count, const int *ri, const int *cj, const double *vals,
                      or_gcsr **out)
 {
@@ -1769,7 +1769,7 @@ void or_gcsr_free(or_gcsr *a)
     free(a->rowptr); free(a->colidx); free(a->values); free(a->diag); free(a);
 }
 
-/* CSRMatrix.Multiply (CSRMatrix.cs:244-262), also BiCgStabSolver.Multiply (:173-187) */
+/* CSRMatrix.Multiply (CSRMatrix.cs:230-248), also BiCgStabSolver.Multiply (:167-180) */
 void or_gspmv(const or_gcsr *a, const double *x, double *y)
 {
     for (int i = 0; i < a->rows; i++) {
@@ -1779,7 +1779,7 @@ void or_gspmv(const or_gcsr *a, const double *x, double *y)
     }
 }
 
-/* CSRMatrix.LineCross (CSRMatrix.cs:103-164) */
+/* CSRMatrix.LineCross (CSRMatrix.cs:110-164) */
 int or_gcsr_linecross(or_gcsr *a, double *rhs, double prescribed, int index)
 {
     if (!a || !rhs) return OR_E_NULL;
@@ -1802,7 +1802,7 @@ int or_gcsr_linecross(or_gcsr *a, double *rhs, double prescribed, int index)
     return OR_OK;
 }
 
-/* Ilu0Preconditioner.FindPosition (:146-162): binary search of (row, col) */
+/* Ilu0Preconditioner.FindPosition (:139-160): binary search of (row, col) */
 static int ilu_find(const or_gcsr *a, int row, int col)
 {
     int left = a->rowptr[row], right = a->rowptr[row + 1] - 1;
@@ -1818,8 +1818,8 @@ static int ilu_find(const or_gcsr *a, int row, int col)
 
 static int ilu_bad_pivot(double pivot, double tol) { return !isfinite(pivot) || fabs(pivot) < tol; }
 
-/* Ilu0Preconditioner(CSRMatrix, double) (:14-31): FindDiagonalIndices (:92-116) takes the
- * first position of column i in row i; Factorize (:118-144) eliminates row by row. */
+/* Ilu0Preconditioner(CSRMatrix, double) (:14-31): FindDiagonalIndices (:86-109) takes the
+ * first position of column i in row i; Factorize (:111-137) eliminates row by row. */
 int or_ilu0_create(const or_gcsr *a, double pivot_tolerance, or_ilu0 **out, int *bad_row)
 {
     if (!a || !out) return OR_E_NULL;
@@ -1868,7 +1868,7 @@ int or_ilu0_create(const or_gcsr *a, double pivot_tolerance, or_ilu0 **out, int 
     return OR_OK;
 }
 
-/* Ilu0Preconditioner.Apply (:43-90) */
+/* Ilu0Preconditioner.Apply (:44-84) */
 void or_ilu0_apply(const or_ilu0 *pc, const double *b, double *x)
 {
     const or_gcsr *a = pc->a;
@@ -1902,7 +1902,7 @@ static double bg_dot(int n, const double *l, const double *r)
 
 static int bg_breakdown(double v, double tol) { return !isfinite(v) || fabs(v) < tol; }
 
-/* BiCgStabSolver.SolveWithInitialGuess (BiCgStabSolver.cs:40-143) */
+/* BiCgStabSolver.SolveWithInitialGuess (BiCgStabSolver.cs:37-148) */
 int or_bicgstab(const or_gcsr *a, const or_ilu0 *pre, int use_pre, const double *b, const double *x0, double tol,
                 int max_iterations, double btol, double *x, or_pcg_result *res, int *fail_code, int *fail_it)
 {
