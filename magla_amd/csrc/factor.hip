@@ -35,6 +35,7 @@ constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
 // Schur tile shape per front-size bucket (f < 128, < 256, < 512, < 1024, < 2048, >= 2048):
 // 64x64 (k_schur) or 128x128 (k_schur128) output tiles, chosen by autotune_device; 64 until tuned.
 constexpr int kSchurBuckets = MG_SCHUR_BUCKETS;
+constexpr int kUbStride = 2 * kSchurBuckets + 1;  // task-list offsets per (level, panel)
 static inline int schur_bucket(int f) {
     int b = 0;
     for (int lim = 128; b < kSchurBuckets - 1 && f >= lim; lim *= 2) ++b;
@@ -67,8 +68,9 @@ struct DevicePlan {
     std::vector<int64_t> lvl_asm_ptr;
     std::vector<int> lvl_npanels;
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_diag_ptr;
-    // Schur tasks per level: for panel q and bucket b, tasks [ub[q * (kB + 1) + b], ub[... + b + 1])
-    // of upd_tasks (64x64 tiles) and, with ub128, of upd128_tasks (128x128 tiles)
+    // Schur tasks per level: for panel q, part p (0: before the next panel, 1: may overlap it) and
+    // bucket b, tasks [ub[q * kUbStride + p * kB + b], ub[... + 1]) of upd_tasks (64x64 tiles) and,
+    // with ub128, of upd128_tasks (128x128 tiles)
     std::vector<std::vector<int>> lvl_ub64, lvl_ub128;
     int schur128[2][kSchurBuckets] = {};  // [precision is FP32][bucket]: 1 = 128x128 tiles
     int64_t bucket_tasks[kSchurBuckets] = {};
@@ -367,29 +369,37 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             P->max_panel_tasks = std::max(P->max_panel_tasks, (int)panel.size() - pbase);
             dp.push_back((int)diag_nodes.size());
             pp.push_back((int)panel.size());
-            // Schur tasks of this panel, grouped by front-size bucket, in both tile shapes
+            // Schur tasks of this panel, grouped by front-size bucket, in both tile shapes, in two
+            // parts: part 0 must precede the next panel of its front (inner steps, and the next
+            // 256-column block of an outer step); part 1 is the rest of an outer step's trailing
+            // update, which the next block's panels do not touch (run_factor may overlap it)
             for (int big = 0; big < 2; ++big) {
                 std::vector<int4> &dst = big ? upd128 : upd;
                 std::vector<int> &ub = big ? ub128 : ub64;
-                for (int b = 0; b < kSchurBuckets; ++b) {
-                    ub.push_back((int)dst.size());
-                    for (int g : bylevel[L]) {
-                        const int f = P->h_f[g], w = P->h_w[g];
-                        if (w <= q * kPanel || schur_bucket(f) != b) continue;
-                        const SchurStep st = schur_step(f, w, q);
-                        if (!st.any) continue;
-                        const int tile = big ? 128 : kTile;
-                        const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
-                        for (int tj = 0; tj < ct; ++tj)
-                            for (int ti = tj; ti < rt; ++ti) dst.push_back(make_int4(g, ti, tj, big));
-                        if (big) continue;
-                        P->bucket_tasks[b] += (int64_t)ct * (2 * rt - ct + 1) / 2;
-                        double elems = 0;
-                        for (int j = st.base; j < st.cend; ++j) elems += f - j;
-                        ufl[q] += 2.0 * (st.ke - st.kb) * elems;
-                        uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+                const int tile = big ? 128 : kTile, next_tiles = kBlock / tile;
+                for (int part = 0; part < 2; ++part)
+                    for (int b = 0; b < kSchurBuckets; ++b) {
+                        ub.push_back((int)dst.size());
+                        for (int g : bylevel[L]) {
+                            const int f = P->h_f[g], w = P->h_w[g];
+                            if (w <= q * kPanel || schur_bucket(f) != b) continue;
+                            const SchurStep st = schur_step(f, w, q);
+                            if (!st.any) continue;
+                            const bool has_next = !st.inner && st.be < w;
+                            const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
+                            for (int tj = 0; tj < ct; ++tj) {
+                                const bool first = st.inner || (has_next && tj < next_tiles);
+                                if (first != (part == 0)) continue;
+                                for (int ti = tj; ti < rt; ++ti) dst.push_back(make_int4(g, ti, tj, big));
+                                if (!big) P->bucket_tasks[b] += rt - tj;
+                            }
+                            if (big || part) continue;
+                            double elems = 0;
+                            for (int j = st.base; j < st.cend; ++j) elems += f - j;
+                            ufl[q] += 2.0 * (st.ke - st.kb) * elems;
+                            uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+                        }
                     }
-                }
                 ub.push_back((int)dst.size());
             }
         }
@@ -1723,9 +1733,21 @@ struct Numeric {
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
     hipStream_t cap_stream = nullptr;
     int graph_precision = -1;
+    hipStream_t side = nullptr;  // lookahead Schur updates (default priority)
+    hipEvent_t ev_fork = nullptr, ev_side[2] = {nullptr, nullptr};
+    void ensure_side() {
+        if (side) return;
+        MG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        MG_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        MG_HIP(hipEventCreateWithFlags(&ev_side[0], hipEventDisableTiming));
+        MG_HIP(hipEventCreateWithFlags(&ev_side[1], hipEventDisableTiming));
+    }
     ~Numeric() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        for (hipEvent_t ev : {ev_fork, ev_side[0], ev_side[1]})
+            if (ev) (void)hipEventDestroy(ev);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
@@ -1832,6 +1854,12 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
     DevicePlan &P = *N.plan;
     T *arena = reinterpret_cast<T *>(N.arena.p);
     const double e = sizeof(T);
+    // Lookahead: the part of an outer Schur step that the next 256-column block's panels do not
+    // touch runs on a second stream, overlapping those panels (off while launches are timed)
+    const bool look = !profile_on();
+    if (look) N.ensure_side();
+    hipEvent_t prev_side = nullptr;
+    int side_k = 0;
     for (int L = 0; L < P.nlevels; ++L) {
         const int np = P.lvl_pull_ptr[L + 1] - P.lvl_pull_ptr[L];
         if (np > 0) {
@@ -1861,27 +1889,52 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                 hipLaunchKernelGGL(k_panel_rows<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.diag.p);
             }
-            const int *b64 = &P.lvl_ub64[L][q * (kSchurBuckets + 1)];
-            const int *b128 = &P.lvl_ub128[L][q * (kSchurBuckets + 1)];
-            if (b64[kSchurBuckets] > b64[0]) {
-                ProfScope ps_(st, KP_SCHUR, P.lvl_upd_flops[L][q], e * P.lvl_upd_elems[L][q]);
-                // one launch per run of consecutive buckets with the same tile shape
+            const int *b64 = &P.lvl_ub64[L][q * kUbStride];
+            const int *b128 = &P.lvl_ub128[L][q * kUbStride];
+            // launches of one part: one per run of consecutive buckets with the same tile shape
+            auto schur_part = [&](int part, hipStream_t s) {
+                const int *u64 = b64 + part * kSchurBuckets, *u128 = b128 + part * kSchurBuckets;
                 const int *big = P.schur128[sizeof(T) == 4];
                 for (int b = 0; b < kSchurBuckets;) {
                     int c = b + 1;
                     while (c < kSchurBuckets && big[c] == big[b]) ++c;
                     if (big[b]) {
-                        if (b128[c] > b128[b])
-                            hipLaunchKernelGGL(k_schur128<T>, dim3(b128[c] - b128[b]), dim3(256), 0, st,
-                                               P.upd128_tasks.p + b128[b], arena, P.f.p, P.w.p, P.off.p, P.first.p, q,
+                        if (u128[c] > u128[b])
+                            hipLaunchKernelGGL(k_schur128<T>, dim3(u128[c] - u128[b]), dim3(256), 0, s,
+                                               P.upd128_tasks.p + u128[b], arena, P.f.p, P.w.p, P.off.p, P.first.p, q,
                                                N.diag.p);
-                    } else if (b64[c] > b64[b]) {
-                        hipLaunchKernelGGL(k_schur<T>, dim3(b64[c] - b64[b]), dim3(256), 0, st, P.upd_tasks.p + b64[b],
+                    } else if (u64[c] > u64[b]) {
+                        hipLaunchKernelGGL(k_schur<T>, dim3(u64[c] - u64[b]), dim3(256), 0, s, P.upd_tasks.p + u64[b],
                                            arena, P.f.p, P.w.p, P.off.p, P.first.p, q, N.diag.p);
                     }
                     b = c;
                 }
+            };
+            const bool has1 = b64[2 * kSchurBuckets] > b64[kSchurBuckets];
+            if (b64[2 * kSchurBuckets] > b64[0]) {
+                ProfScope ps_(st, KP_SCHUR, P.lvl_upd_flops[L][q], e * P.lvl_upd_elems[L][q]);
+                if (look && has1) {
+                    // part 1 on the side stream once this panel's rows are final; part 0, which
+                    // writes the columns the previous part 1 also writes, waits for that one
+                    MG_HIP(hipEventRecord(N.ev_fork, st));
+                    MG_HIP(hipStreamWaitEvent(N.side, N.ev_fork, 0));
+                    schur_part(1, N.side);
+                    hipEvent_t evb = N.ev_side[side_k & 1];
+                    MG_HIP(hipEventRecord(evb, N.side));
+                    if (prev_side) MG_HIP(hipStreamWaitEvent(st, prev_side, 0));
+                    schur_part(0, st);
+                    prev_side = evb;
+                    ++side_k;
+                } else {
+                    schur_part(0, st);
+                    schur_part(1, st);
+                }
             }
+        }
+        // the next level's assembly reads every footer this level wrote
+        if (prev_side) {
+            MG_HIP(hipStreamWaitEvent(st, prev_side, 0));
+            prev_side = nullptr;
         }
     }
     MG_HIP(hipGetLastError());
