@@ -819,8 +819,8 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
                                                const int *__restrict__ F, const int *__restrict__ W,
                                                const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                int q, const double *__restrict__ D) {
-    __shared__ T As[kPanel][kTile + 1];  // As[k][r] = L[i0+r, kc+k] * d_k
-    __shared__ T Bs[kPanel][kTile + 1];  // Bs[k][r] = L[j0+r, kc+k]
+    __shared__ __align__(16) T As[kPanel][kSchurLd<T>];  // As[k][r] = L[i0+r, kc+k] * d_k
+    __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];  // Bs[k][r] = L[j0+r, kc+k]
     __shared__ T Ds[kBlock];             // d_k of the step's K columns
     const int4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
     const int s = tk.x;
@@ -860,14 +860,26 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
     for (int kc = kb; kc < ke; kc += kPanel) {
         const int nk = min(kPanel, ke - kc);
         __syncthreads();  // the previous chunk's MFMA reads are done
+        typedef T vec_t __attribute__((ext_vector_type(VEC)));
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
             const int k = cq + CPI * j;
             const T dk = Ds[min(kc - kb + k, kBlock - 1)];
+            vec_t va, vb;
 #pragma unroll
             for (int i = 0; i < VEC; ++i) {
-                As[k][VEC * qv + i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
-                Bs[k][VEC * qv + i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+                va[i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
+                vb[i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+            }
+            if constexpr (kSchurLd<T> % VEC == 0) {  // aligned rows: one 16-byte store each
+                *reinterpret_cast<vec_t *>(&As[k][VEC * qv]) = va;
+                *reinterpret_cast<vec_t *>(&Bs[k][VEC * qv]) = vb;
+            } else {
+#pragma unroll
+                for (int i = 0; i < VEC; ++i) {
+                    As[k][VEC * qv + i] = va[i];
+                    Bs[k][VEC * qv + i] = vb[i];
+                }
             }
         }
         __syncthreads();

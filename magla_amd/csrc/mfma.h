@@ -18,6 +18,13 @@
 
 namespace mg {
 
+// Row stride (elements) of the LDS operand chunks.  FP32: 72 = 8 mod 32 banks, so the four
+// k-rows one operand read touches (16 lanes of 4 bytes each) cover every bank exactly twice,
+// and a row start is 16-byte aligned for the vector stores that fill it.  FP64: 65 (a row of
+// 16 lanes of 8 bytes already spans all banks; four rows take four cycles at any stride).
+template <typename T>
+constexpr int kSchurLd = sizeof(T) == 4 ? 72 : 65;
+
 typedef double mg_d4 __attribute__((ext_vector_type(4)));
 typedef float mg_f4 __attribute__((ext_vector_type(4)));
 
@@ -52,7 +59,8 @@ __device__ inline void schur_acc_zero(typename MfmaAcc<T>::type (&acc)[2][2]) {
 // from LDS while the previous batch runs on the matrix cores (software pipeline;
 // the scheduling barriers keep the compiler from sinking the reads to their uses).
 template <typename T>
-__device__ inline void schur_read_ops(const T (*As)[65], const T (*Bs)[65], int k0, T (&colv)[2], T (&rowv)[2]) {
+__device__ inline void schur_read_ops(const T (*As)[kSchurLd<T>], const T (*Bs)[kSchurLd<T>], int k0,
+                                      T (&colv)[2], T (&rowv)[2]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int k = k0 + (lane >> 4), l16 = lane & 15;
@@ -63,7 +71,7 @@ __device__ inline void schur_read_ops(const T (*As)[65], const T (*Bs)[65], int 
 }
 
 template <typename T>
-__device__ inline void schur_acc_tile(const T (*As)[65], const T (*Bs)[65],
+__device__ inline void schur_acc_tile(const T (*As)[kSchurLd<T>], const T (*Bs)[kSchurLd<T>],
                                       typename MfmaAcc<T>::type (&acc)[2][2]) {
     constexpr int kSteps = 4;                 // k-steps (of 4) per batch
     constexpr int kBatches = 64 / (4 * kSteps);
