@@ -1318,6 +1318,14 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
     }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
+        // the row block's old values, read with the panel's (clamped, used only when valid)
+        T old[kRhsChunk / 4];
+        {
+            const int orow = min(r0 + lane, f - 1);
+#pragma unroll
+            for (int j = 0; j < kRhsChunk / 4; ++j)
+                old[j] = Vg[(int64_t)(c0 + min(grp + 4 * j, nr - 1)) * f + orow];
+        }
         {
             T tv[4];
             vec_load<T>(tv, Vg + (int64_t)c0 * f + ps, f, nb, nr);
@@ -1336,12 +1344,6 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
             for (int rr = grp; rr < nr; rr += 4)
                 if (lane < nb) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + lane] = yv[lane][rr];
         } else if (lane < nrow) {
-            T old[kRhsChunk / 4];
-#pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j) {
-                const int rr = grp + 4 * j;
-                old[j] = rr < nr ? Vg[(int64_t)(c0 + rr) * f + r0 + lane] : T(0);
-            }
 #pragma unroll
             for (int j = 0; j < kRhsChunk / 4; ++j) {
                 const int rr = grp + 4 * j;
@@ -1478,6 +1480,11 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
     }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
+        // block cb's old values, read with the panel's (cb < q: a full 64-column panel)
+        T old[kRhsChunk / 4];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk / 4; ++j)
+            old[j] = Z[(int64_t)(c0 + min(grp + 4 * j, nr - 1)) * n + fc + min(cb * kPanel + lane, w - 1)];
         {
             T tz[4];
             vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
@@ -1496,12 +1503,6 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
             for (int rr = grp; rr < nr; rr += 4)
                 if (lane < nb) Y[(int64_t)(c0 + rr) * n + fc + ps + lane] = xv[lane][rr];
         } else {
-            T old[kRhsChunk / 4];
-#pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j) {
-                const int rr = grp + 4 * j;
-                old[j] = rr < nr ? Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] : T(0);
-            }
 #pragma unroll
             for (int j = 0; j < kRhsChunk / 4; ++j) {
                 const int rr = grp + 4 * j;
