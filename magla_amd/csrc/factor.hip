@@ -530,11 +530,12 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // pivot > tol*scale (NaN fails), otherwise regularize to reg*scale and count
 // (negative counted separately; bit 2 marks a refused pivot when
 // regularization is off); the comparison is made in float64 for both
-// precisions.  The column is scaled by the reciprocal (:577) and the trailing
-// update is a_rc -= (l_r d) l_c.  The inverse Li = L11^{-1} is formed after the
-// factorization; its strict lower part is stored transposed in the unused upper
-// triangle of the diagonal block (front row ps+m, column ps+k holds Li[k][m]) for
-// the row-block kernel and the solves.
+// precisions.  One barrier per pivot: the column is scaled by the reciprocal
+// (:577), the trailing update is a_rc -= (l_r d) l_c, and the inverse
+// Li = L11^{-1} is built alongside (Li <- (I - l_k e_k^T) Li).  Li's strict lower
+// part is stored transposed in the unused upper triangle of the diagonal block
+// (front row ps+m, column ps+k holds Li[k][m]) for the row-block kernel and the
+// solves.
 // 64x64 tiles staged through registers: thread (lane r, group g = tid / 64) holds
 // column c = g + 4j, j < 16, of rows r0 + r.  Loads are unconditional (addresses
 // clamped into the tile) and raw; the masks are applied when the values are stored
@@ -612,22 +613,6 @@ __device__ inline double lane_value<double>(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Diagonal-block factorization in 16-column blocks: wave v holds columns 16v .. 16v+15
-// of the 64x64 block, lane r row r.  Block s is factored by wave s alone (its 16
-// pivots need no barrier: the pivot column is read across lanes with readlane); the
-// scaled columns then go to LDS (Lf) and the waves to the right apply block s's
-// update to their own columns, one barrier per block.  The updates reach every
-// entry in the same order and with the same products as one pivot at a time
-// (a_rc -= (l_r d_k) l_c for k = 0, 1, ...).
-constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel_diag
-template <typename T>
-constexpr int kDiagLd = sizeof(T) == 4 ? 68 : 66;  // Lf row stride: 16-byte rows, conflict-free
-
-struct DiagPivotRule {
-    double thr, regv;
-    int allow_reg;
-};
-
 // 1 / x for a pivot: hardware reciprocal refined by Newton steps (one for float,
 // two for double), within an ulp of the rounded quotient and without the branchy
 // IEEE division sequence on the pivot-to-pivot path.
@@ -644,89 +629,81 @@ __device__ __forceinline__ double pivot_recip(double x) {
     return __builtin_fma(y, e, y);
 }
 
-// Step KJ of block s, run by wave s (UtduNumericFactorization.cs:560-600).  The
-// pivot rule is evaluated in float64 for both precisions.
-template <typename T, int KJ>
-__device__ __forceinline__ void diag_block_step(T (&a)[kDiagCols], int s, int r, int nb, const DiagPivotRule &rule,
-                                                T *psh, double *dsh, unsigned char *fsh) {
-    const int K = s * kDiagCols + KJ;
-    if (K >= nb) return;  // pivots past the block's width would pivot the identity padding
-    double piv = (double)lane_value<T>(a[KJ], K);
-    unsigned char fl = 0;
-    if (!(piv > rule.thr)) {
-        if (!rule.allow_reg) fl |= 4;
-        if (piv < 0.0) fl |= 2;
-        fl |= 1;
-        piv = rule.regv;
+// State of one 64x64 diagonal-block factorization: four waves, lane r of wave v
+// holds row r of the block for columns 16v .. 16v+15 (a) and the same part of
+// row r of the block's inverse (li).
+struct PanelDiagCtx {
+    int r, v, nb, allow_reg;
+    double thr, regv;
+    double *dsh;          // pivots (LDS)
+    unsigned char *fsh;   // pivot flags (LDS)
+};
+constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel_diag
+
+// Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600).
+// The wave owning column K applies the pivot rule and publishes the scaled column
+// (l_r = a[r][K] / d_K for r > K, zero for r <= K) and the pivot through LDS; lane K
+// of every wave publishes its part of row K of the inverse (final after step K-1).
+// Both are double-buffered by the parity of K, so one barrier per step suffices.
+// Then every wave updates its 16 columns, a[r][c] -= l_r d_K l_c (l_c = 0 for
+// c <= K, so eliminated columns are left as they are), and applies the same row
+// operation to the inverse, li[r][c] -= l_r li[K][c] (li[K][c] = 0 for c > K).
+template <typename T, int K>
+__device__ __forceinline__ void panel_diag_step(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
+                                                T (*colk)[kPanel], T (*rowk)[kPanel], T *pvs) {
+    constexpr int kw = K / kDiagCols, kj = K % kDiagCols, buf = K & 1;
+    // Steps past the block's width would only pivot the identity padding: skipped (uniform).
+    if (K >= x.nb) return;
+    const int cb = x.v * kDiagCols;
+    if (x.r == K) {
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) rowk[buf][cb + j] = li[j];
     }
-    const T pv = (T)piv, pinv = pivot_recip(pv);
-    const bool below = r > K;
-    const T prod = a[KJ] * pinv;
-    const T l = below ? prod : T(0);
-    a[KJ] = below ? prod : a[KJ];
-    const T t = l * pv;
-#pragma unroll
-    for (int j = KJ + 1; j < kDiagCols; ++j) a[j] -= t * lane_value<T>(l, s * kDiagCols + j);
-    // every lane stores the same (uniform) values: no branch on the serial path
-    psh[K] = pv;
-    dsh[K] = (double)pv;
-    fsh[K] = fl;
-}
-
-template <typename T, int... KJ>
-__device__ __forceinline__ void diag_block_steps(std::integer_sequence<int, KJ...>, T (&a)[kDiagCols], int s, int r,
-                                                 int nb, const DiagPivotRule &rule, T *psh, double *dsh,
-                                                 unsigned char *fsh) {
-    (diag_block_step<T, KJ>(a, s, r, nb, rule, psh, dsh, fsh), ...);
-}
-
-// Inverse of the unit lower block, Li = L^{-1}, in place in Lf's strict lower part
-// (blocked as in a lower-triangular inversion from the last block column back:
-// Li_jj = L_jj^{-1}, then Li_{>j,j} = -Li_{>j,>j} (L_{>j,j} Li_jj)).
-template <typename T>
-__device__ __forceinline__ void diag_block_inverse(T (*Lf)[kDiagLd<T>], T (*Xs)[kDiagCols + 4], int nbk) {
-    constexpr int B = kDiagCols;
-    const int tid = threadIdx.x;
-    if (tid < 64) {
-        // Li_jj, column c, for the four diagonal blocks at once (lane 16 j + c)
-        const int j = tid >> 4, c = tid & 15;
-        if (j < nbk) {
-            T y[B];
-#pragma unroll
-            for (int rr = 0; rr < B; ++rr) {
-                T acc = T(0);
-#pragma unroll
-                for (int m = 0; m < rr; ++m) acc += Lf[B * j + rr][B * j + m] * y[m];
-                y[rr] = rr == c ? T(1) : (rr < c ? T(0) : -acc);
-            }
-#pragma unroll
-            for (int rr = 1; rr < B; ++rr)
-                if (rr > c) Lf[B * j + rr][B * j + c] = y[rr];
+    if (x.v == kw) {
+        const T akk = lane_value<T>(a[kj], K);
+        double piv = (double)akk;
+        unsigned char fl = 0;
+        if (K < x.nb && !(piv > x.thr)) {
+            if (!x.allow_reg) fl |= 4;
+            if (piv < 0.0) fl |= 2;
+            fl |= 1;
+            piv = x.regv;
         }
+        const T pv = (T)piv, pinv = pivot_recip(pv);
+        const bool below = x.r > K;
+        const T l = a[kj] * pinv;
+        a[kj] = below ? l : a[kj];
+        colk[buf][x.r] = below ? l : T(0);
+        // every lane of the wave stores the same (uniform) values: no branch
+        pvs[buf] = pv;
+        x.dsh[K] = (double)pv;  // written to D after the last step
+        x.fsh[K] = fl;
     }
     __syncthreads();
-    for (int j = nbk - 2; j >= 0; --j) {
-        const int R0 = B * (j + 1), nrow = B * (nbk - 1 - j);
-        // X = L_{>j,j} Li_jj (Li_jj unit lower): X[i][c] = L[R0+i][Bj+c] + sum_{m>c} L[R0+i][Bj+m] Li_jj[m][c]
-        for (int e = tid; e < nrow * B; e += blockDim.x) {
-            const int i = e >> 4, c = e & 15;
-            const T *lrow = Lf[R0 + i] + B * j;
-            T acc = lrow[c];
+    const T lb = colk[buf][x.r];
+    const T t = lb * pvs[buf];
+    T lc[kDiagCols], rk[kDiagCols];
 #pragma unroll
-            for (int m = 1; m < B; ++m)
-                if (m > c) acc += lrow[m] * Lf[B * j + m][B * j + c];
-            Xs[i][c] = acc;
-        }
-        __syncthreads();
-        // Li[r][Bj+c] = -(X[r-R0][c] + sum_{R0 <= m < r} Li[r][m] X[m-R0][c])
-        for (int e = tid; e < nrow * B; e += blockDim.x) {
-            const int i = e >> 4, c = e & 15, r = R0 + i;
-            T acc = Xs[i][c];
-            for (int m = 0; m < i; ++m) acc += Lf[r][R0 + m] * Xs[m][c];
-            Lf[r][B * j + c] = -acc;
-        }
-        __syncthreads();
+    for (int j = 0; j < kDiagCols; ++j) {
+        lc[j] = colk[buf][cb + j];
+        rk[j] = rowk[buf][cb + j];
     }
+    // Entries above a lane's diagonal take updates too: they are never read.
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) a[j] -= t * lc[j];
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
+    // Materialize every row entry at the end of the step (keeps the compiler from
+    // deferring updates across steps).
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) asm volatile("" : "+v"(a[j]), "+v"(li[j]));
+}
+
+template <typename T, int... K>
+__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...>, T (&a)[kDiagCols],
+                                                 T (&li)[kDiagCols], const PanelDiagCtx &x, T (*colk)[kPanel],
+                                                 T (*rowk)[kPanel], T *pvs) {
+    (panel_diag_step<T, K>(a, li, x, colk, rowk, pvs), ...);
 }
 
 template <typename T>
@@ -738,22 +715,22 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ node
                                                     int allow_reg, double *__restrict__ D,
                                                     unsigned char *__restrict__ flags) {
     // Rows/columns past nb are padded with the identity (pivot 1, never reported).
-    __shared__ __align__(16) T Lf[kPanel][kDiagLd<T>];   // L (strict lower), then Li in its place
-    __shared__ __align__(16) T Xs[3 * kDiagCols][kDiagCols + 4];
-    __shared__ T psh[kPanel];
+    __shared__ __align__(16) T colk[2][kPanel];
+    __shared__ __align__(16) T rowk[2][kPanel];
+    __shared__ T pvs[2];
     __shared__ double dsh[kPanel];
     __shared__ unsigned char fsh[kPanel];
     const int s = nodes[blockIdx.x];
     const int f = F[s], w = W[s];
     if (w <= ps) return;
-    const int pe = min(ps + kPanel, w), nb = pe - ps, nbk = (nb + kDiagCols - 1) / kDiagCols;
+    const int pe = min(ps + kPanel, w), nb = pe - ps;
     T *M = arena + off[s];
     const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double sc = scale[front_sys[s]];
-    const DiagPivotRule rule{tol * sc, regf * sc, allow_reg};
+    PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
     // Unconditional loads (addresses clamped into the block), masked afterwards: a
     // masked load would wait for the previous one before its register is reused.
-    T a[kDiagCols];
+    T a[kDiagCols], li[kDiagCols];
     const T *mcol = M + (int64_t)ps * f + ps + min(r, nb - 1);
 #pragma unroll
     for (int j = 0; j < kDiagCols; ++j) a[j] = mcol[(int64_t)min(v * kDiagCols + j, nb - 1) * f];
@@ -761,30 +738,10 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ node
     for (int j = 0; j < kDiagCols; ++j) {
         const int c = v * kDiagCols + j;
         a[j] = (r < nb && c < nb && r >= c) ? a[j] : (r == c ? T(1) : T(0));
+        li[j] = r == c ? T(1) : T(0);
     }
-    for (int sb = 0; sb < nbk; ++sb) {
-        if (v == sb) {
-            diag_block_steps<T>(std::make_integer_sequence<int, kDiagCols>{}, a, sb, r, nb, rule, psh, dsh, fsh);
-            T *dst = Lf[r] + sb * kDiagCols;
-#pragma unroll
-            for (int j = 0; j < kDiagCols; ++j) dst[j] = r > sb * kDiagCols + j ? a[j] : T(0);
-        }
-        __syncthreads();
-        if (v > sb && v < nbk) {
-            // block sb's update of this wave's columns, pivot by pivot
-            const T *lr = Lf[r] + sb * kDiagCols;
-            T t[kDiagCols];
-#pragma unroll
-            for (int k = 0; k < kDiagCols; ++k) t[k] = lr[k] * psh[sb * kDiagCols + k];
-#pragma unroll
-            for (int j = 0; j < kDiagCols; ++j) {
-                const T *lc = Lf[v * kDiagCols + j] + sb * kDiagCols;
-#pragma unroll
-                for (int k = 0; k < kDiagCols; ++k) a[j] -= t[k] * lc[k];
-            }
-        }
-    }
-    diag_block_inverse<T>(Lf, Xs, nbk);
+    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x, colk, rowk, pvs);
+    __syncthreads();
     if (threadIdx.x < nb) {
         const int fc = firstcol[s] + ps;
         D[fc + threadIdx.x] = dsh[threadIdx.x];
@@ -794,13 +751,15 @@ __global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ node
 #pragma unroll
         for (int j = 0; j < kDiagCols; ++j) {
             const int c = v * kDiagCols + j;
-            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[j];  // L below the diagonal
+            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[j];   // L below the diagonal
             else if (c == r) M[(int64_t)(ps + c) * f + ps + r] = T(1);
         }
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) {
+            const int c = v * kDiagCols + j;
+            if (c < r) M[(int64_t)(ps + r) * f + ps + c] = li[j];  // Li[r][c] at row ps+c, column ps+r
+        }
     }
-    // Li[rr][c] (c < rr) at front row ps+c, column ps+rr: lane c, rows rr = v, v+4, ...
-    for (int rr = v; rr < nb; rr += 4)
-        if (r < rr) M[(int64_t)(ps + rr) * f + ps + r] = Lf[rr][r];
 }
 
 // Panel LDL^T, step 2: each workgroup turns one 64-row block below the diagonal
