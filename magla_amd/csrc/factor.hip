@@ -496,22 +496,34 @@ __global__ __launch_bounds__(256) void k_assemble_pull(const int4 *__restrict__ 
     T *P = arena + off[s];
     const int c0 = child_ptr[s], c1 = child_ptr[s + 1];
     if (i >= f) return;
-    for (int q = 0; q < kTile / 4; ++q) {
-        const int j = jbase + 4 * q;
-        if (j > i) break;
-        T acc = T(0);
-        for (int c = c0; c < c1; ++c) {
-            const int ch = children[c];
-            const int *inv = inv_map + inv_off[ch];
-            const int a = inv[j];
-            if (a < 0) continue;
-            const int b = inv[i];
-            if (b < 0) continue;
-            const int fc = F[ch], wc = W[ch];
-            acc += arena[off[ch] + (int64_t)(wc + a) * fc + (wc + b)];
-        }
-        P[(int64_t)j * f + i] = acc;
+    // columns j = jbase + 4q <= i of row i; each child's row map entry for i and its
+    // 16 column entries are read together, then the 16 update values together
+    const int nq = i < jbase ? 0 : min(kTile / 4, (i - jbase) / 4 + 1);
+    if (nq == 0) return;
+    T acc[kTile / 4];
+#pragma unroll
+    for (int q = 0; q < kTile / 4; ++q) acc[q] = T(0);
+    for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
+        const int ch = children[c];
+        const int *inv = inv_map + inv_off[ch];
+        const int b = inv[i];
+        if (b < 0) continue;
+        const int fc = F[ch], wc = W[ch];
+        const T *U = arena + off[ch] + wc + b;
+        // unconditional loads, masked afterwards: past the thread's last column the
+        // indices repeat that column's (cache hits, no extra traffic)
+        int a[kTile / 4];
+#pragma unroll
+        for (int q = 0; q < kTile / 4; ++q) a[q] = inv[jbase + 4 * min(q, nq - 1)];
+        T u[kTile / 4];
+#pragma unroll
+        for (int q = 0; q < kTile / 4; ++q) u[q] = U[(int64_t)(wc + max(a[q], 0)) * fc];
+#pragma unroll
+        for (int q = 0; q < kTile / 4; ++q) acc[q] = (q < nq && a[q] >= 0) ? acc[q] + u[q] : acc[q];
     }
+#pragma unroll
+    for (int q = 0; q < kTile / 4; ++q)
+        if (q < nq) P[(int64_t)(jbase + 4 * q) * f + i] = acc[q];
 }
 
 // Original matrix entries of the own columns (positions are unique).
