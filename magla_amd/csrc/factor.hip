@@ -44,7 +44,7 @@ static inline int schur_bucket(int f) {
 constexpr int kSmallFront = 64;       // fronts this small are solved by one wave each (all panels, all RHS)
 constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
-constexpr int kFootChunk = 1024;      // footer rows per partial product in the backward solve
+constexpr int kFootChunk = 256;       // footer rows per partial product in the backward solve
 constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
 static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread each (256 threads)");
 
@@ -1087,14 +1087,30 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     if (i >= f) return;
     const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
     T *Vg = V + vbase[g] * nrhs;
-    for (int r = 0; r < nrhs; ++r) {
-        T acc = i < w ? Y[(int64_t)r * n + firstcol[g] + i] : T(0);
-        for (int c = c0; c < c1; ++c) {
+    constexpr int R = 8;  // right-hand sides per pass: their loads are issued together
+    const T *yrow = Y + firstcol[g] + min(i, max(w - 1, 0));
+    for (int r0 = 0; r0 < nrhs; r0 += R) {
+        const int nr = min(R, nrhs - r0);
+        T acc[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc[q] = yrow[(int64_t)(r0 + min(q, nr - 1)) * n];
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc[q] = i < w ? acc[q] : T(0);
+        for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
             const int x = children[c];
             const int a = inv_map[inv_off[x] + i];
-            if (a >= 0) acc += V[vbase[x] * nrhs + (int64_t)r * F[x] + W[x] + a];
+            // a child's footer row W[x] + max(a, 0) < F[x] (children have footer rows)
+            const T *src = V + vbase[x] * nrhs + W[x] + max(a, 0);
+            const int fx = F[x];
+            T u[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) u[q] = src[(int64_t)(r0 + min(q, nr - 1)) * fx];
+#pragma unroll
+            for (int q = 0; q < R; ++q) acc[q] = a >= 0 ? acc[q] + u[q] : acc[q];
         }
-        Vg[(int64_t)r * f + i] = acc;
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (q < nr) Vg[(int64_t)(r0 + q) * f + i] = acc[q];
     }
 }
 
