@@ -782,7 +782,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                     int ps, const double *__restrict__ D) {
-    __shared__ T Bm[kPanel][kPanel + 1];  // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0
+    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0; rows of 16-byte
+    // multiples, read as broadcast vectors
+    __shared__ __align__(16) T Bm[kPanel][kPanel + 16 / sizeof(T)];
     __shared__ T X[kTile][kPanel + 1];
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
@@ -812,8 +814,10 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
         tile_store<T>(X, tx, nr, nb);
     }
     __syncthreads();
-    // thread -> (row r, columns k = c0, c0+4, ...)
-    const int r = lane, c0 = g;
+    // thread -> (row r, columns k = c0 .. c0+15)
+    constexpr int VW = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    const int r = lane, c0 = g * (kPanel / 4);
     T out[kPanel / 4];
 #pragma unroll
     for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
@@ -821,12 +825,16 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     for (int m = 0; m < nb; ++m) {
         const T xm = X[r][m];
 #pragma unroll
-        for (int q = 0; q < kPanel / 4; ++q) out[q] += xm * Bm[m][c0 + 4 * q];
+        for (int q = 0; q < kPanel / 4; q += VW) {
+            const vec_t b = *reinterpret_cast<const vec_t *>(&Bm[m][c0 + q]);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) out[q + e] += xm * b[e];
+        }
     }
     if (r < nr)
 #pragma unroll
         for (int q = 0; q < kPanel / 4; ++q) {
-            const int k = c0 + 4 * q;
+            const int k = c0 + q;
             if (k < nb) M[(int64_t)(ps + k) * f + r0 + r] = out[q];
         }
 }
