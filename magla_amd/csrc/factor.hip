@@ -929,17 +929,17 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
 }
 
 // The same update with 128x128 output tiles (large fronts): K streamed through LDS in chunks
-// of 32, each thread stages rows r = tid % 128 of k = tid / 128 + 2j, j < 16.
+// of 32, staged from 16-byte row vectors; tasks remapped per XCD as in k_schur.
 template <typename T>
 __global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks, T *__restrict__ arena,
                                                   const int *__restrict__ F, const int *__restrict__ W,
                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                   int q, const double *__restrict__ D) {
     constexpr int KC = 32;
-    __shared__ T As[KC][kT128Pad];  // As[k][r] = L[i0+r, kc+k] * d_k
-    __shared__ T Bs[KC][kT128Pad];  // Bs[k][r] = L[j0+r, kc+k]
+    __shared__ __align__(16) T As[KC][kT128Pad];  // As[k][r] = L[i0+r, kc+k] * d_k
+    __shared__ __align__(16) T Bs[KC][kT128Pad];  // Bs[k][r] = L[j0+r, kc+k]
     __shared__ T Ds[kBlock];
-    const int4 tk = tasks[blockIdx.x];
+    const int4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
     const int s = tk.x;
     const int f = F[s], w = W[s];
     const int ps = q * kPanel;
@@ -949,18 +949,23 @@ __global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks
     const int i0 = base + tk.y * 128, j0 = base + tk.z * 128;
     T *M = arena + off[s];
     const double *Dp = D + firstcol[s];
-    const int r = threadIdx.x & 127, kh = threadIdx.x >> 7;
-    const bool ia = i0 + r < f, ib = j0 + r < f;
-    const int rowa = min(i0 + r, f - 1), rowb = min(j0 + r, f - 1);
+    // loader mapping as in k_schur: 16-byte vectors of VEC rows, TPC threads per column, CPI
+    // columns per pass (k = cq + CPI j); vector starts clamped into the front, masked after
+    constexpr int VEC = 16 / sizeof(T), TPC = 128 / VEC, CPI = 256 / TPC, NJ = KC / CPI;
+    const int qv = threadIdx.x % TPC, cq = threadIdx.x / TPC;
+    const int ra0 = i0 + VEC * qv, rb0 = j0 + VEC * qv;
+    const int rowa = min(ra0, f - 1), rowb = min(rb0, f - 1);
     Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
-    T ra[KC / 2], rb[KC / 2];
+    T ra[NJ][VEC], rb[NJ][VEC];
     auto fetch = [&](int kc) {
         const int nk = min(KC, ke - kc);
 #pragma unroll
-        for (int j = 0; j < KC / 2; ++j) {
-            const T *col = M + (int64_t)(kc + min(kh + 2 * j, nk - 1)) * f;
-            ra[j] = col[rowa];
-            rb[j] = col[rowb];
+        for (int j = 0; j < NJ; ++j) {
+            const T *col = M + (int64_t)(kc + min(cq + CPI * j, nk - 1)) * f;
+            const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + rowa);
+            const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + rowb);
+            __builtin_memcpy(ra[j], &va, 16);
+            __builtin_memcpy(rb[j], &vb, 16);
         }
     };
     typename MfmaAcc<T>::type acc[4][4];
@@ -969,11 +974,19 @@ __global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks
     for (int kc = kb; kc < ke; kc += KC) {
         const int nk = min(KC, ke - kc);
         __syncthreads();
+        typedef T vec_t __attribute__((ext_vector_type(VEC)));
 #pragma unroll
-        for (int j = 0; j < KC / 2; ++j) {
-            const int k = kh + 2 * j;
-            As[k][r] = (k < nk && ia) ? ra[j] * Ds[min(kc - kb + k, kBlock - 1)] : T(0);
-            Bs[k][r] = (k < nk && ib) ? rb[j] : T(0);
+        for (int j = 0; j < NJ; ++j) {
+            const int k = cq + CPI * j;
+            const T dk = Ds[min(kc - kb + k, kBlock - 1)];
+            vec_t va, vb;
+#pragma unroll
+            for (int i = 0; i < VEC; ++i) {
+                va[i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
+                vb[i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+            }
+            *reinterpret_cast<vec_t *>(&As[k][VEC * qv]) = va;  // rows of 16-byte multiples
+            *reinterpret_cast<vec_t *>(&Bs[k][VEC * qv]) = vb;
         }
         __syncthreads();
         if (kc + KC < ke) fetch(kc + KC);
