@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 
 #include "device.h"
 #include "mfma.h"
@@ -1898,6 +1899,15 @@ struct ProfScope {
 };
 
 static bool profile_on() { return g_prof.on; }
+// MG_NO_LOOKAHEAD=1 in the environment runs every Schur step on the caller's stream (the
+// schedule of timed launches), e.g. for per-dispatch profiles without concurrent dispatches
+static bool lookahead_allowed() {
+    static const bool off = [] {
+        const char *v = std::getenv("MG_NO_LOOKAHEAD");
+        return v && v[0] == '1';
+    }();
+    return !off;
+}
 
 void profile_enable(int on) {
     g_prof.on = on != 0;
@@ -1937,7 +1947,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
     const double e = sizeof(T);
     // Lookahead: the part of an outer Schur step that the next 256-column block's panels do not
     // touch runs on a second stream, overlapping those panels (off while launches are timed)
-    const bool look = !profile_on();
+    const bool look = !profile_on() && lookahead_allowed();
     if (look) N.ensure_side();
     hipEvent_t prev_side = nullptr;
     int side_k = 0;
