@@ -47,6 +47,7 @@ constexpr int kArenaPad = 64;         // tail elements after the last front (16-
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
 constexpr int kFootChunk = 256;       // footer rows per partial product in the backward solve
 constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
+constexpr int kPanelFusedRows = 4;    // fronts with at most this many row blocks below a panel: one launch
 static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread each (256 threads)");
 
 // A device front is a chain of consecutive supernodes s, s+1, ..., each the
@@ -68,7 +69,7 @@ struct DevicePlan {
     std::vector<int> lvl_node_ptr, lvl_pull_ptr;
     std::vector<int64_t> lvl_asm_ptr;
     std::vector<int> lvl_npanels;
-    std::vector<std::vector<int>> lvl_panel_ptr, lvl_diag_ptr;
+    std::vector<std::vector<int>> lvl_panel_ptr, lvl_ptask_ptr;  // per panel: row-block tasks, panel tasks
     // Schur tasks per level: for panel q, part p (0: before the next panel, 1: may overlap it) and
     // bucket b, tasks [ub[q * kUbStride + p * kB + b], ub[... + 1]) of upd_tasks (64x64 tiles) and,
     // with ub128, of upd128_tasks (128x128 tiles)
@@ -83,7 +84,6 @@ struct DevicePlan {
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
 
     std::vector<std::vector<int>> lvl_sfwd_ptr, lvl_sbwd_ptr;  // per 64-column panel
-    int max_panel_tasks = 0;        // largest panel-task launch (solve partial-sum buffer)
     int64_t vsum = 0;               // sum f over fronts (solve front vectors)
     // algorithmic work per launch
     std::vector<double> lvl_pull_elems, lvl_pull_index;
@@ -96,8 +96,7 @@ struct DevicePlan {
     int nsys = 1;                   // independent systems merged into this plan (batched solves)
     std::vector<int> h_sys_begin;   // first row of each system, nsys+1
     DevArray<int> front_sys, sys_begin;
-    DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm, diag_nodes,
-        diag_tb;
+    DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks, sfwd_tasks, sbwd_tasks;
     DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
@@ -106,7 +105,8 @@ struct DevicePlan {
     int max_foot_tasks = 0;
     DevArray<int> asm_vidx;
     DevArray<int4> pull_tasks, upd_tasks, upd128_tasks;
-    DevArray<int2> panel_tasks;
+    DevArray<int2> panel_tasks;  // k_panel_rows: (front, row block) of fronts with many row blocks
+    DevArray<int2> ptasks;       // k_panel: (front, row block), row block -1 = diagonal block only
 };
 
 int plan_level_count(const DevicePlan &P) { return P.nlevels; }
@@ -247,9 +247,9 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         }
     }
 
-    std::vector<int> level_nodes, diag_nodes, diag_tb;
+    std::vector<int> level_nodes;
     std::vector<int4> pull, upd, upd128;
-    std::vector<int2> panel, init;
+    std::vector<int2> panel, init, ptask;
     std::vector<int64_t> vbase(nf + 1, 0);
     for (int g = 0; g < nf; ++g) vbase[g + 1] = vbase[g] + P->h_f[g];
     P->vsum = vbase[nf];
@@ -353,23 +353,23 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             P->lvl_solve_w.push_back(sw);
         }
         P->lvl_pull_index.push_back(pe_index);
-        std::vector<int> pp{(int)panel.size()}, dp{(int)diag_nodes.size()}, ub64, ub128;
+        std::vector<int> pp{(int)panel.size()}, tp{(int)ptask.size()}, ub64, ub128;
         std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
         for (int q = 0; q < npan; ++q) {
-            const int pbase = (int)panel.size();
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
                 if (w <= q * kPanel) continue;
-                diag_nodes.push_back(g);
-                diag_tb.push_back((int)panel.size() - pbase);
                 const int pe = std::min(q * kPanel + kPanel, w);
                 pel[q] += (double)(f - pe) * (pe - q * kPanel);
                 const int nrb = (f - pe + kTile - 1) / kTile;
-                for (int rb = 0; rb < nrb; ++rb) panel.push_back(make_int2(g, rb));
+                // few row blocks: each repeats the diagonal factorization in one launch; many:
+                // the diagonal block alone, then the row blocks in a k_panel_rows launch
+                if (nrb == 0 || nrb > kPanelFusedRows) ptask.push_back(make_int2(g, -1));
+                for (int rb = 0; rb < nrb; ++rb)
+                    (nrb > kPanelFusedRows ? panel : ptask).push_back(make_int2(g, rb));
             }
-            P->max_panel_tasks = std::max(P->max_panel_tasks, (int)panel.size() - pbase);
-            dp.push_back((int)diag_nodes.size());
             pp.push_back((int)panel.size());
+            tp.push_back((int)ptask.size());
             // Schur tasks of this panel, grouped by front-size bucket, in both tile shapes, in two
             // parts: part 0 must precede the next panel of its front (inner steps, and the next
             // 256-column block of an outer step); part 1 is the rest of an outer step's trailing
@@ -407,7 +407,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->lvl_panel_ptr.push_back(pp);
         P->lvl_ub64.push_back(ub64);
         P->lvl_ub128.push_back(ub128);
-        P->lvl_diag_ptr.push_back(dp);
+        P->lvl_ptask_ptr.push_back(tp);
         P->lvl_upd_flops.push_back(ufl);
         P->lvl_upd_elems.push_back(uel);
         P->lvl_panel_elems.push_back(pel);
@@ -432,8 +432,6 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->pat_ptr.upload(P->h_pat_ptr);
     P->pat_rows.upload(P->h_pat_rows);
     P->level_nodes.upload(level_nodes);
-    P->diag_nodes.upload(diag_nodes);
-    P->diag_tb.upload(diag_tb);
     P->init_tasks.upload(init);
     P->small_nodes.upload(small);
     P->foot_tasks.upload(foot);
@@ -454,6 +452,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->upd_tasks.upload(upd);
     P->upd128_tasks.upload(upd128);
     P->panel_tasks.upload(panel);
+    P->ptasks.upload(ptask);
     return P;
 }
 
@@ -537,9 +536,9 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
     arena[pos[e]] += (T)values[vidx[e]];
 }
 
-// Panel LDL^T, step 1: one workgroup per front factors the (<=64)^2 diagonal
-// block of the panel in LDS and writes it back (unit diagonal, L below) with
-// D.  Pivot rule of UtduNumericFactorization.cs:553-577: accept only
+// Panel LDL^T, step 1: the (<=64)^2 diagonal block of the panel is factored in
+// registers and LDS (k_panel), giving D and the block's inverse Li.  Pivot rule of
+// UtduNumericFactorization.cs:553-577: accept only
 // pivot > tol*scale (NaN fails), otherwise regularize to reg*scale and count
 // (negative counted separately; bit 2 marks a refused pivot when
 // regularization is off); the comparison is made in float64 for both
@@ -651,7 +650,7 @@ struct PanelDiagCtx {
     double *dsh;          // pivots (LDS)
     unsigned char *fsh;   // pivot flags (LDS)
 };
-constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel_diag
+constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel
 
 // Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600).
 // The wave owning column K applies the pivot rule and publishes the scaled column
@@ -719,62 +718,6 @@ __device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...
     (panel_diag_step<T, K>(a, li, x, colk, rowk, pvs), ...);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_panel_diag(const int *__restrict__ nodes, T *__restrict__ arena,
-                                                    const int *__restrict__ F, const int *__restrict__ W,
-                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                    int ps, const double *__restrict__ scale,
-                                                    const int *__restrict__ front_sys, double tol, double regf,
-                                                    int allow_reg, double *__restrict__ D,
-                                                    unsigned char *__restrict__ flags) {
-    // Rows/columns past nb are padded with the identity (pivot 1, never reported).
-    __shared__ __align__(16) T colk[2][kPanel];
-    __shared__ __align__(16) T rowk[2][kPanel];
-    __shared__ T pvs[2];
-    __shared__ double dsh[kPanel];
-    __shared__ unsigned char fsh[kPanel];
-    const int s = nodes[blockIdx.x];
-    const int f = F[s], w = W[s];
-    if (w <= ps) return;
-    const int pe = min(ps + kPanel, w), nb = pe - ps;
-    T *M = arena + off[s];
-    const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const double sc = scale[front_sys[s]];
-    PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
-    // Unconditional loads (addresses clamped into the block), masked afterwards: a
-    // masked load would wait for the previous one before its register is reused.
-    T a[kDiagCols], li[kDiagCols];
-    const T *mcol = M + (int64_t)ps * f + ps + min(r, nb - 1);
-#pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) a[j] = mcol[(int64_t)min(v * kDiagCols + j, nb - 1) * f];
-#pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) {
-        const int c = v * kDiagCols + j;
-        a[j] = (r < nb && c < nb && r >= c) ? a[j] : (r == c ? T(1) : T(0));
-        li[j] = r == c ? T(1) : T(0);
-    }
-    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x, colk, rowk, pvs);
-    __syncthreads();
-    if (threadIdx.x < nb) {
-        const int fc = firstcol[s] + ps;
-        D[fc + threadIdx.x] = dsh[threadIdx.x];
-        flags[fc + threadIdx.x] = fsh[threadIdx.x];
-    }
-    if (r < nb) {
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) {
-            const int c = v * kDiagCols + j;
-            if (c < r) M[(int64_t)(ps + c) * f + ps + r] = a[j];   // L below the diagonal
-            else if (c == r) M[(int64_t)(ps + c) * f + ps + r] = T(1);
-        }
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) {
-            const int c = v * kDiagCols + j;
-            if (c < r) M[(int64_t)(ps + r) * f + ps + c] = li[j];  // Li[r][c] at row ps+c, column ps+r
-        }
-    }
-}
-
 // Panel LDL^T, step 2: each workgroup turns one 64-row block below the diagonal
 // block into L rows: X := X B with B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k
 // (Li from the diagonal block's upper triangle, Li[k][k] = 1).
@@ -819,6 +762,110 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     constexpr int VW = 16 / sizeof(T);
     typedef T vec_t __attribute__((ext_vector_type(VW)));
     const int r = lane, c0 = g * (kPanel / 4);
+    T out[kPanel / 4];
+#pragma unroll
+    for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
+#pragma unroll 4
+    for (int m = 0; m < nb; ++m) {
+        const T xm = X[r][m];
+#pragma unroll
+        for (int q = 0; q < kPanel / 4; q += VW) {
+            const vec_t b = *reinterpret_cast<const vec_t *>(&Bm[m][c0 + q]);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) out[q + e] += xm * b[e];
+        }
+    }
+    if (r < nr)
+#pragma unroll
+        for (int q = 0; q < kPanel / 4; ++q) {
+            const int k = c0 + q;
+            if (k < nb) M[(int64_t)(ps + k) * f + r0 + r] = out[q];
+        }
+}
+
+// Panel LDL^T: every workgroup factors the front's 64x64 diagonal block (four waves, one
+// barrier per pivot, panel_diag_step) and then, for a front with at most kPanelFusedRows row
+// blocks below the panel, turns its 64-row block into L rows, X := X B with
+// B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k (as k_panel_rows).  Those row-block workgroups
+// repeat the diagonal factorization instead of waiting for a launch of its own; the row
+// block's loads are in flight meanwhile.  A front with more row blocks gets one workgroup
+// here (row block -1) and its row blocks go to k_panel_rows.  The workgroup of row block 0
+// or -1 writes D, the pivot flags and Li.  The diagonal block's lower triangle keeps its assembled values: the other
+// workgroups of the launch read them, and nothing later does (the row blocks, the Schur
+// updates and the solves use Li and D; the host's factor export rebuilds L11 from Li).
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T *__restrict__ arena,
+                                               const int *__restrict__ F, const int *__restrict__ W,
+                                               const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+                                               int ps, const double *__restrict__ scale,
+                                               const int *__restrict__ front_sys, double tol, double regf,
+                                               int allow_reg, double *__restrict__ D,
+                                               unsigned char *__restrict__ flags) {
+    // Rows/columns past nb are padded with the identity (pivot 1, never reported).
+    __shared__ __align__(16) T colk[2][kPanel];
+    __shared__ __align__(16) T rowk[2][kPanel];
+    __shared__ T pvs[2];
+    __shared__ double dsh[kPanel];
+    __shared__ unsigned char fsh[kPanel];
+    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0; rows of 16-byte
+    // multiples, read as broadcast vectors
+    __shared__ __align__(16) T Bm[kPanel][kPanel + 16 / sizeof(T)];
+    __shared__ T X[kTile][kPanel + 1];
+    const int2 tk = tasks[blockIdx.x];
+    const int s = tk.x, rb = tk.y;
+    const int f = F[s], w = W[s];
+    const int pe = min(ps + kPanel, w), nb = pe - ps;
+    const int r0 = pe + max(rb, 0) * kTile, nr = rb < 0 ? 0 : min(kTile, f - r0);
+    T *M = arena + off[s];
+    const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    T tx[kPanel / 4];
+    tile_load<T>(tx, M, f, r0, ps, nr, nb);  // zero when nr = 0
+    const double sc = scale[front_sys[s]];
+    PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
+    // Unconditional loads (addresses clamped into the block), masked afterwards: a
+    // masked load would wait for the previous one before its register is reused.
+    T a[kDiagCols], li[kDiagCols];
+    const T *mcol = M + (int64_t)ps * f + ps + min(r, nb - 1);
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) a[j] = mcol[(int64_t)min(v * kDiagCols + j, nb - 1) * f];
+#pragma unroll
+    for (int j = 0; j < kDiagCols; ++j) {
+        const int c = v * kDiagCols + j;
+        a[j] = (r < nb && c < nb && r >= c) ? a[j] : (r == c ? T(1) : T(0));
+        li[j] = r == c ? T(1) : T(0);
+    }
+    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x, colk, rowk, pvs);
+    __syncthreads();
+    if (rb <= 0) {
+        if (threadIdx.x < nb) {
+            const int fc = firstcol[s] + ps;
+            D[fc + threadIdx.x] = dsh[threadIdx.x];
+            flags[fc + threadIdx.x] = fsh[threadIdx.x];
+        }
+        if (r < nb) {
+#pragma unroll
+            for (int j = 0; j < kDiagCols; ++j) {
+                const int c = v * kDiagCols + j;
+                if (c < r) M[(int64_t)(ps + r) * f + ps + c] = li[j];  // Li[r][c] at row ps+c, column ps+r
+            }
+        }
+    }
+    if (nr <= 0) return;  // uniform: the diagonal-block-only workgroup
+    {
+        // Bm[c][r] = Li[r][c] / d_r (lane r holds row r of Li)
+        const T rd = T(1) / (T)dsh[min(r, nb - 1)];
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) {
+            const int c = v * kDiagCols + j;
+            Bm[c][r] = (c < r && r < nb) ? li[j] * rd : (c == r && r < nb ? rd : T(0));
+        }
+        tile_store<T>(X, tx, nr, nb);
+    }
+    __syncthreads();
+    // thread -> (row r, columns k = c0 .. c0+15)
+    constexpr int VW = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    const int c0 = v * (kPanel / 4);
     T out[kPanel / 4];
 #pragma unroll
     for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
@@ -1966,12 +2013,12 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
         }
         for (int q = 0; q < P.lvl_npanels[L]; ++q) {
             const int ps = q * kPanel;
+            const int t0 = P.lvl_ptask_ptr[L][q], t1 = P.lvl_ptask_ptr[L][q + 1];
             const int b0 = P.lvl_panel_ptr[L][q], b1 = P.lvl_panel_ptr[L][q + 1];
-            const int d0 = P.lvl_diag_ptr[L][q], d1 = P.lvl_diag_ptr[L][q + 1];
-            if (d1 > d0) {
+            if (t1 > t0) {
                 ProfScope ps_(st, KP_PDIAG, 0.0, 0.0);
-                hipLaunchKernelGGL(k_panel_diag<T>, dim3(d1 - d0), dim3(256), 0, st, P.diag_nodes.p + d0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, ps, N.scale.p, P.front_sys.p, opts.pivot_tolerance,
+                hipLaunchKernelGGL(k_panel<T>, dim3(t1 - t0), dim3(256), 0, st, P.ptasks.p + t0, arena, P.f.p, P.w.p,
+                                   P.off.p, P.first.p, ps, N.scale.p, P.front_sys.p, opts.pivot_tolerance,
                                    opts.diagonal_regularization, opts.allow_diagonal_regularization, N.diag.p,
                                    N.flags.p);
             }
@@ -2422,6 +2469,24 @@ void numeric_front(const Numeric *N, int s, double *out) {
     } else {
         MG_HIP(hipMemcpy(front.data(), reinterpret_cast<const double *>(N->arena.p) + P.h_off[g],
                          cnt * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // each 64x64 diagonal block holds Li = L11^{-1} (transposed, upper triangle) and its
+    // assembled lower triangle: L11 = Li^{-1}, unit diagonal
+    const int wg = P.h_w[g];
+    for (int ps = 0; ps < wg; ps += kPanel) {
+        const int nb = std::min(kPanel, wg - ps);
+        auto li = [&](int r, int c) { return front[(int64_t)(ps + r) * fg + ps + c]; };  // Li[r][c], c < r
+        std::vector<double> Lb((size_t)nb * nb, 0.0);
+        for (int c = 0; c < nb; ++c) {
+            Lb[(size_t)c * nb + c] = 1.0;
+            for (int r = c + 1; r < nb; ++r) {
+                double acc = li(r, c);
+                for (int m = c + 1; m < r; ++m) acc += li(r, m) * Lb[(size_t)m * nb + c];
+                Lb[(size_t)r * nb + c] = -acc;
+            }
+        }
+        for (int c = 0; c < nb; ++c)
+            for (int r = c; r < nb; ++r) front[(int64_t)(ps + c) * fg + ps + r] = Lb[(size_t)r * nb + c];
     }
     std::vector<int> rowmap(S.n, -1);
     for (int t = 0; t < fg; ++t) rowmap[P.h_pat_rows[P.h_pat_ptr[g] + t]] = t;
