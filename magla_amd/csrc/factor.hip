@@ -1403,9 +1403,11 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     }
 }
 
-// Forward panel (columns [ps, pe) of a front): every workgroup forms y = Li v[ps:pe)
-// from the stored 64x64 inverse; task rb = -1 writes y to Y, task rb >= 0 subtracts
-// L[rows, ps:pe) y from its 64-row block of V below the panel.
+// Forward panel (columns [ps, pe) of a front), on the matrix cores: every workgroup forms
+// y^T = v^T Li^T from the stored 64x64 inverse (wave v: columns 16v .. 16v+15 of y^T, the
+// chunk's right-hand sides as the 16 rows); task rb = -1 writes y to Y, task rb >= 0 forms
+// (V_rows)^T -= y^T L[rows, ps:pe)^T for its 64-row block of V below the panel, the L
+// operand read straight from the front (16 rows of a column per load, coalesced).
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
                                                    const int *__restrict__ F, const int *__restrict__ W,
@@ -1413,34 +1415,39 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
                                                    const int64_t *__restrict__ vbase, T *__restrict__ V,
                                                    T *__restrict__ Y, int ps, int n, int nrhs,
         const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
-    __shared__ T inv[kPanel][kPanel + 1];
-    __shared__ T blk[kPanel][kPanel + 1];
-    __shared__ T vv[kPanel][kRhsChunk + 1];
-    __shared__ T yv[kPanel][kRhsChunk + 1];
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    __shared__ T inv[kPanel][kPanel + 1];     // inv[m][k] = Li[k][m]
+    __shared__ T vv[kPanel][kRhsChunk + 1];   // vv[k][rr] = v[ps + k][c0 + rr]
+    __shared__ T yv[kPanel][kRhsChunk + 1];   // yv[k][rr] = y[k][c0 + rr]
     const int2 tk = tasks[blockIdx.x];
     if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
     const int g = tk.x, rb = tk.y, f = F[g], w = W[g];
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     const T *M = arena + off[g];
     T *Vg = V + vbase[g] * nrhs;
-    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int r0 = pe + (rb < 0 ? 0 : rb) * kTile, nrow = rb < 0 ? 0 : min(kTile, f - r0);
+    const int col = kPanel / 4 * wv + l16;  // this lane's output column (y) or row of the block (V)
+    // B operand of the row product: L[r0 + col][ps + 4 s + lk], s < 16 (clamped; rows past
+    // the block are not stored, columns past nb meet y = 0)
+    T bl[kPanel / 4];
+    if (rb >= 0) {
+        const T *src = M + r0 + min(col, nrow - 1);
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) bl[s] = src[(int64_t)(ps + min(4 * s + lk, nb - 1)) * f];
+    }
     {
-        T ti[kPanel / 4], tb[kPanel / 4];
+        T ti[kPanel / 4];
         tile_load<T>(ti, M, f, ps, ps, nb, nb);
-        tile_load<T>(tb, M, f, r0, ps, nrow, nb);  // zero when rb < 0
-        tile_store_inv<T>(inv, ti, nb);            // inv[m][k] = Li[k][m]
-        tile_store<T>(blk, tb, nrow, nb);          // blk[i][k] = L[r0+i][ps+k]
+        tile_store_inv<T>(inv, ti, nb);
     }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        // the row block's old values, read with the panel's (clamped, used only when valid)
-        T old[kRhsChunk / 4];
-        {
-            const int orow = min(r0 + lane, f - 1);
+        T old[4];
+        if (rb >= 0) {
+            const int orow = min(r0 + col, f - 1);
 #pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j)
-                old[j] = Vg[(int64_t)(c0 + min(grp + 4 * j, nr - 1)) * f + orow];
+            for (int q = 0; q < 4; ++q) old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + orow];
         }
         {
             T tv[4];
@@ -1449,28 +1456,35 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
             vec_store<T>(vv, tv, nb, nr);
         }
         __syncthreads();
-        for (int rr = grp; rr < nr; rr += 4) {
-            T acc = T(0);
-#pragma unroll 8
-            for (int m = 0; m < nb; ++m) acc += inv[m][lane] * vv[m][rr];
-            yv[lane][rr] = acc;
-        }
-        __syncthreads();
-        if (rb < 0) {
-            for (int rr = grp; rr < nr; rr += 4)
-                if (lane < nb) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + lane] = yv[lane][rr];
-        } else if (lane < nrow) {
+        typename MfmaAcc<T>::type acc;
 #pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j) {
-                const int rr = grp + 4 * j;
-                if (rr < nr) {
-                    T acc = T(0);
-#pragma unroll 8
-                    for (int k = 0; k < nb; ++k) acc += blk[lane][k] * yv[k][rr];
-                    Vg[(int64_t)(c0 + rr) * f + r0 + lane] = old[j] - acc;
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s)
+            acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
+        // acc[q] = y[col][rr], rr = row(lk, q)
+        if (rb < 0) {
+            if (col < nb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = MfmaAcc<T>::row(lk, q);
+                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + col] = acc[q];
                 }
-            }
+            continue;  // uniform
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yv[col][MfmaAcc<T>::row(lk, q)] = acc[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(yv[4 * s + lk][l16], bl[s], acc);
+        if (col < nrow)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + col] = old[q] - acc[q];
+            }
     }
 }
 
@@ -1567,17 +1581,18 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
     }
 }
 
-// Backward panel (columns [ps, pe)), right-looking: every workgroup forms
-// x = Li^T z[ps:pe) from the stored inverse; the task of block q writes x to Y, the
-// tasks of blocks cb < q subtract L[ps:pe, block]^T x from their block of Z.
+// Backward panel (columns [ps, pe)), right-looking, on the matrix cores: every workgroup
+// forms x^T = z^T Li from the stored inverse; the task of block q writes x to Y, the tasks of
+// blocks cb < q form (Z_block)^T -= x^T L[ps:pe, block] (the L block staged through LDS).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
                                                    const int *__restrict__ F, const int *__restrict__ W,
                                                    const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                    T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs,
         const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
-    __shared__ T blk[kPanel][kPanel + 1];
-    __shared__ T inv[kPanel][kPanel + 1];
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    __shared__ T blk[kPanel][kPanel + 1];   // blk[k][c] = L[ps+k][64cb+c]
+    __shared__ T inv[kPanel][kPanel + 1];   // inv[m][k] = Li[k][m]
     __shared__ T zz[kPanel][kRhsChunk + 1];
     __shared__ T xv[kPanel][kRhsChunk + 1];
     const int2 tk = tasks[blockIdx.x];
@@ -1586,21 +1601,22 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
     const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
     const int fc = firstcol[g];
     const T *M = arena + off[g];
-    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int col = kPanel / 4 * wv + l16;
     {
         T ti[kPanel / 4], tb[kPanel / 4];
         tile_load<T>(ti, M, f, ps, ps, nb, nb);
         tile_load<T>(tb, M, f, ps, cb * kPanel, cb < q ? nb : 0, kPanel);
-        tile_store_inv<T>(inv, ti, nb);                     // inv[m][k] = Li[k][m]
-        tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);    // blk[k][c] = L[ps+k][64cb+c]
+        tile_store_inv<T>(inv, ti, nb);
+        tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);
     }
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
         // block cb's old values, read with the panel's (cb < q: a full 64-column panel)
-        T old[kRhsChunk / 4];
+        T old[4];
 #pragma unroll
-        for (int j = 0; j < kRhsChunk / 4; ++j)
-            old[j] = Z[(int64_t)(c0 + min(grp + 4 * j, nr - 1)) * n + fc + min(cb * kPanel + lane, w - 1)];
+        for (int qq = 0; qq < 4; ++qq)
+            old[qq] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, qq), nr - 1)) * n + fc + min(cb * kPanel + col, w - 1)];
         {
             T tz[4];
             vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
@@ -1608,27 +1624,33 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
             vec_store<T>(zz, tz, nb, nr);
         }
         __syncthreads();
-        for (int rr = grp; rr < nr; rr += 4) {
-            T acc = T(0);
-#pragma unroll 8
-            for (int k = 0; k < nb; ++k) acc += inv[lane][k] * zz[k][rr];
-            xv[lane][rr] = acc;
-        }
-        __syncthreads();
-        if (cb == q) {
-            for (int rr = grp; rr < nr; rr += 4)
-                if (lane < nb) Y[(int64_t)(c0 + rr) * n + fc + ps + lane] = xv[lane][rr];
-        } else {
+        typename MfmaAcc<T>::type acc;
 #pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j) {
-                const int rr = grp + 4 * j;
-                if (rr < nr) {
-                    T acc = T(0);
-#pragma unroll 8
-                    for (int k = 0; k < nb; ++k) acc += blk[k][lane] * xv[k][rr];
-                    Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + lane] = old[j] - acc;
+        for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
+        // x^T[rr][j] = sum_k z[k][rr] Li[k][j], Li[k][j] = inv[j][k]
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s)
+            acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
+        if (cb == q) {
+            if (col < nb)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const int rr = MfmaAcc<T>::row(lk, qq);
+                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + fc + ps + col] = acc[qq];
                 }
-            }
+            continue;  // uniform
+        }
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) xv[col][MfmaAcc<T>::row(lk, qq)] = acc[qq];
+        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(xv[4 * s + lk][l16], blk[4 * s + lk][col], acc);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            const int rr = MfmaAcc<T>::row(lk, qq);
+            if (rr < nr) Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + col] = old[qq] - acc[qq];
         }
     }
 }
