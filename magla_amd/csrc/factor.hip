@@ -718,6 +718,61 @@ __device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...
     (panel_diag_step<T, K>(a, li, x, colk, rowk, pvs), ...);
 }
 
+// Row stride of the scaled inverse Bm in LDS: the MFMA operand reads (4 rows x 16
+// consecutive columns per wave) hit distinct banks for FP32.
+template <typename T>
+constexpr int kBmLd = sizeof(T) == 4 ? kPanel + 16 : kPanel + 1;
+
+// Row-block product of a panel on the matrix cores: L[r0 + i][ps + k] = sum_m X[i][m] Bm[m][k]
+// for i < nr, k < nb, with X the row block's assembled values.  Computed as
+// C^T = Bm^T X^T in 16x16x4 tiles: wave v takes rows i = 16 v .. 16 v + 15 (the MFMA columns,
+// so loads and stores run down front columns), all 64 k as four 16-row tiles.
+// rows_load issues the X operand loads (X[16 v + l16][4 s + lk], clamped) ahead of use.
+template <typename T>
+__device__ __forceinline__ void rows_load(T (&xb)[kPanel / 4], const T *__restrict__ M, int f, int ps, int nb, int r0,
+                                          int nr) {
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4, v = threadIdx.x >> 6;
+    const T *src = M + r0 + min(16 * v + l16, max(nr, 1) - 1);
+#pragma unroll
+    for (int s = 0; s < kPanel / 4; ++s) xb[s] = src[(int64_t)(ps + min(4 * s + lk, nb - 1)) * f];
+}
+template <typename T>
+__device__ __forceinline__ void rows_product(T *__restrict__ M, int f, int ps, int nb, int r0, int nr,
+                                             const T (*Bm)[kBmLd<T>], const T (&xb)[kPanel / 4]) {
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4, v = threadIdx.x >> 6;
+    if (16 * v >= nr) return;  // wave-uniform: no rows of this wave
+    // only the k-steps and output tiles inside nb (uniform): small panels cost little
+    const int ns = (nb + 3) / 4, nt = (nb + 15) / 16;
+    typename MfmaAcc<T>::type acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[t][q] = T(0);
+    if (nb == kPanel) {  // full panel: straight-line, the compiler interleaves reads and MFMAs
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][16 * t + l16], xb[s], acc[t]);
+    } else {
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) {
+            if (s >= ns) break;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t < nt) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][16 * t + l16], xb[s], acc[t]);
+        }
+    }
+    const int i = 16 * v + l16;
+    if (i < nr)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = 16 * t + MfmaAcc<T>::row(lk, q);
+                if (k < nb) M[(int64_t)(ps + k) * f + r0 + i] = acc[t][q];
+            }
+}
+
 // Panel LDL^T, step 2: each workgroup turns one 64-row block below the diagonal
 // block into L rows: X := X B with B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k
 // (Li from the diagonal block's upper triangle, Li[k][k] = 1).
@@ -726,10 +781,8 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                     int ps, const double *__restrict__ D) {
-    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0; rows of 16-byte
-    // multiples, read as broadcast vectors
-    __shared__ __align__(16) T Bm[kPanel][kPanel + 16 / sizeof(T)];
-    __shared__ T X[kTile][kPanel + 1];
+    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0 (the MFMA A operand)
+    __shared__ T Bm[kPanel][kBmLd<T>];
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
     const int f = F[s], w = W[s];
@@ -740,10 +793,11 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     T *M = arena + off[s];
     const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
     const int fc = firstcol[s] + ps;
+    T xb[kPanel / 4];
+    rows_load<T>(xb, M, f, ps, nb, r0, nr);
     {
-        T ti[kPanel / 4], tx[kPanel / 4], td[kPanel / 4];
+        T ti[kPanel / 4], td[kPanel / 4];
         tile_load<T>(ti, M, f, ps, ps, nb, nb);
-        tile_load<T>(tx, M, f, r0, ps, nr, nb);
 #pragma unroll
         for (int j = 0; j < kPanel / 4; ++j) {
             const int k = g + 4 * j;
@@ -755,32 +809,9 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
             const T rdk = T(1) / td[j];
             Bm[lane][k] = (lane < k && k < nb) ? ti[j] * rdk : (lane == k && k < nb ? rdk : T(0));
         }
-        tile_store<T>(X, tx, nr, nb);
     }
     __syncthreads();
-    // thread -> (row r, columns k = c0 .. c0+15)
-    constexpr int VW = 16 / sizeof(T);
-    typedef T vec_t __attribute__((ext_vector_type(VW)));
-    const int r = lane, c0 = g * (kPanel / 4);
-    T out[kPanel / 4];
-#pragma unroll
-    for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
-#pragma unroll 4
-    for (int m = 0; m < nb; ++m) {
-        const T xm = X[r][m];
-#pragma unroll
-        for (int q = 0; q < kPanel / 4; q += VW) {
-            const vec_t b = *reinterpret_cast<const vec_t *>(&Bm[m][c0 + q]);
-#pragma unroll
-            for (int e = 0; e < VW; ++e) out[q + e] += xm * b[e];
-        }
-    }
-    if (r < nr)
-#pragma unroll
-        for (int q = 0; q < kPanel / 4; ++q) {
-            const int k = c0 + q;
-            if (k < nb) M[(int64_t)(ps + k) * f + r0 + r] = out[q];
-        }
+    rows_product<T>(M, f, ps, nb, r0, nr, Bm, xb);
 }
 
 // Panel LDL^T: every workgroup factors the front's 64x64 diagonal block (four waves, one
@@ -807,10 +838,8 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
     __shared__ T pvs[2];
     __shared__ double dsh[kPanel];
     __shared__ unsigned char fsh[kPanel];
-    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0; rows of 16-byte
-    // multiples, read as broadcast vectors
-    __shared__ __align__(16) T Bm[kPanel][kPanel + 16 / sizeof(T)];
-    __shared__ T X[kTile][kPanel + 1];
+    // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0 (the MFMA A operand)
+    __shared__ T Bm[kPanel][kBmLd<T>];
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
     const int f = F[s], w = W[s];
@@ -818,8 +847,8 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
     const int r0 = pe + max(rb, 0) * kTile, nr = rb < 0 ? 0 : min(kTile, f - r0);
     T *M = arena + off[s];
     const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    T tx[kPanel / 4];
-    tile_load<T>(tx, M, f, r0, ps, nr, nb);  // zero when nr = 0
+    T xb[kPanel / 4];
+    if (nr > 0) rows_load<T>(xb, M, f, ps, nb, r0, nr);  // in flight during the factorization
     const double sc = scale[front_sys[s]];
     PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
     // Unconditional loads (addresses clamped into the block), masked afterwards: a
@@ -859,32 +888,9 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
             const int c = v * kDiagCols + j;
             Bm[c][r] = (c < r && r < nb) ? li[j] * rd : (c == r && r < nb ? rd : T(0));
         }
-        tile_store<T>(X, tx, nr, nb);
     }
     __syncthreads();
-    // thread -> (row r, columns k = c0 .. c0+15)
-    constexpr int VW = 16 / sizeof(T);
-    typedef T vec_t __attribute__((ext_vector_type(VW)));
-    const int c0 = v * (kPanel / 4);
-    T out[kPanel / 4];
-#pragma unroll
-    for (int q = 0; q < kPanel / 4; ++q) out[q] = T(0);
-#pragma unroll 4
-    for (int m = 0; m < nb; ++m) {
-        const T xm = X[r][m];
-#pragma unroll
-        for (int q = 0; q < kPanel / 4; q += VW) {
-            const vec_t b = *reinterpret_cast<const vec_t *>(&Bm[m][c0 + q]);
-#pragma unroll
-            for (int e = 0; e < VW; ++e) out[q + e] += xm * b[e];
-        }
-    }
-    if (r < nr)
-#pragma unroll
-        for (int q = 0; q < kPanel / 4; ++q) {
-            const int k = c0 + q;
-            if (k < nb) M[(int64_t)(ps + k) * f + r0 + r] = out[q];
-        }
+    rows_product<T>(M, f, ps, nb, r0, nr, Bm, xb);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Block b
