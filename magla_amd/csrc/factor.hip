@@ -1409,6 +1409,22 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     }
 }
 
+// acc += sum over k-steps s < 16 (k = 4 s + lk) of step(a(s), b(s)); for a panel narrower
+// than 64 only the k-steps inside nb (uniform), the full case straight-line.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void mfma_k64(typename MfmaAcc<T>::type &acc, int nb, FA a, FB b) {
+    if (nb == kPanel) {
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(a(s), b(s), acc);
+    } else {
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) {
+            if (4 * s >= nb) break;
+            acc = MfmaAcc<T>::step(a(s), b(s), acc);
+        }
+    }
+}
+
 // Forward panel (columns [ps, pe) of a front), on the matrix cores: every workgroup forms
 // y^T = v^T Li^T from the stored 64x64 inverse (wave v: columns 16v .. 16v+15 of y^T, the
 // chunk's right-hand sides as the 16 rows); task rb = -1 writes y to Y, task rb >= 0 forms
@@ -1465,9 +1481,7 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s)
-            acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
+        mfma_k64<T>(acc, nb, [&](int s) { return vv[4 * s + lk][l16]; }, [&](int s) { return inv[4 * s + lk][col]; });
         // acc[q] = y[col][rr], rr = row(lk, q)
         if (rb < 0) {
             if (col < nb)
@@ -1483,8 +1497,7 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(yv[4 * s + lk][l16], bl[s], acc);
+        mfma_k64<T>(acc, nb, [&](int s) { return yv[4 * s + lk][l16]; }, [&](int s) { return bl[s]; });
         if (col < nrow)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1514,6 +1527,9 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const int2 *__restrict__ tas
 
 // Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
 // L[i, 64cb+col] x[row i] (x of the footer rows are final: ancestors are solved first).
+// On the matrix cores: (partial)^T = x^T L[chunk, block] in 16x16x4 tiles, wave v taking
+// columns 16 v .. 16 v + 15; 64-row tiles of L and x go through LDS, the next one's loads in
+// flight during the current one's products.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ tasks, const T *__restrict__ arena,
                                                        const int *__restrict__ F, const int *__restrict__ W,
@@ -1521,6 +1537,7 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
                                                        const int64_t *__restrict__ vbase, const T *__restrict__ V,
                                                        T *__restrict__ part, int nrhs,
         const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T xs[kTile][kRhsChunk + 1];
     const int4 tk = tasks[blockIdx.x];
@@ -1530,12 +1547,13 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
     const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
     const T *M = arena + off[g];
     const T *Vg = V + vbase[g] * nrhs;
-    const int tid = threadIdx.x, col = tid & 63, grp = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int col = kPanel / 4 * wv + l16;
     for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nrhs - c0);
-        T acc[kRhsChunk / 4];
+        typename MfmaAcc<T>::type acc;
 #pragma unroll
-        for (int j = 0; j < kRhsChunk / 4; ++j) acc[j] = T(0);
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
         T tb[kPanel / 4], tx[4];
         tile_load<T>(tb, M, f, ibeg, cb0, iend - ibeg, ncol);
         vec_load<T>(tx, Vg + (int64_t)c0 * f + ibeg, f, iend - ibeg, nr);
@@ -1550,18 +1568,12 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
                 vec_load<T>(tx, Vg + (int64_t)c0 * f + nx, f, iend - nx, nr);
             }
 #pragma unroll
-            for (int j = 0; j < kRhsChunk / 4; ++j) {
-                const int rr = grp + 4 * j;
-                if (rr < nr) {
-#pragma unroll 16
-                    for (int i = 0; i < kTile; ++i) acc[j] += blk[i][col] * xs[i][rr];
-                }
-            }
+            for (int s = 0; s < kTile / 4; ++s) acc = MfmaAcc<T>::step(xs[4 * s + lk][l16], blk[4 * s + lk][col], acc);
         }
 #pragma unroll
-        for (int j = 0; j < kRhsChunk / 4; ++j) {
-            const int rr = grp + 4 * j;
-            if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = acc[j];
+        for (int q = 0; q < 4; ++q) {
+            const int rr = MfmaAcc<T>::row(lk, q);
+            if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = acc[q];
         }
         __syncthreads();
     }
@@ -1634,9 +1646,7 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
         // x^T[rr][j] = sum_k z[k][rr] Li[k][j], Li[k][j] = inv[j][k]
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s)
-            acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
+        mfma_k64<T>(acc, nb, [&](int s) { return zz[4 * s + lk][l16]; }, [&](int s) { return inv[col][4 * s + lk]; });
         if (cb == q) {
             if (col < nb)
 #pragma unroll
@@ -1651,8 +1661,7 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ task
         __syncthreads();
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(xv[4 * s + lk][l16], blk[4 * s + lk][col], acc);
+        mfma_k64<T>(acc, nb, [&](int s) { return xv[4 * s + lk][l16]; }, [&](int s) { return blk[4 * s + lk][col]; });
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
             const int rr = MfmaAcc<T>::row(lk, qq);
