@@ -58,6 +58,14 @@ static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread e
 // elimination order, the supernode partition and every factor entry are
 // those of the reference; only the numeric schedule differs (no extend-add
 // between chain members, Schur updates of up to kBlock columns).
+// A workgroup's task in the panel solves, carrying the front's sizes, offsets and system so
+// the kernel's data loads depend on one load instead of two.
+struct __align__(8) PanelTask {
+    int g, blk, f, w;     // front, row block (forward) or column block (backward), size, width
+    int64_t off, vbase;   // front's arena offset and vector base
+    int fc, sys;          // first column, system
+};
+
 struct DevicePlan {
     int n = 0, nsuper = 0, nfront = 0, nlevels = 0, max_w = 0, max_f = 0;
     int64_t arena = 0;  // sum f^2 over fronts
@@ -98,7 +106,8 @@ struct DevicePlan {
     DevArray<int> front_sys, sys_begin;
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
-    DevArray<int2> init_tasks, sfwd_tasks, sbwd_tasks;
+    DevArray<int2> init_tasks;
+    DevArray<PanelTask> sfwd_tasks, sbwd_tasks;
     DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
     DevArray<int4> foot_tasks;     // (front, column block, row chunk, index of the chunk in the block)
 
@@ -437,8 +446,16 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->foot_tasks.upload(foot);
     P->footr_tasks.upload(footr);
 
-    P->sfwd_tasks.upload(sfwd);
-    P->sbwd_tasks.upload(sbwd);
+    auto pack = [&](const std::vector<int2> &v) {
+        std::vector<PanelTask> o(v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            const int g = v[i].x;
+            o[i] = PanelTask{g, v[i].y, P->h_f[g], P->h_w[g], P->h_off[g], vbase[g], P->h_first[g], fsys[g]};
+        }
+        return o;
+    };
+    P->sfwd_tasks.upload(pack(sfwd));
+    P->sbwd_tasks.upload(pack(sbwd));
     P->vbase.upload(vbase);
     P->inv_map.upload(inv_map);
     P->perm.upload(S.perm);
@@ -1431,22 +1448,19 @@ __device__ __forceinline__ void mfma_k64(typename MfmaAcc<T>::type &acc, int nb,
 // (V_rows)^T -= y^T L[rows, ps:pe)^T for its 64-row block of V below the panel, the L
 // operand read straight from the front (16 rows of a column per load, coalesced).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
-                                                   const int *__restrict__ F, const int *__restrict__ W,
-                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                   const int64_t *__restrict__ vbase, T *__restrict__ V,
-                                                   T *__restrict__ Y, int ps, int n, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+__global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
+                                                   T *__restrict__ V, T *__restrict__ Y, int ps, int n, int nrhs,
+                                                   const unsigned char *__restrict__ act) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T inv[kPanel][kPanel + 1];     // inv[m][k] = Li[k][m]
     __shared__ T vv[kPanel][kRhsChunk + 1];   // vv[k][rr] = v[ps + k][c0 + rr]
     __shared__ T yv[kPanel][kRhsChunk + 1];   // yv[k][rr] = y[k][c0 + rr]
-    const int2 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
-    const int g = tk.x, rb = tk.y, f = F[g], w = W[g];
+    const PanelTask tk = tasks[blockIdx.x];
+    if (act && !act[tk.sys]) return;  // system already converged: right-hand side zero
+    const int rb = tk.blk, f = tk.f, w = tk.w;
     const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const T *M = arena + off[g];
-    T *Vg = V + vbase[g] * nrhs;
+    const T *M = arena + tk.off;
+    T *Vg = V + tk.vbase * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int r0 = pe + (rb < 0 ? 0 : rb) * kTile, nrow = rb < 0 ? 0 : min(kTile, f - r0);
     const int col = kPanel / 4 * wv + l16;  // this lane's output column (y) or row of the block (V)
@@ -1488,7 +1502,7 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const int2 *__restrict__ task
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int rr = MfmaAcc<T>::row(lk, q);
-                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + firstcol[g] + ps + col] = acc[q];
+                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + ps + col] = acc[q];
                 }
             continue;  // uniform
         }
@@ -1603,22 +1617,20 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
 // forms x^T = z^T Li from the stored inverse; the task of block q writes x to Y, the tasks of
 // blocks cb < q form (Z_block)^T -= x^T L[ps:pe, block] (the L block staged through LDS).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_panel(const int2 *__restrict__ tasks, const T *__restrict__ arena,
-                                                   const int *__restrict__ F, const int *__restrict__ W,
-                                                   const int64_t *__restrict__ off, const int *__restrict__ firstcol,
+__global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                    T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+                                                   const unsigned char *__restrict__ act) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T blk[kPanel][kPanel + 1];   // blk[k][c] = L[ps+k][64cb+c]
     __shared__ T inv[kPanel][kPanel + 1];   // inv[m][k] = Li[k][m]
     __shared__ T zz[kPanel][kRhsChunk + 1];
     __shared__ T xv[kPanel][kRhsChunk + 1];
-    const int2 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
-    const int g = tk.x, cb = tk.y, f = F[g], w = W[g];
+    const PanelTask tk = tasks[blockIdx.x];
+    if (act && !act[tk.sys]) return;  // system already converged: right-hand side zero
+    const int cb = tk.blk, f = tk.f, w = tk.w;
     const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
-    const int fc = firstcol[g];
-    const T *M = arena + off[g];
+    const int fc = tk.fc;
+    const T *M = arena + tk.off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
     {
@@ -2334,8 +2346,8 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
             const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
             if (s1 > s0)
-                hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, P.vbase.p, V, Y, q * kPanel, n, nrhs, P.front_sys.p, act);
+                hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, V, Y,
+                                   q * kPanel, n, nrhs, act);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -2358,8 +2370,8 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
             const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
             if (s1 > s0)
-                hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, P.f.p,
-                                   P.w.p, P.off.p, P.first.p, Y, Z, q, n, nrhs, P.front_sys.p, act);
+                hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, Y, Z, q,
+                                   n, nrhs, act);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
