@@ -58,6 +58,13 @@ static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread e
 // elimination order, the supernode partition and every factor entry are
 // those of the reference; only the numeric schedule differs (no extend-add
 // between chain members, Schur updates of up to kBlock columns).
+// A Schur tile task with the front's size, width, first column and arena offset (one
+// dependent load before the panel loads instead of two).
+struct __align__(8) SchurTask {
+    int g, ti, tj, f, w, fc;  // front, tile row and column, size, width, first column
+    int64_t off;              // front's arena offset
+};
+
 // A workgroup's task in the panel solves, carrying the front's sizes, offsets and system so
 // the kernel's data loads depend on one load instead of two.
 struct __align__(8) PanelTask {
@@ -113,7 +120,8 @@ struct DevicePlan {
 
     int max_foot_tasks = 0;
     DevArray<int> asm_vidx;
-    DevArray<int4> pull_tasks, upd_tasks, upd128_tasks;
+    DevArray<int4> pull_tasks;
+    DevArray<SchurTask> upd_tasks, upd128_tasks;
     DevArray<int2> panel_tasks;  // k_panel_rows: (front, row block) of fronts with many row blocks
     DevArray<int2> ptasks;       // k_panel: (front, row block), row block -1 = diagonal block only
 };
@@ -466,8 +474,16 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->asm_pos.upload(asm_pos);
     P->asm_vidx.upload(asm_vidx);
     P->pull_tasks.upload(pull);
-    P->upd_tasks.upload(upd);
-    P->upd128_tasks.upload(upd128);
+    auto pack_schur = [&](const std::vector<int4> &v) {
+        std::vector<SchurTask> o(v.size());
+        for (size_t i = 0; i < v.size(); ++i) {
+            const int g = v[i].x;
+            o[i] = SchurTask{g, v[i].y, v[i].z, P->h_f[g], P->h_w[g], P->h_first[g], P->h_off[g]};
+        }
+        return o;
+    };
+    P->upd_tasks.upload(pack_schur(upd));
+    P->upd128_tasks.upload(pack_schur(upd128));
     P->panel_tasks.upload(panel);
     P->ptasks.upload(ptask);
     return P;
@@ -925,23 +941,20 @@ __device__ __forceinline__ int xcd_task(int b, int n) {
 // of the step's target region (schur_step).  One 64x64 output tile per workgroup;
 // K streamed through LDS in chunks of 64, products on MFMA (mfma.h).
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T *__restrict__ arena,
-                                               const int *__restrict__ F, const int *__restrict__ W,
-                                               const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                               int q, const double *__restrict__ D) {
+__global__ __launch_bounds__(256) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
+                                               const double *__restrict__ D) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];  // Bs[k][r] = L[j0+r, kc+k]
     __shared__ T Ds[kBlock];             // d_k of the step's K columns
-    const int4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
-    const int s = tk.x;
-    const int f = F[s], w = W[s];
+    const SchurTask tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
+    const int f = tk.f, w = tk.w;
     const int ps = q * kPanel;
     const int pe = min(ps + kPanel, w), bs = (ps / kBlock) * kBlock, be = min(bs + kBlock, w);
     const bool inner = pe < be;
     const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
-    const int i0 = base + tk.y * kTile, j0 = base + tk.z * kTile;
-    T *M = arena + off[s];
-    const double *Dp = D + firstcol[s];
+    const int i0 = base + tk.ti * kTile, j0 = base + tk.tj * kTile;
+    T *M = arena + tk.off;
+    const double *Dp = D + tk.fc;
     // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
     // pass (k = cq + CPI j).  The vector start is clamped into the front (reads past a
     // column's end land in the next column or the arena's tail padding and are masked).
@@ -1002,24 +1015,21 @@ __global__ __launch_bounds__(256) void k_schur(const int4 *__restrict__ tasks, T
 // The same update with 128x128 output tiles (large fronts): K streamed through LDS in chunks
 // of 32, staged from 16-byte row vectors; tasks remapped per XCD as in k_schur.
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur128(const int4 *__restrict__ tasks, T *__restrict__ arena,
-                                                  const int *__restrict__ F, const int *__restrict__ W,
-                                                  const int64_t *__restrict__ off, const int *__restrict__ firstcol,
-                                                  int q, const double *__restrict__ D) {
+__global__ __launch_bounds__(256) void k_schur128(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
+                                                  const double *__restrict__ D) {
     constexpr int KC = 32;
     __shared__ __align__(16) T As[KC][kT128Pad];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ __align__(16) T Bs[KC][kT128Pad];  // Bs[k][r] = L[j0+r, kc+k]
     __shared__ T Ds[kBlock];
-    const int4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
-    const int s = tk.x;
-    const int f = F[s], w = W[s];
+    const SchurTask tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
+    const int f = tk.f, w = tk.w;
     const int ps = q * kPanel;
     const int pe = min(ps + kPanel, w), bs = (ps / kBlock) * kBlock, be = min(bs + kBlock, w);
     const bool inner = pe < be;
     const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
-    const int i0 = base + tk.y * 128, j0 = base + tk.z * 128;
-    T *M = arena + off[s];
-    const double *Dp = D + firstcol[s];
+    const int i0 = base + tk.ti * 128, j0 = base + tk.tj * 128;
+    T *M = arena + tk.off;
+    const double *Dp = D + tk.fc;
     // loader mapping as in k_schur: 16-byte vectors of VEC rows, TPC threads per column, CPI
     // columns per pass (k = cq + CPI j); vector starts clamped into the front, masked after
     constexpr int VEC = 16 / sizeof(T), TPC = 128 / VEC, CPI = 256 / TPC, NJ = KC / CPI;
@@ -2088,11 +2098,10 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                     if (big[b]) {
                         if (u128[c] > u128[b])
                             hipLaunchKernelGGL(k_schur128<T>, dim3(u128[c] - u128[b]), dim3(256), 0, s,
-                                               P.upd128_tasks.p + u128[b], arena, P.f.p, P.w.p, P.off.p, P.first.p, q,
-                                               N.diag.p);
+                                               P.upd128_tasks.p + u128[b], arena, q, N.diag.p);
                     } else if (u64[c] > u64[b]) {
                         hipLaunchKernelGGL(k_schur<T>, dim3(u64[c] - u64[b]), dim3(256), 0, s, P.upd_tasks.p + u64[b],
-                                           arena, P.f.p, P.w.p, P.off.p, P.first.p, q, N.diag.p);
+                                           arena, q, N.diag.p);
                     }
                     b = c;
                 }
