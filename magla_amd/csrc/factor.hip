@@ -43,6 +43,7 @@ static inline int schur_bucket(int f) {
     return b;
 }
 constexpr int kSmallFront = 64;       // fronts this small are solved by one wave each (all panels, all RHS)
+constexpr int kSmallFactor = 32;      // fronts this small are factored whole by a lane group (k_factor_small)
 constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
 constexpr int kFootChunk = 256;       // footer rows per partial product in the backward solve
@@ -375,7 +376,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         for (int q = 0; q < npan; ++q) {
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
-                if (w <= q * kPanel) continue;
+                if (w <= q * kPanel || f <= kSmallFactor) continue;
                 const int pe = std::min(q * kPanel + kPanel, w);
                 pel[q] += (double)(f - pe) * (pe - q * kPanel);
                 const int nrb = (f - pe + kTile - 1) / kTile;
@@ -400,7 +401,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
                         ub.push_back((int)dst.size());
                         for (int g : bylevel[L]) {
                             const int f = P->h_f[g], w = P->h_w[g];
-                            if (w <= q * kPanel || schur_bucket(f) != b) continue;
+                            if (w <= q * kPanel || f <= kSmallFactor || schur_bucket(f) != b) continue;
                             const SchurStep st = schur_step(f, w, q);
                             if (!st.any) continue;
                             const bool has_next = !st.inner && st.be < w;
@@ -1252,6 +1253,108 @@ struct SmallCfg {
     static constexpr int KP = (sizeof(T) == 8 && FR == 64) ? 4 : 8;  // right-hand sides per batch
 };
 
+// Small fronts (f <= FR, FR = 8, 16 or 32): the front's whole partial factorization in one
+// FR-lane group of a wave (64 / FR fronts of one level and size class per wave).  Lane i
+// holds row i of the front's lower triangle and row i of Li in registers.  Pivot k
+// (UtduNumericFactorization.cs:541-601; the pivot rule in float64, as in k_panel) comes
+// from lane k by a shuffle; the scaled column and row k of Li are broadcast through LDS
+// (wave-level syncs, no barriers); each lane applies the rank-1 update to its own row.  After
+// w pivots a row holds L (columns < w) and the Schur complement (columns >= w), as the panel
+// and Schur kernels leave them: stores are L below the diagonal, Li transposed above it,
+// the contribution block, D and the pivot flags.
+template <typename T, int FR>
+__global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ nodes, int count, T *__restrict__ arena,
+                                                      const int *__restrict__ F, const int *__restrict__ W,
+                                                      const int64_t *__restrict__ off,
+                                                      const int *__restrict__ firstcol, const double *__restrict__ scale,
+                                                      const int *__restrict__ front_sys, double tol, double regf,
+                                                      int allow_reg, double *__restrict__ D,
+                                                      unsigned char *__restrict__ flags) {
+    constexpr int GW = 64 / FR, VW = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    __shared__ __align__(16) T colb[4][64];  // per wave: the scaled pivot column, lane slots
+    __shared__ __align__(16) T rowb[4][64];  // per wave: row k of Li of each front
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave_first = (blockIdx.x * 4 + wv) * GW;
+    if (wave_first >= count) return;  // whole wave
+    const int i = lane % FR, base = lane - i;
+    const int idx = wave_first + lane / FR;
+    const bool live = idx < count;
+    const int g = nodes[live ? idx : count - 1];
+    const int f = live ? F[g] : 0, w = live ? W[g] : 0;
+    T *M = arena + off[g];
+    const double sc = scale[front_sys[g]];
+    const double thr = tol * sc, regv = regf * sc;
+    // row i of the lower triangle, A[i][j] = M[j f + i] for j <= i (clamped loads, masked)
+    const int last = max(f - 1, 0), ri = min(i, last);
+    T a[FR], li[FR];
+#pragma unroll
+    for (int j = 0; j < FR; ++j) a[j] = M[(int64_t)min(j, ri) * f + ri];
+#pragma unroll
+    for (int j = 0; j < FR; ++j) {
+        a[j] = (i < f && j <= i) ? a[j] : T(0);
+        li[j] = i == j ? T(1) : T(0);
+    }
+    T my_d = T(1);
+    unsigned char my_fl = 0;
+    T *cb = &colb[wv][base];
+    T *rbf = &rowb[wv][base];
+#pragma unroll
+    for (int k = 0; k < FR; ++k) {
+        if (!__any(k < w)) break;  // wave-uniform: every front of the wave is done
+        const T akk = __shfl(a[k], base + k, 64);
+        double piv = (double)akk;
+        unsigned char fl = 0;
+        if (!(piv > thr)) {
+            if (!allow_reg) fl |= 4;
+            if (piv < 0.0) fl |= 2;
+            fl |= 1;
+            piv = regv;
+        }
+        const T pv = (T)piv, pinv = pivot_recip(pv);
+        if (i == k) {
+            my_d = pv;
+            my_fl = fl;
+        }
+        const bool below = k < w && i > k && i < f;
+        const T l = below ? a[k] * pinv : T(0);
+        a[k] = below ? l : a[k];
+        cb[i] = l;
+        if (i == k) {
+#pragma unroll
+            for (int j = 0; j < FR; j += VW) {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) v[e] = li[j + e];
+                *reinterpret_cast<vec_t *>(rbf + j) = v;
+            }
+        }
+        wave_sync();
+        const T t = l * pv;
+#pragma unroll
+        for (int j = k + 1; j < FR; j += 1) a[j] -= t * cb[j];
+#pragma unroll
+        for (int j = 0; j <= k; ++j) li[j] -= l * rbf[j];
+        wave_sync();
+    }
+    if (live && i < f) {
+        const int fc = firstcol[g];
+#pragma unroll
+        for (int k = 0; k < FR; ++k)
+            if (k < w && k < i) M[(int64_t)k * f + i] = a[k];  // L below the diagonal
+        if (i < w) {
+#pragma unroll
+            for (int k = 0; k < FR; ++k)
+                if (k < i) M[(int64_t)i * f + k] = li[k];  // Li[i][k] at row k, column i
+            D[fc + i] = (double)my_d;
+            flags[fc + i] = my_fl;
+        }
+#pragma unroll
+        for (int j = 0; j < FR; ++j)
+            if (j >= w && j <= i) M[(int64_t)j * f + i] = a[j];  // contribution block
+    }
+}
+
 struct SmallFwdArgs {
     const int *F, *W, *firstcol, *child_ptr, *children;
     const int64_t *inv_off;
@@ -1966,7 +2069,8 @@ struct Profiler {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
     Profiler() {
-        for (const char *nm : {"assemble_pull", "assemble_values", "panel_diag", "panel_rows", "schur", "forward", "backward"}) {
+        for (const char *nm : {"assemble_pull", "assemble_values", "panel_diag", "panel_rows", "schur", "forward", "backward",
+                               "small_fronts"}) {
             KernelTally t;
             t.name = nm;
             k.push_back(t);
@@ -1984,7 +2088,7 @@ struct Profiler {
     }
 };
 static Profiler g_prof;
-enum { KP_PULL = 0, KP_VALUES, KP_PDIAG, KP_PROWS, KP_SCHUR, KP_FWD, KP_BWD };
+enum { KP_PULL = 0, KP_VALUES, KP_PDIAG, KP_PROWS, KP_SCHUR, KP_FWD, KP_BWD, KP_SMALL };
 
 struct ProfScope {
     hipStream_t st;
@@ -2069,6 +2173,17 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             ProfScope ps_(st, KP_VALUES, 0.0, (double)na * (8.0 + 4.0 + 8.0 + 2 * e));
             hipLaunchKernelGGL(k_assemble_values<T>, dim3(blocks_for(na, 256)), dim3(256), 0, st,
                                P.asm_pos.p + P.lvl_asm_ptr[L], P.asm_vidx.p + P.lvl_asm_ptr[L], na, N.values.p, arena);
+        }
+        for (int c = 0; c < 3; ++c) {  // fronts of at most 8 / 16 / 32 rows: factored whole by lane groups
+            const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
+            if (m1 <= m0) continue;
+            ProfScope pf_(st, KP_SMALL, 0.0, 0.0);
+            const int per_block = 4 * (8 >> c);  // 4 waves of 64 / FR fronts
+            auto kern = c == 0 ? k_factor_small<T, 8> : (c == 1 ? k_factor_small<T, 16> : k_factor_small<T, 32>);
+            hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st, P.small_nodes.p + m0,
+                               m1 - m0, arena, P.f.p, P.w.p, P.off.p, P.first.p, N.scale.p, P.front_sys.p,
+                               opts.pivot_tolerance, opts.diagonal_regularization, opts.allow_diagonal_regularization,
+                               N.diag.p, N.flags.p);
         }
         for (int q = 0; q < P.lvl_npanels[L]; ++q) {
             const int ps = q * kPanel;
