@@ -295,7 +295,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             if (f <= kSmallFront) small_cls[f <= 8 ? 0 : (f <= 16 ? 1 : (f <= 32 ? 2 : 3))].push_back(g);
             else
                 for (int c = 0; c < (f + 255) / 256; ++c) init.push_back(make_int2(g, c));
-            const int T = (f + kTile - 1) / kTile;
+            const int T = f <= kSmallFactor ? 0 : (f + kTile - 1) / kTile;  // small fronts: k_assemble_small
             for (int tj = 0; tj < T; ++tj)
                 for (int ti = tj; ti < T; ++ti) pull.push_back(make_int4(g, ti, tj, 0));
             pe_elems += (double)f * (f + 1) / 2;
@@ -558,6 +558,54 @@ __global__ __launch_bounds__(256) void k_assemble_pull(const int4 *__restrict__ 
 #pragma unroll
     for (int q = 0; q < kTile / 4; ++q)
         if (q < nq) P[(int64_t)(jbase + 4 * q) * f + i] = acc[q];
+}
+
+// Zero-fill + extend-add of small fronts (f <= FR, FR = 8, 16 or 32) by lane groups, as
+// k_assemble_pull does for 64x64 tiles (the same sums in the same child order): lane i builds
+// row i of the front's lower triangle, 64 / FR fronts per wave.
+template <typename T, int FR>
+__global__ __launch_bounds__(256) void k_assemble_small(const int *__restrict__ nodes, int count, T *__restrict__ arena,
+                                                        const int *__restrict__ F, const int *__restrict__ W,
+                                                        const int64_t *__restrict__ off,
+                                                        const int *__restrict__ child_ptr,
+                                                        const int *__restrict__ children,
+                                                        const int64_t *__restrict__ inv_off,
+                                                        const int *__restrict__ inv_map) {
+    constexpr int GW = 64 / FR;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave_first = (blockIdx.x * 4 + wv) * GW;
+    if (wave_first >= count) return;  // whole wave
+    const int i = lane % FR;
+    const int idx = wave_first + lane / FR;
+    if (idx >= count) return;  // no barriers or wave-level syncs below
+    const int g = nodes[idx];
+    const int f = F[g];
+    if (i >= f) return;
+    T acc[FR];
+#pragma unroll
+    for (int j = 0; j < FR; ++j) acc[j] = T(0);
+    const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
+    for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
+        const int ch = children[c];
+        const int *inv = inv_map + inv_off[ch];
+        const int b = inv[i];
+        if (b < 0) continue;
+        const int fc = F[ch], wc = W[ch];
+        const T *U = arena + off[ch] + wc + b;
+        // unconditional loads (indices clamped to the lane's row), masked afterwards
+        int a[FR];
+#pragma unroll
+        for (int j = 0; j < FR; ++j) a[j] = inv[min(j, i)];
+        T u[FR];
+#pragma unroll
+        for (int j = 0; j < FR; ++j) u[j] = U[(int64_t)(wc + max(a[j], 0)) * fc];
+#pragma unroll
+        for (int j = 0; j < FR; ++j) acc[j] = (j <= i && a[j] >= 0) ? acc[j] + u[j] : acc[j];
+    }
+    T *P = arena + off[g];
+#pragma unroll
+    for (int j = 0; j < FR; ++j)
+        if (j <= i) P[(int64_t)j * f + i] = acc[j];
 }
 
 // Original matrix entries of the own columns (positions are unique).
@@ -2167,6 +2215,16 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             ProfScope ps_(st, KP_PULL, 0.0, e * P.lvl_pull_elems[L] + 4.0 * P.lvl_pull_index[L]);
             hipLaunchKernelGGL(k_assemble_pull<T>, dim3(np), dim3(256), 0, st, P.pull_tasks.p + P.lvl_pull_ptr[L], arena,
                                P.f.p, P.w.p, P.off.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p);
+        }
+        for (int c = 0; c < 3; ++c) {  // fronts of at most 8 / 16 / 32 rows
+            const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
+            if (m1 <= m0) continue;
+            ProfScope pa_(st, KP_PULL, 0.0, 0.0);
+            const int per_block = 4 * (8 >> c);
+            auto kern = c == 0 ? k_assemble_small<T, 8> : (c == 1 ? k_assemble_small<T, 16> : k_assemble_small<T, 32>);
+            hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
+                               P.small_nodes.p + m0, m1 - m0, arena, P.f.p, P.w.p, P.off.p, P.child_ptr.p,
+                               P.children.p, P.inv_off.p, P.inv_map.p);
         }
         const int64_t na = P.lvl_asm_ptr[L + 1] - P.lvl_asm_ptr[L];
         if (na > 0) {
