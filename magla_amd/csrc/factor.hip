@@ -1230,9 +1230,10 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
                                                   const int64_t *__restrict__ inv_off, const int *__restrict__ inv_map,
                                                   const int64_t *__restrict__ vbase, T *__restrict__ V,
                                                   const T *__restrict__ Y, int n, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+        const int *__restrict__ fsys, const int *__restrict__ cols) {
     const int2 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int g = tk.x, f = F[g], w = W[g];
     const int i = tk.y * 256 + threadIdx.x;
     if (i >= f) return;
@@ -1240,8 +1241,8 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     T *Vg = V + vbase[g] * nrhs;
     constexpr int R = 8;  // right-hand sides per pass: their loads are issued together
     const T *yrow = Y + firstcol[g] + min(i, max(w - 1, 0));
-    for (int r0 = 0; r0 < nrhs; r0 += R) {
-        const int nr = min(R, nrhs - r0);
+    for (int r0 = 0; r0 < nc; r0 += R) {
+        const int nr = min(R, nc - r0);
         T acc[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) acc[q] = yrow[(int64_t)(r0 + min(q, nr - 1)) * n];
@@ -1410,7 +1411,7 @@ struct SmallFwdArgs {
     const int64_t *vbase, *off;
     int n, nrhs;
     const int *fsys;           // system of each front
-    const unsigned char *act;  // per system: still refined (NULL: all)
+    const int *cols;           // per system: leading right-hand sides still to solve (NULL: all)
 };
 
 // Forward: v = own right-hand-side rows plus the children's footer contributions (through
@@ -1428,9 +1429,11 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
     const int g = nodes[idx < count ? idx : count - 1];
-    // a front of a converged system has a zero right-hand side: it is left out like a
-    // group past the end of the list (its vectors stay zero)
-    const bool live = idx < count && (!a.act || a.act[a.fsys[g]]);
+    // right-hand sides past the system's count are zero (padding, or converged in a
+    // refinement step): left out, their vectors stay zero; a front with none is left out
+    // like a group past the end of the list
+    const int nc = idx < count ? (a.cols ? min(a.nrhs, a.cols[a.fsys[g]]) : a.nrhs) : 0;
+    const bool live = nc > 0;
     if (!__any(live)) return;  // whole wave
     const int f = live ? a.F[g] : 0, w = live ? a.W[g] : 0, fc = a.firstcol[g];
     const bool own = i < w, foot = i >= w && i < f;
@@ -1472,11 +1475,12 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
     for (int r0 = 0; r0 < nrhs; r0 += KP) {
+        if (!__any(r0 < nc)) break;  // wave-uniform
         T v[KP], y[KP];
 #pragma unroll
         for (int p = 0; p < KP; ++p) {
             const int r = r0 + p;
-            const bool act = r < nrhs && i < f;
+            const bool act = r < nc && i < f;
             const int64_t rr = act ? r : 0;
             const T yv = Y[rr * n + yi];
             T acc = (act && own) ? yv : T(0);
@@ -1492,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
 #pragma unroll
             for (int p = 0; p < KP; ++p) {
                 const int r = r0 + p;
-                if (r >= nrhs) break;
+                if (r >= nc) break;
                 for (int c = c0 + kKeep; c < c1; ++c) {
                     const int x = a.children[c];
                     const int am = a.inv_map[a.inv_off[x] + i];
@@ -1512,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
         for (int p = 0; p < KP; ++p) {
             const int r = r0 + p;
             const T vf = v[p] - lds_dot<T, FR>(R, &b1[p][base]);
-            if (r < nrhs && i < f) {
+            if (r < nc && i < f) {
                 if (own) Y[(int64_t)r * n + fc + i] = y[p];
                 else Vg[(int64_t)r * f + i] = vf;
             }
@@ -1531,7 +1535,7 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
                                                    const int *__restrict__ pat_rows, const T *__restrict__ arena,
                                                    const int64_t *__restrict__ off, const double *__restrict__ D,
                                                    T *__restrict__ Y, int n, int nrhs, const int *__restrict__ fsys,
-                                                   const unsigned char *__restrict__ act) {
+                                                   const int *__restrict__ cols) {
     constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
     __shared__ __align__(16) T sb[4][2][KP][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1540,8 +1544,10 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
     const int g = nodes[idx < count ? idx : count - 1];
-    const bool live = idx < count && (!act || act[fsys[g]]);  // converged systems are left out
-    if (!__any(live)) return;                                   // whole wave
+    // right-hand sides past the system's count are zero and left out (as in k_fwd_small)
+    const int nc = idx < count ? (cols ? min(nrhs, cols[fsys[g]]) : nrhs) : 0;
+    const bool live = nc > 0;
+    if (!__any(live)) return;  // whole wave
     const int f = live ? F[g] : 0, w = live ? W[g] : 0, fc = firstcol[g];
     const bool own = i < w, foot = i >= w && i < f;
     const T *M = arena + off[g];
@@ -1560,11 +1566,12 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
     for (int r0 = 0; r0 < nrhs; r0 += KP) {
+        if (!__any(r0 < nc)) break;  // wave-uniform
         T xf[KP], yo[KP], z[KP];
 #pragma unroll
         for (int p = 0; p < KP; ++p) {
             const int r = r0 + p;
-            const bool act = r < nrhs;
+            const bool act = r < nc;
             const int64_t rr = act ? r : 0;
             const T xv = Y[rr * n + row], yv = Y[rr * n + yi];
             xf[p] = (act && foot) ? xv : T(0);
@@ -1582,7 +1589,7 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
         for (int p = 0; p < KP; ++p) {
             const int r = r0 + p;
             const T x = z[p] + lds_dot<T, FR>(R, &b1[p][base]);
-            if (r < nrhs && own) Y[(int64_t)r * n + fc + i] = x;
+            if (r < nc && own) Y[(int64_t)r * n + fc + i] = x;
         }
     }
 }
@@ -1611,13 +1618,14 @@ __device__ __forceinline__ void mfma_k64(typename MfmaAcc<T>::type &acc, int nb,
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                    T *__restrict__ V, T *__restrict__ Y, int ps, int n, int nrhs,
-                                                   const unsigned char *__restrict__ act) {
+                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T inv[kPanel][kPanel + 1];     // inv[m][k] = Li[k][m]
     __shared__ T vv[kPanel][kRhsChunk + 1];   // vv[k][rr] = v[ps + k][c0 + rr]
     __shared__ T yv[kPanel][kRhsChunk + 1];   // yv[k][rr] = y[k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
-    if (act && !act[tk.sys]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int rb = tk.blk, f = tk.f, w = tk.w;
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     const T *M = arena + tk.off;
@@ -1638,8 +1646,8 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__
         tile_load<T>(ti, M, f, ps, ps, nb, nb);
         tile_store_inv<T>(inv, ti, nb);
     }
-    for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nrhs - c0);
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
         T old[4];
         if (rb >= 0) {
             const int orow = min(r0 + col, f - 1);
@@ -1689,15 +1697,16 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const int2 *__restrict__ tas
                                                     const int *__restrict__ W, const int *__restrict__ pat_ptr,
                                                     const int *__restrict__ pat_rows, const int64_t *__restrict__ vbase,
                                                     const T *__restrict__ Y, T *__restrict__ V, int n, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+        const int *__restrict__ fsys, const int *__restrict__ cols) {
     const int2 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int g = tk.x, f = F[g], w = W[g];
     const int i = tk.y * 256 + threadIdx.x;
     if (i < w || i >= f) return;
     const int row = pat_rows[pat_ptr[g] + i];
     T *Vg = V + vbase[g] * nrhs;
-    for (int r = 0; r < nrhs; ++r) Vg[(int64_t)r * f + i] = Y[(int64_t)r * n + row];
+    for (int r = 0; r < nc; ++r) Vg[(int64_t)r * f + i] = Y[(int64_t)r * n + row];
 }
 
 // Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
@@ -1711,12 +1720,13 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
                                                        const int64_t *__restrict__ off,
                                                        const int64_t *__restrict__ vbase, const T *__restrict__ V,
                                                        T *__restrict__ part, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+        const int *__restrict__ fsys, const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T blk[kPanel][kPanel + 1];
     __shared__ T xs[kTile][kRhsChunk + 1];
     const int4 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int g = tk.x, f = F[g], w = W[g];
     const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
     const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
@@ -1724,8 +1734,8 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
     const T *Vg = V + vbase[g] * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
-    for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nrhs - c0);
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
@@ -1760,12 +1770,13 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
                                                       const int *__restrict__ firstcol, const double *__restrict__ D,
                                                       const T *__restrict__ Y, const T *__restrict__ part,
                                                       T *__restrict__ Z, int n, int nrhs,
-        const int *__restrict__ fsys, const unsigned char *__restrict__ act) {
+        const int *__restrict__ fsys, const int *__restrict__ cols) {
     const int4 tk = tasks[blockIdx.x];
-    if (act && !act[fsys[tk.x]]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int g = tk.x, cb0 = tk.y * kPanel, tb = tk.z, nch = tk.w;
     const int ncol = min(kPanel, W[g] - cb0), fc = firstcol[g] + cb0;
-    for (int idx = threadIdx.x; idx < ncol * nrhs; idx += 256) {
+    for (int idx = threadIdx.x; idx < ncol * nc; idx += 256) {
         const int col = idx % ncol, rr = idx / ncol;
         T s = T(0);
         for (int ch = 0; ch < nch; ++ch) s += part[((int64_t)(tb + ch) * kPanel + col) * nrhs + rr];
@@ -1780,14 +1791,15 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                    T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs,
-                                                   const unsigned char *__restrict__ act) {
+                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
     __shared__ T blk[kPanel][kPanel + 1];   // blk[k][c] = L[ps+k][64cb+c]
     __shared__ T inv[kPanel][kPanel + 1];   // inv[m][k] = Li[k][m]
     __shared__ T zz[kPanel][kRhsChunk + 1];
     __shared__ T xv[kPanel][kRhsChunk + 1];
     const PanelTask tk = tasks[blockIdx.x];
-    if (act && !act[tk.sys]) return;  // system already converged: right-hand side zero
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
     const int cb = tk.blk, f = tk.f, w = tk.w;
     const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
     const int fc = tk.fc;
@@ -1801,8 +1813,8 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__
         tile_store_inv<T>(inv, ti, nb);
         tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);
     }
-    for (int c0 = 0; c0 < nrhs; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nrhs - c0);
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
         // block cb's old values, read with the panel's (cb < q: a full 64-column panel)
         T old[4];
 #pragma unroll
@@ -1977,11 +1989,30 @@ __global__ __launch_bounds__(256) void k_residual_seg(const double *__restrict__
 
 // R[rows of system t, all columns] = 0 for the systems whose every column has converged
 // (one workgroup per system and column)
-__global__ __launch_bounds__(256) void k_zero_done(double *__restrict__ R, int n, const int *__restrict__ sys_begin,
-                                                   const unsigned char *__restrict__ act) {
+// R[c] = 0 on system t's rows for every column c >= cols[t] (columns a correction solve
+// leaves out: their correction must be zero).
+__global__ __launch_bounds__(256) void k_zero_past(double *__restrict__ R, int n, const int *__restrict__ sys_begin,
+                                                   const int *__restrict__ cols) {
     const int t = blockIdx.x, c = blockIdx.y;
-    if (act[t]) return;
+    if (c < cols[t]) return;
     for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256) R[(int64_t)c * n + i] = 0.0;
+}
+
+// cols[t] = 1 + the last column with a nonzero right-hand-side entry on system t's rows (0 if
+// none): the columns past it are zero (padding of uneven right-hand-side counts) and their
+// solutions are zero, so the solves leave them out.
+__global__ __launch_bounds__(256) void k_sys_cols(const double *__restrict__ B, int ldb, int nrhs,
+                                                  const int *__restrict__ sys_begin, int *__restrict__ cols) {
+    const int t = blockIdx.x, r0 = sys_begin[t], r1 = sys_begin[t + 1];
+    for (int c = nrhs - 1; c >= 0; --c) {
+        int nz = 0;
+        for (int i = r0 + threadIdx.x; i < r1; i += 256) nz |= B[(int64_t)c * ldb + i] != 0.0;
+        if (__syncthreads_or(nz)) {
+            if (threadIdx.x == 0) cols[t] = c + 1;
+            return;
+        }
+    }
+    if (threadIdx.x == 0) cols[t] = 0;
 }
 
 __global__ void k_zero_cols(double *__restrict__ X, int ldx, int n, int nrhs) {
@@ -2066,8 +2097,8 @@ struct Numeric {
     // solve workspaces
     DevArray<unsigned char> wsY, wsU, wsP, wsQ;
     DevArray<double> wsR, wsX, wsPart, wsRel;
-    DevArray<unsigned char> wsAct;     // per system: still refined
-    std::vector<unsigned char> h_act;  // host copy (kept alive for the asynchronous upload)
+    DevArray<int> wsCols;     // per system: leading right-hand sides a solve still needs
+    std::vector<int> h_cols;  // host copy (kept alive for the asynchronous upload)
     int ws_nrhs = 0;
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
     hipStream_t cap_stream = nullptr;
@@ -2494,7 +2525,7 @@ void numeric_info(const Numeric *N, mg_numeric_info &info, double *diag_host) {
 
 template <typename T>
 static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, double *d_x, int ldx, int accumulate,
-                           hipStream_t st, const unsigned char *act = nullptr) {
+                           hipStream_t st, const int *cols = nullptr) {
     DevicePlan &P = *N.plan;
     const int n = N.n;
     T *Y = reinterpret_cast<T *>(N.wsY.p);
@@ -2505,7 +2536,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
     const SmallFwdArgs sfa{P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
-                           P.vbase.p, P.off.p, n, nrhs, P.front_sys.p, act};
+                           P.vbase.p, P.off.p, n, nrhs, P.front_sys.p, cols};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
         const double fl = 2.0 * nrhs * (P.lvl_solve_fac[L] - P.lvl_solve_w[L]);
@@ -2524,12 +2555,12 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         if (i1 > i0)
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
                                P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n,
-                               nrhs, P.front_sys.p, act);
+                               nrhs, P.front_sys.p, cols);
         for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
             const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
             if (s1 > s0)
                 hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, V, Y,
-                                   q * kPanel, n, nrhs, act);
+                                   q * kPanel, n, nrhs, cols);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -2541,19 +2572,19 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         if (f1 > f0) {
             const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
             hipLaunchKernelGGL(k_bwd_gather<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
-                               P.pat_ptr.p, P.pat_rows.p, P.vbase.p, Y, V, n, nrhs, P.front_sys.p, act);
+                               P.pat_ptr.p, P.pat_rows.p, P.vbase.p, Y, V, n, nrhs, P.front_sys.p, cols);
             hipLaunchKernelGGL(k_bwd_foot_part<T>, dim3(f1 - f0), dim3(256), 0, st, P.foot_tasks.p + f0, arena, P.f.p,
-                               P.w.p, P.off.p, P.vbase.p, V, part, nrhs, P.front_sys.p, act);
+                               P.w.p, P.off.p, P.vbase.p, V, part, nrhs, P.front_sys.p, cols);
         }
         const int r0 = P.lvl_footr_ptr[L], r1 = P.lvl_footr_ptr[L + 1];
         if (r1 > r0)
             hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p,
-                               P.first.p, N.diag.p, Y, part, Z, n, nrhs, P.front_sys.p, act);
+                               P.first.p, N.diag.p, Y, part, Z, n, nrhs, P.front_sys.p, cols);
         for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
             const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
             if (s1 > s0)
                 hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, Y, Z, q,
-                                   n, nrhs, act);
+                                   n, nrhs, cols);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
@@ -2563,7 +2594,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                                : (c == 1 ? k_bwd_small<T, 16> : (c == 2 ? k_bwd_small<T, 32> : k_bwd_small<T, 64>));
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
                                P.small_nodes.p + m0, m1 - m0, P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p,
-                               arena, P.off.p, N.diag.p, Y, n, nrhs, P.front_sys.p, act);
+                               arena, P.off.p, N.diag.p, Y, n, nrhs, P.front_sys.p, cols);
         }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
@@ -2582,7 +2613,7 @@ static void ensure_ws(Numeric &N, int nrhs) {
     N.wsR.alloc((size_t)n * nrhs);
     N.wsPart.alloc((size_t)residual_blocks(n, N.A.lanes) * nrhs * 2);
     N.wsRel.alloc((size_t)nrhs * std::max(N.plan->nsys, 1));
-    N.wsAct.alloc((size_t)std::max(N.plan->nsys, 1));
+    N.wsCols.alloc((size_t)std::max(N.plan->nsys, 1));
     N.ws_nrhs = nrhs;
 }
 
@@ -2632,11 +2663,16 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     if (N.n == 0 || nrhs == 0) return;
     if (ldb < N.n || ldx < N.n) throw Error(MG_E_ARG, "The right-hand side length does not match the matrix size.");
     ensure_ws(N, nrhs);
-    auto once = [&](const double *rhs, int ldr, int acc, const unsigned char *act) {
-        if (N.precision == MG_FP32) run_solve_once<float>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, act);
-        else run_solve_once<double>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, act);
+    auto once = [&](const double *rhs, int ldr, int acc, const int *cols) {
+        if (N.precision == MG_FP32) run_solve_once<float>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
+        else run_solve_once<double>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
     };
-    once(d_b, ldb, 0, nullptr);
+    const int nsys = N.plan->nsys;
+    // in a batch, every system's solves stop at its last nonzero right-hand side (uneven
+    // counts padded to nrhs columns): the columns past it stay zero
+    if (nsys > 1)
+        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(256), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, N.wsCols.p);
+    once(d_b, ldb, 0, nsys > 1 ? N.wsCols.p : nullptr);
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
     residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
@@ -2647,27 +2683,22 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     };
     double worst = worst_of(rel);
     int steps = 0;
-    const int nsys = N.plan->nsys;
     while (steps < max_refine && !(worst <= tol)) {
-        // in a batch, a correction solve leaves out the systems whose every column has
-        // converged: their residual rows are zeroed and their fronts skip all work
-        const unsigned char *act = nullptr;
+        // in a batch, a correction solve covers each system's columns up to its last
+        // unconverged one (none for a converged system): the residual of the columns left
+        // out is zeroed, so their correction is zero
+        const int *cols = nullptr;
         if (nsys > 1) {
-            N.h_act.assign(nsys, 0);
-            int nact = 0;
-            for (int t = 0; t < nsys; ++t) {
+            N.h_cols.assign(nsys, 0);
+            for (int t = 0; t < nsys; ++t)
                 for (int c = 0; c < nrhs; ++c)
-                    if (!(rel[(size_t)t * nrhs + c] <= tol)) N.h_act[t] = 1;
-                nact += N.h_act[t];
-            }
-            if (nact < nsys) {
-                MG_HIP(hipMemcpyAsync(N.wsAct.p, N.h_act.data(), nsys, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_zero_done, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, N.n,
-                                   N.plan->sys_begin.p, N.wsAct.p);
-                act = N.wsAct.p;
-            }
+                    if (!(rel[(size_t)t * nrhs + c] <= tol)) N.h_cols[t] = c + 1;
+            MG_HIP(hipMemcpyAsync(N.wsCols.p, N.h_cols.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_zero_past, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, N.n, N.plan->sys_begin.p,
+                               N.wsCols.p);
+            cols = N.wsCols.p;
         }
-        once(N.wsR.p, N.n, 1, act);
+        once(N.wsR.p, N.n, 1, cols);
         ++steps;
         residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
         worst = worst_of(rel);
