@@ -72,6 +72,31 @@ def test_converged_systems_leave_the_refinement():
             np.testing.assert_array_equal(xs[i], x1[i])
 
 
+def test_uneven_and_zero_right_hand_sides():
+    # 1..5 right-hand sides per system (padded to 5 slots on the device) and one all-zero
+    # column inside a system's count: the solves stop at each system's last nonzero column, and
+    # in the refinement at its last unconverged one; every solution meets the tolerance and the
+    # zero column's solution is zero
+    sysl = small_batch(count=9, seed=11)
+    mats = [M.SymmetricCSRMatrix.from_coo(n, r, c, v) for n, r, c, v, _ in sysl]
+    batch = M.UtduBatch(mats, M.UtduSolverOptions(precision=M.FP32))
+    batch.factorize()
+    g = np.random.default_rng(4)
+    counts = [1, 5, 3, 2, 4, 1, 5, 3, 2]
+    bs = [g.uniform(-1, 1, (n, k)) for (n, *_), k in zip(sysl, counts)]
+    bs[1][:, 2] = 0.0
+    xs = batch.solve(bs, tol=1e-10, max_refine=30)
+    assert batch.last_report["converged"] and batch.last_report["refinement_steps"] >= 1
+    for (n, r, c, v, _), b, x in zip(sysl, bs, xs):
+        oc = O.Csr.from_coo(n, r, c, v)
+        for k in range(b.shape[1]):
+            nb = np.linalg.norm(b[:, k])
+            if nb == 0.0:
+                assert not np.any(x[:, k])
+            else:
+                assert np.linalg.norm(oc.spmv(x[:, k]) - b[:, k]) <= 1.0001e-10 * nb
+
+
 def test_pivot_scale_is_per_system():
     # system 0 is singular ([[1,1],[1,1]] (+) 2 (+) 3: one regularized pivot); system 1 is an SPD
     # matrix of the same pattern times 1e14.  With one scale for the whole batch (4e14) every
