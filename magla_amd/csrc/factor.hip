@@ -112,6 +112,7 @@ struct DevicePlan {
     int nsys = 1;                   // independent systems merged into this plan (batched solves)
     std::vector<int> h_sys_begin;   // first row of each system, nsys+1
     DevArray<int> front_sys, sys_begin;
+    DevArray<int> row_sys;  // batches: system of every row (elimination order = input order per system)
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks;
@@ -442,6 +443,12 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     if (!upload) return P;
     P->front_sys.upload(fsys);
     P->sys_begin.upload(P->h_sys_begin);
+    if (P->nsys > 1) {
+        std::vector<int> rsys(n);
+        for (int t = 0; t < P->nsys; ++t)
+            for (int i = P->h_sys_begin[t]; i < P->h_sys_begin[t + 1]; ++i) rsys[i] = t;
+        P->row_sys.upload(rsys);
+    }
     P->f.upload(P->h_f);
     P->w.upload(P->h_w);
     P->first.upload(P->h_first);
@@ -1868,7 +1875,8 @@ template <int LANES, int kResCols, bool kNorms>
 __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, const int *__restrict__ ci,
                                                   const double *__restrict__ v, int n, int nrhs,
                                                   const double *__restrict__ B, int ldb, const double *__restrict__ X,
-                                                  int ldx, double *__restrict__ R, double *__restrict__ part) {
+                                                  int ldx, double *__restrict__ R, double *__restrict__ part,
+                                                  const int *__restrict__ row_sys, const int *__restrict__ cols) {
     __shared__ double sr[kResCols][256 / 64], sb[kResCols][256 / 64];
     constexpr int RPB = 256 / LANES;  // rows per block pass
     const int sub = threadIdx.x % LANES;
@@ -1878,6 +1886,10 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
 #pragma unroll
         for (int c = 0; c < kResCols; ++c) rr[c] = bb[c] = 0.0;
         for (int row = blockIdx.x * RPB + threadIdx.x / LANES; row < n; row += gridDim.x * RPB) {
+            // batches: the columns past the row's system count are left alone (zero, or
+            // converged and unchanged; the norms of those columns are not recomputed)
+            const int ncr = cols ? min(nc, cols[row_sys[row]] - c0) : nc;
+            if (ncr <= 0) continue;  // the row's LANES lanes together
             double acc[kResCols];
 #pragma unroll
             for (int c = 0; c < kResCols; ++c) acc[c] = 0.0;
@@ -1887,7 +1899,7 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
                 const int64_t col = ci[k];
                 // columns past nc re-read the last one (unconditional loads issue together)
 #pragma unroll
-                for (int c = 0; c < kResCols; ++c) acc[c] += a * X[(int64_t)(c0 + min(c, nc - 1)) * ldx + col];
+                for (int c = 0; c < kResCols; ++c) acc[c] += a * X[(int64_t)(c0 + min(c, ncr - 1)) * ldx + col];
             }
 #pragma unroll
             for (int c = 0; c < kResCols; ++c)
@@ -1895,7 +1907,7 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
             if (sub == 0) {
 #pragma unroll
                 for (int c = 0; c < kResCols; ++c)
-                    if (c < nc) {
+                    if (c < ncr) {
                         const double b = B[(int64_t)(c0 + c) * ldb + row];
                         const double res = b - acc[c];
                         R[(int64_t)(c0 + c) * n + row] = res;
@@ -1962,9 +1974,13 @@ __global__ __launch_bounds__(256) void k_residual_final(const double *__restrict
 __global__ __launch_bounds__(256) void k_residual_seg(const double *__restrict__ R, int n,
                                                       const double *__restrict__ B, int ldb,
                                                       const int *__restrict__ sys_begin, int nrhs,
-                                                      double *__restrict__ relres) {
+                                                      double *__restrict__ relres, const int *__restrict__ cols) {
     __shared__ double sr[256], sb[256];
     const int t = blockIdx.x, c = blockIdx.y;
+    if (cols && c >= cols[t]) {  // column left alone: the caller keeps its previous value
+        if (threadIdx.x == 0) relres[(int64_t)t * nrhs + c] = -1.0;
+        return;
+    }
     double rr = 0.0, bb = 0.0;
     for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256) {
         const double r = R[(int64_t)c * n + i], b = B[(int64_t)c * ldb + i];
@@ -2619,7 +2635,7 @@ static void ensure_ws(Numeric &N, int nrhs) {
 
 // relres[c] = ||B - A X||/||B|| per column; R receives the residuals.
 static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const double *d_x, int ldx, hipStream_t st,
-                     std::vector<double> &relres) {
+                     std::vector<double> &relres, const int *cols = nullptr) {
     const int n = N.n;
     const int lanes = N.A.lanes;
     const int nb = residual_blocks(n, lanes);
@@ -2630,10 +2646,10 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
 #define MG_RES(LN)                                                                                                  \
     if (seg)                                                                                                        \
         hipLaunchKernelGGL((k_residual<LN, kResColsMax, false>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, \
-                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p);                             \
+                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p, N.plan->row_sys.p, cols);   \
     else                                                                                                            \
         hipLaunchKernelGGL((k_residual<LN, 8, true>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, \
-                           n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p)
+                           n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p, nullptr, nullptr)
     switch (lanes) {
         case 4: MG_RES(4); break;
         case 8: MG_RES(8); break;
@@ -2645,12 +2661,17 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
     const int nsys = N.plan->nsys;
     if (nsys > 1)
         hipLaunchKernelGGL(k_residual_seg, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, n, d_b, ldb,
-                           N.plan->sys_begin.p, nrhs, N.wsRel.p);
+                           N.plan->sys_begin.p, nrhs, N.wsRel.p, cols);
     else
         hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, N.wsPart.p, nb, N.wsRel.p);
+    std::vector<double> prev;
+    prev.swap(relres);
     relres.resize((size_t)nrhs * nsys);
     MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * relres.size(), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
+    // columns left alone keep their previous value (zero before any: those columns are zero)
+    for (size_t e = 0; e < relres.size(); ++e)
+        if (relres[e] < 0.0) relres[e] = e < prev.size() ? prev[e] : 0.0;
 }
 
 // UtduNumericFactorization.Solve (:90-112) for nrhs columns, followed by FP64
@@ -2675,7 +2696,7 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     once(d_b, ldb, 0, nsys > 1 ? N.wsCols.p : nullptr);
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
-    residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
+    residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel, nsys > 1 ? N.wsCols.p : nullptr);
     auto worst_of = [](const std::vector<double> &v) {
         double m = 0.0;
         for (double x : v) m = (std::isnan(x) || std::isnan(m)) ? NAN : std::max(m, x);
@@ -2700,7 +2721,7 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
         }
         once(N.wsR.p, N.n, 1, cols);
         ++steps;
-        residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel);
+        residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel, cols);
         worst = worst_of(rel);
     }
     if (rep) {
