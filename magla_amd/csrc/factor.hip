@@ -74,6 +74,21 @@ struct __align__(8) PanelTask {
     int fc, sys;          // first column, system
 };
 
+// A front of the small-front solve lists with everything the solve kernels read of it, so
+// that the lane groups' data loads depend on this one load.
+struct __align__(16) SmallRec {
+    int f, w, fc, sys;    // size, width, first column, system
+    int64_t off, vbase;   // arena offset, vector base
+    int c0, c1, pat, pad; // children-array range, first pattern row
+    int64_t inv0;         // inverse row map of its first child (the children's maps follow, f entries each)
+    int64_t pad2;
+};
+// A children-array slot: the child's vector base, size and width.
+struct __align__(16) SmallChild {
+    int64_t vb;
+    int f, w;
+};
+
 struct DevicePlan {
     int n = 0, nsuper = 0, nfront = 0, nlevels = 0, max_w = 0, max_f = 0;
     int64_t arena = 0;  // sum f^2 over fronts
@@ -96,6 +111,8 @@ struct DevicePlan {
     std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each),
                                     // per level four classes: f <= 8, <= 16, <= 32, <= 64
     DevArray<int> small_nodes;
+    DevArray<SmallRec> small_recs;     // small_nodes' records, same order
+    DevArray<SmallChild> child_recs;   // per children-array slot
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
 
@@ -250,10 +267,19 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     std::vector<std::vector<int>> bylevel(nl);
     for (int g = 0; g < nf; ++g) bylevel[level[g]].push_back(g);
 
-    // inverse row maps: for child front x of front g, inv_x[i] = index of g's row i in x's footer
+    // inverse row maps: for child front x of front g, inv_x[i] = index of g's row i in x's footer;
+    // laid out in children-array order, so the maps of g's children are consecutive (f_g each)
     std::vector<int64_t> inv_off(nf + 1, 0);
-    for (int g = 0; g < nf; ++g) inv_off[g + 1] = inv_off[g] + (fparent[g] >= 0 ? P->h_f[fparent[g]] : 0);
-    std::vector<int> inv_map(inv_off[nf], -1);
+    int64_t inv_total = 0;
+    for (int g = 0; g < nf; ++g)
+        for (int c = fchild_ptr[g]; c < fchild_ptr[g + 1]; ++c) {
+            inv_off[fchildren[c]] = inv_total;
+            inv_total += P->h_f[g];
+        }
+    for (int g = 0; g < nf; ++g)
+        if (fparent[g] < 0) inv_off[g] = inv_total;  // roots: no map
+    inv_off[nf] = inv_total;
+    std::vector<int> inv_map(inv_total, -1);
     std::vector<int> rowmap(std::max(n, 1), -1);
     for (int g = 0; g < nf; ++g) {
         const int fg = P->h_f[g];
@@ -459,6 +485,21 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->level_nodes.upload(level_nodes);
     P->init_tasks.upload(init);
     P->small_nodes.upload(small);
+    {
+        std::vector<SmallRec> recs(small.size());
+        for (size_t k = 0; k < small.size(); ++k) {
+            const int g = small[k], c0 = fchild_ptr[g], c1 = fchild_ptr[g + 1];
+            recs[k] = SmallRec{P->h_f[g], P->h_w[g], P->h_first[g], fsys[g], P->h_off[g], vbase[g],
+                               c0, c1, P->h_pat_ptr[g], 0, c1 > c0 ? inv_off[fchildren[c0]] : 0, 0};
+        }
+        P->small_recs.upload(recs);
+        std::vector<SmallChild> crec(std::max<size_t>(fchildren.size(), 1));
+        for (size_t c = 0; c < fchildren.size(); ++c) {
+            const int x = fchildren[c];
+            crec[c] = SmallChild{vbase[x], P->h_f[x], P->h_w[x]};
+        }
+        P->child_recs.upload(crec);
+    }
     P->foot_tasks.upload(foot);
     P->footr_tasks.upload(footr);
 
@@ -1412,12 +1453,9 @@ __global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ no
 }
 
 struct SmallFwdArgs {
-    const int *F, *W, *firstcol, *child_ptr, *children;
-    const int64_t *inv_off;
-    const int *inv_map;
-    const int64_t *vbase, *off;
+    const SmallChild *child;   // per children-array slot
+    const int *inv_map;        // a front's children's inverse row maps, consecutive, f entries each
     int n, nrhs;
-    const int *fsys;           // system of each front
     const int *cols;           // per system: leading right-hand sides still to solve (NULL: all)
 };
 
@@ -1425,7 +1463,7 @@ struct SmallFwdArgs {
 // the inverse row maps); y = Li v on the own rows (Li[i][m], m < i, stored at row m,
 // column i; unit diagonal); v -= L y on the footer rows.
 template <typename T, int FR>
-__global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes, int count, SmallFwdArgs a,
+__global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ recs, int count, SmallFwdArgs a,
                                                    const T *__restrict__ arena, T *__restrict__ V,
                                                    T *__restrict__ Y) {
     constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
@@ -1435,16 +1473,16 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     if (wave_first >= count) return;  // whole wave
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
-    const int g = nodes[idx < count ? idx : count - 1];
+    const SmallRec rc = recs[idx < count ? idx : count - 1];
     // right-hand sides past the system's count are zero (padding, or converged in a
     // refinement step): left out, their vectors stay zero; a front with none is left out
     // like a group past the end of the list
-    const int nc = idx < count ? (a.cols ? min(a.nrhs, a.cols[a.fsys[g]]) : a.nrhs) : 0;
+    const int nc = idx < count ? (a.cols ? min(a.nrhs, a.cols[rc.sys]) : a.nrhs) : 0;
     const bool live = nc > 0;
     if (!__any(live)) return;  // whole wave
-    const int f = live ? a.F[g] : 0, w = live ? a.W[g] : 0, fc = a.firstcol[g];
+    const int f = live ? rc.f : 0, w = live ? rc.w : 0, fc = rc.fc;
     const bool own = i < w, foot = i >= w && i < f;
-    const T *M = arena + a.off[g];
+    const T *M = arena + rc.off;
     // own lanes walk column i of the front (stride 1), footer lanes row i (stride f);
     // offsets stay below 64 * 64, so 32-bit per-lane arithmetic
     const int lim = own ? i : (foot ? w : 0);  // entries m < lim are used
@@ -1458,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
     // the first kKeep children's footer offsets stay in registers across right-hand sides
     constexpr int kKeep = 4;
     const int nrhs = a.nrhs, n = a.n;
-    const int c0 = live ? a.child_ptr[g] : 0, c1 = live ? a.child_ptr[g + 1] : 0;
+    const int c0 = live ? rc.c0 : 0, c1 = live ? rc.c1 : 0;
     int64_t cb[kKeep];
     int cf[kKeep];
     bool qany[kKeep];  // some front of the wave has a child in slot q (wave-uniform)
@@ -1468,17 +1506,17 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
         cb[q] = -1;
         cf[q] = 0;
         if (c0 + q < c1 && i < f) {
-            const int x = a.children[c0 + q];
-            const int am = a.inv_map[a.inv_off[x] + i];
-            cf[q] = a.F[x];
-            if (am >= 0) cb[q] = a.vbase[x] * nrhs + a.W[x] + am;
+            const SmallChild ch = a.child[c0 + q];
+            const int am = a.inv_map[rc.inv0 + (int64_t)q * f + i];
+            cf[q] = ch.f;
+            if (am >= 0) cb[q] = ch.vb * nrhs + ch.w + am;
         }
     }
-    T *Vg = V + a.vbase[g] * nrhs;
+    T *Vg = V + rc.vbase * nrhs;
     const int yi = own ? fc + i : fc;  // clamped: other lanes read a valid row, then drop it
     // a missing child's load is clamped into this front's own vector block: an address of the
     // lane's neighbourhood (a fixed address would put every wave of the launch on one line)
-    const int64_t vself = a.vbase[g] * nrhs + min(i, max(f - 1, 0));
+    const int64_t vself = rc.vbase * nrhs + min(i, max(f - 1, 0));
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
     for (int r0 = 0; r0 < nrhs; r0 += KP) {
@@ -1505,9 +1543,9 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
                 const int r = r0 + p;
                 if (r >= nc) break;
                 for (int c = c0 + kKeep; c < c1; ++c) {
-                    const int x = a.children[c];
-                    const int am = a.inv_map[a.inv_off[x] + i];
-                    if (am >= 0) v[p] += V[a.vbase[x] * nrhs + (int64_t)r * a.F[x] + a.W[x] + am];
+                    const SmallChild ch = a.child[c];
+                    const int am = a.inv_map[rc.inv0 + (int64_t)(c - c0) * f + i];
+                    if (am >= 0) v[p] += V[ch.vb * nrhs + (int64_t)r * ch.f + ch.w + am];
                 }
             }
         }
@@ -1536,12 +1574,9 @@ __global__ __launch_bounds__(256) void k_fwd_small(const int *__restrict__ nodes
 // (column i, row j) for w <= j < f: the two index ranges are disjoint, so one register row
 // serves both products, once against x_foot (zero below w) and once against z (zero from w).
 template <typename T, int FR>
-__global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes, int count,
-                                                   const int *__restrict__ F, const int *__restrict__ W,
-                                                   const int *__restrict__ firstcol, const int *__restrict__ pat_ptr,
+__global__ __launch_bounds__(256) void k_bwd_small(const SmallRec *__restrict__ recs, int count,
                                                    const int *__restrict__ pat_rows, const T *__restrict__ arena,
-                                                   const int64_t *__restrict__ off, const double *__restrict__ D,
-                                                   T *__restrict__ Y, int n, int nrhs, const int *__restrict__ fsys,
+                                                   const double *__restrict__ D, T *__restrict__ Y, int n, int nrhs,
                                                    const int *__restrict__ cols) {
     constexpr int GW = SmallCfg<T, FR>::GW, KP = SmallCfg<T, FR>::KP;
     __shared__ __align__(16) T sb[4][2][KP][64];
@@ -1550,14 +1585,14 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
     if (wave_first >= count) return;  // whole wave
     const int i = lane % FR, base = lane - i;
     const int idx = wave_first + lane / FR;
-    const int g = nodes[idx < count ? idx : count - 1];
+    const SmallRec rc = recs[idx < count ? idx : count - 1];
     // right-hand sides past the system's count are zero and left out (as in k_fwd_small)
-    const int nc = idx < count ? (cols ? min(nrhs, cols[fsys[g]]) : nrhs) : 0;
+    const int nc = idx < count ? (cols ? min(nrhs, cols[rc.sys]) : nrhs) : 0;
     const bool live = nc > 0;
     if (!__any(live)) return;  // whole wave
-    const int f = live ? F[g] : 0, w = live ? W[g] : 0, fc = firstcol[g];
+    const int f = live ? rc.f : 0, w = live ? rc.w : 0, fc = rc.fc;
     const bool own = i < w, foot = i >= w && i < f;
-    const T *M = arena + off[g];
+    const T *M = arena + rc.off;
     // j < w: used for i < j (row i, stride f); j >= w: used for j < f (column i, stride 1)
     const int ii = own ? i : 0, flast = max(f - 1, 0);
     T R[FR];
@@ -1568,7 +1603,7 @@ __global__ __launch_bounds__(256) void k_bwd_small(const int *__restrict__ nodes
         R[j] = (own && (j < w ? j > i : j < f)) ? val : T(0);
     }
     const int yi = own ? fc + i : fc;
-    const int row = foot ? pat_rows[pat_ptr[g] + i] : yi;  // clamped to the lane's own row
+    const int row = foot ? pat_rows[rc.pat + i] : yi;  // clamped to the lane's own row
     const T dv = own ? (T)D[fc + i] : T(1);
     T(*b0)[64] = sb[wv][0];
     T(*b1)[64] = sb[wv][1];
@@ -2551,8 +2586,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y);
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
-    const SmallFwdArgs sfa{P.f.p, P.w.p, P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
-                           P.vbase.p, P.off.p, n, nrhs, P.front_sys.p, cols};
+    const SmallFwdArgs sfa{P.child_recs.p, P.inv_map.p, n, nrhs, cols};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
         const double fl = 2.0 * nrhs * (P.lvl_solve_fac[L] - P.lvl_solve_w[L]);
@@ -2566,7 +2600,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
             auto kern = c == 0 ? k_fwd_small<T, 8>
                                : (c == 1 ? k_fwd_small<T, 16> : (c == 2 ? k_fwd_small<T, 32> : k_fwd_small<T, 64>));
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
-                               P.small_nodes.p + m0, m1 - m0, sfa, arena, V, Y);
+                               P.small_recs.p + m0, m1 - m0, sfa, arena, V, Y);
         }
         if (i1 > i0)
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
@@ -2609,8 +2643,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
             auto kern = c == 0 ? k_bwd_small<T, 8>
                                : (c == 1 ? k_bwd_small<T, 16> : (c == 2 ? k_bwd_small<T, 32> : k_bwd_small<T, 64>));
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
-                               P.small_nodes.p + m0, m1 - m0, P.f.p, P.w.p, P.first.p, P.pat_ptr.p, P.pat_rows.p,
-                               arena, P.off.p, N.diag.p, Y, n, nrhs, P.front_sys.p, cols);
+                               P.small_recs.p + m0, m1 - m0, P.pat_rows.p, arena, N.diag.p, Y, n, nrhs, cols);
         }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
