@@ -1459,6 +1459,28 @@ struct SmallFwdArgs {
     const int *cols;           // per system: leading right-hand sides still to solve (NULL: all)
 };
 
+// v[p] = own right-hand side (own rows) plus the footer contributions of child slots q < NQ,
+// for columns r0 + p: straight-line, so every load of the chunk is in flight at once.
+template <typename T, int KP, int NQ>
+__device__ __forceinline__ void fwd_gather(T (&v)[KP], const T *__restrict__ Y, const T *__restrict__ V, int r0,
+                                           int nc, int i, int f, bool own, int n, int yi, const int64_t (&cb)[4],
+                                           const int (&cf)[4], int64_t vself) {
+#pragma unroll
+    for (int p = 0; p < KP; ++p) {
+        const int r = r0 + p;
+        const bool act = r < nc && i < f;
+        const int64_t rr = act ? r : 0;
+        const T yv = Y[rr * n + yi];
+        T acc = (act && own) ? yv : T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : vself + rr * f];
+            acc += (act && cb[q] >= 0) ? cv : T(0);
+        }
+        v[p] = acc;
+    }
+}
+
 // Forward: v = own right-hand-side rows plus the children's footer contributions (through
 // the inverse row maps); y = Li v on the own rows (Li[i][m], m < i, stored at row m,
 // column i; unit diagonal); v -= L y on the footer rows.
@@ -1499,10 +1521,13 @@ __global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ 
     const int c0 = live ? rc.c0 : 0, c1 = live ? rc.c1 : 0;
     int64_t cb[kKeep];
     int cf[kKeep];
-    bool qany[kKeep];  // some front of the wave has a child in slot q (wave-uniform)
+    // child slots some front of the wave uses (wave-uniform): picks a straight-line gather
+    int nq = 0;
+#pragma unroll
+    for (int q = 0; q < kKeep; ++q)
+        if (__any(c0 + q < c1)) nq = q + 1;
 #pragma unroll
     for (int q = 0; q < kKeep; ++q) {
-        qany[q] = __any(c0 + q < c1);
         cb[q] = -1;
         cf[q] = 0;
         if (c0 + q < c1 && i < f) {
@@ -1522,20 +1547,12 @@ __global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ 
     for (int r0 = 0; r0 < nrhs; r0 += KP) {
         if (!__any(r0 < nc)) break;  // wave-uniform
         T v[KP], y[KP];
-#pragma unroll
-        for (int p = 0; p < KP; ++p) {
-            const int r = r0 + p;
-            const bool act = r < nc && i < f;
-            const int64_t rr = act ? r : 0;
-            const T yv = Y[rr * n + yi];
-            T acc = (act && own) ? yv : T(0);
-#pragma unroll
-            for (int q = 0; q < kKeep; ++q) {
-                if (!qany[q]) break;
-                const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : vself + rr * f];
-                acc += (act && cb[q] >= 0) ? cv : T(0);
-            }
-            v[p] = acc;
+        switch (nq) {
+        case 0: fwd_gather<T, KP, 0>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
+        case 1: fwd_gather<T, KP, 1>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
+        case 2: fwd_gather<T, KP, 2>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
+        case 3: fwd_gather<T, KP, 3>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
+        default: fwd_gather<T, KP, 4>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
         }
         if (c1 - c0 > kKeep && i < f) {
 #pragma unroll
