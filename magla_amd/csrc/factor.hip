@@ -83,10 +83,10 @@ struct __align__(16) SmallRec {
     int64_t inv0;         // inverse row map of its first child (the children's maps follow, f entries each)
     int64_t pad2;
 };
-// A children-array slot: the child's vector base, size and width.
+// A children-array slot: the child's vector base, arena offset, size and width.
 struct __align__(16) SmallChild {
-    int64_t vb;
-    int f, w;
+    int64_t vb, off;
+    int f, w, pad0, pad1;
 };
 
 struct DevicePlan {
@@ -496,7 +496,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         std::vector<SmallChild> crec(std::max<size_t>(fchildren.size(), 1));
         for (size_t c = 0; c < fchildren.size(); ++c) {
             const int x = fchildren[c];
-            crec[c] = SmallChild{vbase[x], P->h_f[x], P->h_w[x]};
+            crec[c] = SmallChild{vbase[x], P->h_off[x], P->h_f[x], P->h_w[x], 0, 0};
         }
         P->child_recs.upload(crec);
     }
@@ -612,12 +612,8 @@ __global__ __launch_bounds__(256) void k_assemble_pull(const int4 *__restrict__ 
 // k_assemble_pull does for 64x64 tiles (the same sums in the same child order): lane i builds
 // row i of the front's lower triangle, 64 / FR fronts per wave.
 template <typename T, int FR>
-__global__ __launch_bounds__(256) void k_assemble_small(const int *__restrict__ nodes, int count, T *__restrict__ arena,
-                                                        const int *__restrict__ F, const int *__restrict__ W,
-                                                        const int64_t *__restrict__ off,
-                                                        const int *__restrict__ child_ptr,
-                                                        const int *__restrict__ children,
-                                                        const int64_t *__restrict__ inv_off,
+__global__ __launch_bounds__(256) void k_assemble_small(const SmallRec *__restrict__ recs, int count,
+                                                        T *__restrict__ arena, const SmallChild *__restrict__ child,
                                                         const int *__restrict__ inv_map) {
     constexpr int GW = 64 / FR;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -626,20 +622,20 @@ __global__ __launch_bounds__(256) void k_assemble_small(const int *__restrict__ 
     const int i = lane % FR;
     const int idx = wave_first + lane / FR;
     if (idx >= count) return;  // no barriers or wave-level syncs below
-    const int g = nodes[idx];
-    const int f = F[g];
+    const SmallRec rc = recs[idx];
+    const int f = rc.f;
     if (i >= f) return;
     T acc[FR];
 #pragma unroll
     for (int j = 0; j < FR; ++j) acc[j] = T(0);
-    const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
-    for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
-        const int ch = children[c];
-        const int *inv = inv_map + inv_off[ch];
+    for (int c = rc.c0; c < rc.c1; ++c) {  // children in order: each sum keeps the child order
+        // the children's inverse row maps follow each other, f entries each
+        const SmallChild ch = child[c];
+        const int *inv = inv_map + rc.inv0 + (int64_t)(c - rc.c0) * f;
         const int b = inv[i];
         if (b < 0) continue;
-        const int fc = F[ch], wc = W[ch];
-        const T *U = arena + off[ch] + wc + b;
+        const int fc = ch.f, wc = ch.w;
+        const T *U = arena + ch.off + wc + b;
         // unconditional loads (indices clamped to the lane's row), masked afterwards
         int a[FR];
 #pragma unroll
@@ -650,7 +646,7 @@ __global__ __launch_bounds__(256) void k_assemble_small(const int *__restrict__ 
 #pragma unroll
         for (int j = 0; j < FR; ++j) acc[j] = (j <= i && a[j] >= 0) ? acc[j] + u[j] : acc[j];
     }
-    T *P = arena + off[g];
+    T *P = arena + rc.off;
 #pragma unroll
     for (int j = 0; j < FR; ++j)
         if (j <= i) P[(int64_t)j * f + i] = acc[j];
@@ -2322,8 +2318,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             const int per_block = 4 * (8 >> c);
             auto kern = c == 0 ? k_assemble_small<T, 8> : (c == 1 ? k_assemble_small<T, 16> : k_assemble_small<T, 32>);
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
-                               P.small_nodes.p + m0, m1 - m0, arena, P.f.p, P.w.p, P.off.p, P.child_ptr.p,
-                               P.children.p, P.inv_off.p, P.inv_map.p);
+                               P.small_recs.p + m0, m1 - m0, arena, P.child_recs.p, P.inv_map.p);
         }
         const int64_t na = P.lvl_asm_ptr[L + 1] - P.lvl_asm_ptr[L];
         if (na > 0) {
