@@ -1230,13 +1230,16 @@ __global__ __launch_bounds__(256) void k_pivot_stats_final(const double *__restr
 
 // ---- solves ----------------------------------------------------------------
 
-// Y[i] = (T) B[perm[i]] (elimination order), nrhs columns
+// Y[i] = (T) B[perm[i]] (elimination order), nrhs columns.  Batches (cols non-null, systems
+// contiguous and permuted within their own rows): a column at or past the row's system count
+// is left out, and no solve kernel reads it.
 template <typename T>
 __global__ void k_permute_in(const double *__restrict__ B, int ldb, const int *__restrict__ perm, int n, int nrhs,
-                             T *__restrict__ Y) {
+                             T *__restrict__ Y, const int *__restrict__ row_sys, const int *__restrict__ cols) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * nrhs) return;
     const int r = (int)(e / n), i = (int)(e % n);
+    if (cols && r >= cols[row_sys[i]]) return;
     Y[e] = (T)B[(int64_t)r * ldb + perm[i]];
 }
 
@@ -1244,12 +1247,19 @@ __global__ void k_permute_in(const double *__restrict__ B, int ldb, const int *_
 // on the (narrower) elimination-order vector
 template <typename T>
 __global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ iperm, int n, int nrhs,
-                              double *__restrict__ X, int ldx, int accumulate) {
+                              double *__restrict__ X, int ldx, int accumulate, const int *__restrict__ row_sys,
+                              const int *__restrict__ cols) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * nrhs) return;
     const int r = (int)(e / n), j = (int)(e % n);
-    const double v = (double)Xe[(int64_t)r * n + iperm[j]];
     double *dst = X + (int64_t)r * ldx + j;
+    // batches: a column past the system's count has a zero right-hand side (a first solve:
+    // the solution is zero) or a zero residual (a correction: nothing to add)
+    if (cols && r >= cols[row_sys[j]]) {
+        if (!accumulate) *dst = 0.0;
+        return;
+    }
+    const double v = (double)Xe[(int64_t)r * n + iperm[j]];
     if (accumulate) *dst += v;
     else *dst = v;
 }
@@ -2595,7 +2605,9 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     T *V = reinterpret_cast<T *>(N.wsU.p);
     const T *arena = reinterpret_cast<const T *>(N.arena.p);
     const int64_t tot = (int64_t)n * nrhs;
-    hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y);
+    const int *rsys = cols ? P.row_sys.p : nullptr;  // cols: batches only
+    hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y,
+                       rsys, cols);
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
     const SmallFwdArgs sfa{P.child_recs.p, P.inv_map.p, n, nrhs, cols};
@@ -2659,7 +2671,7 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         }
     }
     hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
-                       ldx, accumulate);
+                       ldx, accumulate, rsys, cols);
     MG_HIP(hipGetLastError());
 }
 
