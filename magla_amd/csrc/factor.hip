@@ -1690,7 +1690,6 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__
     __shared__ T yv[kPanel][kRhsChunk + 1];   // yv[k][rr] = y[k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
-    if (nc == 0) return;
     const int rb = tk.blk, f = tk.f, w = tk.w;
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     const T *M = arena + tk.off;
@@ -1706,25 +1705,30 @@ __global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__
 #pragma unroll
         for (int s = 0; s < kPanel / 4; ++s) bl[s] = src[(int64_t)(ps + min(4 * s + lk, nb - 1)) * f];
     }
-    {
-        T ti[kPanel / 4];
-        tile_load<T>(ti, M, f, ps, ps, nb, nb);
-        tile_store_inv<T>(inv, ti, nb);
-    }
-    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nc - c0);
-        T old[4];
+    // a chunk's vectors: the panel rows (tv) and this block's rows (old)
+    T old[4], tv[4];
+    auto load_chunk = [&](int c0, int nr) {
         if (rb >= 0) {
             const int orow = min(r0 + col, f - 1);
 #pragma unroll
             for (int q = 0; q < 4; ++q) old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + orow];
         }
-        {
-            T tv[4];
-            vec_load<T>(tv, Vg + (int64_t)c0 * f + ps, f, nb, nr);
-            __syncthreads();
-            vec_store<T>(vv, tv, nb, nr);
-        }
+        vec_load<T>(tv, Vg + (int64_t)c0 * f + ps, f, nb, nr);
+    };
+    // the first chunk in flight with the tile loads, before the count arrives (columns past it
+    // lie inside the workspace and are masked when stored)
+    load_chunk(0, min(kRhsChunk, nrhs));
+    {
+        T ti[kPanel / 4];
+        tile_load<T>(ti, M, f, ps, ps, nb, nb);
+        tile_store_inv<T>(inv, ti, nb);
+    }
+    if (nc == 0) return;  // uniform, before any barrier
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        if (c0 > 0) load_chunk(c0, nr);
+        __syncthreads();
+        vec_store<T>(vv, tv, nb, nr);
         __syncthreads();
         typename MfmaAcc<T>::type acc;
 #pragma unroll
@@ -1864,13 +1868,24 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__
     __shared__ T xv[kPanel][kRhsChunk + 1];
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
-    if (nc == 0) return;
     const int cb = tk.blk, f = tk.f, w = tk.w;
     const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
     const int fc = tk.fc;
     const T *M = arena + tk.off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
+    // a chunk's vectors: the panel's z (tz) and block cb's old values (cb < q: a full
+    // 64-column panel)
+    T old[4], tz[4];
+    auto load_chunk = [&](int c0, int nr) {
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+            old[qq] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, qq), nr - 1)) * n + fc + min(cb * kPanel + col, w - 1)];
+        vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
+    };
+    // the first chunk in flight with the tile loads, before the count arrives (columns past it
+    // lie inside the workspace and are masked when stored)
+    load_chunk(0, min(kRhsChunk, nrhs));
     {
         T ti[kPanel / 4], tb[kPanel / 4];
         tile_load<T>(ti, M, f, ps, ps, nb, nb);
@@ -1878,19 +1893,12 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__
         tile_store_inv<T>(inv, ti, nb);
         tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);
     }
+    if (nc == 0) return;  // uniform, before any barrier
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        // block cb's old values, read with the panel's (cb < q: a full 64-column panel)
-        T old[4];
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq)
-            old[qq] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, qq), nr - 1)) * n + fc + min(cb * kPanel + col, w - 1)];
-        {
-            T tz[4];
-            vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
-            __syncthreads();
-            vec_store<T>(zz, tz, nb, nr);
-        }
+        if (c0 > 0) load_chunk(c0, nr);
+        __syncthreads();
+        vec_store<T>(zz, tz, nb, nr);
         __syncthreads();
         typename MfmaAcc<T>::type acc;
 #pragma unroll
