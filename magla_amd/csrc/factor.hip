@@ -1560,16 +1560,19 @@ __global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ 
         case 3: fwd_gather<T, KP, 3>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
         default: fwd_gather<T, KP, 4>(v, Y, V, r0, nc, i, f, own, n, yi, cb, cf, vself); break;
         }
-        if (c1 - c0 > kKeep && i < f) {
+        if (c1 - c0 > kKeep && i < f && r0 < nc) {
+            // children past the first kKeep, in order: per child its record and map entry, then
+            // the chunk's values together (columns clamped to the last one, masked)
+            for (int c = c0 + kKeep; c < c1; ++c) {
+                const SmallChild ch = a.child[c];
+                const int am = a.inv_map[rc.inv0 + (int64_t)(c - c0) * f + i];
+                if (am < 0) continue;
+                const T *src = V + ch.vb * nrhs + ch.w + am;
+                T u[KP];
 #pragma unroll
-            for (int p = 0; p < KP; ++p) {
-                const int r = r0 + p;
-                if (r >= nc) break;
-                for (int c = c0 + kKeep; c < c1; ++c) {
-                    const SmallChild ch = a.child[c];
-                    const int am = a.inv_map[rc.inv0 + (int64_t)(c - c0) * f + i];
-                    if (am >= 0) v[p] += V[ch.vb * nrhs + (int64_t)r * ch.f + ch.w + am];
-                }
+                for (int p = 0; p < KP; ++p) u[p] = src[(int64_t)min(r0 + p, nc - 1) * ch.f];
+#pragma unroll
+                for (int p = 0; p < KP; ++p) v[p] += r0 + p < nc ? u[p] : T(0);
             }
         }
 #pragma unroll
