@@ -1465,24 +1465,41 @@ struct SmallFwdArgs {
     const int *cols;           // per system: leading right-hand sides still to solve (NULL: all)
 };
 
+// Makes a loaded value opaque to the compiler.  Applied after the last load of a batch is
+// issued, it keeps every load of the batch unconditional: a select between a loaded value and
+// zero is otherwise turned into a branch around its load, with a wait for that one load.
+template <typename T>
+__device__ __forceinline__ void pin(T &x) {
+    asm volatile("" : "+v"(x));
+}
+
 // v[p] = own right-hand side (own rows) plus the footer contributions of child slots q < NQ,
 // for columns r0 + p: straight-line, so every load of the chunk is in flight at once.
 template <typename T, int KP, int NQ>
 __device__ __forceinline__ void fwd_gather(T (&v)[KP], const T *__restrict__ Y, const T *__restrict__ V, int r0,
                                            int nc, int i, int f, bool own, int n, int yi, const int64_t (&cb)[4],
                                            const int (&cf)[4], int64_t vself) {
+    T yl[KP], cl[KP][NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int p = 0; p < KP; ++p) {
         const int r = r0 + p;
-        const bool act = r < nc && i < f;
-        const int64_t rr = act ? r : 0;
-        const T yv = Y[rr * n + yi];
-        T acc = (act && own) ? yv : T(0);
+        const int64_t rr = (r < nc && i < f) ? r : 0;
+        yl[p] = Y[rr * n + yi];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T cv = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : vself + rr * f];
-            acc += (act && cb[q] >= 0) ? cv : T(0);
-        }
+        for (int q = 0; q < NQ; ++q) cl[p][q] = V[cb[q] >= 0 ? cb[q] + rr * cf[q] : vself + rr * f];
+    }
+#pragma unroll
+    for (int p = 0; p < KP; ++p) {
+        pin(yl[p]);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) pin(cl[p][q]);
+    }
+#pragma unroll
+    for (int p = 0; p < KP; ++p) {
+        const bool act = r0 + p < nc && i < f;
+        T acc = (act && own) ? yl[p] : T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc += (act && cb[q] >= 0) ? cl[p][q] : T(0);
         v[p] = acc;
     }
 }
@@ -1571,6 +1588,8 @@ __global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ 
                 T u[KP];
 #pragma unroll
                 for (int p = 0; p < KP; ++p) u[p] = src[(int64_t)min(r0 + p, nc - 1) * ch.f];
+#pragma unroll
+                for (int p = 0; p < KP; ++p) pin(u[p]);
 #pragma unroll
                 for (int p = 0; p < KP; ++p) v[p] += r0 + p < nc ? u[p] : T(0);
             }
