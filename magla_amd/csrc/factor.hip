@@ -1981,7 +1981,7 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
             double acc[kResCols];
 #pragma unroll
             for (int c = 0; c < kResCols; ++c) acc[c] = 0.0;
-#pragma unroll 2
+#pragma unroll 1
             for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
                 const double a = v[k];
                 const int64_t col = ci[k];
