@@ -1957,7 +1957,7 @@ __global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__
 // sums of R^2 and B^2 for the relative residual: part[(col * gridDim.x + block) * 2 + {0,1}].
 // One row per LANES-lane group (grid = n * LANES / 256 blocks), so the partial sums and
 // their order depend only on n and LANES.
-constexpr int kResColsMax = 16;
+constexpr int kResColsMax = 8;
 static inline int residual_blocks(int n, int lanes) { return (int)(((int64_t)n * lanes + 255) / 256); }
 template <int LANES, int kResCols, bool kNorms>
 __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, const int *__restrict__ ci,
@@ -2105,16 +2105,33 @@ __global__ __launch_bounds__(256) void k_zero_past(double *__restrict__ R, int n
 // cols[t] = 1 + the last column with a nonzero right-hand-side entry on system t's rows (0 if
 // none): the columns past it are zero (padding of uneven right-hand-side counts) and their
 // solutions are zero, so the solves leave them out.
-__global__ __launch_bounds__(256) void k_sys_cols(const double *__restrict__ B, int ldb, int nrhs,
-                                                  const int *__restrict__ sys_begin, int *__restrict__ cols) {
+// Four columns per step, last first: each thread's loads of a step are issued together.
+__global__ __launch_bounds__(1024) void k_sys_cols(const double *__restrict__ B, int ldb, int nrhs,
+                                                   const int *__restrict__ sys_begin, int *__restrict__ cols) {
+    __shared__ int top;
     const int t = blockIdx.x, r0 = sys_begin[t], r1 = sys_begin[t + 1];
-    for (int c = nrhs - 1; c >= 0; --c) {
-        int nz = 0;
-        for (int i = r0 + threadIdx.x; i < r1; i += 256) nz |= B[(int64_t)c * ldb + i] != 0.0;
-        if (__syncthreads_or(nz)) {
-            if (threadIdx.x == 0) cols[t] = c + 1;
-            return;
+    for (int c1 = nrhs; c1 > 0; c1 -= 4) {
+        const int c0 = max(c1 - 4, 0);
+        int hi = 0;  // one past the highest nonzero column of [c0, c1) among this thread's rows
+#pragma unroll 2
+        for (int i = r0 + threadIdx.x; i < r1; i += 1024) {
+            double x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = B[(int64_t)min(c0 + u, c1 - 1) * ldb + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (c0 + u < c1 && x[u] != 0.0) hi = max(hi, c0 + u + 1);  // NaN counts as nonzero
         }
+        if (threadIdx.x == 0) top = 0;
+        __syncthreads();
+        if (hi) atomicMax(&top, hi);
+        __syncthreads();
+        const int h = top;
+        if (h) {
+            if (threadIdx.x == 0) cols[t] = h;
+            return;  // uniform
+        }
+        __syncthreads();  // every thread has read top before the next step clears it
     }
     if (threadIdx.x == 0) cols[t] = 0;
 }
@@ -2779,7 +2796,7 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     // in a batch, every system's solves stop at its last nonzero right-hand side (uneven
     // counts padded to nrhs columns): the columns past it stay zero
     if (nsys > 1)
-        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(256), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, N.wsCols.p);
+        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(1024), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, N.wsCols.p);
     once(d_b, ldb, 0, nsys > 1 ? N.wsCols.p : nullptr);
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
