@@ -1230,28 +1230,29 @@ __global__ __launch_bounds__(256) void k_pivot_stats_final(const double *__restr
 
 // ---- solves ----------------------------------------------------------------
 
-// Y[i] = (T) B[perm[i]] (elimination order), nrhs columns.  Batches (cols non-null, systems
-// contiguous and permuted within their own rows): a column at or past the row's system count
-// is left out, and no solve kernel reads it.
+// Y[i] = (T) B[perm[i]] (elimination order), nrhs columns; grid (row blocks of 256, nrhs).
+// Batches (cols non-null, systems contiguous and permuted within their own rows): a column at
+// or past the row's system count is left out, and no solve kernel reads it.
 template <typename T>
-__global__ void k_permute_in(const double *__restrict__ B, int ldb, const int *__restrict__ perm, int n, int nrhs,
-                             T *__restrict__ Y, const int *__restrict__ row_sys, const int *__restrict__ cols) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)n * nrhs) return;
-    const int r = (int)(e / n), i = (int)(e % n);
+__global__ __launch_bounds__(256) void k_permute_in(const double *__restrict__ B, int ldb, const int *__restrict__ perm,
+                                                    int n, T *__restrict__ Y, const int *__restrict__ row_sys,
+                                                    const int *__restrict__ cols, int rbase) {
+    const int i = blockIdx.x * 256 + threadIdx.x, r = rbase + blockIdx.y;
+    if (i >= n) return;
     if (cols && r >= cols[row_sys[i]]) return;
-    Y[e] = (T)B[(int64_t)r * ldb + perm[i]];
+    Y[(int64_t)r * n + i] = (T)B[(int64_t)r * ldb + perm[i]];
 }
 
 // X[j] (+)= Xe[iperm[j]]: the FP64 solution is read and written in order, the gather falls
 // on the (narrower) elimination-order vector
 template <typename T>
-__global__ void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ iperm, int n, int nrhs,
-                              double *__restrict__ X, int ldx, int accumulate, const int *__restrict__ row_sys,
-                              const int *__restrict__ cols) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)n * nrhs) return;
-    const int r = (int)(e / n), j = (int)(e % n);
+__global__ __launch_bounds__(256) void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ iperm, int n,
+                                                     double *__restrict__ X, int ldx, int accumulate,
+                                                     const int *__restrict__ row_sys, const int *__restrict__ cols,
+                                                     int rbase) {
+    // grid: (row blocks of 256, columns rbase .. rbase + gridDim.y - 1)
+    const int j = blockIdx.x * 256 + threadIdx.x, r = rbase + blockIdx.y;
+    if (j >= n) return;
     double *dst = X + (int64_t)r * ldx + j;
     // batches: a column past the system's count has a zero right-hand side (a first solve:
     // the solution is zero) or a zero residual (a correction: nothing to add)
@@ -2651,10 +2652,11 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     T *Y = reinterpret_cast<T *>(N.wsY.p);
     T *V = reinterpret_cast<T *>(N.wsU.p);
     const T *arena = reinterpret_cast<const T *>(N.arena.p);
-    const int64_t tot = (int64_t)n * nrhs;
     const int *rsys = cols ? P.row_sys.p : nullptr;  // cols: batches only
-    hipLaunchKernelGGL(k_permute_in<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y,
-                       rsys, cols);
+    constexpr int kGridY = 65535;  // columns per permute launch
+    for (int rb = 0; rb < nrhs; rb += kGridY)
+        hipLaunchKernelGGL(k_permute_in<T>, dim3((n + 255) / 256, min(kGridY, nrhs - rb)), dim3(256), 0, st, d_rhs, ldr,
+                           P.perm.p, n, Y, rsys, cols, rb);
     T *Z = reinterpret_cast<T *>(N.wsP.p);
     T *part = reinterpret_cast<T *>(N.wsQ.p);
     const SmallFwdArgs sfa{P.child_recs.p, P.inv_map.p, n, nrhs, cols};
@@ -2717,8 +2719,9 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
                                P.small_recs.p + m0, m1 - m0, P.pat_rows.p, arena, N.diag.p, Y, n, nrhs, cols);
         }
     }
-    hipLaunchKernelGGL(k_permute_out<T>, dim3(blocks_for(tot, 256)), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x,
-                       ldx, accumulate, rsys, cols);
+    for (int rb = 0; rb < nrhs; rb += kGridY)
+        hipLaunchKernelGGL(k_permute_out<T>, dim3((n + 255) / 256, min(kGridY, nrhs - rb)), dim3(256), 0, st, Y,
+                           P.iperm.p, n, d_x, ldx, accumulate, rsys, cols, rb);
     MG_HIP(hipGetLastError());
 }
 
