@@ -2744,12 +2744,13 @@ static void ensure_ws(Numeric &N, int nrhs) {
 static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const double *d_x, int ldx, hipStream_t st,
                      std::vector<double> &relres, const int *cols = nullptr) {
     const int n = N.n;
-    const int lanes = N.A.lanes;
+    // one system: 8 right-hand sides per pass over A with the norm partials; a batch takes its
+    // per-system norms from R afterwards, kResColsMax columns per pass, each row only up to its
+    // system's count, and rows of at most 4 lanes' worth of entries take 2 lanes each
+    const bool seg = N.plan->nsys > 1;
+    const int lanes = seg && N.A.lanes == 4 ? 2 : N.A.lanes;
     const int nb = residual_blocks(n, lanes);
     const Matrix &A = N.A;
-    // one system: 8 right-hand sides per pass over A with the norm partials; a batch takes its
-    // per-system norms from R afterwards, and 16 columns per pass (half the passes over A)
-    const bool seg = N.plan->nsys > 1;
 #define MG_RES(LN)                                                                                                  \
     if (seg)                                                                                                        \
         hipLaunchKernelGGL((k_residual<LN, kResColsMax, false>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, \
@@ -2758,6 +2759,7 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
         hipLaunchKernelGGL((k_residual<LN, 8, true>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, \
                            n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p, nullptr, nullptr)
     switch (lanes) {
+        case 2: MG_RES(2); break;
         case 4: MG_RES(4); break;
         case 8: MG_RES(8); break;
         case 16: MG_RES(16); break;
