@@ -778,70 +778,110 @@ struct PanelDiagCtx {
 };
 constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel
 
-// Step K of the right-looking LDL^T of the diagonal block (UtduNumericFactorization.cs:560-600).
-// The wave owning column K applies the pivot rule and publishes the scaled column
-// (l_r = a[r][K] / d_K for r > K, zero for r <= K) and the pivot through LDS; lane K
-// of every wave publishes its part of row K of the inverse (final after step K-1).
-// Both are double-buffered by the parity of K, so one barrier per step suffices.
-// Then every wave updates its 16 columns, a[r][c] -= l_r d_K l_c (l_c = 0 for
-// c <= K, so eliminated columns are left as they are), and applies the same row
-// operation to the inverse, li[r][c] -= l_r li[K][c] (li[K][c] = 0 for c > K).
+// Pivot rule of UtduNumericFactorization.cs:553-577 for a pivot candidate, compared in
+// float64: returns the pivot to use and sets the step's flags.
+__device__ __forceinline__ double pivot_rule(double piv, const PanelDiagCtx &x, unsigned char &fl) {
+    fl = 0;
+    if (!(piv > x.thr)) {
+        if (!x.allow_reg) fl |= 4;
+        if (piv < 0.0) fl |= 2;
+        fl |= 1;
+        piv = x.regv;
+    }
+    return piv;
+}
+
+// Steps K and K + 1 (K even: both columns in one wave) of the right-looking LDL^T of the
+// diagonal block (UtduNumericFactorization.cs:560-600), with one barrier.  Lane r holds row r,
+// wave v columns 16 v .. 16 v + 15, of the block (a) and of its inverse (li).  Step K: the
+// wave owning column K applies the pivot rule and publishes the scaled column
+// (l_r = a[r][K] / d_K for r > K, zero for r <= K) and the pivot through LDS; lane K of every
+// wave publishes its part of row K of the inverse (final after step K - 1).  Every wave then
+// updates its columns, a[r][c] -= (l_r d_K) l_c (l_c = 0 for c <= K, so eliminated columns are
+// left as they are), and applies the same row operation to the inverse,
+// li[r][c] -= l_r li[K][c] (li[K][c] = 0 for c > K).
+// Pairing: before the barrier the owner wave also applies step K to column K + 1 (the same
+// operations the other columns get after it), takes pivot K + 1 and publishes that scaled
+// column too.  After the barrier every wave applies step K, then step K + 1, whose row of the
+// inverse (final after step K) comes from lane K + 1 of the wave itself.  Every value is the
+// one the two steps taken one at a time give.  Buffers alternate by the parity of K / 2, so
+// one barrier per pair suffices.
 template <typename T, int K>
-__device__ __forceinline__ void panel_diag_step(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
+__device__ __forceinline__ void panel_diag_pair(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
                                                 T (*colk)[kPanel], T (*rowk)[kPanel], T *pvs) {
-    constexpr int kw = K / kDiagCols, kj = K % kDiagCols, buf = K & 1;
-    // Steps past the block's width would only pivot the identity padding: skipped (uniform).
+    static_assert(K % 2 == 0 && kDiagCols % 2 == 0, "pairs of columns of one wave");
+    constexpr int kw = K / kDiagCols, kj = K % kDiagCols, pb = (K / 2) & 1;
     if (K >= x.nb) return;
+    const bool two = K + 1 < x.nb;  // uniform
     const int cb = x.v * kDiagCols;
+    T *c0 = colk[2 * pb], *c1 = colk[2 * pb + 1];
     if (x.r == K) {
 #pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) rowk[buf][cb + j] = li[j];
+        for (int j = 0; j < kDiagCols; ++j) rowk[pb][cb + j] = li[j];
     }
     if (x.v == kw) {
-        const T akk = lane_value<T>(a[kj], K);
-        double piv = (double)akk;
-        unsigned char fl = 0;
-        if (K < x.nb && !(piv > x.thr)) {
-            if (!x.allow_reg) fl |= 4;
-            if (piv < 0.0) fl |= 2;
-            fl |= 1;
-            piv = x.regv;
-        }
-        const T pv = (T)piv, pinv = pivot_recip(pv);
-        const bool below = x.r > K;
+        unsigned char fl;
+        const T pv = (T)pivot_rule((double)lane_value<T>(a[kj], K), x, fl), pinv = pivot_recip(pv);
         const T l = a[kj] * pinv;
-        a[kj] = below ? l : a[kj];
-        colk[buf][x.r] = below ? l : T(0);
-        // every lane of the wave stores the same (uniform) values: no branch
-        pvs[buf] = pv;
-        x.dsh[K] = (double)pv;  // written to D after the last step
+        a[kj] = x.r > K ? l : a[kj];
+        const T lb0 = x.r > K ? l : T(0);
+        c0[x.r] = lb0;
+        pvs[2 * pb] = pv;
+        x.dsh[K] = (double)pv;
         x.fsh[K] = fl;
+        if (two) {
+            const T t0 = lb0 * pv;
+            a[kj + 1] -= t0 * lane_value<T>(lb0, K + 1);  // step K on column K + 1
+            unsigned char fl1;
+            const T pv1 = (T)pivot_rule((double)lane_value<T>(a[kj + 1], K + 1), x, fl1),
+                    pinv1 = pivot_recip(pv1);
+            const T l1 = a[kj + 1] * pinv1;
+            a[kj + 1] = x.r > K + 1 ? l1 : a[kj + 1];
+            c1[x.r] = x.r > K + 1 ? l1 : T(0);
+            pvs[2 * pb + 1] = pv1;
+            x.dsh[K + 1] = (double)pv1;
+            x.fsh[K + 1] = fl1;
+        }
     }
     __syncthreads();
-    const T lb = colk[buf][x.r];
-    const T t = lb * pvs[buf];
-    T lc[kDiagCols], rk[kDiagCols];
+    const bool done = two && x.v == kw;  // column K + 1 of the owner wave has step K already
+    {
+        const T lb = c0[x.r];
+        const T t = lb * pvs[2 * pb];
+        T lc[kDiagCols], rk[kDiagCols];
 #pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) {
-        lc[j] = colk[buf][cb + j];
-        rk[j] = rowk[buf][cb + j];
+        for (int j = 0; j < kDiagCols; ++j) {
+            lc[j] = c0[cb + j];
+            rk[j] = rowk[pb][cb + j];
+        }
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) a[j] = (j == kj + 1 && done) ? a[j] : a[j] - t * lc[j];
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
     }
-    // Entries above a lane's diagonal take updates too: they are never read.
+    if (two) {
+        const T lb = c1[x.r];
+        const T t = lb * pvs[2 * pb + 1];
+        T lc[kDiagCols], rk[kDiagCols];
 #pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) a[j] -= t * lc[j];
+        for (int j = 0; j < kDiagCols; ++j) {
+            lc[j] = c1[cb + j];
+            rk[j] = lane_value<T>(li[j], K + 1);
+        }
 #pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
-    // Materialize every row entry at the end of the step (keeps the compiler from
-    // deferring updates across steps).
+        for (int j = 0; j < kDiagCols; ++j) a[j] -= t * lc[j];
+#pragma unroll
+        for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
+    }
 #pragma unroll
     for (int j = 0; j < kDiagCols; ++j) asm volatile("" : "+v"(a[j]), "+v"(li[j]));
 }
 
-template <typename T, int... K>
-__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, K...>, T (&a)[kDiagCols],
+template <typename T, int... P>
+__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, P...>, T (&a)[kDiagCols],
                                                  T (&li)[kDiagCols], const PanelDiagCtx &x, T (*colk)[kPanel],
                                                  T (*rowk)[kPanel], T *pvs) {
-    (panel_diag_step<T, K>(a, li, x, colk, rowk, pvs), ...);
+    (panel_diag_pair<T, 2 * P>(a, li, x, colk, rowk, pvs), ...);
 }
 
 // Row stride of the scaled inverse Bm in LDS: the MFMA operand reads (4 rows x 16
@@ -959,9 +999,9 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
                                                int allow_reg, double *__restrict__ D,
                                                unsigned char *__restrict__ flags) {
     // Rows/columns past nb are padded with the identity (pivot 1, never reported).
-    __shared__ __align__(16) T colk[2][kPanel];
+    __shared__ __align__(16) T colk[4][kPanel];  // two pairs of scaled columns
     __shared__ __align__(16) T rowk[2][kPanel];
-    __shared__ T pvs[2];
+    __shared__ T pvs[4];
     __shared__ double dsh[kPanel];
     __shared__ unsigned char fsh[kPanel];
     // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0 (the MFMA A operand)
@@ -989,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
         a[j] = (r < nb && c < nb && r >= c) ? a[j] : (r == c ? T(1) : T(0));
         li[j] = r == c ? T(1) : T(0);
     }
-    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel>{}, a, li, x, colk, rowk, pvs);
+    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel / 2>{}, a, li, x, colk, rowk, pvs);
     __syncthreads();
     if (rb <= 0) {
         if (threadIdx.x < nb) {
