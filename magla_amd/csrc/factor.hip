@@ -668,7 +668,7 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // pivot > tol*scale (NaN fails), otherwise regularize to reg*scale and count
 // (negative counted separately; bit 2 marks a refused pivot when
 // regularization is off); the comparison is made in float64 for both
-// precisions.  One barrier per pivot: the column is scaled by the reciprocal
+// precisions.  One barrier per pair of pivots: the column is scaled by the reciprocal
 // (:577), the trailing update is a_rc -= (l_r d) l_c, and the inverse
 // Li = L11^{-1} is built alongside (Li <- (I - l_k e_k^T) Li).  Li's strict lower
 // part is stored transposed in the unused upper triangle of the diagonal block
@@ -981,7 +981,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
 }
 
 // Panel LDL^T: every workgroup factors the front's 64x64 diagonal block (four waves, one
-// barrier per pivot, panel_diag_step) and then, for a front with at most kPanelFusedRows row
+// barrier per pair of pivots, panel_diag_pair) and then, for a front with at most kPanelFusedRows row
 // blocks below the panel, turns its 64-row block into L rows, X := X B with
 // B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k (as k_panel_rows).  Those row-block workgroups
 // repeat the diagonal factorization instead of waiting for a launch of its own; the row
