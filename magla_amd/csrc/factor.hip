@@ -751,6 +751,14 @@ __device__ inline double lane_value<double>(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Makes a loaded value opaque to the compiler.  Applied after the last load of a batch is
+// issued, it keeps every load of the batch unconditional: a select between a loaded value and
+// zero is otherwise turned into a branch around its load, with a wait for that one load.
+template <typename T>
+__device__ __forceinline__ void pin(T &x) {
+    asm volatile("" : "+v"(x));
+}
+
 // 1 / x for a pivot: hardware reciprocal refined by Newton steps (one for float,
 // two for double), within an ulp of the rounded quotient and without the branchy
 // IEEE division sequence on the pivot-to-pivot path.
@@ -1435,6 +1443,8 @@ __global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ no
 #pragma unroll
     for (int j = 0; j < FR; ++j) a[j] = M[(int64_t)min(j, ri) * f + ri];
 #pragma unroll
+    for (int j = 0; j < FR; ++j) pin(a[j]);  // all loads issued before the masks
+#pragma unroll
     for (int j = 0; j < FR; ++j) {
         a[j] = (i < f && j <= i) ? a[j] : T(0);
         li[j] = i == j ? T(1) : T(0);
@@ -1506,13 +1516,6 @@ struct SmallFwdArgs {
     const int *cols;           // per system: leading right-hand sides still to solve (NULL: all)
 };
 
-// Makes a loaded value opaque to the compiler.  Applied after the last load of a batch is
-// issued, it keeps every load of the batch unconditional: a select between a loaded value and
-// zero is otherwise turned into a branch around its load, with a wait for that one load.
-template <typename T>
-__device__ __forceinline__ void pin(T &x) {
-    asm volatile("" : "+v"(x));
-}
 
 // v[p] = own right-hand side (own rows) plus the footer contributions of child slots q < NQ,
 // for columns r0 + p: straight-line, so every load of the chunk is in flight at once.
@@ -1683,11 +1686,11 @@ __global__ __launch_bounds__(256) void k_bwd_small(const SmallRec *__restrict__ 
     const int ii = own ? i : 0, flast = max(f - 1, 0);
     T R[FR];
 #pragma unroll
-    for (int j = 0; j < FR; ++j) {
-        const int ix = j < w ? ii + j * f : ii * f + min(j, flast);  // clamped: inside the front
-        const T val = M[ix];
-        R[j] = (own && (j < w ? j > i : j < f)) ? val : T(0);
-    }
+    for (int j = 0; j < FR; ++j) R[j] = M[j < w ? ii + j * f : ii * f + min(j, flast)];  // clamped: inside the front
+#pragma unroll
+    for (int j = 0; j < FR; ++j) pin(R[j]);  // all loads issued before the masks
+#pragma unroll
+    for (int j = 0; j < FR; ++j) R[j] = (own && (j < w ? j > i : j < f)) ? R[j] : T(0);
     const int yi = own ? fc + i : fc;
     const int row = foot ? pat_rows[rc.pat + i] : yi;  // clamped to the lane's own row
     const T dv = own ? (T)D[fc + i] : T(1);
