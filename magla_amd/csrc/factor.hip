@@ -2025,13 +2025,21 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
             double acc[kResCols];
 #pragma unroll
             for (int c = 0; c < kResCols; ++c) acc[c] = 0.0;
+            // the next entry's value and column are loaded with the current entry's X values
+            // (indices clamped to an entry of the matrix, which holds at least one)
+            const int ke = rp[row + 1], klast = max(ke - 1, 0);
+            int k = rp[row] + sub;
+            double a = v[min(k, klast)];
+            int64_t col = ci[min(k, klast)];
 #pragma unroll 1
-            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
-                const double a = v[k];
-                const int64_t col = ci[k];
+            for (; k < ke; k += LANES) {
+                const double an = v[min(k + LANES, klast)];
+                const int64_t cn = ci[min(k + LANES, klast)];
                 // columns past nc re-read the last one (unconditional loads issue together)
 #pragma unroll
                 for (int c = 0; c < kResCols; ++c) acc[c] += a * X[(int64_t)(c0 + min(c, ncr - 1)) * ldx + col];
+                a = an;
+                col = cn;
             }
 #pragma unroll
             for (int c = 0; c < kResCols; ++c)

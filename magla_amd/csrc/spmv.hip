@@ -92,6 +92,7 @@ void Matrix::build(int n_, const int *rowptr, const int *colidx) {
         for (int k = frp[i]; k < frp[i + 1]; ++k)
             if (fci[k] == i) dp[i] = k;
     this->rowptr.upload(frp);
+    if (fci.empty()) fci.push_back(0);  // at least one entry: kernels may read a clamped index
     this->colidx.upload(fci);
     src.upload(fsrc);
     dpos.upload(dp);
