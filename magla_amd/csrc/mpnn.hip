@@ -401,8 +401,21 @@ __global__ __launch_bounds__(256) void k_g_apply(const int *__restrict__ gptr, c
     const int gid = blockIdx.x * 256 + threadIdx.x;
     const int i = gid / kRowLanes, sub = gid % kRowLanes;
     double acc = 0.0;
-    if (i < n)
-        for (int e = gptr[i] + sub; e < gptr[i + 1]; e += kRowLanes) acc += (double)gv[e] * v[gcol[e]];
+    if (i < n) {
+        // the next entry's column and value load with the current entry's v (indices clamped
+        // to an entry of the pattern, which holds at least one)
+        const int ee = gptr[i + 1], elast = max(ee - 1, 0);
+        int e = gptr[i] + sub;
+        int col = gcol[min(e, elast)];
+        G gval = gv[min(e, elast)];
+        for (; e < ee; e += kRowLanes) {
+            const int cn = gcol[min(e + kRowLanes, elast)];
+            const G gn = gv[min(e + kRowLanes, elast)];
+            acc += (double)gval * v[col];
+            col = cn;
+            gval = gn;
+        }
+    }
     for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRowLanes);
     if (i < n && sub == 0) y[i] = acc;
 }
@@ -430,7 +443,19 @@ __global__ __launch_bounds__(256) void k_gt_apply(const int *__restrict__ gtptr,
         if (jacobi) {
             acc = sub == 0 ? invd[j] * r[j] : 0.0;
         } else if (gvt) {
-            for (int t = gtptr[j] + sub; t < gtptr[j + 1]; t += kRowLanes) acc += (double)gvt[t] * y[gtrow[t]];
+            // the next entry's row and value load with the current entry's y (indices clamped
+            // to an entry of the pattern, which holds at least one)
+            const int te = gtptr[j + 1], tlast = max(te - 1, 0);
+            int t = gtptr[j] + sub;
+            int row = gtrow[min(t, tlast)];
+            G gval = gvt[min(t, tlast)];
+            for (; t < te; t += kRowLanes) {
+                const int rn = gtrow[min(t + kRowLanes, tlast)];
+                const G gn = gvt[min(t + kRowLanes, tlast)];
+                acc += (double)gval * y[row];
+                row = rn;
+                gval = gn;
+            }
         } else {
             for (int t = gtptr[j] + sub; t < gtptr[j + 1]; t += kRowLanes) acc += (double)gv[gtidx[t]] * y[gtrow[t]];
         }
