@@ -1330,6 +1330,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks, const int *__restrict__ F,
                                                   const int *__restrict__ W, const int *__restrict__ firstcol,
                                                   const int *__restrict__ child_ptr, const int *__restrict__ children,
+                                                  const SmallChild *__restrict__ child,
                                                   const int64_t *__restrict__ inv_off, const int *__restrict__ inv_map,
                                                   const int64_t *__restrict__ vbase, T *__restrict__ V,
                                                   const T *__restrict__ Y, int n, int nrhs,
@@ -1341,6 +1342,8 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
     const int i = tk.y * 256 + threadIdx.x;
     if (i >= f) return;
     const int c0 = child_ptr[g], c1 = child_ptr[g + 1];
+    // the children's inverse row maps follow each other, f entries each
+    const int64_t inv0 = c1 > c0 ? inv_off[children[c0]] : 0;
     T *Vg = V + vbase[g] * nrhs;
     constexpr int R = 8;  // right-hand sides per pass: their loads are issued together
     const T *yrow = Y + firstcol[g] + min(i, max(w - 1, 0));
@@ -1350,16 +1353,19 @@ __global__ __launch_bounds__(256) void k_fwd_init(const int2 *__restrict__ tasks
 #pragma unroll
         for (int q = 0; q < R; ++q) acc[q] = yrow[(int64_t)(r0 + min(q, nr - 1)) * n];
 #pragma unroll
+        for (int q = 0; q < R; ++q) pin(acc[q]);
+#pragma unroll
         for (int q = 0; q < R; ++q) acc[q] = i < w ? acc[q] : T(0);
         for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
-            const int x = children[c];
-            const int a = inv_map[inv_off[x] + i];
-            // a child's footer row W[x] + max(a, 0) < F[x] (children have footer rows)
-            const T *src = V + vbase[x] * nrhs + W[x] + max(a, 0);
-            const int fx = F[x];
+            const SmallChild ch = child[c];
+            const int a = inv_map[inv0 + (int64_t)(c - c0) * f + i];
+            // a child's footer row w_x + max(a, 0) < f_x (children have footer rows)
+            const T *src = V + ch.vb * nrhs + ch.w + max(a, 0);
             T u[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) u[q] = src[(int64_t)(r0 + min(q, nr - 1)) * fx];
+            for (int q = 0; q < R; ++q) u[q] = src[(int64_t)(r0 + min(q, nr - 1)) * ch.f];
+#pragma unroll
+            for (int q = 0; q < R; ++q) pin(u[q]);
 #pragma unroll
             for (int q = 0; q < R; ++q) acc[q] = a >= 0 ? acc[q] + u[q] : acc[q];
         }
@@ -2728,8 +2734,8 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
         }
         if (i1 > i0)
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
-                               P.first.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p, P.vbase.p, V, Y, n,
-                               nrhs, P.front_sys.p, cols);
+                               P.first.p, P.child_ptr.p, P.children.p, P.child_recs.p, P.inv_off.p, P.inv_map.p,
+                               P.vbase.p, V, Y, n, nrhs, P.front_sys.p, cols);
         for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
             const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
             if (s1 > s0)
