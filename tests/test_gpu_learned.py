@@ -2,7 +2,7 @@
 BF16 network factors and the FP32 gradient (pair term and Kaporin term) against the oracle's
 binary64 restatement (itself pinned by finite differences in test_oracle_mpnn.py), training on
 pairs from the direct solver, the flexible PCG against Jacobi-PCG with the direct solve as the
-judge of the error, and the Jacobi fallback."""
+reference for the error, and the Jacobi fallback."""
 import numpy as np
 import pytest
 
@@ -126,7 +126,7 @@ def test_learned_pcg_needs_fewer_iterations(trained):
     assert not lrn.fell_back
     assert lrn.last_result.iterations < jac.last_result.iterations, (lrn.last_result.iterations,
                                                                       jac.last_result.iterations)
-    # the direct solve judges the error of both
+    # the error of both is measured against the direct solve
     x = M.SymmetricUtduSolver().solve(a, b)
     for xi in (xj, xl):
         assert np.linalg.norm(dm.multiply(xi) - b) <= 1.0001e-8 * np.linalg.norm(b)
