@@ -231,7 +231,9 @@ int mg_bicgstab_solve_dev(mg_gmatrix *m, mg_ilu0 *pre, int32_t use_preconditione
  * one tree level, from every system, go through the same launches; pivot scale and pivot rule
  * are per system (UtduNumericFactorization.cs:411-418,559-572).  Values and right-hand sides
  * are concatenated in system order (offsets from mg_batch_layout); a system with fewer
- * right-hand sides than nrhs passes zero columns. */
+ * right-hand sides than nrhs passes zero columns.  Trailing zero columns cost no solve work:
+ * each system's solves stop at its last nonzero column, and its solution columns past it are
+ * set to zero. */
 typedef struct mg_batch mg_batch;
 int mg_batch_analyze(int32_t count, const int32_t *sizes, const int32_t *const *rowptrs,
                      const int32_t *const *colidxs, const mg_options *opts, mg_batch **out);
