@@ -1120,6 +1120,7 @@ __global__ __launch_bounds__(256) void k_schur(const SchurTask *__restrict__ tas
     };
     typename MfmaAcc<T>::type acc[2][2];
     schur_acc_zero<T>(acc);
+    static_assert(NJ * VEC == 16, "the target tile's 16 values reuse the chunk registers");
     fetch(kb);
     for (int kc = kb; kc < ke; kc += kPanel) {
         const int nk = min(kPanel, ke - kc);
@@ -1147,10 +1148,13 @@ __global__ __launch_bounds__(256) void k_schur(const SchurTask *__restrict__ tas
             }
         }
         __syncthreads();
+        // last chunk (uniform; kb < ke always): the target tile's values go into the chunk
+        // registers, which are free now, and load during the chunk's products
         if (kc + kPanel < ke) fetch(kc + kPanel);
+        else schur_load_tile<T>(*reinterpret_cast<T(*)[16]>(&ra[0][0]), M, f, i0, j0);
         schur_acc_tile<T>(As, Bs, acc);
     }
-    schur_store_tile<T>(acc, M, f, i0, j0, cend);
+    schur_store_tile<T>(acc, *reinterpret_cast<const T(*)[16]>(&ra[0][0]), M, f, i0, j0, cend);
 }
 
 // The same update with 128x128 output tiles (large fronts): K streamed through LDS in chunks

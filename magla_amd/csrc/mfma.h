@@ -99,15 +99,14 @@ __device__ inline void schur_acc_tile(const T (*As)[kSchurLd<T>], const T (*Bs)[
 }
 
 // M[j*f + i] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
-// acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].  All 16 reads
-// are issued before the first write so their latencies overlap.
+// acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].  schur_load_tile issues
+// the 16 reads of the old values together (before the last chunk's products, so that their
+// latency overlaps the matrix cores' work); schur_store_tile subtracts and writes.
 template <typename T>
-__device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2][2], T *__restrict__ M, int f,
-                                        int i0, int j0, int jend) {
+__device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ M, int f, int i0, int j0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = lane >> 4;
-    T old[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -117,9 +116,18 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
                 const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
                 const int ic = min(i, f - 1), jc = min(j, ic);  // clamped: unconditional load
-                const T x = M[(int64_t)jc * f + ic];
-                old[a][b][q] = (i < f && j <= i && j < jend) ? x : T(0);
+                old[(2 * a + b) * 4 + q] = M[(int64_t)jc * f + ic];
             }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) asm volatile("" : "+v"(old[e]));  // loads stay unconditional
+}
+
+template <typename T>
+__device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2][2], const T (&old)[16],
+                                        T *__restrict__ M, int f, int i0, int j0, int jend) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
+    const int l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -128,7 +136,7 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[a][b][q] - acc[a][b][q];
+                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[(2 * a + b) * 4 + q] - acc[a][b][q];
             }
 }
 
