@@ -1082,7 +1082,7 @@ __device__ __forceinline__ int xcd_task(int b, int n) {
 // of the step's target region (schur_step).  One 64x64 output tile per workgroup;
 // K streamed through LDS in chunks of 64, products on MFMA (mfma.h).
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
+__global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
                                                const double *__restrict__ D) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];  // Bs[k][r] = L[j0+r, kc+k]
