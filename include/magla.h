@@ -28,11 +28,11 @@ extern "C" {
 /* Status codes; each maps to the exception the reference throws. */
 typedef enum {
     MG_OK = 0,
-    MG_E_NULL = 1,      /* ArgumentNullException            (SymmetricUtduSolver.cs:188-189,203-204) */
+    MG_E_NULL = 1,      /* ArgumentNullException            (SymmetricUtduSolver.cs:72-73,87-88) */
     MG_E_ARG = 2,       /* ArgumentException                (UtduNumericFactorization.cs:96-98, UtduSymbolicFactorization.cs:306-319) */
-    MG_E_RANGE = 3,     /* ArgumentOutOfRangeException / IndexOutOfRangeException (IterativeLinearSolver.cs:68-75) */
-    MG_E_NOT_PD = 4,    /* InvalidOperationException: non-positive pivot (UtduNumericFactorization.cs:559-566), p'Ap<=0 (ConjugateGradientGaussPreSolver.cs:139-141), zero diagonal (JacobiPreconditioner.cs:224-225) */
-    MG_E_TOO_DENSE = 5, /* InvalidOperationException: front/block too large (UtduNumericFactorization.cs:298-304, FactorStorageLayout.cs:744-747) */
+    MG_E_RANGE = 3,     /* ArgumentOutOfRangeException / IndexOutOfRangeException (IterativeLinearSolver.cs:26-33) */
+    MG_E_NOT_PD = 4,    /* InvalidOperationException: non-positive pivot (UtduNumericFactorization.cs:559-566), p'Ap<=0 (ConjugateGradientGaussPreSolver.cs:139-141), zero diagonal (JacobiPreconditioner.cs:27-28) */
+    MG_E_TOO_DENSE = 5, /* InvalidOperationException: front/block too large (UtduNumericFactorization.cs:298-304, FactorStorageLayout.cs:73-75) */
     MG_E_DEVICE = 6,    /* InvalidOperationException wrapping a HIP error (UtduNumericFactorization.cs:1017-1022) */
     MG_E_INTERNAL = 7,  /* InvalidOperationException: internal symbolic/AMD error (ApproximateMinimumDegreeOrdering.cs:220-223) */
     MG_E_NOMEM = 8      /* OutOfMemoryException */
@@ -129,7 +129,7 @@ int mg_csr_line_cross(int32_t n, const int32_t *rowptr, const int32_t *colidx, d
 
 /* ---- symbolic phase (host) ---- */
 /* UtduSymbolicFactorization.Analyze (UtduSymbolicFactorization.cs:240-295) /
- * SymmetricUtduSolver.Analyze (SymmetricUtduSolver.cs:186-192). */
+ * SymmetricUtduSolver.Analyze (SymmetricUtduSolver.cs:70-76). */
 int mg_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const mg_options *opts,
                mg_symbolic **out);
 /* Permutation / InversePermutation / cost estimates (UtduSymbolicFactorization.cs:126-232);
@@ -146,7 +146,7 @@ int64_t mg_symbolic_estimate_peak_bytes(const mg_symbolic *sym, int32_t worker_c
 void mg_symbolic_release(mg_symbolic *sym);
 
 /* ---- numeric phase (device) ---- */
-/* SymmetricUtduSolver.Factorize(matrix, symbolic) (SymmetricUtduSolver.cs:200-212):
+/* SymmetricUtduSolver.Factorize(matrix, symbolic) (SymmetricUtduSolver.cs:84-96):
  * EnsureCompatible (n and nnz) then the multifrontal factorization on the GPU. */
 int mg_factorize(mg_symbolic *sym, int32_t n, int32_t nnz, const double *values,
                  const mg_options *opts, mg_numeric **out);
@@ -179,7 +179,7 @@ int mg_spmv_dev(mg_matrix *m, int32_t nrhs, const double *d_x, int32_t ldx, doub
 int mg_spmv(mg_matrix *m, const double *x, double *y);
 /* ConjugateGradientGaussPreSolver.SolveWithInitialGuess (CGGP/ConjugateGradientGaussPreSolver.cs:78-166):
  * x0 may be NULL (x0 = 0); max_iterations = 0 means n; use_jacobi selects JacobiPreconditioner
- * (CGGP/JacobiPreconditioner.cs:209-253).  x receives the last iterate. */
+ * (CGGP/JacobiPreconditioner.cs:12-56).  x receives the last iterate. */
 int mg_pcg_solve(mg_matrix *m, const double *b, const double *x0, double relative_tolerance,
                  int32_t max_iterations, int32_t use_jacobi, double *x, mg_pcg_result *result);
 int mg_pcg_solve_dev(mg_matrix *m, const double *d_b, const double *d_x0, double relative_tolerance,

@@ -22,9 +22,10 @@ int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out)
  * must equal i + 16 j (256 values). */
 int mg_test_mfma_layout(int32_t precision, double *out);
 
-/* Device schedule summary computed on the host (no GPU needed): out[0..5] =
+/* Device schedule summary computed on the host (no GPU needed): out[0..6] =
  * fronts (fused supernode chains), levels, arena elements (sum f^2), largest
- * front, widest front, kernel launches per factorization. */
+ * front, widest front, kernel launches per factorization, 64x64 Schur tiles of
+ * the lookahead part (the trailing update the next panels do not touch). */
 int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out);
 /* One v_mfma_f32_16x16x16_bf16 with the operand layout of the learned preconditioner's
  * layer kernel: d = a b for 16x16 row-major a, b (rounded to bf16), d in FP32. */

@@ -5,7 +5,7 @@ benchmark read like the reference's own code:
 
   SymmetricCSRMatrix          SymmetricCSRMatrix.cs (upper triangle + diagonal)
   UtduSolverOptions           UCHOL/UtduSymbolicFactorization.cs:31-98
-  SymmetricUtduSolver         UCHOL/SymmetricUtduSolver.cs:161-225
+  SymmetricUtduSolver         UCHOL/SymmetricUtduSolver.cs:45-109
   UtduSymbolicFactorization   UCHOL/UtduSymbolicFactorization.cs:109-320
   UtduNumericFactorization    UCHOL/UtduNumericFactorization.cs:24-235
   ConjugateGradientGaussPreSolver  CGGP/ConjugateGradientGaussPreSolver.cs:40-196
@@ -217,10 +217,11 @@ def test_mfma_layout(precision):
 
 def plan_stats(symbolic):
     """Device schedule summary (host only): fronts, levels, arena elements, max front, max width,
-    launches per factorization."""
-    out = np.zeros(6, dtype=np.int64)
+    launches per factorization, Schur tiles of the lookahead part."""
+    out = np.zeros(7, dtype=np.int64)
     _check(lib().mg_test_plan_stats(symbolic._h, _ptr(out)))
-    return dict(zip(["fronts", "levels", "arena_elems", "max_front", "max_width", "launches"], out.tolist()))
+    return dict(zip(["fronts", "levels", "arena_elems", "max_front", "max_width", "launches", "lookahead_tiles"],
+                    out.tolist()))
 
 
 def test_factor_block(numeric, supernode, front_size, width):
@@ -521,7 +522,7 @@ class UtduNumericFactorization:
 
 
 class SymmetricUtduSolver:
-    """SymmetricUtduSolver (UCHOL/SymmetricUtduSolver.cs:161-225) on the GPU."""
+    """SymmetricUtduSolver (UCHOL/SymmetricUtduSolver.cs:45-109) on the GPU."""
 
     def __init__(self, options: UtduSolverOptions | None = None):
         self.options = options or UtduSolverOptions()
@@ -551,7 +552,7 @@ class SymmetricUtduSolver:
         return numeric if numeric is not None else UtduNumericFactorization(h, symbolic)
 
     def solve(self, matrix: SymmetricCSRMatrix, rhs):
-        """ILinearSolver.Solve via SolveCore (SymmetricUtduSolver.cs:221-224)."""
+        """ILinearSolver.Solve via SolveCore (SymmetricUtduSolver.cs:105-108)."""
         return self.factorize(matrix).solve(rhs)
 
 

@@ -90,6 +90,18 @@ void fill_symbolic_info(const mg::Symbolic &S, mg_symbolic_info &info) {
     info.matrix_nonzero_count = S.matrix_nnz;
 }
 
+// The device plan (index maps, level schedule) is built with the analysis when a device is
+// visible, so factorizations never mutate the symbolic handle; without a device (host-only
+// use of the symbolic phase) the first factorization builds it.
+void plan_if_device(mg::Symbolic &S) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return;
+    }
+    mg::ensure_plan(S);
+}
+
 // Structural checks the C# types guarantee by construction.
 void check_csr(int n, const int32_t *rowptr, const int32_t *colidx) {
     require(n >= 0, MG_E_ARG, "Matrix size cannot be negative.");
@@ -287,6 +299,7 @@ int mg_analyze(int32_t n, const int32_t *rowptr, const int32_t *colidx, const mg
         check_csr(n, rowptr, colidx);
         auto h = std::make_unique<mg_symbolic>();
         mg::analyze(n, rowptr, colidx, *opts, h->s);
+        plan_if_device(h->s);
         *out = h.release();
     });
 }
@@ -508,6 +521,7 @@ int mg_batch_analyze(int32_t count, const int32_t *sizes, const int32_t *const *
             b->row_begin = b->s.sys_begin;
             b->nnz_begin.assign(count + 1, 0);
             for (int t = 0; t < count; ++t) b->nnz_begin[t + 1] = b->nnz_begin[t] + rowptrs[t][sizes[t]];
+            plan_if_device(b->s);
         } catch (...) {
             delete b;
             throw;

@@ -75,6 +75,8 @@ struct DevicePlan;
 struct Numeric;
 
 std::shared_ptr<DevicePlan> build_plan(const Symbolic &S);
+// Builds and uploads S.plan once (thread-safe); a no-op when it exists.
+void ensure_plan(Symbolic &S);
 int plan_level_count(const DevicePlan &P);
 
 // Factorization: values (upper CSR order, float64) already on the device.

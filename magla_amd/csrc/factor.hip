@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 
 #include "device.h"
 #include "mfma.h"
@@ -540,14 +541,24 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
 
 std::shared_ptr<DevicePlan> build_plan(const Symbolic &S) { return build_plan_host(S, true); }
 
+void ensure_plan(Symbolic &S) {
+    std::lock_guard<std::mutex> lk(*S.plan_mu);
+    if (!S.plan) S.plan = build_plan(S);
+}
+
 // Host-only summary of the device schedule: fronts, levels, arena elements, largest front,
-// widest front, kernel launches per factorization.
+// widest front, kernel launches per factorization, Schur tiles (64x64) of the lookahead part
+// that may overlap the next panels.
 void plan_stats(const Symbolic &S, int64_t *out) {
     auto P = build_plan_host(S, false);
-    int64_t launches = 0;
+    int64_t launches = 0, part1 = 0;
     for (int L = 0; L < P->nlevels; ++L) {
         launches += 2;
-        for (int q = 0; q < P->lvl_npanels[L]; ++q) launches += 3;
+        for (int q = 0; q < P->lvl_npanels[L]; ++q) {
+            launches += 3;
+            const int *ub = &P->lvl_ub64[L][q * kUbStride];
+            part1 += ub[2 * kSchurBuckets] - ub[kSchurBuckets];
+        }
     }
     out[0] = P->nfront;
     out[1] = P->nlevels;
@@ -555,6 +566,7 @@ void plan_stats(const Symbolic &S, int64_t *out) {
     out[3] = P->max_f;
     out[4] = P->max_w;
     out[5] = launches;
+    out[6] = part1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2264,6 +2276,28 @@ __global__ __launch_bounds__(256) void k_diag_scale_seg(const int *__restrict__ 
     if (threadIdx.x == 0) out[t] = sm[0] > 0.0 ? sm[0] : 1.0;
 }
 
+// Device workspace of one solve call: every buffer the solve and refinement kernels use
+// besides the factor, carved out of one block (sub-buffers 256-byte aligned).  The kernels
+// never rely on its contents at entry.
+struct SolveWs {
+    void *Y = nullptr, *V = nullptr, *Z = nullptr, *part = nullptr;  // factor precision
+    double *R = nullptr, *Part = nullptr, *Rel = nullptr;
+    int *Cols = nullptr;  // per system: leading right-hand sides a solve still needs
+};
+// Bytes of the block for nrhs right-hand sides, and the view of a block at base.
+size_t solve_ws_bytes(const Numeric &N, int nrhs);
+SolveWs solve_ws_view(const Numeric &N, int nrhs, void *base);
+
+// A pooled solve workspace: its block and the event recorded after the last solve that used
+// it (a later solve on another stream waits for that event before touching the block).
+struct WsSlot {
+    DevArray<unsigned char> buf;
+    hipEvent_t done = nullptr;
+    ~WsSlot() {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
 struct Numeric {
     std::shared_ptr<DevicePlan> plan;
     const Symbolic *sym = nullptr;
@@ -2277,15 +2311,21 @@ struct Numeric {
     DevArray<double> values;  // upper CSR values (float64), kept for refinement
     Matrix A;                 // full CSR for residuals
     mg_numeric_info info{};
-    // solve workspaces
-    DevArray<unsigned char> wsY, wsU, wsP, wsQ;
-    DevArray<double> wsR, wsX, wsPart, wsRel;
-    DevArray<int> wsCols;     // per system: leading right-hand sides a solve still needs
-    std::vector<int> h_cols;  // host copy (kept alive for the asynchronous upload)
-    int ws_nrhs = 0;
+    // set only when the last factorization into this handle succeeded: a handle whose
+    // factorization threw (e.g. a refused pivot) is never solved with
+    bool factorized = false;
+    // Solve workspaces, one per concurrent solve: the factor itself is read-only during
+    // solves, so solves of one handle from several threads (on their own streams) are
+    // independent (cf. UtduNumericFactorization.cs:101,121, per-call scratch).
+    std::mutex ws_mu;
+    std::vector<std::unique_ptr<WsSlot>> ws_free;
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
     hipStream_t cap_stream = nullptr;
     int graph_precision = -1;
+    // pivot options baked into the captured launches; a refactorization with other values
+    // captures again
+    double graph_tol = 0.0, graph_reg = 0.0;
+    int graph_allow = -1;
     hipStream_t side = nullptr;  // lookahead Schur updates (default priority)
     hipEvent_t ev_fork = nullptr, ev_side[2] = {nullptr, nullptr};
     void ensure_side() {
@@ -2525,7 +2565,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
 void factorize_device(Symbolic &S, const double *d_values, int precision, const mg_options &opts, hipStream_t st,
                       Numeric *&num) {
     if (precision != MG_FP64 && precision != MG_FP32) throw Error(MG_E_ARG, "Unknown factor precision.");
-    if (!S.plan) S.plan = build_plan(S);
+    ensure_plan(S);
     if (S.plan->max_w > 8192)
         throw Error(MG_E_TOO_DENSE, "A supernode wider than 8192 columns is not supported on the device.");
     const size_t esz = precision == MG_FP32 ? sizeof(float) : sizeof(double);
@@ -2554,6 +2594,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->values.alloc(std::max(S.matrix_nnz, 1));
             N->A.build(S.n, S.a_rowptr.data(), S.a_colidx.data());
         }
+        N->factorized = false;
         N->info = mg_numeric_info{};
         N->info.precision = precision;
         if (S.n > 0) {
@@ -2561,7 +2602,9 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                 MG_HIP(hipMemcpyAsync(N->values.p, d_values, sizeof(double) * S.matrix_nnz, hipMemcpyDeviceToDevice, st));
             N->A.set_values_dev(N->values.p, st);
             const bool graph = opts_use_graph(opts) && !profile_on();
-            if (graph && N->gexec && N->graph_precision == precision) {
+            const bool same_opts = N->graph_tol == opts.pivot_tolerance && N->graph_reg == opts.diagonal_regularization &&
+                                   N->graph_allow == opts.allow_diagonal_regularization;
+            if (graph && N->gexec && N->graph_precision == precision && same_opts) {
                 MG_HIP(hipGraphLaunch(N->gexec, st));
             } else {
                 // capture on a private stream (the null stream cannot be captured); the graph
@@ -2603,6 +2646,9 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                     (void)hipGraphDestroy(g);
                     MG_HIP(ie);
                     N->graph_precision = precision;
+                    N->graph_tol = opts.pivot_tolerance;
+                    N->graph_reg = opts.diagonal_regularization;
+                    N->graph_allow = opts.allow_diagonal_regularization;
                     MG_HIP(hipGraphLaunch(N->gexec, st));
                 }
             }
@@ -2625,6 +2671,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->info.largest_pivot_magnitude = mags[1];
         }
         N->info.is_positive_definite = N->info.regularized_pivot_count == 0 && N->info.negative_pivot_count == 0;
+        N->factorized = true;
     } catch (...) {
         if (created) delete N;
         throw;
@@ -2706,20 +2753,20 @@ void numeric_info(const Numeric *N, mg_numeric_info &info, double *diag_host) {
 }
 
 template <typename T>
-static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, double *d_x, int ldx, int accumulate,
-                           hipStream_t st, const int *cols = nullptr) {
-    DevicePlan &P = *N.plan;
+static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_rhs, int ldr, double *d_x,
+                           int ldx, int accumulate, hipStream_t st, const int *cols = nullptr) {
+    const DevicePlan &P = *N.plan;
     const int n = N.n;
-    T *Y = reinterpret_cast<T *>(N.wsY.p);
-    T *V = reinterpret_cast<T *>(N.wsU.p);
+    T *Y = reinterpret_cast<T *>(ws.Y);
+    T *V = reinterpret_cast<T *>(ws.V);
     const T *arena = reinterpret_cast<const T *>(N.arena.p);
     const int *rsys = cols ? P.row_sys.p : nullptr;  // cols: batches only
     constexpr int kGridY = 65535;  // columns per permute launch
     for (int rb = 0; rb < nrhs; rb += kGridY)
         hipLaunchKernelGGL(k_permute_in<T>, dim3((n + 255) / 256, min(kGridY, nrhs - rb)), dim3(256), 0, st, d_rhs, ldr,
                            P.perm.p, n, Y, rsys, cols, rb);
-    T *Z = reinterpret_cast<T *>(N.wsP.p);
-    T *part = reinterpret_cast<T *>(N.wsQ.p);
+    T *Z = reinterpret_cast<T *>(ws.Z);
+    T *part = reinterpret_cast<T *>(ws.part);
     const SmallFwdArgs sfa{P.child_recs.p, P.inv_map.p, n, nrhs, cols};
     for (int L = 0; L < P.nlevels; ++L) {
         const int i0 = P.lvl_init_ptr[L], i1 = P.lvl_init_ptr[L + 1];
@@ -2786,24 +2833,88 @@ static void run_solve_once(Numeric &N, int nrhs, const double *d_rhs, int ldr, d
     MG_HIP(hipGetLastError());
 }
 
-static void ensure_ws(Numeric &N, int nrhs) {
-    if (N.ws_nrhs >= nrhs) return;
+size_t solve_ws_bytes(const Numeric &N, int nrhs) {
     const size_t esz = N.precision == MG_FP32 ? sizeof(float) : sizeof(double);
-    const int n = std::max(N.n, 1);
-    N.wsY.alloc((size_t)n * nrhs * esz);
-    N.wsU.alloc((size_t)std::max<int64_t>(N.plan->vsum, 1) * nrhs * esz);
-    N.wsP.alloc((size_t)n * nrhs * esz);
-    N.wsQ.alloc((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * nrhs * esz);
-    N.wsR.alloc((size_t)n * nrhs);
-    N.wsPart.alloc((size_t)residual_blocks(n, N.A.lanes) * nrhs * 2);
-    N.wsRel.alloc((size_t)nrhs * std::max(N.plan->nsys, 1));
-    N.wsCols.alloc((size_t)std::max(N.plan->nsys, 1));
-    N.ws_nrhs = nrhs;
+    const size_t n = std::max(N.n, 1), k = std::max(nrhs, 1), nsys = std::max(N.plan->nsys, 1);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(n * k * esz) + al((size_t)std::max<int64_t>(N.plan->vsum, 1) * k * esz) + al(n * k * esz) +
+           al((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * k * esz) + al(n * k * sizeof(double)) +
+           al((size_t)residual_blocks((int)n, N.A.lanes) * k * 2 * sizeof(double)) + al(k * nsys * sizeof(double)) +
+           al(nsys * sizeof(int));
 }
 
+SolveWs solve_ws_view(const Numeric &N, int nrhs, void *base) {
+    const size_t esz = N.precision == MG_FP32 ? sizeof(float) : sizeof(double);
+    const size_t n = std::max(N.n, 1), k = std::max(nrhs, 1), nsys = std::max(N.plan->nsys, 1);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    unsigned char *p = static_cast<unsigned char *>(base);
+    auto take = [&](size_t bytes) {
+        void *q = p;
+        p += al(bytes);
+        return q;
+    };
+    SolveWs w;
+    w.Y = take(n * k * esz);
+    w.V = take((size_t)std::max<int64_t>(N.plan->vsum, 1) * k * esz);
+    w.Z = take(n * k * esz);
+    w.part = take((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * k * esz);
+    w.R = static_cast<double *>(take(n * k * sizeof(double)));
+    w.Part = static_cast<double *>(take((size_t)residual_blocks((int)n, N.A.lanes) * k * 2 * sizeof(double)));
+    w.Rel = static_cast<double *>(take(k * nsys * sizeof(double)));
+    w.Cols = static_cast<int *>(take(nsys * sizeof(int)));
+    return w;
+}
+
+// A workspace from the handle's pool for one solve on stream st: the smallest free block that
+// is large enough, else a free block grown, else a new one.  The block is re-entered into
+// the pool (with an event recorded on st) when the lease ends, also after an error; a grown
+// block replaces the old one only once its allocation has succeeded.
+struct WsLease {
+    Numeric &N;
+    std::unique_ptr<WsSlot> slot;
+    hipStream_t st;
+    WsLease(Numeric &num, int nrhs, hipStream_t s) : N(num), st(s) {
+        const size_t need = solve_ws_bytes(N, nrhs);
+        {
+            std::lock_guard<std::mutex> lk(N.ws_mu);
+            size_t best = (size_t)-1;
+            int pick = -1;
+            for (int i = 0; i < (int)N.ws_free.size(); ++i) {
+                const size_t have = N.ws_free[i]->buf.n;
+                if (have >= need && have < best) best = have, pick = i;
+            }
+            if (pick < 0 && !N.ws_free.empty()) pick = (int)N.ws_free.size() - 1;
+            if (pick >= 0) {
+                slot = std::move(N.ws_free[pick]);
+                N.ws_free.erase(N.ws_free.begin() + pick);
+            }
+        }
+        if (!slot) slot = std::make_unique<WsSlot>();
+        try {
+            if (slot->buf.n < need) {
+                if (slot->done) MG_HIP(hipEventSynchronize(slot->done));  // its last solve is over
+                DevArray<unsigned char> grown(need);
+                slot->buf = std::move(grown);
+            }
+            if (!slot->done) MG_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
+            else MG_HIP(hipStreamWaitEvent(st, slot->done, 0));  // the previous solve may run elsewhere
+        } catch (...) {
+            give_back(false);
+            throw;
+        }
+    }
+    void give_back(bool record) {
+        if (!slot) return;
+        if (record && slot->done) (void)hipEventRecord(slot->done, st);
+        std::lock_guard<std::mutex> lk(N.ws_mu);
+        N.ws_free.push_back(std::move(slot));
+    }
+    ~WsLease() { give_back(true); }
+};
+
 // relres[c] = ||B - A X||/||B|| per column; R receives the residuals.
-static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const double *d_x, int ldx, hipStream_t st,
-                     std::vector<double> &relres, const int *cols = nullptr) {
+static void residual(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, const double *d_x,
+                     int ldx, hipStream_t st, std::vector<double> &relres, const int *cols = nullptr) {
     const int n = N.n;
     // one system: 8 right-hand sides per pass over A with the norm partials; a batch takes its
     // per-system norms from R afterwards, kResColsMax columns per pass, each row only up to its
@@ -2815,10 +2926,10 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
 #define MG_RES(LN)                                                                                                  \
     if (seg)                                                                                                        \
         hipLaunchKernelGGL((k_residual<LN, kResColsMax, false>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, \
-                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p, N.plan->row_sys.p, cols);   \
+                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, N.plan->row_sys.p, cols);          \
     else                                                                                                            \
         hipLaunchKernelGGL((k_residual<LN, 8, true>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, \
-                           n, nrhs, d_b, ldb, d_x, ldx, N.wsR.p, N.wsPart.p, nullptr, nullptr)
+                           n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, nullptr, nullptr)
     switch (lanes) {
         case 2: MG_RES(2); break;
         case 4: MG_RES(4); break;
@@ -2830,14 +2941,14 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
 #undef MG_RES
     const int nsys = N.plan->nsys;
     if (nsys > 1)
-        hipLaunchKernelGGL(k_residual_seg, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, n, d_b, ldb,
-                           N.plan->sys_begin.p, nrhs, N.wsRel.p, cols);
+        hipLaunchKernelGGL(k_residual_seg, dim3(nsys, nrhs), dim3(256), 0, st, ws.R, n, d_b, ldb, N.plan->sys_begin.p,
+                           nrhs, ws.Rel, cols);
     else
-        hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, N.wsPart.p, nb, N.wsRel.p);
+        hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, ws.Part, nb, ws.Rel);
     std::vector<double> prev;
     prev.swap(relres);
     relres.resize((size_t)nrhs * nsys);
-    MG_HIP(hipMemcpyAsync(relres.data(), N.wsRel.p, sizeof(double) * relres.size(), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipMemcpyAsync(relres.data(), ws.Rel, sizeof(double) * relres.size(), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
     // columns left alone keep their previous value (zero before any: those columns are zero)
     for (size_t e = 0; e < relres.size(); ++e)
@@ -2845,28 +2956,23 @@ static void residual(Numeric &N, int nrhs, const double *d_b, int ldb, const dou
 }
 
 // UtduNumericFactorization.Solve (:90-112) for nrhs columns, followed by FP64
-// iterative refinement: x += solve(b - A x) until max relres <= tol.
-void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol, int max_refine,
-                  hipStream_t st, mg_solve_report *rep) {
-    Numeric &N = *Np;
-    if (nrhs < 0) throw Error(MG_E_ARG, "The number of right-hand sides cannot be negative.");
-    if (rep) *rep = mg_solve_report{0, 1, 0.0};
-    if (N.n == 0 || nrhs == 0) return;
-    if (ldb < N.n || ldx < N.n) throw Error(MG_E_ARG, "The right-hand side length does not match the matrix size.");
-    ensure_ws(N, nrhs);
+// iterative refinement: x += solve(b - A x) until max relres <= tol.  Reads the handle only;
+// every buffer it writes besides x is in ws.
+static void solve_with_ws(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, double *d_x,
+                          int ldx, double tol, int max_refine, hipStream_t st, mg_solve_report *rep) {
     auto once = [&](const double *rhs, int ldr, int acc, const int *cols) {
-        if (N.precision == MG_FP32) run_solve_once<float>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
-        else run_solve_once<double>(N, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
+        if (N.precision == MG_FP32) run_solve_once<float>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
+        else run_solve_once<double>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
     };
     const int nsys = N.plan->nsys;
     // in a batch, every system's solves stop at its last nonzero right-hand side (uneven
     // counts padded to nrhs columns): the columns past it stay zero
     if (nsys > 1)
-        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(1024), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, N.wsCols.p);
-    once(d_b, ldb, 0, nsys > 1 ? N.wsCols.p : nullptr);
+        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(1024), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, ws.Cols);
+    once(d_b, ldb, 0, nsys > 1 ? ws.Cols : nullptr);
     if (!rep && max_refine <= 0) return;
     std::vector<double> rel;
-    residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel, nsys > 1 ? N.wsCols.p : nullptr);
+    residual(N, ws, nrhs, d_b, ldb, d_x, ldx, st, rel, nsys > 1 ? ws.Cols : nullptr);
     auto worst_of = [](const std::vector<double> &v) {
         double m = 0.0;
         for (double x : v) m = (std::isnan(x) || std::isnan(m)) ? NAN : std::max(m, x);
@@ -2874,24 +2980,26 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     };
     double worst = worst_of(rel);
     int steps = 0;
+    std::vector<int> h_cols;
     while (steps < max_refine && !(worst <= tol)) {
         // in a batch, a correction solve covers each system's columns up to its last
         // unconverged one (none for a converged system): the residual of the columns left
         // out is zeroed, so their correction is zero
         const int *cols = nullptr;
         if (nsys > 1) {
-            N.h_cols.assign(nsys, 0);
+            h_cols.assign(nsys, 0);
             for (int t = 0; t < nsys; ++t)
                 for (int c = 0; c < nrhs; ++c)
-                    if (!(rel[(size_t)t * nrhs + c] <= tol)) N.h_cols[t] = c + 1;
-            MG_HIP(hipMemcpyAsync(N.wsCols.p, N.h_cols.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_zero_past, dim3(nsys, nrhs), dim3(256), 0, st, N.wsR.p, N.n, N.plan->sys_begin.p,
-                               N.wsCols.p);
-            cols = N.wsCols.p;
+                    if (!(rel[(size_t)t * nrhs + c] <= tol)) h_cols[t] = c + 1;
+            // h_cols is next changed after the stream synchronization of the next residual
+            MG_HIP(hipMemcpyAsync(ws.Cols, h_cols.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_zero_past, dim3(nsys, nrhs), dim3(256), 0, st, ws.R, N.n, N.plan->sys_begin.p,
+                               ws.Cols);
+            cols = ws.Cols;
         }
-        once(N.wsR.p, N.n, 1, cols);
+        once(ws.R, N.n, 1, cols);
         ++steps;
-        residual(N, nrhs, d_b, ldb, d_x, ldx, st, rel, cols);
+        residual(N, ws, nrhs, d_b, ldb, d_x, ldx, st, rel, cols);
         worst = worst_of(rel);
     }
     if (rep) {
@@ -2899,6 +3007,24 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
         rep->relative_residual = worst;
         rep->converged = worst <= tol;
     }
+}
+
+static void solve_checks(const Numeric &N, int nrhs, int ldb, int ldx) {
+    if (!N.factorized)
+        throw Error(MG_E_ARG, "The numeric factorization did not complete; factorize the matrix again before solving.");
+    if (nrhs < 0) throw Error(MG_E_ARG, "The number of right-hand sides cannot be negative.");
+    if (N.n > 0 && nrhs > 0 && (ldb < N.n || ldx < N.n))
+        throw Error(MG_E_ARG, "The right-hand side length does not match the matrix size.");
+}
+
+void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol, int max_refine,
+                  hipStream_t st, mg_solve_report *rep) {
+    Numeric &N = *Np;
+    solve_checks(N, nrhs, ldb, ldx);
+    if (rep) *rep = mg_solve_report{0, 1, 0.0};
+    if (N.n == 0 || nrhs == 0) return;
+    WsLease lease(N, nrhs, st);
+    solve_with_ws(N, solve_ws_view(N, nrhs, lease.slot->buf.p), nrhs, d_b, ldb, d_x, ldx, tol, max_refine, st, rep);
 }
 
 // ---- test hooks -----------------------------------------------------------

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,7 +47,11 @@ struct Symbolic {
     // elimination order alike) of each system, count+1 entries; empty for a single system.
     std::vector<int> sys_begin;
 
-    std::shared_ptr<DevicePlan> plan;  // device index maps and level schedule (lazy)
+    // device index maps and level schedule: built by the analysis when a device is visible,
+    // else by the first factorization (under plan_mu, so concurrent factorizations from one
+    // symbolic handle build it once)
+    std::shared_ptr<DevicePlan> plan;
+    std::shared_ptr<std::mutex> plan_mu = std::make_shared<std::mutex>();
 
     int front(int s) const { return pat_ptr[s + 1] - pat_ptr[s]; }
     int width(int s) const { return first[s + 1] - first[s]; }

@@ -31,7 +31,7 @@ __global__ void k_full_gather(const int *__restrict__ src, int64_t nnz, const do
     if (e < nnz) full[e] = upper[src[e]];
 }
 
-// Jacobi inverse diagonal (JacobiPreconditioner.cs:221-228); status[0] = 1 on
+// Jacobi inverse diagonal (JacobiPreconditioner.cs:24-31); status[0] = 1 on
 // a zero / non-finite diagonal.
 __global__ void k_inv_diag(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v, int n,
                            double *__restrict__ invd, int *__restrict__ status) {

@@ -195,7 +195,7 @@ class MinimumDegree {
         return done;
     }
 
-    // FactorGraph.Compact (FactorGraph.cs:218-266)
+    // FactorGraph.Compact (FactorGraph.cs:95-143)
     void squeeze(int &lo) {
         std::vector<std::pair<int, int>> live;
         live.reserve(n_);

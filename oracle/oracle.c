@@ -186,7 +186,7 @@ int or_linecross(or_csr *a, double *rhs, double prescribed, int index)
 
 double or_norm2(int n, const double *v)
 {
-    /* ResidualCalculator.cs:99-107 */
+    /* ResidualCalculator.cs:22-30 */
     double s = 0.0;
     for (int i = 0; i < n; i++) s += v[i] * v[i];
     return sqrt(s);
@@ -301,7 +301,7 @@ static int kv_cmp(const void *pa, const void *pb)
     return a < b ? -1 : (a > b);
 }
 
-/* FactorGraph.cs:218-266 -- live lists (Pe>=0, Len>0) moved left in offset
+/* FactorGraph.cs:95-143 -- live lists (Pe>=0, Len>0) moved left in offset
  * order, then the half-built element; grow when still short of room. */
 static int fg_compact(amd_state *s, int *first_entry)
 {
@@ -542,7 +542,7 @@ static void restore_degree_lists(amd_state *s, int pivot, int pivot_elen, int ed
 
 /* ApproximateMinimumDegreeOrdering.cs:71-84 + WorkSpace ctor (:129-155),
  * Run (:157-224); SymmetricPatternGraph.FromMatrix (AMD/SymmetricPatternGraph.cs:71-121);
- * FactorGraph ctor (AMD/FactorGraph.cs:171-195). */
+ * FactorGraph ctor (AMD/FactorGraph.cs:48-78). */
 int or_amd(int n, const int *rowptr, const int *colidx, int *perm_out)
 {
     if (n == 0) return OR_OK;
@@ -1275,7 +1275,7 @@ int or_solve(const or_numeric *F, const double *b, double *x)
 
 /* ======================================================================
  * CGGP -- CGGP/ConjugateGradientGaussPreSolver.cs:78-175,
- * CGGP/JacobiPreconditioner.cs:209-253, IterativeLinearSolver.cs:68-75
+ * CGGP/JacobiPreconditioner.cs:12-56, IterativeLinearSolver.cs:26-33
  * ====================================================================== */
 int or_pcg(const or_csr *a, const double *b, const double *x0, double tol,
            int max_iterations, int use_jacobi, double *x, or_pcg_result *res)
