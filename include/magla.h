@@ -168,6 +168,16 @@ int mg_solve(mg_numeric *num, int32_t nrhs, const double *b, int32_t ldb, double
              double tol, int32_t max_refine, mg_solve_report *report);
 int mg_solve_dev(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x,
                  int32_t ldx, double tol, int32_t max_refine, void *stream, mg_solve_report *report);
+/* No counterpart in the reference: mg_solve_dev for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid allocation).
+ * Same arguments, kernels and results; the solve workspace lives in a static 256 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of the handle's
+ * pool, so no call allocates.  A call whose workspace would exceed 256 MiB (about
+ * (2 n + sum of front sizes) x nrhs x factor bytes + 8 n x nrhs x 2) returns MG_E_RANGE.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (other threads or streams)
+ * race, so callers serialize them. */
+int mg_solve_dev_noalloc(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x,
+                         int32_t ldx, double tol, int32_t max_refine, void *stream, mg_solve_report *report);
 
 /* ---- device matrix, SpMV, CGGP ---- */
 int mg_matrix_create(int32_t n, const int32_t *rowptr, const int32_t *colidx, const double *values,

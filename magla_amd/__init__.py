@@ -142,6 +142,8 @@ def lib():
                            C.POINTER(_SolveReport)]
     L.mg_solve_dev.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_double, C.c_int32, vp,
                                C.POINTER(_SolveReport)]
+    L.mg_solve_dev_noalloc.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_double, C.c_int32, vp,
+                                       C.POINTER(_SolveReport)]
     L.mg_matrix_create.argtypes = [C.c_int32, _i32p, vp, vp, C.POINTER(vp)]
     L.mg_matrix_release.argtypes = [vp]
     L.mg_matrix_release.restype = None
@@ -512,6 +514,15 @@ class UtduNumericFactorization:
         rep = _SolveReport()
         _check(lib().mg_solve_dev(self._h, nrhs, b_ptr, ldb or self.size, x_ptr, ldx or self.size, tol, max_refine,
                                   stream, C.byref(rep)))
+        return dict(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
+                    relative_residual=rep.relative_residual)
+
+    def solve_dev_noalloc(self, b_ptr, x_ptr, nrhs, tol=0.0, max_refine=0, stream=None, ldb=None, ldx=None):
+        """solve_dev with the workspace in the library's static device scratch (no allocation;
+        calls overlapping on one GPU race)."""
+        rep = _SolveReport()
+        _check(lib().mg_solve_dev_noalloc(self._h, nrhs, b_ptr, ldb or self.size, x_ptr, ldx or self.size, tol,
+                                          max_refine, stream, C.byref(rep)))
         return dict(refinement_steps=rep.refinement_steps, converged=bool(rep.converged),
                     relative_residual=rep.relative_residual)
 

@@ -420,6 +420,15 @@ int mg_solve_dev(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, 
     });
 }
 
+int mg_solve_dev_noalloc(mg_numeric *num, int32_t nrhs, const double *d_b, int32_t ldb, double *d_x, int32_t ldx,
+                         double tol, int32_t max_refine, void *stream, mg_solve_report *report) {
+    return guarded([&] {
+        require(num != nullptr, MG_E_NULL, "numeric is null.");
+        require(nrhs == 0 || (d_b && d_x), MG_E_NULL, "rightHandSide is null.");
+        mg::solve_device_noalloc(num->num, nrhs, d_b, ldb, d_x, ldx, tol, max_refine, (hipStream_t)stream, report);
+    });
+}
+
 int mg_solve(mg_numeric *num, int32_t nrhs, const double *b, int32_t ldb, double *x, int32_t ldx, double tol,
              int32_t max_refine, mg_solve_report *report) {
     return guarded([&] {

@@ -91,6 +91,9 @@ void numeric_release(Numeric *num);
 void numeric_info(const Numeric *num, mg_numeric_info &info, double *diag_host);
 void solve_device(Numeric *num, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol,
                   int max_refine, hipStream_t st, mg_solve_report *rep);
+// The same solve with its workspace in the library's static device scratch (no allocation).
+void solve_device_noalloc(Numeric *num, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol,
+                          int max_refine, hipStream_t st, mg_solve_report *rep);
 
 // SpMV / PCG
 void spmv_device(const Matrix &A, int nrhs, const double *d_x, int ldx, double *d_y, int ldy, hipStream_t st);

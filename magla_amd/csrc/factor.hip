@@ -3027,6 +3027,50 @@ void solve_device(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x
     solve_with_ws(N, solve_ws_view(N, nrhs, lease.slot->buf.p), nrhs, d_b, ldb, d_x, ldx, tol, max_refine, st, rep);
 }
 
+// ---- allocation-free solve --------------------------------------------------
+//
+// For callers that cannot allocate device memory (code under stream capture, tight serving
+// loops, hosts that forbid it): the solve workspace comes from one file-scope device array
+// instead of the handle's pool.  Each device holds its own copy of the array for the life of
+// the process; its device address is looked up once per device (hipGetSymbolAddress) and only
+// that address is handed to kernels.  Every allocation-free solve on one device shares the
+// array: calls that overlap on one device race, and a call larger than the array is refused.
+constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static void *scratch_base() {
+    static std::mutex mu;
+    static void *base[64] = {};
+    int dev = 0;
+    MG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error(MG_E_DEVICE, "Device index out of range for the solve scratch.");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!base[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(MG_E_DEVICE, std::string("HIP error '") + hipGetErrorString(e) +
+                                         "' looking up the device address of the solve scratch.");
+        base[dev] = p;
+    }
+    return base[dev];
+}
+
+void solve_device_noalloc(Numeric *Np, int nrhs, const double *d_b, int ldb, double *d_x, int ldx, double tol,
+                          int max_refine, hipStream_t st, mg_solve_report *rep) {
+    Numeric &N = *Np;
+    solve_checks(N, nrhs, ldb, ldx);
+    if (rep) *rep = mg_solve_report{0, 1, 0.0};
+    if (N.n == 0 || nrhs == 0) return;
+    const size_t need = solve_ws_bytes(N, nrhs);
+    if (need > kScratchBytes)
+        throw Error(MG_E_RANGE, "The allocation-free solve needs " + std::to_string(need) + " bytes of workspace for " +
+                                    std::to_string(nrhs) + " right-hand sides, more than its " +
+                                    std::to_string(kScratchBytes) + "-byte device scratch; use fewer right-hand "
+                                    "sides per call or mg_solve_dev.");
+    solve_with_ws(N, solve_ws_view(N, nrhs, scratch_base()), nrhs, d_b, ldb, d_x, ldx, tol, max_refine, st, rep);
+}
+
 // ---- test hooks -----------------------------------------------------------
 
 // Factor block of supernode s as MAGLA stores it (UtduNumericFactorization.cs:517-519, 591):
