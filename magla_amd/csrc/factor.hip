@@ -117,7 +117,14 @@ struct DevicePlan {
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
 
-    std::vector<std::vector<int>> lvl_sfwd_ptr, lvl_sbwd_ptr;  // per 64-column panel
+    // large-front solves per 256-column block J of a level: forward y of the block's
+    // sub-blocks (fa) and the rows below it (fb), backward x of the sub-blocks (ba) and the
+    // 64-column blocks to its left (bb); task ranges [ptr[J], ptr[J + 1]) of the task arrays
+    std::vector<std::vector<int>> lvl_fa_ptr, lvl_fb_ptr, lvl_ba_ptr, lvl_bb_ptr;
+    // 256-column diagonal block inverses (end of the factorization): (front, block, a, b) tasks
+    // of distance d = a - b in [inv_ptr[d - 1], inv_ptr[d])
+    int inv_ptr[4] = {0, 0, 0, 0};
+    DevArray<int4> inv_tasks;
     int64_t vsum = 0;               // sum f over fronts (solve front vectors)
     // algorithmic work per launch
     std::vector<double> lvl_pull_elems, lvl_pull_index;
@@ -134,7 +141,7 @@ struct DevicePlan {
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
     DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks;
-    DevArray<PanelTask> sfwd_tasks, sbwd_tasks;
+    DevArray<PanelTask> fa_tasks, fb_tasks, ba_tasks, bb_tasks;
     DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
     DevArray<int4> foot_tasks;     // (front, column block, row chunk, index of the chunk in the block)
 
@@ -304,8 +311,9 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     std::vector<int> small;
     P->lvl_foot_ptr.push_back(0);
     P->lvl_footr_ptr.push_back(0);
-    std::vector<int2> sfwd, sbwd;
+    std::vector<int2> fa, fb, ba, bb;
     std::vector<int4> foot, footr;
+    std::vector<int4> inv_d[3];
 
     std::vector<int64_t> asm_pos;
     std::vector<int> asm_vidx;
@@ -359,21 +367,32 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             }
         }
         P->max_foot_tasks = std::max(P->max_foot_tasks, (int)foot.size() - foot_base);
-        std::vector<int> sf{(int)sfwd.size()}, sb{(int)sbwd.size()};
-        for (int q = 0; q < npan; ++q) {
+        std::vector<int> pfa{(int)fa.size()}, pfb{(int)fb.size()}, pba{(int)ba.size()}, pbb{(int)bb.size()};
+        const int nblk = (npan + kBlock / kPanel - 1) / (kBlock / kPanel);
+        for (int J = 0; J < nblk; ++J) {
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
-                if (w <= q * kPanel || f <= kSmallFront) continue;
-                const int pe = std::min(q * kPanel + kPanel, w);
-                sfwd.push_back(make_int2(g, -1));
-                for (int rb = 0; rb < (f - pe + kTile - 1) / kTile; ++rb) sfwd.push_back(make_int2(g, rb));
-                for (int cb = 0; cb <= q; ++cb) sbwd.push_back(make_int2(g, cb));
+                if (w <= J * kBlock || f <= kSmallFront) continue;
+                const int bs = J * kBlock, be = std::min(bs + kBlock, w);
+                const int nsub = (be - bs + kPanel - 1) / kPanel;
+                for (int a = 0; a < nsub; ++a) {
+                    fa.push_back(make_int2(g, a));
+                    ba.push_back(make_int2(g, a));
+                }
+                for (int rb = 0; rb < (f - be + kTile - 1) / kTile; ++rb) fb.push_back(make_int2(g, rb));
+                for (int cb = 0; cb < bs / kPanel; ++cb) bb.push_back(make_int2(g, cb));
+                for (int a = 1; a < nsub; ++a)
+                    for (int b = 0; b < a; ++b) inv_d[a - b - 1].push_back(make_int4(g, J, a, b));
             }
-            sf.push_back((int)sfwd.size());
-            sb.push_back((int)sbwd.size());
+            pfa.push_back((int)fa.size());
+            pfb.push_back((int)fb.size());
+            pba.push_back((int)ba.size());
+            pbb.push_back((int)bb.size());
         }
-        P->lvl_sfwd_ptr.push_back(sf);
-        P->lvl_sbwd_ptr.push_back(sb);
+        P->lvl_fa_ptr.push_back(pfa);
+        P->lvl_fb_ptr.push_back(pfb);
+        P->lvl_ba_ptr.push_back(pba);
+        P->lvl_bb_ptr.push_back(pbb);
         P->lvl_foot_ptr.push_back((int)foot.size());
         P->lvl_footr_ptr.push_back((int)footr.size());
         P->lvl_node_ptr.push_back((int)level_nodes.size());
@@ -512,8 +531,18 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         }
         return o;
     };
-    P->sfwd_tasks.upload(pack(sfwd));
-    P->sbwd_tasks.upload(pack(sbwd));
+    P->fa_tasks.upload(pack(fa));
+    P->fb_tasks.upload(pack(fb));
+    P->ba_tasks.upload(pack(ba));
+    P->bb_tasks.upload(pack(bb));
+    {
+        std::vector<int4> all;
+        for (int d = 0; d < 3; ++d) {
+            all.insert(all.end(), inv_d[d].begin(), inv_d[d].end());
+            P->inv_ptr[d + 1] = (int)all.size();
+        }
+        P->inv_tasks.upload(all);
+    }
     P->vbase.upload(vbase);
     P->inv_map.upload(inv_map);
     P->perm.upload(S.perm);
@@ -1334,12 +1363,12 @@ __global__ __launch_bounds__(256) void k_permute_out(const T *__restrict__ Xe, c
 // Every front owns a vector V (f rows x nrhs) in a workspace: the forward pass
 // gathers it (own rows from the permuted right-hand side, plus the children's
 // footer vectors through the inverse row maps), then walks the own columns in
-// 64-column panels, one launch each (k_fwd_panel: y from the stored 64x64
-// inverse, then the product with the rows below); the footer left in V is the
+// 256-column blocks, two launches each (k_fwd_diag: y of the block from its stored
+// inverse; k_fwd_upd: the product with the rows below); the footer left in V is the
 // update the parent pulls.  The backward pass starts each level with the
 // footer product (two-stage: k_bwd_foot_part / k_bwd_foot_red, including the
-// division by D) and walks the panels from the last one down, right-looking
-// (k_bwd_panel).  Y holds y after the
+// division by D) and walks the blocks from the last one down, right-looking
+// (k_bwd_diag, k_bwd_upd).  Y holds y after the
 // forward pass and x after the backward pass (elimination order, n x nrhs).
 
 template <typename T>
@@ -1747,106 +1776,6 @@ __global__ __launch_bounds__(256) void k_bwd_small(const SmallRec *__restrict__ 
     }
 }
 
-// acc += sum over k-steps s < 16 (k = 4 s + lk) of step(a(s), b(s)); for a panel narrower
-// than 64 only the k-steps inside nb (uniform), the full case straight-line.
-template <typename T, typename FA, typename FB>
-__device__ __forceinline__ void mfma_k64(typename MfmaAcc<T>::type &acc, int nb, FA a, FB b) {
-    if (nb == kPanel) {
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(a(s), b(s), acc);
-    } else {
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) {
-            if (4 * s >= nb) break;
-            acc = MfmaAcc<T>::step(a(s), b(s), acc);
-        }
-    }
-}
-
-// Forward panel (columns [ps, pe) of a front), on the matrix cores: every workgroup forms
-// y^T = v^T Li^T from the stored 64x64 inverse (wave v: columns 16v .. 16v+15 of y^T, the
-// chunk's right-hand sides as the 16 rows); task rb = -1 writes y to Y, task rb >= 0 forms
-// (V_rows)^T -= y^T L[rows, ps:pe)^T for its 64-row block of V below the panel, the L
-// operand read straight from the front (16 rows of a column per load, coalesced).
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
-                                                   T *__restrict__ V, T *__restrict__ Y, int ps, int n, int nrhs,
-                                                   const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    __shared__ T inv[kPanel][kPanel + 1];     // inv[m][k] = Li[k][m]
-    __shared__ T vv[kPanel][kRhsChunk + 1];   // vv[k][rr] = v[ps + k][c0 + rr]
-    __shared__ T yv[kPanel][kRhsChunk + 1];   // yv[k][rr] = y[k][c0 + rr]
-    const PanelTask tk = tasks[blockIdx.x];
-    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
-    const int rb = tk.blk, f = tk.f, w = tk.w;
-    const int pe = min(ps + kPanel, w), nb = pe - ps;
-    const T *M = arena + tk.off;
-    T *Vg = V + tk.vbase * nrhs;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int r0 = pe + (rb < 0 ? 0 : rb) * kTile, nrow = rb < 0 ? 0 : min(kTile, f - r0);
-    const int col = kPanel / 4 * wv + l16;  // this lane's output column (y) or row of the block (V)
-    // B operand of the row product: L[r0 + col][ps + 4 s + lk], s < 16 (clamped; rows past
-    // the block are not stored, columns past nb meet y = 0)
-    T bl[kPanel / 4];
-    if (rb >= 0) {
-        const T *src = M + r0 + min(col, nrow - 1);
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) bl[s] = src[(int64_t)(ps + min(4 * s + lk, nb - 1)) * f];
-    }
-    // a chunk's vectors: the panel rows (tv) and this block's rows (old)
-    T old[4], tv[4];
-    auto load_chunk = [&](int c0, int nr) {
-        if (rb >= 0) {
-            const int orow = min(r0 + col, f - 1);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + orow];
-        }
-        vec_load<T>(tv, Vg + (int64_t)c0 * f + ps, f, nb, nr);
-    };
-    // the first chunk in flight with the tile loads, before the count arrives (columns past it
-    // lie inside the workspace and are masked when stored)
-    load_chunk(0, min(kRhsChunk, nrhs));
-    {
-        T ti[kPanel / 4];
-        tile_load<T>(ti, M, f, ps, ps, nb, nb);
-        tile_store_inv<T>(inv, ti, nb);
-    }
-    if (nc == 0) return;  // uniform, before any barrier
-    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nc - c0);
-        if (c0 > 0) load_chunk(c0, nr);
-        __syncthreads();
-        vec_store<T>(vv, tv, nb, nr);
-        __syncthreads();
-        typename MfmaAcc<T>::type acc;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        mfma_k64<T>(acc, nb, [&](int s) { return vv[4 * s + lk][l16]; }, [&](int s) { return inv[4 * s + lk][col]; });
-        // acc[q] = y[col][rr], rr = row(lk, q)
-        if (rb < 0) {
-            if (col < nb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int rr = MfmaAcc<T>::row(lk, q);
-                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + ps + col] = acc[q];
-                }
-            continue;  // uniform
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) yv[col][MfmaAcc<T>::row(lk, q)] = acc[q];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        mfma_k64<T>(acc, nb, [&](int s) { return yv[4 * s + lk][l16]; }, [&](int s) { return bl[s]; });
-        if (col < nrow)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int rr = MfmaAcc<T>::row(lk, q);
-                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + col] = old[q] - acc[q];
-            }
-    }
-}
-
 // Backward footer, gather: the x values of every front's footer rows (final:
 // ancestors are solved first) into the front's vector, V[g][rr][w + a] = x[rows[w + a]].
 template <typename T>
@@ -1942,77 +1871,305 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
     }
 }
 
-// Backward panel (columns [ps, pe)), right-looking, on the matrix cores: every workgroup
-// forms x^T = z^T Li from the stored inverse; the task of block q writes x to Y, the tasks of
-// blocks cb < q form (Z_block)^T -= x^T L[ps:pe, block] (the L block staged through LDS).
+// ---- 256-column block solves of the large fronts ------------------------------
+//
+// At the end of the factorization the unit-lower diagonal block L_JJ of every 256-column
+// block J of a front (sub-blocks of 64 columns, the last one possibly narrower) is inverted:
+// X = L_JJ^{-1} with X_aa = Li_a (the sub-blocks' stored inverses) and, for a > b,
+// X_ab = -Li_a sum_{c=b}^{a-1} L_ac X_cb.  X[k][m] (k > m) is kept at front row m, column k,
+// the transposed layout of the 64-column inverses, which are its diagonal blocks: the
+// sub-diagonal blocks X_ab fill the block's otherwise unused upper triangle.  One launch per
+// distance d = a - b: X_ab reads only blocks of smaller distance, written by earlier launches.
+// The solves then take one dependent step per 256 columns instead of per 64.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_panel(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
-                                                   T *__restrict__ Y, T *__restrict__ Z, int q, int n, int nrhs,
-                                                   const int *__restrict__ cols) {
+__global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, T *__restrict__ arena,
+                                                const int *__restrict__ F, const int *__restrict__ W,
+                                                const int64_t *__restrict__ off) {
+    __shared__ __align__(16) T As[kPanel][kSchurLd<T>];
+    __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];
+    const int4 tk = tasks[blockIdx.x];
+    const int g = tk.x, a = tk.z, b = tk.w;
+    const int f = F[g], w = W[g];
+    const int bs = tk.y * kBlock, be = min(bs + kBlock, w);
+    T *M = arena + off[g];
+    const int pa = bs + kPanel * a, pb = bs + kPanel * b;
+    const int na = min(kPanel, be - pa), nbb = min(kPanel, be - pb);
+    const int tid = threadIdx.x, l = tid & 63, q4 = tid >> 6;
+    typename MfmaAcc<T>::type acc[2][2];
+    schur_acc_zero<T>(acc);
+    // S = sum_{c=b}^{a-1} L_ac X_cb: As[k][r] = L_ac[r][k] = M[(pc + k) f + pa + r],
+    // Bs[k][m] = X_cb[k][m] = M[(pc + k) f + pb + m] (c = b: Li_b, unit diagonal, zero above)
+    for (int c = b; c < a; ++c) {
+        const int pc = bs + kPanel * c, nc = min(kPanel, be - pc);
+        T av[kPanel / 4], bv[kPanel / 4];
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const T *col = M + (int64_t)(pc + min(q4 + 4 * j, nc - 1)) * f;
+            av[j] = col[pa + min(l, na - 1)];
+            bv[j] = col[pb + min(l, nbb - 1)];
+        }
+        __syncthreads();  // the previous product's reads are done
+#pragma unroll
+        for (int j = 0; j < kPanel / 4; ++j) {
+            const int k = q4 + 4 * j;
+            T x = bv[j];
+            if (c == b) x = l < k ? x : (l == k ? T(1) : T(0));
+            As[k][l] = (k < nc && l < na) ? av[j] : T(0);
+            Bs[k][l] = (k < nc && l < nbb) ? x : T(0);
+        }
+        __syncthreads();
+        schur_acc_tile<T>(As, Bs, acc);
+    }
+    // X_ab = -Li_a S: As[k][r] = Li_a[r][k] = M[(pa + r) f + pa + k] (k < r; unit diagonal),
+    // Bs[k][m] = S[k][m] from the accumulators
+    const int lane = tid & 63, wave = tid >> 6;
+    const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32, l16 = lane & 15, lk = lane >> 4;
+    T li[kPanel / 4];
+#pragma unroll
+    for (int j = 0; j < kPanel / 4; ++j) li[j] = M[(int64_t)(pa + min(q4 + 4 * j, na - 1)) * f + pa + min(l, na - 1)];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[rq + 16 * x + l16][cq + 16 * y + MfmaAcc<T>::row(lk, q)] = acc[x][y][q];
+#pragma unroll
+    for (int j = 0; j < kPanel / 4; ++j) {
+        const int r = q4 + 4 * j;  // li[j] = M[(pa + r) f + pa + l] = Li_a[r][l] for l < r
+        As[l][r] = (r < na && l < na) ? (l < r ? li[j] : (l == r ? T(1) : T(0))) : T(0);
+    }
+    __syncthreads();
+    schur_acc_zero<T>(acc);
+    schur_acc_tile<T>(As, Bs, acc);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = rq + 16 * x + l16, m = cq + 16 * y + MfmaAcc<T>::row(lk, q);
+                if (r < na && m < nbb) M[(int64_t)(pa + r) * f + pb + m] = -acc[x][y][q];
+            }
+}
+
+// Forward, step 1 of block J: y of sub-block a, y[pa + k] = sum_{m <= k} X[pa + k][m] v[m] over
+// the block's sub-blocks c <= a, on the matrix cores as y^T = v^T X^T (right-hand sides as the
+// 16 MFMA rows, wave v: columns 16 v .. 16 v + 15 of the sub-block); y goes to Y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
+                                                  const T *__restrict__ V, T *__restrict__ Y, int J, int n, int nrhs,
+                                                  const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    __shared__ T blk[kPanel][kPanel + 1];   // blk[k][c] = L[ps+k][64cb+c]
-    __shared__ T inv[kPanel][kPanel + 1];   // inv[m][k] = Li[k][m]
-    __shared__ T zz[kPanel][kRhsChunk + 1];
-    __shared__ T xv[kPanel][kRhsChunk + 1];
+    __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pa + k][pc + m]
+    __shared__ T vv[kPanel][kRhsChunk + 1];  // vv[m][rr] = v[pc + m][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
-    const int cb = tk.blk, f = tk.f, w = tk.w;
-    const int ps = q * kPanel, pe = min(ps + kPanel, w), nb = pe - ps;
-    const int fc = tk.fc;
+    if (nc == 0) return;
+    const int a = tk.blk, f = tk.f, w = tk.w;
+    const int bs = J * kBlock, be = min(bs + kBlock, w);
+    const int pa = bs + kPanel * a, na = min(kPanel, be - pa);
     const T *M = arena + tk.off;
+    const T *Vg = V + tk.vbase * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
-    // a chunk's vectors: the panel's z (tz) and block cb's old values (cb < q: a full
-    // 64-column panel)
-    T old[4], tz[4];
-    auto load_chunk = [&](int c0, int nr) {
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq)
-            old[qq] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, qq), nr - 1)) * n + fc + min(cb * kPanel + col, w - 1)];
-        vec_load<T>(tz, Z + (int64_t)c0 * n + fc + ps, n, nb, nr);
-    };
-    // the first chunk in flight with the tile loads, before the count arrives (columns past it
-    // lie inside the workspace and are masked when stored)
-    load_chunk(0, min(kRhsChunk, nrhs));
-    {
-        T ti[kPanel / 4], tb[kPanel / 4];
-        tile_load<T>(ti, M, f, ps, ps, nb, nb);
-        tile_load<T>(tb, M, f, ps, cb * kPanel, cb < q ? nb : 0, kPanel);
-        tile_store_inv<T>(inv, ti, nb);
-        tile_store<T>(blk, tb, cb < q ? nb : 0, kPanel);
-    }
-    if (nc == 0) return;  // uniform, before any barrier
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        if (c0 > 0) load_chunk(c0, nr);
+        typename MfmaAcc<T>::type acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        T ti[kPanel / 4], tv[4];
+        tile_load<T>(ti, M, f, bs, pa, min(kPanel, be - bs), na);
+        vec_load<T>(tv, Vg + (int64_t)c0 * f + bs, f, min(kPanel, be - bs), nr);
+        for (int c = 0; c <= a; ++c) {
+            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+            __syncthreads();
+            if (c == a) tile_store_inv<T>(inv, ti, na);
+            else tile_store<T>(inv, ti, ncc, na);
+            vec_store<T>(vv, tv, ncc, nr);
+            __syncthreads();
+            if (c < a) {  // the next sub-block in flight during this one's products
+                const int pn = pc + kPanel, nn = min(kPanel, be - pn);
+                tile_load<T>(ti, M, f, pn, pa, nn, na);
+                vec_load<T>(tv, Vg + (int64_t)c0 * f + pn, f, nn, nr);
+            }
+#pragma unroll
+            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
+        }
+        // acc[q] = y[pa + col][c0 + rr], rr = row(lk, q)
+        if (col < na)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pa + col] = acc[q];
+            }
         __syncthreads();
-        vec_store<T>(zz, tz, nb, nr);
+    }
+}
+
+// Forward, step 2 of block J: the 64 rows r0 + i below the block, V[r0 + i] -= sum_k
+// L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (the L
+// operand read straight from the front, 16 rows of a column per load).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
+                                                 T *__restrict__ V, const T *__restrict__ Y, int J, int n, int nrhs,
+                                                 const int *__restrict__ cols) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    __shared__ T yv[kBlock][kRhsChunk + 1];  // yv[k][rr] = y[bs + k][c0 + rr]
+    const PanelTask tk = tasks[blockIdx.x];
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
+    if (nc == 0) return;
+    const int f = tk.f, w = tk.w;
+    const int bs = J * kBlock, be = min(bs + kBlock, w), nk = be - bs;
+    const int r0 = be + tk.blk * kTile, nrow = min(kTile, f - r0);
+    const T *M = arena + tk.off;
+    T *Vg = V + tk.vbase * nrhs;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int col = kPanel / 4 * wv + l16;  // this lane's row of the block
+    const T *lrow = M + r0 + min(col, nrow - 1);
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        T old[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(col, nrow - 1)];
+        T bl[kPanel / 4];
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) bl[s] = lrow[(int64_t)(bs + min(4 * s + lk, nk - 1)) * f];
+        for (int idx = tid; idx < kBlock * kRhsChunk; idx += 256) {
+            const int k = idx % kBlock, rr = idx / kBlock;
+            const T y = Y[(int64_t)(c0 + min(rr, nr - 1)) * n + tk.fc + bs + min(k, nk - 1)];
+            yv[k][rr] = (k < nk && rr < nr) ? y : T(0);
+        }
         __syncthreads();
         typename MfmaAcc<T>::type acc;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
-        // x^T[rr][j] = sum_k z[k][rr] Li[k][j], Li[k][j] = inv[j][k]
-        mfma_k64<T>(acc, nb, [&](int s) { return zz[4 * s + lk][l16]; }, [&](int s) { return inv[col][4 * s + lk]; });
-        if (cb == q) {
-            if (col < nb)
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        for (int kc = 0; kc < nk; kc += kPanel) {
+            T cur[kPanel / 4];
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
-                    const int rr = MfmaAcc<T>::row(lk, qq);
-                    if (rr < nr) Y[(int64_t)(c0 + rr) * n + fc + ps + col] = acc[qq];
-                }
-            continue;  // uniform
+            for (int s = 0; s < kPanel / 4; ++s) cur[s] = bl[s];
+            const int kn = kc + kPanel;
+            if (kn < nk)  // the next 64 columns in flight during this chunk's products
+#pragma unroll
+                for (int s = 0; s < kPanel / 4; ++s) bl[s] = lrow[(int64_t)(bs + kn + min(4 * s + lk, nk - kn - 1)) * f];
+            // columns past nk meet y = 0
+#pragma unroll
+            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(yv[kc + 4 * s + lk][l16], cur[s], acc);
         }
+        if (col < nrow)
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) xv[col][MfmaAcc<T>::row(lk, qq)] = acc[qq];
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + col] = old[q] - acc[q];
+            }
         __syncthreads();
+    }
+}
+
+// Backward, step 1 of block K: x of sub-block b, x[pb + m] = sum_{k >= m} X[k][pb + m] z[k] over
+// the block's sub-blocks c >= b (z final: the footer and every block to the right are applied),
+// as x^T = z^T X; x goes to Y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
+                                                  T *__restrict__ Y, const T *__restrict__ Z, int K, int n, int nrhs,
+                                                  const int *__restrict__ cols) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pc + k][pb + m]
+    __shared__ T zz[kPanel][kRhsChunk + 1];  // zz[k][rr] = z[pc + k][c0 + rr]
+    const PanelTask tk = tasks[blockIdx.x];
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
+    if (nc == 0) return;
+    const int b = tk.blk, f = tk.f, w = tk.w;
+    const int bs = K * kBlock, be = min(bs + kBlock, w);
+    const int pb = bs + kPanel * b, nbb = min(kPanel, be - pb);
+    const int nsub = (be - bs + kPanel - 1) / kPanel;
+    const T *M = arena + tk.off;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int col = kPanel / 4 * wv + l16;
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        typename MfmaAcc<T>::type acc;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) acc[qq] = T(0);
-        mfma_k64<T>(acc, nb, [&](int s) { return xv[4 * s + lk][l16]; }, [&](int s) { return blk[4 * s + lk][col]; });
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        T ti[kPanel / 4], tz[4];
+        tile_load<T>(ti, M, f, pb, pb, nbb, nbb);
+        vec_load<T>(tz, Z + (int64_t)c0 * n + tk.fc + pb, n, nbb, nr);
+        for (int c = b; c < nsub; ++c) {
+            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+            __syncthreads();
+            if (c == b) tile_store_inv<T>(inv, ti, nbb);
+            else tile_store<T>(inv, ti, nbb, ncc);
+            vec_store<T>(zz, tz, ncc, nr);
+            __syncthreads();
+            if (c + 1 < nsub) {
+                const int pn = pc + kPanel, nn = min(kPanel, be - pn);
+                tile_load<T>(ti, M, f, pb, pn, nbb, nn);
+                vec_load<T>(tz, Z + (int64_t)c0 * n + tk.fc + pn, n, nn, nr);
+            }
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-            const int rr = MfmaAcc<T>::row(lk, qq);
-            if (rr < nr) Z[(int64_t)(c0 + rr) * n + fc + cb * kPanel + col] = old[qq] - acc[qq];
+            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
         }
+        if (col < nbb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pb + col] = acc[q];
+            }
+        __syncthreads();
+    }
+}
+
+// Backward, step 2 of block K: every 64-column block cb to its left, z[64 cb + j] -= sum_k
+// L[k][64 cb + j] x[k] over the block's rows k, as (z_cb)^T -= x^T L[block rows, cb] (L tiles
+// staged through LDS, the next one's loads in flight).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
+                                                 const T *__restrict__ Y, T *__restrict__ Z, int K, int n, int nrhs,
+                                                 const int *__restrict__ cols) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    __shared__ T blk[kPanel][kPanel + 1];    // blk[k][c] = L[pk + k][64 cb + c]
+    __shared__ T xv[kBlock][kRhsChunk + 1];  // xv[k][rr] = x[bs + k][c0 + rr]
+    const PanelTask tk = tasks[blockIdx.x];
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
+    if (nc == 0) return;
+    const int cb = tk.blk, f = tk.f, w = tk.w;
+    const int bs = K * kBlock, be = min(bs + kBlock, w), nk = be - bs;
+    const T *M = arena + tk.off;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int col = kPanel / 4 * wv + l16;
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        T old[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + cb * kPanel + col];
+        T tb[kPanel / 4];
+        tile_load<T>(tb, M, f, bs, cb * kPanel, min(kPanel, nk), kPanel);
+        for (int idx = tid; idx < kBlock * kRhsChunk; idx += 256) {
+            const int k = idx % kBlock, rr = idx / kBlock;
+            const T x = Y[(int64_t)(c0 + min(rr, nr - 1)) * n + tk.fc + bs + min(k, nk - 1)];
+            xv[k][rr] = (k < nk && rr < nr) ? x : T(0);
+        }
+        typename MfmaAcc<T>::type acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        for (int kc = 0; kc < nk; kc += kPanel) {
+            __syncthreads();
+            tile_store<T>(blk, tb, min(kPanel, nk - kc), kPanel);
+            __syncthreads();
+            const int kn = kc + kPanel;
+            if (kn < nk) tile_load<T>(tb, M, f, bs + kn, cb * kPanel, min(kPanel, nk - kn), kPanel);
+#pragma unroll
+            for (int s = 0; s < kPanel / 4; ++s)
+                acc = MfmaAcc<T>::step(xv[kc + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = MfmaAcc<T>::row(lk, q);
+            if (rr < nr) Z[(int64_t)(c0 + rr) * n + tk.fc + cb * kPanel + col] = old[q] - acc[q];
+        }
+        __syncthreads();
     }
 }
 
@@ -2559,6 +2716,13 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             prev_side = nullptr;
         }
     }
+    // inverses of the 256-column diagonal blocks for the solves, by distance of the sub-blocks
+    for (int d = 0; d < 3; ++d) {
+        const int t0 = P.inv_ptr[d], t1 = P.inv_ptr[d + 1];
+        if (t1 > t0)
+            hipLaunchKernelGGL(k_inv256<T>, dim3(t1 - t0), dim3(256), 0, st, P.inv_tasks.p + t0, arena, P.f.p, P.w.p,
+                               P.off.p);
+    }
     MG_HIP(hipGetLastError());
 }
 
@@ -2787,11 +2951,15 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
                                P.first.p, P.child_ptr.p, P.children.p, P.child_recs.p, P.inv_off.p, P.inv_map.p,
                                P.vbase.p, V, Y, n, nrhs, P.front_sys.p, cols);
-        for (int q = 0; q + 1 < (int)P.lvl_sfwd_ptr[L].size(); ++q) {
-            const int s0 = P.lvl_sfwd_ptr[L][q], s1 = P.lvl_sfwd_ptr[L][q + 1];
-            if (s1 > s0)
-                hipLaunchKernelGGL(k_fwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sfwd_tasks.p + s0, arena, V, Y,
-                                   q * kPanel, n, nrhs, cols);
+        for (int J = 0; J + 1 < (int)P.lvl_fa_ptr[L].size(); ++J) {
+            const int a0 = P.lvl_fa_ptr[L][J], a1 = P.lvl_fa_ptr[L][J + 1];
+            const int u0 = P.lvl_fb_ptr[L][J], u1 = P.lvl_fb_ptr[L][J + 1];
+            if (a1 > a0)
+                hipLaunchKernelGGL(k_fwd_diag<T>, dim3(a1 - a0), dim3(256), 0, st, P.fa_tasks.p + a0, arena, V, Y, J, n,
+                                   nrhs, cols);
+            if (u1 > u0)
+                hipLaunchKernelGGL(k_fwd_upd<T>, dim3(u1 - u0), dim3(256), 0, st, P.fb_tasks.p + u0, arena, V, Y, J, n,
+                                   nrhs, cols);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -2811,11 +2979,15 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
         if (r1 > r0)
             hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p,
                                P.first.p, N.diag.p, Y, part, Z, n, nrhs, P.front_sys.p, cols);
-        for (int q = (int)P.lvl_sbwd_ptr[L].size() - 2; q >= 0; --q) {
-            const int s0 = P.lvl_sbwd_ptr[L][q], s1 = P.lvl_sbwd_ptr[L][q + 1];
-            if (s1 > s0)
-                hipLaunchKernelGGL(k_bwd_panel<T>, dim3(s1 - s0), dim3(256), 0, st, P.sbwd_tasks.p + s0, arena, Y, Z, q,
-                                   n, nrhs, cols);
+        for (int K = (int)P.lvl_ba_ptr[L].size() - 2; K >= 0; --K) {
+            const int a0 = P.lvl_ba_ptr[L][K], a1 = P.lvl_ba_ptr[L][K + 1];
+            const int u0 = P.lvl_bb_ptr[L][K], u1 = P.lvl_bb_ptr[L][K + 1];
+            if (a1 > a0)
+                hipLaunchKernelGGL(k_bwd_diag<T>, dim3(a1 - a0), dim3(256), 0, st, P.ba_tasks.p + a0, arena, Y, Z, K, n,
+                                   nrhs, cols);
+            if (u1 > u0)
+                hipLaunchKernelGGL(k_bwd_upd<T>, dim3(u1 - u0), dim3(256), 0, st, P.bb_tasks.p + u0, arena, Y, Z, K, n,
+                                   nrhs, cols);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
