@@ -367,27 +367,35 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             }
         }
         P->max_foot_tasks = std::max(P->max_foot_tasks, (int)foot.size() - foot_base);
+        // per block J: tasks [ptr[2J], ptr[2J + 1]) are those of blocks of at most 64 columns
+        // (one tile in registers), [ptr[2J + 1], ptr[2J + 2]) those of wider blocks
+        // (kernels with NS = 1 and NS = kSub)
         std::vector<int> pfa{(int)fa.size()}, pfb{(int)fb.size()}, pba{(int)ba.size()}, pbb{(int)bb.size()};
         const int nblk = (npan + kBlock / kPanel - 1) / (kBlock / kPanel);
         for (int J = 0; J < nblk; ++J) {
+            std::vector<int2> t[4][2];  // fa, fb, ba, bb x (narrow, wide)
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
                 if (w <= J * kBlock || f <= kSmallFront) continue;
                 const int bs = J * kBlock, be = std::min(bs + kBlock, w);
                 const int nsub = (be - bs + kPanel - 1) / kPanel;
+                const int wide = nsub > 1;
                 for (int a = 0; a < nsub; ++a) {
-                    fa.push_back(make_int2(g, a));
-                    ba.push_back(make_int2(g, a));
+                    t[0][wide].push_back(make_int2(g, a));
+                    t[2][wide].push_back(make_int2(g, a));
                 }
-                for (int rb = 0; rb < (f - be + kTile - 1) / kTile; ++rb) fb.push_back(make_int2(g, rb));
-                for (int cb = 0; cb < bs / kPanel; ++cb) bb.push_back(make_int2(g, cb));
+                for (int rb = 0; rb < (f - be + kTile - 1) / kTile; ++rb) t[1][wide].push_back(make_int2(g, rb));
+                for (int cb = 0; cb < bs / kPanel; ++cb) t[3][wide].push_back(make_int2(g, cb));
                 for (int a = 1; a < nsub; ++a)
                     for (int b = 0; b < a; ++b) inv_d[a - b - 1].push_back(make_int4(g, J, a, b));
             }
-            pfa.push_back((int)fa.size());
-            pfb.push_back((int)fb.size());
-            pba.push_back((int)ba.size());
-            pbb.push_back((int)bb.size());
+            std::vector<int2> *dst[4] = {&fa, &fb, &ba, &bb};
+            std::vector<int> *ptr[4] = {&pfa, &pfb, &pba, &pbb};
+            for (int k = 0; k < 4; ++k)
+                for (int c = 0; c < 2; ++c) {
+                    dst[k]->insert(dst[k]->end(), t[k][c].begin(), t[k][c].end());
+                    ptr[k]->push_back((int)dst[k]->size());
+                }
         }
         P->lvl_fa_ptr.push_back(pfa);
         P->lvl_fb_ptr.push_back(pfb);
@@ -1953,10 +1961,18 @@ __global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, 
             }
 }
 
+// The block solves issue every global load of a task before the first one is consumed (up to
+// four 64x64 tiles and their vector blocks per lane, clamped addresses, masks applied at the
+// LDS stores): a task then costs one memory round trip instead of one per 64 columns.
+// NS = the 64x64 tiles a task holds in registers: 1 for the tasks of blocks of at most 64
+// columns, kSub otherwise; the two kinds go to separate launches so that the many narrow fronts
+// of the lower levels keep a high occupancy.
+constexpr int kSub = kBlock / kPanel;  // 64-column sub-blocks of a 256-column block
+
 // Forward, step 1 of block J: y of sub-block a, y[pa + k] = sum_{m <= k} X[pa + k][m] v[m] over
 // the block's sub-blocks c <= a, on the matrix cores as y^T = v^T X^T (right-hand sides as the
 // 16 MFMA rows, wave v: columns 16 v .. 16 v + 15 of the sub-block); y goes to Y.
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   const T *__restrict__ V, T *__restrict__ Y, int J, int n, int nrhs,
                                                   const int *__restrict__ cols) {
@@ -1975,24 +1991,26 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
     const int col = kPanel / 4 * wv + l16;
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
+        T ti[NS][kPanel / 4], tv[NS][4];
+#pragma unroll
+        for (int c = 0; c < NS; ++c)
+            if (c <= a) {  // uniform
+                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+                tile_load<T>(ti[c], M, f, pc, pa, ncc, na);
+                vec_load<T>(tv[c], Vg + (int64_t)c0 * f + pc, f, ncc, nr);
+            }
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        T ti[kPanel / 4], tv[4];
-        tile_load<T>(ti, M, f, bs, pa, min(kPanel, be - bs), na);
-        vec_load<T>(tv, Vg + (int64_t)c0 * f + bs, f, min(kPanel, be - bs), nr);
-        for (int c = 0; c <= a; ++c) {
+#pragma unroll
+        for (int c = 0; c < NS; ++c) {
+            if (c > a) break;
             const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
             __syncthreads();
-            if (c == a) tile_store_inv<T>(inv, ti, na);
-            else tile_store<T>(inv, ti, ncc, na);
-            vec_store<T>(vv, tv, ncc, nr);
+            if (c == a) tile_store_inv<T>(inv, ti[c], na);
+            else tile_store<T>(inv, ti[c], ncc, na);
+            vec_store<T>(vv, tv[c], ncc, nr);
             __syncthreads();
-            if (c < a) {  // the next sub-block in flight during this one's products
-                const int pn = pc + kPanel, nn = min(kPanel, be - pn);
-                tile_load<T>(ti, M, f, pn, pa, nn, na);
-                vec_load<T>(tv, Vg + (int64_t)c0 * f + pn, f, nn, nr);
-            }
 #pragma unroll
             for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
         }
@@ -2007,14 +2025,30 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
     }
 }
 
+// yv[k][rr] = V[c0 + rr][base + k] (k < nk, rr < nr; zero elsewhere) for k < 256, from a
+// column-major n-row vector block: thread k loads its 16 values together.
+template <typename T>
+__device__ __forceinline__ void block_vec_lds(T (*yv)[kRhsChunk + 1], const T *__restrict__ src, int64_t ld, int nk,
+                                              int nr) {
+    const int k = threadIdx.x;  // blockDim = kBlock
+    const T *p = src + min(k, nk - 1);
+    T v[kRhsChunk];
+#pragma unroll
+    for (int rr = 0; rr < kRhsChunk; ++rr) v[rr] = p[(int64_t)min(rr, nr - 1) * ld];
+#pragma unroll
+    for (int rr = 0; rr < kRhsChunk; ++rr) pin(v[rr]);
+#pragma unroll
+    for (int rr = 0; rr < kRhsChunk; ++rr) yv[k][rr] = (k < nk && rr < nr) ? v[rr] : T(0);
+}
+
 // Forward, step 2 of block J: the 64 rows r0 + i below the block, V[r0 + i] -= sum_k
 // L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (the L
 // operand read straight from the front, 16 rows of a column per load).
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  T *__restrict__ V, const T *__restrict__ Y, int J, int n, int nrhs,
                                                  const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    static_assert(kRhsChunk == 16 && kBlock == 256, "one 16-row MFMA tile of right-hand sides; a thread per column");
     __shared__ T yv[kBlock][kRhsChunk + 1];  // yv[k][rr] = y[bs + k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
@@ -2027,35 +2061,29 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;  // this lane's row of the block
     const T *lrow = M + r0 + min(col, nrow - 1);
+    // the block's L rows, all 256 columns (clamped; columns past nk meet y = 0)
+    T bl[NS][kPanel / 4];
+#pragma unroll
+    for (int c = 0; c < NS; ++c)
+#pragma unroll
+        for (int s = 0; s < kPanel / 4; ++s) bl[c][s] = lrow[(int64_t)(bs + min(kPanel * c + 4 * s + lk, nk - 1)) * f];
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         T old[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(col, nrow - 1)];
-        T bl[kPanel / 4];
-#pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) bl[s] = lrow[(int64_t)(bs + min(4 * s + lk, nk - 1)) * f];
-        for (int idx = tid; idx < kBlock * kRhsChunk; idx += 256) {
-            const int k = idx % kBlock, rr = idx / kBlock;
-            const T y = Y[(int64_t)(c0 + min(rr, nr - 1)) * n + tk.fc + bs + min(k, nk - 1)];
-            yv[k][rr] = (k < nk && rr < nr) ? y : T(0);
-        }
+        block_vec_lds<T>(yv, Y + (int64_t)c0 * n + tk.fc + bs, n, nk, nr);
         __syncthreads();
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        for (int kc = 0; kc < nk; kc += kPanel) {
-            T cur[kPanel / 4];
 #pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s) cur[s] = bl[s];
-            const int kn = kc + kPanel;
-            if (kn < nk)  // the next 64 columns in flight during this chunk's products
+        for (int c = 0; c < NS; ++c) {
+            if (kPanel * c >= nk) break;  // uniform
 #pragma unroll
-                for (int s = 0; s < kPanel / 4; ++s) bl[s] = lrow[(int64_t)(bs + kn + min(4 * s + lk, nk - kn - 1)) * f];
-            // columns past nk meet y = 0
-#pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(yv[kc + 4 * s + lk][l16], cur[s], acc);
+            for (int s = 0; s < kPanel / 4; ++s)
+                acc = MfmaAcc<T>::step(yv[kPanel * c + 4 * s + lk][l16], bl[c][s], acc);
         }
         if (col < nrow)
 #pragma unroll
@@ -2070,7 +2098,7 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
 // Backward, step 1 of block K: x of sub-block b, x[pb + m] = sum_{k >= m} X[k][pb + m] z[k] over
 // the block's sub-blocks c >= b (z final: the footer and every block to the right are applied),
 // as x^T = z^T X; x goes to Y.
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   T *__restrict__ Y, const T *__restrict__ Z, int K, int n, int nrhs,
                                                   const int *__restrict__ cols) {
@@ -2089,24 +2117,26 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
     const int col = kPanel / 4 * wv + l16;
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
+        T ti[NS][kPanel / 4], tz[NS][4];
+#pragma unroll
+        for (int e = 0; e < NS; ++e)
+            if (b + e < nsub) {  // uniform
+                const int pc = bs + kPanel * (b + e), ncc = min(kPanel, be - pc);
+                tile_load<T>(ti[e], M, f, pb, pc, nbb, ncc);
+                vec_load<T>(tz[e], Z + (int64_t)c0 * n + tk.fc + pc, n, ncc, nr);
+            }
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        T ti[kPanel / 4], tz[4];
-        tile_load<T>(ti, M, f, pb, pb, nbb, nbb);
-        vec_load<T>(tz, Z + (int64_t)c0 * n + tk.fc + pb, n, nbb, nr);
-        for (int c = b; c < nsub; ++c) {
-            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+            if (b + e >= nsub) break;  // uniform
+            const int pc = bs + kPanel * (b + e), ncc = min(kPanel, be - pc);
             __syncthreads();
-            if (c == b) tile_store_inv<T>(inv, ti, nbb);
-            else tile_store<T>(inv, ti, nbb, ncc);
-            vec_store<T>(zz, tz, ncc, nr);
+            if (e == 0) tile_store_inv<T>(inv, ti[e], nbb);
+            else tile_store<T>(inv, ti[e], nbb, ncc);
+            vec_store<T>(zz, tz[e], ncc, nr);
             __syncthreads();
-            if (c + 1 < nsub) {
-                const int pn = pc + kPanel, nn = min(kPanel, be - pn);
-                tile_load<T>(ti, M, f, pb, pn, nbb, nn);
-                vec_load<T>(tz, Z + (int64_t)c0 * n + tk.fc + pn, n, nn, nr);
-            }
 #pragma unroll
             for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
         }
@@ -2122,12 +2152,12 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
 
 // Backward, step 2 of block K: every 64-column block cb to its left, z[64 cb + j] -= sum_k
 // L[k][64 cb + j] x[k] over the block's rows k, as (z_cb)^T -= x^T L[block rows, cb] (L tiles
-// staged through LDS, the next one's loads in flight).
-template <typename T>
+// staged through LDS).
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  const T *__restrict__ Y, T *__restrict__ Z, int K, int n, int nrhs,
                                                  const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    static_assert(kRhsChunk == 16 && kBlock == 256, "one 16-row MFMA tile of right-hand sides; a thread per row");
     __shared__ T blk[kPanel][kPanel + 1];    // blk[k][c] = L[pk + k][64 cb + c]
     __shared__ T xv[kBlock][kRhsChunk + 1];  // xv[k][rr] = x[bs + k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
@@ -2138,31 +2168,30 @@ __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ t
     const T *M = arena + tk.off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
+    T tb[NS][kPanel / 4];
+#pragma unroll
+    for (int c = 0; c < NS; ++c)
+        if (kPanel * c < nk)  // uniform
+            tile_load<T>(tb[c], M, f, bs + kPanel * c, cb * kPanel, min(kPanel, nk - kPanel * c), kPanel);
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         T old[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + cb * kPanel + col];
-        T tb[kPanel / 4];
-        tile_load<T>(tb, M, f, bs, cb * kPanel, min(kPanel, nk), kPanel);
-        for (int idx = tid; idx < kBlock * kRhsChunk; idx += 256) {
-            const int k = idx % kBlock, rr = idx / kBlock;
-            const T x = Y[(int64_t)(c0 + min(rr, nr - 1)) * n + tk.fc + bs + min(k, nk - 1)];
-            xv[k][rr] = (k < nk && rr < nr) ? x : T(0);
-        }
+        block_vec_lds<T>(xv, Y + (int64_t)c0 * n + tk.fc + bs, n, nk, nr);
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        for (int kc = 0; kc < nk; kc += kPanel) {
+#pragma unroll
+        for (int c = 0; c < NS; ++c) {
+            if (kPanel * c >= nk) break;
             __syncthreads();
-            tile_store<T>(blk, tb, min(kPanel, nk - kc), kPanel);
+            tile_store<T>(blk, tb[c], min(kPanel, nk - kPanel * c), kPanel);
             __syncthreads();
-            const int kn = kc + kPanel;
-            if (kn < nk) tile_load<T>(tb, M, f, bs + kn, cb * kPanel, min(kPanel, nk - kn), kPanel);
 #pragma unroll
             for (int s = 0; s < kPanel / 4; ++s)
-                acc = MfmaAcc<T>::step(xv[kc + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
+                acc = MfmaAcc<T>::step(xv[kPanel * c + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -2730,8 +2759,15 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                       Numeric *&num) {
     if (precision != MG_FP64 && precision != MG_FP32) throw Error(MG_E_ARG, "Unknown factor precision.");
     ensure_plan(S);
-    if (S.plan->max_w > 8192)
-        throw Error(MG_E_TOO_DENSE, "A supernode wider than 8192 columns is not supported on the device.");
+    // the reference's limit (UtduNumericFactorization.cs:298-304): the packed form of the
+    // largest front must fit in one array, f (f + 1) / 2 <= int.MaxValue (the device's fused
+    // fronts are checked the same way)
+    const int64_t max_front_length = (int64_t)S.max_front * (S.max_front + 1) / 2;
+    if (max_front_length > INT32_MAX || (int64_t)S.plan->max_f * (S.plan->max_f + 1) / 2 > INT32_MAX)
+        throw Error(MG_E_TOO_DENSE,
+                    "The largest frontal matrix (" + std::to_string(S.max_front) + " x " + std::to_string(S.max_front) +
+                        ") cannot be represented by a single array. The problem is too dense for the direct solver in "
+                        "this formulation. Check the model for equations that connect too many unknowns.");
     const size_t esz = precision == MG_FP32 ? sizeof(float) : sizeof(double);
     Numeric *N = num;
     if (N && (N->sym != &S || N->precision != precision))
@@ -2951,15 +2987,20 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
             hipLaunchKernelGGL(k_fwd_init<T>, dim3(i1 - i0), dim3(256), 0, st, P.init_tasks.p + i0, P.f.p, P.w.p,
                                P.first.p, P.child_ptr.p, P.children.p, P.child_recs.p, P.inv_off.p, P.inv_map.p,
                                P.vbase.p, V, Y, n, nrhs, P.front_sys.p, cols);
-        for (int J = 0; J + 1 < (int)P.lvl_fa_ptr[L].size(); ++J) {
-            const int a0 = P.lvl_fa_ptr[L][J], a1 = P.lvl_fa_ptr[L][J + 1];
-            const int u0 = P.lvl_fb_ptr[L][J], u1 = P.lvl_fb_ptr[L][J + 1];
-            if (a1 > a0)
-                hipLaunchKernelGGL(k_fwd_diag<T>, dim3(a1 - a0), dim3(256), 0, st, P.fa_tasks.p + a0, arena, V, Y, J, n,
-                                   nrhs, cols);
-            if (u1 > u0)
-                hipLaunchKernelGGL(k_fwd_upd<T>, dim3(u1 - u0), dim3(256), 0, st, P.fb_tasks.p + u0, arena, V, Y, J, n,
-                                   nrhs, cols);
+        for (int J = 0; 2 * J + 2 < (int)P.lvl_fa_ptr[L].size(); ++J) {
+            const int *pa = &P.lvl_fa_ptr[L][2 * J], *pu = &P.lvl_fb_ptr[L][2 * J];
+            if (pa[1] > pa[0])
+                hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.fa_tasks.p + pa[0],
+                                   arena, V, Y, J, n, nrhs, cols);
+            if (pa[2] > pa[1])
+                hipLaunchKernelGGL((k_fwd_diag<T, kSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.fa_tasks.p + pa[1],
+                                   arena, V, Y, J, n, nrhs, cols);
+            if (pu[1] > pu[0])
+                hipLaunchKernelGGL((k_fwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.fb_tasks.p + pu[0],
+                                   arena, V, Y, J, n, nrhs, cols);
+            if (pu[2] > pu[1])
+                hipLaunchKernelGGL((k_fwd_upd<T, kSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.fb_tasks.p + pu[1],
+                                   arena, V, Y, J, n, nrhs, cols);
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -2979,15 +3020,20 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
         if (r1 > r0)
             hipLaunchKernelGGL(k_bwd_foot_red<T>, dim3(r1 - r0), dim3(256), 0, st, P.footr_tasks.p + r0, P.w.p,
                                P.first.p, N.diag.p, Y, part, Z, n, nrhs, P.front_sys.p, cols);
-        for (int K = (int)P.lvl_ba_ptr[L].size() - 2; K >= 0; --K) {
-            const int a0 = P.lvl_ba_ptr[L][K], a1 = P.lvl_ba_ptr[L][K + 1];
-            const int u0 = P.lvl_bb_ptr[L][K], u1 = P.lvl_bb_ptr[L][K + 1];
-            if (a1 > a0)
-                hipLaunchKernelGGL(k_bwd_diag<T>, dim3(a1 - a0), dim3(256), 0, st, P.ba_tasks.p + a0, arena, Y, Z, K, n,
-                                   nrhs, cols);
-            if (u1 > u0)
-                hipLaunchKernelGGL(k_bwd_upd<T>, dim3(u1 - u0), dim3(256), 0, st, P.bb_tasks.p + u0, arena, Y, Z, K, n,
-                                   nrhs, cols);
+        for (int K = ((int)P.lvl_ba_ptr[L].size() - 1) / 2 - 1; K >= 0; --K) {
+            const int *pa = &P.lvl_ba_ptr[L][2 * K], *pu = &P.lvl_bb_ptr[L][2 * K];
+            if (pa[1] > pa[0])
+                hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.ba_tasks.p + pa[0],
+                                   arena, Y, Z, K, n, nrhs, cols);
+            if (pa[2] > pa[1])
+                hipLaunchKernelGGL((k_bwd_diag<T, kSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.ba_tasks.p + pa[1],
+                                   arena, Y, Z, K, n, nrhs, cols);
+            if (pu[1] > pu[0])
+                hipLaunchKernelGGL((k_bwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.bb_tasks.p + pu[0],
+                                   arena, Y, Z, K, n, nrhs, cols);
+            if (pu[2] > pu[1])
+                hipLaunchKernelGGL((k_bwd_upd<T, kSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.bb_tasks.p + pu[1],
+                                   arena, Y, Z, K, n, nrhs, cols);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
