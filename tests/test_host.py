@@ -224,3 +224,70 @@ def test_general_line_cross_equals_oracle():
     b = M.CSRMatrix.from_dense(np.array([[1.0, 2.0], [3.0, 0.0]]))
     with pytest.raises(M.InvalidOperationError):
         b.line_cross(np.zeros(2), 1.0, 1)  # structurally zero diagonal
+
+
+# ---- known answers (hand-traced from the reference) and full-size workloads ----
+
+def _product_symbolic(a_oracle, ordering=M.AMD, limit=64):
+    rp, ci, va = a_oracle.arrays()
+    pm = M.SymmetricCSRMatrix(a_oracle.n, rp.copy(), ci.copy(), va.copy())
+    return M.SymmetricUtduSolver(M.UtduSolverOptions(ordering=ordering, supernode_amalgamation_limit=limit)).analyze(pm)
+
+
+def test_product_amd_matches_hand_traces():
+    from test_oracle import AMD_KNOWN, graph_csr
+    for name, (n, edges, order) in AMD_KNOWN.items():
+        sym = _product_symbolic(graph_csr(n, edges))
+        assert [int(x) for x in sym.structure()["fill_order"]] == order, name
+    sym = _product_symbolic(graph_csr(300, [(0, k) for k in range(1, 300)]))
+    assert [int(x) for x in sym.structure()["fill_order"]] == list(range(299, 0, -1)) + [0]
+
+
+def test_product_amalgamation_matches_known_answers():
+    from test_oracle import AMALGAMATION_KNOWN
+    for name, make, limit, first in AMALGAMATION_KNOWN:
+        sym = _product_symbolic(make(), ordering=M.NATURAL, limit=limit)
+        assert [int(x) for x in sym.structure()["first"]] == first, name
+
+
+def _assert_symbolic_equal(pm, oc, label):
+    sym = M.SymmetricUtduSolver().analyze(pm)
+    st = sym.structure()
+    os_ = O.Symbolic(oc)
+    for key, mine, ref in [("fill_order", st["fill_order"], os_.amd), ("perm", sym.permutation, os_.perm),
+                           ("iperm", sym.inverse_permutation, os_.iperm), ("etree", st["etree"], os_.etree),
+                           ("colcount", st["colcount"], os_.colcount), ("first", st["first"], os_.first),
+                           ("sparent", st["sparent"], os_.sparent), ("pat_ptr", st["pat_ptr"], os_.pat_ptr),
+                           ("pat_rows", st["pat_rows"], os_.pat_rows)]:
+        assert np.array_equal(mine, ref), (label, key)
+    inf = sym.info()
+    assert (inf["factor_entry_count"], inf["factor_operation_count"], inf["peak_contribution_bytes"]) == \
+        (os_.factor_entries, os_.factor_ops, 8 * os_.peak_contribution), label
+
+
+@pytest.mark.parametrize("config", ["c1", "c2", "c3", "c5"])
+def test_symbolic_equals_oracle_full_size(config):
+    """The benchmark workloads at full size: permutation, elimination tree, column counts,
+    supernodes, patterns and costs bit for bit against the oracle."""
+    if config == "c3":
+        n, r, c, v, clamped, _ = G.config_c3()
+        pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+        pm.line_cross(np.zeros(n), 0.0, clamped)
+    else:
+        n, r, c, v, _ = {"c1": G.config_c1, "c2": G.config_c2, "c5": G.config_c5}[config]()
+        pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+    oc = O.Csr.from_csr(n, pm.row_pointers, pm.column_indices, pm.values)
+    _assert_symbolic_equal(pm, oc, config)
+
+
+def test_symbolic_equals_oracle_all_batched_systems():
+    """All 1,024 systems of the batched workload, each analysed by the product and the oracle;
+    the batch's merged analysis keeps each system's permutation."""
+    sysl = G.config_c4()
+    mats = []
+    for t, (n, r, c, v, _) in enumerate(sysl):
+        pm = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+        _assert_symbolic_equal(pm, O.Csr.from_coo(n, r, c, v), t)
+        mats.append(pm)
+    batch = M.UtduBatch(mats)
+    assert batch.count == 1024 and batch.info()["size"] == sum(m.size for m in mats)

@@ -219,24 +219,95 @@ def test_etree_colcounts_patterns_brute_force(args, ordering):
     assert s.factor_ops == int(sum(((f[q] - np.arange(w[q])) ** 2).sum() for q in range(s.nsuper)))
 
 
-def test_amd_hand_traces():
+def graph_csr(n, edges):
+    """Upper CSR of a graph (unit off-diagonals, dominant diagonal): only the pattern matters."""
+    r = np.array([min(e) for e in edges] + list(range(n)), dtype=np.int64)
+    c = np.array([max(e) for e in edges] + list(range(n)), dtype=np.int64)
+    v = np.concatenate([np.full(len(edges), -1.0), np.full(n, 10.0)])
+    return csr(n, r, c, v)
+
+
+def amd_of(a):
+    rp, ci, _ = a.arrays()
+    return [int(x) for x in O.amd(a.n, rp, ci)]
+
+
+# AMD orders traced by hand through ApproximateMinimumDegreeOrdering.Run
+# (AMD/ApproximateMinimumDegreeOrdering.cs:157-224) with the reference's data structures:
+# neighbour lists ascending (SymmetricPatternGraph.FromMatrix, AMD/SymmetricPatternGraph.cs:71-121),
+# degree buckets pushed and popped at the head (AMD/DegreeBuckets.cs:57-69,96-110), so the last
+# vertex inserted with the minimum degree is the next pivot.
+AMD_KNOWN = {
+    # 4 (last of bucket 1) first; each pivot's element absorbs the previous one (:384-389); pivot 1
+    # mass-eliminates 0, emitted right after it (:494-507, MemberLists.Append)
+    "path5": (5, [(0, 1), (1, 2), (2, 3), (3, 4)], [4, 3, 2, 1, 0]),
+    # leaves 7, 6, ... 2 pivot in turn (each older element aggressively absorbed, :466-472); the
+    # centre's degree falls to 1 and, inserted after leaf 1, it pops first and mass-eliminates 1
+    "star8": (8, [(0, k) for k in range(1, 8)], [7, 6, 5, 4, 3, 2, 0, 1]),
+    # components {0, 2} and {1, 3, 4}: 4 pivots, then 3 (degree 1 after 4) mass-eliminates 1;
+    # 2 pops before 0 (LIFO) and mass-eliminates it
+    "two_components": (5, [(0, 2), (1, 3), (3, 4)], [4, 3, 1, 2, 0]),
+    # the 8x8 CgComparisonData pattern (Tests/BiCGSTAB/CgComparisonData.cs:8-18): isolated 0, 1, 3
+    # are emitted at once (:243-253); pivot 6 makes 4 and 5 indistinguishable (same lists after
+    # the element, :531-592), supervariable 5 = {5, 4} pops next and mass-eliminates 7 and 2
+    "cg8": (8, [(2, 4), (2, 5), (2, 7), (4, 5), (4, 6), (4, 7), (5, 6), (5, 7)], [0, 1, 3, 6, 5, 4, 7, 2]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(AMD_KNOWN))
+def test_amd_hand_traces(name):
+    n, edges, order = AMD_KNOWN[name]
+    assert amd_of(graph_csr(n, edges)) == order
+
+
+def test_amd_hand_traces_golden_pattern_matches_fixture(golden):
+    A = np.array(golden["matrix"])
+    r, c = np.nonzero(np.triu(A, 1))
+    assert sorted(zip(r.tolist(), c.tolist())) == sorted(AMD_KNOWN["cg8"][1])
+
+
+def test_amd_isolated_and_dense_rows():
     # isolated vertices first, then pivot 1 mass-eliminates 0 (singular test matrix, natural ids)
     a = csr(4, [0, 1, 0, 2, 3], [0, 1, 1, 2, 3], [1.0, 1.0, 1.0, 2.0, 3.0])
-    rp, ci, _ = a.arrays()
-    assert list(O.amd(4, rp, ci)) == [2, 3, 1, 0]
-    # path 0-1-2-3-4: degree-1 ends first, LIFO bucket -> 4 popped before 0
-    p = csr(*tridiagonal(5, 4.0, -1.0))
-    rp, ci, _ = p.arrays()
-    perm = O.amd(5, rp, ci)
-    assert sorted(perm) == list(range(5))
-    # star with dense centre (degree 299 > max(16, 10 sqrt(300))): centre ordered last
-    r = np.concatenate([np.zeros(299, int), np.arange(300)])
-    c = np.concatenate([np.arange(1, 300), np.arange(300)])
-    v = np.concatenate([np.full(299, -1.0), [301.0], np.full(299, 2.0)])
-    s = csr(300, r, c, v)
-    rp, ci, _ = s.arrays()
-    perm = O.amd(300, rp, ci)
-    assert perm[-1] == 0 and sorted(perm) == list(range(300))
+    assert amd_of(a) == [2, 3, 1, 0]
+    # star with a dense centre (degree 299 > max(16, 10 sqrt(300)), :233-261): the centre is left
+    # out and ordered last; every leaf then has no live neighbour, an empty element, and the
+    # leaves pop from the degree-1 bucket last-inserted first: 299, 298, ..., 1, then 0
+    edges = [(0, k) for k in range(1, 300)]
+    assert amd_of(graph_csr(300, edges)) == list(range(299, 0, -1)) + [0]
+
+
+def band_csr(n, b):
+    r, c = [], []
+    for i in range(n):
+        for j in range(i, min(n, i + b + 1)):
+            r.append(i)
+            c.append(j)
+    r, c = np.array(r), np.array(c)
+    return csr(n, r, c, np.where(r == c, 4.0 * b, -1.0))
+
+
+# Supernode boundaries derived by hand from SupernodalStructure.Amalgamate / IsMergeWorthwhile
+# (UCHOL/SupernodalStructure.cs:175-286), natural order.  In a band of half-width b every column
+# before the last b + 1 is its own fundamental supernode (column counts b + 1, :175-193) and the
+# dense tail of b + 1 columns is one.  Merging k single columns stores k(k+b) - k(k-1)/2 entries
+# for k(b+1) nonzeros: added zeros k(k-1)/2, a fraction (k-1)/(k+2b+1).  Tridiagonal (b = 1):
+# 0.2, 0.33, 0.43 for 2, 3, 4 columns (<= 0.8: merged), 0.5 for 5 (> 0.1: not).  b = 100: 16
+# columns accepted (0.069 <= 0.1), 17 rejected (0.073 > 0.05).  b = 500: 48 accepted
+# (0.045 <= 0.05), 49 rejected (more than 48 columns with any added zero).  The amalgamation
+# limit (64 or 32 columns) caps each merge, and a run before the tail stops at the tail.
+AMALGAMATION_KNOWN = [
+    ("tridiagonal10", lambda: band_csr(10, 1), 64, [0, 4, 8, 10]),
+    ("band300_100", lambda: band_csr(300, 100), 64, list(range(0, 193, 16)) + [199, 300]),
+    ("band1200_500", lambda: band_csr(1200, 500), 64, list(range(0, 673, 48)) + [699, 1200]),
+    ("band1200_500_limit32", lambda: band_csr(1200, 500), 32, list(range(0, 673, 32)) + [699, 1200]),
+]
+
+
+@pytest.mark.parametrize("name,make,limit,first", AMALGAMATION_KNOWN, ids=[k[0] for k in AMALGAMATION_KNOWN])
+def test_amalgamation_known_answers(name, make, limit, first):
+    s = O.Symbolic(make(), O.options(ordering=0, amalgamation_limit=limit))
+    assert [int(x) for x in s.first] == first
 
 
 def test_amd_reduces_fill_on_grid():
