@@ -8,7 +8,10 @@ factorization of the resident matrix values + solve of the 8 right-hand sides
 to max relative residual <= 1e-10:
   * headline: FP32 factor (Schur updates on FP32 MFMA) + FP64 iterative refinement;
   * reported beside it: plain FP64 factor + solve on the GPU, and the CPU
-    oracle (serial FP64 restatement of the reference) on a bounded sample.
+    restatement of the reference (oracle/, FP64, threaded with the reference's
+    schedule) timed on the whole workload on the host's cores;
+  * extra keys: the batched workload (1,024 uneven 2D systems, --config c4)
+    and the learned-preconditioner PCG (--config c5), each also runnable alone.
 Inputs (matrix values, right-hand sides) are resident in HBM before timing;
 the symbolic phase (host, bit-exact AMD etc.) is done once and reported
 separately, as the reference's benchmark does (UtduSolverBenchmark.cs:74-77).
@@ -100,9 +103,10 @@ def build_system(config):
     return a, np.asfortranarray(b), desc
 
 
-def bench_batch(args):
+def run_batch(args):
     """1,024 independent 2D 5/9-point systems (side 16..128, shift U[0,0.1], 1..32 RHS each):
-    one step = batched FP32 factorization + solve with FP64 refinement of every system to 1e-10."""
+    one step = batched FP32 factorization + solve with FP64 refinement of every system to 1e-10.
+    Returns the result dict."""
     import torch
 
     import magla_amd as M
@@ -205,13 +209,17 @@ def bench_batch(args):
         "symbolic_s": round(symbolic_s, 3), "roofline": roof,
         "kernels": kernel_table(prof), "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_learned(args):
+def bench_batch(args):
+    print(json.dumps(run_batch(args)), flush=True)
+
+
+def run_learned(args):
     """Learned-preconditioner PCG on 3D 7-point diffusion m^3 with kappa = exp(GRF): pairs from
     Jacobi-PCG snapshots and the direct solver, GPU training, then iterations and wall time to
-    1e-8 against Jacobi-PCG."""
+    1e-8 against Jacobi-PCG.  Returns the result dict."""
     import torch
 
     import magla_amd as M
@@ -306,7 +314,35 @@ def bench_learned(args):
         "train_s": round(train_s, 2), "learned_ms_incl_inference": round(ms_inf, 3),
         "roofline": roof, "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_learned(args):
+    print(json.dumps(run_learned(args)), flush=True)
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPU share this process may use (affinity), at
+    most 16 (the host share of one GPU on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_bicgstab(args):
@@ -411,7 +447,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "bicg"])
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bounded CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="c4: bounded CPU-baseline sample in seconds (0 = skip)")
+    ap.add_argument("--cpu", type=int, default=1, help="c3: time the threaded CPU restatement on the whole workload")
+    ap.add_argument("--extras", type=int, default=1, help="c3: add the batched (c4) and learned-PCG (c5) results")
     ap.add_argument("--graph", type=int, default=0, help="replay the factorization from a captured hipGraph")
     ap.add_argument("--autotune", type=int, default=1, help="pick Schur tile shapes per front-size bucket first")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run one plain copy of 256 MiB first (PMC scale)")
@@ -519,22 +558,30 @@ def main():
     log(f"[bench] kernels: {json.dumps(kernels)}")
 
     cpu = None
-    if args.cpu_seconds > 0:
+    fwd_err_ref = None
+    if args.cpu:
+        # the reference's FP64 factorization and solve restated in C (oracle/), threaded with the
+        # reference's schedule, timed whole on the host's cores: no extrapolation
         import oracle as O
+        threads = cpu_threads()
         oc = O.Csr.from_csr(n, a.row_pointers, a.column_indices, a.values)
         osym = O.Symbolic(oc)
-        osym.opts = O.options(time_budget_s=args.cpu_seconds)
         c0 = time.time()
-        onum = O.Numeric(osym, oc)
-        cpu_s = time.time() - c0
-        frac = onum.ops_done / osym.factor_ops
-        est_ms = cpu_s / frac * 1e3
-        cpu = {"value": est_ms, "unit": "ms", "cores": 1, "kind": "port",
-               "sample": (f"oracle (serial FP64 C restatement of UtduNumericFactorization) factored the first "
-                          f"{onum.supernodes_done}/{osym.nsuper} supernodes in postorder = {100 * frac:.2f}% of "
-                          f"FactorOperationCount in {cpu_s:.1f}s; value = that time scaled by operation count to "
-                          f"the whole factorization (solve omitted: <1% of the flops)")}
-        log(f"[bench] cpu sample: {cpu}")
+        onum = O.Numeric(osym, oc, threads=threads)
+        fact_s = time.time() - c0
+        c0 = time.time()
+        x_ref = onum.solve_many(b, threads=threads)
+        solve_s = time.time() - c0
+        rel_ref = max(np.linalg.norm(b[:, j] - oc.spmv(x_ref[:, j])) / np.linalg.norm(b[:, j]) for j in range(k))
+        x32 = d_x.cpu().numpy().T  # the FP32+IR solution of the forward-error pair above
+        fwd_err_ref = float(np.linalg.norm(x32 - x_ref) / np.linalg.norm(x_ref))
+        cpu = {"value": round((fact_s + solve_s) * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+               "cpu": cpu_model(),
+               "sample": (f"the whole workload: oracle (C restatement of UtduNumericFactorization, -O3 AVX2, "
+                          f"the reference's tree schedule and parallel trailing updates over {threads} threads) "
+                          f"FP64 factorization {fact_s:.2f}s ({osym.factor_ops / fact_s / 1e9:.1f} GFLOP/s) + "
+                          f"{k} solves {solve_s:.2f}s (one thread per right-hand side); relres {rel_ref:.2e}")}
+        log(f"[bench] cpu baseline: {cpu}")
 
     gflops32 = info["factor_operation_count"] / (ms32 * 1e-3) / 1e9
     out = {
@@ -563,12 +610,29 @@ def main():
         "fp32_refinement_steps": reps32[-1]["refinement_steps"],
         "fp64_relres": reps64[-1]["relative_residual"],
         "forward_error_vs_fp64": fwd_err,
+        "forward_error_vs_reference": fwd_err_ref,
         "factor_gflops_fp32": round(gflops32, 1),
         "symbolic_s": round(symbolic_s, 3),
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if args.extras:
+        # the other two benchmark workloads, as extra keys of this line (their own lines: --config c4 / c5)
+        sub = argparse.Namespace(**vars(args))
+        sub.cpu_seconds = 0
+        sub.graph = 1
+        c4 = run_batch(sub)
+        out["c4_batched"] = {k_: c4[k_] for k_ in ("value", "fp64_ms_per_step", "speedup_fp32ir_vs_fp64",
+                                                   "fp32_relres", "fp32_refinement_steps", "cold_ms_per_step")}
+        out["c4_batched"].update(workload=c4["config"]["workload"], hipgraph=True, unit="ms")
+        sub.graph = 0
+        c5 = run_learned(sub)
+        out["c5_learned_pcg"] = {k_: c5[k_] for k_ in ("jacobi_iterations", "jacobi_ms", "learned_iterations",
+                                                       "learned_ms", "learned_ms_incl_inference", "iteration_ratio",
+                                                       "time_ratio", "fell_back", "learned_relres", "pairs_s",
+                                                       "train_s")}
+        out["c5_learned_pcg"].update(workload=c5["config"]["workload"], unit="ms")
     print(json.dumps(out), flush=True)
 
 

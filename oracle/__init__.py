@@ -101,6 +101,9 @@ def lib():
         L.or_factorize.argtypes = [C.POINTER(_Sym), C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.POINTER(_Num))]
         L.or_numeric_free.argtypes = [C.POINTER(_Num)]
         L.or_solve.argtypes = [C.POINTER(_Num), _f64p, _f64p]
+        L.or_factorize_mt.argtypes = [C.POINTER(_Sym), C.POINTER(_Csr), C.POINTER(_Opts), C.c_int,
+                                      C.POINTER(C.POINTER(_Num))]
+        L.or_solve_mt.argtypes = [C.POINTER(_Num), C.c_int, _f64p, _f64p, C.c_int]
         L.or_pcg.argtypes = [C.POINTER(_Csr), _f64p, C.c_void_p, C.c_double, C.c_int, C.c_int, _f64p,
                              C.POINTER(_PcgRes)]
         L.or_mpnn_apply.argtypes = [C.POINTER(_Csr), _f64p, _f64p, _f64p]
@@ -255,10 +258,15 @@ class Symbolic:
 
 
 class Numeric:
-    def __init__(self, sym: Symbolic, csr: Csr):
+    def __init__(self, sym: Symbolic, csr: Csr, threads=None):
+        """threads=None: the serial restatement; threads=T: the reference's threaded schedule
+        over T workers (the same factor bit for bit)."""
         self.sym = sym
         p = C.POINTER(_Num)()
-        _check(lib().or_factorize(sym._p, csr._p, C.byref(sym.opts), C.byref(p)), "factorize")
+        if threads is None:
+            _check(lib().or_factorize(sym._p, csr._p, C.byref(sym.opts), C.byref(p)), "factorize")
+        else:
+            _check(lib().or_factorize_mt(sym._p, csr._p, C.byref(sym.opts), int(threads), C.byref(p)), "factorize")
         self._p = p
         f = p.contents
         self.diag = _arr(f.diag, sym.n, np.float64)
@@ -279,6 +287,13 @@ class Numeric:
         x = np.zeros_like(b)
         _check(lib().or_solve(self._p, b, x), "solve")
         return x
+
+    def solve_many(self, B, threads=1):
+        """columns of B (n x k) solved independently over `threads` threads"""
+        Bt = np.ascontiguousarray(np.asarray(B, dtype=np.float64).T)
+        Xt = np.zeros_like(Bt)
+        _check(lib().or_solve_mt(self._p, Bt.shape[0], Bt, Xt, int(threads)), "solve")
+        return Xt.T
 
     def __del__(self):
         if getattr(self, "_p", None) and _lib is not None:

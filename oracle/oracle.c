@@ -2001,3 +2001,435 @@ done:
     free(r); free(rh); free(p); free(ph); free(v); free(s); free(sh); free(t);
     return rc;
 }
+
+/* ======================================================================
+ * Threaded factorization: the reference's schedule
+ * (SupernodalMultifrontalFactorizer.Factorize, UtduNumericFactorization.cs:281-403,
+ * Schedule :943-1023, ApplyPanelUpdate / UpdateTrailingColumns / UpdateColumnTile
+ * :624-852).  T workers take ready supernodes from a LIFO stack (a supernode is
+ * ready once all its children are done, :313-324, :1005-1018); a worker alone at the
+ * top of the tree splits large trailing updates (work > parallel threshold, :660-673)
+ * into min(trailing, degree * 8) column chunks that idle workers help with, degree =
+ * T - active workers + 1 (:341).  The trailing update is blocked like the reference:
+ * the panel times D laid out in blocks of 128 rows (:637-658), four columns at a time
+ * with the rows vectorised (4 doubles, Vector<double>.Count on AVX2), each element's
+ * sum over the panel taken in ascending k from 0.0 and subtracted once -- the same
+ * operations, in the same order, as or_factorize, so the factor is bit-identical to
+ * it for any T.  CPU timing baseline only.
+ * ====================================================================== */
+#include <pthread.h>
+
+#define OR_ROW_BLOCK 128
+#define OR_COL_TILE 4
+#define OR_PARALLEL_THRESHOLD (1LL << 20) /* UtduSolverOptions.ParallelUpdateThreshold default */
+typedef double or_v4 __attribute__((vector_size(32)));
+
+typedef struct {
+    double *front;
+    const int *offs;
+    const double *buf; /* buf[blk * RB * np + kk * RB + r] = L[pe + blk RB + r][ps + kk] * d */
+    int f, ps, pe, np;
+} mt_update;
+
+static inline or_v4 v4_load(const double *p) { or_v4 v; memcpy(&v, p, sizeof(v)); return v; }
+static inline void v4_sub_store(double *p, or_v4 a) { or_v4 v = v4_load(p); v = v - a; memcpy(p, &v, sizeof(v)); }
+
+/* UpdateTrailingColumns (:693-762) for target columns [from, to) */
+static void mt_update_columns(const mt_update *u, int from, int to)
+{
+    const int np = u->np, f = u->f, pe = u->pe, ps = u->ps;
+    const int *offs = u->offs;
+    double *front = u->front;
+    double tile[128 * OR_COL_TILE];
+    const int nblk = (f - pe + OR_ROW_BLOCK - 1) / OR_ROW_BLOCK;
+    for (int blk = 0; blk < nblk; blk++) {
+        const double *bb = u->buf + (size_t)blk * OR_ROW_BLOCK * np;
+        const int r0 = pe + blk * OR_ROW_BLOCK;
+        const int r1 = r0 + OR_ROW_BLOCK < f ? r0 + OR_ROW_BLOCK : f;
+        const int jlim = to < r1 ? to : r1;
+        int j = from;
+        for (; j + OR_COL_TILE - 1 < jlim; j += OR_COL_TILE) {
+            for (int k = 0; k < np; k++) /* GatherPanelTile (:771-784) */
+                for (int t = 0; t < OR_COL_TILE; t++) tile[k * OR_COL_TILE + t] = front[offs[ps + k] - (ps + k) + j + t];
+            const int full = (j + OR_COL_TILE - 1 > r0) ? j + OR_COL_TILE - 1 : r0;
+            double *c0 = front + offs[j] - j, *c1 = front + offs[j + 1] - (j + 1);
+            double *c2 = front + offs[j + 2] - (j + 2), *c3 = front + offs[j + 3] - (j + 3);
+            int i = full;
+            for (; i + 4 <= r1; i += 4) { /* UpdateColumnTile (:799-852), vector part */
+                const double *pr = bb + (i - r0);
+                or_v4 a0 = {0, 0, 0, 0}, a1 = a0, a2 = a0, a3 = a0;
+                for (int k = 0; k < np; k++) {
+                    const or_v4 p = v4_load(pr + (size_t)k * OR_ROW_BLOCK);
+                    const double *sc = tile + k * OR_COL_TILE;
+                    a0 += p * sc[0];
+                    a1 += p * sc[1];
+                    a2 += p * sc[2];
+                    a3 += p * sc[3];
+                }
+                v4_sub_store(c0 + i, a0);
+                v4_sub_store(c1 + i, a1);
+                v4_sub_store(c2 + i, a2);
+                v4_sub_store(c3 + i, a3);
+            }
+            for (; i < r1; i++) { /* scalar tail */
+                const double *pr = bb + (i - r0);
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                for (int k = 0; k < np; k++) {
+                    const double p = pr[(size_t)k * OR_ROW_BLOCK];
+                    const double *sc = tile + k * OR_COL_TILE;
+                    s0 += p * sc[0];
+                    s1 += p * sc[1];
+                    s2 += p * sc[2];
+                    s3 += p * sc[3];
+                }
+                c0[i] -= s0; c1[i] -= s1; c2[i] -= s2; c3[i] -= s3;
+            }
+            const int lo = j > r0 ? j : r0; /* the corner rows (PanelDot, :787-795) */
+            for (int ii = lo; ii < full; ii++)
+                for (int t = 0; t <= ii - j; t++) {
+                    double s = 0.0;
+                    const double *pr = bb + (ii - r0);
+                    for (int k = 0; k < np; k++) s += pr[(size_t)k * OR_ROW_BLOCK] * tile[k * OR_COL_TILE + t];
+                    front[offs[j + t] + (ii - j - t)] -= s;
+                }
+        }
+        for (; j < jlim; j++) { /* single columns */
+            for (int k = 0; k < np; k++) tile[k * OR_COL_TILE] = front[offs[ps + k] - (ps + k) + j];
+            double *cj = front + offs[j] - j;
+            for (int i = (j > r0 ? j : r0); i < r1; i++) {
+                const double *pr = bb + (i - r0);
+                double s = 0.0;
+                for (int k = 0; k < np; k++) s += pr[(size_t)k * OR_ROW_BLOCK] * tile[k * OR_COL_TILE];
+                cj[i] -= s;
+            }
+        }
+    }
+}
+
+/* shared state of one threaded factorization */
+typedef struct mt_job {
+    const mt_update *u;
+    int chunks, trailing;
+    int next, done; /* under the scheduler's lock */
+    struct mt_job *link;
+} mt_job;
+
+typedef struct {
+    const or_symbolic *s;
+    const or_csr *a;
+    const or_options *o;
+    or_numeric *F;
+    int T;
+    double scale, thr, regv;
+    int pw;
+    double **updates;
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    int *stack, top, *remaining, completed, active, failed;
+    mt_job *jobs; /* inner chunk jobs with chunks left */
+    int regularized, negative;
+    double smallest, largest;
+} mt_state;
+
+static void mt_run_chunk(const mt_job *jb, int c)
+{
+    const mt_update *u = jb->u;
+    const int from = u->pe + (int)((int64_t)jb->trailing * c / jb->chunks);
+    const int to = u->pe + (int)((int64_t)jb->trailing * (c + 1) / jb->chunks);
+    mt_update_columns(u, from, to);
+}
+
+/* helps with one chunk of a posted job (lock held on entry and exit); 0 if none */
+static int mt_help_locked(mt_state *S)
+{
+    mt_job *jb = S->jobs;
+    if (!jb) return 0;
+    const int c = jb->next++;
+    if (jb->next >= jb->chunks) S->jobs = jb->link; /* no chunks left to hand out */
+    pthread_mutex_unlock(&S->mu);
+    mt_run_chunk(jb, c);
+    pthread_mutex_lock(&S->mu);
+    jb->done++;
+    if (jb->done == jb->chunks) pthread_cond_broadcast(&S->cv);
+    return 1;
+}
+
+/* Parallel.For over the chunks (:660-673): posts the job, runs chunks itself, waits for the rest */
+static void mt_parallel_update(mt_state *S, const mt_update *u, int chunks, int trailing)
+{
+    mt_job jb = {u, chunks, trailing, 0, 0, NULL};
+    pthread_mutex_lock(&S->mu);
+    jb.link = S->jobs;
+    S->jobs = &jb;
+    pthread_cond_broadcast(&S->cv);
+    while (jb.next < jb.chunks) {
+        const int c = jb.next++;
+        if (jb.next >= jb.chunks) { /* unlink: the last chunk is handed out */
+            mt_job **pp = &S->jobs;
+            while (*pp && *pp != &jb) pp = &(*pp)->link;
+            if (*pp) *pp = jb.link;
+        }
+        pthread_mutex_unlock(&S->mu);
+        mt_run_chunk(&jb, c);
+        pthread_mutex_lock(&S->mu);
+        jb.done++;
+    }
+    while (jb.done < jb.chunks) pthread_cond_wait(&S->cv, &S->mu);
+    pthread_mutex_unlock(&S->mu);
+}
+
+typedef struct {
+    mt_state *S;
+    double *front, *buf;
+    int *offs, *rowmap, *rel;
+    int regularized, negative;
+    double smallest, largest;
+    int rc;
+} mt_worker;
+
+/* FactorizeSupernode (:420-531) of supernode q; the operations are or_factorize's */
+static int mt_factor_supernode(mt_worker *W, int q, int degree)
+{
+    mt_state *S = W->S;
+    const or_symbolic *s = S->s;
+    const or_csr *a = S->a;
+    or_numeric *F = S->F;
+    double *front = W->front;
+    int *offs = W->offs, *rowmap = W->rowmap, *rel = W->rel;
+    const int first = s->first[q], w = s->first[q + 1] - first, f = s->pat_ptr[q + 1] - s->pat_ptr[q];
+    const int *prow = s->pat_rows + s->pat_ptr[q];
+    const int flen = (int)((int64_t)f * (f + 1) / 2);
+    offs[0] = 0;
+    for (int j = 0; j < f; j++) offs[j + 1] = offs[j] + (f - j);
+    memset(front, 0, (size_t)flen * sizeof(double));
+    for (int t = 0; t < f; t++) rowmap[prow[t]] = t;
+    for (int loc = 0; loc < w; loc++) {
+        const int col = first + loc, di = s->dvidx[col];
+        if (di >= 0) front[offs[loc]] += a->values[di];
+        const int lb = s->lvptr[col], st = s->lower_start[col], en = s->pptr[col + 1], cb = offs[loc] - loc;
+        for (int p = st; p < en; p++) front[cb + rowmap[s->prows[p]]] += a->values[s->lvidx[lb + (p - st)]];
+    }
+    for (int c = s->child_ptr[q]; c < s->child_ptr[q + 1]; c++) {
+        const int ch = s->children[c];
+        double *upd = S->updates[ch];
+        if (!upd) continue;
+        const int cw = s->first[ch + 1] - s->first[ch], ct = (s->pat_ptr[ch + 1] - s->pat_ptr[ch]) - cw;
+        const int *crow = s->pat_rows + s->pat_ptr[ch] + cw;
+        for (int x = 0; x < ct; x++) rel[x] = rowmap[crow[x]];
+        int src = 0;
+        for (int x = 0; x < ct; x++) {
+            const int tc = rel[x], tb = offs[tc] - tc;
+            for (int y = x; y < ct; y++) front[tb + rel[y]] += upd[src + (y - x)];
+            src += ct - x;
+        }
+        free(upd);
+        S->updates[ch] = NULL;
+    }
+    const int pw = S->pw;
+    for (int ps = 0; ps < w; ps += pw) {
+        const int pe = ps + pw < w ? ps + pw : w;
+        for (int k = ps; k < pe; k++) {
+            const int co = offs[k], cl = f - k;
+            double pivot = front[co];
+            if (!(pivot > S->thr)) {
+                if (!S->o->allow_regularization) return OR_E_NOT_PD;
+                if (pivot < 0.0) W->negative++;
+                W->regularized++;
+                pivot = S->regv;
+            }
+            F->diag[first + k] = pivot;
+            const double mag = fabs(pivot);
+            if (mag < W->smallest) W->smallest = mag;
+            if (mag > W->largest) W->largest = mag;
+            const double inv = 1.0 / pivot;
+            for (int t = 1; t < cl; t++) front[co + t] *= inv;
+            for (int j = k + 1; j < pe; j++) {
+                const double factor = pivot * front[co + (j - k)];
+                if (factor == 0.0) continue;
+                double *tgt = front + offs[j];
+                const double *src = front + co + (j - k);
+                for (int t = 0; t < f - j; t++) tgt[t] -= src[t] * factor;
+            }
+            front[co] = 1.0;
+        }
+        if (pe >= f) continue;
+        const int np = pe - ps, trailing = f - pe, nblk = (trailing + OR_ROW_BLOCK - 1) / OR_ROW_BLOCK;
+        double *buf = W->buf;
+        for (int blk = 0; blk < nblk; blk++) {
+            const int r0 = pe + blk * OR_ROW_BLOCK, nr = (f - r0) < OR_ROW_BLOCK ? (f - r0) : OR_ROW_BLOCK;
+            for (int kk = 0; kk < np; kk++) {
+                const int col = ps + kk;
+                const double d = F->diag[first + col];
+                const double *L = front + offs[col] - col;
+                double *dst = buf + (size_t)blk * OR_ROW_BLOCK * np + (size_t)kk * OR_ROW_BLOCK;
+                for (int r = 0; r < nr; r++) dst[r] = L[r0 + r] * d;
+                for (int r = nr; r < OR_ROW_BLOCK; r++) dst[r] = 0.0;
+            }
+        }
+        const mt_update u = {front, offs, buf, f, ps, pe, np};
+        const int64_t work = (int64_t)trailing * trailing * np;
+        if (degree > 1 && work > OR_PARALLEL_THRESHOLD) {
+            const int chunks = trailing < degree * 8 ? trailing : degree * 8;
+            mt_parallel_update(S, &u, chunks, trailing);
+        } else {
+            mt_update_columns(&u, pe, f);
+        }
+    }
+    const int blen = offs[w];
+    memcpy(F->factor + s->block_off[q], front, (size_t)blen * sizeof(double));
+    if (f - w > 0) {
+        const int ulen = flen - blen;
+        double *upd = (double *)xmalloc((size_t)ulen * sizeof(double));
+        memcpy(upd, front + blen, (size_t)ulen * sizeof(double));
+        S->updates[q] = upd;
+    }
+    return OR_OK;
+}
+
+static void *mt_worker_main(void *arg)
+{
+    mt_worker *W = (mt_worker *)arg;
+    mt_state *S = W->S;
+    const or_symbolic *s = S->s;
+    pthread_mutex_lock(&S->mu);
+    for (;;) {
+        if (S->failed || S->completed == s->nsuper) break;
+        if (S->top > 0) {
+            const int q = S->stack[--S->top];
+            S->active++;
+            const int degree = S->T - S->active + 1 > 1 ? S->T - S->active + 1 : 1;
+            pthread_mutex_unlock(&S->mu);
+            const int rc = mt_factor_supernode(W, q, degree);
+            pthread_mutex_lock(&S->mu);
+            S->active--;
+            if (rc != OR_OK) {
+                W->rc = rc;
+                S->failed = 1;
+                pthread_cond_broadcast(&S->cv);
+                break;
+            }
+            S->completed++;
+            const int p = s->sparent[q];
+            if (p >= 0 && --S->remaining[p] == 0) S->stack[S->top++] = p;
+            pthread_cond_broadcast(&S->cv);
+            continue;
+        }
+        if (mt_help_locked(S)) continue;
+        pthread_cond_wait(&S->cv, &S->mu);
+    }
+    pthread_mutex_unlock(&S->mu);
+    return NULL;
+}
+
+int or_factorize_mt(const or_symbolic *s, const or_csr *a, const or_options *o, int threads, or_numeric **out)
+{
+    if (!s || !a || !o || !out) return OR_E_NULL;
+    if (a->n != s->n || a->nnz != s->matrix_nnz) return OR_E_ARG;
+    const int n = s->n, ns = s->nsuper;
+    or_numeric *F = (or_numeric *)xcalloc(1, sizeof(or_numeric));
+    F->sym = s;
+    F->diag = (double *)xcalloc((size_t)n, sizeof(double));
+    F->factor = (double *)xmalloc((size_t)s->factor_entries * sizeof(double));
+    if (ns == 0) { *out = F; return OR_OK; }
+    const int64_t maxlen = (int64_t)s->max_front * (s->max_front + 1) / 2;
+    if (maxlen > 2147483647LL) { or_numeric_free(F); return OR_E_TOO_DENSE; }
+    mt_state S;
+    memset(&S, 0, sizeof(S));
+    S.s = s; S.a = a; S.o = o; S.F = F;
+    S.T = threads < 1 ? 1 : (threads < ns ? threads : ns); /* Math.Min(workers, supernodes) :309 */
+    double scale = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double d = a->diag[i] >= 0 ? a->values[a->diag[i]] : 0.0, m = fabs(d);
+        if (isnan(m) || isnan(scale)) scale = NAN;
+        else if (m > scale) scale = m;
+    }
+    if (!(scale > 0.0)) scale = 1.0;
+    S.scale = scale;
+    S.thr = o->pivot_tolerance * scale;
+    S.regv = o->regularization * scale;
+    S.pw = o->panel_width < 4 ? 4 : (o->panel_width > 128 ? 128 : o->panel_width);
+    S.updates = (double **)xcalloc((size_t)ns, sizeof(double *));
+    S.stack = (int *)xmalloc((size_t)ns * sizeof(int));
+    S.remaining = (int *)xmalloc((size_t)ns * sizeof(int));
+    for (int q = 0; q < ns; q++) {
+        S.remaining[q] = s->child_ptr[q + 1] - s->child_ptr[q];
+        if (S.remaining[q] == 0) S.stack[S.top++] = q;
+    }
+    pthread_mutex_init(&S.mu, NULL);
+    pthread_cond_init(&S.cv, NULL);
+    mt_worker *W = (mt_worker *)xcalloc((size_t)S.T, sizeof(mt_worker));
+    pthread_t *tid = (pthread_t *)xcalloc((size_t)S.T, sizeof(pthread_t));
+    const size_t bufn = (((size_t)s->max_front + OR_ROW_BLOCK - 1) / OR_ROW_BLOCK) * OR_ROW_BLOCK * (size_t)S.pw;
+    for (int t = 0; t < S.T; t++) {
+        W[t].S = &S;
+        W[t].front = (double *)xmalloc((size_t)maxlen * sizeof(double));
+        W[t].buf = (double *)xmalloc(bufn * sizeof(double));
+        W[t].offs = (int *)xmalloc(((size_t)s->max_front + 1) * sizeof(int));
+        W[t].rowmap = (int *)xmalloc((size_t)n * sizeof(int));
+        W[t].rel = (int *)xmalloc((size_t)s->max_front * sizeof(int) + sizeof(int));
+        W[t].smallest = DBL_MAX;
+    }
+    for (int t = 1; t < S.T; t++) pthread_create(&tid[t], NULL, mt_worker_main, &W[t]);
+    mt_worker_main(&W[0]);
+    for (int t = 1; t < S.T; t++) pthread_join(tid[t], NULL);
+    int rc = OR_OK;
+    double smallest = DBL_MAX, largest = 0.0;
+    for (int t = 0; t < S.T; t++) {
+        if (W[t].rc) rc = W[t].rc;
+        F->regularized += W[t].regularized;
+        F->negative += W[t].negative;
+        if (W[t].smallest < smallest) smallest = W[t].smallest;
+        if (W[t].largest > largest) largest = W[t].largest;
+        free(W[t].front); free(W[t].buf); free(W[t].offs); free(W[t].rowmap); free(W[t].rel);
+    }
+    for (int q = 0; q < ns; q++) free(S.updates[q]);
+    free(S.updates); free(S.stack); free(S.remaining); free(W); free(tid);
+    pthread_mutex_destroy(&S.mu);
+    pthread_cond_destroy(&S.cv);
+    if (rc) { or_numeric_free(F); return rc; }
+    F->smallest = smallest == DBL_MAX ? 0.0 : smallest;
+    F->largest = largest;
+    F->supernodes_done = ns;
+    int64_t ops = 0;
+    for (int q = 0; q < ns; q++) {
+        const int64_t w = s->first[q + 1] - s->first[q], f = s->pat_ptr[q + 1] - s->pat_ptr[q];
+        for (int64_t k = 0; k < w; k++) ops += (f - k) * (f - k);
+    }
+    F->ops_done = ops;
+    *out = F;
+    return OR_OK;
+}
+
+/* Solve (:90-112) of nrhs independent right-hand sides (column-major, leading dimension n)
+ * over `threads` threads; each column is or_solve's serial solve (the reference solves
+ * different right-hand sides with one factorization in parallel, SymmetricUtduSolver.cs:39-43). */
+typedef struct { const or_numeric *F; const double *b; double *x; int n, nrhs, next; pthread_mutex_t *mu; } mt_solve_arg;
+
+static void *mt_solve_main(void *p)
+{
+    mt_solve_arg *A = (mt_solve_arg *)p;
+    for (;;) {
+        pthread_mutex_lock(A->mu);
+        const int c = A->next++;
+        pthread_mutex_unlock(A->mu);
+        if (c >= A->nrhs) break;
+        or_solve(A->F, A->b + (size_t)c * A->n, A->x + (size_t)c * A->n);
+    }
+    return NULL;
+}
+
+int or_solve_mt(const or_numeric *F, int nrhs, const double *b, double *x, int threads)
+{
+    if (!F || (nrhs > 0 && (!b || !x))) return OR_E_NULL;
+    pthread_mutex_t mu;
+    pthread_mutex_init(&mu, NULL);
+    mt_solve_arg A = {F, b, x, F->sym->n, nrhs, 0, &mu};
+    const int T = threads < 1 ? 1 : (threads < nrhs ? threads : (nrhs > 0 ? nrhs : 1));
+    pthread_t *tid = (pthread_t *)xcalloc((size_t)T, sizeof(pthread_t));
+    for (int t = 1; t < T; t++) pthread_create(&tid[t], NULL, mt_solve_main, &A);
+    mt_solve_main(&A);
+    for (int t = 1; t < T; t++) pthread_join(tid[t], NULL);
+    free(tid);
+    pthread_mutex_destroy(&mu);
+    return OR_OK;
+}

@@ -119,6 +119,13 @@ int or_factorize(const or_symbolic *s, const or_csr *a, const or_options *o,
 void or_numeric_free(or_numeric *f);
 /* UtduNumericFactorization.Solve: x = A^-1 b (b unchanged) */
 int or_solve(const or_numeric *f, const double *b, double *x);
+/* The reference's threaded factorization (tree schedule over `threads` workers, parallel
+ * column chunks of large trailing updates, blocked trailing update): the same factor as
+ * or_factorize, bit for bit, for any thread count (CPU timing baseline). */
+int or_factorize_mt(const or_symbolic *s, const or_csr *a, const or_options *o, int threads,
+                    or_numeric **out);
+/* nrhs right-hand sides (column-major, leading dimension n) solved over `threads` threads */
+int or_solve_mt(const or_numeric *f, int nrhs, const double *b, double *x, int threads);
 
 /* ---- CGGP ---- */
 typedef struct {

@@ -330,3 +330,22 @@ def test_pcg_exact_counts_on_diagonal():
     sol = np.array([5.0, -2.0])
     x, res = O.pcg(a2, a2.spmv(sol), x0=sol)
     assert res["iterations"] == 0
+
+
+@pytest.mark.parametrize("threads", [2, 5, 8])
+def test_threaded_factorization_is_bit_identical(threads):
+    """The CPU baseline (the reference's tree schedule, parallel trailing-update chunks and blocked
+    update, or_factorize_mt) computes every element with the serial restatement's operations in
+    the same order: factor, D and pivot statistics equal bit for bit, as do its solves."""
+    from magla_amd import generators as G
+    for args in (grid_matrix(9, 9, 9, 2), random_matrix(300, 1.0, 8), G.elasticity_3d(6, 2)):
+        a = csr(*args)
+        s = O.Symbolic(a)
+        f1, ft = O.Numeric(s, a), O.Numeric(s, a, threads=threads)
+        assert np.array_equal(f1.factor, ft.factor) and np.array_equal(f1.diag, ft.diag)
+        assert (f1.regularized, f1.negative, f1.smallest, f1.largest) == \
+            (ft.regularized, ft.negative, ft.smallest, ft.largest)
+        B = np.column_stack([vector(a.n, 3 + j) for j in range(3)])
+        X = ft.solve_many(B, threads=threads)
+        for j in range(3):
+            np.testing.assert_array_equal(X[:, j], f1.solve(B[:, j]))
