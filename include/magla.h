@@ -258,13 +258,16 @@ int mg_batch_factorize(mg_batch *batch, const double *const *values, const mg_op
 int mg_batch_solve(mg_batch *batch, const int32_t *nrhs, const double *const *b, double *const *x, double tol,
                    int32_t max_refine, mg_solve_report *report);
 int mg_batch_numeric_info(const mg_batch *batch, mg_numeric_info *info);
-/* new: Schur tile autotuner.  For each front-size bucket with Schur work it times the whole
- * factorization of d_values with 64x64 and with 128x128 MFMA output tiles (best of three
- * runs each, the other buckets at their current choice) and keeps 128 only when it is at
- * least 2% faster; the choice is stored with the symbolic factorization for the handle's
- * precision and used by every later factorization (a captured graph is re-captured).  The
- * handle is left factorized from d_values.  tiles (MG_SCHUR_BUCKETS entries, may be NULL)
- * receives 64 or 128 per bucket; ms (may be NULL) the tuned factorization time. */
+/* new: Schur tile autotuner.  The trailing (outer) Schur updates of a front run with 64x64
+ * or 128x128 MFMA output tiles per front-size bucket (untuned: 128 for fronts of 1024 rows
+ * or more); the narrow updates inside a front's column blocks always use 64x64.  For each
+ * bucket with outer updates it times the whole factorization of d_values with 64x64 and
+ * with 128x128 tiles (best of three runs each, the other buckets at their current choice)
+ * and keeps 128 only when it is at least 2% faster; the choice is stored with the symbolic
+ * factorization for the handle's precision and used by every later factorization (a
+ * captured graph is re-captured).  The handle is left factorized from d_values.  tiles
+ * (MG_SCHUR_BUCKETS entries, may be NULL) receives 64 or 128 per bucket; ms (may be NULL)
+ * the tuned factorization time. */
 int mg_autotune_dev(mg_numeric *num, const double *d_values, const mg_options *opts, void *stream,
                     int32_t *tiles, double *ms);
 int mg_batch_autotune_dev(mg_batch *batch, const double *d_values, const mg_options *opts, void *stream,

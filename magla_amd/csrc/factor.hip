@@ -24,6 +24,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <mutex>
+#include <type_traits>
 
 #include "device.h"
 #include "mfma.h"
@@ -32,12 +33,17 @@ namespace mg {
 
 constexpr int kTile = 64;   // tile edge of assembly / Schur tiles
 constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag / panel_rows pass
-constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of the front at once
+constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of its outer block at once
+constexpr int kOuter = 1024; // outer blocking: the trailing update after every kOuter columns
+static_assert(kOuter % kBlock == 0 && kBlock % kPanel == 0, "nested blocking");
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
-// Schur tile shape per front-size bucket (f < 128, < 256, < 512, < 1024, < 2048, >= 2048):
-// 64x64 (k_schur) or 128x128 (k_schur128) output tiles, chosen by autotune_device; 64 until tuned.
+// Schur tile shape of the outer steps per front-size bucket (f < 128, < 256, < 512, < 1024,
+// < 2048, >= 2048): 64x64 (k_schur) or 128x128 (k_schur128) output tiles, chosen by
+// autotune_device; 128 for f >= 1024 until tuned.  Inner and middle steps (narrow targets)
+// use 64x64.
 constexpr int kSchurBuckets = MG_SCHUR_BUCKETS;
-constexpr int kUbStride = 2 * kSchurBuckets + 1;  // task-list offsets per (level, panel)
+constexpr int kUb64Stride = 2 * (kSchurBuckets + 1) + 1;  // 64x64 task-list offsets per (level, panel)
+constexpr int kUb128Stride = 2 * kSchurBuckets + 1;       // 128x128 task-list offsets per (level, panel)
 static inline int schur_bucket(int f) {
     int b = 0;
     for (int lim = 128; b < kSchurBuckets - 1 && f >= lim; lim *= 2) ++b;
@@ -50,7 +56,6 @@ typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  //
 constexpr int kFootChunk = 256;       // footer rows per partial product in the backward solve
 constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
 constexpr int kPanelFusedRows = 4;    // fronts with at most this many row blocks below a panel: one launch
-static_assert(kBlock == 256, "k_schur loads the block's pivots with one thread each (256 threads)");
 
 // A device front is a chain of consecutive supernodes s, s+1, ..., each the
 // elimination-tree parent of the previous one (the last column of s has the
@@ -102,11 +107,18 @@ struct DevicePlan {
     std::vector<int64_t> lvl_asm_ptr;
     std::vector<int> lvl_npanels;
     std::vector<std::vector<int>> lvl_panel_ptr, lvl_ptask_ptr;  // per panel: row-block tasks, panel tasks
-    // Schur tasks per level: for panel q, part p (0: before the next panel, 1: may overlap it) and
-    // bucket b, tasks [ub[q * kUbStride + p * kB + b], ub[... + 1]) of upd_tasks (64x64 tiles) and,
-    // with ub128, of upd128_tasks (128x128 tiles)
+    // Schur tasks per level and panel q, in two parts p (0: before the next panel, 1: may overlap
+    // it).  upd_tasks (64x64 tiles), from u = &ub64[q * kUb64Stride + p * (kB + 1)]: inner and
+    // middle steps [u[0], u[1]), outer steps of bucket b [u[1 + b], u[2 + b]).  upd128_tasks
+    // (128x128 tiles, outer steps only), from u = &ub128[q * kUb128Stride + p * kB]: bucket b
+    // [u[b], u[b + 1]).  The outer steps of a bucket run in the shape schur128 selects.
     std::vector<std::vector<int>> lvl_ub64, lvl_ub128;
-    int schur128[2][kSchurBuckets] = {};  // [precision is FP32][bucket]: 1 = 128x128 tiles
+    // per panel: 1 when some front's step writes columns past its 256-column block (those
+    // steps wait for the previous step's overlapped part)
+    std::vector<std::vector<char>> lvl_beyond;
+    // [precision is FP32][bucket]: 1 = 128x128 tiles for the outer steps (fronts of 1024 rows
+    // and more until tuned)
+    int schur128[2][kSchurBuckets] = {{0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1}};
     int64_t bucket_tasks[kSchurBuckets] = {};
     std::vector<int> lvl_init_ptr;  // solve: (front, 256-row chunk) tasks of the forward gather
     std::vector<int> lvl_small_ptr; // solve: fronts of at most kSmallFront rows (one wave each),
@@ -155,28 +167,37 @@ struct DevicePlan {
 
 int plan_level_count(const DevicePlan &P) { return P.nlevels; }
 
-// Schur work after panel q of a front of width w and size f: "inner" updates the
-// remaining columns of the current kBlock block with this panel alone, "outer"
-// (after the block's last panel) updates the trailing [be, f)^2 with the whole block.
+// Schur work after panel q of a front of width w and size f, blocked at three widths
+// (64-column panels in 256-column blocks in kOuter-column outer blocks):
+//  * inner (not the block's last panel): the block's remaining columns [pe, be), rows [pe, f),
+//    with this panel (K = 64);
+//  * middle (the block's last panel, not the outer block's last): the outer block's remaining
+//    columns [be, oe), rows [be, f), with the block (K <= 256);
+//  * outer (the outer block's last panel): the trailing [oe, f)^2 with the whole outer block
+//    (K <= kOuter), which keeps the matrix cores fed longest per tile.
+// Host and device decode a step the same way.
 struct SchurStep {
-    bool any, inner;
-    int ps, pe, bs, be, base, kb, ke, cend, row_tiles, col_tiles;
+    bool any, inner, middle;
+    int ps, pe, bs, be, os, oe, base, kb, ke, cend;
 };
-static inline SchurStep schur_step(int f, int w, int q) {
+__host__ __device__ inline SchurStep schur_step(int f, int w, int q) {
     SchurStep st{};
     st.ps = q * kPanel;
-    st.pe = std::min(st.ps + kPanel, w);
+    st.pe = st.ps + kPanel < w ? st.ps + kPanel : w;
     st.bs = (st.ps / kBlock) * kBlock;
-    st.be = std::min(st.bs + kBlock, w);
+    st.be = st.bs + kBlock < w ? st.bs + kBlock : w;
+    st.os = (st.ps / kOuter) * kOuter;
+    st.oe = st.os + kOuter < w ? st.os + kOuter : w;
     st.inner = st.pe < st.be;
+    st.middle = !st.inner && st.be < st.oe;
     if (st.inner) {
         st.base = st.pe; st.kb = st.ps; st.ke = st.pe; st.cend = st.be;
+    } else if (st.middle) {
+        st.base = st.be; st.kb = st.bs; st.ke = st.be; st.cend = st.oe;
     } else {
-        st.base = st.be; st.kb = st.bs; st.ke = st.be; st.cend = f;
+        st.base = st.oe; st.kb = st.os; st.ke = st.oe; st.cend = f;
     }
     st.any = st.base < f;
-    st.row_tiles = (f - st.base + kTile - 1) / kTile;
-    st.col_tiles = (st.cend - st.base + kTile - 1) / kTile;
     return st;
 }
 
@@ -427,6 +448,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         }
         P->lvl_pull_index.push_back(pe_index);
         std::vector<int> pp{(int)panel.size()}, tp{(int)ptask.size()}, ub64, ub128;
+        std::vector<char> bey;
         std::vector<double> ufl(npan, 0.0), uel(npan, 0.0), pel(npan, 0.0);
         for (int q = 0; q < npan; ++q) {
             for (int g : bylevel[L]) {
@@ -445,41 +467,75 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             tp.push_back((int)ptask.size());
             // Schur tasks of this panel, grouped by front-size bucket, in both tile shapes, in two
             // parts: part 0 must precede the next panel of its front (inner steps, and the next
-            // 256-column block of an outer step); part 1 is the rest of an outer step's trailing
-            // update, which the next block's panels do not touch (run_factor may overlap it)
-            for (int big = 0; big < 2; ++big) {
-                std::vector<int4> &dst = big ? upd128 : upd;
-                std::vector<int> &ub = big ? ub128 : ub64;
-                const int tile = big ? 128 : kTile, next_tiles = kBlock / tile;
-                for (int part = 0; part < 2; ++part)
-                    for (int b = 0; b < kSchurBuckets; ++b) {
-                        ub.push_back((int)dst.size());
-                        for (int g : bylevel[L]) {
-                            const int f = P->h_f[g], w = P->h_w[g];
-                            if (w <= q * kPanel || f <= kSmallFactor || schur_bucket(f) != b) continue;
-                            const SchurStep st = schur_step(f, w, q);
-                            if (!st.any) continue;
-                            const bool has_next = !st.inner && st.be < w;
-                            const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
-                            for (int tj = 0; tj < ct; ++tj) {
-                                const bool first = st.inner || (has_next && tj < next_tiles);
-                                if (first != (part == 0)) continue;
-                                for (int ti = tj; ti < rt; ++ti) dst.push_back(make_int4(g, ti, tj, big));
-                                if (!big) P->bucket_tasks[b] += rt - tj;
-                            }
-                            if (big || part) continue;
-                            double elems = 0;
-                            for (int j = st.base; j < st.cend; ++j) elems += f - j;
-                            ufl[q] += 2.0 * (st.ke - st.kb) * elems;
-                            uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
-                        }
-                    }
-                ub.push_back((int)dst.size());
+            // 256-column block of a middle or outer step); part 1 is the rest of a middle or outer
+            // step, which the next block's panels do not touch (run_factor may overlap it)
+            char beyond = 0;  // some step writes columns past its own 256-column block
+            for (int g : bylevel[L]) {
+                const int f = P->h_f[g], w = P->h_w[g];
+                if (w <= q * kPanel || f <= kSmallFactor) continue;
+                const SchurStep st = schur_step(f, w, q);
+                if (st.any && !st.inner) beyond = 1;
             }
+            bey.push_back(beyond);
+            // one front's tiles of one part in one shape: columns in ascending order, rows
+            // down from the diagonal
+            auto push_tiles = [&](int g, const SchurStep &st, int part, bool big) {
+                const int f = P->h_f[g], w = P->h_w[g];
+                const int tile = big ? 128 : kTile, next_tiles = kBlock / tile;
+                const bool has_next = !st.inner && st.base < w;
+                const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
+                int count = 0;
+                for (int tj = 0; tj < ct; ++tj) {
+                    const bool first = st.inner || (has_next && tj < next_tiles);
+                    if (first != (part == 0)) continue;
+                    for (int ti = tj; ti < rt; ++ti) (big ? upd128 : upd).push_back(make_int4(g, ti, tj, big));
+                    count += rt - tj;
+                }
+                return count;
+            };
+            auto steps_of = [&](auto fn) {  // fn(g, step) for the fronts with Schur work after panel q
+                for (int g : bylevel[L]) {
+                    const int f = P->h_f[g], w = P->h_w[g];
+                    if (w <= q * kPanel || f <= kSmallFactor) continue;
+                    const SchurStep st = schur_step(f, w, q);
+                    if (st.any) fn(g, st);
+                }
+            };
+            for (int part = 0; part < 2; ++part) {
+                // 64x64: inner and middle steps, then the outer steps bucket by bucket
+                ub64.push_back((int)upd.size());
+                steps_of([&](int g, const SchurStep &st) {
+                    if (st.inner || st.middle) push_tiles(g, st, part, false);
+                });
+                for (int b = 0; b < kSchurBuckets; ++b) {
+                    ub64.push_back((int)upd.size());
+                    steps_of([&](int g, const SchurStep &st) {
+                        if (!st.inner && !st.middle && schur_bucket(P->h_f[g]) == b)
+                            P->bucket_tasks[b] += push_tiles(g, st, part, false);
+                    });
+                }
+                // 128x128: the outer steps bucket by bucket
+                for (int b = 0; b < kSchurBuckets; ++b) {
+                    ub128.push_back((int)upd128.size());
+                    steps_of([&](int g, const SchurStep &st) {
+                        if (!st.inner && !st.middle && schur_bucket(P->h_f[g]) == b) push_tiles(g, st, part, true);
+                    });
+                }
+            }
+            ub64.push_back((int)upd.size());
+            ub128.push_back((int)upd128.size());
+            steps_of([&](int g, const SchurStep &st) {
+                const int f = P->h_f[g];
+                double elems = 0;
+                for (int j = st.base; j < st.cend; ++j) elems += f - j;
+                ufl[q] += 2.0 * (st.ke - st.kb) * elems;
+                uel[q] += 2.0 * elems + (double)(st.ke - st.kb) * (f - st.base);
+            });
         }
         P->lvl_panel_ptr.push_back(pp);
         P->lvl_ub64.push_back(ub64);
         P->lvl_ub128.push_back(ub128);
+        P->lvl_beyond.push_back(bey);
         P->lvl_ptask_ptr.push_back(tp);
         P->lvl_upd_flops.push_back(ufl);
         P->lvl_upd_elems.push_back(uel);
@@ -593,8 +649,8 @@ void plan_stats(const Symbolic &S, int64_t *out) {
         launches += 2;
         for (int q = 0; q < P->lvl_npanels[L]; ++q) {
             launches += 3;
-            const int *ub = &P->lvl_ub64[L][q * kUbStride];
-            part1 += ub[2 * kSchurBuckets] - ub[kSchurBuckets];
+            const int *ub = &P->lvl_ub64[L][q * kUb64Stride];
+            part1 += ub[2 * (kSchurBuckets + 1)] - ub[kSchurBuckets + 1];
         }
     }
     out[0] = P->nfront;
@@ -1129,40 +1185,41 @@ __device__ __forceinline__ int xcd_task(int b, int n) {
 // Schur update after panel q (cf. ApplyPanelUpdate, UtduNumericFactorization.cs:624-852):
 // F[i,j] -= sum_k (L[i,k] d_k) L[j,k] over the step's K columns, on the lower part
 // of the step's target region (schur_step).  One 64x64 output tile per workgroup;
-// K streamed through LDS in chunks of 64, products on MFMA (mfma.h).
+// K streamed through LDS in chunks of 64, products on MFMA (mfma.h).  Operand rows past the
+// front's end are not masked: they only reach output rows or columns >= f, which are not
+// stored.  Columns past the step's K (a narrow block's last chunk) are zeroed.
 template <typename T>
 __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
                                                const double *__restrict__ D) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];  // Bs[k][r] = L[j0+r, kc+k]
-    __shared__ T Ds[kBlock];             // d_k of the step's K columns
+    __shared__ T Ds[kOuter];                              // d_k of the step's K columns
     const SchurTask tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
-    const int f = tk.f, w = tk.w;
-    const int ps = q * kPanel;
-    const int pe = min(ps + kPanel, w), bs = (ps / kBlock) * kBlock, be = min(bs + kBlock, w);
-    const bool inner = pe < be;
-    const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
-    const int i0 = base + tk.ti * kTile, j0 = base + tk.tj * kTile;
+    const int f = tk.f;
+    const SchurStep sp = schur_step(f, tk.w, q);
+    const int kb = sp.kb, ke = sp.ke;
+    const int i0 = sp.base + tk.ti * kTile, j0 = sp.base + tk.tj * kTile;
     T *M = arena + tk.off;
     const double *Dp = D + tk.fc;
     // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
     // pass (k = cq + CPI j).  The vector start is clamped into the front (reads past a
-    // column's end land in the next column or the arena's tail padding and are masked).
+    // column's end land in the next column or the arena's tail padding).  Byte offsets from
+    // a chunk's first column fit 32 bits (f <= 65535), so the loads take a uniform base.
     constexpr int VEC = 16 / sizeof(T), TPC = kTile / VEC, CPI = 256 / TPC, NJ = kPanel / CPI;
     const int qv = threadIdx.x % TPC, cq = threadIdx.x / TPC;
-    const int ra0 = i0 + VEC * qv, rb0 = j0 + VEC * qv;
-    const int rowa = min(ra0, f - 1), rowb = min(rb0, f - 1);
-    Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
+    const uint32_t rowa = min(i0 + VEC * qv, f - 1), rowb = min(j0 + VEC * qv, f - 1);
+    for (int t = threadIdx.x; t < ke - kb; t += 256) Ds[t] = (T)Dp[kb + t];
     T ra[NJ][VEC], rb[NJ][VEC];
-    // Chunk kc of K into registers; all loads are unconditional and issued before any is
-    // consumed, so the next chunk is in flight while the current one runs on the MFMA units.
+    // Chunk kc of K into registers; all loads are issued before any is consumed, so the next
+    // chunk is in flight while the current one runs on the MFMA units.
     auto fetch = [&](int kc) {
-        const int nk = min(kPanel, ke - kc);
+        const char *col = reinterpret_cast<const char *>(M + (int64_t)kc * f);
+        const int nk = ke - kc;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const T *col = M + (int64_t)(kc + min(cq + CPI * j, nk - 1)) * f;
-            const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + rowa);
-            const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + rowb);
+            const uint32_t oc = (uint32_t)min(cq + CPI * j, nk - 1) * f;
+            const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + (oc + rowa) * sizeof(T));
+            const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + (oc + rowb) * sizeof(T));
             __builtin_memcpy(ra[j], &va, 16);
             __builtin_memcpy(rb[j], &vb, 16);
         }
@@ -1172,18 +1229,19 @@ __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ 
     static_assert(NJ * VEC == 16, "the target tile's 16 values reuse the chunk registers");
     fetch(kb);
     for (int kc = kb; kc < ke; kc += kPanel) {
-        const int nk = min(kPanel, ke - kc);
+        const int nk = ke - kc;
         __syncthreads();  // the previous chunk's MFMA reads are done
         typedef T vec_t __attribute__((ext_vector_type(VEC)));
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
             const int k = cq + CPI * j;
-            const T dk = Ds[min(kc - kb + k, kBlock - 1)];
+            const bool live = nk >= kPanel || k < nk;
+            const T d = Ds[kc - kb + k];  // < kOuter; past K only under the mask
             vec_t va, vb;
 #pragma unroll
             for (int i = 0; i < VEC; ++i) {
-                va[i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
-                vb[i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+                va[i] = live ? ra[j][i] * d : T(0);
+                vb[i] = live ? rb[j][i] : T(0);
             }
             if constexpr (kSchurLd<T> % VEC == 0) {  // aligned rows: one 16-byte store each
                 *reinterpret_cast<vec_t *>(&As[k][VEC * qv]) = va;
@@ -1203,71 +1261,121 @@ __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ 
         else schur_load_tile<T>(*reinterpret_cast<T(*)[16]>(&ra[0][0]), M, f, i0, j0);
         schur_acc_tile<T>(As, Bs, acc);
     }
-    schur_store_tile<T>(acc, *reinterpret_cast<const T(*)[16]>(&ra[0][0]), M, f, i0, j0, cend);
+    schur_store_tile<T>(acc, *reinterpret_cast<const T(*)[16]>(&ra[0][0]), M, f, i0, j0, sp.cend);
 }
 
-// The same update with 128x128 output tiles (large fronts): K streamed through LDS in chunks
-// of 32, staged from 16-byte row vectors; tasks remapped per XCD as in k_schur.
+// The same update with 128x128 output tiles (large fronts).  K goes through two LDS buffers
+// in chunks of KC: while chunk c runs on the MFMA units, chunk c+1 (in registers) is staged
+// into the other buffer and chunk c+2 is fetched, piece by piece between the k-steps, with
+// one barrier per chunk.  Pivots past the step's K are zero and columns past K are clamped
+// to its last column, so a partial last chunk adds nothing and no operand is masked.
+// Tasks remapped per XCD as in k_schur.
 template <typename T>
-__global__ __launch_bounds__(256) void k_schur128(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
-                                                  const double *__restrict__ D) {
-    constexpr int KC = 32;
-    __shared__ __align__(16) T As[KC][kT128Pad];  // As[k][r] = L[i0+r, kc+k] * d_k
-    __shared__ __align__(16) T Bs[KC][kT128Pad];  // Bs[k][r] = L[j0+r, kc+k]
-    __shared__ T Ds[kBlock];
+__global__ __launch_bounds__(256, 2) void k_schur128(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
+                                                     const double *__restrict__ D) {
+    constexpr int KC = kT128K<T>;
+    __shared__ __align__(16) T As[2][KC][kT128Pad];  // As[.][k][r] = L[i0+r, kc+k] * d_k
+    __shared__ __align__(16) T Bs[2][KC][kT128Pad];  // Bs[.][k][r] = L[j0+r, kc+k]
+    __shared__ T Ds[kOuter];                         // d_k of the step's K columns, zero past K
     const SchurTask tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
-    const int f = tk.f, w = tk.w;
-    const int ps = q * kPanel;
-    const int pe = min(ps + kPanel, w), bs = (ps / kBlock) * kBlock, be = min(bs + kBlock, w);
-    const bool inner = pe < be;
-    const int base = inner ? pe : be, kb = inner ? ps : bs, ke = inner ? pe : be, cend = inner ? be : f;
-    const int i0 = base + tk.ti * 128, j0 = base + tk.tj * 128;
+    const int f = tk.f;
+    const SchurStep sp = schur_step(f, tk.w, q);
+    const int kb = sp.kb, ke = sp.ke;
+    const int i0 = sp.base + tk.ti * 128, j0 = sp.base + tk.tj * 128;
     T *M = arena + tk.off;
     const double *Dp = D + tk.fc;
-    // loader mapping as in k_schur: 16-byte vectors of VEC rows, TPC threads per column, CPI
-    // columns per pass (k = cq + CPI j); vector starts clamped into the front, masked after
+    for (int t = threadIdx.x; t < kOuter; t += 256) Ds[t] = t < ke - kb ? (T)Dp[kb + t] : T(0);
+    // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
+    // piece (k = cq + CPI j, NJ pieces per chunk); vector starts clamped into the front (rows
+    // past f only reach outputs that are not stored).  Byte offsets from a chunk's first
+    // column fit 32 bits (f <= 65535).
     constexpr int VEC = 16 / sizeof(T), TPC = 128 / VEC, CPI = 256 / TPC, NJ = KC / CPI;
+    constexpr int kSteps = KC / 4;
+    static_assert(NJ <= kSteps, "one staged piece per k-step at most");
     const int qv = threadIdx.x % TPC, cq = threadIdx.x / TPC;
-    const int ra0 = i0 + VEC * qv, rb0 = j0 + VEC * qv;
-    const int rowa = min(ra0, f - 1), rowb = min(rb0, f - 1);
-    Ds[threadIdx.x] = (int)threadIdx.x < ke - kb ? (T)Dp[kb + threadIdx.x] : T(0);  // blockDim = kBlock
+    const uint32_t rowa = min(i0 + VEC * qv, f - 1), rowb = min(j0 + VEC * qv, f - 1);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
     T ra[NJ][VEC], rb[NJ][VEC];
-    auto fetch = [&](int kc) {
-        const int nk = min(KC, ke - kc);
+    auto fetch = [&](int kc, int j) {
+        const char *col = reinterpret_cast<const char *>(M + (int64_t)kc * f);
+        const uint32_t oc = (uint32_t)min(cq + CPI * j, ke - 1 - kc) * f;
+        const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + (oc + rowa) * sizeof(T));
+        const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + (oc + rowb) * sizeof(T));
+        __builtin_memcpy(ra[j], &va, 16);
+        __builtin_memcpy(rb[j], &vb, 16);
+    };
+    auto stage = [&](int kc, int buf, int j) {  // kc - kb + k < kOuter
+        const int k = cq + CPI * j;
+        const T d = Ds[kc - kb + k];
+        vec_t va, vb;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const T *col = M + (int64_t)(kc + min(cq + CPI * j, nk - 1)) * f;
-            const mg_u4u va = *reinterpret_cast<const mg_u4u *>(col + rowa);
-            const mg_u4u vb = *reinterpret_cast<const mg_u4u *>(col + rowb);
-            __builtin_memcpy(ra[j], &va, 16);
-            __builtin_memcpy(rb[j], &vb, 16);
+        for (int i = 0; i < VEC; ++i) {
+            va[i] = ra[j][i] * d;
+            vb[i] = rb[j][i];
         }
+        *reinterpret_cast<vec_t *>(&As[buf][k][VEC * qv]) = va;
+        *reinterpret_cast<vec_t *>(&Bs[buf][k][VEC * qv]) = vb;
     };
     typename MfmaAcc<T>::type acc[4][4];
     schur128_acc_zero<T>(acc);
-    fetch(kb);
-    for (int kc = kb; kc < ke; kc += KC) {
-        const int nk = min(KC, ke - kc);
-        __syncthreads();
-        typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    constexpr int NA = sizeof(T) == 4 ? 4 : 1;  // row blocks of old values held at once
+    T old[NA * 16];
+    // products of chunk kc (buffer buf); STAGE: chunk kc+KC into the other buffer; FETCH:
+    // chunk kc+2KC into the registers
+    auto chunk = [&](int kc, int buf, auto stage_t, auto fetch_t) {
+        constexpr bool STAGE = decltype(stage_t)::value, FETCH = decltype(fetch_t)::value;
+        T colv[2][4], rowv[2][4];
+        schur128_read_ops<T>(As[buf], Bs[buf], 0, colv[0], rowv[0]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) {
-            const int k = cq + CPI * j;
-            const T dk = Ds[min(kc - kb + k, kBlock - 1)];
-            vec_t va, vb;
-#pragma unroll
-            for (int i = 0; i < VEC; ++i) {
-                va[i] = (k < nk && ra0 + i < f) ? ra[j][i] * dk : T(0);
-                vb[i] = (k < nk && rb0 + i < f) ? rb[j][i] : T(0);
+        for (int s = 0; s < kSteps; ++s) {
+            const int cur = s & 1;
+            if (s + 1 < kSteps) schur128_read_ops<T>(As[buf], Bs[buf], 4 * (s + 1), colv[cur ^ 1], rowv[cur ^ 1]);
+            schur128_step<T>(colv[cur], rowv[cur], acc);
+            if constexpr (STAGE) {
+                if (s < NJ) {
+                    stage(kc + KC, buf ^ 1, s);
+                    if constexpr (FETCH) fetch(kc + 2 * KC, s);
+                }
             }
-            *reinterpret_cast<vec_t *>(&As[k][VEC * qv]) = va;  // rows of 16-byte multiples
-            *reinterpret_cast<vec_t *>(&Bs[k][VEC * qv]) = vb;
+            __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();
-        if (kc + KC < ke) fetch(kc + KC);
-        schur128_acc_tile<T>(As, Bs, acc);
+    };
+    // chunk kb into buffer 0, chunk kb+KC into the registers (or, for one chunk, the old values)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) fetch(kb, j);
+    __syncthreads();  // Ds complete
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) stage(kb, 0, j);
+    int kc = kb, buf = 0;
+    if (kb + KC < ke) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) fetch(kb + KC, j);
+    } else {
+        schur128_load_old<T, NA, 0>(old, M, f, i0, j0);
     }
-    schur128_store_tile<T>(acc, M, f, i0, j0, cend);
+    __syncthreads();
+    for (; kc + 2 * KC < ke; kc += KC, buf ^= 1) {
+        chunk(kc, buf, std::true_type{}, std::true_type{});
+        __syncthreads();
+    }
+    if (kc + KC < ke) {
+        chunk(kc, buf, std::true_type{}, std::false_type{});
+        schur128_load_old<T, NA, 0>(old, M, f, i0, j0);
+        __syncthreads();
+        kc += KC;
+        buf ^= 1;
+    }
+    chunk(kc, buf, std::false_type{}, std::false_type{});
+    schur128_store_old<T, NA, 0>(acc, old, M, f, i0, j0, sp.cend);
+    if constexpr (NA == 1) {
+        schur128_load_old<T, NA, 1>(old, M, f, i0, j0);
+        schur128_store_old<T, NA, 1>(acc, old, M, f, i0, j0, sp.cend);
+        schur128_load_old<T, NA, 2>(old, M, f, i0, j0);
+        schur128_store_old<T, NA, 2>(acc, old, M, f, i0, j0, sp.cend);
+        schur128_load_old<T, NA, 3>(old, M, f, i0, j0);
+        schur128_store_old<T, NA, 3>(acc, old, M, f, i0, j0, sp.cend);
+    }
 }
 
 // Deterministic statistics over D and the pivot flags: kStatParts workgroups reduce fixed
@@ -2643,7 +2751,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
     DevicePlan &P = *N.plan;
     T *arena = reinterpret_cast<T *>(N.arena.p);
     const double e = sizeof(T);
-    // Lookahead: the part of an outer Schur step that the next 256-column block's panels do not
+    // Lookahead: the part of a middle or outer Schur step that the next 256-column block's panels do not
     // touch runs on a second stream, overlapping those panels (off while launches are timed)
     const bool look = !profile_on() && lookahead_allowed();
     if (look) N.ensure_side();
@@ -2698,11 +2806,16 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                 hipLaunchKernelGGL(k_panel_rows<T>, dim3(b1 - b0), dim3(256), 0, st, P.panel_tasks.p + b0, arena, P.f.p,
                                    P.w.p, P.off.p, P.first.p, ps, N.diag.p);
             }
-            const int *b64 = &P.lvl_ub64[L][q * kUbStride];
-            const int *b128 = &P.lvl_ub128[L][q * kUbStride];
-            // launches of one part: one per run of consecutive buckets with the same tile shape
+            const int *b64 = &P.lvl_ub64[L][q * kUb64Stride];
+            const int *b128 = &P.lvl_ub128[L][q * kUb128Stride];
+            // launches of one part: the inner and middle steps, then the outer steps in one launch
+            // per run of consecutive buckets with the same tile shape
             auto schur_part = [&](int part, hipStream_t s) {
-                const int *u64 = b64 + part * kSchurBuckets, *u128 = b128 + part * kSchurBuckets;
+                const int *u64 = b64 + part * (kSchurBuckets + 1), *u128 = b128 + part * kSchurBuckets;
+                if (u64[1] > u64[0])
+                    hipLaunchKernelGGL(k_schur<T>, dim3(u64[1] - u64[0]), dim3(256), 0, s, P.upd_tasks.p + u64[0],
+                                       arena, q, N.diag.p);
+                const int *o64 = u64 + 1;
                 const int *big = P.schur128[sizeof(T) == 4];
                 for (int b = 0; b < kSchurBuckets;) {
                     int c = b + 1;
@@ -2711,15 +2824,15 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                         if (u128[c] > u128[b])
                             hipLaunchKernelGGL(k_schur128<T>, dim3(u128[c] - u128[b]), dim3(256), 0, s,
                                                P.upd128_tasks.p + u128[b], arena, q, N.diag.p);
-                    } else if (u64[c] > u64[b]) {
-                        hipLaunchKernelGGL(k_schur<T>, dim3(u64[c] - u64[b]), dim3(256), 0, s, P.upd_tasks.p + u64[b],
+                    } else if (o64[c] > o64[b]) {
+                        hipLaunchKernelGGL(k_schur<T>, dim3(o64[c] - o64[b]), dim3(256), 0, s, P.upd_tasks.p + o64[b],
                                            arena, q, N.diag.p);
                     }
                     b = c;
                 }
             };
-            const bool has1 = b64[2 * kSchurBuckets] > b64[kSchurBuckets];
-            if (b64[2 * kSchurBuckets] > b64[0]) {
+            const bool has1 = b64[2 * (kSchurBuckets + 1)] > b64[kSchurBuckets + 1];
+            if (b64[2 * (kSchurBuckets + 1)] > b64[0]) {
                 ProfScope ps_(st, KP_SCHUR, P.lvl_upd_flops[L][q], e * P.lvl_upd_elems[L][q]);
                 if (look && has1) {
                     // part 1 on the side stream once this panel's rows are final; part 0, which
@@ -2734,6 +2847,11 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
                     prev_side = evb;
                     ++side_k;
                 } else {
+                    // a step past its block writes columns an earlier overlapped part may still write
+                    if (prev_side && P.lvl_beyond[L][q]) {
+                        MG_HIP(hipStreamWaitEvent(st, prev_side, 0));
+                        prev_side = nullptr;
+                    }
                     schur_part(0, st);
                     schur_part(1, st);
                 }

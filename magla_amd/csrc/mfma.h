@@ -101,7 +101,8 @@ __device__ inline void schur_acc_tile(const T (*As)[kSchurLd<T>], const T (*Bs)[
 // M[j*f + i] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
 // acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].  schur_load_tile issues
 // the 16 reads of the old values together (before the last chunk's products, so that their
-// latency overlaps the matrix cores' work); schur_store_tile subtracts and writes.
+// latency overlaps the matrix cores' work); schur_store_tile subtracts on every lane (which
+// keeps the reads unconditional, where they were issued) and writes the tile's entries.
 template <typename T>
 __device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ M, int f, int i0, int j0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -118,8 +119,6 @@ __device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ M, in
                 const int ic = min(i, f - 1), jc = min(j, ic);  // clamped: unconditional load
                 old[(2 * a + b) * 4 + q] = M[(int64_t)jc * f + ic];
             }
-#pragma unroll
-    for (int e = 0; e < 16; ++e) asm volatile("" : "+v"(old[e]));  // loads stay unconditional
 }
 
 template <typename T>
@@ -128,6 +127,13 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = lane >> 4;
+    T v[16];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[(2 * a + b) * 4 + q] = old[(2 * a + b) * 4 + q] - acc[a][b][q];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -136,16 +142,19 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[(2 * a + b) * 4 + q] - acc[a][b][q];
+                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = v[(2 * a + b) * 4 + q];
             }
 }
 
 // ---- 128x128 tiles (large fronts) --------------------------------------------------
 //
 // Four waves as 2x2; wave v owns the 64x64 quadrant (rows 64*(v&1), cols 64*(v>>1)) as
-// 4x4 blocks of 16x16 MFMA tiles, so every operand read from LDS feeds four MFMAs.
-// LDS chunks hold 32 k-rows of 128 (+16 pad) elements.
+// 4x4 blocks of 16x16 MFMA tiles, so every operand read from LDS feeds four MFMAs and a
+// workgroup does twice the products of a 64x64 tile per byte it loads.  LDS chunks hold
+// kT128K<T> k-rows of 128 (+16 pad, which puts rows k and k+1 in disjoint bank halves).
 constexpr int kT128Pad = 128 + 16;
+template <typename T>
+constexpr int kT128K = sizeof(T) == 4 ? 32 : 16;  // two buffers of both operands fit twice per CU
 
 template <typename T>
 __device__ inline void schur128_acc_zero(typename MfmaAcc<T>::type (&acc)[4][4]) {
@@ -169,51 +178,59 @@ __device__ inline void schur128_read_ops(const T (*As)[kT128Pad], const T (*Bs)[
     for (int a = 0; a < 4; ++a) rowv[a] = As[k][rq + 16 * a + l16];
 }
 
-// acc += over the 32 k-rows of the LDS chunk (rows past the chunk's length hold zeros),
-// operands of the next k-step read while the current one runs (software pipeline).
 template <typename T>
-__device__ inline void schur128_acc_tile(const T (*As)[kT128Pad], const T (*Bs)[kT128Pad],
-                                         typename MfmaAcc<T>::type (&acc)[4][4]) {
-    T colv[2][4], rowv[2][4];
-    schur128_read_ops<T>(As, Bs, 0, colv[0], rowv[0]);
-    __builtin_amdgcn_sched_barrier(0);
+__device__ inline void schur128_step(const T (&colv)[4], const T (&rowv)[4], typename MfmaAcc<T>::type (&acc)[4][4]) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int cur = s & 1;
-        if (s + 1 < 8) schur128_read_ops<T>(As, Bs, 4 * (s + 1), colv[cur ^ 1], rowv[cur ^ 1]);
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = MfmaAcc<T>::step(colv[cur][b], rowv[cur][a], acc[a][b]);
-        __builtin_amdgcn_sched_barrier(0);
-    }
+        for (int b = 0; b < 4; ++b) acc[a][b] = MfmaAcc<T>::step(colv[b], rowv[a], acc[a][b]);
 }
 
-// M[j*f + i] -= C[i][j] on the tile's lower entries with i < f, j < jend (reads first, then writes)
-template <typename T>
-__device__ inline void schur128_store_tile(const typename MfmaAcc<T>::type (&acc)[4][4], T *__restrict__ M, int f,
-                                           int i0, int j0, int jend) {
+// The old values of the wave's outputs in row blocks [a0, a0 + NA): reads clamped into the
+// front (unconditional), issued together.  old[(a - a0) * 16 + b * 4 + q] holds output
+// (row rq+16a+l16, col cq+16b+MfmaAcc<T>::row(lk,q)).
+template <typename T, int NA, int a0>
+__device__ inline void schur128_load_old(T (&old)[NA * 16], const T *__restrict__ M, int f, int i0, int j0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
     const int l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        T old[4][4];
-        const int i = i0 + rq + 16 * a + l16;
-        const int ic = min(i, f - 1);
+    for (int a = 0; a < NA; ++a) {
+        const int ic = min(i0 + rq + 16 * (a0 + a) + l16, f - 1);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                old[b][q] = M[(int64_t)min(j, ic) * f + ic];
+                old[a * 16 + b * 4 + q] = M[(int64_t)min(j, ic) * f + ic];
             }
+    }
+}
+
+// M[j*f + i] = old - C[i][j] on the lower entries with i < f, j < jend of row blocks
+// [a0, a0 + NA); the differences are formed on every lane (the reads stay where issued).
+template <typename T, int NA, int a0>
+__device__ inline void schur128_store_old(const typename MfmaAcc<T>::type (&acc)[4][4], const T (&old)[NA * 16],
+                                          T *__restrict__ M, int f, int i0, int j0, int jend) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
+    const int l16 = lane & 15, lk = lane >> 4;
+    T v[NA * 16];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[a * 16 + b * 4 + q] = old[a * 16 + b * 4 + q] - acc[a0 + a][b][q];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int i = i0 + rq + 16 * (a0 + a) + l16;
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = old[b][q] - acc[a][b][q];
+                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = v[a * 16 + b * 4 + q];
             }
     }
 }

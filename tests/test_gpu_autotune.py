@@ -1,6 +1,7 @@
-"""Schur tile shapes per front-size bucket: 64x64 and 128x128 MFMA output tiles (and any mix
-of them over the buckets) give the same factorization to rounding, and the autotuner's choice
-leaves a factorization that solves to the refinement target, for one system and for a batch."""
+"""Schur tile shapes of the outer updates per front-size bucket: 64x64 and 128x128 MFMA output
+tiles (and any mix of them over the buckets) give the same factorization to rounding, and the
+autotuner's choice leaves a factorization that solves to the refinement target, for one system
+and for a batch."""
 import numpy as np
 import pytest
 
@@ -63,7 +64,7 @@ def test_autotune_choice_solves(system):
     tiles, ms = f.autotune_dev(vals.data_ptr())
     torch.cuda.synchronize()
     assert set(tiles.tolist()) <= {64, 128}
-    assert tiles[-1] == 64  # no front of 2048 rows here: the bucket keeps the default
+    assert tiles[-1] == 128  # no front of 2048 rows here: the bucket keeps its untuned 128x128
     assert ms > 0
     b = G.uniform_rhs(n, 3, 5)
     rep = {}
