@@ -458,7 +458,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64, help="c5 / bicg: grid edge")
     ap.add_argument("--pairs", type=int, default=2000, help="c5: training pairs")
     ap.add_argument("--epochs", type=int, default=20, help="c5: training epochs")
-    ap.add_argument("--lr", type=float, default=3e-3, help="c5: Adam learning rate")
+    ap.add_argument("--lr", type=float, default=1e-2, help="c5: Adam learning rate")
     ap.add_argument("--kappa", type=float, default=1.0, help="c5: weight of the Kaporin condition-number term")
     ap.add_argument("--snapshot-its", type=int, default=256, help="c5: largest Jacobi-PCG snapshot iteration")
     args = ap.parse_args()

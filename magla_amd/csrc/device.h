@@ -65,6 +65,13 @@ struct Matrix {
     DevArray<double> invdiag;             // Jacobi: 1 / A[i,i]
     int diag_status = 0;                  // 0 ok, else MG_E_NOT_PD (zero/invalid diagonal)
     int lanes = 8;                        // lanes per row in the SpMV kernel
+    // sliced ELL copy for the PCG direction kernel: slices of 64 rows, slice s holds
+    // sell_ptr[s + 1] - sell_ptr[s] = 64 w_s entries, entry k of row 64 s + l at
+    // sell_ptr[s] + 64 k + l (padding: column of the row itself, value 0)
+    int nslices = 0;
+    bool sell = false;                    // built (padding at most half the entries)
+    DevArray<int> sell_ptr, sell_ci, sell_src;  // sell_src: upper value index or -1
+    DevArray<double> sell_vals;
     std::shared_ptr<MpGraph> mp;          // learned-preconditioner graph data (built on first use)
     void build(int n, const int *rowptr, const int *colidx);
     void set_values_host(const double *values);
@@ -128,7 +135,17 @@ constexpr int kRedBlocks = 512;  // fixed grid of the deterministic vector reduc
 // Device scalars of a (flexible) PCG run; kernels of an iteration turn into no-ops once done.
 struct PcgScalars {
     double rz, pap, alpha, beta, rnorm, thr, bnorm, best;
+    double rz2[2];  // two-kernel PCG: r'z of iteration it in rz2[it & 1]
     int done, iters, fail, fallback, restart, best_it, pad0, pad1;
+};
+
+// State of the fused flexible PCG between its kernels, one slot per iteration parity: the
+// residual-norm test of iteration it (best norm, stall fallback) is written into slot it & 1
+// (fallback1 / restart1), the next iteration's first kernel completes it with the direction
+// update (rz2, beta, fallback2 / restart2) in the same slot; kernels read the other slot.
+struct FcgState {
+    double rnorm, best, rz2, beta;
+    int best_it, fallback1, restart1, fallback2, restart2, pad;
 };
 
 // ---- learned preconditioner (message-passing network on the matrix graph) ----
@@ -150,6 +167,11 @@ struct MpGraph {
     DevArray<float> ew, x, ge1, ge2;
     DevArray<double> diag;
     DevArray<int> gptr, gcol, gtptr, gtrow, gtidx;
+    // sliced ELL copies of G and G' for the solves (64-row slices, lane = row; built when
+    // the padding stays under half the entries): columns / rows and the index of each
+    // entry's value in the factor values (-1: padding)
+    int nslices = 0;
+    DevArray<int> gs_ptr, gs_col, gs_src, gts_ptr, gts_row, gts_src;
 };
 
 struct Mpnn {
@@ -162,6 +184,7 @@ struct Mpnn {
     DevArray<float> H, Mb, Pre, dH, dHp, dM, part, gvf, dG, gaE;
     DevArray<uint16_t> hb0, hb1;
     DevArray<float> gv, gvt;        // factor values for the solves (from the BF16 network), also transposed order
+    DevArray<float> gvs, gvts;      // the same in the sliced ELL orders of G and G' (when built)
     const MpGraph *gv_graph = nullptr;  // graph the factor values belong to
     DevArray<double> y, z, az, u, dpart, scal;
 };
@@ -177,6 +200,13 @@ void mpnn_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, int precis
 void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st);
 void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const PcgScalars *S, const double *d_ap,
                        double *rz_part, double *zap_part, hipStream_t st);
+// The fused flexible PCG's preconditioner step of iteration it (both products on the sliced
+// ELL copies of G and G', with the residual-norm test folded in); false when the graph has
+// no sliced copies.  rz / zap partials: (nslices + 3) / 4 of each.
+bool mpnn_fcg_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const double *d_ap, const double *prr,
+                    int nstep, int it, int stall, const FcgState *fs_prev, FcgState *fs_cur, PcgScalars *S,
+                    double *rz_part, double *zap_part, hipStream_t st);
+int mpnn_fcg_partials(const Matrix &A);
 int mpnn_partial_blocks(int n);  // partial sums written by mpnn_apply_factor
 // mean loss over ns pairs (columns of R, E with leading dimension ld) plus kappa times the log
 // Kaporin number of G A G'; gradient into net.grad

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "device.h"
+#include "reduce.h"
 
 namespace mg {
 
@@ -485,6 +486,213 @@ __global__ __launch_bounds__(256) void k_gt_apply(const int *__restrict__ gtptr,
     }
 }
 
+__global__ void k_gather_sell(const int *__restrict__ src, int64_t count, const float *__restrict__ gv,
+                              float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) out[e] = src[e] >= 0 ? gv[src[e]] : 0.0f;
+}
+
+// y = G v on the sliced ELL copy of G: wave = 64-row slice, lane = row, the row's entries
+// accumulated in order in float64
+__global__ __launch_bounds__(256) void k_g_apply_sell(const int *__restrict__ sp, const int *__restrict__ scol,
+                                                      const float *__restrict__ sv, int n, int nslices,
+                                                      const double *__restrict__ v, double *__restrict__ y,
+                                                      const PcgScalars *__restrict__ S) {
+    if (gated(S)) return;
+    const int lane = threadIdx.x & 63, slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (slice >= nslices) return;
+    const int base = sp[slice], w = (sp[slice + 1] - base) >> 6, row = 64 * slice + lane;
+    double acc = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const int e = base + 64 * k + lane;
+        acc += (double)sv[e] * v[scol[e]];
+    }
+    if (row < n) y[row] = acc;
+}
+
+// z = G' y on the sliced ELL copy of G' (Jacobi z = D^{-1} r after a fallback), with block
+// partials of r'z and z'Ap over a fixed grid (slices strided over the blocks)
+__global__ __launch_bounds__(256) void k_gt_apply_sell(const int *__restrict__ sp, const int *__restrict__ srow,
+                                                       const float *__restrict__ sv, int n, int nslices,
+                                                       const double *__restrict__ y, double *__restrict__ z,
+                                                       const double *__restrict__ r, const double *__restrict__ ap,
+                                                       double *__restrict__ rz_part, double *__restrict__ zap_part,
+                                                       const double *__restrict__ invd,
+                                                       const PcgScalars *__restrict__ S) {
+    if (S && S->done) return;
+    __shared__ double sm[2][4];
+    const bool jacobi = S && S->fallback;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double arz = 0.0, azap = 0.0;
+    for (int slice = blockIdx.x * 4 + wave; slice < nslices; slice += gridDim.x * 4) {
+        const int j = 64 * slice + lane;
+        double acc = 0.0;
+        if (jacobi) {
+            if (j < n) acc = invd[j] * r[j];
+        } else {
+            const int base = sp[slice], w = (sp[slice + 1] - base) >> 6;
+            for (int k = 0; k < w; ++k) {
+                const int e = base + 64 * k + lane;
+                acc += (double)sv[e] * y[srow[e]];
+            }
+        }
+        if (j < n) {
+            z[j] = acc;
+            if (rz_part) {
+                arz += r[j] * acc;
+                if (ap) azap += acc * ap[j];
+            }
+        }
+    }
+    if (!rz_part) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        arz += __shfl_xor(arz, o, 64);
+        azap += __shfl_xor(azap, o, 64);
+    }
+    if (lane == 0) {
+        sm[0][wave] = arz;
+        sm[1][wave] = azap;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rz_part[blockIdx.x] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+        zap_part[blockIdx.x] = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+    }
+}
+
+// sum_k sv[e_k] * x[idx[e_k]] over one sliced-ELL row (e_k = base + 64 k + lane, k < w), in
+// batches of eight entries whose index, value and gathered loads are issued together; the
+// products are added in k order
+__device__ __forceinline__ double sell_row_dot(const int *__restrict__ idx, const float *__restrict__ sv, int base,
+                                               int w, int lane, const double *__restrict__ x) {
+    double acc = 0.0;
+    for (int k0 = 0; k0 < w; k0 += 8) {
+        int c[8];
+        float g[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = base + 64 * min(k0 + q, w - 1) + lane;
+            c[q] = idx[e];
+            g[q] = k0 + q < w ? sv[e] : 0.0f;
+        }
+        double xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) xv[q] = x[c[q]];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += (double)g[q] * xv[q];
+    }
+    return acc;
+}
+
+// Residual-norm test of the fused flexible PCG's iteration `it` (the reference's stopping rule,
+// ConjugateGradientGaussPreSolver.cs:151-153, and the stall fallback), formed identically by
+// every workgroup of the preconditioner kernels from the step's partials and the previous
+// slot of the state.
+struct FcgNorm {
+    double rnorm, best;
+    int best_it, conv, fallback, restart, trig;
+};
+__device__ inline FcgNorm fcg_norm_decide(const double *__restrict__ prr, int nstep, int it, int stall, double thr,
+                                          const FcgState *__restrict__ pv, double *sm4) {
+    FcgNorm d;
+    d.rnorm = sqrt(sum_partials(prr, nstep, sm4));
+    d.conv = d.rnorm <= thr;
+    d.fallback = pv->fallback2;
+    d.restart = pv->restart2;
+    d.best = pv->best;
+    d.best_it = pv->best_it;
+    d.trig = 0;
+    if (!d.conv && !d.fallback) {
+        if (d.rnorm < d.best) {
+            d.best = d.rnorm;
+            d.best_it = it;
+        } else if (it - d.best_it >= stall) {
+            d.fallback = 1;
+            d.restart = 1;
+            d.trig = 1;
+        }
+    }
+    return d;
+}
+
+// y = G r for iteration `it` of the fused flexible PCG (skipped once converged or fallen back)
+__global__ __launch_bounds__(256) void k_fcg_g_sell(const int *__restrict__ sp, const int *__restrict__ scol,
+                                                    const float *__restrict__ sv, int n, int nslices,
+                                                    const double *__restrict__ r, double *__restrict__ y,
+                                                    const double *__restrict__ prr, int nstep, int it, int stall,
+                                                    const FcgState *__restrict__ pv, const PcgScalars *__restrict__ S) {
+    __shared__ double sm4[4];
+    if (S->done) return;
+    const FcgNorm d = fcg_norm_decide(prr, nstep, it, stall, S->thr, pv, sm4);
+    if (d.conv || d.fallback) return;  // uniform
+    const int lane = threadIdx.x & 63, slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (slice >= nslices) return;
+    const int base = sp[slice], w = (sp[slice + 1] - base) >> 6, row = 64 * slice + lane;
+    const double acc = sell_row_dot(scol, sv, base, w, lane, r);
+    if (row < n) y[row] = acc;
+}
+
+// z = G' y (Jacobi z = D^{-1} r once fallen back) for iteration `it`, with block partials of
+// r'z and z'Ap; workgroup 0 records the norm test in slot it & 1 (and the result scalars)
+__global__ __launch_bounds__(256) void k_fcg_gt_sell(const int *__restrict__ sp, const int *__restrict__ srow,
+                                                     const float *__restrict__ sv, int n, int nslices,
+                                                     const double *__restrict__ y, double *__restrict__ z,
+                                                     const double *__restrict__ r, const double *__restrict__ ap,
+                                                     const double *__restrict__ invd, const double *__restrict__ prr,
+                                                     int nstep, int it, int stall, const FcgState *__restrict__ pv,
+                                                     FcgState *__restrict__ cur, PcgScalars *__restrict__ S,
+                                                     double *__restrict__ rz_part, double *__restrict__ zap_part) {
+    __shared__ double sm4[4];
+    __shared__ double sm[2][4];
+    if (S->done) return;
+    const FcgNorm d = fcg_norm_decide(prr, nstep, it, stall, S->thr, pv, sm4);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->rnorm = d.rnorm;
+        S->iters = it;
+        if (d.conv) {
+            S->done = 1;
+        } else {
+            cur->rnorm = d.rnorm;
+            cur->best = d.best;
+            cur->best_it = d.best_it;
+            cur->fallback1 = d.fallback;
+            cur->restart1 = d.restart;
+            S->fallback = d.fallback;
+        }
+    }
+    if (d.conv) return;  // uniform
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double arz = 0.0, azap = 0.0;
+    for (int slice = blockIdx.x * 4 + wave; slice < nslices; slice += gridDim.x * 4) {
+        const int j = 64 * slice + lane;
+        double acc = 0.0;
+        if (d.fallback) {
+            if (j < n) acc = invd[j] * r[j];
+        } else {
+            const int base = sp[slice], w = (sp[slice + 1] - base) >> 6;
+            acc = sell_row_dot(srow, sv, base, w, lane, y);
+        }
+        if (j < n) {
+            z[j] = acc;
+            arz += r[j] * acc;
+            azap += acc * ap[j];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        arz += __shfl_xor(arz, o, 64);
+        azap += __shfl_xor(azap, o, 64);
+    }
+    if (lane == 0) {
+        sm[0][wave] = arz;
+        sm[1][wave] = azap;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rz_part[blockIdx.x] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+        zap_part[blockIdx.x] = (sm[1][0] + sm[1][1]) + (sm[1][2] + sm[1][3]);
+    }
+}
+
 // gvt[t] = gv[gtidx[t]]: factor values in the order of the transposed pattern
 __global__ void k_gather_t(const int *__restrict__ gtidx, const float *__restrict__ gv, int64_t ng,
                            float *__restrict__ gvt) {
@@ -826,6 +1034,40 @@ void mp_graph_prepare(Matrix &A, hipStream_t st) {
             }
     }
     auto g = std::make_shared<MpGraph>();
+    {
+        // sliced ELL of G (entry e's value at gv[e]) and of G' (value at gv[gtidx[t]])
+        const int nsl = (n + 63) / 64;
+        auto sell = [&](const std::vector<int> &ptr, const std::vector<int> &idx, const int *vmap,
+                        DevArray<int> &dptr, DevArray<int> &didx, DevArray<int> &dsrc) {
+            std::vector<int> sp(nsl + 1, 0);
+            for (int sl = 0; sl < nsl; ++sl) {
+                int w = 0;
+                for (int r = 64 * sl; r < std::min(n, 64 * sl + 64); ++r) w = std::max(w, ptr[r + 1] - ptr[r]);
+                sp[sl + 1] = sp[sl] + 64 * w;
+            }
+            if ((double)sp[nsl] > 1.5 * (double)std::max<int64_t>(ng, 1)) return false;
+            std::vector<int> si(std::max(sp[nsl], 1)), ss(std::max(sp[nsl], 1), -1);
+            for (int sl = 0; sl < nsl; ++sl) {
+                const int w = (sp[sl + 1] - sp[sl]) / 64;
+                for (int l = 0; l < 64; ++l) {
+                    const int r = std::min(64 * sl + l, n - 1);
+                    const int len = 64 * sl + l < n ? ptr[r + 1] - ptr[r] : 0;
+                    for (int k = 0; k < w; ++k) {
+                        const int e = sp[sl] + 64 * k + l;
+                        si[e] = k < len ? idx[ptr[r] + k] : r;
+                        ss[e] = k < len ? (vmap ? vmap[ptr[r] + k] : ptr[r] + k) : -1;
+                    }
+                }
+            }
+            dptr.upload(sp);
+            didx.upload(si);
+            dsrc.upload(ss);
+            return true;
+        };
+        if (n > 0 && sell(gptr, gcol, nullptr, g->gs_ptr, g->gs_col, g->gs_src) &&
+            sell(gtptr, gtrow, gtidx.data(), g->gts_ptr, g->gts_row, g->gts_src))
+            g->nslices = nsl;
+    }
     g->n = n;
     g->ng = ng;
     g->ew.upload(ew);
@@ -995,6 +1237,14 @@ void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st) {
                        wv + 3 * W, g.gptr.p, g.gcol.p, g.ge1.p, g.ge2.p, g.diag.p, n, net.gv.p);
     if (g.ng)
         hipLaunchKernelGGL(k_gather_t, dim3(nblk(g.ng, 256)), dim3(256), 0, st, g.gtidx.p, net.gv.p, g.ng, net.gvt.p);
+    if (g.nslices) {
+        if (net.gvs.n != g.gs_src.n) net.gvs.alloc(g.gs_src.n);
+        if (net.gvts.n != g.gts_src.n) net.gvts.alloc(g.gts_src.n);
+        hipLaunchKernelGGL(k_gather_sell, dim3(nblk((int64_t)g.gs_src.n, 256)), dim3(256), 0, st, g.gs_src.p,
+                           (int64_t)g.gs_src.n, net.gv.p, net.gvs.p);
+        hipLaunchKernelGGL(k_gather_sell, dim3(nblk((int64_t)g.gts_src.n, 256)), dim3(256), 0, st, g.gts_src.p,
+                           (int64_t)g.gts_src.n, net.gv.p, net.gvts.p);
+    }
     MG_HIP(hipGetLastError());
     net.gv_graph = &g;
 }
@@ -1003,12 +1253,35 @@ void mpnn_apply_factor(Mpnn &net, Matrix &A, const double *d_r, double *d_z, con
                        double *rz_part, double *zap_part, hipStream_t st) {
     const MpGraph &g = *A.mp;
     const int n = A.n;
+    if (g.nslices && net.gv_graph == &g && net.gvs.n == g.gs_src.n) {
+        hipLaunchKernelGGL(k_g_apply_sell, dim3((g.nslices + 3) / 4), dim3(256), 0, st, g.gs_ptr.p, g.gs_col.p,
+                           net.gvs.p, n, g.nslices, d_r, net.y.p, S);
+        hipLaunchKernelGGL(k_gt_apply_sell, dim3(mpnn_partial_blocks(n)), dim3(256), 0, st, g.gts_ptr.p, g.gts_row.p,
+                           net.gvts.p, n, g.nslices, net.y.p, d_z, d_r, d_ap, rz_part, zap_part, A.invdiag.p, S);
+        return;
+    }
     // the transposed product runs on a fixed grid (one partial sum per block; after a fallback it
     // computes the Jacobi z); the row product is skipped after a fallback
     hipLaunchKernelGGL((k_g_apply<float>), dim3(nblk((int64_t)n * kRowLanes, 256)), dim3(256), 0, st, g.gptr.p,
                        g.gcol.p, net.gv.p, d_r, n, net.y.p, S);
     hipLaunchKernelGGL((k_gt_apply<float>), dim3(mpnn_partial_blocks(n)), dim3(256), 0, st, g.gtptr.p, g.gtrow.p,
                        g.gtidx.p, net.gv.p, net.gvt.p, net.y.p, n, d_z, d_r, d_ap, rz_part, zap_part, A.invdiag.p, S);
+}
+
+int mpnn_fcg_partials(const Matrix &A) { return A.mp ? std::max(1, (A.mp->nslices + 3) / 4) : 1; }
+
+bool mpnn_fcg_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const double *d_ap, const double *prr,
+                    int nstep, int it, int stall, const FcgState *fs_prev, FcgState *fs_cur, PcgScalars *S,
+                    double *rz_part, double *zap_part, hipStream_t st) {
+    const MpGraph &g = *A.mp;
+    if (!g.nslices || net.gv_graph != &g || net.gvs.n != g.gs_src.n) return false;
+    const int grid = mpnn_fcg_partials(A);
+    hipLaunchKernelGGL(k_fcg_g_sell, dim3(grid), dim3(256), 0, st, g.gs_ptr.p, g.gs_col.p, net.gvs.p, A.n, g.nslices,
+                       d_r, net.y.p, prr, nstep, it, stall, fs_prev, S);
+    hipLaunchKernelGGL(k_fcg_gt_sell, dim3(grid), dim3(256), 0, st, g.gts_ptr.p, g.gts_row.p, net.gvts.p, A.n,
+                       g.nslices, net.y.p, d_z, d_r, d_ap, A.invdiag.p, prr, nstep, it, stall, fs_prev, fs_cur, S,
+                       rz_part, zap_part);
+    return true;
 }
 
 double mpnn_loss_grad(Mpnn &net, Matrix &A, int ns, const double *d_R, const double *d_E, int64_t ld, double kappa,

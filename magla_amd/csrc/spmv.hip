@@ -7,15 +7,18 @@
 // traffic per product: 12 B per stored entry + 4 B per row pointer + 16 B per
 // row and right-hand side (x read once, y written once).
 //
-// PCG keeps every vector and scalar in HBM; each iteration is seven short
-// kernels whose reductions are two-stage over a fixed grid (deterministic),
-// and a device flag turns the remaining launches into no-ops once
-// ||r|| <= tol*||b|| (same test, same place as
-// ConjugateGradientGaussPreSolver.cs:151-153).
+// PCG keeps every vector and scalar in HBM.  An iteration is two kernels: the new
+// direction p = z + beta p formed inside the SpMV's gather (into the other of two
+// direction buffers) with Ap and p'Ap, then the x / r update with z = M r, ||r||^2 and
+// r'z.  Each reduction is finished by the workgroup that arrives last, in a fixed order
+// (deterministic), which also computes alpha, or the convergence test and beta.  A device
+// flag turns the remaining launches into no-ops once ||r|| <= tol*||b|| (same test, same
+// place as ConjugateGradientGaussPreSolver.cs:151-153).
 #include <algorithm>
 #include <cmath>
 
 #include "device.h"
+#include "reduce.h"
 
 namespace mg {
 
@@ -98,16 +101,53 @@ void Matrix::build(int n_, const int *rowptr, const int *colidx) {
     dpos.upload(dp);
     vals.alloc(std::max<int64_t>(nnz_full, 1));
     invdiag.alloc(std::max(n, 1));
+    // sliced ELL: one 64-row slice per wave, lane = row, the slice padded to its longest row
+    nslices = (n + 63) / 64;
+    std::vector<int> sp(nslices + 1, 0);
+    for (int sl = 0; sl < nslices; ++sl) {
+        int w = 0;
+        for (int r = 64 * sl; r < std::min(n, 64 * sl + 64); ++r) w = std::max(w, frp[r + 1] - frp[r]);
+        sp[sl + 1] = sp[sl] + 64 * w;
+    }
+    sell = n > 0 && (double)sp[nslices] <= 1.5 * (double)nnz_full;
+    if (sell) {
+        std::vector<int> sci(sp[nslices]), ssrc(sp[nslices]);
+        for (int sl = 0; sl < nslices; ++sl) {
+            const int w = (sp[sl + 1] - sp[sl]) / 64;
+            for (int l = 0; l < 64; ++l) {
+                const int r = std::min(64 * sl + l, n - 1);
+                const int len = 64 * sl + l < n ? frp[r + 1] - frp[r] : 0;
+                for (int k = 0; k < w; ++k) {
+                    const int e = sp[sl] + 64 * k + l;
+                    sci[e] = k < len ? fci[frp[r] + k] : r;
+                    ssrc[e] = k < len ? fsrc[frp[r] + k] : -1;
+                }
+            }
+        }
+        sell_ptr.upload(sp);
+        sell_ci.upload(sci);
+        sell_src.upload(ssrc);
+        sell_vals.alloc(std::max(sp[nslices], 1));
+    }
     const double avg = n ? (double)nnz_full / n : 1.0;
     // About ten entries per lane: a wave then works on several rows at once, which
     // keeps enough independent loads in flight (one row per wave is latency-bound).
     lanes = avg > 512 ? 64 : avg > 256 ? 32 : avg > 128 ? 16 : avg > 24 ? 8 : 4;
 }
 
+__global__ void k_sell_gather(const int *__restrict__ src, int64_t count, const double *__restrict__ upper,
+                              double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) out[e] = src[e] >= 0 ? upper[src[e]] : 0.0;
+}
+
 void Matrix::set_values_dev(const double *d_values, hipStream_t st) {
     if (nnz_full > 0)
         hipLaunchKernelGGL(k_full_gather, dim3(nblk(nnz_full, kBS)), dim3(kBS), 0, st, src.p, nnz_full, d_values,
                            vals.p);
+    if (sell && sell_src.n > 0)
+        hipLaunchKernelGGL(k_sell_gather, dim3(nblk((int64_t)sell_src.n, kBS)), dim3(kBS), 0, st, sell_src.p,
+                           (int64_t)sell_src.n, d_values, sell_vals.p);
     MG_HIP(hipGetLastError());
 }
 
@@ -203,7 +243,10 @@ __global__ void k_pcg_rz0(const double *__restrict__ part, PcgScalars *__restric
     __shared__ double sm[kBS];
     if (S->done) return;
     const double rz = finish_sum(part, sm);
-    if (threadIdx.x == 0) S->rz = rz;
+    if (threadIdx.x == 0) {
+        S->rz = rz;
+        S->rz2[1] = rz;  // iteration 1
+    }
 }
 
 __global__ void k_residual0(const double *__restrict__ b, const double *__restrict__ ax, int n, double *__restrict__ r) {
@@ -312,6 +355,286 @@ __global__ void k_pcg_pupdate(const double *__restrict__ z, double *__restrict__
     if (i < n) p[i] = z[i] + S->beta * p[i];
 }
 
+// the direction entry p_new = z + beta p_old, formed identically by its row's owner and by
+// every row that gathers it
+__device__ __forceinline__ double pcg_dir(double z, double p, double beta) { return __fma_rn(beta, p, z); }
+
+// Iteration `it`, first kernel.  For it > 1 it finishes iteration it - 1: ||r|| from the
+// previous step's partials and the convergence test (:151-153; iterations = it - 1), then
+// rz_new = r'z and beta = rz_new / rz (:157, :162).  Then p_new = z + beta p_old (:159-160;
+// beta = 0 and p_old = 0 give the first direction p = z), Ap = A p_new and partial p'Ap.
+// rz of iteration it is kept in S->rz2[it & 1]: the kernel reads the other slot.  The rows'
+// first two entries per lane are loaded before the partial sums are formed, so the gather's
+// latency overlaps them.  After convergence the kernel still runs but changes nothing that
+// is read again (the step kernel returns at once).
+template <int LANES>
+__global__ __launch_bounds__(256) void k_pcg_dir_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                      const double *__restrict__ v, int n, const double *__restrict__ z,
+                                                      const double *__restrict__ p_old, double *__restrict__ p_new,
+                                                      double *__restrict__ ap, const double *__restrict__ part_rr,
+                                                      const double *__restrict__ part_rz, int np_step, int it,
+                                                      double *__restrict__ part_pap, PcgScalars *__restrict__ S) {
+    __shared__ double sm4[4];
+    constexpr int kRows = kBS / LANES;  // rows per pass of the block
+    const int sub = threadIdx.x % LANES;
+    int row = blockIdx.x * kRows + (int)threadIdx.x / LANES;
+    // first pass: this lane's first two entries and the row's own direction entry
+    const int r0 = min(row, n - 1);
+    const int kb = rp[r0], ke = rp[r0 + 1];
+    const int ka = kb + sub, kc = ka + LANES;
+    const bool ha = row < n && ka < ke, hc = row < n && kc < ke;
+    const int ca = ci[ha ? ka : kb], cc = ci[hc ? kc : kb];
+    const double va = ha ? v[ka] : 0.0, vc = hc ? v[kc] : 0.0;
+    const double za = z[ca], pa = p_old[ca], zc = z[cc], pc = p_old[cc], zr = z[r0], pr = p_old[r0];
+    double beta = 0.0;
+    if (it > 1) {
+        const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
+        const double rz = sum_partials(part_rz, np_step, sm4);
+        const bool conv = rnorm <= S->thr;
+        beta = rz / S->rz2[(it - 1) & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !S->done) {
+            S->rnorm = rnorm;
+            S->iters = it - 1;
+            if (conv) S->done = 1;
+            else S->rz2[it & 1] = rz;
+        }
+    }
+    double dot = 0.0;
+    for (bool first = true; row < n; row += gridDim.x * kRows, first = false) {
+        double acc;
+        double pn;
+        if (first) {
+            acc = va * pcg_dir(za, pa, beta) + vc * pcg_dir(zc, pc, beta);
+            for (int k = kc + LANES; k < ke; k += LANES) acc += v[k] * pcg_dir(z[ci[k]], p_old[ci[k]], beta);
+            pn = pcg_dir(zr, pr, beta);
+        } else {
+            acc = 0.0;
+            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
+                const int c = ci[k];
+                acc += v[k] * pcg_dir(z[c], p_old[c], beta);
+            }
+            pn = pcg_dir(z[row], p_old[row], beta);
+        }
+        for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+        if (sub == 0) {
+            p_new[row] = pn;
+            ap[row] = acc;
+            dot += pn * acc;
+        }
+    }
+    const double t = block_sum_waves(dot, sm4);
+    if (threadIdx.x == 0) part_pap[blockIdx.x] = t;
+}
+
+// The same first kernel on the sliced ELL copy: wave = 64-row slice, lane = row; the first
+// kPre entries of each row are loaded before the partial sums are formed.  With fcg set
+// (the flexible PCG of the learned preconditioner) beta comes from S->beta, set by the
+// previous iteration's kernels, and the convergence test is theirs.
+__global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp, const int *__restrict__ sci,
+                                                      const double *__restrict__ sv, int n, int nslices,
+                                                      const double *__restrict__ z, const double *__restrict__ p_old,
+                                                      double *__restrict__ p_new, double *__restrict__ ap,
+                                                      const double *__restrict__ part_rr,
+                                                      const double *__restrict__ part_rz, int np_step, int it,
+                                                      double *__restrict__ part_pap, PcgScalars *__restrict__ S,
+                                                      int fcg, FcgState *__restrict__ fsp) {
+    __shared__ double sm4[4];
+    constexpr int kPre = 8;
+    const int lane = threadIdx.x & 63;
+    int slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int s0 = min(slice, nslices - 1);
+    const int base0 = sp[s0], w0 = slice < nslices ? (sp[s0 + 1] - base0) >> 6 : 0;
+    const int r0 = min(64 * s0 + lane, n - 1);
+    int cpre[kPre];
+    double vpre[kPre], zpre[kPre], ppre[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int e = base0 + 64 * min(k, max(w0 - 1, 0)) + lane;
+        cpre[k] = sci[e];
+        vpre[k] = k < w0 ? sv[e] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        zpre[k] = z[cpre[k]];
+        ppre[k] = p_old[cpre[k]];
+    }
+    const double zr = z[r0], pr = p_old[r0];
+    double beta = 0.0;
+    if (fcg) {
+        // the flexible direction update of iteration it - 1 (part_rr / part_rz: its r'z and
+        // z'Ap partials): beta = -(z'Ap) / (p'Ap), or 0 on a restart or a non-positive r'z
+        // (which also falls back to Jacobi); workgroup 0 completes the state slot
+        if (it > 1) {
+            const double rz = sum_partials(part_rr, np_step, sm4), zap = sum_partials(part_rz, np_step, sm4);
+            int fb = fsp->fallback1, rs = fsp->restart1;
+            if (rs || !(rz > 0.0)) {
+                beta = 0.0;
+                const int fb0 = fb;
+                rs = 0;
+                if (!(rz > 0.0) && !fb0) {
+                    fb = 1;
+                    rs = 1;
+                }
+            } else {
+                beta = -zap / S->pap;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                fsp->rz2 = rz;
+                fsp->beta = beta;
+                fsp->fallback2 = fb;
+                fsp->restart2 = rs;
+                S->fallback = fb;
+            }
+        }
+    } else if (it > 1) {
+        const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
+        const double rz = sum_partials(part_rz, np_step, sm4);
+        const bool conv = rnorm <= S->thr;
+        beta = rz / S->rz2[(it - 1) & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !S->done) {
+            S->rnorm = rnorm;
+            S->iters = it - 1;
+            if (conv) S->done = 1;
+            else S->rz2[it & 1] = rz;
+        }
+    }
+    double dot = 0.0;
+    for (bool first = true; slice < nslices; slice += gridDim.x * 4, first = false) {
+        const int row = 64 * slice + lane;
+        const int base = first ? base0 : sp[slice], w = first ? w0 : (sp[slice + 1] - base) >> 6;
+        double acc = 0.0;
+        int k = 0;
+        if (first) {
+#pragma unroll
+            for (int q = 0; q < kPre; ++q) acc += vpre[q] * pcg_dir(zpre[q], ppre[q], beta);
+            k = kPre;
+        }
+        for (; k < w; ++k) {
+            const int e = base + 64 * k + lane;
+            const int c = sci[e];
+            acc += sv[e] * pcg_dir(z[c], p_old[c], beta);
+        }
+        if (row < n) {
+            const double pn = first ? pcg_dir(zr, pr, beta) : pcg_dir(z[row], p_old[row], beta);
+            p_new[row] = pn;
+            ap[row] = acc;
+            dot += pn * acc;
+        }
+    }
+    const double t = block_sum_waves(dot, sm4);
+    if (threadIdx.x == 0) part_pap[blockIdx.x] = t;
+}
+
+// Iteration `it`, second kernel: alpha = rz / p'Ap (:137-143; p'Ap <= 0 fails), x += alpha p,
+// r -= alpha Ap (:145-150), z = M r (:155), partial ||r||^2 and r'z.  Each thread's first
+// kPre elements are loaded before the partial sum of p'Ap is formed.  use: 1 Jacobi, 0 none,
+// 2 none and no z (the flexible PCG forms z itself; rz from *rzp, p'Ap and alpha kept in S).
+__global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double *__restrict__ r,
+                                                  double *__restrict__ z, const double *__restrict__ p,
+                                                  const double *__restrict__ ap, const double *__restrict__ invd, int n,
+                                                  int use, int it, const double *__restrict__ part_pap, int np_dir,
+                                                  double *__restrict__ part_rr, double *__restrict__ part_rz,
+                                                  PcgScalars *__restrict__ S, const double *__restrict__ rzp) {
+    __shared__ double sm4[4];
+    constexpr int kPre = 4;
+    const int stride = gridDim.x * kBS, i0 = blockIdx.x * kBS + threadIdx.x;
+    double px[kPre], pp[kPre], pr[kPre], pa[kPre], pd[kPre];
+#pragma unroll
+    for (int e = 0; e < kPre; ++e) {
+        const int i = min(i0 + e * stride, n - 1);
+        px[e] = x[i];
+        pp[e] = p[i];
+        pr[e] = r[i];
+        pa[e] = ap[i];
+        pd[e] = invd[i];
+    }
+    const double pap = sum_partials(part_pap, np_dir, sm4);
+    if (S->done) return;  // uniform
+    if (pap <= 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->pap = pap;
+            S->fail = 1;
+            S->done = 1;
+        }
+        return;
+    }
+    const double alpha = (use == 2 ? *rzp : S->rz2[it & 1]) / pap;
+    if (use == 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+        S->pap = pap;
+        S->alpha = alpha;
+    }
+    double rr = 0.0, rz = 0.0;
+#pragma unroll
+    for (int e = 0; e < kPre; ++e) {
+        const int i = i0 + e * stride;
+        if (i < n) {
+            x[i] = px[e] + alpha * pp[e];
+            const double ri = pr[e] - alpha * pa[e];
+            r[i] = ri;
+            rr += ri * ri;
+            if (use != 2) {
+                const double zi = use ? pd[e] * ri : ri;
+                z[i] = zi;
+                rz += ri * zi;
+            }
+        }
+    }
+    for (int i = i0 + kPre * stride; i < n; i += stride) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+        if (use != 2) {
+            const double zi = use ? invd[i] * ri : ri;
+            z[i] = zi;
+            rz += ri * zi;
+        }
+    }
+    const double t0 = block_sum_waves(rr, sm4), t1 = block_sum_waves(rz, sm4);
+    if (threadIdx.x == 0) {
+        part_rr[blockIdx.x] = t0;
+        if (use != 2) part_rz[blockIdx.x] = t1;
+    }
+}
+
+// after the last iteration: its residual norm and convergence test
+__global__ void k_pcg_last(const double *__restrict__ part_rr, int np_step, int it, PcgScalars *__restrict__ S) {
+    __shared__ double sm4[4];
+    if (S->done) return;
+    const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
+    if (threadIdx.x == 0) {
+        S->rnorm = rnorm;
+        S->iters = it;
+        if (rnorm <= S->thr) S->done = 1;
+    }
+}
+
+// grids of the two kernels (fixed per matrix, so the reductions are deterministic): the
+// direction kernel covers the rows in one pass up to 4096 workgroups, the step kernel's 256
+// workgroups leave 256 partials for the next direction kernel's workgroups to sum
+static int pcg_dir_blocks(const Matrix &A) {
+    return A.sell ? std::min(1024, (A.nslices + 3) / 4) : (int)std::min<int64_t>(1024, nblk((int64_t)A.n * A.lanes, kBS));
+}
+static int pcg_step_blocks(int n) { return (int)std::min<int64_t>(512, nblk(n, kBS)); }
+
+static void launch_pcg_dir(const Matrix &A, const double *z, const double *p_old, double *p_new, double *ap,
+                           const double *part_rr, const double *part_rz, int np_step, int it, double *part,
+                           PcgScalars *S, hipStream_t st) {
+    const dim3 g(pcg_dir_blocks(A)), b(kBS);
+    if (A.sell) {
+        hipLaunchKernelGGL(k_pcg_dir_sell, g, b, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, A.n, A.nslices, z,
+                           p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S, 0, (FcgState *)nullptr);
+        return;
+    }
+    switch (A.lanes) {
+        case 4: hipLaunchKernelGGL(k_pcg_dir_spmv<4>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
+        case 8: hipLaunchKernelGGL(k_pcg_dir_spmv<8>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
+        case 16: hipLaunchKernelGGL(k_pcg_dir_spmv<16>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
+        case 32: hipLaunchKernelGGL(k_pcg_dir_spmv<32>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
+        default: hipLaunchKernelGGL(k_pcg_dir_spmv<64>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
+    }
+}
+
 static void launch_spmv_guard(const Matrix &A, const double *x, double *y, const PcgScalars *S, hipStream_t st) {
     const int64_t threads = (int64_t)A.n * A.lanes;
     const dim3 g(nblk(threads, kBS)), b(kBS);
@@ -334,8 +657,10 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
     if (n == 0) return;
     if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    DevArray<double> r(n), z(n), p(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
+    DevArray<double> r(n), z(n), pa(n), pb(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
+    DevArray<double> pdir(pcg_dir_blocks(A)), prr(pcg_step_blocks(n)), prz(pcg_step_blocks(n));
     DevArray<PcgScalars> S(1);
+    MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
     const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks);
     // r0 = b - A x0 (:115-117), ||r0||, ||b||
     launch_spmv(A, d_x, ap.p, st);
@@ -349,26 +674,27 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
     if (!h.done) {
         if (use_jacobi && A.diag_status)
             throw Error(MG_E_NOT_PD, "The diagonal element is zero or invalid while building the Jacobi preconditioner.");
+        // z0 = M r0, rz = r0'z0 (:119-121); the first direction comes from beta = 0, p_old = 0
         hipLaunchKernelGGL(k_pcg_precond_dot, gr, bv, 0, st, A.invdiag.p, r.p, z.p, n, use_jacobi, part.p, S.p);
         hipLaunchKernelGGL(k_pcg_rz0, dim3(1), bv, 0, st, part.p, S.p);
-        MG_HIP(hipMemcpyAsync(p.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        MG_HIP(hipMemsetAsync(pa.p, 0, sizeof(double) * n, st));
         const int maxit = max_it > 0 ? max_it : n;
         const int check_every = 16;
-        for (int it = 1; it <= maxit; ++it) {
-            launch_spmv_guard(A, p.p, ap.p, S.p, st);
-            hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, p.p, ap.p, n, part.p, S.p);
-            hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
-            hipLaunchKernelGGL(k_pcg_update, gr, bv, 0, st, d_x, r.p, p.p, ap.p, n, part.p, S.p);
-            hipLaunchKernelGGL(k_pcg_norm, dim3(1), bv, 0, st, part.p, it, S.p);
-            hipLaunchKernelGGL(k_pcg_precond_dot, gr, bv, 0, st, A.invdiag.p, r.p, z.p, n, use_jacobi, part.p, S.p);
-            hipLaunchKernelGGL(k_pcg_beta, dim3(1), bv, 0, st, part.p, S.p);
-            hipLaunchKernelGGL(k_pcg_pupdate, gv, bv, 0, st, z.p, p.p, n, S.p);
-            if (it % check_every == 0 || it == maxit) {
+        const int ndir = pcg_dir_blocks(A), nstep = pcg_step_blocks(n);
+        double *p_old = pa.p, *p_new = pb.p;
+        int it = 1;
+        for (; it <= maxit; ++it) {
+            launch_pcg_dir(A, z.p, p_old, p_new, ap.p, prr.p, prz.p, nstep, it, pdir.p, S.p, st);
+            hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n,
+                               use_jacobi, it, pdir.p, ndir, prr.p, prz.p, S.p, (const double *)nullptr);
+            std::swap(p_old, p_new);
+            if (it % check_every == 0) {
                 MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
                 MG_HIP(hipStreamSynchronize(st));
                 if (h.done) break;
             }
         }
+        if (it > maxit) hipLaunchKernelGGL(k_pcg_last, dim3(1), bv, 0, st, prr.p, nstep, maxit, S.p);
         MG_HIP(hipGetLastError());
         MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
@@ -402,6 +728,29 @@ __global__ void k_fcg_norm(const double *__restrict__ part, int it, int stall, P
     __shared__ double sm[kBS];
     if (S->done) return;
     const double rr = finish_sum(part, sm);
+    if (threadIdx.x == 0) {
+        S->rnorm = sqrt(rr);
+        S->iters = it;
+        if (S->rnorm <= S->thr) {
+            S->done = 1;
+        } else if (!S->fallback) {
+            if (S->rnorm < S->best) {
+                S->best = S->rnorm;
+                S->best_it = it;
+            } else if (it - S->best_it >= stall) {
+                S->fallback = 1;
+                S->restart = 1;
+            }
+        }
+    }
+}
+
+// the same test over the nparts partials of the two-kernel step
+__global__ void k_fcg_norm_n(const double *__restrict__ part, int nparts, int it, int stall,
+                             PcgScalars *__restrict__ S) {
+    __shared__ double sm4[4];
+    if (S->done) return;
+    const double rr = sum_partials(part, nparts, sm4);
     if (threadIdx.x == 0) {
         S->rnorm = sqrt(rr);
         S->iters = it;
@@ -459,11 +808,10 @@ __global__ void k_fcg_start(const double *__restrict__ rz_part, int nparts, PcgS
 // beta = -(z'Ap) / (p'Ap) (0 on a restart); rz = r'z for the next step length
 __global__ void k_fcg_beta(const double *__restrict__ rz_part, const double *__restrict__ zap_part, int nparts,
                            PcgScalars *__restrict__ S) {
-    __shared__ double sm[kBS];
+    __shared__ double sm4[4];
     if (S->done) return;
-    const double rz = finish_sum_n(rz_part, nparts, sm);
-    __syncthreads();
-    const double zap = finish_sum_n(zap_part, nparts, sm);
+    const double rz = sum_partials(rz_part, nparts, sm4);
+    const double zap = sum_partials(zap_part, nparts, sm4);
     if (threadIdx.x == 0) {
         S->rz = rz;
         if (S->restart || !(rz > 0.0)) {
@@ -477,6 +825,17 @@ __global__ void k_fcg_beta(const double *__restrict__ rz_part, const double *__r
             S->beta = -zap / S->pap;
         }
     }
+}
+
+// slot 0 of the fused flexible PCG's state from the start (r'z, ||r0||, a fallback at the start)
+__global__ void k_fcg_state0(const PcgScalars *__restrict__ S, FcgState *__restrict__ fs) {
+    fs[0].rnorm = S->rnorm;
+    fs[0].best = S->best;
+    fs[0].best_it = 0;
+    fs[0].rz2 = S->rz;
+    fs[0].beta = 0.0;
+    fs[0].fallback1 = fs[0].fallback2 = S->fallback;
+    fs[0].restart1 = fs[0].restart2 = 0;
 }
 
 void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d_x0, double tol, int max_it,
@@ -494,7 +853,8 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
     else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
     // the preconditioner kernels write one partial sum per block of their full grid
     const int npart = mpnn_partial_blocks(n);
-    DevArray<double> r(n), z(n), p(n), ap(n), part(std::max(npart, kRedBlocks)), part2(std::max(npart, kRedBlocks));
+    DevArray<double> r(n), z(n), p(n), p2(n), ap(n), part(std::max(npart, kRedBlocks)),
+        part2(std::max(npart, kRedBlocks));
     DevArray<PcgScalars> S(1);
     MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
     const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks), gp(npart);
@@ -519,7 +879,37 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
         const int maxit = max_it > 0 ? max_it : n;
         const int check_every = 16;
         const int stall = stall_window > 0 ? stall_window : 50;
-        for (int it = 1; it <= maxit; ++it) {
+        bool fused = false;
+        if (A.sell && A.mp && A.mp->nslices) {
+            // four kernels per iteration: the direction (p = z + beta p in the gather, beta from
+            // the previous iteration's partials; beta = 0 and p_old = 0 at the start) with the
+            // SpMV and p'Ap partials; alpha and the x / r update with ||r||^2 partials; y = G r
+            // and z = G' y (or Jacobi), each with the residual-norm test of the step folded in
+            MG_HIP(hipMemsetAsync(p2.p, 0, sizeof(double) * n, st));
+            const int ndir = pcg_dir_blocks(A), nstep = pcg_step_blocks(n), nfcg = mpnn_fcg_partials(A);
+            DevArray<double> pdir(ndir), prr(nstep);
+            DevArray<FcgState> fs(2);
+            hipLaunchKernelGGL(k_fcg_state0, dim3(1), dim3(1), 0, st, S.p, fs.p);
+            double *p_old = p2.p, *p_new = p.p;
+            fused = true;
+            for (int it = 1; it <= maxit; ++it) {
+                FcgState *prev = fs.p + ((it - 1) & 1), *cur = fs.p + (it & 1);
+                hipLaunchKernelGGL(k_pcg_dir_sell, dim3(ndir), bv, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, n,
+                                   A.nslices, z.p, p_old, p_new, ap.p, part.p, part2.p, nfcg, it, pdir.p, S.p, 1, prev);
+                hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n, 2,
+                                   it, pdir.p, ndir, prr.p, (double *)nullptr, S.p, (const double *)&prev->rz2);
+                if (!mpnn_fcg_apply(net, A, r.p, z.p, ap.p, prr.p, nstep, it, stall, prev, cur, S.p, part.p, part2.p,
+                                    st))
+                    throw Error(MG_E_DEVICE, "The learned factor has no sliced copy for the fused iteration.");
+                std::swap(p_old, p_new);
+                if (it % check_every == 0 || it == maxit) {
+                    MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
+                    MG_HIP(hipStreamSynchronize(st));
+                    if (h.done) break;
+                }
+            }
+        }
+        for (int it = 1; !fused && it <= maxit; ++it) {
             launch_spmv_guard(A, p.p, ap.p, S.p, st);
             hipLaunchKernelGGL(k_dot_part, gr, bv, 0, st, p.p, ap.p, n, part.p, S.p);
             hipLaunchKernelGGL(k_pcg_alpha, dim3(1), bv, 0, st, part.p, S.p);
