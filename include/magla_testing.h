@@ -22,11 +22,25 @@ int mg_test_numeric_front(const mg_numeric *num, int32_t supernode, double *out)
  * must equal i + 16 j (256 values). */
 int mg_test_mfma_layout(int32_t precision, double *out);
 
-/* Device schedule summary computed on the host (no GPU needed): out[0..6] =
- * fronts (fused supernode chains), levels, arena elements (sum f^2), largest
- * front, widest front, kernel launches per factorization, 64x64 Schur tiles of
- * the lookahead part (the trailing update the next panels do not touch). */
+/* Device schedule summary computed on the host (no GPU needed): out[0..9] =
+ * fronts (fused supernode chains), levels, arena elements (factor store,
+ * contribution pool and their padding), largest front, widest front, kernel
+ * launches per factorization, 64x64 Schur tiles of the lookahead part (the
+ * trailing update the next panels do not touch), factor-store elements
+ * (sum f w), contribution-pool elements, largest sum of contribution-block
+ * elements live at once. */
 int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out);
+
+/* Placement of every device front, computed on the host (no GPU needed): for
+ * front g, out[6 g .. 6 g + 5] = size f, width w, level, parent front (-1 for a
+ * root), factor-store offset, contribution-pool offset (arena elements).
+ * out holds 6 x fronts values (fronts = mg_test_plan_stats out[0]). */
+int mg_test_plan_layout(const mg_symbolic *sym, int64_t *out);
+
+/* Arena offsets of n front entries (i[k], j[k]), i >= j, of a front of size f and
+ * width w placed at factor-store offset off and pool offset coff. */
+int mg_test_front_addr(int32_t f, int32_t w, int64_t off, int64_t coff, int32_t n, const int32_t *i,
+                       const int32_t *j, int64_t *out);
 /* One v_mfma_f32_16x16x16_bf16 with the operand layout of the learned preconditioner's
  * layer kernel: d = a b for 16x16 row-major a, b (rounded to bf16), d in FP32. */
 int mg_test_mfma_bf16(const float *a, const float *b, float *d);

@@ -201,6 +201,8 @@ def lib():
     L.mg_test_numeric_front.argtypes = [vp, C.c_int32, vp]
     L.mg_test_mfma_layout.argtypes = [C.c_int32, vp]
     L.mg_test_plan_stats.argtypes = [vp, vp]
+    L.mg_test_plan_layout.argtypes = [vp, vp]
+    L.mg_test_front_addr.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, vp, vp, vp]
     L.mg_profile_enable.argtypes = [C.c_int32]
     L.mg_profile_enable.restype = None
     L.mg_profile_read.argtypes = [C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), C.POINTER(C.c_double),
@@ -219,11 +221,29 @@ def test_mfma_layout(precision):
 
 def plan_stats(symbolic):
     """Device schedule summary (host only): fronts, levels, arena elements, max front, max width,
-    launches per factorization, Schur tiles of the lookahead part."""
-    out = np.zeros(7, dtype=np.int64)
+    launches per factorization, Schur tiles of the lookahead part, factor-store elements,
+    contribution-pool elements, largest sum of contribution elements live at once."""
+    out = np.zeros(10, dtype=np.int64)
     _check(lib().mg_test_plan_stats(symbolic._h, _ptr(out)))
-    return dict(zip(["fronts", "levels", "arena_elems", "max_front", "max_width", "launches", "lookahead_tiles"],
-                    out.tolist()))
+    return dict(zip(["fronts", "levels", "arena_elems", "max_front", "max_width", "launches", "lookahead_tiles",
+                     "factor_elems", "pool_elems", "pool_live_elems"], out.tolist()))
+
+
+def plan_layout(symbolic):
+    """Placement of every device front (host only): (fronts, 6) int64 array of size, width, level,
+    parent front (-1: root), factor-store offset, contribution-pool offset."""
+    out = np.zeros((plan_stats(symbolic)["fronts"], 6), dtype=np.int64)
+    _check(lib().mg_test_plan_layout(symbolic._h, _ptr(out)))
+    return out
+
+
+def front_addr(f, w, off, coff, i, j):
+    """Arena offsets of the entries (i[k], j[k]), i >= j, of a front placed at (off, coff)."""
+    i = np.ascontiguousarray(i, dtype=np.int32)
+    j = np.ascontiguousarray(j, dtype=np.int32)
+    out = np.zeros(len(i), dtype=np.int64)
+    _check(lib().mg_test_front_addr(f, w, off, coff, len(i), _ptr(i), _ptr(j), _ptr(out)))
+    return out
 
 
 def test_factor_block(numeric, supernode, front_size, width):

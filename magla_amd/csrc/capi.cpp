@@ -15,6 +15,8 @@ void numeric_front(const Numeric *N, int s, double *out);
 void selftest_mfma(int precision, double *host_out);
 void profile_enable(int on);
 void plan_stats(const Symbolic &S, int64_t *out);
+void plan_layout(const Symbolic &S, int64_t *out);
+int64_t front_addr(int64_t off, int64_t coff, int f, int w, int i, int j);
 int profile_read(int kind, const char **name, int64_t *launches, double *ms, double *flops, double *bytes);
 void csr_from_coo(int n, int64_t count, const int *rows, const int *cols, const double *vals, int *nnz_out,
                   int *rowptr, int *colidx, double *values);
@@ -793,6 +795,25 @@ int mg_test_plan_stats(const mg_symbolic *sym, int64_t *out) {
     return guarded([&] {
         require(sym != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
         mg::plan_stats(sym->s, out);
+    });
+}
+
+int mg_test_plan_layout(const mg_symbolic *sym, int64_t *out) {
+    return guarded([&] {
+        require(sym != nullptr && out != nullptr, MG_E_NULL, "argument is null.");
+        mg::plan_layout(sym->s, out);
+    });
+}
+
+int mg_test_front_addr(int32_t f, int32_t w, int64_t off, int64_t coff, int32_t n, const int32_t *i,
+                       const int32_t *j, int64_t *out) {
+    return guarded([&] {
+        require(n == 0 || (i != nullptr && j != nullptr && out != nullptr), MG_E_NULL, "argument is null.");
+        require(f >= 0 && w >= 0 && w <= f && n >= 0, MG_E_RANGE, "f, w or n is out of range.");
+        for (int32_t k = 0; k < n; ++k) {
+            require(i[k] >= j[k] && j[k] >= 0 && i[k] < f, MG_E_RANGE, "entry is not in the front's lower triangle.");
+            out[k] = mg::front_addr(off, coff, f, w, i[k], j[k]);
+        }
     });
 }
 

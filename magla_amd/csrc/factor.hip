@@ -2,12 +2,16 @@
 // supernodal solves on MI355X (gfx950).
 //
 // Device layout
-//   * Every supernode s owns a dense f_s x f_s column-major front (ld = f_s)
-//     in one arena (sum f^2 elements of the factor precision).  After the
-//     partial factorization, columns [0, w_s) hold the factor block (unit
-//     diagonal, L below it: MAGLA's factor column k = [1, L[k+1.., k]],
-//     UtduNumericFactorization.cs:517-519 / 591) and the trailing
-//     t_s x t_s block is the contribution read by the parent (:521-530).
+//   * A front of f rows and w pivot columns lives in two places of one arena
+//     (elements of the factor precision, layout in front.h).  Its pivot
+//     columns [0, w) are an f x w column-major block in the factor store,
+//     kept for the solves: unit diagonal, L below it (MAGLA's factor column
+//     k = [1, L[k+1.., k]], UtduNumericFactorization.cs:517-519 / 591).  Its
+//     trailing t x t block (t = f - w) is the contribution read by the
+//     parent (:521-530); it lives in a contribution pool only from the
+//     front's level to its parent's, and blocks whose lifetimes do not
+//     overlap share memory, as the reference's peak-contribution estimate
+//     assumes (SupernodalStructure.cs:368-418).
 //     Nothing is copied between fronts except the extend-add itself.
 //   * The supernodal tree is cut into levels by height (leaves = 0); all
 //     fronts of a level are independent and go through one launch per step.
@@ -23,10 +27,14 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <climits>
+#include <map>
 #include <mutex>
+#include <set>
 #include <type_traits>
 
 #include "device.h"
+#include "front.h"
 #include "mfma.h"
 
 namespace mg {
@@ -51,7 +59,9 @@ static inline int schur_bucket(int f) {
 }
 constexpr int kSmallFront = 64;       // fronts this small are solved by one wave each (all panels, all RHS)
 constexpr int kSmallFactor = 32;      // fronts this small are factored whole by a lane group (k_factor_small)
-constexpr int kArenaPad = 64;         // tail elements after the last front (16-byte vector over-reads)
+constexpr int kArenaPad = 64;         // tail elements after the factor store and after the pool (16-byte vector over-reads)
+constexpr int kPoolAlign = 64;        // contribution blocks start at multiples of this many elements
+
 typedef unsigned int mg_u4u __attribute__((ext_vector_type(4), aligned(4)));  // unaligned 16-byte load
 constexpr int kFootChunk = 256;       // footer rows per partial product in the backward solve
 constexpr int kRhsChunk = 16;         // right-hand sides per pass of the solve kernels
@@ -68,8 +78,8 @@ constexpr int kPanelFusedRows = 4;    // fronts with at most this many row block
 // A Schur tile task with the front's size, width, first column and arena offset (one
 // dependent load before the panel loads instead of two).
 struct __align__(8) SchurTask {
-    int g, ti, tj, f, w, fc;  // front, tile row and column, size, width, first column
-    int64_t off;              // front's arena offset
+    int g, i0, j0, f, w, fc;  // front, tile's first row and column, size, width, first column
+    int64_t off, coff;        // front's factor-store and contribution-pool offsets
 };
 
 // A workgroup's task in the panel solves, carrying the front's sizes, offsets and system so
@@ -84,23 +94,27 @@ struct __align__(8) PanelTask {
 // that the lane groups' data loads depend on this one load.
 struct __align__(16) SmallRec {
     int f, w, fc, sys;    // size, width, first column, system
-    int64_t off, vbase;   // arena offset, vector base
+    int64_t off, vbase;   // factor-store offset, vector base
     int c0, c1, pat, pad; // children-array range, first pattern row
     int64_t inv0;         // inverse row map of its first child (the children's maps follow, f entries each)
-    int64_t pad2;
+    int64_t coff;         // contribution-pool offset
 };
-// A children-array slot: the child's vector base, arena offset, size and width.
+// A children-array slot: the child's vector base, factor-store and pool offsets, size and width.
 struct __align__(16) SmallChild {
-    int64_t vb, off;
-    int f, w, pad0, pad1;
+    int64_t vb, off, coff;
+    int f, w;
 };
 
 struct DevicePlan {
     int n = 0, nsuper = 0, nfront = 0, nlevels = 0, max_w = 0, max_f = 0;
-    int64_t arena = 0;  // sum f^2 over fronts
+    int64_t arena = 0;  // factor store, pad, contribution pool, pad (elements)
+    int64_t fstore = 0; // factor store: sum f w over fronts
+    int64_t pool = 0;   // contribution pool (peak of the live blocks, with fragmentation)
+    int64_t pool_live = 0;  // largest sum of contribution blocks live at once
     int64_t usum = 0;   // sum (f - w) over fronts
     std::vector<int> h_f, h_w, h_first;
-    std::vector<int64_t> h_off;
+    std::vector<int64_t> h_off, h_coff;  // per front: factor-store offset, contribution-pool offset
+    std::vector<int> h_level, h_parent;  // per front: level (height in the front tree), parent front (-1: root)
     std::vector<int> front_of;   // supernode -> front
     std::vector<int> h_pat_ptr, h_pat_rows;
     std::vector<int> lvl_node_ptr, lvl_pull_ptr;
@@ -151,7 +165,10 @@ struct DevicePlan {
     DevArray<int> front_sys, sys_begin;
     DevArray<int> row_sys;  // batches: system of every row (elimination order = input order per system)
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
-    DevArray<int64_t> off, inv_off, u_off, asm_pos, vbase;
+    // per inverse-map entry a (>= 0): the child's contribution column a as an offset from its
+    // pool offset, contrib_col(0, t, a) (below 2^32: t (t + 1) / 2 <= INT32_MAX); ~0u where a < 0
+    DevArray<uint32_t> inv_col;
+    DevArray<int64_t> off, coff, inv_off, u_off, asm_pos, vbase;
     DevArray<int2> init_tasks;
     DevArray<PanelTask> fa_tasks, fb_tasks, ba_tasks, bb_tasks;
     DevArray<int4> footr_tasks;    // (front, column block, first partial task, partial count)
@@ -199,6 +216,88 @@ __host__ __device__ inline SchurStep schur_step(int f, int w, int q) {
     }
     st.any = st.base < f;
     return st;
+}
+
+// Contribution pool.  Front g's block (contrib_size(f - w) elements) is written by the assembly
+// of g's level and read by the assembly of its parent's level, so it is live over the levels
+// [level(g), level(parent)], both included; levels run one after another, so blocks whose level
+// ranges do not meet may share memory.  Before the fronts of level L are placed, the blocks last
+// read at level L - 1 are released (coalesced with free neighbours); the level's blocks then go,
+// largest first, to the smallest free range that holds them (lowest offset among equals), or at
+// the end of the pool.  O(log) per block: batches have millions of fronts.  Sets h_coff (arena
+// offsets: the pool follows the factor store and its padding), pool, pool_live and arena.
+static void assign_contributions(DevicePlan &P, const std::vector<int> &level, const std::vector<int> &parent,
+                                 const std::vector<std::vector<int>> &bylevel) {
+    const int nf = P.nfront, nl = P.nlevels;
+    std::vector<int64_t> size(nf, 0);
+    for (int g = 0; g < nf; ++g)
+        size[g] = (contrib_size(P.h_f[g] - P.h_w[g]) + kPoolAlign - 1) / kPoolAlign * kPoolAlign;
+    std::vector<int64_t> at(nf, 0);
+    std::vector<std::vector<int>> release(nl + 1);        // blocks released before level L
+    std::map<int64_t, int64_t> by_off;                     // free ranges: offset -> size, none adjacent
+    std::set<std::pair<int64_t, int64_t>> by_size;         // the same ranges as (size, offset)
+    auto take = [&](std::map<int64_t, int64_t>::iterator it) {
+        by_size.erase({it->second, it->first});
+        return by_off.erase(it);
+    };
+    int64_t top = 0, live = 0;  // the pool's free tail [top, ...) is not in the lists
+    P.pool_live = 0;
+    for (int L = 0; L < nl; ++L) {
+        for (int g : release[L]) {
+            int64_t o = at[g], sz = size[g];
+            live -= sz;
+            auto nx = by_off.lower_bound(o);
+            if (nx != by_off.end() && nx->first == o + sz) {
+                sz += nx->second;
+                nx = take(nx);
+            }
+            if (nx != by_off.begin()) {
+                auto pv = std::prev(nx);
+                if (pv->first + pv->second == o) {
+                    o = pv->first;
+                    sz += pv->second;
+                    take(pv);
+                }
+            }
+            if (o + sz == top) {
+                top = o;
+            } else {
+                by_off[o] = sz;
+                by_size.insert({sz, o});
+            }
+        }
+        std::vector<int> blocks;
+        for (int g : bylevel[L])
+            if (size[g] > 0) blocks.push_back(g);
+        std::stable_sort(blocks.begin(), blocks.end(), [&](int a, int b) { return size[a] > size[b]; });
+        for (int g : blocks) {
+            const int64_t sz = size[g];
+            int64_t o;
+            auto fit = by_size.lower_bound({sz, INT64_MIN});
+            if (fit != by_size.end()) {
+                o = fit->second;
+                const int64_t rest = fit->first - sz;
+                take(by_off.find(o));
+                if (rest > 0) {
+                    by_off[o + sz] = rest;
+                    by_size.insert({rest, o + sz});
+                }
+            } else {
+                o = top;
+                top += sz;
+            }
+            at[g] = o;
+            live += sz;
+            P.pool_live = std::max(P.pool_live, live);
+            release[(parent[g] >= 0 ? level[parent[g]] : L) + 1].push_back(g);
+        }
+        P.pool = std::max(P.pool, top);
+    }
+    const int64_t base = P.fstore + kArenaPad;
+    P.h_coff.assign(nf, base);
+    for (int g = 0; g < nf; ++g)
+        if (size[g] > 0) P.h_coff[g] = base + at[g];
+    P.arena = base + P.pool + kArenaPad;
 }
 
 static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool upload) {
@@ -277,10 +376,10 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         const int f = P->h_f[g], w = P->h_w[g];
         P->max_w = std::max(P->max_w, w);
         P->max_f = std::max(P->max_f, f);
-        P->h_off[g + 1] = P->h_off[g] + (int64_t)f * f;
+        P->h_off[g + 1] = P->h_off[g] + (int64_t)f * w;
         uoff[g + 1] = uoff[g] + (f - w);
     }
-    P->arena = P->h_off[nf];
+    P->fstore = P->h_off[nf];
     P->usum = uoff[nf];
 
     // levels by height in the front tree
@@ -295,6 +394,9 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->nlevels = nl;
     std::vector<std::vector<int>> bylevel(nl);
     for (int g = 0; g < nf; ++g) bylevel[level[g]].push_back(g);
+    assign_contributions(*P, level, fparent, bylevel);
+    P->h_level = level;
+    P->h_parent = fparent;
 
     // inverse row maps: for child front x of front g, inv_x[i] = index of g's row i in x's footer;
     // laid out in children-array order, so the maps of g's children are consecutive (f_g each)
@@ -352,9 +454,13 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
             if (f <= kSmallFront) small_cls[f <= 8 ? 0 : (f <= 16 ? 1 : (f <= 32 ? 2 : 3))].push_back(g);
             else
                 for (int c = 0; c < (f + 255) / 256; ++c) init.push_back(make_int2(g, c));
-            const int T = f <= kSmallFactor ? 0 : (f + kTile - 1) / kTile;  // small fronts: k_assemble_small
-            for (int tj = 0; tj < T; ++tj)
-                for (int ti = tj; ti < T; ++ti) pull.push_back(make_int4(g, ti, tj, 0));
+            // 64x64 tiles of the lower triangle, (front, first row, first column): the pivot
+            // columns tiled from column 0, the contribution's columns from column w, so that a
+            // tile's columns lie in one region (small fronts: k_assemble_small)
+            if (f > kSmallFactor)
+                for (int fam = 0; fam < 2; ++fam)
+                    for (int j0 = fam ? w : 0; j0 < (fam ? f : w); j0 += kTile)
+                        for (int i0 = j0; i0 < f; i0 += kTile) pull.push_back(make_int4(g, i0, j0, 0));
             pe_elems += (double)f * (f + 1) / 2;
             for (int c = fchild_ptr[g]; c < fchild_ptr[g + 1]; ++c) {
                 const double tc = P->h_f[fchildren[c]] - P->h_w[fchildren[c]];
@@ -477,19 +583,26 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
                 if (st.any && !st.inner) beyond = 1;
             }
             bey.push_back(beyond);
-            // one front's tiles of one part in one shape: columns in ascending order, rows
-            // down from the diagonal
+            // one front's tiles of one part in one shape, (front, first row, first column):
+            // columns in ascending order, rows down from the diagonal.  Target columns before w
+            // are tiled from the step's base, those from w on (the contribution, outer steps)
+            // from w, so that every wave's columns lie in one region and one 64-column block
+            // of the contribution (FrontCols::run)
             auto push_tiles = [&](int g, const SchurStep &st, int part, bool big) {
                 const int f = P->h_f[g], w = P->h_w[g];
                 const int tile = big ? 128 : kTile, next_tiles = kBlock / tile;
                 const bool has_next = !st.inner && st.base < w;
-                const int rt = (f - st.base + tile - 1) / tile, ct = (st.cend - st.base + tile - 1) / tile;
                 int count = 0;
-                for (int tj = 0; tj < ct; ++tj) {
-                    const bool first = st.inner || (has_next && tj < next_tiles);
-                    if (first != (part == 0)) continue;
-                    for (int ti = tj; ti < rt; ++ti) (big ? upd128 : upd).push_back(make_int4(g, ti, tj, big));
-                    count += rt - tj;
+                for (int fam = 0; fam < 2; ++fam) {
+                    const int c0 = fam ? std::max(st.base, w) : st.base, c1 = fam ? st.cend : std::min(st.cend, w);
+                    for (int j0 = c0, tj = 0; j0 < c1; j0 += tile, ++tj) {
+                        const bool first = st.inner || (fam == 0 && has_next && tj < next_tiles);
+                        if (first != (part == 0)) continue;
+                        for (int i0 = j0; i0 < f; i0 += tile) {
+                            (big ? upd128 : upd).push_back(make_int4(g, i0, j0, big));
+                            ++count;
+                        }
+                    }
                 }
                 return count;
             };
@@ -574,13 +687,13 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         for (size_t k = 0; k < small.size(); ++k) {
             const int g = small[k], c0 = fchild_ptr[g], c1 = fchild_ptr[g + 1];
             recs[k] = SmallRec{P->h_f[g], P->h_w[g], P->h_first[g], fsys[g], P->h_off[g], vbase[g],
-                               c0, c1, P->h_pat_ptr[g], 0, c1 > c0 ? inv_off[fchildren[c0]] : 0, 0};
+                               c0, c1, P->h_pat_ptr[g], 0, c1 > c0 ? inv_off[fchildren[c0]] : 0, P->h_coff[g]};
         }
         P->small_recs.upload(recs);
         std::vector<SmallChild> crec(std::max<size_t>(fchildren.size(), 1));
         for (size_t c = 0; c < fchildren.size(); ++c) {
             const int x = fchildren[c];
-            crec[c] = SmallChild{vbase[x], P->h_off[x], P->h_f[x], P->h_w[x], 0, 0};
+            crec[c] = SmallChild{vbase[x], P->h_off[x], P->h_coff[x], P->h_f[x], P->h_w[x]};
         }
         P->child_recs.upload(crec);
     }
@@ -609,9 +722,22 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     }
     P->vbase.upload(vbase);
     P->inv_map.upload(inv_map);
+    {
+        std::vector<uint32_t> inv_col(inv_map.size(), ~0u);
+        for (int x = 0; x < nf; ++x) {
+            if (fparent[x] < 0) continue;
+            const int tx = P->h_f[x] - P->h_w[x], fp = P->h_f[fparent[x]];
+            for (int k = 0; k < fp; ++k) {
+                const int a = inv_map[inv_off[x] + k];
+                if (a >= 0) inv_col[inv_off[x] + k] = (uint32_t)contrib_col(0, tx, a);
+            }
+        }
+        P->inv_col.upload(inv_col);
+    }
     P->perm.upload(S.perm);
     P->iperm.upload(S.iperm);
     P->off.upload(P->h_off);
+    P->coff.upload(P->h_coff);
     P->inv_off.upload(inv_off);
     P->u_off.upload(uoff);
     P->asm_pos.upload(asm_pos);
@@ -621,7 +747,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         std::vector<SchurTask> o(v.size());
         for (size_t i = 0; i < v.size(); ++i) {
             const int g = v[i].x;
-            o[i] = SchurTask{g, v[i].y, v[i].z, P->h_f[g], P->h_w[g], P->h_first[g], P->h_off[g]};
+            o[i] = SchurTask{g, v[i].y, v[i].z, P->h_f[g], P->h_w[g], P->h_first[g], P->h_off[g], P->h_coff[g]};
         }
         return o;
     };
@@ -641,7 +767,8 @@ void ensure_plan(Symbolic &S) {
 
 // Host-only summary of the device schedule: fronts, levels, arena elements, largest front,
 // widest front, kernel launches per factorization, Schur tiles (64x64) of the lookahead part
-// that may overlap the next panels.
+// that may overlap the next panels, factor-store elements, contribution-pool elements, largest
+// sum of contribution blocks live at once.
 void plan_stats(const Symbolic &S, int64_t *out) {
     auto P = build_plan_host(S, false);
     int64_t launches = 0, part1 = 0;
@@ -660,7 +787,25 @@ void plan_stats(const Symbolic &S, int64_t *out) {
     out[4] = P->max_w;
     out[5] = launches;
     out[6] = part1;
+    out[7] = P->fstore;
+    out[8] = P->pool;
+    out[9] = P->pool_live;
 }
+
+void plan_layout(const Symbolic &S, int64_t *out) {
+    auto P = build_plan_host(S, false);
+    for (int g = 0; g < P->nfront; ++g) {
+        int64_t *o = out + 6 * (int64_t)g;
+        o[0] = P->h_f[g];
+        o[1] = P->h_w[g];
+        o[2] = P->h_level[g];
+        o[3] = P->h_parent[g];
+        o[4] = P->h_off[g];
+        o[5] = P->h_coff[g];
+    }
+}
+
+int64_t front_addr(int64_t off, int64_t coff, int f, int w, int i, int j) { return front_col(off, coff, f, w, j) + i; }
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -671,46 +816,48 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_pull(const int4 *__restrict__ tasks, T *__restrict__ arena,
                                                        const int *__restrict__ F, const int *__restrict__ W,
                                                        const int64_t *__restrict__ off,
+                                                       const int64_t *__restrict__ coff,
                                                        const int *__restrict__ child_ptr,
                                                        const int *__restrict__ children,
                                                        const int64_t *__restrict__ inv_off,
-                                                       const int *__restrict__ inv_map) {
-    const int4 tk = tasks[blockIdx.x];
+                                                       const int *__restrict__ inv_map,
+                                                       const uint32_t *__restrict__ inv_col) {
+    const int4 tk = tasks[blockIdx.x];  // (front, first row, first column)
     const int s = tk.x;
     const int f = F[s];
-    const int i = tk.y * kTile + (threadIdx.x & 63);
-    const int jbase = tk.z * kTile + (threadIdx.x >> 6);
-    T *P = arena + off[s];
+    const int i = tk.y + (threadIdx.x & 63);
+    const int jbase = tk.z + (threadIdx.x >> 6);
+    const ColRun dst = FrontCols{off[s], coff[s], f, W[s]}.run(tk.z, tk.z, kTile);
     const int c0 = child_ptr[s], c1 = child_ptr[s + 1];
     if (i >= f) return;
-    // columns j = jbase + 4q <= i of row i; each child's row map entry for i and its
-    // 16 column entries are read together, then the 16 update values together
-    const int nq = i < jbase ? 0 : min(kTile / 4, (i - jbase) / 4 + 1);
-    if (nq == 0) return;
+    // columns j = jbase + 4q <= i of row i inside the tile's run; each child's row map entry
+    // for i and its 16 column entries are read together, then the 16 update values together
+    const int nq = i < jbase ? 0 : min(min(kTile / 4, (i - jbase) / 4 + 1), (dst.jend - jbase + 3) / 4);
+    if (nq <= 0) return;
     T acc[kTile / 4];
 #pragma unroll
     for (int q = 0; q < kTile / 4; ++q) acc[q] = T(0);
     for (int c = c0; c < c1; ++c) {  // children in order: each sum keeps the child order
         const int ch = children[c];
-        const int *inv = inv_map + inv_off[ch];
-        const int b = inv[i];
+        const int b = inv_map[inv_off[ch] + i];
         if (b < 0) continue;
-        const int fc = F[ch], wc = W[ch];
-        const T *U = arena + off[ch] + wc + b;
+        // the child's contribution entry (b, a), b >= a, is U[inv_col entry of a + b]
+        const T *U = arena + coff[ch];
+        const uint32_t *icol = inv_col + inv_off[ch];
         // unconditional loads, masked afterwards: past the thread's last column the
         // indices repeat that column's (cache hits, no extra traffic)
-        int a[kTile / 4];
+        uint32_t a[kTile / 4];
 #pragma unroll
-        for (int q = 0; q < kTile / 4; ++q) a[q] = inv[jbase + 4 * min(q, nq - 1)];
+        for (int q = 0; q < kTile / 4; ++q) a[q] = icol[jbase + 4 * min(q, nq - 1)];
         T u[kTile / 4];
 #pragma unroll
-        for (int q = 0; q < kTile / 4; ++q) u[q] = U[(int64_t)(wc + max(a[q], 0)) * fc];
+        for (int q = 0; q < kTile / 4; ++q) u[q] = U[(a[q] == ~0u ? 0u : a[q]) + (uint32_t)b];
 #pragma unroll
-        for (int q = 0; q < kTile / 4; ++q) acc[q] = (q < nq && a[q] >= 0) ? acc[q] + u[q] : acc[q];
+        for (int q = 0; q < kTile / 4; ++q) acc[q] = (q < nq && a[q] != ~0u) ? acc[q] + u[q] : acc[q];
     }
 #pragma unroll
     for (int q = 0; q < kTile / 4; ++q)
-        if (q < nq) P[(int64_t)(jbase + 4 * q) * f + i] = acc[q];
+        if (q < nq) arena[dst.at(i, jbase + 4 * q)] = acc[q];
 }
 
 // Zero-fill + extend-add of small fronts (f <= FR, FR = 8, 16 or 32) by lane groups, as
@@ -719,7 +866,8 @@ __global__ __launch_bounds__(256) void k_assemble_pull(const int4 *__restrict__ 
 template <typename T, int FR>
 __global__ __launch_bounds__(256) void k_assemble_small(const SmallRec *__restrict__ recs, int count,
                                                         T *__restrict__ arena, const SmallChild *__restrict__ child,
-                                                        const int *__restrict__ inv_map) {
+                                                        const int *__restrict__ inv_map,
+                                                        const uint32_t *__restrict__ inv_col) {
     constexpr int GW = 64 / FR;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wave_first = (blockIdx.x * 4 + wv) * GW;
@@ -736,25 +884,25 @@ __global__ __launch_bounds__(256) void k_assemble_small(const SmallRec *__restri
     for (int c = rc.c0; c < rc.c1; ++c) {  // children in order: each sum keeps the child order
         // the children's inverse row maps follow each other, f entries each
         const SmallChild ch = child[c];
-        const int *inv = inv_map + rc.inv0 + (int64_t)(c - rc.c0) * f;
-        const int b = inv[i];
+        const int64_t m0 = rc.inv0 + (int64_t)(c - rc.c0) * f;
+        const int b = inv_map[m0 + i];
         if (b < 0) continue;
-        const int fc = ch.f, wc = ch.w;
-        const T *U = arena + ch.off + wc + b;
-        // unconditional loads (indices clamped to the lane's row), masked afterwards
-        int a[FR];
+        // unconditional loads (indices clamped to the lane's row), masked afterwards; the
+        // child's contribution entry (b, a), b >= a, is U[inv_col entry of a + b]
+        const T *U = arena + ch.coff;
+        uint32_t a[FR];
 #pragma unroll
-        for (int j = 0; j < FR; ++j) a[j] = inv[min(j, i)];
+        for (int j = 0; j < FR; ++j) a[j] = inv_col[m0 + min(j, i)];
         T u[FR];
 #pragma unroll
-        for (int j = 0; j < FR; ++j) u[j] = U[(int64_t)(wc + max(a[j], 0)) * fc];
+        for (int j = 0; j < FR; ++j) u[j] = U[(a[j] == ~0u ? 0u : a[j]) + (uint32_t)b];
 #pragma unroll
-        for (int j = 0; j < FR; ++j) acc[j] = (j <= i && a[j] >= 0) ? acc[j] + u[j] : acc[j];
+        for (int j = 0; j < FR; ++j) acc[j] = (j <= i && a[j] != ~0u) ? acc[j] + u[j] : acc[j];
     }
-    T *P = arena + rc.off;
+    const FrontCols dst{rc.off, rc.coff, f, rc.w};  // f <= 32: one contribution column block
 #pragma unroll
     for (int j = 0; j < FR; ++j)
-        if (j <= i) P[(int64_t)j * f + i] = acc[j];
+        if (j <= i) arena[dst.col_small(j) + i] = acc[j];
 }
 
 // Original matrix entries of the own columns (positions are unique).
@@ -1198,8 +1346,9 @@ __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ 
     const int f = tk.f;
     const SchurStep sp = schur_step(f, tk.w, q);
     const int kb = sp.kb, ke = sp.ke;
-    const int i0 = sp.base + tk.ti * kTile, j0 = sp.base + tk.tj * kTile;
-    T *M = arena + tk.off;
+    const int i0 = tk.i0, j0 = tk.j0;
+    const T *M = arena + tk.off;  // the step's K columns (pivot columns, factor store)
+    const FrontCols fr{tk.off, tk.coff, f, tk.w};  // the target tile's columns
     const double *Dp = D + tk.fc;
     // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
     // pass (k = cq + CPI j).  The vector start is clamped into the front (reads past a
@@ -1258,10 +1407,10 @@ __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ 
         // last chunk (uniform; kb < ke always): the target tile's values go into the chunk
         // registers, which are free now, and load during the chunk's products
         if (kc + kPanel < ke) fetch(kc + kPanel);
-        else schur_load_tile<T>(*reinterpret_cast<T(*)[16]>(&ra[0][0]), M, f, i0, j0);
+        else schur_load_tile<T>(*reinterpret_cast<T(*)[16]>(&ra[0][0]), arena, fr, i0, j0);
         schur_acc_tile<T>(As, Bs, acc);
     }
-    schur_store_tile<T>(acc, *reinterpret_cast<const T(*)[16]>(&ra[0][0]), M, f, i0, j0, sp.cend);
+    schur_store_tile<T>(acc, *reinterpret_cast<const T(*)[16]>(&ra[0][0]), arena, fr, i0, j0, sp.cend);
 }
 
 // The same update with 128x128 output tiles (large fronts).  K goes through two LDS buffers
@@ -1281,8 +1430,9 @@ __global__ __launch_bounds__(256, 2) void k_schur128(const SchurTask *__restrict
     const int f = tk.f;
     const SchurStep sp = schur_step(f, tk.w, q);
     const int kb = sp.kb, ke = sp.ke;
-    const int i0 = sp.base + tk.ti * 128, j0 = sp.base + tk.tj * 128;
-    T *M = arena + tk.off;
+    const int i0 = tk.i0, j0 = tk.j0;
+    const T *M = arena + tk.off;  // the step's K columns (pivot columns, factor store)
+    const FrontCols fr{tk.off, tk.coff, f, tk.w};  // the target tile's columns
     const double *Dp = D + tk.fc;
     for (int t = threadIdx.x; t < kOuter; t += 256) Ds[t] = t < ke - kb ? (T)Dp[kb + t] : T(0);
     // loader mapping: 16-byte vectors of VEC rows, TPC threads per column, CPI columns per
@@ -1352,7 +1502,7 @@ __global__ __launch_bounds__(256, 2) void k_schur128(const SchurTask *__restrict
 #pragma unroll
         for (int j = 0; j < NJ; ++j) fetch(kb + KC, j);
     } else {
-        schur128_load_old<T, NA, 0>(old, M, f, i0, j0);
+        schur128_load_old<T, NA, 0>(old, arena, fr, i0, j0);
     }
     __syncthreads();
     for (; kc + 2 * KC < ke; kc += KC, buf ^= 1) {
@@ -1361,20 +1511,20 @@ __global__ __launch_bounds__(256, 2) void k_schur128(const SchurTask *__restrict
     }
     if (kc + KC < ke) {
         chunk(kc, buf, std::true_type{}, std::false_type{});
-        schur128_load_old<T, NA, 0>(old, M, f, i0, j0);
+        schur128_load_old<T, NA, 0>(old, arena, fr, i0, j0);
         __syncthreads();
         kc += KC;
         buf ^= 1;
     }
     chunk(kc, buf, std::false_type{}, std::false_type{});
-    schur128_store_old<T, NA, 0>(acc, old, M, f, i0, j0, sp.cend);
+    schur128_store_old<T, NA, 0>(acc, old, arena, fr, i0, j0, sp.cend);
     if constexpr (NA == 1) {
-        schur128_load_old<T, NA, 1>(old, M, f, i0, j0);
-        schur128_store_old<T, NA, 1>(acc, old, M, f, i0, j0, sp.cend);
-        schur128_load_old<T, NA, 2>(old, M, f, i0, j0);
-        schur128_store_old<T, NA, 2>(acc, old, M, f, i0, j0, sp.cend);
-        schur128_load_old<T, NA, 3>(old, M, f, i0, j0);
-        schur128_store_old<T, NA, 3>(acc, old, M, f, i0, j0, sp.cend);
+        schur128_load_old<T, NA, 1>(old, arena, fr, i0, j0);
+        schur128_store_old<T, NA, 1>(acc, old, arena, fr, i0, j0, sp.cend);
+        schur128_load_old<T, NA, 2>(old, arena, fr, i0, j0);
+        schur128_store_old<T, NA, 2>(acc, old, arena, fr, i0, j0, sp.cend);
+        schur128_load_old<T, NA, 3>(old, arena, fr, i0, j0);
+        schur128_store_old<T, NA, 3>(acc, old, arena, fr, i0, j0, sp.cend);
     }
 }
 
@@ -1585,6 +1735,7 @@ template <typename T, int FR>
 __global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ nodes, int count, T *__restrict__ arena,
                                                       const int *__restrict__ F, const int *__restrict__ W,
                                                       const int64_t *__restrict__ off,
+                                                      const int64_t *__restrict__ coff,
                                                       const int *__restrict__ firstcol, const double *__restrict__ scale,
                                                       const int *__restrict__ front_sys, double tol, double regf,
                                                       int allow_reg, double *__restrict__ D,
@@ -1602,13 +1753,15 @@ __global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ no
     const int g = nodes[live ? idx : count - 1];
     const int f = live ? F[g] : 0, w = live ? W[g] : 0;
     T *M = arena + off[g];
+    const FrontCols fr{off[g], coff[g], f, w};  // f <= 32: one contribution column block
     const double sc = scale[front_sys[g]];
     const double thr = tol * sc, regv = regf * sc;
-    // row i of the lower triangle, A[i][j] = M[j f + i] for j <= i (clamped loads, masked)
+    // row i of the lower triangle, A[i][j] = arena[fr.col(j) + i] for j <= i (clamped loads,
+    // masked)
     const int last = max(f - 1, 0), ri = min(i, last);
     T a[FR], li[FR];
 #pragma unroll
-    for (int j = 0; j < FR; ++j) a[j] = M[(int64_t)min(j, ri) * f + ri];
+    for (int j = 0; j < FR; ++j) a[j] = arena[fr.col_small(min(j, ri)) + ri];
 #pragma unroll
     for (int j = 0; j < FR; ++j) pin(a[j]);  // all loads issued before the masks
 #pragma unroll
@@ -1672,7 +1825,7 @@ __global__ __launch_bounds__(256) void k_factor_small(const int *__restrict__ no
         }
 #pragma unroll
         for (int j = 0; j < FR; ++j)
-            if (j >= w && j <= i) M[(int64_t)j * f + i] = a[j];  // contribution block
+            if (j >= w && j <= i) arena[fr.col_small(j) + i] = a[j];  // contribution block
     }
 }
 
@@ -2762,7 +2915,8 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
         if (np > 0) {
             ProfScope ps_(st, KP_PULL, 0.0, e * P.lvl_pull_elems[L] + 4.0 * P.lvl_pull_index[L]);
             hipLaunchKernelGGL(k_assemble_pull<T>, dim3(np), dim3(256), 0, st, P.pull_tasks.p + P.lvl_pull_ptr[L], arena,
-                               P.f.p, P.w.p, P.off.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p);
+                               P.f.p, P.w.p, P.off.p, P.coff.p, P.child_ptr.p, P.children.p, P.inv_off.p, P.inv_map.p,
+                               P.inv_col.p);
         }
         for (int c = 0; c < 3; ++c) {  // fronts of at most 8 / 16 / 32 rows
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
@@ -2771,7 +2925,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             const int per_block = 4 * (8 >> c);
             auto kern = c == 0 ? k_assemble_small<T, 8> : (c == 1 ? k_assemble_small<T, 16> : k_assemble_small<T, 32>);
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st,
-                               P.small_recs.p + m0, m1 - m0, arena, P.child_recs.p, P.inv_map.p);
+                               P.small_recs.p + m0, m1 - m0, arena, P.child_recs.p, P.inv_map.p, P.inv_col.p);
         }
         const int64_t na = P.lvl_asm_ptr[L + 1] - P.lvl_asm_ptr[L];
         if (na > 0) {
@@ -2786,7 +2940,7 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             const int per_block = 4 * (8 >> c);  // 4 waves of 64 / FR fronts
             auto kern = c == 0 ? k_factor_small<T, 8> : (c == 1 ? k_factor_small<T, 16> : k_factor_small<T, 32>);
             hipLaunchKernelGGL(kern, dim3((m1 - m0 + per_block - 1) / per_block), dim3(256), 0, st, P.small_nodes.p + m0,
-                               m1 - m0, arena, P.f.p, P.w.p, P.off.p, P.first.p, N.scale.p, P.front_sys.p,
+                               m1 - m0, arena, P.f.p, P.w.p, P.off.p, P.coff.p, P.first.p, N.scale.p, P.front_sys.p,
                                opts.pivot_tolerance, opts.diagonal_regularization, opts.allow_diagonal_regularization,
                                N.diag.p, N.flags.p);
         }
@@ -2901,7 +3055,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
             N->sym = &S;
             N->precision = precision;
             N->n = S.n;
-            N->arena.alloc((size_t)(S.plan->arena + kArenaPad) * esz);
+            N->arena.alloc((size_t)S.plan->arena * esz);
             N->diag.alloc(std::max(S.n, 1));
             N->flags.alloc(std::max(S.n, 1));
             N->counts.alloc(4);
@@ -3418,7 +3572,7 @@ void numeric_front(const Numeric *N, int s, double *out) {
     if (s < 0 || s >= P.nsuper) throw Error(MG_E_RANGE, "supernode index is out of range.");
     const int g = P.front_of[s];
     const int fg = P.h_f[g];
-    const int64_t cnt = (int64_t)fg * fg;
+    const int64_t cnt = (int64_t)fg * P.h_w[g];  // the pivot columns (factor store)
     std::vector<double> front(cnt);
     if (N->precision == MG_FP32) {
         std::vector<float> tmp(cnt);

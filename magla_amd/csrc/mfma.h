@@ -1,7 +1,7 @@
 // mfma.h -- 64x64 Schur-update tile on CDNA4 matrix cores.
 //
 // C[i0+r, j0+c] -= sum_k As[k][r] * Bs[k][c] for the lower triangle
-// (i0+r >= j0+c) of a column-major front with leading dimension f; K is
+// (i0+r >= j0+c) of a front whose columns front.h places; K is
 // accumulated in registers over LDS chunks (schur_acc_tile), then stored once.
 //
 // Four waves; wave v owns the 32x32 quadrant (rows 32*(v&1), cols 32*(v>>1))
@@ -15,6 +15,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "front.h"
 
 namespace mg {
 
@@ -98,16 +100,20 @@ __device__ inline void schur_acc_tile(const T (*As)[kSchurLd<T>], const T (*Bs)[
     }
 }
 
-// M[j*f + i] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
+// F[i][j] -= C[i][j] for the tile's lower entries with i < f and j < jend, where
 // acc[a][b][q] = C[row rq+16a+l16][col cq+16b+MfmaAcc<T>::row(lk,q)].  schur_load_tile issues
 // the 16 reads of the old values together (before the last chunk's products, so that their
 // latency overlaps the matrix cores' work); schur_store_tile subtracts on every lane (which
 // keeps the reads unconditional, where they were issued) and writes the tile's entries.
 template <typename T>
-__device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ M, int f, int i0, int j0) {
+__device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ arena, const FrontCols &fr, int i0,
+                                       int j0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = lane >> 4;
+    const int f = fr.f;
+    const ColRun cr = fr.run(j0, j0 + cq, 32);
+    const int64_t fb = fr.diag(j0);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -116,17 +122,20 @@ __device__ inline void schur_load_tile(T (&old)[16], const T *__restrict__ M, in
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                const int ic = min(i, f - 1), jc = min(j, ic);  // clamped: unconditional load
-                old[(2 * a + b) * 4 + q] = M[(int64_t)jc * f + ic];
+                const int ic = min(i, f - 1);  // unconditional load: entries not stored read fb
+                old[(2 * a + b) * 4 + q] = arena[(j < cr.jend && j <= ic) ? cr.at(ic, j) : fb];
             }
 }
 
 template <typename T>
 __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2][2], const T (&old)[16],
-                                        T *__restrict__ M, int f, int i0, int j0, int jend) {
+                                        T *__restrict__ arena, const FrontCols &fr, int i0, int j0, int jend) {
+    const int f = fr.f;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = lane >> 4;
+    const ColRun cr = fr.run(j0, j0 + cq, 32);
+    jend = min(jend, cr.jend);
     T v[16];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -142,7 +151,7 @@ __device__ inline void schur_store_tile(const typename MfmaAcc<T>::type (&acc)[2
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + rq + 16 * a + l16;
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = v[(2 * a + b) * 4 + q];
+                if (i < f && j <= i && j < jend) arena[cr.at(i, j)] = v[(2 * a + b) * 4 + q];
             }
 }
 
@@ -186,14 +195,18 @@ __device__ inline void schur128_step(const T (&colv)[4], const T (&rowv)[4], typ
         for (int b = 0; b < 4; ++b) acc[a][b] = MfmaAcc<T>::step(colv[b], rowv[a], acc[a][b]);
 }
 
-// The old values of the wave's outputs in row blocks [a0, a0 + NA): reads clamped into the
-// front (unconditional), issued together.  old[(a - a0) * 16 + b * 4 + q] holds output
+// The old values of the wave's outputs in row blocks [a0, a0 + NA): unconditional reads (an
+// entry that is not stored reads the tile's first diagonal entry), issued together.  old[(a - a0) * 16 + b * 4 + q] holds output
 // (row rq+16a+l16, col cq+16b+MfmaAcc<T>::row(lk,q)).
 template <typename T, int NA, int a0>
-__device__ inline void schur128_load_old(T (&old)[NA * 16], const T *__restrict__ M, int f, int i0, int j0) {
+__device__ inline void schur128_load_old(T (&old)[NA * 16], const T *__restrict__ arena, const FrontCols &fr, int i0,
+                                         int j0) {
+    const int f = fr.f;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
     const int l16 = lane & 15, lk = lane >> 4;
+    const ColRun cr = fr.run(j0, j0 + cq, 64);
+    const int64_t fb = fr.diag(j0);
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int ic = min(i0 + rq + 16 * (a0 + a) + l16, f - 1);
@@ -202,19 +215,22 @@ __device__ inline void schur128_load_old(T (&old)[NA * 16], const T *__restrict_
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                old[a * 16 + b * 4 + q] = M[(int64_t)min(j, ic) * f + ic];
+                old[a * 16 + b * 4 + q] = arena[(j < cr.jend && j <= ic) ? cr.at(ic, j) : fb];
             }
     }
 }
 
-// M[j*f + i] = old - C[i][j] on the lower entries with i < f, j < jend of row blocks
+// F[i][j] = old - C[i][j] on the lower entries with i < f, j < jend of row blocks
 // [a0, a0 + NA); the differences are formed on every lane (the reads stay where issued).
 template <typename T, int NA, int a0>
 __device__ inline void schur128_store_old(const typename MfmaAcc<T>::type (&acc)[4][4], const T (&old)[NA * 16],
-                                          T *__restrict__ M, int f, int i0, int j0, int jend) {
+                                          T *__restrict__ arena, const FrontCols &fr, int i0, int j0, int jend) {
+    const int f = fr.f;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rq = (wave & 1) * 64, cq = (wave >> 1) * 64;
     const int l16 = lane & 15, lk = lane >> 4;
+    const ColRun cr = fr.run(j0, j0 + cq, 64);
+    jend = min(jend, cr.jend);
     T v[NA * 16];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
@@ -230,7 +246,7 @@ __device__ inline void schur128_store_old(const typename MfmaAcc<T>::type (&acc)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int j = j0 + cq + 16 * b + MfmaAcc<T>::row(lk, q);
-                if (i < f && j <= i && j < jend) M[(int64_t)j * f + i] = v[a * 16 + b * 4 + q];
+                if (i < f && j <= i && j < jend) arena[cr.at(i, j)] = v[a * 16 + b * 4 + q];
             }
     }
 }

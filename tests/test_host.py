@@ -291,3 +291,81 @@ def test_symbolic_equals_oracle_all_batched_systems():
         mats.append(pm)
     batch = M.UtduBatch(mats)
     assert batch.count == 1024 and batch.info()["size"] == sum(m.size for m in mats)
+
+
+def _contrib_slots(t):
+    """Slots of a t x t contribution block in the pool: (r, c) with r >= 64 floor(c / 64), counted one by one."""
+    return sum(t - 64 * (c // 64) for c in range(t))
+
+
+@pytest.mark.parametrize("f,w", [(1, 1), (2, 1), (5, 5), (70, 3), (64 + 3, 3), (133, 5), (200, 70), (300, 300),
+                                 (257, 1), (64 * 3 + 1, 64), (130, 2)])
+def test_front_entries_map_one_to_one_into_their_regions(f, w):
+    """Every lower entry (i >= j) of a front gets its own arena slot; pivot columns are the f x w
+    column-major factor block at off, the trailing block fills [coff, coff + slots) with no gap."""
+    off, coff = 1000, 10 ** 7
+    i, j = np.tril_indices(f)
+    addr = M.front_addr(f, w, off, coff, i, j)
+    assert len(np.unique(addr)) == len(addr)
+    fac = j < w
+    np.testing.assert_array_equal(addr[fac], off + j[fac].astype(np.int64) * f + i[fac])
+    t = f - w
+    con = addr[~fac]
+    assert len(con) == t * (t + 1) // 2
+    if t > 0:
+        assert con.min() == coff and con.max() == coff + _contrib_slots(t) - 1
+        # a column's entries are consecutive rows
+        for c in (0, t // 2, t - 1):
+            col = addr[(j == w + c)]
+            np.testing.assert_array_equal(np.diff(col), 1)
+
+
+LAYOUT_CASES = [G.diffusion_3d_lognormal(12, 3), G.elasticity_3d(5, 1), G.laplacian_2d(40, nine_point=True),
+                random_matrix(300, 0.02, 5)]
+
+
+@pytest.mark.parametrize("idx", range(len(LAYOUT_CASES)))
+def test_plan_layout_regions_never_overlap_while_live(idx):
+    """Factor blocks tile the factor store; a contribution block lives from its front's level to
+    its parent's (both included), and blocks live at the same level never share pool slots."""
+    n, r, c, v = LAYOUT_CASES[idx]
+    if idx == 3:  # two disconnected copies: a forest with two roots
+        n, r, c, v = 2 * n, np.concatenate([r, r + n]), np.concatenate([c, c + n]), np.concatenate([v, v])
+    sym = M.SymmetricUtduSolver().analyze(M.SymmetricCSRMatrix.from_coo(n, r, c, v))
+    st = M.plan_stats(sym)
+    lay = M.plan_layout(sym)
+    f, w, lev, par, off, coff = (lay[:, k] for k in range(6))
+    assert st["factor_elems"] == int((f * w).sum())
+    order = np.argsort(off)
+    assert off[order[0]] == 0
+    np.testing.assert_array_equal(off[order[1:]], (off + f * w)[order[:-1]])
+    assert (lev[par >= 0] < lev[par[par >= 0]]).all()
+    assert (par >= 0).sum() < len(par)  # roots exist
+    t = f - w
+    slots = np.array([_contrib_slots(int(x)) for x in t])
+    pool0 = st["factor_elems"]
+    end = np.where(par >= 0, lev[np.maximum(par, 0)], lev)
+    assert (coff[slots > 0] >= pool0).all()
+    assert ((coff + slots)[slots > 0] <= st["arena_elems"]).all()
+    assert (coff + slots).max() - coff[slots > 0].min() <= st["pool_elems"]
+    peak = 0
+    for L in range(st["levels"]):
+        live = np.nonzero((slots > 0) & (lev <= L) & (end >= L))[0]
+        iv = sorted(zip(coff[live], coff[live] + slots[live]))
+        for (a0, a1), (b0, b1) in zip(iv, iv[1:]):
+            assert a1 <= b0, (L, a0, a1, b0, b1)
+        peak = max(peak, int(slots[live].sum()))
+    assert peak <= st["pool_live_elems"] <= st["pool_elems"]
+
+
+def test_plan_layout_c3_within_the_reference_memory_estimate():
+    """3D elasticity 40^3: factor store plus contribution pool in FP64 stay within 1.5x the
+    reference's estimate (factor entries + peak contribution entries)."""
+    n, r, c, v, clamped, b = G.config_c3()
+    a = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
+    a.line_cross(np.zeros(n), 0.0, clamped)
+    sym = M.SymmetricUtduSolver().analyze(a)
+    st = M.plan_stats(sym)
+    inf = sym.info()
+    ref = inf["estimated_factor_bytes"] + inf["peak_contribution_bytes"]
+    assert 8 * st["arena_elems"] <= 1.5 * ref, (8 * st["arena_elems"], ref)
