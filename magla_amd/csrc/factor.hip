@@ -24,6 +24,7 @@
 // (diagonal block in LDS + row-block triangular solve), and the Schur update
 // F22 -= L21 D L21^T on the lower triangle (MFMA tiles).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
@@ -163,7 +164,7 @@ struct DevicePlan {
     int nsys = 1;                   // independent systems merged into this plan (batched solves)
     std::vector<int> h_sys_begin;   // first row of each system, nsys+1
     DevArray<int> front_sys, sys_begin;
-    DevArray<int> row_sys;  // batches: system of every row (elimination order = input order per system)
+    DevArray<int> row_sys;  // system of every row (elimination order = input order per system)
     DevArray<int> f, w, first, child_ptr, children, pat_ptr, pat_rows, level_nodes, inv_map, perm, iperm;
     // per inverse-map entry a (>= 0): the child's contribution column a as an offset from its
     // pool offset, contrib_col(0, t, a) (below 2^32: t (t + 1) / 2 <= INT32_MAX); ~0u where a < 0
@@ -666,8 +667,8 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     if (!upload) return P;
     P->front_sys.upload(fsys);
     P->sys_begin.upload(P->h_sys_begin);
-    if (P->nsys > 1) {
-        std::vector<int> rsys(n);
+    {
+        std::vector<int> rsys(std::max(n, 1), 0);
         for (int t = 0; t < P->nsys; ++t)
             for (int i = P->h_sys_begin[t]; i < P->h_sys_begin[t + 1]; ++i) rsys[i] = t;
         P->row_sys.upload(rsys);
@@ -2480,6 +2481,9 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
                                                   const int *__restrict__ row_sys, const int *__restrict__ cols) {
     __shared__ double sr[kResCols][256 / 64], sb[kResCols][256 / 64];
     constexpr int RPB = 256 / LANES;  // rows per block pass
+    // one system with every column converged: nothing to recompute (k_residual_final keeps
+    // the previous norms)
+    if (kNorms && cols && cols[0] <= 0) return;
     const int sub = threadIdx.x % LANES;
     for (int c0 = 0; c0 < nrhs; c0 += kResCols) {
         const int nc = min(kResCols, nrhs - c0);
@@ -2552,11 +2556,13 @@ __global__ __launch_bounds__(256) void k_residual(const int *__restrict__ rp, co
     }
 }
 
-// relres[c] = sqrt(sum R^2) / sqrt(sum B^2) per column (one workgroup per column).
+// relres[c] = sqrt(sum R^2) / sqrt(sum B^2) per column (one workgroup per column); a column
+// past cols[0] (left out of the last solve: converged) keeps its previous value.
 __global__ __launch_bounds__(256) void k_residual_final(const double *__restrict__ part, int nblocks,
-                                                        double *__restrict__ relres) {
+                                                        double *__restrict__ relres, const int *__restrict__ cols) {
     __shared__ double sr[256], sb[256];
     const int c = blockIdx.x;
+    if (cols && c >= cols[0]) return;
     double rr = 0.0, bb = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += 256) {
         rr += part[((int64_t)c * nblocks + b) * 2 + 0];
@@ -2586,10 +2592,7 @@ __global__ __launch_bounds__(256) void k_residual_seg(const double *__restrict__
                                                       double *__restrict__ relres, const int *__restrict__ cols) {
     __shared__ double sr[256], sb[256];
     const int t = blockIdx.x, c = blockIdx.y;
-    if (cols && c >= cols[t]) {  // column left alone: the caller keeps its previous value
-        if (threadIdx.x == 0) relres[(int64_t)t * nrhs + c] = -1.0;
-        return;
-    }
+    if (cols && c >= cols[t]) return;  // column left alone: it keeps its previous value
     double rr = 0.0, bb = 0.0;
     for (int i = sys_begin[t] + threadIdx.x; i < sys_begin[t + 1]; i += 256) {
         const double r = R[(int64_t)c * n + i], b = B[(int64_t)c * ldb + i];
@@ -2655,6 +2658,39 @@ __global__ __launch_bounds__(1024) void k_sys_cols(const double *__restrict__ B,
         __syncthreads();  // every thread has read top before the next step clears it
     }
     if (threadIdx.x == 0) cols[t] = 0;
+}
+
+// Start of a solve with refinement: relres = 0 (columns no solve touches are zero), the
+// round flags cleared, and for one system every column active.
+__global__ __launch_bounds__(256) void k_refine_init(double *__restrict__ rel, int nrel, int *__restrict__ act,
+                                                     int nact, int *__restrict__ cols, int single_nrhs) {
+    for (int e = threadIdx.x; e < nrel; e += 256) rel[e] = 0.0;
+    for (int e = threadIdx.x; e < nact; e += 256) act[e] = 0;
+    if (single_nrhs >= 0 && threadIdx.x == 0) cols[0] = single_nrhs;
+}
+
+// Before a correction solve: cols[t] = 1 + the last column of system t whose relative
+// residual is not <= tol (NaN included), 0 when all of them are; *active = 1 when some
+// system has a column left (the round does work).
+__global__ __launch_bounds__(1024) void k_refine_cols(const double *__restrict__ rel, int nsys, int nrhs, double tol,
+                                                      int *__restrict__ cols, int *__restrict__ active) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int t = threadIdx.x; t < nsys; t += 1024) {
+        int c1 = 0;
+        for (int c = nrhs - 1; c >= 0; --c)
+            if (!(rel[(int64_t)t * nrhs + c] <= tol)) {
+                c1 = c + 1;
+                break;
+            }
+        cols[t] = c1;
+        mine |= c1 > 0;
+    }
+    if (mine) any = 1;  // every writer stores the same value
+    __syncthreads();
+    if (threadIdx.x == 0) *active = any;
 }
 
 __global__ void k_zero_cols(double *__restrict__ X, int ldx, int n, int nrhs) {
@@ -2726,10 +2762,12 @@ __global__ __launch_bounds__(256) void k_diag_scale_seg(const int *__restrict__ 
 // Device workspace of one solve call: every buffer the solve and refinement kernels use
 // besides the factor, carved out of one block (sub-buffers 256-byte aligned).  The kernels
 // never rely on its contents at entry.
+constexpr int kActSlots = 64;  // refinement rounds whose "did work" flags a workspace holds
 struct SolveWs {
     void *Y = nullptr, *V = nullptr, *Z = nullptr, *part = nullptr;  // factor precision
     double *R = nullptr, *Part = nullptr, *Rel = nullptr;
     int *Cols = nullptr;  // per system: leading right-hand sides a solve still needs
+    int *Act = nullptr;   // per refinement round (mod kActSlots): 1 when the round did work
 };
 // Bytes of the block for nrhs right-hand sides, and the view of a block at base.
 size_t solve_ws_bytes(const Numeric &N, int nrhs);
@@ -2742,6 +2780,19 @@ struct WsSlot {
     hipEvent_t done = nullptr;
     ~WsSlot() {
         if (done) (void)hipEventDestroy(done);
+    }
+};
+
+// A captured solve: the launches of solve_rounds for one set of arguments.
+struct SolveGraph {
+    const void *b = nullptr, *x = nullptr, *ws = nullptr;
+    int ldb = 0, ldx = 0, nrhs = 0, rounds = -1, check = 0;
+    double tol = 0.0;
+    hipGraphExec_t exec = nullptr;
+    bool same(const double *b_, int ldb_, const double *x_, int ldx_, int nrhs_, const void *ws_, int rounds_,
+              int check_, double tol_) const {
+        return exec && b == b_ && ldb == ldb_ && x == x_ && ldx == ldx_ && nrhs == nrhs_ && ws == ws_ &&
+               rounds == rounds_ && check == check_ && tol == tol_;
     }
 };
 
@@ -2768,6 +2819,13 @@ struct Numeric {
     std::vector<std::unique_ptr<WsSlot>> ws_free;
     hipGraphExec_t gexec = nullptr;  // captured numeric factorization (refactorizations replay it)
     hipStream_t cap_stream = nullptr;
+    // captured solves (mg_options.device_graph of the factorization): one graph for the last
+    // set of solve arguments; the refinement rounds a solve runs before its first host read
+    // are the number the previous solve of this handle needed
+    bool solve_graph = false;
+    std::mutex sg_mu;
+    SolveGraph sgraph;
+    std::atomic<int> refine_hint{2};
     int graph_precision = -1;
     // pivot options baked into the captured launches; a refactorization with other values
     // captures again
@@ -2784,6 +2842,7 @@ struct Numeric {
     }
     ~Numeric() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (sgraph.exec) (void)hipGraphExecDestroy(sgraph.exec);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         for (hipEvent_t ev : {ev_fork, ev_side[0], ev_side[1]})
             if (ev) (void)hipEventDestroy(ev);
@@ -3074,6 +3133,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                 MG_HIP(hipMemcpyAsync(N->values.p, d_values, sizeof(double) * S.matrix_nnz, hipMemcpyDeviceToDevice, st));
             N->A.set_values_dev(N->values.p, st);
             const bool graph = opts_use_graph(opts) && !profile_on();
+            N->solve_graph = opts_use_graph(opts);
             const bool same_opts = N->graph_tol == opts.pivot_tolerance && N->graph_reg == opts.diagonal_regularization &&
                                    N->graph_allow == opts.allow_diagonal_regularization;
             if (graph && N->gexec && N->graph_precision == precision && same_opts) {
@@ -3330,7 +3390,7 @@ size_t solve_ws_bytes(const Numeric &N, int nrhs) {
     return al(n * k * esz) + al((size_t)std::max<int64_t>(N.plan->vsum, 1) * k * esz) + al(n * k * esz) +
            al((size_t)std::max(N.plan->max_foot_tasks, 1) * kPanel * k * esz) + al(n * k * sizeof(double)) +
            al((size_t)residual_blocks((int)n, N.A.lanes) * k * 2 * sizeof(double)) + al(k * nsys * sizeof(double)) +
-           al(nsys * sizeof(int));
+           al(nsys * sizeof(int)) + al(kActSlots * sizeof(int));
 }
 
 SolveWs solve_ws_view(const Numeric &N, int nrhs, void *base) {
@@ -3352,6 +3412,7 @@ SolveWs solve_ws_view(const Numeric &N, int nrhs, void *base) {
     w.Part = static_cast<double *>(take((size_t)residual_blocks((int)n, N.A.lanes) * k * 2 * sizeof(double)));
     w.Rel = static_cast<double *>(take(k * nsys * sizeof(double)));
     w.Cols = static_cast<int *>(take(nsys * sizeof(int)));
+    w.Act = static_cast<int *>(take(kActSlots * sizeof(int)));
     return w;
 }
 
@@ -3402,9 +3463,11 @@ struct WsLease {
     ~WsLease() { give_back(true); }
 };
 
-// relres[c] = ||B - A X||/||B|| per column; R receives the residuals.
-static void residual(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, const double *d_x,
-                     int ldx, hipStream_t st, std::vector<double> &relres, const int *cols = nullptr) {
+// ws.Rel = per-column ||B - A X|| / ||B|| (per system and column in a batch); ws.R receives the
+// residuals.  Columns past cols[t] (left out of the last solve) keep their previous values.
+// Device work only: no host synchronization.
+static void residual_dev(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb,
+                         const double *d_x, int ldx, hipStream_t st, const int *cols) {
     const int n = N.n;
     // one system: 8 right-hand sides per pass over A with the norm partials; a batch takes its
     // per-system norms from R afterwards, kResColsMax columns per pass, each row only up to its
@@ -3413,13 +3476,14 @@ static void residual(const Numeric &N, const SolveWs &ws, int nrhs, const double
     const int lanes = seg && N.A.lanes == 4 ? 2 : N.A.lanes;
     const int nb = residual_blocks(n, lanes);
     const Matrix &A = N.A;
+    const int *rsys = N.plan->row_sys.p;
 #define MG_RES(LN)                                                                                                  \
     if (seg)                                                                                                        \
         hipLaunchKernelGGL((k_residual<LN, kResColsMax, false>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, \
-                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, N.plan->row_sys.p, cols);          \
+                           A.vals.p, n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, rsys, cols);                       \
     else                                                                                                            \
         hipLaunchKernelGGL((k_residual<LN, 8, true>), dim3(nb), dim3(256), 0, st, A.rowptr.p, A.colidx.p, A.vals.p, \
-                           n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, nullptr, nullptr)
+                           n, nrhs, d_b, ldb, d_x, ldx, ws.R, ws.Part, rsys, cols)
     switch (lanes) {
         case 2: MG_RES(2); break;
         case 4: MG_RES(4); break;
@@ -3434,64 +3498,119 @@ static void residual(const Numeric &N, const SolveWs &ws, int nrhs, const double
         hipLaunchKernelGGL(k_residual_seg, dim3(nsys, nrhs), dim3(256), 0, st, ws.R, n, d_b, ldb, N.plan->sys_begin.p,
                            nrhs, ws.Rel, cols);
     else
-        hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, ws.Part, nb, ws.Rel);
-    std::vector<double> prev;
-    prev.swap(relres);
-    relres.resize((size_t)nrhs * nsys);
-    MG_HIP(hipMemcpyAsync(relres.data(), ws.Rel, sizeof(double) * relres.size(), hipMemcpyDeviceToHost, st));
-    MG_HIP(hipStreamSynchronize(st));
-    // columns left alone keep their previous value (zero before any: those columns are zero)
-    for (size_t e = 0; e < relres.size(); ++e)
-        if (relres[e] < 0.0) relres[e] = e < prev.size() ? prev[e] : 0.0;
+        hipLaunchKernelGGL(k_residual_final, dim3(nrhs), dim3(256), 0, st, ws.Part, nb, ws.Rel, cols);
 }
 
-// UtduNumericFactorization.Solve (:90-112) for nrhs columns, followed by FP64
-// iterative refinement: x += solve(b - A x) until max relres <= tol.  Reads the handle only;
-// every buffer it writes besides x is in ws.
-static void solve_with_ws(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, double *d_x,
-                          int ldx, double tol, int max_refine, hipStream_t st, mg_solve_report *rep) {
-    auto once = [&](const double *rhs, int ldr, int acc, const int *cols) {
-        if (N.precision == MG_FP32) run_solve_once<float>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
-        else run_solve_once<double>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, st, cols);
-    };
-    const int nsys = N.plan->nsys;
+// One refinement round k: the columns still above tol (per system) are found on the device
+// (ws.Act[k mod kActSlots] = 1 when there are any), the residual columns past them are zeroed,
+// x += solve(r) on those columns, and their residuals and norms are recomputed.  A round after
+// convergence leaves everything as it is (every kernel finds no column to work on).
+template <typename Once>
+static void refine_round(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb,
+                         const double *d_x, int ldx, double tol, int k, hipStream_t st, Once &&once) {
+    const DevicePlan &P = *N.plan;
+    hipLaunchKernelGGL(k_refine_cols, dim3(1), dim3(1024), 0, st, ws.Rel, P.nsys, nrhs, tol, ws.Cols,
+                       ws.Act + k % kActSlots);
+    hipLaunchKernelGGL(k_zero_past, dim3(P.nsys, nrhs), dim3(256), 0, st, ws.R, N.n, P.sys_begin.p, ws.Cols);
+    once(ws.R, N.n, 1, ws.Cols, st);
+    residual_dev(N, ws, nrhs, d_b, ldb, d_x, ldx, st, ws.Cols);
+}
+
+// The first solve, its residual and the first `rounds` refinement rounds, with no host
+// synchronization: what a captured solve graph holds.
+template <typename Once>
+static void solve_rounds(const Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, double *d_x,
+                         int ldx, double tol, bool check, int rounds, hipStream_t st, Once &&once) {
+    const DevicePlan &P = *N.plan;
+    const int nsys = P.nsys;
+    hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(256), 0, st, ws.Rel, nsys * nrhs, ws.Act, kActSlots, ws.Cols,
+                       nsys > 1 ? -1 : nrhs);
     // in a batch, every system's solves stop at its last nonzero right-hand side (uneven
     // counts padded to nrhs columns): the columns past it stay zero
     if (nsys > 1)
-        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(1024), 0, st, d_b, ldb, nrhs, N.plan->sys_begin.p, ws.Cols);
-    once(d_b, ldb, 0, nsys > 1 ? ws.Cols : nullptr);
-    if (!rep && max_refine <= 0) return;
-    std::vector<double> rel;
-    residual(N, ws, nrhs, d_b, ldb, d_x, ldx, st, rel, nsys > 1 ? ws.Cols : nullptr);
+        hipLaunchKernelGGL(k_sys_cols, dim3(nsys), dim3(1024), 0, st, d_b, ldb, nrhs, P.sys_begin.p, ws.Cols);
+    once(d_b, ldb, 0, ws.Cols, st);
+    if (!check) return;
+    residual_dev(N, ws, nrhs, d_b, ldb, d_x, ldx, st, ws.Cols);
+    for (int k = 0; k < rounds; ++k) refine_round(N, ws, nrhs, d_b, ldb, d_x, ldx, tol, k, st, once);
+}
+
+
+// UtduNumericFactorization.Solve (:90-112) for nrhs columns, followed by FP64 iterative
+// refinement: x += solve(b - A x) until every column's relres <= tol.  The convergence test is
+// made on the device: the first solve and the number of rounds the previous solve of this
+// handle needed run without host synchronization (replayed from a captured graph when the
+// factorization was captured, mg_options.device_graph); the host reads the norms once, and
+// further rounds (each followed by a read) run only if some column is still above tol.  Reads
+// the handle only (besides its graph cache); every buffer it writes besides x is in ws.
+static void solve_with_ws(Numeric &N, const SolveWs &ws, int nrhs, const double *d_b, int ldb, double *d_x,
+                          int ldx, double tol, int max_refine, hipStream_t st, mg_solve_report *rep) {
+    auto once = [&](const double *rhs, int ldr, int acc, const int *cols, hipStream_t s) {
+        if (N.precision == MG_FP32) run_solve_once<float>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, s, cols);
+        else run_solve_once<double>(N, ws, nrhs, rhs, ldr, d_x, ldx, acc, s, cols);
+    };
+    const int nsys = N.plan->nsys;
+    const bool check = rep || max_refine > 0;
+    const int rounds = check ? std::min({std::max(max_refine, 0), N.refine_hint.load(), kActSlots}) : 0;
+    bool ran = false;
+    if (N.solve_graph && !profile_on()) {
+        std::unique_lock<std::mutex> lk(N.sg_mu, std::try_to_lock);  // a concurrent solve runs eagerly
+        if (lk.owns_lock()) {
+            SolveGraph &G = N.sgraph;
+            if (!G.same(d_b, ldb, d_x, ldx, nrhs, ws.Y, rounds, check, tol)) {
+                if (G.exec) (void)hipGraphExecDestroy(G.exec);
+                G = SolveGraph{};
+                if (!N.cap_stream) MG_HIP(hipStreamCreateWithFlags(&N.cap_stream, hipStreamNonBlocking));
+                hipStream_t cs = N.cap_stream;
+                MG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                try {
+                    solve_rounds(N, ws, nrhs, d_b, ldb, d_x, ldx, tol, check, rounds, cs, once);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(cs, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hipGraph_t g = nullptr;
+                MG_HIP(hipStreamEndCapture(cs, &g));
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                MG_HIP(ie);
+                G = SolveGraph{d_b, d_x, ws.Y, ldb, ldx, nrhs, rounds, check, tol, ex};
+            }
+            MG_HIP(hipGraphLaunch(G.exec, st));
+            ran = true;
+        }
+    }
+    if (!ran) solve_rounds(N, ws, nrhs, d_b, ldb, d_x, ldx, tol, check, rounds, st, once);
+    if (!check) return;
+    std::vector<double> rel((size_t)nrhs * nsys);
+    int act[kActSlots];
+    auto read_back = [&] {
+        MG_HIP(hipMemcpyAsync(rel.data(), ws.Rel, sizeof(double) * rel.size(), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipMemcpyAsync(act, ws.Act, sizeof(act), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipStreamSynchronize(st));
+    };
     auto worst_of = [](const std::vector<double> &v) {
         double m = 0.0;
         for (double x : v) m = (std::isnan(x) || std::isnan(m)) ? NAN : std::max(m, x);
         return m;
     };
-    double worst = worst_of(rel);
+    read_back();
+    // rounds after the one that found every column converged do no work
     int steps = 0;
-    std::vector<int> h_cols;
-    while (steps < max_refine && !(worst <= tol)) {
-        // in a batch, a correction solve covers each system's columns up to its last
-        // unconverged one (none for a converged system): the residual of the columns left
-        // out is zeroed, so their correction is zero
-        const int *cols = nullptr;
-        if (nsys > 1) {
-            h_cols.assign(nsys, 0);
-            for (int t = 0; t < nsys; ++t)
-                for (int c = 0; c < nrhs; ++c)
-                    if (!(rel[(size_t)t * nrhs + c] <= tol)) h_cols[t] = c + 1;
-            // h_cols is next changed after the stream synchronization of the next residual
-            MG_HIP(hipMemcpyAsync(ws.Cols, h_cols.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_zero_past, dim3(nsys, nrhs), dim3(256), 0, st, ws.R, N.n, N.plan->sys_begin.p,
-                               ws.Cols);
-            cols = ws.Cols;
-        }
-        once(ws.R, N.n, 1, cols);
+    while (steps < rounds && act[steps]) ++steps;
+    double worst = worst_of(rel);
+    const bool more = steps == rounds;  // every round so far did work
+    while (more && steps < max_refine && !(worst <= tol)) {
+        refine_round(N, ws, nrhs, d_b, ldb, d_x, ldx, tol, steps, st, once);
+        read_back();
+        if (!act[steps % kActSlots]) break;
         ++steps;
-        residual(N, ws, nrhs, d_b, ldb, d_x, ldx, st, rel, cols);
         worst = worst_of(rel);
     }
+    N.refine_hint.store(steps);
     if (rep) {
         rep->refinement_steps = steps;
         rep->relative_residual = worst;
