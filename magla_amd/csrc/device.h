@@ -172,6 +172,11 @@ struct MpGraph {
     // entry's value in the factor values (-1: padding)
     int nslices = 0;
     DevArray<int> gs_ptr, gs_col, gs_src, gts_ptr, gts_row, gts_src;
+    // when every sliced entry's column (row, for G') is within 32767 of its slot's row: those
+    // differences as 16 bits, so that the fused PCG streams one 32-bit word per entry (BF16
+    // value in the high half, the difference in the low half)
+    bool packed = false;
+    DevArray<uint16_t> gs_dl, gts_dl;
 };
 
 struct Mpnn {
@@ -185,6 +190,7 @@ struct Mpnn {
     DevArray<uint16_t> hb0, hb1;
     DevArray<float> gv, gvt;        // factor values for the solves (from the BF16 network), also transposed order
     DevArray<float> gvs, gvts;      // the same in the sliced ELL orders of G and G' (when built)
+    DevArray<uint32_t> gps, gpts;   // packed sliced entries (BF16 value | 16-bit index difference), when packed
     const MpGraph *gv_graph = nullptr;  // graph the factor values belong to
     DevArray<double> y, z, az, u, dpart, scal;
 };

@@ -492,6 +492,17 @@ __global__ void k_gather_sell(const int *__restrict__ src, int64_t count, const 
     if (e < count) out[e] = src[e] >= 0 ? gv[src[e]] : 0.0f;
 }
 
+// Packed sliced entries for the fused PCG: the value rounded to BF16 (nearest, ties to even) in
+// the high 16 bits, the index difference from the slot's row in the low 16 bits.
+__global__ void k_pack_sell(const int *__restrict__ src, const uint16_t *__restrict__ dl, int64_t count,
+                            const float *__restrict__ gv, uint32_t *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const uint32_t u = src[e] >= 0 ? __float_as_uint(gv[src[e]]) : 0u;
+    const uint32_t hi = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    out[e] = hi | (uint32_t)dl[e];
+}
+
 // y = G v on the sliced ELL copy of G: wave = 64-row slice, lane = row, the row's entries
 // accumulated in order in float64
 __global__ __launch_bounds__(256) void k_g_apply_sell(const int *__restrict__ sp, const int *__restrict__ scol,
@@ -584,6 +595,27 @@ __device__ __forceinline__ double sell_row_dot(const int *__restrict__ idx, cons
     return acc;
 }
 
+// The same over packed entries (k_pack_sell): one 32-bit load per entry gives the value and
+// the index, row + the signed low half.
+__device__ __forceinline__ double sell_row_dot_packed(const uint32_t *__restrict__ pk, int base, int w, int lane,
+                                                      int row, const double *__restrict__ x) {
+    double acc = 0.0;
+    for (int k0 = 0; k0 < w; k0 += 8) {
+        uint32_t u[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t v = pk[base + 64 * min(k0 + q, w - 1) + lane];
+            u[q] = k0 + q < w ? v : (v & 0xffffu);  // past the row's width: value 0, same index
+        }
+        double xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) xv[q] = x[row + (int)(int16_t)(u[q] & 0xffffu)];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += (double)__uint_as_float(u[q] & 0xffff0000u) * xv[q];
+    }
+    return acc;
+}
+
 // Residual-norm test of the fused flexible PCG's iteration `it` (the reference's stopping rule,
 // ConjugateGradientGaussPreSolver.cs:151-153, and the stall fallback), formed identically by
 // every workgroup of the preconditioner kernels from the step's partials and the previous
@@ -615,12 +647,16 @@ __device__ inline FcgNorm fcg_norm_decide(const double *__restrict__ prr, int ns
     return d;
 }
 
-// y = G r for iteration `it` of the fused flexible PCG (skipped once converged or fallen back)
+// y = G r for iteration `it` of the fused flexible PCG (skipped once converged or fallen back);
+// PK: G's entries packed (BF16 value, 16-bit index difference), else FP32 values and int
+// indices
+template <bool PK>
 __global__ __launch_bounds__(256) void k_fcg_g_sell(const int *__restrict__ sp, const int *__restrict__ scol,
-                                                    const float *__restrict__ sv, int n, int nslices,
-                                                    const double *__restrict__ r, double *__restrict__ y,
-                                                    const double *__restrict__ prr, int nstep, int it, int stall,
-                                                    const FcgState *__restrict__ pv, const PcgScalars *__restrict__ S) {
+                                                    const float *__restrict__ sv, const uint32_t *__restrict__ pk,
+                                                    int n, int nslices, const double *__restrict__ r,
+                                                    double *__restrict__ y, const double *__restrict__ prr, int nstep,
+                                                    int it, int stall, const FcgState *__restrict__ pv,
+                                                    const PcgScalars *__restrict__ S) {
     __shared__ double sm4[4];
     if (S->done) return;
     const FcgNorm d = fcg_norm_decide(prr, nstep, it, stall, S->thr, pv, sm4);
@@ -628,14 +664,16 @@ __global__ __launch_bounds__(256) void k_fcg_g_sell(const int *__restrict__ sp, 
     const int lane = threadIdx.x & 63, slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (slice >= nslices) return;
     const int base = sp[slice], w = (sp[slice + 1] - base) >> 6, row = 64 * slice + lane;
-    const double acc = sell_row_dot(scol, sv, base, w, lane, r);
+    const double acc = PK ? sell_row_dot_packed(pk, base, w, lane, row, r) : sell_row_dot(scol, sv, base, w, lane, r);
     if (row < n) y[row] = acc;
 }
 
 // z = G' y (Jacobi z = D^{-1} r once fallen back) for iteration `it`, with block partials of
 // r'z and z'Ap; workgroup 0 records the norm test in slot it & 1 (and the result scalars)
+template <bool PK>
 __global__ __launch_bounds__(256) void k_fcg_gt_sell(const int *__restrict__ sp, const int *__restrict__ srow,
-                                                     const float *__restrict__ sv, int n, int nslices,
+                                                     const float *__restrict__ sv, const uint32_t *__restrict__ pk,
+                                                     int n, int nslices,
                                                      const double *__restrict__ y, double *__restrict__ z,
                                                      const double *__restrict__ r, const double *__restrict__ ap,
                                                      const double *__restrict__ invd, const double *__restrict__ prr,
@@ -670,7 +708,7 @@ __global__ __launch_bounds__(256) void k_fcg_gt_sell(const int *__restrict__ sp,
             if (j < n) acc = invd[j] * r[j];
         } else {
             const int base = sp[slice], w = (sp[slice + 1] - base) >> 6;
-            acc = sell_row_dot(srow, sv, base, w, lane, y);
+            acc = PK ? sell_row_dot_packed(pk, base, w, lane, j, y) : sell_row_dot(srow, sv, base, w, lane, y);
         }
         if (j < n) {
             z[j] = acc;
@@ -1037,8 +1075,9 @@ void mp_graph_prepare(Matrix &A, hipStream_t st) {
     {
         // sliced ELL of G (entry e's value at gv[e]) and of G' (value at gv[gtidx[t]])
         const int nsl = (n + 63) / 64;
+        bool fits = true;  // every index difference fits 16 bits
         auto sell = [&](const std::vector<int> &ptr, const std::vector<int> &idx, const int *vmap,
-                        DevArray<int> &dptr, DevArray<int> &didx, DevArray<int> &dsrc) {
+                        DevArray<int> &dptr, DevArray<int> &didx, DevArray<int> &dsrc, DevArray<uint16_t> &ddl) {
             std::vector<int> sp(nsl + 1, 0);
             for (int sl = 0; sl < nsl; ++sl) {
                 int w = 0;
@@ -1047,6 +1086,7 @@ void mp_graph_prepare(Matrix &A, hipStream_t st) {
             }
             if ((double)sp[nsl] > 1.5 * (double)std::max<int64_t>(ng, 1)) return false;
             std::vector<int> si(std::max(sp[nsl], 1)), ss(std::max(sp[nsl], 1), -1);
+            std::vector<uint16_t> dl(std::max(sp[nsl], 1), 0);
             for (int sl = 0; sl < nsl; ++sl) {
                 const int w = (sp[sl + 1] - sp[sl]) / 64;
                 for (int l = 0; l < 64; ++l) {
@@ -1056,17 +1096,23 @@ void mp_graph_prepare(Matrix &A, hipStream_t st) {
                         const int e = sp[sl] + 64 * k + l;
                         si[e] = k < len ? idx[ptr[r] + k] : r;
                         ss[e] = k < len ? (vmap ? vmap[ptr[r] + k] : ptr[r] + k) : -1;
+                        const int d = si[e] - (64 * sl + l);  // from the slot's row (past n too)
+                        fits = fits && d >= -32768 && d <= 32767;
+                        dl[e] = (uint16_t)(int16_t)(fits ? d : 0);
                     }
                 }
             }
             dptr.upload(sp);
             didx.upload(si);
             dsrc.upload(ss);
+            ddl.upload(dl);
             return true;
         };
-        if (n > 0 && sell(gptr, gcol, nullptr, g->gs_ptr, g->gs_col, g->gs_src) &&
-            sell(gtptr, gtrow, gtidx.data(), g->gts_ptr, g->gts_row, g->gts_src))
+        if (n > 0 && sell(gptr, gcol, nullptr, g->gs_ptr, g->gs_col, g->gs_src, g->gs_dl) &&
+            sell(gtptr, gtrow, gtidx.data(), g->gts_ptr, g->gts_row, g->gts_src, g->gts_dl)) {
             g->nslices = nsl;
+            g->packed = fits;
+        }
     }
     g->n = n;
     g->ng = ng;
@@ -1244,6 +1290,14 @@ void mpnn_factor_bf16(Mpnn &net, Matrix &A, hipStream_t st) {
                            (int64_t)g.gs_src.n, net.gv.p, net.gvs.p);
         hipLaunchKernelGGL(k_gather_sell, dim3(nblk((int64_t)g.gts_src.n, 256)), dim3(256), 0, st, g.gts_src.p,
                            (int64_t)g.gts_src.n, net.gv.p, net.gvts.p);
+        if (g.packed) {
+            if (net.gps.n != g.gs_src.n) net.gps.alloc(g.gs_src.n);
+            if (net.gpts.n != g.gts_src.n) net.gpts.alloc(g.gts_src.n);
+            hipLaunchKernelGGL(k_pack_sell, dim3(nblk((int64_t)g.gs_src.n, 256)), dim3(256), 0, st, g.gs_src.p,
+                               g.gs_dl.p, (int64_t)g.gs_src.n, net.gv.p, net.gps.p);
+            hipLaunchKernelGGL(k_pack_sell, dim3(nblk((int64_t)g.gts_src.n, 256)), dim3(256), 0, st, g.gts_src.p,
+                               g.gts_dl.p, (int64_t)g.gts_src.n, net.gv.p, net.gpts.p);
+        }
     }
     MG_HIP(hipGetLastError());
     net.gv_graph = &g;
@@ -1276,11 +1330,13 @@ bool mpnn_fcg_apply(Mpnn &net, Matrix &A, const double *d_r, double *d_z, const 
     const MpGraph &g = *A.mp;
     if (!g.nslices || net.gv_graph != &g || net.gvs.n != g.gs_src.n) return false;
     const int grid = mpnn_fcg_partials(A);
-    hipLaunchKernelGGL(k_fcg_g_sell, dim3(grid), dim3(256), 0, st, g.gs_ptr.p, g.gs_col.p, net.gvs.p, A.n, g.nslices,
-                       d_r, net.y.p, prr, nstep, it, stall, fs_prev, S);
-    hipLaunchKernelGGL(k_fcg_gt_sell, dim3(grid), dim3(256), 0, st, g.gts_ptr.p, g.gts_row.p, net.gvts.p, A.n,
-                       g.nslices, net.y.p, d_z, d_r, d_ap, A.invdiag.p, prr, nstep, it, stall, fs_prev, fs_cur, S,
-                       rz_part, zap_part);
+    const bool pk = g.packed && net.gps.n == g.gs_src.n && net.gpts.n == g.gts_src.n;
+    hipLaunchKernelGGL(pk ? k_fcg_g_sell<true> : k_fcg_g_sell<false>, dim3(grid), dim3(256), 0, st, g.gs_ptr.p,
+                       g.gs_col.p, net.gvs.p, net.gps.p, A.n, g.nslices, d_r, net.y.p, prr, nstep, it, stall, fs_prev,
+                       S);
+    hipLaunchKernelGGL(pk ? k_fcg_gt_sell<true> : k_fcg_gt_sell<false>, dim3(grid), dim3(256), 0, st, g.gts_ptr.p,
+                       g.gts_row.p, net.gvts.p, net.gpts.p, A.n, g.nslices, net.y.p, d_z, d_r, d_ap, A.invdiag.p, prr,
+                       nstep, it, stall, fs_prev, fs_cur, S, rz_part, zap_part);
     return true;
 }
 
