@@ -44,6 +44,7 @@ constexpr int kTile = 64;   // tile edge of assembly / Schur tiles
 constexpr int kPanel = 64;  // panel width: columns factored by one panel_diag / panel_rows pass
 constexpr int kBlock = 256; // Schur blocking: panels of one block update the rest of its outer block at once
 constexpr int kOuter = 1024; // outer blocking: the trailing update after every kOuter columns
+constexpr int kSBlock = 1024; // columns of one block step of the large-front solves (diagonal blocks inverted)
 static_assert(kOuter % kBlock == 0 && kBlock % kPanel == 0, "nested blocking");
 constexpr int kMaxFrontWidth = 4096;  // widest chain fusion
 // Schur tile shape of the outer steps per front-size bucket (f < 128, < 256, < 512, < 1024,
@@ -144,13 +145,13 @@ struct DevicePlan {
     std::vector<int> lvl_foot_ptr;  // solve: (front, 64-column block, row chunk) footer partial products
     std::vector<int> lvl_footr_ptr; // solve: (front, 64-column block) footer reductions
 
-    // large-front solves per 256-column block J of a level: forward y of the block's
+    // large-front solves per kSBlock-column block J of a level: forward y of the block's
     // sub-blocks (fa) and the rows below it (fb), backward x of the sub-blocks (ba) and the
     // 64-column blocks to its left (bb); task ranges [ptr[J], ptr[J + 1]) of the task arrays
     std::vector<std::vector<int>> lvl_fa_ptr, lvl_fb_ptr, lvl_ba_ptr, lvl_bb_ptr;
-    // 256-column diagonal block inverses (end of the factorization): (front, block, a, b) tasks
-    // of distance d = a - b in [inv_ptr[d - 1], inv_ptr[d])
-    int inv_ptr[4] = {0, 0, 0, 0};
+    // solve-block (kSBlock columns) diagonal inverses (end of the factorization): (front, block,
+    // a, b) tasks of distance d = a - b in [inv_ptr[d - 1], inv_ptr[d])
+    int inv_ptr[kSBlock / kPanel] = {};
     DevArray<int4> inv_tasks;
     int64_t vsum = 0;               // sum f over fronts (solve front vectors)
     // algorithmic work per launch
@@ -437,7 +438,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->lvl_footr_ptr.push_back(0);
     std::vector<int2> fa, fb, ba, bb;
     std::vector<int4> foot, footr;
-    std::vector<int4> inv_d[3];
+    std::vector<int4> inv_d[kSBlock / kPanel - 1];
 
     std::vector<int64_t> asm_pos;
     std::vector<int> asm_vidx;
@@ -497,15 +498,15 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
         P->max_foot_tasks = std::max(P->max_foot_tasks, (int)foot.size() - foot_base);
         // per block J: tasks [ptr[2J], ptr[2J + 1]) are those of blocks of at most 64 columns
         // (one tile in registers), [ptr[2J + 1], ptr[2J + 2]) those of wider blocks
-        // (kernels with NS = 1 and NS = kSub)
+        // (kernels with NS = 1 and NS = kSSub)
         std::vector<int> pfa{(int)fa.size()}, pfb{(int)fb.size()}, pba{(int)ba.size()}, pbb{(int)bb.size()};
-        const int nblk = (npan + kBlock / kPanel - 1) / (kBlock / kPanel);
+        const int nblk = (npan + kSBlock / kPanel - 1) / (kSBlock / kPanel);
         for (int J = 0; J < nblk; ++J) {
             std::vector<int2> t[4][2];  // fa, fb, ba, bb x (narrow, wide)
             for (int g : bylevel[L]) {
                 const int f = P->h_f[g], w = P->h_w[g];
-                if (w <= J * kBlock || f <= kSmallFront) continue;
-                const int bs = J * kBlock, be = std::min(bs + kBlock, w);
+                if (w <= J * kSBlock || f <= kSmallFront) continue;
+                const int bs = J * kSBlock, be = std::min(bs + kSBlock, w);
                 const int nsub = (be - bs + kPanel - 1) / kPanel;
                 const int wide = nsub > 1;
                 for (int a = 0; a < nsub; ++a) {
@@ -715,7 +716,7 @@ static std::shared_ptr<DevicePlan> build_plan_host(const Symbolic &S, bool uploa
     P->bb_tasks.upload(pack(bb));
     {
         std::vector<int4> all;
-        for (int d = 0; d < 3; ++d) {
+        for (int d = 0; d < kSBlock / kPanel - 1; ++d) {
             all.insert(all.end(), inv_d[d].begin(), inv_d[d].end());
             P->inv_ptr[d + 1] = (int)all.size();
         }
@@ -2141,26 +2142,26 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
     }
 }
 
-// ---- 256-column block solves of the large fronts ------------------------------
+// ---- block solves of the large fronts --------------------------------------------
 //
-// At the end of the factorization the unit-lower diagonal block L_JJ of every 256-column
+// At the end of the factorization the unit-lower diagonal block L_JJ of every kSBlock-column
 // block J of a front (sub-blocks of 64 columns, the last one possibly narrower) is inverted:
 // X = L_JJ^{-1} with X_aa = Li_a (the sub-blocks' stored inverses) and, for a > b,
 // X_ab = -Li_a sum_{c=b}^{a-1} L_ac X_cb.  X[k][m] (k > m) is kept at front row m, column k,
 // the transposed layout of the 64-column inverses, which are its diagonal blocks: the
 // sub-diagonal blocks X_ab fill the block's otherwise unused upper triangle.  One launch per
 // distance d = a - b: X_ab reads only blocks of smaller distance, written by earlier launches.
-// The solves then take one dependent step per 256 columns instead of per 64.
+// The solves then take one dependent step per kSBlock columns instead of per 64.
 template <typename T>
-__global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, T *__restrict__ arena,
-                                                const int *__restrict__ F, const int *__restrict__ W,
-                                                const int64_t *__restrict__ off) {
+__global__ __launch_bounds__(256) void k_inv_blk(const int4 *__restrict__ tasks, T *__restrict__ arena,
+                                                 const int *__restrict__ F, const int *__restrict__ W,
+                                                 const int64_t *__restrict__ off) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];
     __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];
     const int4 tk = tasks[blockIdx.x];
     const int g = tk.x, a = tk.z, b = tk.w;
     const int f = F[g], w = W[g];
-    const int bs = tk.y * kBlock, be = min(bs + kBlock, w);
+    const int bs = tk.y * kSBlock, be = min(bs + kSBlock, w);
     T *M = arena + off[g];
     const int pa = bs + kPanel * a, pb = bs + kPanel * b;
     const int na = min(kPanel, be - pa), nbb = min(kPanel, be - pb);
@@ -2168,16 +2169,21 @@ __global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, 
     typename MfmaAcc<T>::type acc[2][2];
     schur_acc_zero<T>(acc);
     // S = sum_{c=b}^{a-1} L_ac X_cb: As[k][r] = L_ac[r][k] = M[(pc + k) f + pa + r],
-    // Bs[k][m] = X_cb[k][m] = M[(pc + k) f + pb + m] (c = b: Li_b, unit diagonal, zero above)
-    for (int c = b; c < a; ++c) {
+    // Bs[k][m] = X_cb[k][m] = M[(pc + k) f + pb + m] (c = b: Li_b, unit diagonal, zero above);
+    // the next c's loads are in flight during this one's products
+    T av[kPanel / 4], bv[kPanel / 4];
+    auto load = [&](int c) {
         const int pc = bs + kPanel * c, nc = min(kPanel, be - pc);
-        T av[kPanel / 4], bv[kPanel / 4];
 #pragma unroll
         for (int j = 0; j < kPanel / 4; ++j) {
             const T *col = M + (int64_t)(pc + min(q4 + 4 * j, nc - 1)) * f;
             av[j] = col[pa + min(l, na - 1)];
             bv[j] = col[pb + min(l, nbb - 1)];
         }
+    };
+    load(b);
+    for (int c = b; c < a; ++c) {
+        const int pc = bs + kPanel * c, nc = min(kPanel, be - pc);
         __syncthreads();  // the previous product's reads are done
 #pragma unroll
         for (int j = 0; j < kPanel / 4; ++j) {
@@ -2188,6 +2194,7 @@ __global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, 
             Bs[k][l] = (k < nc && l < nbb) ? x : T(0);
         }
         __syncthreads();
+        if (c + 1 < a) load(c + 1);
         schur_acc_tile<T>(As, Bs, acc);
     }
     // X_ab = -Li_a S: As[k][r] = Li_a[r][k] = M[(pa + r) f + pa + k] (k < r; unit diagonal),
@@ -2223,13 +2230,14 @@ __global__ __launch_bounds__(256) void k_inv256(const int4 *__restrict__ tasks, 
             }
 }
 
-// The block solves issue every global load of a task before the first one is consumed (up to
-// four 64x64 tiles and their vector blocks per lane, clamped addresses, masks applied at the
-// LDS stores): a task then costs one memory round trip instead of one per 64 columns.
-// NS = the 64x64 tiles a task holds in registers: 1 for the tasks of blocks of at most 64
-// columns, kSub otherwise; the two kinds go to separate launches so that the many narrow fronts
-// of the lower levels keep a high occupancy.
-constexpr int kSub = kBlock / kPanel;  // 64-column sub-blocks of a 256-column block
+// The block solves issue the global loads of a group of up to four 64x64 tiles (and their
+// vector blocks) together, clamped, with the masks applied at the LDS stores, and the next
+// group's loads while the current group runs on the matrix cores: a block of kSBlock columns
+// costs about one memory round trip per group.  NS = 1 for the tasks of blocks of at most 64
+// columns, kSSub otherwise; the two kinds go to separate launches so that the many narrow
+// fronts of the lower levels keep a high occupancy.
+constexpr int kSSub = kSBlock / kPanel;  // 64-column sub-blocks of a solve block
+constexpr int kSChunk = 256;             // most columns (rows) of a solve block staged per pass in the updates
 
 // Forward, step 1 of block J: y of sub-block a, y[pa + k] = sum_{m <= k} X[pa + k][m] v[m] over
 // the block's sub-blocks c <= a, on the matrix cores as y^T = v^T X^T (right-hand sides as the
@@ -2239,42 +2247,60 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
                                                   const T *__restrict__ V, T *__restrict__ Y, int J, int n, int nrhs,
                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    constexpr int G = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // tiles per load group
     __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pa + k][pc + m]
     __shared__ T vv[kPanel][kRhsChunk + 1];  // vv[m][rr] = v[pc + m][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
     if (nc == 0) return;
     const int a = tk.blk, f = tk.f, w = tk.w;
-    const int bs = J * kBlock, be = min(bs + kBlock, w);
+    const int bs = J * kSBlock, be = min(bs + kSBlock, w);
     const int pa = bs + kPanel * a, na = min(kPanel, be - pa);
     const T *M = arena + tk.off;
     const T *Vg = V + tk.vbase * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
+    const int ngrp = NS == 1 ? 1 : a / G + 1;  // groups holding sub-blocks 0 .. a
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        T ti[NS][kPanel / 4], tv[NS][4];
+        T ti0[G][kPanel / 4], tv0[G][4], ti1[G][kPanel / 4], tv1[G][4];
+        auto load = [&](int grp, T (&ti)[G][kPanel / 4], T (&tv)[G][4]) {
 #pragma unroll
-        for (int c = 0; c < NS; ++c)
-            if (c <= a) {  // uniform
-                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-                tile_load<T>(ti[c], M, f, pc, pa, ncc, na);
-                vec_load<T>(tv[c], Vg + (int64_t)c0 * f + pc, f, ncc, nr);
+            for (int e = 0; e < G; ++e) {
+                const int c = grp * G + e;
+                if (c <= a) {  // uniform
+                    const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+                    tile_load<T>(ti[e], M, f, pc, pa, ncc, na);
+                    vec_load<T>(tv[e], Vg + (int64_t)c0 * f + pc, f, ncc, nr);
+                }
             }
+        };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        auto use = [&](int grp, const T (&ti)[G][kPanel / 4], const T (&tv)[G][4]) {
 #pragma unroll
-        for (int c = 0; c < NS; ++c) {
-            if (c > a) break;
-            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-            __syncthreads();
-            if (c == a) tile_store_inv<T>(inv, ti[c], na);
-            else tile_store<T>(inv, ti[c], ncc, na);
-            vec_store<T>(vv, tv[c], ncc, nr);
-            __syncthreads();
+            for (int e = 0; e < G; ++e) {
+                const int c = grp * G + e;
+                if (c > a) break;  // uniform
+                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+                __syncthreads();
+                if (c == a) tile_store_inv<T>(inv, ti[e], na);
+                else tile_store<T>(inv, ti[e], ncc, na);
+                vec_store<T>(vv, tv[e], ncc, nr);
+                __syncthreads();
 #pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
+                for (int s = 0; s < kPanel / 4; ++s)
+                    acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
+            }
+        };
+        load(0, ti0, tv0);
+        for (int grp = 0; grp < ngrp; grp += 2) {
+            if (grp + 1 < ngrp) load(grp + 1, ti1, tv1);
+            use(grp, ti0, tv0);
+            if (grp + 1 >= ngrp) break;
+            if (grp + 2 < ngrp) load(grp + 2, ti0, tv0);
+            use(grp + 1, ti1, tv1);
         }
         // acc[q] = y[pa + col][c0 + rr], rr = row(lk, q)
         if (col < na)
@@ -2287,12 +2313,13 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
     }
 }
 
-// yv[k][rr] = V[c0 + rr][base + k] (k < nk, rr < nr; zero elsewhere) for k < 256, from a
+// yv[k][rr] = V[c0 + rr][base + k] (k < nk, rr < nr; zero elsewhere) for k < kSChunk, from a
 // column-major n-row vector block: thread k loads its 16 values together.
 template <typename T>
 __device__ __forceinline__ void block_vec_lds(T (*yv)[kRhsChunk + 1], const T *__restrict__ src, int64_t ld, int nk,
                                               int nr) {
-    const int k = threadIdx.x;  // blockDim = kBlock
+    static_assert(kSChunk == 256, "a thread per staged column");
+    const int k = threadIdx.x;
     const T *p = src + min(k, nk - 1);
     T v[kRhsChunk];
 #pragma unroll
@@ -2305,47 +2332,66 @@ __device__ __forceinline__ void block_vec_lds(T (*yv)[kRhsChunk + 1], const T *_
 
 // Forward, step 2 of block J: the 64 rows r0 + i below the block, V[r0 + i] -= sum_k
 // L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (the L
-// operand read straight from the front, 16 rows of a column per load).
+// operand read straight from the front, 16 rows of a column per load), 256 (FP64: 128)
+// columns per pass with the next pass's L values in flight.
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  T *__restrict__ V, const T *__restrict__ Y, int J, int n, int nrhs,
                                                  const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16 && kBlock == 256, "one 16-row MFMA tile of right-hand sides; a thread per column");
-    __shared__ T yv[kBlock][kRhsChunk + 1];  // yv[k][rr] = y[bs + k][c0 + rr]
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
+    constexpr int CS = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // 64-column tiles per pass
+    constexpr int PW = CS * kPanel;                               // columns per pass (<= kSChunk)
+    __shared__ T yv[kSChunk][kRhsChunk + 1];  // yv[k][rr] = y[bs + kc + k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int f = tk.f, w = tk.w;
-    const int bs = J * kBlock, be = min(bs + kBlock, w), nk = be - bs;
+    const int bs = J * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
     const int r0 = be + tk.blk * kTile, nrow = min(kTile, f - r0);
     const T *M = arena + tk.off;
     T *Vg = V + tk.vbase * nrhs;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;  // this lane's row of the block
     const T *lrow = M + r0 + min(col, nrow - 1);
-    // the block's L rows, all 256 columns (clamped; columns past nk meet y = 0)
-    T bl[NS][kPanel / 4];
+    const int npass = NS == 1 ? 1 : (nk + PW - 1) / PW;
+    // the block's L rows of one pass (clamped; columns past nk meet y = 0)
+    T bl0[CS][kPanel / 4], bl1[CS][kPanel / 4];
+    auto load = [&](int ps, T (&bl)[CS][kPanel / 4]) {
 #pragma unroll
-    for (int c = 0; c < NS; ++c)
+        for (int c = 0; c < CS; ++c)
 #pragma unroll
-        for (int s = 0; s < kPanel / 4; ++s) bl[c][s] = lrow[(int64_t)(bs + min(kPanel * c + 4 * s + lk, nk - 1)) * f];
+            for (int s = 0; s < kPanel / 4; ++s)
+                bl[c][s] = lrow[(int64_t)(bs + min(ps * PW + kPanel * c + 4 * s + lk, nk - 1)) * f];
+    };
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         T old[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(col, nrow - 1)];
-        block_vec_lds<T>(yv, Y + (int64_t)c0 * n + tk.fc + bs, n, nk, nr);
-        __syncthreads();
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        auto use = [&](int ps, const T (&bl)[CS][kPanel / 4]) {
+            const int kc = ps * PW, nkc = min(PW, nk - kc);
+            __syncthreads();  // the previous pass's reads of yv are done
+            block_vec_lds<T>(yv, Y + (int64_t)c0 * n + tk.fc + bs + kc, n, nkc, nr);
+            __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NS; ++c) {
-            if (kPanel * c >= nk) break;  // uniform
+            for (int c = 0; c < CS; ++c) {
+                if (kPanel * c >= nkc) break;  // uniform
 #pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s)
-                acc = MfmaAcc<T>::step(yv[kPanel * c + 4 * s + lk][l16], bl[c][s], acc);
+                for (int s = 0; s < kPanel / 4; ++s)
+                    acc = MfmaAcc<T>::step(yv[kPanel * c + 4 * s + lk][l16], bl[c][s], acc);
+            }
+        };
+        load(0, bl0);
+        for (int ps = 0; ps < npass; ps += 2) {
+            if (ps + 1 < npass) load(ps + 1, bl1);
+            use(ps, bl0);
+            if (ps + 1 >= npass) break;
+            if (ps + 2 < npass) load(ps + 2, bl0);
+            use(ps + 1, bl1);
         }
         if (col < nrow)
 #pragma unroll
@@ -2353,7 +2399,6 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
                 const int rr = MfmaAcc<T>::row(lk, q);
                 if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + col] = old[q] - acc[q];
             }
-        __syncthreads();
     }
 }
 
@@ -2365,42 +2410,60 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
                                                   T *__restrict__ Y, const T *__restrict__ Z, int K, int n, int nrhs,
                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    constexpr int G = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // tiles per load group
     __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pc + k][pb + m]
     __shared__ T zz[kPanel][kRhsChunk + 1];  // zz[k][rr] = z[pc + k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int b = tk.blk, f = tk.f, w = tk.w;
-    const int bs = K * kBlock, be = min(bs + kBlock, w);
+    const int bs = K * kSBlock, be = min(bs + kSBlock, w);
     const int pb = bs + kPanel * b, nbb = min(kPanel, be - pb);
     const int nsub = (be - bs + kPanel - 1) / kPanel;
     const T *M = arena + tk.off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
+    const int ngrp = NS == 1 ? 1 : (nsub - b + G - 1) / G;  // groups holding sub-blocks b .. nsub - 1
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        T ti[NS][kPanel / 4], tz[NS][4];
+        T ti0[G][kPanel / 4], tz0[G][4], ti1[G][kPanel / 4], tz1[G][4];
+        auto load = [&](int grp, T (&ti)[G][kPanel / 4], T (&tz)[G][4]) {
 #pragma unroll
-        for (int e = 0; e < NS; ++e)
-            if (b + e < nsub) {  // uniform
-                const int pc = bs + kPanel * (b + e), ncc = min(kPanel, be - pc);
-                tile_load<T>(ti[e], M, f, pb, pc, nbb, ncc);
-                vec_load<T>(tz[e], Z + (int64_t)c0 * n + tk.fc + pc, n, ncc, nr);
+            for (int e = 0; e < G; ++e) {
+                const int c = b + grp * G + e;
+                if (c < nsub) {  // uniform
+                    const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+                    tile_load<T>(ti[e], M, f, pb, pc, nbb, ncc);
+                    vec_load<T>(tz[e], Z + (int64_t)c0 * n + tk.fc + pc, n, ncc, nr);
+                }
             }
+        };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        auto use = [&](int grp, const T (&ti)[G][kPanel / 4], const T (&tz)[G][4]) {
 #pragma unroll
-        for (int e = 0; e < NS; ++e) {
-            if (b + e >= nsub) break;  // uniform
-            const int pc = bs + kPanel * (b + e), ncc = min(kPanel, be - pc);
-            __syncthreads();
-            if (e == 0) tile_store_inv<T>(inv, ti[e], nbb);
-            else tile_store<T>(inv, ti[e], nbb, ncc);
-            vec_store<T>(zz, tz[e], ncc, nr);
-            __syncthreads();
+            for (int e = 0; e < G; ++e) {
+                const int c = b + grp * G + e;
+                if (c >= nsub) break;  // uniform
+                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+                __syncthreads();
+                if (c == b) tile_store_inv<T>(inv, ti[e], nbb);
+                else tile_store<T>(inv, ti[e], nbb, ncc);
+                vec_store<T>(zz, tz[e], ncc, nr);
+                __syncthreads();
 #pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s) acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
+                for (int s = 0; s < kPanel / 4; ++s)
+                    acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
+            }
+        };
+        load(0, ti0, tz0);
+        for (int grp = 0; grp < ngrp; grp += 2) {
+            if (grp + 1 < ngrp) load(grp + 1, ti1, tz1);
+            use(grp, ti0, tz0);
+            if (grp + 1 >= ngrp) break;
+            if (grp + 2 < ngrp) load(grp + 2, ti0, tz0);
+            use(grp + 1, ti1, tz1);
         }
         if (col < nbb)
 #pragma unroll
@@ -2414,46 +2477,65 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
 
 // Backward, step 2 of block K: every 64-column block cb to its left, z[64 cb + j] -= sum_k
 // L[k][64 cb + j] x[k] over the block's rows k, as (z_cb)^T -= x^T L[block rows, cb] (L tiles
-// staged through LDS).
+// staged through LDS), 256 (FP64: 128) rows per pass with the next pass's tiles in flight.
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  const T *__restrict__ Y, T *__restrict__ Z, int K, int n, int nrhs,
                                                  const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16 && kBlock == 256, "one 16-row MFMA tile of right-hand sides; a thread per row");
-    __shared__ T blk[kPanel][kPanel + 1];    // blk[k][c] = L[pk + k][64 cb + c]
-    __shared__ T xv[kBlock][kRhsChunk + 1];  // xv[k][rr] = x[bs + k][c0 + rr]
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
+    constexpr int CS = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // 64-row tiles per pass
+    constexpr int PW = CS * kPanel;                               // rows per pass (<= kSChunk)
+    __shared__ T blk[kPanel][kPanel + 1];     // blk[k][c] = L[pk + k][64 cb + c]
+    __shared__ T xv[kSChunk][kRhsChunk + 1];  // xv[k][rr] = x[bs + kc + k][c0 + rr]
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int cb = tk.blk, f = tk.f, w = tk.w;
-    const int bs = K * kBlock, be = min(bs + kBlock, w), nk = be - bs;
+    const int bs = K * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
     const T *M = arena + tk.off;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
     const int col = kPanel / 4 * wv + l16;
-    T tb[NS][kPanel / 4];
+    const int npass = NS == 1 ? 1 : (nk + PW - 1) / PW;
+    T tb0[CS][kPanel / 4], tb1[CS][kPanel / 4];
+    auto load = [&](int ps, T (&tb)[CS][kPanel / 4]) {
 #pragma unroll
-    for (int c = 0; c < NS; ++c)
-        if (kPanel * c < nk)  // uniform
-            tile_load<T>(tb[c], M, f, bs + kPanel * c, cb * kPanel, min(kPanel, nk - kPanel * c), kPanel);
+        for (int c = 0; c < CS; ++c) {
+            const int kr = ps * PW + kPanel * c;
+            if (kr < nk)  // uniform
+                tile_load<T>(tb[c], M, f, bs + kr, cb * kPanel, min(kPanel, nk - kr), kPanel);
+        }
+    };
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         T old[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + cb * kPanel + col];
-        block_vec_lds<T>(xv, Y + (int64_t)c0 * n + tk.fc + bs, n, nk, nr);
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        auto use = [&](int ps, const T (&tb)[CS][kPanel / 4]) {
+            const int kc = ps * PW, nkc = min(PW, nk - kc);
+            __syncthreads();  // the previous pass's reads of xv and blk are done
+            block_vec_lds<T>(xv, Y + (int64_t)c0 * n + tk.fc + bs + kc, n, nkc, nr);
 #pragma unroll
-        for (int c = 0; c < NS; ++c) {
-            if (kPanel * c >= nk) break;
-            __syncthreads();
-            tile_store<T>(blk, tb[c], min(kPanel, nk - kPanel * c), kPanel);
-            __syncthreads();
+            for (int c = 0; c < CS; ++c) {
+                if (kPanel * c >= nkc) break;  // uniform
+                if (c > 0) __syncthreads();    // the previous tile's reads of blk are done
+                tile_store<T>(blk, tb[c], min(kPanel, nkc - kPanel * c), kPanel);
+                __syncthreads();
 #pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s)
-                acc = MfmaAcc<T>::step(xv[kPanel * c + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
+                for (int s = 0; s < kPanel / 4; ++s)
+                    acc = MfmaAcc<T>::step(xv[kPanel * c + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
+            }
+        };
+        load(0, tb0);
+        for (int ps = 0; ps < npass; ps += 2) {
+            if (ps + 1 < npass) load(ps + 1, tb1);
+            use(ps, tb0);
+            if (ps + 1 >= npass) break;
+            if (ps + 2 < npass) load(ps + 2, tb0);
+            use(ps + 1, tb1);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -3076,12 +3158,12 @@ static void run_factor(Numeric &N, const mg_options &opts, hipStream_t st) {
             prev_side = nullptr;
         }
     }
-    // inverses of the 256-column diagonal blocks for the solves, by distance of the sub-blocks
-    for (int d = 0; d < 3; ++d) {
+    // inverses of the solve blocks' diagonal blocks, by distance of the sub-blocks
+    for (int d = 0; d < kSBlock / kPanel - 1; ++d) {
         const int t0 = P.inv_ptr[d], t1 = P.inv_ptr[d + 1];
         if (t1 > t0)
-            hipLaunchKernelGGL(k_inv256<T>, dim3(t1 - t0), dim3(256), 0, st, P.inv_tasks.p + t0, arena, P.f.p, P.w.p,
-                               P.off.p);
+            hipLaunchKernelGGL(k_inv_blk<T>, dim3(t1 - t0), dim3(256), 0, st, P.inv_tasks.p + t0, arena, P.f.p,
+                               P.w.p, P.off.p);
     }
     MG_HIP(hipGetLastError());
 }
@@ -3325,13 +3407,13 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
                 hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.fa_tasks.p + pa[0],
                                    arena, V, Y, J, n, nrhs, cols);
             if (pa[2] > pa[1])
-                hipLaunchKernelGGL((k_fwd_diag<T, kSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.fa_tasks.p + pa[1],
+                hipLaunchKernelGGL((k_fwd_diag<T, kSSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.fa_tasks.p + pa[1],
                                    arena, V, Y, J, n, nrhs, cols);
             if (pu[1] > pu[0])
                 hipLaunchKernelGGL((k_fwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.fb_tasks.p + pu[0],
                                    arena, V, Y, J, n, nrhs, cols);
             if (pu[2] > pu[1])
-                hipLaunchKernelGGL((k_fwd_upd<T, kSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.fb_tasks.p + pu[1],
+                hipLaunchKernelGGL((k_fwd_upd<T, kSSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.fb_tasks.p + pu[1],
                                    arena, V, Y, J, n, nrhs, cols);
         }
     }
@@ -3358,13 +3440,13 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
                 hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.ba_tasks.p + pa[0],
                                    arena, Y, Z, K, n, nrhs, cols);
             if (pa[2] > pa[1])
-                hipLaunchKernelGGL((k_bwd_diag<T, kSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.ba_tasks.p + pa[1],
+                hipLaunchKernelGGL((k_bwd_diag<T, kSSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.ba_tasks.p + pa[1],
                                    arena, Y, Z, K, n, nrhs, cols);
             if (pu[1] > pu[0])
                 hipLaunchKernelGGL((k_bwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.bb_tasks.p + pu[0],
                                    arena, Y, Z, K, n, nrhs, cols);
             if (pu[2] > pu[1])
-                hipLaunchKernelGGL((k_bwd_upd<T, kSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.bb_tasks.p + pu[1],
+                hipLaunchKernelGGL((k_bwd_upd<T, kSSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.bb_tasks.p + pu[1],
                                    arena, Y, Z, K, n, nrhs, cols);
         }
         for (int c = 0; c < 4; ++c) {
