@@ -2230,319 +2230,270 @@ __global__ __launch_bounds__(256) void k_inv_blk(const int4 *__restrict__ tasks,
             }
 }
 
-// The block solves issue the global loads of a group of up to four 64x64 tiles (and their
-// vector blocks) together, clamped, with the masks applied at the LDS stores, and the next
-// group's loads while the current group runs on the matrix cores: a block of kSBlock columns
-// costs about one memory round trip per group.  NS = 1 for the tasks of blocks of at most 64
-// columns, kSSub otherwise; the two kinds go to separate launches so that the many narrow
-// fronts of the lower levels keep a high occupancy.
+// The block solves feed the matrix cores from registers: no LDS staging and no barriers.  A
+// task walks up to kSSub 64-deep tiles along k, with the next kSPre tiles' loads in flight.  In
+// a tile, MFMA step s = 4 i + e takes on lane group lk = lane / 16 the index k = 16 i + 4 lk + e,
+// so that a lane's four k of one i are consecutive in memory (one vector load where the operand
+// runs along k).  Every kernel sums over k in this order.  The right-hand sides are the MFMA
+// rows (16 per pass), the block's rows or columns the MFMA columns (16 per wave).
+// NS = 1 for the tasks of blocks of at most 64 columns, kSSub otherwise (separate launches).
 constexpr int kSSub = kSBlock / kPanel;  // 64-column sub-blocks of a solve block
-constexpr int kSChunk = 256;             // most columns (rows) of a solve block staged per pass in the updates
+template <typename T>
+constexpr int kSPre = sizeof(T) == 4 ? 4 : 2;  // tiles in flight per task
 
-// Forward, step 1 of block J: y of sub-block a, y[pa + k] = sum_{m <= k} X[pa + k][m] v[m] over
-// the block's sub-blocks c <= a, on the matrix cores as y^T = v^T X^T (right-hand sides as the
-// 16 MFMA rows, wave v: columns 16 v .. 16 v + 15 of the sub-block); y goes to Y.
+// v[4 i + e] = p[16 i + 4 lk + e] if that index is < nk, else 0.  Vector loads; the four values
+// past a column's end (or nk) that one of them may touch are in the same allocation (the next
+// column, or the next buffer of the workspace, or the factor store's padding) and are masked.
+template <typename T>
+__device__ __forceinline__ void kvec_load(T (&v)[16], const T *__restrict__ p, int nk) {
+    const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int kb = 16 * i + 4 * lk, kc = kb < nk ? kb : 0;
+        const T *q = p + kc;
+        T x[4];
+        if constexpr (sizeof(T) == 4) {
+            const mg_u4u u = *reinterpret_cast<const mg_u4u *>(q);
+            __builtin_memcpy(x, &u, 16);
+        } else {
+            const mg_u4u u0 = *reinterpret_cast<const mg_u4u *>(q), u1 = *reinterpret_cast<const mg_u4u *>(q + 2);
+            __builtin_memcpy(x, &u0, 16);
+            __builtin_memcpy(x + 2, &u1, 16);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[4 * i + e] = x[e];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[4 * i + e] = 16 * i + 4 * lk + e < nk ? v[4 * i + e] : T(0);
+}
+// v[4 i + e] = p[(16 i + 4 lk + e) * ld] with the index clamped to nk - 1 (values past nk are
+// used only against zero operands).
+template <typename T>
+__device__ __forceinline__ void kstride_load(T (&v)[16], const T *__restrict__ p, int64_t ld, int nk) {
+    const int lk = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[4 * i + e] = p[(int64_t)min(16 * i + 4 * lk + e, nk - 1) * ld];
+}
+template <typename T>
+__device__ __forceinline__ void tile_mfma(typename MfmaAcc<T>::type &acc, const T (&a)[16], const T (&b)[16]) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) acc = MfmaAcc<T>::step(a[s], b[s], acc);
+}
+// Tiles [t0, t1) (t0 < t1) of a task through the matrix cores with P tiles' loads in flight, in
+// named registers (no dynamic indexing): load(t, a, b) fills tile t's operands, prep(t, b)
+// adjusts b before use (the diagonal tile's mask).
+template <typename T, int P, typename Load, typename Prep>
+__device__ __forceinline__ void tile_chain(typename MfmaAcc<T>::type &acc, int t0, int t1, Load &&load, Prep &&prep) {
+    static_assert(P >= 1 && P <= 4, "up to four tiles in flight");
+    T a0[16], b0[16], a1[16], b1[16], a2[16], b2[16], a3[16], b3[16];
+    load(t0, a0, b0);
+    if constexpr (P > 1)
+        if (t0 + 1 < t1) load(t0 + 1, a1, b1);
+    if constexpr (P > 2)
+        if (t0 + 2 < t1) load(t0 + 2, a2, b2);
+    if constexpr (P > 3)
+        if (t0 + 3 < t1) load(t0 + 3, a3, b3);
+    auto step = [&](int t, T(&av)[16], T(&bv)[16]) {
+        if (t >= t1) return;  // uniform
+        prep(t, bv);
+        tile_mfma<T>(acc, av, bv);
+        if (t + P < t1) load(t + P, av, bv);
+    };
+    for (int t = t0; t < t1; t += P) {
+        step(t, a0, b0);
+        if constexpr (P > 1) step(t + 1, a1, b1);
+        if constexpr (P > 2) step(t + 2, a2, b2);
+        if constexpr (P > 3) step(t + 3, a3, b3);
+    }
+}
+
+// Forward, step 1 of block J: y of sub-block a, y[pa + j] = sum_{k <= j} X[pa + j][k] v[k] over
+// the block's sub-blocks c <= a (X[pa + j][pc + k] at front row pc + k, column pa + j: one
+// vector load along k), as y^T = v^T X^T; y goes to Y.
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   const T *__restrict__ V, T *__restrict__ Y, int J, int n, int nrhs,
                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    constexpr int G = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // tiles per load group
-    __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pa + k][pc + m]
-    __shared__ T vv[kPanel][kRhsChunk + 1];  // vv[m][rr] = v[pc + m][c0 + rr]
+    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
     if (nc == 0) return;
-    const int a = tk.blk, f = tk.f, w = tk.w;
+    const int a = NS == 1 ? 0 : tk.blk, f = tk.f, w = tk.w;
     const int bs = J * kSBlock, be = min(bs + kSBlock, w);
     const int pa = bs + kPanel * a, na = min(kPanel, be - pa);
     const T *M = arena + tk.off;
-    const T *Vg = V + tk.vbase * nrhs;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int col = kPanel / 4 * wv + l16;
-    const int ngrp = NS == 1 ? 1 : a / G + 1;  // groups holding sub-blocks 0 .. a
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int j = 16 * wv + l16;  // this lane's output row of the sub-block (MFMA column)
+    const T *xcol = M + (int64_t)(pa + min(j, na - 1)) * f;  // X[pa + j][.] along the rows of column pa + j
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        T ti0[G][kPanel / 4], tv0[G][4], ti1[G][kPanel / 4], tv1[G][4];
-        auto load = [&](int grp, T (&ti)[G][kPanel / 4], T (&tv)[G][4]) {
+        const T *vcol = V + tk.vbase * nrhs + (int64_t)(c0 + min(l16, nr - 1)) * f;  // v[.][c0 + l16]
+        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
+            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+            kvec_load<T>(av, vcol + pc, ncc);
+            kvec_load<T>(bv, xcol + pc, ncc);
+        };
+        auto prep = [&](int c, T(&bv)[16]) {
+            if (c != a) return;  // X_aa = Li_a: unit diagonal, zero above it
 #pragma unroll
-            for (int e = 0; e < G; ++e) {
-                const int c = grp * G + e;
-                if (c <= a) {  // uniform
-                    const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-                    tile_load<T>(ti[e], M, f, pc, pa, ncc, na);
-                    vec_load<T>(tv[e], Vg + (int64_t)c0 * f + pc, f, ncc, nr);
-                }
+            for (int s = 0; s < 16; ++s) {
+                const int k = 16 * (s >> 2) + 4 * lk + (s & 3);
+                bv[s] = k < j ? bv[s] : (k == j ? T(1) : T(0));
             }
         };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        auto use = [&](int grp, const T (&ti)[G][kPanel / 4], const T (&tv)[G][4]) {
-#pragma unroll
-            for (int e = 0; e < G; ++e) {
-                const int c = grp * G + e;
-                if (c > a) break;  // uniform
-                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-                __syncthreads();
-                if (c == a) tile_store_inv<T>(inv, ti[e], na);
-                else tile_store<T>(inv, ti[e], ncc, na);
-                vec_store<T>(vv, tv[e], ncc, nr);
-                __syncthreads();
-#pragma unroll
-                for (int s = 0; s < kPanel / 4; ++s)
-                    acc = MfmaAcc<T>::step(vv[4 * s + lk][l16], inv[4 * s + lk][col], acc);
-            }
-        };
-        load(0, ti0, tv0);
-        for (int grp = 0; grp < ngrp; grp += 2) {
-            if (grp + 1 < ngrp) load(grp + 1, ti1, tv1);
-            use(grp, ti0, tv0);
-            if (grp + 1 >= ngrp) break;
-            if (grp + 2 < ngrp) load(grp + 2, ti0, tv0);
-            use(grp + 1, ti1, tv1);
-        }
-        // acc[q] = y[pa + col][c0 + rr], rr = row(lk, q)
-        if (col < na)
+        tile_chain<T, P>(acc, 0, a + 1, load, prep);
+        // acc[q] = y[pa + j][c0 + rr], rr = row(lk, q)
+        if (j < na)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
-                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pa + col] = acc[q];
+                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pa + j] = acc[q];
             }
-        __syncthreads();
     }
 }
 
-// yv[k][rr] = V[c0 + rr][base + k] (k < nk, rr < nr; zero elsewhere) for k < kSChunk, from a
-// column-major n-row vector block: thread k loads its 16 values together.
-template <typename T>
-__device__ __forceinline__ void block_vec_lds(T (*yv)[kRhsChunk + 1], const T *__restrict__ src, int64_t ld, int nk,
-                                              int nr) {
-    static_assert(kSChunk == 256, "a thread per staged column");
-    const int k = threadIdx.x;
-    const T *p = src + min(k, nk - 1);
-    T v[kRhsChunk];
-#pragma unroll
-    for (int rr = 0; rr < kRhsChunk; ++rr) v[rr] = p[(int64_t)min(rr, nr - 1) * ld];
-#pragma unroll
-    for (int rr = 0; rr < kRhsChunk; ++rr) pin(v[rr]);
-#pragma unroll
-    for (int rr = 0; rr < kRhsChunk; ++rr) yv[k][rr] = (k < nk && rr < nr) ? v[rr] : T(0);
-}
-
 // Forward, step 2 of block J: the 64 rows r0 + i below the block, V[r0 + i] -= sum_k
-// L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (the L
-// operand read straight from the front, 16 rows of a column per load), 256 (FP64: 128)
-// columns per pass with the next pass's L values in flight.
+// L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (y along
+// k by vector loads, L down the block's columns).
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  T *__restrict__ V, const T *__restrict__ Y, int J, int n, int nrhs,
                                                  const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
-    constexpr int CS = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // 64-column tiles per pass
-    constexpr int PW = CS * kPanel;                               // columns per pass (<= kSChunk)
-    __shared__ T yv[kSChunk][kRhsChunk + 1];  // yv[k][rr] = y[bs + kc + k][c0 + rr]
+    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int f = tk.f, w = tk.w;
     const int bs = J * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
+    const int nt = NS == 1 ? 1 : (nk + kPanel - 1) / kPanel;
     const int r0 = be + tk.blk * kTile, nrow = min(kTile, f - r0);
     const T *M = arena + tk.off;
     T *Vg = V + tk.vbase * nrhs;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int col = kPanel / 4 * wv + l16;  // this lane's row of the block
-    const T *lrow = M + r0 + min(col, nrow - 1);
-    const int npass = NS == 1 ? 1 : (nk + PW - 1) / PW;
-    // the block's L rows of one pass (clamped; columns past nk meet y = 0)
-    T bl0[CS][kPanel / 4], bl1[CS][kPanel / 4];
-    auto load = [&](int ps, T (&bl)[CS][kPanel / 4]) {
-#pragma unroll
-        for (int c = 0; c < CS; ++c)
-#pragma unroll
-            for (int s = 0; s < kPanel / 4; ++s)
-                bl[c][s] = lrow[(int64_t)(bs + min(ps * PW + kPanel * c + 4 * s + lk, nk - 1)) * f];
-    };
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int i = 16 * wv + l16;  // this lane's row below the block (MFMA column)
+    const T *lrow = M + (int64_t)bs * f + r0 + min(i, nrow - 1);  // L[r0 + i][bs + k] = lrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
+        const T *ycol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // y[bs + .][c0 + l16]
         T old[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(col, nrow - 1)];
+            old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(i, nrow - 1)];
+        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
+            const int kc = kPanel * c, nkc = min(kPanel, nk - kc);
+            kvec_load<T>(av, ycol + kc, nkc);
+            kstride_load<T>(bv, lrow + (int64_t)kc * f, f, nkc);
+        };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        auto use = [&](int ps, const T (&bl)[CS][kPanel / 4]) {
-            const int kc = ps * PW, nkc = min(PW, nk - kc);
-            __syncthreads();  // the previous pass's reads of yv are done
-            block_vec_lds<T>(yv, Y + (int64_t)c0 * n + tk.fc + bs + kc, n, nkc, nr);
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < CS; ++c) {
-                if (kPanel * c >= nkc) break;  // uniform
-#pragma unroll
-                for (int s = 0; s < kPanel / 4; ++s)
-                    acc = MfmaAcc<T>::step(yv[kPanel * c + 4 * s + lk][l16], bl[c][s], acc);
-            }
-        };
-        load(0, bl0);
-        for (int ps = 0; ps < npass; ps += 2) {
-            if (ps + 1 < npass) load(ps + 1, bl1);
-            use(ps, bl0);
-            if (ps + 1 >= npass) break;
-            if (ps + 2 < npass) load(ps + 2, bl0);
-            use(ps + 1, bl1);
-        }
-        if (col < nrow)
+        tile_chain<T, P>(acc, 0, nt, load, [](int, T(&)[16]) {});
+        if (i < nrow)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
-                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + col] = old[q] - acc[q];
+                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + i] = old[q] - acc[q];
             }
     }
 }
 
 // Backward, step 1 of block K: x of sub-block b, x[pb + m] = sum_{k >= m} X[k][pb + m] z[k] over
-// the block's sub-blocks c >= b (z final: the footer and every block to the right are applied),
-// as x^T = z^T X; x goes to Y.
+// the block's sub-blocks c >= b (z final: the footer and every block to the right are applied;
+// X[pc + k][pb + m] at front row pb + m, column pc + k), as x^T = z^T X; x goes to Y.
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   T *__restrict__ Y, const T *__restrict__ Z, int K, int n, int nrhs,
                                                   const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    constexpr int G = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // tiles per load group
-    __shared__ T inv[kPanel][kPanel + 1];    // inv[m][k] = X[pc + k][pb + m]
-    __shared__ T zz[kPanel][kRhsChunk + 1];  // zz[k][rr] = z[pc + k][c0 + rr]
+    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
-    const int b = tk.blk, f = tk.f, w = tk.w;
+    const int b = NS == 1 ? 0 : tk.blk, f = tk.f, w = tk.w;
     const int bs = K * kSBlock, be = min(bs + kSBlock, w);
     const int pb = bs + kPanel * b, nbb = min(kPanel, be - pb);
-    const int nsub = (be - bs + kPanel - 1) / kPanel;
+    const int nsub = NS == 1 ? 1 : (be - bs + kPanel - 1) / kPanel;
     const T *M = arena + tk.off;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int col = kPanel / 4 * wv + l16;
-    const int ngrp = NS == 1 ? 1 : (nsub - b + G - 1) / G;  // groups holding sub-blocks b .. nsub - 1
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int m = 16 * wv + l16;  // this lane's output row of the sub-block (MFMA column)
+    const T *xrow = M + pb + min(m, nbb - 1);  // X[k][pb + m] = xrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
-        T ti0[G][kPanel / 4], tz0[G][4], ti1[G][kPanel / 4], tz1[G][4];
-        auto load = [&](int grp, T (&ti)[G][kPanel / 4], T (&tz)[G][4]) {
+        const T *zcol = Z + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc;  // z[.][c0 + l16]
+        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
+            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+            kvec_load<T>(av, zcol + pc, ncc);
+            kstride_load<T>(bv, xrow + (int64_t)pc * f, f, ncc);
+        };
+        auto prep = [&](int c, T(&bv)[16]) {
+            if (c != b) return;  // X_bb = Li_b: unit diagonal, zero above it (k < m)
 #pragma unroll
-            for (int e = 0; e < G; ++e) {
-                const int c = b + grp * G + e;
-                if (c < nsub) {  // uniform
-                    const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-                    tile_load<T>(ti[e], M, f, pb, pc, nbb, ncc);
-                    vec_load<T>(tz[e], Z + (int64_t)c0 * n + tk.fc + pc, n, ncc, nr);
-                }
+            for (int s = 0; s < 16; ++s) {
+                const int k = 16 * (s >> 2) + 4 * lk + (s & 3);
+                bv[s] = k > m ? bv[s] : (k == m ? T(1) : T(0));
             }
         };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        auto use = [&](int grp, const T (&ti)[G][kPanel / 4], const T (&tz)[G][4]) {
-#pragma unroll
-            for (int e = 0; e < G; ++e) {
-                const int c = b + grp * G + e;
-                if (c >= nsub) break;  // uniform
-                const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
-                __syncthreads();
-                if (c == b) tile_store_inv<T>(inv, ti[e], nbb);
-                else tile_store<T>(inv, ti[e], nbb, ncc);
-                vec_store<T>(zz, tz[e], ncc, nr);
-                __syncthreads();
-#pragma unroll
-                for (int s = 0; s < kPanel / 4; ++s)
-                    acc = MfmaAcc<T>::step(zz[4 * s + lk][l16], inv[col][4 * s + lk], acc);
-            }
-        };
-        load(0, ti0, tz0);
-        for (int grp = 0; grp < ngrp; grp += 2) {
-            if (grp + 1 < ngrp) load(grp + 1, ti1, tz1);
-            use(grp, ti0, tz0);
-            if (grp + 1 >= ngrp) break;
-            if (grp + 2 < ngrp) load(grp + 2, ti0, tz0);
-            use(grp + 1, ti1, tz1);
-        }
-        if (col < nbb)
+        tile_chain<T, P>(acc, b, nsub, load, prep);
+        if (m < nbb)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
-                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pb + col] = acc[q];
+                if (rr < nr) Y[(int64_t)(c0 + rr) * n + tk.fc + pb + m] = acc[q];
             }
-        __syncthreads();
     }
 }
 
 // Backward, step 2 of block K: every 64-column block cb to its left, z[64 cb + j] -= sum_k
-// L[k][64 cb + j] x[k] over the block's rows k, as (z_cb)^T -= x^T L[block rows, cb] (L tiles
-// staged through LDS), 256 (FP64: 128) rows per pass with the next pass's tiles in flight.
+// L[bs + k][64 cb + j] x[bs + k] over the block's rows, as (z_cb)^T -= x^T L[block rows, cb]
+// (both operands along k by vector loads: L down column 64 cb + j).
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  const T *__restrict__ Y, T *__restrict__ Z, int K, int n, int nrhs,
                                                  const int *__restrict__ cols) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
-    constexpr int CS = NS < 4 ? NS : (sizeof(T) == 8 ? 2 : 4);  // 64-row tiles per pass
-    constexpr int PW = CS * kPanel;                               // rows per pass (<= kSChunk)
-    __shared__ T blk[kPanel][kPanel + 1];     // blk[k][c] = L[pk + k][64 cb + c]
-    __shared__ T xv[kSChunk][kRhsChunk + 1];  // xv[k][rr] = x[bs + kc + k][c0 + rr]
+    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
     const PanelTask tk = tasks[blockIdx.x];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int cb = tk.blk, f = tk.f, w = tk.w;
     const int bs = K * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
+    const int nt = NS == 1 ? 1 : (nk + kPanel - 1) / kPanel;
     const T *M = arena + tk.off;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int col = kPanel / 4 * wv + l16;
-    const int npass = NS == 1 ? 1 : (nk + PW - 1) / PW;
-    T tb0[CS][kPanel / 4], tb1[CS][kPanel / 4];
-    auto load = [&](int ps, T (&tb)[CS][kPanel / 4]) {
-#pragma unroll
-        for (int c = 0; c < CS; ++c) {
-            const int kr = ps * PW + kPanel * c;
-            if (kr < nk)  // uniform
-                tile_load<T>(tb[c], M, f, bs + kr, cb * kPanel, min(kPanel, nk - kr), kPanel);
-        }
-    };
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int jj = kPanel * cb + 16 * wv + l16;        // this lane's column (MFMA column), < 64 cb + 64 <= bs
+    const T *lcol = M + (int64_t)jj * f + bs;          // L[bs + k][jj] = lcol[k]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
+        const T *xcol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // x[bs + .][c0 + l16]
         T old[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + cb * kPanel + col];
+        for (int q = 0; q < 4; ++q) old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + jj];
+        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
+            const int kc = kPanel * c, nkc = min(kPanel, nk - kc);
+            kvec_load<T>(av, xcol + kc, nkc);
+            kvec_load<T>(bv, lcol + kc, nkc);
+        };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        auto use = [&](int ps, const T (&tb)[CS][kPanel / 4]) {
-            const int kc = ps * PW, nkc = min(PW, nk - kc);
-            __syncthreads();  // the previous pass's reads of xv and blk are done
-            block_vec_lds<T>(xv, Y + (int64_t)c0 * n + tk.fc + bs + kc, n, nkc, nr);
-#pragma unroll
-            for (int c = 0; c < CS; ++c) {
-                if (kPanel * c >= nkc) break;  // uniform
-                if (c > 0) __syncthreads();    // the previous tile's reads of blk are done
-                tile_store<T>(blk, tb[c], min(kPanel, nkc - kPanel * c), kPanel);
-                __syncthreads();
-#pragma unroll
-                for (int s = 0; s < kPanel / 4; ++s)
-                    acc = MfmaAcc<T>::step(xv[kPanel * c + 4 * s + lk][l16], blk[4 * s + lk][col], acc);
-            }
-        };
-        load(0, tb0);
-        for (int ps = 0; ps < npass; ps += 2) {
-            if (ps + 1 < npass) load(ps + 1, tb1);
-            use(ps, tb0);
-            if (ps + 1 >= npass) break;
-            if (ps + 2 < npass) load(ps + 2, tb0);
-            use(ps + 1, tb1);
-        }
+        tile_chain<T, P>(acc, 0, nt, load, [](int, T(&)[16]) {});
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int rr = MfmaAcc<T>::row(lk, q);
-            if (rr < nr) Z[(int64_t)(c0 + rr) * n + tk.fc + cb * kPanel + col] = old[q] - acc[q];
+            if (rr < nr) Z[(int64_t)(c0 + rr) * n + tk.fc + jj] = old[q] - acc[q];
         }
-        __syncthreads();
     }
 }
 
