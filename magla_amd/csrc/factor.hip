@@ -1591,39 +1591,56 @@ __global__ __launch_bounds__(256) void k_pivot_stats_final(const double *__restr
 
 // ---- solves ----------------------------------------------------------------
 
-// Y[i] = (T) B[perm[i]] (elimination order), nrhs columns; grid (row blocks of 256, nrhs).
-// Batches (cols non-null, systems contiguous and permuted within their own rows): a column at
-// or past the row's system count is left out, and no solve kernel reads it.
+// Y[i] = (T) B[perm[i]] (elimination order): a thread per row i, its columns eight at a time
+// (the eight gathered loads issued together, the stores coalesced across the wave).  Columns at
+// or past the count of the row's system (cols, per system; rows of a system are contiguous and
+// permuted among themselves) are left out: no solve kernel reads them.
 template <typename T>
 __global__ __launch_bounds__(256) void k_permute_in(const double *__restrict__ B, int ldb, const int *__restrict__ perm,
-                                                    int n, T *__restrict__ Y, const int *__restrict__ row_sys,
-                                                    const int *__restrict__ cols, int rbase) {
-    const int i = blockIdx.x * 256 + threadIdx.x, r = rbase + blockIdx.y;
+                                                    int n, int nrhs, T *__restrict__ Y, const int *__restrict__ row_sys,
+                                                    const int *__restrict__ cols) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    if (cols && r >= cols[row_sys[i]]) return;
-    Y[(int64_t)r * n + i] = (T)B[(int64_t)r * ldb + perm[i]];
+    const int nc = cols ? min(nrhs, cols[row_sys[i]]) : nrhs;
+    const double *src = B + perm[i];
+    for (int r0 = 0; r0 < nc; r0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)min(r0 + u, nc - 1) * ldb];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (r0 + u < nc) Y[(int64_t)(r0 + u) * n + i] = (T)v[u];
+    }
 }
 
-// X[j] (+)= Xe[iperm[j]]: the FP64 solution is read and written in order, the gather falls
-// on the (narrower) elimination-order vector
+// X[j] (+)= Xe[iperm[j]]: a thread per row j in the caller's order, its columns eight at a time
+// (the FP64 solution read and written in order, the gather on the elimination-order vector).  A
+// column past the row's system count has a zero right-hand side (a first solve: the solution is
+// zero) or a zero residual (a correction: nothing to add).
 template <typename T>
 __global__ __launch_bounds__(256) void k_permute_out(const T *__restrict__ Xe, const int *__restrict__ iperm, int n,
-                                                     double *__restrict__ X, int ldx, int accumulate,
-                                                     const int *__restrict__ row_sys, const int *__restrict__ cols,
-                                                     int rbase) {
-    // grid: (row blocks of 256, columns rbase .. rbase + gridDim.y - 1)
-    const int j = blockIdx.x * 256 + threadIdx.x, r = rbase + blockIdx.y;
+                                                     int nrhs, double *__restrict__ X, int ldx, int accumulate,
+                                                     const int *__restrict__ row_sys, const int *__restrict__ cols) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
-    double *dst = X + (int64_t)r * ldx + j;
-    // batches: a column past the system's count has a zero right-hand side (a first solve:
-    // the solution is zero) or a zero residual (a correction: nothing to add)
-    if (cols && r >= cols[row_sys[j]]) {
-        if (!accumulate) *dst = 0.0;
-        return;
+    const int nc = cols ? min(nrhs, cols[row_sys[j]]) : nrhs;
+    const T *src = Xe + iperm[j];
+    double *dst = X + j;
+    if (!accumulate)
+        for (int r = nc; r < nrhs; ++r) dst[(int64_t)r * ldx] = 0.0;
+    for (int r0 = 0; r0 < nc; r0 += 8) {
+        T v[8];
+        double o[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int r = min(r0 + u, nc - 1);
+            v[u] = src[(int64_t)r * n];
+            o[u] = accumulate ? dst[(int64_t)r * ldx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (r0 + u < nc) dst[(int64_t)(r0 + u) * ldx] = o[u] + (double)v[u];
     }
-    const double v = (double)Xe[(int64_t)r * n + iperm[j]];
-    if (accumulate) *dst += v;
-    else *dst = v;
 }
 
 // Supernodal solves, level by level over the fronts
@@ -3326,10 +3343,8 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
     T *V = reinterpret_cast<T *>(ws.V);
     const T *arena = reinterpret_cast<const T *>(N.arena.p);
     const int *rsys = cols ? P.row_sys.p : nullptr;  // cols: batches only
-    constexpr int kGridY = 65535;  // columns per permute launch
-    for (int rb = 0; rb < nrhs; rb += kGridY)
-        hipLaunchKernelGGL(k_permute_in<T>, dim3((n + 255) / 256, min(kGridY, nrhs - rb)), dim3(256), 0, st, d_rhs, ldr,
-                           P.perm.p, n, Y, rsys, cols, rb);
+    hipLaunchKernelGGL(k_permute_in<T>, dim3((n + 255) / 256), dim3(256), 0, st, d_rhs, ldr, P.perm.p, n, nrhs, Y,
+                       rsys, cols);
     T *Z = reinterpret_cast<T *>(ws.Z);
     T *part = reinterpret_cast<T *>(ws.part);
     const SmallFwdArgs sfa{P.child_recs.p, P.inv_map.p, n, nrhs, cols};
@@ -3410,9 +3425,8 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
                                P.small_recs.p + m0, m1 - m0, P.pat_rows.p, arena, N.diag.p, Y, n, nrhs, cols);
         }
     }
-    for (int rb = 0; rb < nrhs; rb += kGridY)
-        hipLaunchKernelGGL(k_permute_out<T>, dim3((n + 255) / 256, min(kGridY, nrhs - rb)), dim3(256), 0, st, Y,
-                           P.iperm.p, n, d_x, ldx, accumulate, rsys, cols, rb);
+    hipLaunchKernelGGL(k_permute_out<T>, dim3((n + 255) / 256), dim3(256), 0, st, Y, P.iperm.p, n, nrhs, d_x, ldx,
+                       accumulate, rsys, cols);
     MG_HIP(hipGetLastError());
 }
 
