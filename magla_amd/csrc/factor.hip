@@ -1150,7 +1150,16 @@ __device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, P...
 // Row stride of the scaled inverse Bm in LDS: the MFMA operand reads (4 rows x 16
 // consecutive columns per wave) hit distinct banks for FP32.
 template <typename T>
-constexpr int kBmLd = sizeof(T) == 4 ? kPanel + 16 : kPanel + 1;
+constexpr int kBmLd = sizeof(T) == 4 ? kPanel : kPanel + 1;
+// FP32: entry (m, k) of Bm sits at column k ^ h(m), h(m) = 16 (m mod 4) + (m / 4) mod 16 (a
+// bijection of 0..63): the 64 rows of one column written together hit 64 different banks, and
+// the MFMA reads (rows 4 s .. 4 s + 3, 16 consecutive columns) fall on four different 16-bank
+// groups.  FP64: plain, stride 65.
+template <typename T>
+__device__ __forceinline__ int bm_col(int m, int k) {
+    if constexpr (sizeof(T) == 4) return k ^ (((m & 3) << 4) | ((m >> 2) & 15));
+    else return k;
+}
 
 // Row-block product of a panel on the matrix cores: L[r0 + i][ps + k] = sum_m X[i][m] Bm[m][k]
 // for i < nr, k < nb, with X the row block's assembled values.  Computed as
@@ -1181,14 +1190,14 @@ __device__ __forceinline__ void rows_product(T *__restrict__ M, int f, int ps, i
 #pragma unroll
         for (int s = 0; s < kPanel / 4; ++s)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][16 * t + l16], xb[s], acc[t]);
+            for (int t = 0; t < 4; ++t) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][bm_col<T>(4 * s + lk, 16 * t + l16)], xb[s], acc[t]);
     } else {
 #pragma unroll
         for (int s = 0; s < kPanel / 4; ++s) {
             if (s >= ns) break;
 #pragma unroll
             for (int t = 0; t < 4; ++t)
-                if (t < nt) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][16 * t + l16], xb[s], acc[t]);
+                if (t < nt) acc[t] = MfmaAcc<T>::step(Bm[4 * s + lk][bm_col<T>(4 * s + lk, 16 * t + l16)], xb[s], acc[t]);
         }
     }
     const int i = 16 * v + l16;
@@ -1236,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
         for (int j = 0; j < kPanel / 4; ++j) {
             const int k = g + 4 * j;
             const T rdk = T(1) / td[j];
-            Bm[lane][k] = (lane < k && k < nb) ? ti[j] * rdk : (lane == k && k < nb ? rdk : T(0));
+            Bm[lane][bm_col<T>(lane, k)] = (lane < k && k < nb) ? ti[j] * rdk : (lane == k && k < nb ? rdk : T(0));
         }
     }
     __syncthreads();
@@ -1315,7 +1324,7 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
 #pragma unroll
         for (int j = 0; j < kDiagCols; ++j) {
             const int c = v * kDiagCols + j;
-            Bm[c][r] = (c < r && r < nb) ? li[j] * rd : (c == r && r < nb ? rd : T(0));
+            Bm[c][bm_col<T>(c, r)] = (c < r && r < nb) ? li[j] * rd : (c == r && r < nb ? rd : T(0));
         }
     }
     __syncthreads();
@@ -1395,13 +1404,13 @@ __global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ 
                 vb[i] = live ? rb[j][i] : T(0);
             }
             if constexpr (kSchurLd<T> % VEC == 0) {  // aligned rows: one 16-byte store each
-                *reinterpret_cast<vec_t *>(&As[k][VEC * qv]) = va;
-                *reinterpret_cast<vec_t *>(&Bs[k][VEC * qv]) = vb;
+                *reinterpret_cast<vec_t *>(&As[k][sch_col<T>(k, VEC * qv)]) = va;
+                *reinterpret_cast<vec_t *>(&Bs[k][sch_col<T>(k, VEC * qv)]) = vb;
             } else {
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) {
-                    As[k][VEC * qv + i] = va[i];
-                    Bs[k][VEC * qv + i] = vb[i];
+                    As[k][sch_col<T>(k, VEC * qv + i)] = va[i];
+                    Bs[k][sch_col<T>(k, VEC * qv + i)] = vb[i];
                 }
             }
         }
@@ -2207,8 +2216,8 @@ __global__ __launch_bounds__(256) void k_inv_blk(const int4 *__restrict__ tasks,
             const int k = q4 + 4 * j;
             T x = bv[j];
             if (c == b) x = l < k ? x : (l == k ? T(1) : T(0));
-            As[k][l] = (k < nc && l < na) ? av[j] : T(0);
-            Bs[k][l] = (k < nc && l < nbb) ? x : T(0);
+            As[k][sch_col<T>(k, l)] = (k < nc && l < na) ? av[j] : T(0);
+            Bs[k][sch_col<T>(k, l)] = (k < nc && l < nbb) ? x : T(0);
         }
         __syncthreads();
         if (c + 1 < a) load(c + 1);
@@ -2227,11 +2236,14 @@ __global__ __launch_bounds__(256) void k_inv_blk(const int4 *__restrict__ tasks,
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Bs[rq + 16 * x + l16][cq + 16 * y + MfmaAcc<T>::row(lk, q)] = acc[x][y][q];
+            for (int q = 0; q < 4; ++q) {
+                const int kk = rq + 16 * x + l16;
+                Bs[kk][sch_col<T>(kk, cq + 16 * y + MfmaAcc<T>::row(lk, q))] = acc[x][y][q];
+            }
 #pragma unroll
     for (int j = 0; j < kPanel / 4; ++j) {
         const int r = q4 + 4 * j;  // li[j] = M[(pa + r) f + pa + l] = Li_a[r][l] for l < r
-        As[l][r] = (r < na && l < na) ? (l < r ? li[j] : (l == r ? T(1) : T(0))) : T(0);
+        As[l][sch_col<T>(l, r)] = (r < na && l < na) ? (l < r ? li[j] : (l == r ? T(1) : T(0))) : T(0);
     }
     __syncthreads();
     schur_acc_zero<T>(acc);

@@ -20,12 +20,20 @@
 
 namespace mg {
 
-// Row stride (elements) of the LDS operand chunks.  FP32: 72 = 8 mod 32 banks, so the four
-// k-rows one operand read touches (16 lanes of 4 bytes each) cover every bank exactly twice,
-// and a row start is 16-byte aligned for the vector stores that fill it.  FP64: 65 (a row of
-// 16 lanes of 8 bytes already spans all banks; four rows take four cycles at any stride).
+// Row stride (elements) of the LDS operand chunks.  FP32: 64, with the 16-column groups of row
+// k stored XOR-swizzled by (k mod 4) (sch_col): the four k-rows one operand read touches
+// (16 lanes of 4 bytes each) then fall on four different 16-bank groups, so the read is free
+// of bank conflicts with 64 banks (and with 32 banks served per half wave); a swizzled 4-float
+// vector stays contiguous and 16-byte aligned for the stores that fill a row.  FP64: 65 (a
+// row of 16 lanes of 8 bytes already spans all banks; four rows take four cycles at any
+// stride), no swizzle.
 template <typename T>
-constexpr int kSchurLd = sizeof(T) == 4 ? 72 : 65;
+constexpr int kSchurLd = sizeof(T) == 4 ? 64 : 65;
+template <typename T>
+__device__ __forceinline__ int sch_col(int k, int c) {
+    if constexpr (sizeof(T) == 4) return c ^ ((k & 3) << 4);
+    else return c;
+}
 
 typedef double mg_d4 __attribute__((ext_vector_type(4)));
 typedef float mg_f4 __attribute__((ext_vector_type(4)));
@@ -67,9 +75,9 @@ __device__ inline void schur_read_ops(const T (*As)[kSchurLd<T>], const T (*Bs)[
     const int rq = (wave & 1) * 32, cq = (wave >> 1) * 32;
     const int k = k0 + (lane >> 4), l16 = lane & 15;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) colv[b] = Bs[k][cq + 16 * b + l16];
+    for (int b = 0; b < 2; ++b) colv[b] = Bs[k][sch_col<T>(k, cq + 16 * b + l16)];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) rowv[a] = As[k][rq + 16 * a + l16];
+    for (int a = 0; a < 2; ++a) rowv[a] = As[k][sch_col<T>(k, rq + 16 * a + l16)];
 }
 
 template <typename T>
