@@ -1030,121 +1030,169 @@ __device__ __forceinline__ double pivot_recip(double x) {
     return __builtin_fma(y, e, y);
 }
 
-// State of one 64x64 diagonal-block factorization: four waves, lane r of wave v
-// holds row r of the block for columns 16v .. 16v+15 (a) and the same part of
-// row r of the block's inverse (li).
-struct PanelDiagCtx {
-    int r, v, nb, allow_reg;
-    double thr, regv;
-    double *dsh;          // pivots (LDS)
-    unsigned char *fsh;   // pivot flags (LDS)
-};
-constexpr int kDiagCols = kPanel / 4;  // columns per wave in k_panel
-
 // Pivot rule of UtduNumericFactorization.cs:553-577 for a pivot candidate, compared in
 // float64: returns the pivot to use and sets the step's flags.
-__device__ __forceinline__ double pivot_rule(double piv, const PanelDiagCtx &x, unsigned char &fl) {
+__device__ __forceinline__ double pivot_rule(double piv, double thr, double regv, int allow_reg, unsigned char &fl) {
     fl = 0;
-    if (!(piv > x.thr)) {
-        if (!x.allow_reg) fl |= 4;
+    if (!(piv > thr)) {
+        if (!allow_reg) fl |= 4;
         if (piv < 0.0) fl |= 2;
         fl |= 1;
-        piv = x.regv;
+        piv = regv;
     }
     return piv;
 }
 
-// Steps K and K + 1 (K even: both columns in one wave) of the right-looking LDL^T of the
-// diagonal block (UtduNumericFactorization.cs:560-600), with one barrier.  Lane r holds row r,
-// wave v columns 16 v .. 16 v + 15, of the block (a) and of its inverse (li).  Step K: the
-// wave owning column K applies the pivot rule and publishes the scaled column
-// (l_r = a[r][K] / d_K for r > K, zero for r <= K) and the pivot through LDS; lane K of every
-// wave publishes its part of row K of the inverse (final after step K - 1).  Every wave then
-// updates its columns, a[r][c] -= (l_r d_K) l_c (l_c = 0 for c <= K, so eliminated columns are
-// left as they are), and applies the same row operation to the inverse,
-// li[r][c] -= l_r li[K][c] (li[K][c] = 0 for c > K).
-// Pairing: before the barrier the owner wave also applies step K to column K + 1 (the same
-// operations the other columns get after it), takes pivot K + 1 and publishes that scaled
-// column too.  After the barrier every wave applies step K, then step K + 1, whose row of the
-// inverse (final after step K) comes from lane K + 1 of the wave itself.  Every value is the
-// one the two steps taken one at a time give.  Buffers alternate by the parity of K / 2, so
-// one barrier per pair suffices.
-template <typename T, int K>
-__device__ __forceinline__ void panel_diag_pair(T (&a)[kDiagCols], T (&li)[kDiagCols], const PanelDiagCtx &x,
-                                                T (*colk)[kPanel], T (*rowk)[kPanel], T *pvs) {
-    static_assert(K % 2 == 0 && kDiagCols % 2 == 0, "pairs of columns of one wave");
-    constexpr int kw = K / kDiagCols, kj = K % kDiagCols, pb = (K / 2) & 1;
-    if (K >= x.nb) return;
-    const bool two = K + 1 < x.nb;  // uniform
-    const int cb = x.v * kDiagCols;
-    T *c0 = colk[2 * pb], *c1 = colk[2 * pb + 1];
-    if (x.r == K) {
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) rowk[pb][cb + j] = li[j];
-    }
-    if (x.v == kw) {
-        unsigned char fl;
-        const T pv = (T)pivot_rule((double)lane_value<T>(a[kj], K), x, fl), pinv = pivot_recip(pv);
-        const T l = a[kj] * pinv;
-        a[kj] = x.r > K ? l : a[kj];
-        const T lb0 = x.r > K ? l : T(0);
-        c0[x.r] = lb0;
-        pvs[2 * pb] = pv;
-        x.dsh[K] = (double)pv;
-        x.fsh[K] = fl;
-        if (two) {
-            const T t0 = lb0 * pv;
-            a[kj + 1] -= t0 * lane_value<T>(lb0, K + 1);  // step K on column K + 1
-            unsigned char fl1;
-            const T pv1 = (T)pivot_rule((double)lane_value<T>(a[kj + 1], K + 1), x, fl1),
-                    pinv1 = pivot_recip(pv1);
-            const T l1 = a[kj + 1] * pinv1;
-            a[kj + 1] = x.r > K + 1 ? l1 : a[kj + 1];
-            c1[x.r] = x.r > K + 1 ? l1 : T(0);
-            pvs[2 * pb + 1] = pv1;
-            x.dsh[K + 1] = (double)pv1;
-            x.fsh[K + 1] = fl1;
-        }
-    }
-    __syncthreads();
-    const bool done = two && x.v == kw;  // column K + 1 of the owner wave has step K already
-    {
-        const T lb = c0[x.r];
-        const T t = lb * pvs[2 * pb];
-        T lc[kDiagCols], rk[kDiagCols];
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) {
-            lc[j] = c0[cb + j];
-            rk[j] = rowk[pb][cb + j];
-        }
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) a[j] = (j == kj + 1 && done) ? a[j] : a[j] - t * lc[j];
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
-    }
-    if (two) {
-        const T lb = c1[x.r];
-        const T t = lb * pvs[2 * pb + 1];
-        T lc[kDiagCols], rk[kDiagCols];
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) {
-            lc[j] = c1[cb + j];
-            rk[j] = lane_value<T>(li[j], K + 1);
-        }
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) a[j] -= t * lc[j];
-#pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) li[j] -= lb * rk[j];
-    }
-#pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) asm volatile("" : "+v"(a[j]), "+v"(li[j]));
+// Diagonal block of a panel: the right-looking LDL^T of UtduNumericFactorization.cs:560-600 on a
+// (<=64)^2 block, taken in sixteen groups of four pivots by one workgroup of four waves.  Wave v
+// holds block columns 16v..16v+15 of the block (a) and of its inverse Li = L11^{-1} (li) as
+// four 16x16 matrix-core accumulator tiles: register q of tile t in lane (n, g), n = lane mod 16,
+// g = lane / 16, holds row 16t + g + 4q, column 16v + n.  For the group of pivots k..k+3:
+//  - the wave owning columns k..k+3 copies them to LDS and reads them back one row per lane,
+//    takes the four pivots in order (pivot rule; l_r = a_rK / d_K; a_rc -= (l_r d_K) l_c on the
+//    group's later columns) and publishes -(l d), l and -(L Q) for all 64 rows, Q the inverse of
+//    the group's unit lower 4x4 block P of L;
+//  - after one barrier every wave applies the group on the matrix cores: the rank-4 update
+//    a -= (L D) L^T on its columns right of the group, and the four row operations of the
+//    eliminations on the inverse, Li <- Li - (L Q) Li[k..k+3, :], which leaves Q Li[k..k+3, :]
+//    in the group's rows (M = L Q is I - Q there, the stored l being zero on and above the
+//    diagonal) and Li[r, :] - sum_i l_ri Li_new[k+i, :] below them.
+// Entries above the diagonal and in eliminated columns receive updates that nothing reads.
+// Sixteen barriers; per group four serial pivots in one wave and at most eight products per wave.
+constexpr int kDgLd = 80;  // row stride of the published columns: the four operand rows g = 0..3 of
+                           // a read start 16 banks apart
+constexpr int kDgColLd = 68;  // the owner's copy: its 4 x 4 writing lanes hit distinct banks
+template <typename T>
+struct DiagLds {
+    T wn[2][4][kDgLd];            // -(l_r d_K), K = k + i, row r (two groups in flight)
+    T lp[2][4][kDgLd];            // l_r
+    T mn[2][4][kDgLd];            // -(L Q)[r][i]
+    T pcol[4][kDgColLd];          // the next group's four columns, as its owner wave reads them
+    T rd[kPanel];                 // 1 / d_k for the row block's operand
+    double dsh[kPanel];           // pivots
+    unsigned char fsh[kPanel];    // pivot flags
+};
+
+// M index m of a product's first operand -> row offset in its tile, so that accumulator
+// register q of lane group g holds row g + 4q for both precisions (the FP32 instruction
+// writes M rows 4g + q, the FP64 one g + 4q).
+template <typename T>
+__device__ __forceinline__ int dg_row(int m) {
+    if constexpr (sizeof(T) == 4) return (m >> 2) | ((m & 3) << 2);
+    else return m;
 }
 
-template <typename T, int... P>
-__device__ __forceinline__ void panel_diag_steps(std::integer_sequence<int, P...>, T (&a)[kDiagCols],
-                                                 T (&li)[kDiagCols], const PanelDiagCtx &x, T (*colk)[kPanel],
-                                                 T (*rowk)[kPanel], T *pvs) {
-    (panel_diag_pair<T, 2 * P>(a, li, x, colk, rowk, pvs), ...);
+// The owner wave of group G copies block columns 4G..4G+3 (its lanes n = 4 (G mod 4) + i) to
+// S.pcol, all 64 rows.
+template <typename T>
+__device__ __forceinline__ void dg_extract(DiagLds<T> &S, const typename MfmaAcc<T>::type (&a)[4], int G) {
+    const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4, i = n - 4 * (G & 3);
+    if (i >= 0 && i < 4) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) S.pcol[i][16 * t + g + 4 * q] = a[t][q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// The owner wave of group G takes pivots k = 4G .. k+3 (lane r: row r of the four columns)
+// and publishes the group's operands into buffer b.  Pivots past nb (identity padding) are 1.
+template <typename T>
+__device__ __forceinline__ void dg_factor_group(DiagLds<T> &S, int G, int b, int nb, double thr, double regv,
+                                                int allow_reg) {
+    const int r = threadIdx.x & 63, k = 4 * G;
+    T c[4], l[4], p[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = S.pcol[i][r];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int K = k + i;
+        unsigned char fl = 0;
+        const T cand = lane_value<T>(c[i], K);
+        const T pv = K < nb ? (T)pivot_rule((double)cand, thr, regv, allow_reg, fl) : T(1);
+        const T pinv = pivot_recip(pv);
+        l[i] = r > K ? c[i] * pinv : T(0);
+        const T w = l[i] * pv;
+        S.wn[b][i][r] = -w;
+        S.lp[b][i][r] = l[i];
+        if (r == 0) {
+            S.dsh[K] = (double)pv;
+            S.fsh[K] = fl;
+        }
+#pragma unroll
+        for (int i2 = i + 1; i2 < 4; ++i2) {
+            p[i2][i] = lane_value<T>(l[i], k + i2);  // P[i2][i] = l of row k + i2
+            c[i2] -= w * p[i2][i];
+        }
+    }
+    // Q = P^{-1}: Q[a][e] = -(P[a][e] + sum_{e<m<a} P[a][m] Q[m][e])
+    T q[4][4];
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+        for (int e = a - 1; e >= 0; --e) {
+            T s = p[a][e];
+#pragma unroll
+            for (int m = e + 1; m < a; ++m) s += p[a][m] * q[m][e];
+            q[a][e] = -s;
+        }
+    // (L Q)[r][e] = l_e + sum_{a>e} l_a Q[a][e]
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        T m = l[e];
+#pragma unroll
+        for (int a = e + 1; a < 4; ++a) m += l[a] * q[a][e];
+        S.mn[b][e][r] = -m;
+    }
+}
+
+// Groups 4 T0 .. 4 T0 + 3 of the diagonal block (the pivots of tile column T0); every wave
+// passes the same barriers.
+template <typename T, int T0>
+__device__ __forceinline__ void dg_tile_column(DiagLds<T> &S, typename MfmaAcc<T>::type (&a)[4],
+                                               typename MfmaAcc<T>::type (&li)[4], int v, int nb, double thr,
+                                               double regv, int allow_reg) {
+    const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int gp = 0; gp < 4; ++gp) {
+        const int G = 4 * T0 + gp, b = G & 1;
+        if (4 * G >= nb) break;  // uniform
+        if (v > T0 || (v == T0 && gp < 3)) {  // columns right of the group
+            const T bl = S.lp[b][g][16 * v + n];
+#pragma unroll
+            for (int t = T0; t < 4; ++t)
+                if (t >= v) a[t] = MfmaAcc<T>::step(S.wn[b][g][16 * t + dg_row<T>(n)], bl, a[t]);
+        }
+        if (v <= T0) {  // inverse columns up to the group's last
+            const T bli = gp == 0 ? li[T0][0] : (gp == 1 ? li[T0][1] : (gp == 2 ? li[T0][2] : li[T0][3]));
+#pragma unroll
+            for (int t = T0; t < 4; ++t) li[t] = MfmaAcc<T>::step(S.mn[b][g][16 * t + dg_row<T>(n)], bli, li[t]);
+        }
+        const int G1 = G + 1;
+        if (G1 < kPanel / 4 && 4 * G1 < nb && v == G1 / 4) {
+            dg_extract<T>(S, a, G1);
+            dg_factor_group<T>(S, G1, b ^ 1, nb, thr, regv, allow_reg);
+        }
+        __syncthreads();
+    }
+}
+
+// Factors the panel's diagonal block (nb columns, identity past nb) held in a, forming li.
+// Ends with S.dsh / S.fsh complete.
+template <typename T>
+__device__ __forceinline__ void dg_factor(DiagLds<T> &S, typename MfmaAcc<T>::type (&a)[4],
+                                          typename MfmaAcc<T>::type (&li)[4], int v, int nb, double thr,
+                                          double regv, int allow_reg) {
+    if (v == 0) {
+        dg_extract<T>(S, a, 0);
+        dg_factor_group<T>(S, 0, 0, nb, thr, regv, allow_reg);
+    }
+    __syncthreads();
+    dg_tile_column<T, 0>(S, a, li, v, nb, thr, regv, allow_reg);
+    if (nb > 16) dg_tile_column<T, 1>(S, a, li, v, nb, thr, regv, allow_reg);
+    if (nb > 32) dg_tile_column<T, 2>(S, a, li, v, nb, thr, regv, allow_reg);
+    if (nb > 48) dg_tile_column<T, 3>(S, a, li, v, nb, thr, regv, allow_reg);
 }
 
 // Row stride of the scaled inverse Bm in LDS: the MFMA operand reads (4 rows x 16
@@ -1252,16 +1300,16 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
     rows_product<T>(M, f, ps, nb, r0, nr, Bm, xb);
 }
 
-// Panel LDL^T: every workgroup factors the front's 64x64 diagonal block (four waves, one
-// barrier per pair of pivots, panel_diag_pair) and then, for a front with at most kPanelFusedRows row
-// blocks below the panel, turns its 64-row block into L rows, X := X B with
-// B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k (as k_panel_rows).  Those row-block workgroups
-// repeat the diagonal factorization instead of waiting for a launch of its own; the row
-// block's loads are in flight meanwhile.  A front with more row blocks gets one workgroup
-// here (row block -1) and its row blocks go to k_panel_rows.  The workgroup of row block 0
-// or -1 writes D, the pivot flags and Li.  The diagonal block's lower triangle keeps its assembled values: the other
-// workgroups of the launch read them, and nothing later does (the row blocks, the Schur
-// updates and the solves use Li and D; the host's factor export rebuilds L11 from Li).
+// Panel LDL^T: every workgroup factors the front's 64x64 diagonal block (dg_factor) and then,
+// for a front with at most kPanelFusedRows row blocks below the panel, turns its 64-row block
+// into L rows, X := X B with B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k (as k_panel_rows).
+// Those row-block workgroups repeat the diagonal factorization instead of waiting for a launch
+// of its own; the row block's loads are in flight meanwhile.  A front with more row blocks gets
+// one workgroup here (row block -1) and its row blocks go to k_panel_rows.  The workgroup of row
+// block 0 or -1 writes D, the pivot flags and Li.  The diagonal block's lower triangle keeps its
+// assembled values: the other workgroups of the launch read them, and nothing later does (the
+// row blocks, the Schur updates and the solves use Li and D; the host's factor export rebuilds
+// L11 from Li).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T *__restrict__ arena,
                                                const int *__restrict__ F, const int *__restrict__ W,
@@ -1270,12 +1318,7 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
                                                const int *__restrict__ front_sys, double tol, double regf,
                                                int allow_reg, double *__restrict__ D,
                                                unsigned char *__restrict__ flags) {
-    // Rows/columns past nb are padded with the identity (pivot 1, never reported).
-    __shared__ __align__(16) T colk[4][kPanel];  // two pairs of scaled columns
-    __shared__ __align__(16) T rowk[2][kPanel];
-    __shared__ T pvs[4];
-    __shared__ double dsh[kPanel];
-    __shared__ unsigned char fsh[kPanel];
+    __shared__ DiagLds<T> S;
     // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0 (the MFMA A operand)
     __shared__ T Bm[kPanel][kBmLd<T>];
     const int2 tk = tasks[blockIdx.x];
@@ -1284,49 +1327,60 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
     const int pe = min(ps + kPanel, w), nb = pe - ps;
     const int r0 = pe + max(rb, 0) * kTile, nr = rb < 0 ? 0 : min(kTile, f - r0);
     T *M = arena + off[s];
-    const int r = threadIdx.x & 63, v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4;
+    const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T xb[kPanel / 4];
     if (nr > 0) rows_load<T>(xb, M, f, ps, nb, r0, nr);  // in flight during the factorization
     const double sc = scale[front_sys[s]];
-    PanelDiagCtx x{r, v, nb, allow_reg, tol * sc, regf * sc, dsh, fsh};
-    // Unconditional loads (addresses clamped into the block), masked afterwards: a
-    // masked load would wait for the previous one before its register is reused.
-    T a[kDiagCols], li[kDiagCols];
-    const T *mcol = M + (int64_t)ps * f + ps + min(r, nb - 1);
+    // Block column c = 16v + n, rows 16t + g + 4q: lower triangle, identity past nb, zero above
+    // (addresses clamped into the block, masked after every load is issued).
+    typename MfmaAcc<T>::type a[4], li[4];
+    const int c = 16 * v + n;
+    {
+        const T *col = M + (int64_t)(ps + min(c, nb - 1)) * f + ps;
 #pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) a[j] = mcol[(int64_t)min(v * kDiagCols + j, nb - 1) * f];
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int j = 0; j < kDiagCols; ++j) {
-        const int c = v * kDiagCols + j;
-        a[j] = (r < nb && c < nb && r >= c) ? a[j] : (r == c ? T(1) : T(0));
-        li[j] = r == c ? T(1) : T(0);
+            for (int q = 0; q < 4; ++q) a[t][q] = col[min(16 * t + g + 4 * q, nb - 1)];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = 16 * t + g + 4 * q;
+                T x = a[t][q];
+                pin(x);
+                a[t][q] = (rr < nb && c < nb && rr >= c) ? x : (rr == c ? T(1) : T(0));
+                li[t][q] = rr == c ? T(1) : T(0);
+            }
     }
-    panel_diag_steps<T>(std::make_integer_sequence<int, kPanel / 2>{}, a, li, x, colk, rowk, pvs);
-    __syncthreads();
+    dg_factor<T>(S, a, li, v, nb, tol * sc, regf * sc, allow_reg);
     if (rb <= 0) {
         if (threadIdx.x < nb) {
             const int fc = firstcol[s] + ps;
-            D[fc + threadIdx.x] = dsh[threadIdx.x];
-            flags[fc + threadIdx.x] = fsh[threadIdx.x];
+            D[fc + threadIdx.x] = S.dsh[threadIdx.x];
+            flags[fc + threadIdx.x] = S.fsh[threadIdx.x];
         }
-        if (r < nb) {
+        // Li[rr][c] at front row ps + c, column ps + rr
 #pragma unroll
-            for (int j = 0; j < kDiagCols; ++j) {
-                const int c = v * kDiagCols + j;
-                if (c < r) M[(int64_t)(ps + r) * f + ps + c] = li[j];  // Li[r][c] at row ps+c, column ps+r
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = 16 * t + g + 4 * q;
+                if (c < rr && rr < nb) M[(int64_t)(ps + rr) * f + ps + c] = li[t][q];
             }
-        }
     }
     if (nr <= 0) return;  // uniform: the diagonal-block-only workgroup
-    {
-        // Bm[c][r] = Li[r][c] / d_r (lane r holds row r of Li)
-        const T rd = T(1) / (T)dsh[min(r, nb - 1)];
+    if (threadIdx.x < kPanel) S.rd[threadIdx.x] = T(1) / (T)S.dsh[min((int)threadIdx.x, nb - 1)];
+    __syncthreads();
+    // Bm[c][rr] = Li[rr][c] / d_rr
 #pragma unroll
-        for (int j = 0; j < kDiagCols; ++j) {
-            const int c = v * kDiagCols + j;
-            Bm[c][bm_col<T>(c, r)] = (c < r && r < nb) ? li[j] * rd : (c == r && r < nb ? rd : T(0));
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = 16 * t + g + 4 * q;
+            const T rd = S.rd[rr];
+            Bm[c][bm_col<T>(c, rr)] = (c < rr && rr < nb) ? li[t][q] * rd : (c == rr && rr < nb ? rd : T(0));
         }
-    }
     __syncthreads();
     rows_product<T>(M, f, ps, nb, r0, nr, Bm, xb);
 }
