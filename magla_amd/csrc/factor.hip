@@ -1062,6 +1062,8 @@ __device__ __forceinline__ double pivot_rule(double piv, double thr, double regv
 constexpr int kDgLd = 80;  // row stride of the published columns: the four operand rows g = 0..3 of
                            // a read start 16 banks apart
 constexpr int kDgColLd = 68;  // the owner's copy: its 4 x 4 writing lanes hit distinct banks
+constexpr int kDgStage = 68;  // row stride of the block's staging copy: the tile reads (16 columns
+                              // x 4 rows per instruction) hit distinct banks
 template <typename T>
 struct DiagLds {
     T wn[2][4][kDgLd];            // -(l_r d_K), K = k + i, row r (two groups in flight)
@@ -1069,8 +1071,8 @@ struct DiagLds {
     T mn[2][4][kDgLd];            // -(L Q)[r][i]
     T pcol[4][kDgColLd];          // the next group's four columns, as its owner wave reads them
     T rd[kPanel];                 // 1 / d_k for the row block's operand
-    double dsh[kPanel];           // pivots
-    unsigned char fsh[kPanel];    // pivot flags
+    T cand[kPanel];               // pivot candidates (the pivot rule's input)
+    T piv[kPanel];                // pivots
 };
 
 // M index m of a product's first operand -> row offset in its tile, so that accumulator
@@ -1082,10 +1084,10 @@ __device__ __forceinline__ int dg_row(int m) {
     else return m;
 }
 
-// The owner wave of group G copies block columns 4G..4G+3 (its lanes n = 4 (G mod 4) + i) to
-// S.pcol, all 64 rows.
+// The owner wave of group G copies block columns 4G..4G+3 (its lanes n = 4 (G mod 4) + i)
+// through S.pcol into c, one row per lane: c[i] = A[r][4G + i].
 template <typename T>
-__device__ __forceinline__ void dg_extract(DiagLds<T> &S, const typename MfmaAcc<T>::type (&a)[4], int G) {
+__device__ __forceinline__ void dg_extract(DiagLds<T> &S, const typename MfmaAcc<T>::type (&a)[4], int G, T (&c)[4]) {
     const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4, i = n - 4 * (G & 3);
     if (i >= 0 && i < 4) {
 #pragma unroll
@@ -1094,65 +1096,100 @@ __device__ __forceinline__ void dg_extract(DiagLds<T> &S, const typename MfmaAcc
             for (int q = 0; q < 4; ++q) S.pcol[i][16 * t + g + 4 * q] = a[t][q];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = S.pcol[j][lane];
 }
 
-// The owner wave of group G takes pivots k = 4G .. k+3 (lane r: row r of the four columns)
-// and publishes the group's operands into buffer b.  Pivots past nb (identity padding) are 1.
+// Pivot rule operands of one front: the float64 threshold and regularized value, and the same
+// rule in the working precision (FP32: cand > thr_t, thr_t the largest float <= thr, decides
+// exactly as (double)cand > thr).
 template <typename T>
-__device__ __forceinline__ void dg_factor_group(DiagLds<T> &S, int G, int b, int nb, double thr, double regv,
-                                                int allow_reg) {
+struct DgRule {
+    double thr, regv;
+    T thr_t, regv_t;
+    int allow_reg;
+    __device__ DgRule(double t, double r, int a) : thr(t), regv(r), regv_t((T)r), allow_reg(a) {
+        if constexpr (sizeof(T) == 4) {
+            float tf = (float)t;
+            if ((double)tf > t) tf = nextafterf(tf, -INFINITY);
+            thr_t = tf;
+        } else {
+            thr_t = t;
+        }
+    }
+};
+
+// The owner wave of group G takes pivots k = 4G .. k+3 (lane r: row r of the four columns in
+// c) and publishes the group's operands into buffer b.  A pivot past nb (identity padding) has
+// candidate 1 and a zero column: whatever the rule makes of it multiplies zeros.
+// The group's 4x4 diagonal block is factored first on wave-uniform values (the pivot-to-pivot
+// chain); the rows follow, l_r = a_rK / d_K and a_rc -= (l_r d_K) l_c, with the same operations
+// in the same order as the uniform values of rows k..k+3.
+template <typename T>
+__device__ __forceinline__ void dg_factor_group(DiagLds<T> &S, int G, int b, const DgRule<T> &R, T (&c)[4]) {
     const int r = threadIdx.x & 63, k = 4 * G;
-    T c[4], l[4], p[4][4];
+    T e[4][4];  // e[i2][i] = A[k + i2][k + i], i2 >= i (uniform)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) c[i] = S.pcol[i][r];
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int i2 = i; i2 < 4; ++i2) e[i2][i] = lane_value<T>(c[i], k + i2);
+    T pv[4], pinv[4], p[4][4], cand[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        cand[i] = e[i][i];
+        pv[i] = cand[i] > R.thr_t ? cand[i] : R.regv_t;  // padding: candidate 1, column zero
+        pinv[i] = pivot_recip(pv[i]);
+#pragma unroll
+        for (int i2 = i + 1; i2 < 4; ++i2) p[i2][i] = e[i2][i] * pinv[i];
+#pragma unroll
+        for (int i3 = i + 1; i3 < 4; ++i3) {
+            const T w3 = p[i3][i] * pv[i];
+#pragma unroll
+            for (int i2 = i + 1; i2 <= i3; ++i2) e[i3][i2] -= w3 * p[i2][i];
+        }
+    }
+    T l[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int K = k + i;
-        unsigned char fl = 0;
-        const T cand = lane_value<T>(c[i], K);
-        const T pv = K < nb ? (T)pivot_rule((double)cand, thr, regv, allow_reg, fl) : T(1);
-        const T pinv = pivot_recip(pv);
-        l[i] = r > K ? c[i] * pinv : T(0);
-        const T w = l[i] * pv;
+        l[i] = r > K ? c[i] * pinv[i] : T(0);
+        const T w = l[i] * pv[i];
         S.wn[b][i][r] = -w;
         S.lp[b][i][r] = l[i];
-        if (r == 0) {
-            S.dsh[K] = (double)pv;
-            S.fsh[K] = fl;
-        }
 #pragma unroll
-        for (int i2 = i + 1; i2 < 4; ++i2) {
-            p[i2][i] = lane_value<T>(l[i], k + i2);  // P[i2][i] = l of row k + i2
-            c[i2] -= w * p[i2][i];
-        }
+        for (int i2 = i + 1; i2 < 4; ++i2) c[i2] -= w * p[i2][i];
+    }
+    if (r < 4) {  // flags are formed from the candidates at the end (dg_pivots)
+        S.cand[k + r] = r == 0 ? cand[0] : (r == 1 ? cand[1] : (r == 2 ? cand[2] : cand[3]));
+        S.piv[k + r] = r == 0 ? pv[0] : (r == 1 ? pv[1] : (r == 2 ? pv[2] : pv[3]));
     }
     // Q = P^{-1}: Q[a][e] = -(P[a][e] + sum_{e<m<a} P[a][m] Q[m][e])
     T q[4][4];
 #pragma unroll
     for (int a = 1; a < 4; ++a)
 #pragma unroll
-        for (int e = a - 1; e >= 0; --e) {
-            T s = p[a][e];
+        for (int e2 = a - 1; e2 >= 0; --e2) {
+            T s = p[a][e2];
 #pragma unroll
-            for (int m = e + 1; m < a; ++m) s += p[a][m] * q[m][e];
-            q[a][e] = -s;
+            for (int m = e2 + 1; m < a; ++m) s += p[a][m] * q[m][e2];
+            q[a][e2] = -s;
         }
     // (L Q)[r][e] = l_e + sum_{a>e} l_a Q[a][e]
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        T m = l[e];
+    for (int e2 = 0; e2 < 4; ++e2) {
+        T m = l[e2];
 #pragma unroll
-        for (int a = e + 1; a < 4; ++a) m += l[a] * q[a][e];
-        S.mn[b][e][r] = -m;
+        for (int a = e2 + 1; a < 4; ++a) m += l[a] * q[a][e2];
+        S.mn[b][e2][r] = -m;
     }
 }
 
 // Groups 4 T0 .. 4 T0 + 3 of the diagonal block (the pivots of tile column T0); every wave
-// passes the same barriers.
+// passes the same barriers.  The owner of the next group takes it right after updating its
+// tiles and applies its inverse update last.
 template <typename T, int T0>
 __device__ __forceinline__ void dg_tile_column(DiagLds<T> &S, typename MfmaAcc<T>::type (&a)[4],
-                                               typename MfmaAcc<T>::type (&li)[4], int v, int nb, double thr,
-                                               double regv, int allow_reg) {
+                                               typename MfmaAcc<T>::type (&li)[4], int v, int nb, const DgRule<T> &R) {
     const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4;
 #pragma unroll 1
     for (int gp = 0; gp < 4; ++gp) {
@@ -1164,35 +1201,36 @@ __device__ __forceinline__ void dg_tile_column(DiagLds<T> &S, typename MfmaAcc<T
             for (int t = T0; t < 4; ++t)
                 if (t >= v) a[t] = MfmaAcc<T>::step(S.wn[b][g][16 * t + dg_row<T>(n)], bl, a[t]);
         }
+        const int G1 = G + 1;
+        if (G1 < kPanel / 4 && 4 * G1 < nb && v == G1 / 4) {
+            T c[4];
+            dg_extract<T>(S, a, G1, c);
+            dg_factor_group<T>(S, G1, b ^ 1, R, c);
+        }
         if (v <= T0) {  // inverse columns up to the group's last
             const T bli = gp == 0 ? li[T0][0] : (gp == 1 ? li[T0][1] : (gp == 2 ? li[T0][2] : li[T0][3]));
 #pragma unroll
             for (int t = T0; t < 4; ++t) li[t] = MfmaAcc<T>::step(S.mn[b][g][16 * t + dg_row<T>(n)], bli, li[t]);
-        }
-        const int G1 = G + 1;
-        if (G1 < kPanel / 4 && 4 * G1 < nb && v == G1 / 4) {
-            dg_extract<T>(S, a, G1);
-            dg_factor_group<T>(S, G1, b ^ 1, nb, thr, regv, allow_reg);
         }
         __syncthreads();
     }
 }
 
 // Factors the panel's diagonal block (nb columns, identity past nb) held in a, forming li.
-// Ends with S.dsh / S.fsh complete.
+// Ends with S.cand / S.piv complete.
 template <typename T>
 __device__ __forceinline__ void dg_factor(DiagLds<T> &S, typename MfmaAcc<T>::type (&a)[4],
-                                          typename MfmaAcc<T>::type (&li)[4], int v, int nb, double thr,
-                                          double regv, int allow_reg) {
+                                          typename MfmaAcc<T>::type (&li)[4], int v, int nb, const DgRule<T> &R) {
     if (v == 0) {
-        dg_extract<T>(S, a, 0);
-        dg_factor_group<T>(S, 0, 0, nb, thr, regv, allow_reg);
+        T c[4];
+        dg_extract<T>(S, a, 0, c);
+        dg_factor_group<T>(S, 0, 0, R, c);
     }
     __syncthreads();
-    dg_tile_column<T, 0>(S, a, li, v, nb, thr, regv, allow_reg);
-    if (nb > 16) dg_tile_column<T, 1>(S, a, li, v, nb, thr, regv, allow_reg);
-    if (nb > 32) dg_tile_column<T, 2>(S, a, li, v, nb, thr, regv, allow_reg);
-    if (nb > 48) dg_tile_column<T, 3>(S, a, li, v, nb, thr, regv, allow_reg);
+    dg_tile_column<T, 0>(S, a, li, v, nb, R);
+    if (nb > 16) dg_tile_column<T, 1>(S, a, li, v, nb, R);
+    if (nb > 32) dg_tile_column<T, 2>(S, a, li, v, nb, R);
+    if (nb > 48) dg_tile_column<T, 3>(S, a, li, v, nb, R);
 }
 
 // Row stride of the scaled inverse Bm in LDS: the MFMA operand reads (4 rows x 16
@@ -1263,7 +1301,7 @@ __device__ __forceinline__ void rows_product(T *__restrict__ M, int f, int ps, i
 // block into L rows: X := X B with B = L11^{-T} D^{-1}, B[m][k] = Li[k][m] / d_k
 // (Li from the diagonal block's upper triangle, Li[k][k] = 1).
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tasks, T *__restrict__ arena,
+__global__ __launch_bounds__(256, 2) void k_panel_rows(const int2 *__restrict__ tasks, T *__restrict__ arena,
                                                     const int *__restrict__ F, const int *__restrict__ W,
                                                     const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                     int ps, const double *__restrict__ D) {
@@ -1311,7 +1349,7 @@ __global__ __launch_bounds__(256) void k_panel_rows(const int2 *__restrict__ tas
 // row blocks, the Schur updates and the solves use Li and D; the host's factor export rebuilds
 // L11 from Li).
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T *__restrict__ arena,
+__global__ __launch_bounds__(256, 2) void k_panel(const int2 *__restrict__ tasks, T *__restrict__ arena,
                                                const int *__restrict__ F, const int *__restrict__ W,
                                                const int64_t *__restrict__ off, const int *__restrict__ firstcol,
                                                int ps, const double *__restrict__ scale,
@@ -1319,8 +1357,11 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
                                                int allow_reg, double *__restrict__ D,
                                                unsigned char *__restrict__ flags) {
     __shared__ DiagLds<T> S;
+    // The block's staging copy (stg[c][r], wave v's columns 16v..16v+15), later
     // Bm[m][k]: Li[k][m] / d_k for m < k, 1 / d_k for m = k, else 0 (the MFMA A operand)
-    __shared__ T Bm[kPanel][kBmLd<T>];
+    __shared__ __align__(16) T Bst[kPanel * kDgStage];
+    T (*stg)[kDgStage] = reinterpret_cast<T (*)[kDgStage]>(Bst);
+    T (*Bm)[kBmLd<T>] = reinterpret_cast<T (*)[kBmLd<T>]>(Bst);
     const int2 tk = tasks[blockIdx.x];
     const int s = tk.x, rb = tk.y;
     const int f = F[s], w = W[s];
@@ -1332,33 +1373,38 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
     T xb[kPanel / 4];
     if (nr > 0) rows_load<T>(xb, M, f, ps, nb, r0, nr);  // in flight during the factorization
     const double sc = scale[front_sys[s]];
-    // Block column c = 16v + n, rows 16t + g + 4q: lower triangle, identity past nb, zero above
-    // (addresses clamped into the block, masked after every load is issued).
+    // Block column c = 16v + n, rows 16t + g + 4q: lower triangle, identity past nb, zero above.
+    // Wave v loads its columns a column per instruction (lane = row; addresses clamped into the
+    // block) and turns them into its tiles through stg; masks applied on the way.
     typename MfmaAcc<T>::type a[4], li[4];
     const int c = 16 * v + n;
     {
-        const T *col = M + (int64_t)(ps + min(c, nb - 1)) * f + ps;
+        T x[kPanel / 4];
+        const T *col0 = M + (int64_t)ps * f + ps + min(lane, nb - 1);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int j = 0; j < kPanel / 4; ++j) x[j] = col0[(int64_t)min(16 * v + j, nb - 1) * f];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) a[t][q] = col[min(16 * t + g + 4 * q, nb - 1)];
+        for (int j = 0; j < kPanel / 4; ++j) stg[16 * v + j][lane] = x[j];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = 16 * t + g + 4 * q;
-                T x = a[t][q];
-                pin(x);
-                a[t][q] = (rr < nb && c < nb && rr >= c) ? x : (rr == c ? T(1) : T(0));
+                const T y = stg[c][rr];
+                a[t][q] = (rr < nb && c < nb && rr >= c) ? y : (rr == c ? T(1) : T(0));
                 li[t][q] = rr == c ? T(1) : T(0);
             }
     }
-    dg_factor<T>(S, a, li, v, nb, tol * sc, regf * sc, allow_reg);
+    const DgRule<T> R(tol * sc, regf * sc, allow_reg);
+    dg_factor<T>(S, a, li, v, nb, R);
     if (rb <= 0) {
         if (threadIdx.x < nb) {
             const int fc = firstcol[s] + ps;
-            D[fc + threadIdx.x] = S.dsh[threadIdx.x];
-            flags[fc + threadIdx.x] = S.fsh[threadIdx.x];
+            unsigned char fl;
+            (void)pivot_rule((double)S.cand[threadIdx.x], R.thr, R.regv, R.allow_reg, fl);
+            D[fc + threadIdx.x] = (double)S.piv[threadIdx.x];
+            flags[fc + threadIdx.x] = fl;
         }
         // Li[rr][c] at front row ps + c, column ps + rr
 #pragma unroll
@@ -1370,7 +1416,7 @@ __global__ __launch_bounds__(256) void k_panel(const int2 *__restrict__ tasks, T
             }
     }
     if (nr <= 0) return;  // uniform: the diagonal-block-only workgroup
-    if (threadIdx.x < kPanel) S.rd[threadIdx.x] = T(1) / (T)S.dsh[min((int)threadIdx.x, nb - 1)];
+    if (threadIdx.x < kPanel) S.rd[threadIdx.x] = T(1) / S.piv[min((int)threadIdx.x, nb - 1)];
     __syncthreads();
     // Bm[c][rr] = Li[rr][c] / d_rr
 #pragma unroll
