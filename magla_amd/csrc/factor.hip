@@ -1448,7 +1448,7 @@ __device__ __forceinline__ int xcd_task(int b, int n) {
 // front's end are not masked: they only reach output rows or columns >= f, which are not
 // stored.  Columns past the step's K (a narrow block's last chunk) are zeroed.
 template <typename T>
-__global__ __launch_bounds__(256, 4) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
+__global__ __launch_bounds__(256, 3) void k_schur(const SchurTask *__restrict__ tasks, T *__restrict__ arena, int q,
                                                const double *__restrict__ D) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];  // As[k][r] = L[i0+r, kc+k] * d_k
     __shared__ __align__(16) T Bs[kPanel][kSchurLd<T>];  // Bs[k][r] = L[j0+r, kc+k]
@@ -2279,7 +2279,7 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
 // distance d = a - b: X_ab reads only blocks of smaller distance, written by earlier launches.
 // The solves then take one dependent step per kSBlock columns instead of per 64.
 template <typename T>
-__global__ __launch_bounds__(256) void k_inv_blk(const int4 *__restrict__ tasks, T *__restrict__ arena,
+__global__ __launch_bounds__(256, 2) void k_inv_blk(const int4 *__restrict__ tasks, T *__restrict__ arena,
                                                  const int *__restrict__ F, const int *__restrict__ W,
                                                  const int64_t *__restrict__ off) {
     __shared__ __align__(16) T As[kPanel][kSchurLd<T>];
