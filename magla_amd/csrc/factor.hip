@@ -188,8 +188,10 @@ int plan_level_count(const DevicePlan &P) { return P.nlevels; }
 
 // Schur work after panel q of a front of width w and size f, blocked at three widths
 // (64-column panels in 256-column blocks in kOuter-column outer blocks):
-//  * inner (not the block's last panel): the block's remaining columns [pe, be), rows [pe, f),
-//    with this panel (K = 64);
+//  * inner (not the block's last panel): the next panel's columns [pe, pe + 64), rows [pe, f),
+//    with every panel of the block so far (K = pe - bs <= 192; left-looking inside the block:
+//    each column of the block is updated once, just before its panel, with the K of all the
+//    block's panels before it);
 //  * middle (the block's last panel, not the outer block's last): the outer block's remaining
 //    columns [be, oe), rows [be, f), with the block (K <= 256);
 //  * outer (the outer block's last panel): the trailing [oe, f)^2 with the whole outer block
@@ -210,7 +212,7 @@ __host__ __device__ inline SchurStep schur_step(int f, int w, int q) {
     st.inner = st.pe < st.be;
     st.middle = !st.inner && st.be < st.oe;
     if (st.inner) {
-        st.base = st.pe; st.kb = st.ps; st.ke = st.pe; st.cend = st.be;
+        st.base = st.pe; st.kb = st.bs; st.ke = st.pe; st.cend = st.pe + kPanel < st.be ? st.pe + kPanel : st.be;
     } else if (st.middle) {
         st.base = st.be; st.kb = st.bs; st.ke = st.be; st.cend = st.oe;
     } else {
