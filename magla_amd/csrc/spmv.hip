@@ -16,6 +16,8 @@
 // place as ConjugateGradientGaussPreSolver.cs:151-153).
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <memory>
 
 #include "device.h"
 #include "reduce.h"
@@ -365,8 +367,8 @@ __device__ __forceinline__ double pcg_dir(double z, double p, double beta) { ret
 // beta = 0 and p_old = 0 give the first direction p = z), Ap = A p_new and partial p'Ap.
 // rz of iteration it is kept in S->rz2[it & 1]: the kernel reads the other slot.  The rows'
 // first two entries per lane are loaded before the partial sums are formed, so the gather's
-// latency overlaps them.  After convergence the kernel still runs but changes nothing that
-// is read again (the step kernel returns at once).
+// latency overlaps them.  An iteration queued after convergence returns at once (its step
+// kernel too).
 template <int LANES>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
                                                       const double *__restrict__ v, int n, const double *__restrict__ z,
@@ -375,6 +377,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv(const int *__restrict__ rp
                                                       const double *__restrict__ part_rz, int np_step, int it,
                                                       double *__restrict__ part_pap, PcgScalars *__restrict__ S) {
     __shared__ double sm4[4];
+    if (S->done) return;  // an iteration queued past convergence (uniform)
     constexpr int kRows = kBS / LANES;  // rows per pass of the block
     const int sub = threadIdx.x % LANES;
     int row = blockIdx.x * kRows + (int)threadIdx.x / LANES;
@@ -439,6 +442,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
                                                       double *__restrict__ part_pap, PcgScalars *__restrict__ S,
                                                       int fcg, FcgState *__restrict__ fsp) {
     __shared__ double sm4[4];
+    if (S->done) return;  // an iteration queued past convergence (uniform)
     constexpr int kPre = 8;
     const int lane = threadIdx.x & 63;
     int slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -612,6 +616,45 @@ __global__ void k_pcg_last(const double *__restrict__ part_rr, int np_step, int 
 // grids of the two kernels (fixed per matrix, so the reductions are deterministic): the
 // direction kernel covers the rows in one pass up to 4096 workgroups, the step kernel's 256
 // workgroups leave 256 partials for the next direction kernel's workgroups to sum
+// Host polling of the iteration state without draining the stream: batch k of iterations ends
+// with a copy of the scalars into pinned slot k & 1 and an event; the host then waits for batch
+// k - 1's event only, so batch k keeps the GPU busy meanwhile.  One per host thread and device
+// (pinned memory and events are made once).
+struct ScalarPoll {
+    PcgScalars *h = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int pending = -1;
+    ScalarPoll() {
+        MG_HIP(hipHostMalloc(reinterpret_cast<void **>(&h), 2 * sizeof(PcgScalars), hipHostMallocDefault));
+        for (hipEvent_t &e : ev) MG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~ScalarPoll() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h) (void)hipHostFree(h);
+    }
+    static ScalarPoll &get() {
+        static thread_local std::map<int, std::unique_ptr<ScalarPoll>> polls;
+        int dev = 0;
+        MG_HIP(hipGetDevice(&dev));
+        auto &q = polls[dev];
+        if (!q) q.reset(new ScalarPoll());
+        q->pending = -1;
+        return *q;
+    }
+    // enqueues batch `batch`'s copy; true when the previous batch had ended with done set
+    bool post(const PcgScalars *S, int batch, hipStream_t st) {
+        const int slot = batch & 1;
+        MG_HIP(hipMemcpyAsync(h + slot, S, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
+        MG_HIP(hipEventRecord(ev[slot], st));
+        const int prev = pending;
+        pending = slot;
+        if (prev < 0) return false;
+        MG_HIP(hipEventSynchronize(ev[prev]));
+        return h[prev].done != 0;
+    }
+};
+
 static int pcg_dir_blocks(const Matrix &A) {
     return A.sell ? std::min(1024, (A.nslices + 3) / 4) : (int)std::min<int64_t>(1024, nblk((int64_t)A.n * A.lanes, kBS));
 }
@@ -682,17 +725,15 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
         const int check_every = 16;
         const int ndir = pcg_dir_blocks(A), nstep = pcg_step_blocks(n);
         double *p_old = pa.p, *p_new = pb.p;
+        ScalarPoll &poll = ScalarPoll::get();
         int it = 1;
         for (; it <= maxit; ++it) {
             launch_pcg_dir(A, z.p, p_old, p_new, ap.p, prr.p, prz.p, nstep, it, pdir.p, S.p, st);
             hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n,
                                use_jacobi, it, pdir.p, ndir, prr.p, prz.p, S.p, (const double *)nullptr);
             std::swap(p_old, p_new);
-            if (it % check_every == 0) {
-                MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
-                MG_HIP(hipStreamSynchronize(st));
-                if (h.done) break;
-            }
+            // iterations past convergence leave the state as it is (their kernels return at once)
+            if (it % check_every == 0 && poll.post(S.p, it / check_every, st)) break;
         }
         if (it > maxit) hipLaunchKernelGGL(k_pcg_last, dim3(1), bv, 0, st, prr.p, nstep, maxit, S.p);
         MG_HIP(hipGetLastError());
@@ -879,6 +920,7 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
         const int maxit = max_it > 0 ? max_it : n;
         const int check_every = 16;
         const int stall = stall_window > 0 ? stall_window : 50;
+        ScalarPoll &poll = ScalarPoll::get();
         bool fused = false;
         if (A.sell && A.mp && A.mp->nslices) {
             // four kernels per iteration: the direction (p = z + beta p in the gather, beta from
@@ -902,11 +944,7 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
                                     st))
                     throw Error(MG_E_DEVICE, "The learned factor has no sliced copy for the fused iteration.");
                 std::swap(p_old, p_new);
-                if (it % check_every == 0 || it == maxit) {
-                    MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
-                    MG_HIP(hipStreamSynchronize(st));
-                    if (h.done) break;
-                }
+                if (it % check_every == 0 && poll.post(S.p, it / check_every, st)) break;
             }
         }
         for (int it = 1; !fused && it <= maxit; ++it) {
