@@ -135,7 +135,8 @@ constexpr int kRedBlocks = 512;  // fixed grid of the deterministic vector reduc
 // Device scalars of a (flexible) PCG run; kernels of an iteration turn into no-ops once done.
 struct PcgScalars {
     double rz, pap, alpha, beta, rnorm, thr, bnorm, best;
-    double rz2[2];  // two-kernel PCG: r'z of iteration it in rz2[it & 1]
+    double rz2[2];  // r'z of iteration it in rz2[it & 1] (flexible PCG: two-kernel iteration)
+    double al2[2];  // single-reduction PCG: alpha of iteration it in al2[it & 1] (gamma in rz2)
     int done, iters, fail, fallback, restart, best_it, pad0, pad1;
 };
 

@@ -241,16 +241,6 @@ __global__ void k_pcg_init(const double *__restrict__ part_rr, const double *__r
     }
 }
 
-__global__ void k_pcg_rz0(const double *__restrict__ part, PcgScalars *__restrict__ S) {
-    __shared__ double sm[kBS];
-    if (S->done) return;
-    const double rz = finish_sum(part, sm);
-    if (threadIdx.x == 0) {
-        S->rz = rz;
-        S->rz2[1] = rz;  // iteration 1
-    }
-}
-
 __global__ void k_residual0(const double *__restrict__ b, const double *__restrict__ ax, int n, double *__restrict__ r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) r[i] = b[i] - ax[i];
@@ -322,22 +312,6 @@ __global__ void k_pcg_norm(const double *__restrict__ part, int it, PcgScalars *
     }
 }
 
-// z = M r and partial r'z  (:155-156)
-__global__ __launch_bounds__(256) void k_pcg_precond_dot(const double *__restrict__ invd, const double *__restrict__ r,
-                                                         double *__restrict__ z, int n, int use,
-                                                         double *__restrict__ part, const PcgScalars *__restrict__ S) {
-    __shared__ double sm[kBS];
-    if (S->done) return;
-    double acc = 0.0;
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        const double zi = use ? invd[i] * r[i] : r[i];
-        z[i] = zi;
-        acc += r[i] * zi;
-    }
-    const double t = block_sum(acc, sm);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
 // beta = rz_new / rz (:157), rz = rz_new (:162)
 __global__ void k_pcg_beta(const double *__restrict__ part, PcgScalars *__restrict__ S) {
     __shared__ double sm[kBS];
@@ -357,82 +331,24 @@ __global__ void k_pcg_pupdate(const double *__restrict__ z, double *__restrict__
     if (i < n) p[i] = z[i] + S->beta * p[i];
 }
 
+// S->done as the workgroup's first thread reads it: workgroup 0 of the same kernel may set it
+// meanwhile, and every thread of a workgroup must take the same branch.
+__device__ __forceinline__ bool pcg_done(const PcgScalars *__restrict__ S) {
+    __shared__ int done;
+    if (threadIdx.x == 0) done = S->done;
+    __syncthreads();
+    return done != 0;
+}
+
 // the direction entry p_new = z + beta p_old, formed identically by its row's owner and by
 // every row that gathers it
 __device__ __forceinline__ double pcg_dir(double z, double p, double beta) { return __fma_rn(beta, p, z); }
 
-// Iteration `it`, first kernel.  For it > 1 it finishes iteration it - 1: ||r|| from the
-// previous step's partials and the convergence test (:151-153; iterations = it - 1), then
-// rz_new = r'z and beta = rz_new / rz (:157, :162).  Then p_new = z + beta p_old (:159-160;
-// beta = 0 and p_old = 0 give the first direction p = z), Ap = A p_new and partial p'Ap.
-// rz of iteration it is kept in S->rz2[it & 1]: the kernel reads the other slot.  The rows'
-// first two entries per lane are loaded before the partial sums are formed, so the gather's
-// latency overlaps them.  An iteration queued after convergence returns at once (its step
-// kernel too).
-template <int LANES>
-__global__ __launch_bounds__(256) void k_pcg_dir_spmv(const int *__restrict__ rp, const int *__restrict__ ci,
-                                                      const double *__restrict__ v, int n, const double *__restrict__ z,
-                                                      const double *__restrict__ p_old, double *__restrict__ p_new,
-                                                      double *__restrict__ ap, const double *__restrict__ part_rr,
-                                                      const double *__restrict__ part_rz, int np_step, int it,
-                                                      double *__restrict__ part_pap, PcgScalars *__restrict__ S) {
-    __shared__ double sm4[4];
-    if (S->done) return;  // an iteration queued past convergence (uniform)
-    constexpr int kRows = kBS / LANES;  // rows per pass of the block
-    const int sub = threadIdx.x % LANES;
-    int row = blockIdx.x * kRows + (int)threadIdx.x / LANES;
-    // first pass: this lane's first two entries and the row's own direction entry
-    const int r0 = min(row, n - 1);
-    const int kb = rp[r0], ke = rp[r0 + 1];
-    const int ka = kb + sub, kc = ka + LANES;
-    const bool ha = row < n && ka < ke, hc = row < n && kc < ke;
-    const int ca = ci[ha ? ka : kb], cc = ci[hc ? kc : kb];
-    const double va = ha ? v[ka] : 0.0, vc = hc ? v[kc] : 0.0;
-    const double za = z[ca], pa = p_old[ca], zc = z[cc], pc = p_old[cc], zr = z[r0], pr = p_old[r0];
-    double beta = 0.0;
-    if (it > 1) {
-        const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
-        const double rz = sum_partials(part_rz, np_step, sm4);
-        const bool conv = rnorm <= S->thr;
-        beta = rz / S->rz2[(it - 1) & 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0 && !S->done) {
-            S->rnorm = rnorm;
-            S->iters = it - 1;
-            if (conv) S->done = 1;
-            else S->rz2[it & 1] = rz;
-        }
-    }
-    double dot = 0.0;
-    for (bool first = true; row < n; row += gridDim.x * kRows, first = false) {
-        double acc;
-        double pn;
-        if (first) {
-            acc = va * pcg_dir(za, pa, beta) + vc * pcg_dir(zc, pc, beta);
-            for (int k = kc + LANES; k < ke; k += LANES) acc += v[k] * pcg_dir(z[ci[k]], p_old[ci[k]], beta);
-            pn = pcg_dir(zr, pr, beta);
-        } else {
-            acc = 0.0;
-            for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
-                const int c = ci[k];
-                acc += v[k] * pcg_dir(z[c], p_old[c], beta);
-            }
-            pn = pcg_dir(z[row], p_old[row], beta);
-        }
-        for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
-        if (sub == 0) {
-            p_new[row] = pn;
-            ap[row] = acc;
-            dot += pn * acc;
-        }
-    }
-    const double t = block_sum_waves(dot, sm4);
-    if (threadIdx.x == 0) part_pap[blockIdx.x] = t;
-}
-
-// The same first kernel on the sliced ELL copy: wave = 64-row slice, lane = row; the first
-// kPre entries of each row are loaded before the partial sums are formed.  With fcg set
-// (the flexible PCG of the learned preconditioner) beta comes from S->beta, set by the
-// previous iteration's kernels, and the convergence test is theirs.
+// Flexible PCG (the learned preconditioner), iteration `it`, first kernel on the sliced ELL copy
+// (wave = 64-row slice, lane = row): p_new = z + beta p_old in the gather (beta from the
+// previous iteration's partials; beta = 0 and p_old = 0 give the first direction p = z),
+// Ap = A p_new and partial p'Ap.  The first kPre entries of each row are loaded before the
+// partial sums are formed.  The convergence test is the preconditioner kernels'.
 __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp, const int *__restrict__ sci,
                                                       const double *__restrict__ sv, int n, int nslices,
                                                       const double *__restrict__ z, const double *__restrict__ p_old,
@@ -440,9 +356,9 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
                                                       const double *__restrict__ part_rr,
                                                       const double *__restrict__ part_rz, int np_step, int it,
                                                       double *__restrict__ part_pap, PcgScalars *__restrict__ S,
-                                                      int fcg, FcgState *__restrict__ fsp) {
+                                                      FcgState *__restrict__ fsp) {
     __shared__ double sm4[4];
-    if (S->done) return;  // an iteration queued past convergence (uniform)
+    if (pcg_done(S)) return;  // an iteration queued past convergence
     constexpr int kPre = 8;
     const int lane = threadIdx.x & 63;
     int slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -464,7 +380,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
     }
     const double zr = z[r0], pr = p_old[r0];
     double beta = 0.0;
-    if (fcg) {
+    {
         // the flexible direction update of iteration it - 1 (part_rr / part_rz: its r'z and
         // z'Ap partials): beta = -(z'Ap) / (p'Ap), or 0 on a restart or a non-positive r'z
         // (which also falls back to Jacobi); workgroup 0 completes the state slot
@@ -489,17 +405,6 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
                 fsp->restart2 = rs;
                 S->fallback = fb;
             }
-        }
-    } else if (it > 1) {
-        const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
-        const double rz = sum_partials(part_rz, np_step, sm4);
-        const bool conv = rnorm <= S->thr;
-        beta = rz / S->rz2[(it - 1) & 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0 && !S->done) {
-            S->rnorm = rnorm;
-            S->iters = it - 1;
-            if (conv) S->done = 1;
-            else S->rz2[it & 1] = rz;
         }
     }
     double dot = 0.0;
@@ -601,11 +506,236 @@ __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double
     }
 }
 
-// after the last iteration: its residual norm and convergence test
-__global__ void k_pcg_last(const double *__restrict__ part_rr, int np_step, int it, PcgScalars *__restrict__ S) {
-    __shared__ double sm4[4];
-    if (S->done) return;
-    const double rnorm = sqrt(sum_partials(part_rr, np_step, sm4));
+// grids of the two kernels (fixed per matrix, so the reductions are deterministic): the
+// direction kernel covers the rows in one pass up to 4096 workgroups, the step kernel's 256
+// workgroups leave 256 partials for the next direction kernel's workgroups to sum
+// ---- single-reduction Jacobi-PCG (one kernel per iteration) --------------------------
+//
+// ConjugateGradientGaussPreSolver's PCG (CGGP/ConjugateGradientGaussPreSolver.cs:78-166) in the
+// Chronopoulos-Gear form: the same iterates in exact arithmetic, with one reduction per
+// iteration.  With u = M r and w = A u, gamma = r'u and delta = w'u give both scalars:
+// beta_i = gamma_i / gamma_{i-1}, and p_i'A p_i = delta_i - beta_i gamma_i / alpha_{i-1}
+// (the reference's p'Ap <= 0 failure is tested on it), alpha_i = gamma_i / p_i'A p_i.  The
+// directions carry s = A p by the same recurrence as p (p_i = u_i + beta_i p_{i-1},
+// s_i = w_i + beta_i s_{i-1}), so x_{i+1} = x_i + alpha_i p_i and r_{i+1} = r_i - alpha_i s_i
+// need no product with A of their own.  Kernel i updates its rows and forms w_{i+1} = A u_{i+1}:
+// each neighbour's u_{i+1} is recomputed from iteration i's r, w and s (which alternate between
+// two buffers, so no row is overwritten while another workgroup reads it), by the same
+// operations as the neighbour's own row, so both agree bit for bit.  Its partials of gamma_{i+1},
+// delta_{i+1} and ||r_{i+1}||^2 go to the next kernel, whose every workgroup sums them in a fixed
+// order; that kernel also makes iteration i + 1's convergence test (:151-153).
+struct Cg1Vec {
+    const double *r, *w, *s;  // iteration i: r_i, w_i, s_{i-1}
+    double *r1, *w1, *s1;     // written: r_{i+1}, w_{i+1}, s_i
+};
+__device__ __forceinline__ double cg1_u(double r, double invd, int use) { return use ? invd * r : r; }
+// u_{i+1} of row c from iteration i's values
+__device__ __forceinline__ double cg1_u_next(double r, double w, double s, double invd, double alpha, double beta,
+                                             int use) {
+    return cg1_u(__fma_rn(-alpha, __fma_rn(beta, s, w), r), invd, use);
+}
+
+// Block sums of three values at once (one pass of wave reductions, two barriers), in a fixed
+// order; sm12 holds 12 doubles.
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *sm12) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sm12[wv] = a;
+        sm12[4 + wv] = b;
+        sm12[8 + wv] = c;
+    }
+    __syncthreads();
+    a = (sm12[0] + sm12[1]) + (sm12[2] + sm12[3]);
+    b = (sm12[4] + sm12[5]) + (sm12[6] + sm12[7]);
+    c = (sm12[8] + sm12[9]) + (sm12[10] + sm12[11]);
+    __syncthreads();
+}
+// sums of the three partial arrays part[0..np), [np..2np), [2np..3np) in one pass
+__device__ __forceinline__ void sum_partials3(const double *__restrict__ part, int np, double *sm12, double &a,
+                                              double &b, double &c) {
+    a = b = c = 0.0;
+    for (int i = threadIdx.x; i < np; i += kBS) {
+        a += part[i];
+        b += part[np + i];
+        c += part[2 * np + i];
+    }
+    block_sum3(a, b, c, sm12);
+}
+
+// Scalars of kernel `it` (iteration i = it - 1; it = 0 is the start pass, w_0 = A M r_0, with
+// alpha = beta = 0) from the previous kernel's partials part[0..np) (gamma), [np..2np) (delta),
+// [2np..3np) (||r||^2).  False when the kernel does nothing: done before, iteration it - 1
+// converged, or p'Ap <= 0.  Workgroup 0 records the state.
+__device__ bool cg1_scalars(const double *__restrict__ part, int np, int it, PcgScalars *__restrict__ S,
+                            double *sm12, double &alpha, double &beta) {
+    alpha = beta = 0.0;
+    if (it == 0) return true;
+    double gamma, delta, rho;
+    sum_partials3(part, np, sm12, gamma, delta, rho);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    double pap = delta;  // p_0 = u_0
+    if (it > 1) {
+        const double rnorm = sqrt(rho);
+        if (rnorm <= S->thr) {
+            if (lead) {
+                S->rnorm = rnorm;
+                S->iters = it - 1;
+                S->done = 1;
+            }
+            return false;
+        }
+        if (lead) {
+            S->rnorm = rnorm;
+            S->iters = it - 1;
+        }
+        beta = gamma / S->rz2[(it - 1) & 1];
+        pap = delta - beta * gamma / S->al2[(it - 1) & 1];
+    }
+    if (pap <= 0.0) {
+        if (lead) {
+            S->pap = pap;
+            S->fail = 1;
+            S->done = 1;
+        }
+        return false;
+    }
+    alpha = gamma / pap;
+    if (lead) {
+        S->rz2[it & 1] = gamma;
+        S->al2[it & 1] = alpha;
+    }
+    return true;
+}
+
+// A row's own update; returns its u_{i+1} and adds its terms to the partials.
+// the row's own values of iteration i
+struct Cg1Own {
+    double r, w, s, d, p, x;
+};
+__device__ __forceinline__ Cg1Own cg1_own(int row, const Cg1Vec &v, const double *__restrict__ invd, int use,
+                                          const double *__restrict__ x, const double *__restrict__ p) {
+    return Cg1Own{v.r[row], v.w[row], v.s[row], use ? invd[row] : 1.0, p[row], x[row]};
+}
+__device__ __forceinline__ void cg1_row(int row, double aw, const Cg1Own &o, const Cg1Vec &v, int use,
+                                        double *__restrict__ x, double *__restrict__ p, double alpha, double beta,
+                                        double &g, double &dl, double &rr) {
+    const double r = o.r, d = o.d;
+    const double sn = __fma_rn(beta, o.s, o.w);
+    const double pn = __fma_rn(beta, o.p, cg1_u(r, d, use));
+    const double r1 = __fma_rn(-alpha, sn, r);
+    const double u1 = cg1_u(r1, d, use);
+    x[row] = __fma_rn(alpha, pn, o.x);
+    p[row] = pn;
+    v.s1[row] = sn;
+    v.r1[row] = r1;
+    v.w1[row] = aw;
+    g += r1 * u1;
+    dl += aw * u1;
+    rr += r1 * r1;
+}
+
+__device__ __forceinline__ void cg1_partials(double g, double dl, double rr, double *__restrict__ part_out,
+                                             double *sm12) {
+    block_sum3(g, dl, rr, sm12);
+    if (threadIdx.x == 0) {
+        part_out[blockIdx.x] = g;
+        part_out[gridDim.x + blockIdx.x] = dl;
+        part_out[2 * gridDim.x + blockIdx.x] = rr;
+    }
+}
+
+// sliced ELL: wave = 64-row slice, lane = row
+__global__ __launch_bounds__(256) void k_pcg_cg1_sell(const int *__restrict__ sp, const int *__restrict__ sci,
+                                                      const double *__restrict__ sv, int n, int nslices,
+                                                      const double *__restrict__ invd, int use, double *__restrict__ x,
+                                                      double *__restrict__ p, Cg1Vec v,
+                                                      const double *__restrict__ part_in, int np_in,
+                                                      double *__restrict__ part_out, int it,
+                                                      PcgScalars *__restrict__ S) {
+    __shared__ double sm12[12];
+    if (pcg_done(S)) return;  // uniform
+    // The first slice's first kPre entries and the values they gather are loaded before the
+    // scalars are summed (none of them depends on alpha or beta), so their latency overlaps it.
+    constexpr int kPre = 8;
+    const int lane = threadIdx.x & 63;
+    int slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int s0 = min(slice, nslices - 1);
+    const int base0 = sp[s0], w0 = slice < nslices ? (sp[s0 + 1] - base0) >> 6 : 0;
+    double vpre[kPre], rpre[kPre], wpre[kPre], spre[kPre], dpre[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+        const int e = base0 + 64 * min(k, max(w0 - 1, 0)) + lane;
+        const int c = sci[e];
+        vpre[k] = k < w0 ? sv[e] : 0.0;
+        rpre[k] = v.r[c];
+        wpre[k] = v.w[c];
+        spre[k] = v.s[c];
+        dpre[k] = use ? invd[c] : 1.0;
+    }
+    double alpha, beta;
+    if (!cg1_scalars(part_in, np_in, it, S, sm12, alpha, beta)) return;  // uniform
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (bool first = true; slice < nslices; slice += gridDim.x * 4, first = false) {
+        const int row = 64 * slice + lane;
+        const int base = first ? base0 : sp[slice], w = first ? w0 : (sp[slice + 1] - base) >> 6;
+        double acc = 0.0;
+        int k = 0;
+        if (first) {
+#pragma unroll
+            for (int q = 0; q < kPre; ++q)
+                acc += vpre[q] * cg1_u_next(rpre[q], wpre[q], spre[q], dpre[q], alpha, beta, use);
+            k = kPre;
+        }
+        for (; k < w; ++k) {
+            const int e = base + 64 * k + lane;
+            const int c = sci[e];
+            acc += sv[e] * cg1_u_next(v.r[c], v.w[c], v.s[c], use ? invd[c] : 1.0, alpha, beta, use);
+        }
+        if (row < n) cg1_row(row, acc, cg1_own(row, v, invd, use, x, p), v, use, x, p, alpha, beta, g, dl, rr);
+    }
+    cg1_partials(g, dl, rr, part_out, sm12);
+}
+
+// CSR: LANES lanes per row
+template <int LANES>
+__global__ __launch_bounds__(256) void k_pcg_cg1_csr(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const double *__restrict__ val, int n,
+                                                     const double *__restrict__ invd, int use, double *__restrict__ x,
+                                                     double *__restrict__ p, Cg1Vec v,
+                                                     const double *__restrict__ part_in, int np_in,
+                                                     double *__restrict__ part_out, int it,
+                                                     PcgScalars *__restrict__ S) {
+    __shared__ double sm12[12];
+    if (pcg_done(S)) return;  // uniform
+    double alpha, beta;
+    if (!cg1_scalars(part_in, np_in, it, S, sm12, alpha, beta)) return;  // uniform
+    constexpr int kRows = kBS / LANES;
+    const int sub = threadIdx.x % LANES;
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int row = blockIdx.x * kRows + (int)threadIdx.x / LANES; row < n; row += gridDim.x * kRows) {
+        double acc = 0.0;
+        for (int k = rp[row] + sub; k < rp[row + 1]; k += LANES) {
+            const int c = ci[k];
+            acc += val[k] * cg1_u_next(v.r[c], v.w[c], v.s[c], use ? invd[c] : 1.0, alpha, beta, use);
+        }
+        for (int o = LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LANES);
+        if (sub == 0) cg1_row(row, acc, cg1_own(row, v, invd, use, x, p), v, use, x, p, alpha, beta, g, dl, rr);
+    }
+    cg1_partials(g, dl, rr, part_out, sm12);
+}
+
+// after the last iteration: its convergence test
+__global__ void k_pcg_cg1_last(const double *__restrict__ part, int np, int it, PcgScalars *__restrict__ S) {
+    __shared__ double sm12[12];
+    if (pcg_done(S)) return;
+    double gamma, delta, rho;
+    sum_partials3(part, np, sm12, gamma, delta, rho);
+    const double rnorm = sqrt(rho);
     if (threadIdx.x == 0) {
         S->rnorm = rnorm;
         S->iters = it;
@@ -613,9 +743,6 @@ __global__ void k_pcg_last(const double *__restrict__ part_rr, int np_step, int 
     }
 }
 
-// grids of the two kernels (fixed per matrix, so the reductions are deterministic): the
-// direction kernel covers the rows in one pass up to 4096 workgroups, the step kernel's 256
-// workgroups leave 256 partials for the next direction kernel's workgroups to sum
 // Host polling of the iteration state without draining the stream: batch k of iterations ends
 // with a copy of the scalars into pinned slot k & 1 and an event; the host then waits for batch
 // k - 1's event only, so batch k keeps the GPU busy meanwhile.  One per host thread and device
@@ -660,24 +787,6 @@ static int pcg_dir_blocks(const Matrix &A) {
 }
 static int pcg_step_blocks(int n) { return (int)std::min<int64_t>(512, nblk(n, kBS)); }
 
-static void launch_pcg_dir(const Matrix &A, const double *z, const double *p_old, double *p_new, double *ap,
-                           const double *part_rr, const double *part_rz, int np_step, int it, double *part,
-                           PcgScalars *S, hipStream_t st) {
-    const dim3 g(pcg_dir_blocks(A)), b(kBS);
-    if (A.sell) {
-        hipLaunchKernelGGL(k_pcg_dir_sell, g, b, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, A.n, A.nslices, z,
-                           p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S, 0, (FcgState *)nullptr);
-        return;
-    }
-    switch (A.lanes) {
-        case 4: hipLaunchKernelGGL(k_pcg_dir_spmv<4>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
-        case 8: hipLaunchKernelGGL(k_pcg_dir_spmv<8>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
-        case 16: hipLaunchKernelGGL(k_pcg_dir_spmv<16>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
-        case 32: hipLaunchKernelGGL(k_pcg_dir_spmv<32>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
-        default: hipLaunchKernelGGL(k_pcg_dir_spmv<64>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, A.n, z, p_old, p_new, ap, part_rr, part_rz, np_step, it, part, S); break;
-    }
-}
-
 static void launch_spmv_guard(const Matrix &A, const double *x, double *y, const PcgScalars *S, hipStream_t st) {
     const int64_t threads = (int64_t)A.n * A.lanes;
     const dim3 g(nblk(threads, kBS)), b(kBS);
@@ -700,8 +809,7 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
     if (n == 0) return;
     if (d_x0) MG_HIP(hipMemcpyAsync(d_x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     else MG_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-    DevArray<double> r(n), z(n), pa(n), pb(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
-    DevArray<double> pdir(pcg_dir_blocks(A)), prr(pcg_step_blocks(n)), prz(pcg_step_blocks(n));
+    DevArray<double> r(n), ap(n), part(kRedBlocks), part2(kRedBlocks);
     DevArray<PcgScalars> S(1);
     MG_HIP(hipMemsetAsync(S.p, 0, sizeof(PcgScalars), st));
     const dim3 gv(nblk(n, kBS)), bv(kBS), gr(kRedBlocks);
@@ -717,25 +825,49 @@ void pcg_device(const Matrix &A, const double *d_b, const double *d_x0, double t
     if (!h.done) {
         if (use_jacobi && A.diag_status)
             throw Error(MG_E_NOT_PD, "The diagonal element is zero or invalid while building the Jacobi preconditioner.");
-        // z0 = M r0, rz = r0'z0 (:119-121); the first direction comes from beta = 0, p_old = 0
-        hipLaunchKernelGGL(k_pcg_precond_dot, gr, bv, 0, st, A.invdiag.p, r.p, z.p, n, use_jacobi, part.p, S.p);
-        hipLaunchKernelGGL(k_pcg_rz0, dim3(1), bv, 0, st, part.p, S.p);
-        MG_HIP(hipMemsetAsync(pa.p, 0, sizeof(double) * n, st));
+        // single-reduction iteration: the start pass (kernel 0) forms w_0 = A M r_0 and the
+        // partials of gamma_0 and delta_0; kernel it runs iteration it - 1 of the recurrences and
+        // tests iteration it - 1's residual.  The iterate vectors alternate between two halves.
+        DevArray<double> r2(n), w(2 * (size_t)n), s2(2 * (size_t)n), p(n);
+        MG_HIP(hipMemsetAsync(w.p, 0, sizeof(double) * n, st));   // w_0 slot: zero before the start pass
+        MG_HIP(hipMemsetAsync(s2.p, 0, sizeof(double) * n, st));  // s_{-1} = 0
+        MG_HIP(hipMemsetAsync(p.p, 0, sizeof(double) * n, st));   // p_{-1} = 0
+        const double *rb[2] = {r.p, r2.p}, *wb[2] = {w.p, w.p + n}, *sb[2] = {s2.p, s2.p + n};
+        const int ng = pcg_dir_blocks(A);
+        DevArray<double> cpart(2 * 3 * (size_t)ng);
         const int maxit = max_it > 0 ? max_it : n;
         const int check_every = 16;
-        const int ndir = pcg_dir_blocks(A), nstep = pcg_step_blocks(n);
-        double *p_old = pa.p, *p_new = pb.p;
         ScalarPoll &poll = ScalarPoll::get();
+        auto launch = [&](int k) {
+            const int a = k & 1, b1 = a ^ 1;
+            const Cg1Vec v{rb[a], wb[a], sb[a], const_cast<double *>(rb[b1]), const_cast<double *>(wb[b1]),
+                           const_cast<double *>(sb[b1])};
+            const double *pin = cpart.p + 3 * (size_t)ng * (b1);
+            double *pout = cpart.p + 3 * (size_t)ng * a;
+            const dim3 g(ng), b(kBS);
+            if (A.sell) {
+                hipLaunchKernelGGL(k_pcg_cg1_sell, g, b, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, n, A.nslices,
+                                   A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p);
+                return;
+            }
+            switch (A.lanes) {
+                case 4: hipLaunchKernelGGL(k_pcg_cg1_csr<4>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p); break;
+                case 8: hipLaunchKernelGGL(k_pcg_cg1_csr<8>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p); break;
+                case 16: hipLaunchKernelGGL(k_pcg_cg1_csr<16>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p); break;
+                case 32: hipLaunchKernelGGL(k_pcg_cg1_csr<32>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p); break;
+                default: hipLaunchKernelGGL(k_pcg_cg1_csr<64>, g, b, 0, st, A.rowptr.p, A.colidx.p, A.vals.p, n, A.invdiag.p, use_jacobi, d_x, p.p, v, pin, ng, pout, k, S.p); break;
+            }
+        };
+        launch(0);
         int it = 1;
         for (; it <= maxit; ++it) {
-            launch_pcg_dir(A, z.p, p_old, p_new, ap.p, prr.p, prz.p, nstep, it, pdir.p, S.p, st);
-            hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n,
-                               use_jacobi, it, pdir.p, ndir, prr.p, prz.p, S.p, (const double *)nullptr);
-            std::swap(p_old, p_new);
+            launch(it);
             // iterations past convergence leave the state as it is (their kernels return at once)
             if (it % check_every == 0 && poll.post(S.p, it / check_every, st)) break;
         }
-        if (it > maxit) hipLaunchKernelGGL(k_pcg_last, dim3(1), bv, 0, st, prr.p, nstep, maxit, S.p);
+        if (it > maxit)
+            hipLaunchKernelGGL(k_pcg_cg1_last, dim3(1), bv, 0, st, cpart.p + 3 * (size_t)ng * (maxit & 1), ng, maxit,
+                               S.p);
         MG_HIP(hipGetLastError());
         MG_HIP(hipMemcpyAsync(&h, S.p, sizeof(h), hipMemcpyDeviceToHost, st));
         MG_HIP(hipStreamSynchronize(st));
@@ -937,7 +1069,7 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
             for (int it = 1; it <= maxit; ++it) {
                 FcgState *prev = fs.p + ((it - 1) & 1), *cur = fs.p + (it & 1);
                 hipLaunchKernelGGL(k_pcg_dir_sell, dim3(ndir), bv, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, n,
-                                   A.nslices, z.p, p_old, p_new, ap.p, part.p, part2.p, nfcg, it, pdir.p, S.p, 1, prev);
+                                   A.nslices, z.p, p_old, p_new, ap.p, part.p, part2.p, nfcg, it, pdir.p, S.p, prev);
                 hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n, 2,
                                    it, pdir.p, ndir, prr.p, (double *)nullptr, S.p, (const double *)&prev->rz2);
                 if (!mpnn_fcg_apply(net, A, r.p, z.p, ap.p, prr.p, nstep, it, stall, prev, cur, S.p, part.p, part2.p,
