@@ -434,20 +434,19 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
     if (threadIdx.x == 0) part_pap[blockIdx.x] = t;
 }
 
-// Iteration `it`, second kernel: alpha = rz / p'Ap (:137-143; p'Ap <= 0 fails), x += alpha p,
-// r -= alpha Ap (:145-150), z = M r (:155), partial ||r||^2 and r'z.  Each thread's first
-// kPre elements are loaded before the partial sum of p'Ap is formed.  use: 1 Jacobi, 0 none,
-// 2 none and no z (the flexible PCG forms z itself; rz from *rzp, p'Ap and alpha kept in S).
+// Flexible PCG, iteration `it`, second kernel: alpha = rz / p'Ap (:137-143; p'Ap <= 0 fails;
+// rz from *rzp), x += alpha p, r -= alpha Ap (:145-150), partial ||r||^2; p'Ap and alpha are
+// kept in S.  Each thread's first kPre elements are loaded before the partial sum of p'Ap is
+// formed.
 __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double *__restrict__ r,
-                                                  double *__restrict__ z, const double *__restrict__ p,
-                                                  const double *__restrict__ ap, const double *__restrict__ invd, int n,
-                                                  int use, int it, const double *__restrict__ part_pap, int np_dir,
-                                                  double *__restrict__ part_rr, double *__restrict__ part_rz,
-                                                  PcgScalars *__restrict__ S, const double *__restrict__ rzp) {
+                                                  const double *__restrict__ p, const double *__restrict__ ap, int n,
+                                                  const double *__restrict__ part_pap, int np_dir,
+                                                  double *__restrict__ part_rr, PcgScalars *__restrict__ S,
+                                                  const double *__restrict__ rzp) {
     __shared__ double sm4[4];
     constexpr int kPre = 4;
     const int stride = gridDim.x * kBS, i0 = blockIdx.x * kBS + threadIdx.x;
-    double px[kPre], pp[kPre], pr[kPre], pa[kPre], pd[kPre];
+    double px[kPre], pp[kPre], pr[kPre], pa[kPre];
 #pragma unroll
     for (int e = 0; e < kPre; ++e) {
         const int i = min(i0 + e * stride, n - 1);
@@ -455,7 +454,6 @@ __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double
         pp[e] = p[i];
         pr[e] = r[i];
         pa[e] = ap[i];
-        pd[e] = invd[i];
     }
     const double pap = sum_partials(part_pap, np_dir, sm4);
     if (S->done) return;  // uniform
@@ -467,12 +465,12 @@ __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double
         }
         return;
     }
-    const double alpha = (use == 2 ? *rzp : S->rz2[it & 1]) / pap;
-    if (use == 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+    const double alpha = *rzp / pap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         S->pap = pap;
         S->alpha = alpha;
     }
-    double rr = 0.0, rz = 0.0;
+    double rr = 0.0;
 #pragma unroll
     for (int e = 0; e < kPre; ++e) {
         const int i = i0 + e * stride;
@@ -481,11 +479,6 @@ __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double
             const double ri = pr[e] - alpha * pa[e];
             r[i] = ri;
             rr += ri * ri;
-            if (use != 2) {
-                const double zi = use ? pd[e] * ri : ri;
-                z[i] = zi;
-                rz += ri * zi;
-            }
         }
     }
     for (int i = i0 + kPre * stride; i < n; i += stride) {
@@ -493,22 +486,11 @@ __global__ __launch_bounds__(256) void k_pcg_step(double *__restrict__ x, double
         const double ri = r[i] - alpha * ap[i];
         r[i] = ri;
         rr += ri * ri;
-        if (use != 2) {
-            const double zi = use ? invd[i] * ri : ri;
-            z[i] = zi;
-            rz += ri * zi;
-        }
     }
-    const double t0 = block_sum_waves(rr, sm4), t1 = block_sum_waves(rz, sm4);
-    if (threadIdx.x == 0) {
-        part_rr[blockIdx.x] = t0;
-        if (use != 2) part_rz[blockIdx.x] = t1;
-    }
+    const double t0 = block_sum_waves(rr, sm4);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = t0;
 }
 
-// grids of the two kernels (fixed per matrix, so the reductions are deterministic): the
-// direction kernel covers the rows in one pass up to 4096 workgroups, the step kernel's 256
-// workgroups leave 256 partials for the next direction kernel's workgroups to sum
 // ---- single-reduction Jacobi-PCG (one kernel per iteration) --------------------------
 //
 // ConjugateGradientGaussPreSolver's PCG (CGGP/ConjugateGradientGaussPreSolver.cs:78-166) in the
@@ -782,6 +764,9 @@ struct ScalarPoll {
     }
 };
 
+// grids (fixed per matrix, so the reductions are deterministic): the direction kernels cover
+// the rows in one pass up to 4096 workgroups; the flexible PCG's step kernel's workgroups leave
+// their partials for the next direction kernel's workgroups to sum
 static int pcg_dir_blocks(const Matrix &A) {
     return A.sell ? std::min(1024, (A.nslices + 3) / 4) : (int)std::min<int64_t>(1024, nblk((int64_t)A.n * A.lanes, kBS));
 }
@@ -1070,8 +1055,8 @@ void fcg_learned_device(Matrix &A, Mpnn &net, const double *d_b, const double *d
                 FcgState *prev = fs.p + ((it - 1) & 1), *cur = fs.p + (it & 1);
                 hipLaunchKernelGGL(k_pcg_dir_sell, dim3(ndir), bv, 0, st, A.sell_ptr.p, A.sell_ci.p, A.sell_vals.p, n,
                                    A.nslices, z.p, p_old, p_new, ap.p, part.p, part2.p, nfcg, it, pdir.p, S.p, prev);
-                hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, z.p, p_new, ap.p, A.invdiag.p, n, 2,
-                                   it, pdir.p, ndir, prr.p, (double *)nullptr, S.p, (const double *)&prev->rz2);
+                hipLaunchKernelGGL(k_pcg_step, dim3(nstep), bv, 0, st, d_x, r.p, p_new, ap.p, n, pdir.p, ndir, prr.p,
+                                   S.p, (const double *)&prev->rz2);
                 if (!mpnn_fcg_apply(net, A, r.p, z.p, ap.p, prr.p, nstep, it, stall, prev, cur, S.p, part.p, part2.p,
                                     st))
                     throw Error(MG_E_DEVICE, "The learned factor has no sliced copy for the fused iteration.");
