@@ -933,9 +933,8 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // solves.
 // 64x64 tiles staged through registers: thread (lane r, group g = tid / 64) holds
 // column c = g + 4j, j < 16, of rows r0 + r.  Loads are unconditional (addresses
-// clamped into the tile) and raw; the masks are applied when the values are stored
-// to LDS, so every load of a tile (and of all tiles a kernel needs) is in flight
-// before the first one is consumed.
+// clamped into the tile) and raw; the caller masks the values when it stores them to
+// LDS, so every load of a tile is in flight before the first one is consumed.
 template <typename T>
 __device__ inline void tile_load(T (&v)[kPanel / 4], const T *__restrict__ M, int f, int r0, int c0, int nr, int nc) {
     const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -948,52 +947,6 @@ __device__ inline void tile_load(T (&v)[kPanel / 4], const T *__restrict__ M, in
 #pragma unroll
     for (int j = 0; j < kPanel / 4; ++j) v[j] = base[(int64_t)min(g + 4 * j, nc - 1) * f];
 }
-// dst[r][c] = tile value for r < nr, c < nc, zero elsewhere
-template <typename T>
-__device__ inline void tile_store(T (*dst)[kPanel + 1], const T (&v)[kPanel / 4], int nr, int nc) {
-    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < kPanel / 4; ++j) {
-        const int c = g + 4 * j;
-        dst[r][c] = (r < nr && c < nc) ? v[j] : T(0);
-    }
-}
-// The stored inverse of a diagonal block (nb x nb tile) as a full matrix U with
-// U[m][k] = Li[k][m]: strict upper part from the block (Li transposed), unit
-// diagonal below nb, zero elsewhere.
-template <typename T>
-__device__ inline void tile_store_inv(T (*dst)[kPanel + 1], const T (&v)[kPanel / 4], int nb) {
-    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < kPanel / 4; ++j) {
-        const int c = g + 4 * j;
-        dst[r][c] = (r < c && c < nb) ? v[j] : (r == c && r < nb ? T(1) : T(0));
-    }
-}
-// Vector block: dst[i][rr] = src[rr * ld + i] for i < ni, rr < nr (zero-padded to
-// 64 x kRhsChunk); four values per thread.
-template <typename T>
-__device__ inline void vec_load(T (&v)[4], const T *__restrict__ src, int64_t ld, int ni, int nr) {
-    if (ni <= 0 || nr <= 0) {  // vec_store writes zeros
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = T(0);
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int idx = threadIdx.x + 256 * j, i = idx & 63, rr = idx >> 6;
-        v[j] = src[(int64_t)min(rr, nr - 1) * ld + min(i, ni - 1)];
-    }
-}
-template <typename T>
-__device__ inline void vec_store(T (*dst)[kRhsChunk + 1], const T (&v)[4], int ni, int nr) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int idx = threadIdx.x + 256 * j, i = idx & 63, rr = idx >> 6;
-        dst[i][rr] = (i < ni && rr < nr) ? v[j] : T(0);
-    }
-}
-
 template <typename T>
 __device__ inline T lane_value(T v, int lane);
 template <>
@@ -1759,7 +1712,7 @@ __global__ __launch_bounds__(256) void k_permute_out(const T *__restrict__ Xe, c
 // Every front owns a vector V (f rows x nrhs) in a workspace: the forward pass
 // gathers it (own rows from the permuted right-hand side, plus the children's
 // footer vectors through the inverse row maps), then walks the own columns in
-// 256-column blocks, two launches each (k_fwd_diag: y of the block from its stored
+// kSBlock-column blocks, two launches each (k_fwd_diag: y of the block from its stored
 // inverse; k_fwd_upd: the product with the rows below); the footer left in V is the
 // update the parent pulls.  The backward pass starts each level with the
 // footer product (two-stage: k_bwd_foot_part / k_bwd_foot_red, including the
@@ -2194,61 +2147,6 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const int2 *__restrict__ tas
     for (int r = 0; r < nc; ++r) Vg[(int64_t)r * f + i] = Y[(int64_t)r * n + row];
 }
 
-// Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
-// L[i, 64cb+col] x[row i] (x of the footer rows are final: ancestors are solved first).
-// On the matrix cores: (partial)^T = x^T L[chunk, block] in 16x16x4 tiles, wave v taking
-// columns 16 v .. 16 v + 15; 64-row tiles of L and x go through LDS, the next one's loads in
-// flight during the current one's products.
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ tasks, const T *__restrict__ arena,
-                                                       const int *__restrict__ F, const int *__restrict__ W,
-                                                       const int64_t *__restrict__ off,
-                                                       const int64_t *__restrict__ vbase, const T *__restrict__ V,
-                                                       T *__restrict__ part, int nrhs,
-        const int *__restrict__ fsys, const int *__restrict__ cols) {
-    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    __shared__ T blk[kPanel][kPanel + 1];
-    __shared__ T xs[kTile][kRhsChunk + 1];
-    const int4 tk = tasks[blockIdx.x];
-    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
-    if (nc == 0) return;
-    const int g = tk.x, f = F[g], w = W[g];
-    const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
-    const int ibeg = w + tk.z * kFootChunk, iend = min(ibeg + kFootChunk, f);
-    const T *M = arena + off[g];
-    const T *Vg = V + vbase[g] * nrhs;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int col = kPanel / 4 * wv + l16;
-    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
-        const int nr = min(kRhsChunk, nc - c0);
-        typename MfmaAcc<T>::type acc;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        T tb[kPanel / 4], tx[4];
-        tile_load<T>(tb, M, f, ibeg, cb0, iend - ibeg, ncol);
-        vec_load<T>(tx, Vg + (int64_t)c0 * f + ibeg, f, iend - ibeg, nr);
-        for (int i0 = ibeg; i0 < iend; i0 += kTile) {
-            __syncthreads();
-            tile_store<T>(blk, tb, iend - i0, ncol);  // blk[i][c] = L[i0+i][cb0+c]
-            vec_store<T>(xs, tx, iend - i0, nr);      // xs[i][rr] = x[row i0+i][c0+rr]
-            __syncthreads();
-            const int nx = i0 + kTile;
-            if (nx < iend) {  // next block in flight during this one's products
-                tile_load<T>(tb, M, f, nx, cb0, iend - nx, ncol);
-                vec_load<T>(tx, Vg + (int64_t)c0 * f + nx, f, iend - nx, nr);
-            }
-#pragma unroll
-            for (int s = 0; s < kTile / 4; ++s) acc = MfmaAcc<T>::step(xs[4 * s + lk][l16], blk[4 * s + lk][col], acc);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int rr = MfmaAcc<T>::row(lk, q);
-            if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = acc[q];
-        }
-        __syncthreads();
-    }
-}
-
 // Backward footer, stage 2: Z[own c] = y_c / d_c - sum of the partials (fixed order).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ tasks, const int *__restrict__ W,
@@ -2370,7 +2268,56 @@ __global__ __launch_bounds__(256, 2) void k_inv_blk(const int4 *__restrict__ tas
 // NS = 1 for the tasks of blocks of at most 64 columns, kSSub otherwise (separate launches).
 constexpr int kSSub = kSBlock / kPanel;  // 64-column sub-blocks of a solve block
 template <typename T>
-constexpr int kSPre = sizeof(T) == 4 ? 4 : 2;  // tiles in flight per task
+constexpr int kSPre = sizeof(T) == 4 ? 4 : 2;  // tiles in flight per wave
+// Split tasks (KS = 2 or 4, for blocks wider than 64 columns): the task's 64 outputs (four
+// groups of 16) spread over KS workgroups of 4 / KS groups each, and the KS waves of a group
+// split the tiles along k (wave ks sums tiles ks, ks + KS, ...); the group's ks = 0 wave adds
+// the others' partial sums in wave order.  The chain of dependent loads per wave is 1 / KS of
+// the block's.  KS = 1: one workgroup per task, wave v takes group v over every tile.
+// Workgroups go to the 8 XCDs in turn, so the KS workgroups of a task take ids 8 apart: one XCD
+// and one L2, where a 128-byte line of a row-strided operand serves all of them.  The grid is
+// padded to a multiple of 8 KS; ids past the tasks return.
+constexpr int kSSplit = 4;  // the diagonal-block solves and the backward update
+// The wide forward update splits k over four groups of one workgroup when a launch has at most
+// this many tasks (one per CU); above it the tasks alone fill the chip (C3: 4.89 ms of forward
+// kernels per step with the split at most 256 tasks, 5.27 never split, 5.09 always).
+constexpr int kSolveTasksPerChip = 256;
+template <int KS>
+static inline int split_grid(int ntask) { return KS == 1 ? ntask : (ntask + 7) / 8 * 8 * KS; }
+// This wave's task, output group (16 outputs of the task's 64) and k share.
+template <int KS>
+__device__ __forceinline__ void split_task(int &task, int &grp, int &ks) {
+    const int wv = threadIdx.x >> 6;
+    if constexpr (KS == 1) {
+        task = blockIdx.x;
+        grp = wv;
+        ks = 0;
+    } else {
+        constexpr int G = 4 / KS;  // groups per workgroup
+        const int b = blockIdx.x, r = b >> 3;
+        task = ((r / KS) << 3) | (b & 7);
+        grp = (r % KS) * G + wv % G;
+        ks = wv / G;
+    }
+}
+
+// acc (wave ks's partial sum) -> the group's ks = 0 wave holds the sum over its KS waves, added
+// in wave order; red holds KS - 1 waves' accumulators per group of the workgroup.
+template <typename T, int KS>
+__device__ __forceinline__ void split_reduce(typename MfmaAcc<T>::type &acc, T (*red)[4 / KS][4][64], int ks) {
+    constexpr int G = 4 / KS;
+    const int lane = threadIdx.x & 63, gl = (threadIdx.x >> 6) % G;
+    if (ks > 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[ks - 1][gl][q][lane] = acc[q];
+    __syncthreads();
+    if (ks == 0)
+#pragma unroll
+        for (int v = 0; v < KS - 1; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += red[v][gl][q][lane];
+    __syncthreads();  // red is free for the next pass
+}
 
 // v[4 i + e] = p[16 i + 4 lk + e] if that index is < nk, else 0.  Vector loads; the four values
 // past a column's end (or nk) that one of them may touch are in the same allocation (the next
@@ -2442,35 +2389,132 @@ __device__ __forceinline__ void tile_chain(typename MfmaAcc<T>::type &acc, int t
     }
 }
 
+// Tiles [0, nu) (nu <= NT) of a task through the matrix cores, every tile's loads issued first.
+template <typename T, int NT, typename Load, typename Prep>
+__device__ __forceinline__ void tile_all(typename MfmaAcc<T>::type &acc, int nu, Load &&load, Prep &&prep) {
+    T a[NT][16], b[NT][16];
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+        if (u < nu) load(u, a[u], b[u]);
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+        if (u < nu) {
+            prep(u, b[u]);
+            tile_mfma<T>(acc, a[u], b[u]);
+        }
+}
+// A wave's nu tiles (at most NT): all in flight at once where NT <= 4, a chain with up to P in
+// flight otherwise.
+template <typename T, int NT, int P, typename Load, typename Prep>
+__device__ __forceinline__ void tile_run(typename MfmaAcc<T>::type &acc, int nu, Load &&load, Prep &&prep) {
+    if constexpr (NT <= 4) {
+        tile_all<T, NT>(acc, nu, load, prep);
+    } else {
+        if (nu > 0) tile_chain<T, P>(acc, 0, nu, load, prep);
+    }
+}
+
+// Backward footer, stage 1: partial[task][col][rr] = sum over one chunk of footer rows of
+// L[i, 64cb+col] x[row i] (x of the footer rows are final: ancestors are solved first).
+// On the matrix cores: (partial)^T = x^T L[chunk, block] in 16x16x4 tiles.  Wave v takes the
+// chunk's 64-row tile v for all 64 columns (four 16-column products sharing its x operand);
+// both operands run along k (the footer rows of a column of L and of x), so they go from
+// memory straight into the MFMA registers, every load of the chunk in flight at once.  Wave v
+// then adds the four waves' sums for columns 16 v .. 16 v + 15 in wave order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ tasks, const T *__restrict__ arena,
+                                                       const int *__restrict__ F, const int *__restrict__ W,
+                                                       const int64_t *__restrict__ off,
+                                                       const int64_t *__restrict__ vbase, const T *__restrict__ V,
+                                                       T *__restrict__ part, int nrhs,
+        const int *__restrict__ fsys, const int *__restrict__ cols) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
+    static_assert(kFootChunk == 4 * kPanel, "one 64-row tile per wave");
+    __shared__ T red[4][4][4][64];  // [wave][column group][q][lane]
+    const int4 tk = tasks[blockIdx.x];
+    const int nc = cols ? min(nrhs, cols[fsys[tk.x]]) : nrhs;  // right-hand sides the system needs
+    if (nc == 0) return;
+    const int g = tk.x, f = F[g], w = W[g];
+    const int cb0 = tk.y * kPanel, ncol = min(kPanel, w - cb0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int i0 = w + tk.z * kFootChunk + kPanel * wv, nk = min(kPanel, f - i0);  // this wave's rows
+    const T *M = arena + off[g];
+    const T *Vg = V + vbase[g] * nrhs;
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        typename MfmaAcc<T>::type acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[t][q] = T(0);
+        if (nk > 0) {
+            T a[16], b[4][16];
+            kvec_load<T>(a, Vg + (int64_t)(c0 + min(l16, nr - 1)) * f + i0, nk);  // x[row i][c0 + l16]
+#pragma unroll
+            for (int t = 0; t < 4; ++t)  // L[i][cb0 + 16 t + l16]
+                kvec_load<T>(b[t], M + (int64_t)(cb0 + min(16 * t + l16, ncol - 1)) * f + i0, nk);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) tile_mfma<T>(acc[t], a, b[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[wv][t][q][lane] = acc[t][q];
+        __syncthreads();
+        T sum[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sum[q] = red[0][wv][q][lane];
+#pragma unroll
+            for (int v = 1; v < 4; ++v) sum[q] += red[v][wv][q][lane];
+        }
+        const int col = 16 * wv + l16;
+        if (col < ncol)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) part[((int64_t)blockIdx.x * kPanel + col) * nrhs + c0 + rr] = sum[q];
+            }
+        __syncthreads();  // red is free for the next pass
+    }
+}
+
 // Forward, step 1 of block J: y of sub-block a, y[pa + j] = sum_{k <= j} X[pa + j][k] v[k] over
 // the block's sub-blocks c <= a (X[pa + j][pc + k] at front row pc + k, column pa + j: one
 // vector load along k), as y^T = v^T X^T; y goes to Y.
-template <typename T, int NS>
+template <typename T, int NS, int KS>
 __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   const T *__restrict__ V, T *__restrict__ Y, int J, int n, int nrhs,
-                                                  const int *__restrict__ cols) {
+                                                  const int *__restrict__ cols, int ntask) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
-    const PanelTask tk = tasks[blockIdx.x];
+    static_assert(KS == 1 || KS == 2 || KS == 4, "1, 2 or 4 waves per output group");
+    constexpr int NT = (NS + KS - 1) / KS;  // tiles per wave, at most
+    constexpr int P = NT < kSPre<T> ? NT : kSPre<T>;
+    __shared__ T red[KS > 1 ? KS - 1 : 1][4 / KS][4][64];
+    int task, grp, ks;
+    split_task<KS>(task, grp, ks);
+    if (task >= ntask) return;
+    const PanelTask tk = tasks[task];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;  // right-hand sides the system needs
     if (nc == 0) return;
     const int a = NS == 1 ? 0 : tk.blk, f = tk.f, w = tk.w;
     const int bs = J * kSBlock, be = min(bs + kSBlock, w);
     const int pa = bs + kPanel * a, na = min(kPanel, be - pa);
     const T *M = arena + tk.off;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int j = 16 * wv + l16;  // this lane's output row of the sub-block (MFMA column)
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const int nu = (a + 1 - ks + KS - 1) / KS;  // this wave's tiles c = ks + KS u <= a
+    const int j = 16 * grp + l16;  // this lane's output row of the sub-block (MFMA column)
     const T *xcol = M + (int64_t)(pa + min(j, na - 1)) * f;  // X[pa + j][.] along the rows of column pa + j
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         const T *vcol = V + tk.vbase * nrhs + (int64_t)(c0 + min(l16, nr - 1)) * f;  // v[.][c0 + l16]
-        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
-            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+        auto load = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int pc = bs + kPanel * (ks + KS * u), ncc = min(kPanel, be - pc);
             kvec_load<T>(av, vcol + pc, ncc);
             kvec_load<T>(bv, xcol + pc, ncc);
         };
-        auto prep = [&](int c, T(&bv)[16]) {
-            if (c != a) return;  // X_aa = Li_a: unit diagonal, zero above it
+        auto prep = [&](int u, T(&bv)[16]) {
+            if (ks + KS * u != a) return;  // X_aa = Li_a: unit diagonal, zero above it
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
                 const int k = 16 * (s >> 2) + 4 * lk + (s & 3);
@@ -2480,9 +2524,10 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_chain<T, P>(acc, 0, a + 1, load, prep);
+        tile_run<T, NT, P>(acc, nu, load, prep);
+        if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
         // acc[q] = y[pa + j][c0 + rr], rr = row(lk, q)
-        if (j < na)
+        if (ks == 0 && j < na)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
@@ -2494,13 +2539,19 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
 // Forward, step 2 of block J: the 64 rows r0 + i below the block, V[r0 + i] -= sum_k
 // L[r0 + i][k] y[k] over the block's columns, as (V_rows)^T -= y^T L[rows, block]^T (y along
 // k by vector loads, L down the block's columns).
-template <typename T, int NS>
+template <typename T, int NS, int KS>
 __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  T *__restrict__ V, const T *__restrict__ Y, int J, int n, int nrhs,
-                                                 const int *__restrict__ cols) {
+                                                 const int *__restrict__ cols, int ntask) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
-    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
-    const PanelTask tk = tasks[blockIdx.x];
+    static_assert(KS == 1 || KS == 2 || KS == 4, "1, 2 or 4 waves per output group");
+    constexpr int NT = (NS + KS - 1) / KS;
+    constexpr int P = NT < kSPre<T> ? NT : kSPre<T>;
+    __shared__ T red[KS > 1 ? KS - 1 : 1][4 / KS][4][64];
+    int task, grp, ks;
+    split_task<KS>(task, grp, ks);
+    if (task >= ntask) return;
+    const PanelTask tk = tasks[task];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int f = tk.f, w = tk.w;
@@ -2509,26 +2560,29 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
     const int r0 = be + tk.blk * kTile, nrow = min(kTile, f - r0);
     const T *M = arena + tk.off;
     T *Vg = V + tk.vbase * nrhs;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int i = 16 * wv + l16;  // this lane's row below the block (MFMA column)
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const int nu = (nt - ks + KS - 1) / KS;  // this wave's tiles ks + KS u < nt
+    const int i = 16 * grp + l16;  // this lane's row below the block (MFMA column)
     const T *lrow = M + (int64_t)bs * f + r0 + min(i, nrow - 1);  // L[r0 + i][bs + k] = lrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         const T *ycol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // y[bs + .][c0 + l16]
-        T old[4];
+        T old[4] = {};
+        if (ks == 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(i, nrow - 1)];
-        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
-            const int kc = kPanel * c, nkc = min(kPanel, nk - kc);
+            for (int q = 0; q < 4; ++q)
+                old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(i, nrow - 1)];
+        auto load = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int kc = kPanel * (ks + KS * u), nkc = min(kPanel, nk - kc);
             kvec_load<T>(av, ycol + kc, nkc);
             kstride_load<T>(bv, lrow + (int64_t)kc * f, f, nkc);
         };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_chain<T, P>(acc, 0, nt, load, [](int, T(&)[16]) {});
-        if (i < nrow)
+        tile_run<T, NT, P>(acc, nu, load, [](int, T(&)[16]) {});
+        if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
+        if (ks == 0 && i < nrow)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
@@ -2537,16 +2591,127 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
     }
 }
 
+// Forward, step 2 of block J for blocks wider than 64 columns: the same product with the L
+// tiles staged through LDS.  The L operand runs along rows (down a column of the front), so a
+// group of 256 threads copies each 64 x 64 tile with 16-byte loads, four threads per column
+// (256 contiguous bytes), into Ls[k][i], from where a lane reads its row i along k.  Two tiles
+// are in flight in registers and the LDS copy is double-buffered (one barrier per tile); y
+// comes from memory one tile ahead.  KQ = 4 (launches with few tasks): four such groups in one
+// workgroup take every fourth tile each, and group 0 adds the four sums in group order.  Rows
+// of the last 64-row strip past the front's end are read (at most 63 elements past the column,
+// inside the arena's tail padding) and never stored; columns past the block's end are clamped
+// to its last column and meet zero y.
+template <typename T, int KQ>
+__global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__restrict__ tasks,
+                                                           const T *__restrict__ arena, T *__restrict__ V,
+                                                           const T *__restrict__ Y, int J, int n, int nrhs,
+                                                           const int *__restrict__ cols) {
+    static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
+    static_assert(KQ == 1 || (KQ == 4 && sizeof(T) == 4), "one group, or four (FP32: 136 KB of LDS)");
+    constexpr int LD = kPanel + 16 / (int)sizeof(T);  // row of Ls: 16-byte aligned, reads without conflicts
+    constexpr int VW = 16 / (int)sizeof(T), NV = 16 / VW;  // elements per vector, vectors per thread and tile
+    __shared__ __align__(16) T Ls[KQ][2][kPanel][LD];
+    const PanelTask tk = tasks[blockIdx.x];
+    const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
+    if (nc == 0) return;
+    const int f = tk.f, w = tk.w;
+    const int bs = J * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
+    const int nt = (nk + kPanel - 1) / kPanel;
+    const int r0 = be + tk.blk * kTile, nrow = min(kTile, f - r0);
+    const T *M = arena + tk.off;
+    T *Vg = V + tk.vbase * nrhs;
+    const int tid = threadIdx.x & 255, kq = threadIdx.x >> 8;
+    const int lane = tid & 63, wv = tid >> 6, l16 = lane & 15, lk = lane >> 4;
+    const int i = 16 * wv + l16;  // this lane's row below the block (MFMA column)
+    const int nu = (nt - kq + KQ - 1) / KQ;  // this group's tiles u = kq + KQ j, j < nu
+    const int nj = (nt + KQ - 1) / KQ;       // steps (group 0 has the most tiles)
+    const int lc = tid >> 2, lr = 16 * (tid & 3);  // staging: column lc of a tile, rows lr .. lr + 15
+    const T *lsrc = M + (int64_t)bs * f + r0 + lr;
+    auto gload = [&](int j, mg_u4u(&r)[NV]) {
+        const T *p = lsrc + (int64_t)min(kPanel * (kq + KQ * j) + lc, nk - 1) * f;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) r[v] = *reinterpret_cast<const mg_u4u *>(p + VW * v);
+    };
+    for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
+        const int nr = min(kRhsChunk, nc - c0);
+        const T *ycol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // y[bs + .][c0 + l16]
+        auto yload = [&](int j, T(&yv)[16]) {
+            const int kc = kPanel * (kq + KQ * j);
+            kvec_load<T>(yv, ycol + kc, min(kPanel, nk - kc));
+        };
+        typename MfmaAcc<T>::type acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = T(0);
+        mg_u4u ra[NV], rb[NV];
+        T ya[16], yb[16];
+        if (nu > 0) gload(0, ra);
+        if (nu > 1) gload(1, rb);
+        if (nu > 0) yload(0, ya);
+        // step j: the copy of R (tile j of the group) to Ls[kq][j % 2], then the loads of tile
+        // j + 2 into R and of y(j + 1) into yn, then the products with yc.  Every thread takes
+        // every barrier; a group past its tiles only waits.
+        auto step = [&](int j, mg_u4u(&R)[NV], T(&yc)[16], T(&yn)[16]) {
+            const int b = j & 1;
+            if (j < nu)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) *reinterpret_cast<mg_u4u *>(&Ls[kq][b][lc][lr + VW * v]) = R[v];
+            __syncthreads();
+            if (j + 2 < nu) gload(j + 2, R);
+            if (j + 1 < nu) yload(j + 1, yn);
+            if (j < nu) {
+                T bv[16];
+#pragma unroll
+                for (int s = 0; s < 16; ++s) bv[s] = Ls[kq][b][16 * (s >> 2) + 4 * lk + (s & 3)][i];
+                tile_mfma<T>(acc, yc, bv);
+            }
+        };
+        for (int j = 0; j < nj; j += 2) {
+            step(j, ra, ya, yb);
+            if (j + 1 < nj) step(j + 1, rb, yb, ya);
+        }
+        T old[4] = {};  // loaded here, in flight during the groups' sum
+        if (kq == 0)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                old[q] = Vg[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * f + r0 + min(i, nrow - 1)];
+        if constexpr (KQ > 1) {
+            __syncthreads();  // every group is done with Ls
+            T(*red)[4][4][64] = reinterpret_cast<T(*)[4][4][64]>(&Ls[0][0][0][0]);  // [group][wave][q][lane]
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[kq][wv][q][lane] = acc[q];
+            __syncthreads();
+            if (kq == 0)
+#pragma unroll
+                for (int g = 1; g < KQ; ++g)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] += red[g][wv][q][lane];
+        }
+        if (kq == 0 && i < nrow)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Vg[(int64_t)(c0 + rr) * f + r0 + i] = old[q] - acc[q];
+            }
+        __syncthreads();  // Ls is free for the next pass
+    }
+}
+
 // Backward, step 1 of block K: x of sub-block b, x[pb + m] = sum_{k >= m} X[k][pb + m] z[k] over
 // the block's sub-blocks c >= b (z final: the footer and every block to the right are applied;
 // X[pc + k][pb + m] at front row pb + m, column pc + k), as x^T = z^T X; x goes to Y.
-template <typename T, int NS>
+template <typename T, int NS, int KS>
 __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                   T *__restrict__ Y, const T *__restrict__ Z, int K, int n, int nrhs,
-                                                  const int *__restrict__ cols) {
+                                                  const int *__restrict__ cols, int ntask) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides per pass");
-    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
-    const PanelTask tk = tasks[blockIdx.x];
+    static_assert(KS == 1 || KS == 2 || KS == 4, "1, 2 or 4 waves per output group");
+    constexpr int NT = (NS + KS - 1) / KS;
+    constexpr int P = NT < kSPre<T> ? NT : kSPre<T>;
+    __shared__ T red[KS > 1 ? KS - 1 : 1][4 / KS][4][64];
+    int task, grp, ks;
+    split_task<KS>(task, grp, ks);
+    if (task >= ntask) return;
+    const PanelTask tk = tasks[task];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int b = NS == 1 ? 0 : tk.blk, f = tk.f, w = tk.w;
@@ -2554,19 +2719,20 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
     const int pb = bs + kPanel * b, nbb = min(kPanel, be - pb);
     const int nsub = NS == 1 ? 1 : (be - bs + kPanel - 1) / kPanel;
     const T *M = arena + tk.off;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int m = 16 * wv + l16;  // this lane's output row of the sub-block (MFMA column)
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const int nu = (nsub - b - ks + KS - 1) / KS;  // this wave's tiles c = b + ks + KS u < nsub
+    const int m = 16 * grp + l16;  // this lane's output row of the sub-block (MFMA column)
     const T *xrow = M + pb + min(m, nbb - 1);  // X[k][pb + m] = xrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         const T *zcol = Z + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc;  // z[.][c0 + l16]
-        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
-            const int pc = bs + kPanel * c, ncc = min(kPanel, be - pc);
+        auto load = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int pc = bs + kPanel * (b + ks + KS * u), ncc = min(kPanel, be - pc);
             kvec_load<T>(av, zcol + pc, ncc);
             kstride_load<T>(bv, xrow + (int64_t)pc * f, f, ncc);
         };
-        auto prep = [&](int c, T(&bv)[16]) {
-            if (c != b) return;  // X_bb = Li_b: unit diagonal, zero above it (k < m)
+        auto prep = [&](int u, T(&bv)[16]) {
+            if (ks + KS * u != 0) return;  // X_bb = Li_b: unit diagonal, zero above it (k < m)
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
                 const int k = 16 * (s >> 2) + 4 * lk + (s & 3);
@@ -2576,8 +2742,9 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_chain<T, P>(acc, b, nsub, load, prep);
-        if (m < nbb)
+        tile_run<T, NT, P>(acc, nu, load, prep);
+        if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
+        if (ks == 0 && m < nbb)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = MfmaAcc<T>::row(lk, q);
@@ -2589,42 +2756,53 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
 // Backward, step 2 of block K: every 64-column block cb to its left, z[64 cb + j] -= sum_k
 // L[bs + k][64 cb + j] x[bs + k] over the block's rows, as (z_cb)^T -= x^T L[block rows, cb]
 // (both operands along k by vector loads: L down column 64 cb + j).
-template <typename T, int NS>
+template <typename T, int NS, int KS>
 __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ tasks, const T *__restrict__ arena,
                                                  const T *__restrict__ Y, T *__restrict__ Z, int K, int n, int nrhs,
-                                                 const int *__restrict__ cols) {
+                                                 const int *__restrict__ cols, int ntask) {
     static_assert(kRhsChunk == 16, "one 16-row MFMA tile of right-hand sides");
-    constexpr int P = NS < kSPre<T> ? NS : kSPre<T>;
-    const PanelTask tk = tasks[blockIdx.x];
+    static_assert(KS == 1 || KS == 2 || KS == 4, "1, 2 or 4 waves per output group");
+    constexpr int NT = (NS + KS - 1) / KS;
+    constexpr int P = NT < kSPre<T> ? NT : kSPre<T>;
+    __shared__ T red[KS > 1 ? KS - 1 : 1][4 / KS][4][64];
+    int task, grp, ks;
+    split_task<KS>(task, grp, ks);
+    if (task >= ntask) return;
+    const PanelTask tk = tasks[task];
     const int nc = cols ? min(nrhs, cols[tk.sys]) : nrhs;
     if (nc == 0) return;
     const int cb = tk.blk, f = tk.f, w = tk.w;
     const int bs = K * kSBlock, be = min(bs + kSBlock, w), nk = be - bs;
     const int nt = NS == 1 ? 1 : (nk + kPanel - 1) / kPanel;
     const T *M = arena + tk.off;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, lk = lane >> 4;
-    const int jj = kPanel * cb + 16 * wv + l16;        // this lane's column (MFMA column), < 64 cb + 64 <= bs
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
+    const int nu = (nt - ks + KS - 1) / KS;             // this wave's tiles ks + KS u < nt
+    const int jj = kPanel * cb + 16 * grp + l16;       // this lane's column (MFMA column), < 64 cb + 64 <= bs
     const T *lcol = M + (int64_t)jj * f + bs;          // L[bs + k][jj] = lcol[k]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
         const T *xcol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // x[bs + .][c0 + l16]
-        T old[4];
+        T old[4] = {};
+        if (ks == 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + jj];
-        auto load = [&](int c, T(&av)[16], T(&bv)[16]) {
-            const int kc = kPanel * c, nkc = min(kPanel, nk - kc);
+            for (int q = 0; q < 4; ++q)
+                old[q] = Z[(int64_t)(c0 + min(MfmaAcc<T>::row(lk, q), nr - 1)) * n + tk.fc + jj];
+        auto load = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int kc = kPanel * (ks + KS * u), nkc = min(kPanel, nk - kc);
             kvec_load<T>(av, xcol + kc, nkc);
             kvec_load<T>(bv, lcol + kc, nkc);
         };
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_chain<T, P>(acc, 0, nt, load, [](int, T(&)[16]) {});
+        tile_run<T, NT, P>(acc, nu, load, [](int, T(&)[16]) {});
+        if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
+        if (ks == 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int rr = MfmaAcc<T>::row(lk, q);
-            if (rr < nr) Z[(int64_t)(c0 + rr) * n + tk.fc + jj] = old[q] - acc[q];
-        }
+            for (int q = 0; q < 4; ++q) {
+                const int rr = MfmaAcc<T>::row(lk, q);
+                if (rr < nr) Z[(int64_t)(c0 + rr) * n + tk.fc + jj] = old[q] - acc[q];
+            }
     }
 }
 
@@ -3484,17 +3662,24 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
         for (int J = 0; 2 * J + 2 < (int)P.lvl_fa_ptr[L].size(); ++J) {
             const int *pa = &P.lvl_fa_ptr[L][2 * J], *pu = &P.lvl_fb_ptr[L][2 * J];
             if (pa[1] > pa[0])
-                hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.fa_tasks.p + pa[0],
-                                   arena, V, Y, J, n, nrhs, cols);
+                hipLaunchKernelGGL((k_fwd_diag<T, 1, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.fa_tasks.p + pa[0],
+                                   arena, V, Y, J, n, nrhs, cols, pa[1] - pa[0]);
             if (pa[2] > pa[1])
-                hipLaunchKernelGGL((k_fwd_diag<T, kSSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.fa_tasks.p + pa[1],
-                                   arena, V, Y, J, n, nrhs, cols);
+                hipLaunchKernelGGL((k_fwd_diag<T, kSSub, kSSplit>), dim3(split_grid<kSSplit>(pa[2] - pa[1])), dim3(256), 0, st,
+                                   P.fa_tasks.p + pa[1], arena, V, Y, J, n, nrhs, cols, pa[2] - pa[1]);
             if (pu[1] > pu[0])
-                hipLaunchKernelGGL((k_fwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.fb_tasks.p + pu[0],
-                                   arena, V, Y, J, n, nrhs, cols);
+                hipLaunchKernelGGL((k_fwd_upd<T, 1, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.fb_tasks.p + pu[0],
+                                   arena, V, Y, J, n, nrhs, cols, pu[1] - pu[0]);
             if (pu[2] > pu[1])
-                hipLaunchKernelGGL((k_fwd_upd<T, kSSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.fb_tasks.p + pu[1],
-                                   arena, V, Y, J, n, nrhs, cols);
+            {
+                const int nt = pu[2] - pu[1];
+                if (sizeof(T) == 4 && nt <= kSolveTasksPerChip)  // few tasks: k split in four groups
+                    hipLaunchKernelGGL((k_fwd_upd_wide<T, sizeof(T) == 4 ? 4 : 1>), dim3(nt), dim3(sizeof(T) == 4 ? 1024 : 256),
+                                       0, st, P.fb_tasks.p + pu[1], arena, V, Y, J, n, nrhs, cols);
+                else
+                    hipLaunchKernelGGL((k_fwd_upd_wide<T, 1>), dim3(nt), dim3(256), 0, st, P.fb_tasks.p + pu[1], arena,
+                                       V, Y, J, n, nrhs, cols);
+            }
         }
     }
     for (int L = P.nlevels - 1; L >= 0; --L) {
@@ -3517,17 +3702,17 @@ static void run_solve_once(const Numeric &N, const SolveWs &ws, int nrhs, const 
         for (int K = ((int)P.lvl_ba_ptr[L].size() - 1) / 2 - 1; K >= 0; --K) {
             const int *pa = &P.lvl_ba_ptr[L][2 * K], *pu = &P.lvl_bb_ptr[L][2 * K];
             if (pa[1] > pa[0])
-                hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.ba_tasks.p + pa[0],
-                                   arena, Y, Z, K, n, nrhs, cols);
+                hipLaunchKernelGGL((k_bwd_diag<T, 1, 1>), dim3(pa[1] - pa[0]), dim3(256), 0, st, P.ba_tasks.p + pa[0],
+                                   arena, Y, Z, K, n, nrhs, cols, pa[1] - pa[0]);
             if (pa[2] > pa[1])
-                hipLaunchKernelGGL((k_bwd_diag<T, kSSub>), dim3(pa[2] - pa[1]), dim3(256), 0, st, P.ba_tasks.p + pa[1],
-                                   arena, Y, Z, K, n, nrhs, cols);
+                hipLaunchKernelGGL((k_bwd_diag<T, kSSub, kSSplit>), dim3(split_grid<kSSplit>(pa[2] - pa[1])), dim3(256),
+                                   0, st, P.ba_tasks.p + pa[1], arena, Y, Z, K, n, nrhs, cols, pa[2] - pa[1]);
             if (pu[1] > pu[0])
-                hipLaunchKernelGGL((k_bwd_upd<T, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.bb_tasks.p + pu[0],
-                                   arena, Y, Z, K, n, nrhs, cols);
+                hipLaunchKernelGGL((k_bwd_upd<T, 1, 1>), dim3(pu[1] - pu[0]), dim3(256), 0, st, P.bb_tasks.p + pu[0],
+                                   arena, Y, Z, K, n, nrhs, cols, pu[1] - pu[0]);
             if (pu[2] > pu[1])
-                hipLaunchKernelGGL((k_bwd_upd<T, kSSub>), dim3(pu[2] - pu[1]), dim3(256), 0, st, P.bb_tasks.p + pu[1],
-                                   arena, Y, Z, K, n, nrhs, cols);
+                hipLaunchKernelGGL((k_bwd_upd<T, kSSub, kSSplit>), dim3(split_grid<kSSplit>(pu[2] - pu[1])), dim3(256), 0, st,
+                                   P.bb_tasks.p + pu[1], arena, Y, Z, K, n, nrhs, cols, pu[2] - pu[1]);
         }
         for (int c = 0; c < 4; ++c) {
             const int m0 = P.lvl_small_ptr[4 * L + c], m1 = P.lvl_small_ptr[4 * L + c + 1];
