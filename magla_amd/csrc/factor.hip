@@ -2161,10 +2161,18 @@ __global__ __launch_bounds__(256) void k_bwd_foot_red(const int4 *__restrict__ t
     const int ncol = min(kPanel, W[g] - cb0), fc = firstcol[g] + cb0;
     for (int idx = threadIdx.x; idx < ncol * nc; idx += 256) {
         const int col = idx % ncol, rr = idx / ncol;
-        T s = T(0);
-        for (int ch = 0; ch < nch; ++ch) s += part[((int64_t)(tb + ch) * kPanel + col) * nrhs + rr];
+        const T *pc = part + ((int64_t)tb * kPanel + col) * nrhs + rr;  // chunk ch at pc[ch * kPanel * nrhs]
         const int64_t e = (int64_t)rr * n + fc + col;
-        Z[e] = Y[e] / (T)D[fc + col] - s;
+        const T yd = Y[e] / (T)D[fc + col];
+        T s = T(0);
+        for (int ch0 = 0; ch0 < nch; ch0 += 8) {  // eight loads in flight, then the sums in chunk order
+            T v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = ch0 + j < nch ? pc[(int64_t)(ch0 + j) * kPanel * nrhs] : T(0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        Z[e] = yd - s;
     }
 }
 
@@ -2451,10 +2459,11 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
             T a[16], b[4][16];
             kvec_load<T>(a, Vg + (int64_t)(c0 + min(l16, nr - 1)) * f + i0, nk);  // x[row i][c0 + l16]
 #pragma unroll
-            for (int t = 0; t < 4; ++t)  // L[i][cb0 + 16 t + l16]
-                kvec_load<T>(b[t], M + (int64_t)(cb0 + min(16 * t + l16, ncol - 1)) * f + i0, nk);
+            for (int t = 0; t < 4; ++t)  // L[i][cb0 + 16 t + l16]; column groups past ncol skipped
+                if (16 * t < ncol) kvec_load<T>(b[t], M + (int64_t)(cb0 + min(16 * t + l16, ncol - 1)) * f + i0, nk);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) tile_mfma<T>(acc[t], a, b[t]);
+            for (int t = 0; t < 4; ++t)
+                if (16 * t < ncol) tile_mfma<T>(acc[t], a, b[t]);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
@@ -2504,6 +2513,7 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     const int nu = (a + 1 - ks + KS - 1) / KS;  // this wave's tiles c = ks + KS u <= a
     const int j = 16 * grp + l16;  // this lane's output row of the sub-block (MFMA column)
+    if (KS == 1 && 16 * grp >= na) return;  // a wave past the sub-block's columns (no barriers when KS = 1)
     const T *xcol = M + (int64_t)(pa + min(j, na - 1)) * f;  // X[pa + j][.] along the rows of column pa + j
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
@@ -2563,6 +2573,7 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     const int nu = (nt - ks + KS - 1) / KS;  // this wave's tiles ks + KS u < nt
     const int i = 16 * grp + l16;  // this lane's row below the block (MFMA column)
+    if (KS == 1 && 16 * grp >= nrow) return;  // a wave past the strip's rows (no barriers when KS = 1)
     const T *lrow = M + (int64_t)bs * f + r0 + min(i, nrow - 1);  // L[r0 + i][bs + k] = lrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
@@ -2722,6 +2733,7 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
     const int lane = threadIdx.x & 63, l16 = lane & 15, lk = lane >> 4;
     const int nu = (nsub - b - ks + KS - 1) / KS;  // this wave's tiles c = b + ks + KS u < nsub
     const int m = 16 * grp + l16;  // this lane's output row of the sub-block (MFMA column)
+    if (KS == 1 && 16 * grp >= nbb) return;  // a wave past the sub-block's columns (no barriers when KS = 1)
     const T *xrow = M + pb + min(m, nbb - 1);  // X[k][pb + m] = xrow[k f]
     for (int c0 = 0; c0 < nc; c0 += kRhsChunk) {
         const int nr = min(kRhsChunk, nc - c0);
