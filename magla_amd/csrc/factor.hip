@@ -2327,9 +2327,12 @@ __device__ __forceinline__ void split_reduce(typename MfmaAcc<T>::type &acc, T (
     __syncthreads();  // red is free for the next pass
 }
 
-// v[4 i + e] = p[16 i + 4 lk + e] if that index is < nk, else 0.  Vector loads; the four values
-// past a column's end (or nk) that one of them may touch are in the same allocation (the next
-// column, or the next buffer of the workspace, or the factor store's padding) and are masked.
+// v[4 i + e] = p[16 i + 4 lk + e] where that index is < nk (a vector starting at or past nk reads
+// p[0 .. 3] instead).  Vector loads; the four values past a column's end (or nk) that one of
+// them may touch are in the same allocation (the next column, or the next buffer of the
+// workspace, or the factor store's padding).  kvec_mask zeroes the entries at or past nk; it is
+// applied where the tile is used, so that the loads of several tiles stay in flight (a mask
+// right after the load would wait for it).
 template <typename T>
 __device__ __forceinline__ void kvec_load(T (&v)[16], const T *__restrict__ p, int nk) {
     const int lk = (threadIdx.x & 63) >> 4;
@@ -2349,6 +2352,10 @@ __device__ __forceinline__ void kvec_load(T (&v)[16], const T *__restrict__ p, i
 #pragma unroll
         for (int e = 0; e < 4; ++e) v[4 * i + e] = x[e];
     }
+}
+template <typename T>
+__device__ __forceinline__ void kvec_mask(T (&v)[16], int nk) {
+    const int lk = (threadIdx.x & 63) >> 4;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -2370,8 +2377,8 @@ __device__ __forceinline__ void tile_mfma(typename MfmaAcc<T>::type &acc, const 
     for (int s = 0; s < 16; ++s) acc = MfmaAcc<T>::step(a[s], b[s], acc);
 }
 // Tiles [t0, t1) (t0 < t1) of a task through the matrix cores with P tiles' loads in flight, in
-// named registers (no dynamic indexing): load(t, a, b) fills tile t's operands, prep(t, b)
-// adjusts b before use (the diagonal tile's mask).
+// named registers (no dynamic indexing): load(t, a, b) fills tile t's operands, prep(t, a, b)
+// adjusts them before use (the masks past the tile's end, the diagonal tile's mask).
 template <typename T, int P, typename Load, typename Prep>
 __device__ __forceinline__ void tile_chain(typename MfmaAcc<T>::type &acc, int t0, int t1, Load &&load, Prep &&prep) {
     static_assert(P >= 1 && P <= 4, "up to four tiles in flight");
@@ -2385,7 +2392,7 @@ __device__ __forceinline__ void tile_chain(typename MfmaAcc<T>::type &acc, int t
         if (t0 + 3 < t1) load(t0 + 3, a3, b3);
     auto step = [&](int t, T(&av)[16], T(&bv)[16]) {
         if (t >= t1) return;  // uniform
-        prep(t, bv);
+        prep(t, av, bv);
         tile_mfma<T>(acc, av, bv);
         if (t + P < t1) load(t + P, av, bv);
     };
@@ -2407,7 +2414,7 @@ __device__ __forceinline__ void tile_all(typename MfmaAcc<T>::type &acc, int nu,
 #pragma unroll
     for (int u = 0; u < NT; ++u)
         if (u < nu) {
-            prep(u, b[u]);
+            prep(u, a[u], b[u]);
             tile_mfma<T>(acc, a[u], b[u]);
         }
 }
@@ -2461,9 +2468,13 @@ __global__ __launch_bounds__(256) void k_bwd_foot_part(const int4 *__restrict__ 
 #pragma unroll
             for (int t = 0; t < 4; ++t)  // L[i][cb0 + 16 t + l16]; column groups past ncol skipped
                 if (16 * t < ncol) kvec_load<T>(b[t], M + (int64_t)(cb0 + min(16 * t + l16, ncol - 1)) * f + i0, nk);
+            kvec_mask<T>(a, nk);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
-                if (16 * t < ncol) tile_mfma<T>(acc[t], a, b[t]);
+                if (16 * t < ncol) {
+                    kvec_mask<T>(b[t], nk);
+                    tile_mfma<T>(acc[t], a, b[t]);
+                }
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
@@ -2523,7 +2534,10 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const PanelTask *__restrict__ 
             kvec_load<T>(av, vcol + pc, ncc);
             kvec_load<T>(bv, xcol + pc, ncc);
         };
-        auto prep = [&](int u, T(&bv)[16]) {
+        auto prep = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int ncc = min(kPanel, be - bs - kPanel * (ks + KS * u));
+            kvec_mask<T>(av, ncc);
+            kvec_mask<T>(bv, ncc);
             if (ks + KS * u != a) return;  // X_aa = Li_a: unit diagonal, zero above it
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
@@ -2591,7 +2605,8 @@ __global__ __launch_bounds__(256) void k_fwd_upd(const PanelTask *__restrict__ t
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_run<T, NT, P>(acc, nu, load, [](int, T(&)[16]) {});
+        auto prep = [&](int u, T(&av)[16], T(&)[16]) { kvec_mask<T>(av, min(kPanel, nk - kPanel * (ks + KS * u))); };
+        tile_run<T, NT, P>(acc, nu, load, prep);
         if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
         if (ks == 0 && i < nrow)
 #pragma unroll
@@ -2636,10 +2651,12 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
     const int i = 16 * wv + l16;  // this lane's row below the block (MFMA column)
     const int nu = (nt - kq + KQ - 1) / KQ;  // this group's tiles u = kq + KQ j, j < nu
     const int nj = (nt + KQ - 1) / KQ;       // steps (group 0 has the most tiles)
+    const int jl = max(nu - 1, 0);           // loads past the group's tiles repeat its last one
+    // (unconditional loads: under a branch a load lands in a temporary whose copy waits for it)
     const int lc = tid >> 2, lr = 16 * (tid & 3);  // staging: column lc of a tile, rows lr .. lr + 15
     const T *lsrc = M + (int64_t)bs * f + r0 + lr;
     auto gload = [&](int j, mg_u4u(&r)[NV]) {
-        const T *p = lsrc + (int64_t)min(kPanel * (kq + KQ * j) + lc, nk - 1) * f;
+        const T *p = lsrc + (int64_t)min(kPanel * (kq + KQ * min(j, jl)) + lc, nk - 1) * f;
 #pragma unroll
         for (int v = 0; v < NV; ++v) r[v] = *reinterpret_cast<const mg_u4u *>(p + VW * v);
     };
@@ -2647,7 +2664,7 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
         const int nr = min(kRhsChunk, nc - c0);
         const T *ycol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // y[bs + .][c0 + l16]
         auto yload = [&](int j, T(&yv)[16]) {
-            const int kc = kPanel * (kq + KQ * j);
+            const int kc = min(kPanel * (kq + KQ * min(j, jl)), kPanel * (nt - 1));
             kvec_load<T>(yv, ycol + kc, min(kPanel, nk - kc));
         };
         typename MfmaAcc<T>::type acc;
@@ -2655,9 +2672,9 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
         mg_u4u ra[NV], rb[NV];
         T ya[16], yb[16];
-        if (nu > 0) gload(0, ra);
-        if (nu > 1) gload(1, rb);
-        if (nu > 0) yload(0, ya);
+        gload(0, ra);
+        gload(1, rb);
+        yload(0, ya);
         // step j: the copy of R (tile j of the group) to Ls[kq][j % 2], then the loads of tile
         // j + 2 into R and of y(j + 1) into yn, then the products with yc.  Every thread takes
         // every barrier; a group past its tiles only waits.
@@ -2667,12 +2684,13 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
 #pragma unroll
                 for (int v = 0; v < NV; ++v) *reinterpret_cast<mg_u4u *>(&Ls[kq][b][lc][lr + VW * v]) = R[v];
             __syncthreads();
-            if (j + 2 < nu) gload(j + 2, R);
-            if (j + 1 < nu) yload(j + 1, yn);
+            gload(j + 2, R);
+            yload(j + 1, yn);
             if (j < nu) {
                 T bv[16];
 #pragma unroll
                 for (int s = 0; s < 16; ++s) bv[s] = Ls[kq][b][16 * (s >> 2) + 4 * lk + (s & 3)][i];
+                kvec_mask<T>(yc, min(kPanel, nk - kPanel * (kq + KQ * j)));
                 tile_mfma<T>(acc, yc, bv);
             }
         };
@@ -2741,9 +2759,12 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const PanelTask *__restrict__ 
         auto load = [&](int u, T(&av)[16], T(&bv)[16]) {
             const int pc = bs + kPanel * (b + ks + KS * u), ncc = min(kPanel, be - pc);
             kvec_load<T>(av, zcol + pc, ncc);
+            // masked here: with the sixteen strided loads of X per tile the registers are tight,
+            // and the tiles' loads wait for each other anyway (measured faster)
+            kvec_mask<T>(av, ncc);
             kstride_load<T>(bv, xrow + (int64_t)pc * f, f, ncc);
         };
-        auto prep = [&](int u, T(&bv)[16]) {
+        auto prep = [&](int u, T(&)[16], T(&bv)[16]) {
             if (ks + KS * u != 0) return;  // X_bb = Li_b: unit diagonal, zero above it (k < m)
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
@@ -2807,7 +2828,12 @@ __global__ __launch_bounds__(256) void k_bwd_upd(const PanelTask *__restrict__ t
         typename MfmaAcc<T>::type acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = T(0);
-        tile_run<T, NT, P>(acc, nu, load, [](int, T(&)[16]) {});
+        auto prep = [&](int u, T(&av)[16], T(&bv)[16]) {
+            const int nkc = min(kPanel, nk - kPanel * (ks + KS * u));
+            kvec_mask<T>(av, nkc);
+            kvec_mask<T>(bv, nkc);
+        };
+        tile_run<T, NT, P>(acc, nu, load, prep);
         if constexpr (KS > 1) split_reduce<T, KS>(acc, red, ks);
         if (ks == 0)
 #pragma unroll
