@@ -2651,12 +2651,12 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
     const int i = 16 * wv + l16;  // this lane's row below the block (MFMA column)
     const int nu = (nt - kq + KQ - 1) / KQ;  // this group's tiles u = kq + KQ j, j < nu
     const int nj = (nt + KQ - 1) / KQ;       // steps (group 0 has the most tiles)
-    const int jl = max(nu - 1, 0);           // loads past the group's tiles repeat its last one
-    // (unconditional loads: under a branch a load lands in a temporary whose copy waits for it)
+    // The loads are unconditional (under a branch a load lands in a temporary whose copy waits
+    // for it); past the group's tiles every lane reads the strip's first 64 bytes (one cached line).
     const int lc = tid >> 2, lr = 16 * (tid & 3);  // staging: column lc of a tile, rows lr .. lr + 15
-    const T *lsrc = M + (int64_t)bs * f + r0 + lr;
+    const T *lbase = M + (int64_t)bs * f + r0, *lsrc = lbase + lr;
     auto gload = [&](int j, mg_u4u(&r)[NV]) {
-        const T *p = lsrc + (int64_t)min(kPanel * (kq + KQ * min(j, jl)) + lc, nk - 1) * f;
+        const T *p = j < nu ? lsrc + (int64_t)min(kPanel * (kq + KQ * j) + lc, nk - 1) * f : lbase;
 #pragma unroll
         for (int v = 0; v < NV; ++v) r[v] = *reinterpret_cast<const mg_u4u *>(p + VW * v);
     };
@@ -2664,7 +2664,7 @@ __global__ __launch_bounds__(256 * KQ) void k_fwd_upd_wide(const PanelTask *__re
         const int nr = min(kRhsChunk, nc - c0);
         const T *ycol = Y + (int64_t)(c0 + min(l16, nr - 1)) * n + tk.fc + bs;  // y[bs + .][c0 + l16]
         auto yload = [&](int j, T(&yv)[16]) {
-            const int kc = min(kPanel * (kq + KQ * min(j, jl)), kPanel * (nt - 1));
+            const int kc = j < nu ? kPanel * (kq + KQ * j) : 0;
             kvec_load<T>(yv, ycol + kc, min(kPanel, nk - kc));
         };
         typename MfmaAcc<T>::type acc;
