@@ -365,13 +365,16 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
     const int s0 = min(slice, nslices - 1);
     const int base0 = sp[s0], w0 = slice < nslices ? (sp[s0 + 1] - base0) >> 6 : 0;
     const int r0 = min(64 * s0 + lane, n - 1);
+    // Every prefetch load is unconditional (entries past the slice's width repeat its last
+    // one and are zeroed where they are used): a load under a branch lands in a temporary
+    // whose copy waits for it, and the waits would serialize the prefetch.
     int cpre[kPre];
     double vpre[kPre], zpre[kPre], ppre[kPre];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
         const int e = base0 + 64 * min(k, max(w0 - 1, 0)) + lane;
         cpre[k] = sci[e];
-        vpre[k] = k < w0 ? sv[e] : 0.0;
+        vpre[k] = sv[e];
     }
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_sell(const int *__restrict__ sp
         int k = 0;
         if (first) {
 #pragma unroll
-            for (int q = 0; q < kPre; ++q) acc += vpre[q] * pcg_dir(zpre[q], ppre[q], beta);
+            for (int q = 0; q < kPre; ++q) acc += (q < w0 ? vpre[q] : 0.0) * pcg_dir(zpre[q], ppre[q], beta);
             k = kPre;
         }
         for (; k < w; ++k) {
@@ -648,16 +651,23 @@ __global__ __launch_bounds__(256) void k_pcg_cg1_sell(const int *__restrict__ sp
     int slice = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int s0 = min(slice, nslices - 1);
     const int base0 = sp[s0], w0 = slice < nslices ? (sp[s0 + 1] - base0) >> 6 : 0;
+    // All column indices first, then every value and gather, each load unconditional (entries
+    // past the slice's width repeat its last one and are zeroed where they are used; without
+    // the Jacobi scaling the D^-1 slot reads r): a load under a branch lands in a temporary
+    // whose copy waits for it, and in-order waits would serialize the prefetch entry by entry.
+    int cpre[kPre];
     double vpre[kPre], rpre[kPre], wpre[kPre], spre[kPre], dpre[kPre];
+    const double *dsrc = use ? invd : v.r;
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) cpre[k] = sci[base0 + 64 * min(k, max(w0 - 1, 0)) + lane];
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-        const int e = base0 + 64 * min(k, max(w0 - 1, 0)) + lane;
-        const int c = sci[e];
-        vpre[k] = k < w0 ? sv[e] : 0.0;
+        const int c = cpre[k];
+        vpre[k] = sv[base0 + 64 * min(k, max(w0 - 1, 0)) + lane];
         rpre[k] = v.r[c];
         wpre[k] = v.w[c];
         spre[k] = v.s[c];
-        dpre[k] = use ? invd[c] : 1.0;
+        dpre[k] = dsrc[c];
     }
     double alpha, beta;
     if (!cg1_scalars(part_in, np_in, it, S, sm12, alpha, beta)) return;  // uniform
@@ -670,7 +680,8 @@ __global__ __launch_bounds__(256) void k_pcg_cg1_sell(const int *__restrict__ sp
         if (first) {
 #pragma unroll
             for (int q = 0; q < kPre; ++q)
-                acc += vpre[q] * cg1_u_next(rpre[q], wpre[q], spre[q], dpre[q], alpha, beta, use);
+                acc += (q < w0 ? vpre[q] : 0.0) *
+                       cg1_u_next(rpre[q], wpre[q], spre[q], use ? dpre[q] : 1.0, alpha, beta, use);
             k = kPre;
         }
         for (; k < w; ++k) {
