@@ -1996,16 +1996,27 @@ __global__ __launch_bounds__(256) void k_fwd_small(const SmallRec *__restrict__ 
 #pragma unroll
     for (int q = 0; q < kKeep; ++q)
         if (__any(c0 + q < c1)) nq = q + 1;
+    // Unconditional loads, from this front's record where there is no such child: under a branch
+    // a load lands in a temporary whose copy waits for it, so the children would load one after
+    // the other.
+    // Every child's loads are issued before any of them is used.
+    const SmallRec *rsafe = recs + (idx < count ? idx : count - 1);
+    int64_t chv[kKeep];
+    int chw[kKeep], chf[kKeep], amv[kKeep];
 #pragma unroll
     for (int q = 0; q < kKeep; ++q) {
-        cb[q] = -1;
-        cf[q] = 0;
-        if (c0 + q < c1 && i < f) {
-            const SmallChild ch = a.child[c0 + q];
-            const int am = a.inv_map[rc.inv0 + (int64_t)q * f + i];
-            cf[q] = ch.f;
-            if (am >= 0) cb[q] = ch.vb * nrhs + ch.w + am;
-        }
+        const bool has = c0 + q < c1 && i < f;
+        const SmallChild *chp = has ? a.child + (c0 + q) : reinterpret_cast<const SmallChild *>(rsafe);
+        chv[q] = chp->vb;
+        chw[q] = chp->w;
+        chf[q] = chp->f;
+        amv[q] = *(has ? a.inv_map + (rc.inv0 + (int64_t)q * f + i) : reinterpret_cast<const int *>(rsafe));
+    }
+#pragma unroll
+    for (int q = 0; q < kKeep; ++q) {
+        const bool has = c0 + q < c1 && i < f;
+        cf[q] = has ? chf[q] : 0;
+        cb[q] = has && amv[q] >= 0 ? chv[q] * nrhs + chw[q] + amv[q] : -1;
     }
     T *Vg = V + rc.vbase * nrhs;
     const int yi = own ? fc + i : fc;  // clamped: other lanes read a valid row, then drop it
