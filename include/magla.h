@@ -57,9 +57,13 @@ typedef struct {
     double pivot_tolerance;               /* default 1e-13 */
     double diagonal_regularization;       /* default 1e-10 */
     int64_t parallel_update_threshold;    /* default 1<<20 (kept for ABI parity; unused on the GPU) */
-    int32_t device_graph;                 /* new: capture the factorization launches in a hipGraph on the
-                                             first mg_factorize_dev of a numeric handle and replay it on
-                                             refactorization (default 0) */
+    int32_t device_graph;                 /* new: 1 = capture the factorization launches in a hipGraph on
+                                             the first mg_factorize_dev of a numeric handle and replay it
+                                             on refactorization, and capture the handle's solves (first
+                                             solve, residual, refinement rounds) and replay them while
+                                             their arguments repeat; 2 = the solves only (the
+                                             factorization stays eager, with its lookahead stream);
+                                             0 = neither (default) */
     int32_t reserved;
 } mg_options;
 

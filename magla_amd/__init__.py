@@ -386,7 +386,8 @@ class UtduSolverOptions:
     diagonal_regularization: float = 1e-10
     parallel_update_threshold: int = 1 << 20
     precision: int = FP64
-    device_graph: bool = False  # replay the factorization launches from a captured hipGraph
+    # hipGraph replay: True or 1 = the factorization launches and the solves, 2 = the solves only
+    device_graph: int = 0
 
     def _c(self):
         o = _Options()
@@ -400,7 +401,7 @@ class UtduSolverOptions:
         o.diagonal_regularization = self.diagonal_regularization
         o.parallel_update_threshold = self.parallel_update_threshold
         o.precision = self.precision
-        o.device_graph = 1 if self.device_graph else 0
+        o.device_graph = int(self.device_graph)
         return o
 
 
@@ -788,7 +789,7 @@ class UtduBatch:
         if precision is not None:
             o.precision = precision
         if device_graph is not None:
-            o.device_graph = 1 if device_graph else 0
+            o.device_graph = int(device_graph)
         _check(lib().mg_batch_factorize_dev(self._h, C.c_void_p(values_ptr), C.byref(o), C.c_void_p(stream)))
 
     def autotune_dev(self, values_ptr, precision=None, stream=None):

@@ -3241,8 +3241,10 @@ struct Numeric {
     }
 };
 
-// hipGraph capture of the factorization launches: requested by mg_options.device_graph.
-static inline bool opts_use_graph(const mg_options &o) { return o.device_graph != 0; }
+// hipGraph capture (mg_options.device_graph): 1 captures the factorization launches and the
+// solves, 2 only the solves (the factorization then keeps its lookahead stream).
+static inline bool opts_use_graph(const mg_options &o) { return o.device_graph == 1; }
+static inline bool opts_graph_solves(const mg_options &o) { return o.device_graph == 1 || o.device_graph == 2; }
 static bool profile_on();
 
 constexpr int kScaleParts = 256;
@@ -3524,7 +3526,7 @@ void factorize_device(Symbolic &S, const double *d_values, int precision, const 
                 MG_HIP(hipMemcpyAsync(N->values.p, d_values, sizeof(double) * S.matrix_nnz, hipMemcpyDeviceToDevice, st));
             N->A.set_values_dev(N->values.p, st);
             const bool graph = opts_use_graph(opts) && !profile_on();
-            N->solve_graph = opts_use_graph(opts);
+            N->solve_graph = opts_graph_solves(opts);
             const bool same_opts = N->graph_tol == opts.pivot_tolerance && N->graph_reg == opts.diagonal_regularization &&
                                    N->graph_allow == opts.allow_diagonal_regularization;
             if (graph && N->gexec && N->graph_precision == precision && same_opts) {

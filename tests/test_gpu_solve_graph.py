@@ -21,15 +21,16 @@ def dev_cols(b):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(b).T)).to(DEV)
 
 
+@pytest.mark.parametrize("graph", [1, 2])  # factorization and solves captured, or the solves only
 @pytest.mark.parametrize("precision", [M.FP32, M.FP64])
-def test_captured_solve_replays_equal_eager_solves(precision):
+def test_captured_solve_replays_equal_eager_solves(precision, graph):
     n, r, c, v, clamped, _ = G.config_c3(m=8, k=4)
     a = M.SymmetricCSRMatrix.from_coo(n, r, c, v)
     a.line_cross(np.zeros(n), 0.0, clamped)
     oc = O.Csr.from_csr(n, a.row_pointers, a.column_indices, a.values)
     vals = torch.from_numpy(a.values).to(DEV)
     eager_s = M.SymmetricUtduSolver(M.UtduSolverOptions(precision=precision))
-    graph_s = M.SymmetricUtduSolver(M.UtduSolverOptions(precision=precision, device_graph=True))
+    graph_s = M.SymmetricUtduSolver(M.UtduSolverOptions(precision=precision, device_graph=graph))
     sym = eager_s.analyze(a)
     fe = eager_s.factorize_dev(sym, vals.data_ptr(), a.nonzero_count)
     fg = graph_s.factorize_dev(sym, vals.data_ptr(), a.nonzero_count)
