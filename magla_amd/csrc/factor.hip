@@ -933,16 +933,12 @@ __global__ __launch_bounds__(256) void k_assemble_values(const int64_t *__restri
 // solves.
 // 64x64 tiles staged through registers: thread (lane r, group g = tid / 64) holds
 // column c = g + 4j, j < 16, of rows r0 + r.  Loads are unconditional (addresses
-// clamped into the tile) and raw; the caller masks the values when it stores them to
-// LDS, so every load of a tile is in flight before the first one is consumed.
+// clamped into the tile, which is not empty: nr, nc >= 1) and raw; the caller masks the
+// values when it stores them to LDS, so every load of a tile is in flight before the first
+// one is consumed (a branch around the loads would make the next loads wait for them).
 template <typename T>
 __device__ inline void tile_load(T (&v)[kPanel / 4], const T *__restrict__ M, int f, int r0, int c0, int nr, int nc) {
     const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-    if (nr <= 0 || nc <= 0) {  // tile_store writes zeros
-#pragma unroll
-        for (int j = 0; j < kPanel / 4; ++j) v[j] = T(0);
-        return;
-    }
     const T *base = M + (int64_t)c0 * f + r0 + min(r, nr - 1);
 #pragma unroll
     for (int j = 0; j < kPanel / 4; ++j) v[j] = base[(int64_t)min(g + 4 * j, nc - 1) * f];
